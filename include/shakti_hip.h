@@ -1,0 +1,205 @@
+/*
+ * shakti_hip.h -- C ABI of libshakti_hip.so, the MI355X (gfx950) hydraulic-head
+ * solver that stands behind Shakti.jl's linear-solver plugin seam.
+ *
+ * The library owns a device-resident copy of the whole State (every field of
+ * the reference's src/state.jl:13-57, same column-major ix-fastest layout) and
+ * the solver workspace.  Host code moves fields in and out with shk_upload /
+ * shk_download and calls the entry points below; each one does the same work,
+ * in the same order, as the reference function it names, on the GPU.
+ *
+ * Conventions
+ *   - Every function returns SHK_OK (0) or a negative SHK_ERR_* code; the text
+ *     of the last error of a context is returned by shk_last_error().  Nothing
+ *     throws across the ABI.
+ *   - One context is used by one host thread at a time; calls on a context are
+ *     serialised on the context's own HIP stream and return after that stream
+ *     has synchronised (except the *_async benchmarking helpers, which say so).
+ *   - Host buffers belong to the caller and are copied.  All device memory
+ *     belongs to the context and is released by shk_destroy().
+ *   - Indexing: centre fields are nx*ny doubles (index ix + iy*nx), x-face
+ *     fields (nx+1)*ny (index ix + iy*(nx+1)), y-face fields nx*(ny+1)
+ *     (index ix + iy*nx).
+ */
+#ifndef SHAKTI_HIP_H
+#define SHAKTI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHK_ABI_VERSION 1
+
+#define SHK_OK 0
+#define SHK_ERR_ARG (-1)      /* bad argument, shape or field id */
+#define SHK_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define SHK_ERR_NUMERIC (-3)  /* NaN/Inf or breakdown detected */
+#define SHK_ERR_STATE (-4)    /* call out of order (e.g. solve before init) */
+#define SHK_ERR_NODEVICE (-5) /* no usable gfx950 device */
+
+/* Grid geometry -- reference src/grid.jl:8-35 (dx = lx/(nx-1), dy = ly/(ny-1)). */
+typedef struct {
+    int32_t nx, ny;
+    double lx, ly, dx, dy, dx2, dy2;
+} shk_grid;
+
+/* Physical parameters -- reference src/model_parameters.jl:9-29 (defaults :55-84).
+ * The Glen exponent n is canonicalised as the reference does
+ * (src/model_parameters.jl:122-134): integer-valued n, n-1 and 1/n use the
+ * integer power path, anything else the general power. */
+typedef struct {
+    double rho_w, rho_sw, rho_i, g, nu, n, omega, L, br, lr, ct, cw, p_atm, b_min, b_max, e_v;
+} shk_params;
+
+/* Scheme choices of a Simulation -- reference src/simulation.jl:105-146,
+ * src/sliding_law.jl:33,45.  Sensible heat is on iff ct*cw != 0 and opening by
+ * sliding iff br != 0, decided from shk_params as the reference does. */
+typedef struct {
+    int32_t kfs;         /* 0 Arithmetic, 1 Harmonic (src/k_face_scheme.jl:15,23) */
+    int32_t gap_scheme;  /* 0 implicit, 1 explicit (src/run.jl:130-145) */
+    int32_t sliding_law; /* 0 RegularizedCoulombSlidingLaw, 1 PrescribedSlidingLaw */
+    double sliding_C;    /* RegularizedCoulombSlidingLaw.C */
+} shk_model;
+
+#define SHK_PRECOND_JACOBI 0    /* CGIterativeSolver(g, MatrixFreeLinearSystem) */
+#define SHK_PRECOND_CHEBYSHEV 1 /* ... ; chebyshev_degree = k */
+
+#define SHK_FORM_FP64 0         /* plain FP64 PCG */
+#define SHK_FORM_FP32_REFINE 1  /* FP32 inner PCG inside FP64 iterative refinement */
+
+/* Linear-solver configuration -- reference CGIterativeSolver(g,
+ * MatrixFreeLinearSystem; chebyshev_degree, chebyshev_nsteps_estimate)
+ * (src/linear_solver.jl:832-843) plus Krylov's stopping controls. */
+typedef struct {
+    int32_t precond;          /* SHK_PRECOND_* */
+    int32_t cheb_degree;      /* >= 1 */
+    int32_t cheb_nsteps;      /* Lanczos steps for the bound estimate, >= 2 (default 15) */
+    int32_t form;             /* SHK_FORM_* */
+    int32_t itmax;            /* 0 -> 2*nx*ny, Krylov's default */
+    int32_t check_every;      /* CG iterations queued between host convergence checks (>= 1) */
+    double atol, rtol;        /* stop when sqrt(r'M^-1 r) <= atol + rtol*initial; Krylov default sqrt(eps) each */
+    double inner_rtol;        /* FP32_REFINE: relative reduction asked of each inner solve */
+} shk_solver_cfg;
+
+/* Statistics of the last linear solve (reference ws.stats, ls.precond bounds). */
+typedef struct {
+    int32_t niter;          /* CG iterations (all inner iterations for FP32_REFINE) */
+    int32_t solved;         /* 1 when the stopping test was met */
+    int32_t zero_curvature; /* 1 when p'Ap vanished (Krylov's zero-curvature exit) */
+    int32_t bounds_warning; /* 1 when the Chebyshev bound estimate was rejected (previous bounds kept) */
+    int32_t refinements;    /* FP32_REFINE outer iterations */
+    int32_t pad;
+    double rnorm0, rnorm;   /* preconditioned residual norm at start / end */
+    double lambda_min, lambda_max; /* Chebyshev bounds in use */
+} shk_solve_stats;
+
+/* State field ids, in the order of src/state.jl:13-57. */
+enum shk_field {
+    SHK_H = 0, SHK_PW, SHK_PO, SHK_B, SHK_BETA, SHK_ABS_UB, SHK_MDOT, SHK_SHEAR, SHK_POTENTIAL, SHK_SENSIBLE,
+    SHK_RE, SHK_K, SHK_G, SHK_ZB, SHK_ZS, SHK_HICE, SHK_IEB, SHK_LAMBDA, SHK_A_VISC, SHK_N, SHK_MASK,
+    SHK_DHDX, SHK_Q_X, SHK_RE_X, SHK_B_X, SHK_UB_X, SHK_TAUB_X, SHK_DPWDX, SHK_VALID_X,
+    SHK_DHDY, SHK_Q_Y, SHK_RE_Y, SHK_B_Y, SHK_UB_Y, SHK_TAUB_Y, SHK_DPWDY, SHK_VALID_Y,
+    SHK_NUM_FIELDS
+};
+
+typedef struct shk_ctx shk_ctx;
+
+/* Library identification: ABI version and the device the context runs on. */
+int32_t shk_abi_version(void);
+const char* shk_last_error(const shk_ctx* ctx);
+
+/* Create / destroy a context on the current HIP device (allocates every State
+ * field, zeroed, with mask = GROUNDED and valid_x/valid_y = 1 as State(g) does,
+ * src/state.jl:86-137, plus the solver workspace).
+ * Replaces: State(g) + CGIterativeSolver(g, MatrixFreeLinearSystem; ...)
+ * (src/state.jl:86, src/linear_solver.jl:832). */
+int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* model, const shk_solver_cfg* cfg,
+               shk_ctx** out);
+int shk_destroy(shk_ctx* ctx);
+
+/* Replace the solver configuration (keeps the persistent Chebyshev bounds). */
+int shk_set_solver_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg);
+
+/* Replace the physical parameters / scheme choices used by later calls (the
+ * reference passes p and kfs to every solve, src/linear_solver.jl:915). */
+int shk_set_params(shk_ctx* ctx, const shk_params* params);
+int shk_set_model(shk_ctx* ctx, const shk_model* model);
+
+/* Copy one State field to / from the device; count must equal its size. */
+int shk_upload(shk_ctx* ctx, int32_t field, const double* host, size_t count);
+int shk_download(shk_ctx* ctx, int32_t field, double* host, size_t count);
+
+/* set_initial_conditions! (src/initial_conditions.jl:14-126) on the device.
+ * The raw inputs mask, A_visc, zb, zs, b, G, ub_x, ub_y, ieb (and taub_x,
+ * taub_y for the prescribed law) must have been uploaded first. */
+int shk_set_initial_conditions(shk_ctx* ctx);
+
+/* solve_elliptic_linear_system!(ls, s, g, p, kfs) for the matrix-free CG
+ * solver (src/linear_solver.jl:915-925): assemble, refresh Jacobi and the
+ * Chebyshev bounds, PCG warm-started from h; overwrites device h. */
+int shk_solve_elliptic(shk_ctx* ctx, shk_solve_stats* stats);
+
+/* solve_parabolic_linear_system!(ls, s, g, p, kfs, dt) (src/linear_solver.jl:935-945). */
+int shk_solve_parabolic(shk_ctx* ctx, double dt, shk_solve_stats* stats);
+
+/* The field refresh that follows every linear solve: dhdxy, pw, dpwdxy, N,
+ * q_and_Re_xy, Re, taub_xy, mdot, K (src/elliptic_solver.jl:193-206). */
+int shk_update_derived(shk_ctx* ctx);
+
+/* One Picard_iteration! (src/elliptic_solver.jl:187-208); relax_alpha <= 0
+ * means NoHeadRelaxation. */
+int shk_picard_iteration(shk_ctx* ctx, double relax_alpha, shk_solve_stats* stats);
+
+/* Picard_loop! (src/elliptic_solver.jl:138-176): converged / last_iter as the
+ * reference's ps.converged / ps.last_iter; cg_iters accumulates CG iterations. */
+int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every, double relax_alpha,
+                    int32_t* converged, int32_t* last_iter, int64_t* cg_iters);
+
+/* parabolic_solver! (src/parabolic_solver.jl:16-38). */
+int shk_parabolic_step(shk_ctx* ctx, double dt, int64_t* cg_iters);
+
+/* step_b! (src/run.jl:154-164): compute_b! by the context's gap scheme, then
+ * beta (when br != 0), b_x, b_y. */
+int shk_step_b(shk_ctx* ctx, double dt);
+
+/* update_ieb! (src/melt_input.jl:30, 86-94): mode 0 leaves ieb untouched
+ * (ConstantMeltInput), mode 1 overwrites it with `uniform` (SeasonalMeltInput),
+ * mode 2 writes uniform + the static moulin field set by shk_set_moulin_field
+ * (point moulins on top of the seasonal input; no reference counterpart). */
+int shk_set_moulin_field(shk_ctx* ctx, const double* host, size_t count);
+int shk_update_ieb(shk_ctx* ctx, int32_t mode, double uniform);
+
+/* One whole step! (src/run.jl:82-88): update_ieb!, step_h! (Picard when
+ * e_v == 0, else the parabolic solve), step_b!. */
+int shk_step(shk_ctx* ctx, double dt, int32_t ieb_mode, double ieb_uniform, int32_t iters, double tol,
+             int32_t check_every, double relax_alpha, int32_t* converged, int32_t* last_iter, int64_t* cg_iters);
+
+/* Stencil operator apply y = A x with the coefficients of the last assembly
+ * (StencilOperator mul!, src/linear_solver.jl:616-669).  Host buffers of
+ * nx*ny doubles. */
+int shk_assemble(shk_ctx* ctx, int32_t parabolic, double dt);
+int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host);
+/* Download the last assembly in the reference's MatrixFreeLinearSystem layout
+ * (aP, aE, aW, aN, aS, rhs; each nx*ny). Any pointer may be NULL. */
+int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double* aN, double* aS, double* rhs);
+
+/* Benchmark helpers: time `reps` back-to-back launches of one kernel on the
+ * context's stream with HIP events; returns the mean duration in ms and the
+ * algorithmic bytes one launch moves.  kernel: 0 FP64 operator apply,
+ * 1 FP32 operator apply, 2 FP64 Chebyshev sweep, 3 derived-field refresh. */
+int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch);
+
+/* Per-kernel timing collected with HIP events on the context's stream while
+ * enabled: names[i] (static strings), launch counts and total milliseconds. */
+int shk_profile_enable(shk_ctx* ctx, int32_t on);
+int shk_profile_read(shk_ctx* ctx, int32_t max_entries, const char** names, int64_t* counts, double* total_ms,
+                     double* bytes, int32_t* n_entries);
+int shk_profile_reset(shk_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHAKTI_HIP_H */

@@ -1,0 +1,777 @@
+// C-ABI implementation of libshakti_hip.so (declared in include/shakti_hip.h).
+//
+// A context owns one HIP stream (default priority), every State field on the
+// device, the linear-system arrays and the solver workspace.  The solver loops
+// run on the host but never wait on the device per CG iteration: every kernel
+// reads the device-resident scalars (alpha, beta, done, ...) written by the
+// finishing workgroup of the previous reduction, kernels queued after
+// convergence return at once, and the host looks at the `done` flag only every
+// `check_every` iterations.
+
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "shk_internal.hpp"
+
+using namespace shk;
+
+namespace {
+
+struct ProfEntry {
+    const char* name;
+    int64_t count = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct Pending {
+    int id;
+    hipEvent_t a, b;
+    double bytes;
+};
+
+}  // namespace
+
+struct shk_ctx {
+    shk_grid g{};
+    shk_params p{};
+    shk_model m{};
+    shk_solver_cfg cfg{};
+    Phys P{};
+    Geom G{};
+    hipStream_t st = nullptr;
+    int device = 0;
+    size_t n = 0, nxf = 0, nyf = 0;
+
+    Fields F{};
+    double* h_alt = nullptr;  // ping-pong partner of the h field
+    double* moulin = nullptr;
+
+    // linear system (FP64) and workspace
+    double *aP = nullptr, *aE = nullptr, *aN = nullptr, *rhs = nullptr;
+    double *x = nullptr, *r = nullptr, *z = nullptr, *pA = nullptr, *pB = nullptr, *Ap = nullptr;
+    double *s = nullptr, *cA = nullptr, *cB = nullptr;
+    // FP32 copies for the mixed-precision form
+    float *aP32 = nullptr, *aE32 = nullptr, *aN32 = nullptr;
+    float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *pA32 = nullptr, *pB32 = nullptr,
+          *Ap32 = nullptr, *s32 = nullptr, *cA32 = nullptr, *cB32 = nullptr;
+    double* d64 = nullptr;  // FP64 correction / residual scratch
+
+    double* partials = nullptr;
+    Scalars* sc = nullptr;       // device
+    Scalars* sc_host = nullptr;  // pinned host mirror
+    bool have_system = false;
+    int system_parabolic = 0;
+    double system_dt = 0.0;
+
+    // profiling
+    bool prof = false;
+    std::vector<ProfEntry> prof_tab;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+
+    std::string err;
+};
+
+namespace {
+
+const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_init",     "cheb_pass",
+                            "cg_matvec",  "cg_update",    "cg_finish",   "derived",     "gap",
+                            "stagger_b",  "ieb",          "ic",          "matvec",      "ir"};
+enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR, P_NUM };
+
+int fail(shk_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess) return fail(ctx, SHK_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+hipEvent_t take_event(shk_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Profiling brackets around one launch (no-ops unless enabled).
+struct Prof {
+    shk_ctx* c;
+    int id;
+    double bytes;
+    hipEvent_t a = nullptr;
+    Prof(shk_ctx* c_, int id_, double bytes_) : c(c_), id(id_), bytes(bytes_) {
+        if (c->prof) {
+            a = take_event(c);
+            if (a) hipEventRecord(a, c->st);
+        }
+    }
+    ~Prof() {
+        if (c->prof && a) {
+            hipEvent_t b = take_event(c);
+            if (!b) return;
+            hipEventRecord(b, c->st);
+            c->pending.push_back({id, a, b, bytes});
+        }
+    }
+};
+
+void prof_collect(shk_ctx* c) {
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->st);
+    for (auto& pe : c->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, pe.a, pe.b);
+        c->prof_tab[pe.id].count++;
+        c->prof_tab[pe.id].ms += ms;
+        c->prof_tab[pe.id].bytes += pe.bytes;
+        c->ev_pool.push_back(pe.a);
+        c->ev_pool.push_back(pe.b);
+    }
+    c->pending.clear();
+}
+
+Expo canon(double e) {
+    Expo x;
+    x.e = e;
+    x.is_int = (e == (double)(long)e) && e > -64 && e < 64;
+    x.k = x.is_int ? (int)e : 0;
+    return x;
+}
+
+void fill_phys(shk_ctx* c) {
+    Phys& P = c->P;
+    P.nx = c->g.nx; P.ny = c->g.ny;
+    P.dx = c->g.dx; P.dy = c->g.dy; P.dx2 = c->g.dx2; P.dy2 = c->g.dy2;
+    const shk_params& p = c->p;
+    P.rho_w = p.rho_w; P.rho_sw = p.rho_sw; P.rho_i = p.rho_i; P.g = p.g; P.nu = p.nu; P.n = p.n;
+    P.omega = p.omega; P.L = p.L; P.br = p.br; P.lr = p.lr; P.ct = p.ct; P.cw = p.cw; P.p_atm = p.p_atm;
+    P.b_min = p.b_min; P.b_max = p.b_max; P.e_v = p.e_v;
+    P.n_exp = canon(p.n);
+    P.n1_exp = canon(p.n - 1.0);
+    P.inv_n_exp = canon(1.0 / p.n);
+    P.with_sensible = !(p.ct == 0.0 || p.cw == 0.0);  // src/simulation.jl:132
+    P.with_open_by_sliding = !(p.br == 0.0);           // src/simulation.jl:136
+    P.kfs = c->m.kfs;
+    P.gap_scheme = c->m.gap_scheme;
+    P.sliding_law = c->m.sliding_law;
+    P.C = c->m.sliding_C;
+}
+
+size_t field_count(const shk_ctx* c, int f) {
+    if (f < 0 || f >= SHK_NUM_FIELDS) return 0;
+    if (f <= SHK_MASK) return c->n;
+    if (f <= SHK_VALID_X) return c->nxf;
+    return c->nyf;
+}
+
+int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
+    if (cfg->precond != SHK_PRECOND_JACOBI && cfg->precond != SHK_PRECOND_CHEBYSHEV)
+        return fail(ctx, SHK_ERR_ARG, "precond must be SHK_PRECOND_JACOBI or SHK_PRECOND_CHEBYSHEV");
+    if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_degree < 1 || cfg->cheb_degree > MAX_CHEB))
+        return fail(ctx, SHK_ERR_ARG, "ChebyshevPreconditioner: degree must be >= 1 (and <= 32)");
+    if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_nsteps < 2 || cfg->cheb_nsteps > MAX_LANCZOS))
+        return fail(ctx, SHK_ERR_ARG, "ChebyshevPreconditioner: nsteps_estimate must be >= 2 (and <= 64)");
+    if (cfg->form != SHK_FORM_FP64 && cfg->form != SHK_FORM_FP32_REFINE)
+        return fail(ctx, SHK_ERR_ARG, "unknown solver form");
+    if (cfg->check_every < 1) return fail(ctx, SHK_ERR_ARG, "check_every must be >= 1");
+    if (!(cfg->atol >= 0) || !(cfg->rtol >= 0)) return fail(ctx, SHK_ERR_ARG, "atol/rtol must be >= 0");
+    return SHK_OK;
+}
+
+// Upload the host-side scalars the next solve starts from.
+int reset_scalars(shk_ctx* ctx, double atol, double rtol, int itmax) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    Scalars* h = ctx->sc_host;
+    HIPCHK(hipMemcpy(h, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    h->done = 0; h->solved = 0; h->zero_curv = 0; h->tired = 0; h->iter = 0;
+    h->atol = atol; h->rtol = rtol; h->itmax = itmax;
+    h->counter = 0;
+    HIPCHK(hipMemcpyAsync(ctx->sc, h, sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
+    return SHK_OK;
+}
+
+int read_scalars(shk_ctx* ctx) {
+    HIPCHK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+template <class T>
+struct Work {
+    const T *aP, *aE, *aN, *b;
+    T *x, *r, *z, *pA, *pB, *Ap, *s, *cA, *cB;
+};
+
+// Chebyshev application z = P^-1 r (passes 1..degree-1), finishing gamma.
+template <class T>
+void cheb_apply(shk_ctx* c, const Work<T>& w, int degree, int mode) {
+    const Geom& G = c->G;
+    const double nb = (double)c->n * sizeof(T);
+    if (degree == 1) {
+        Prof pr(c, P_CHEB, 3 * nb);
+        launch::cheb_degree1<T>(G, w.aP, w.r, w.z, mode, c->partials, c->sc, c->st);
+        return;
+    }
+    const T* s_in = nullptr;
+    const T* pc_in = nullptr;
+    for (int k = 1; k < degree; ++k) {
+        T* s_out = w.s;
+        T* pc_out = (k % 2) ? w.cA : w.cB;
+        const bool last = k == degree - 1;
+        // bytes: k==1 reads r, aP, aE, aN; later passes read aP, aE, aN, pc, s, y; the last adds r
+        double bytes = k == 1 ? (last ? 6 : 7) * nb : (last ? 7 : 9) * nb;
+        Prof pr(c, P_CHEB, bytes);
+        launch::cheb_pass<T>(G, w.aP, w.aE, w.aN, w.r, s_in, pc_in, s_out, pc_out, w.z, k, degree, mode,
+                             c->partials, c->sc, c->st);
+        s_in = s_out;
+        pc_in = pc_out;
+    }
+}
+
+// Lanczos bound estimate + coefficients (Chebyshev only).
+template <class T>
+void cheb_refresh(shk_ctx* c, const Work<T>& w, int nsteps, int degree) {
+    const Geom& G = c->G;
+    const double nb = (double)c->n * sizeof(T);
+    {
+        Prof pr(c, P_LZ, 3 * nb);
+        launch::lanczos_init<T>(G, w.b, w.r, w.pA, c->partials, c->sc, c->st);
+    }
+    T* p_old = w.pA;
+    T* p_new = w.pB;
+    for (int k = 0; k < nsteps; ++k) {
+        {
+            Prof pr(c, P_LZ, 7 * nb);
+            launch::lanczos_step_a<T>(G, w.aP, w.aE, w.aN, w.r, p_old, p_new, w.Ap, k, c->partials, c->sc, c->st);
+        }
+        if (k < nsteps - 1) {
+            Prof pr(c, P_LZ, 3 * nb);
+            launch::lanczos_step_b<T>(G, w.Ap, w.r, k, c->partials, c->sc, c->st);
+        }
+        T* t = p_old;
+        p_old = p_new;
+        p_new = t;
+    }
+    Prof pr(c, P_CB, 0);
+    launch::cheb_bounds(nsteps, degree, 1, c->sc, c->st);
+}
+
+// Preconditioned CG on A x = b from x0 (nullptr: zero), Krylov cg! semantics.
+// The accumulated correction is left in w.x.  Returns SHK_OK or an error.
+template <class T>
+int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, int check_every, double atol,
+        double rtol, int itmax) {
+    const Geom& G = ctx->G;
+    const double nb = (double)ctx->n * sizeof(T);
+    const int jacobi = precond == SHK_PRECOND_JACOBI;
+    int rc = reset_scalars(ctx, atol, rtol, itmax);
+    if (rc) return rc;
+    {
+        Prof pr(ctx, P_INIT, (x0 ? 8 : 5) * nb);
+        launch::cg_init<T>(G, w.aP, w.aE, w.aN, w.b, x0, w.r, w.z, w.x, w.pA, jacobi, ctx->partials, ctx->sc,
+                           ctx->st);
+    }
+    if (!jacobi) cheb_apply<T>(ctx, w, degree, 0);
+    T* p_old = w.pA;
+    T* p_new = w.pB;
+    long queued = 0;
+    for (;;) {
+        for (int k = 0; k < check_every; ++k) {
+            {
+                Prof pr(ctx, P_MV, 6 * nb);
+                launch::cg_matvec<T>(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap, ctx->partials, ctx->sc, ctx->st);
+            }
+            {
+                Prof pr(ctx, P_UPD, (jacobi ? 8 : 6) * nb);
+                launch::cg_update<T>(G, w.aP, p_new, w.Ap, w.x, w.r, w.z, jacobi, ctx->partials, ctx->sc, ctx->st);
+            }
+            if (!jacobi) cheb_apply<T>(ctx, w, degree, 1);
+            T* t = p_old;
+            p_old = p_new;
+            p_new = t;
+            ++queued;
+        }
+        if (ctx->prof) prof_collect(ctx);
+        rc = read_scalars(ctx);
+        if (rc) return rc;
+        if (ctx->sc_host->done) break;
+        if (queued > (long)itmax + check_every) break;  // cannot happen: tired sets done
+    }
+    return SHK_OK;
+}
+
+int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats* stats) {
+    const Geom& G = ctx->G;
+    const double nb = (double)ctx->n * 8;
+    const shk_solver_cfg& cfg = ctx->cfg;
+    const int itmax = cfg.itmax > 0 ? cfg.itmax : (int)(2 * ctx->n < 2147483647 ? 2 * ctx->n : 2147483647);
+    {
+        Prof pr(ctx, P_ASM, 17 * nb);
+        launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st);
+    }
+    ctx->have_system = true;
+    ctx->system_parabolic = parabolic;
+    ctx->system_dt = dt;
+    Work<double> w{ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
+                   ctx->s, ctx->cA, ctx->cB};
+    const int cheb = cfg.precond == SHK_PRECOND_CHEBYSHEV;
+    if (cheb) cheb_refresh<double>(ctx, w, cfg.cheb_nsteps, cfg.cheb_degree);
+    int rc = pcg<double>(ctx, w, ctx->F.f[SHK_H], cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol,
+                         itmax);
+    if (rc) return rc;
+    // h_new = x + h_prev into the alternate buffer, then swap so F.f[SHK_H] is current
+    double* h_prev = ctx->F.f[SHK_H];
+    {
+        Prof pr(ctx, P_FIN, 3 * nb);
+        launch::cg_finish((int)ctx->n, ctx->x, h_prev, ctx->h_alt, relax, ctx->st);
+    }
+    ctx->F.f[SHK_H] = ctx->h_alt;
+    ctx->h_alt = h_prev;
+    const Scalars* h = ctx->sc_host;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->niter = h->iter;
+        stats->solved = h->solved;
+        stats->zero_curvature = h->zero_curv;
+        stats->bounds_warning = cheb ? h->bounds_warning : 0;
+        stats->rnorm0 = h->rNorm0;
+        stats->rnorm = h->rNorm;
+        stats->lambda_min = h->lmin;
+        stats->lambda_max = h->lmax;
+    }
+    if (ctx->prof) prof_collect(ctx);
+    return SHK_OK;
+}
+
+int derived(shk_ctx* ctx, const double* h_prev, int check, double tol) {
+    if (check) {
+        ctx->sc_host->picard_tol = tol;
+        // only picard_tol changes; write that field alone
+        HIPCHK(hipMemcpyAsync(&ctx->sc->picard_tol, &ctx->sc_host->picard_tol, sizeof(double), hipMemcpyHostToDevice,
+                              ctx->st));
+    }
+    Prof pr(ctx, P_DER, 0);
+    launch::derived(ctx->G, ctx->P, ctx->F, h_prev, 0, check, ctx->partials, ctx->sc, ctx->st);
+    return SHK_OK;
+}
+
+template <class T>
+int dalloc(shk_ctx* ctx, T** p, size_t count) {
+    HIPCHK(hipMalloc((void**)p, count * sizeof(T) + 256));
+    HIPCHK(hipMemsetAsync(*p, 0, count * sizeof(T) + 256, ctx->st));
+    return SHK_OK;
+}
+
+}  // namespace
+
+// ============================================================= C ABI
+
+extern "C" {
+
+int32_t shk_abi_version(void) { return SHK_ABI_VERSION; }
+
+const char* shk_last_error(const shk_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* model, const shk_solver_cfg* cfg,
+               shk_ctx** out) {
+    if (!out || !grid || !params || !model || !cfg) return SHK_ERR_ARG;
+    *out = nullptr;
+    if (grid->nx < 2 || grid->ny < 2) return SHK_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SHK_ERR_NODEVICE;
+    shk_ctx* ctx = new shk_ctx();
+    ctx->g = *grid;
+    ctx->p = *params;
+    ctx->m = *model;
+    int rc = check_cfg(ctx, cfg);
+    if (rc) { delete ctx; return rc; }
+    ctx->cfg = *cfg;
+    HIPCHK(hipGetDevice(&ctx->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        std::string arch = prop.gcnArchName;
+        delete ctx;
+        (void)arch;
+        return SHK_ERR_NODEVICE;
+    }
+    HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    const size_t nx = grid->nx, ny = grid->ny;
+    ctx->n = nx * ny;
+    ctx->nxf = (nx + 1) * ny;
+    ctx->nyf = nx * (ny + 1);
+    fill_phys(ctx);
+    ctx->G = make_geom((int)nx, (int)ny);
+    for (int f = 0; f < SHK_NUM_FIELDS; ++f)
+        if ((rc = dalloc(ctx, &ctx->F.f[f], field_count(ctx, f)))) return rc;
+    if ((rc = dalloc(ctx, &ctx->h_alt, ctx->n))) return rc;
+    // State(g): mask GROUNDED (0), valid_x/valid_y = 1
+    {
+        std::vector<double> ones(std::max(ctx->nxf, ctx->nyf), 1.0);
+        HIPCHK(hipMemcpyAsync(ctx->F.f[SHK_VALID_X], ones.data(), ctx->nxf * 8, hipMemcpyHostToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->F.f[SHK_VALID_Y], ones.data(), ctx->nyf * 8, hipMemcpyHostToDevice, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+    }
+    double** d64[] = {&ctx->aP, &ctx->aE, &ctx->aN, &ctx->rhs, &ctx->x, &ctx->r, &ctx->z, &ctx->pA,
+                      &ctx->pB, &ctx->Ap, &ctx->s, &ctx->cA, &ctx->cB, &ctx->d64};
+    for (auto pp : d64)
+        if ((rc = dalloc(ctx, pp, ctx->n))) return rc;
+    float** d32[] = {&ctx->aP32, &ctx->aE32, &ctx->aN32, &ctx->b32, &ctx->x32, &ctx->r32, &ctx->z32,
+                     &ctx->pA32, &ctx->pB32, &ctx->Ap32, &ctx->s32, &ctx->cA32, &ctx->cB32};
+    for (auto pp : d32)
+        if ((rc = dalloc(ctx, pp, ctx->n))) return rc;
+    Geom cells = make_geom((int)nx, (int)ny, 1);
+    const size_t nparts = 2 * (size_t)std::max(ctx->G.nblocks, cells.nblocks) + 64;
+    if ((rc = dalloc(ctx, &ctx->partials, nparts))) return rc;
+    HIPCHK(hipMalloc((void**)&ctx->sc, sizeof(Scalars)));
+    HIPCHK(hipHostMalloc((void**)&ctx->sc_host, sizeof(Scalars), hipHostMallocDefault));
+    memset(ctx->sc_host, 0, sizeof(Scalars));
+    ctx->sc_host->lmin = 1.0;  // ChebyshevPreconditioner placeholder bounds (src/preconditioner.jl:185)
+    ctx->sc_host->lmax = 1.0;
+    HIPCHK(hipMemcpy(ctx->sc, ctx->sc_host, sizeof(Scalars), hipMemcpyHostToDevice));
+    ctx->prof_tab.resize(P_NUM);
+    for (int i = 0; i < P_NUM; ++i) ctx->prof_tab[i].name = kProfNames[i];
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    *out = ctx;
+    return SHK_OK;
+}
+
+int shk_destroy(shk_ctx* ctx) {
+    if (!ctx) return SHK_ERR_ARG;
+    if (ctx->st) hipStreamSynchronize(ctx->st);
+    for (int f = 0; f < SHK_NUM_FIELDS; ++f) hipFree(ctx->F.f[f]);
+    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
+                    ctx->pA, ctx->pB, ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->d64, ctx->aP32, ctx->aE32,
+                    ctx->aN32, ctx->b32, ctx->x32, ctx->r32, ctx->z32, ctx->pA32, ctx->pB32, ctx->Ap32,
+                    ctx->s32, ctx->cA32, ctx->cB32, ctx->partials, ctx->sc};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (ctx->sc_host) hipHostFree(ctx->sc_host);
+    for (auto& pe : ctx->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
+    for (auto e : ctx->ev_pool) hipEventDestroy(e);
+    if (ctx->st) hipStreamDestroy(ctx->st);
+    delete ctx;
+    return SHK_OK;
+}
+
+int shk_set_solver_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
+    if (!ctx || !cfg) return SHK_ERR_ARG;
+    int rc = check_cfg(ctx, cfg);
+    if (rc) return rc;
+    ctx->cfg = *cfg;
+    return SHK_OK;
+}
+
+int shk_set_params(shk_ctx* ctx, const shk_params* params) {
+    if (!ctx || !params) return SHK_ERR_ARG;
+    ctx->p = *params;
+    fill_phys(ctx);
+    return SHK_OK;
+}
+
+int shk_set_model(shk_ctx* ctx, const shk_model* model) {
+    if (!ctx || !model) return SHK_ERR_ARG;
+    if (model->kfs < 0 || model->kfs > 1 || model->gap_scheme < 0 || model->gap_scheme > 1 ||
+        model->sliding_law < 0 || model->sliding_law > 1)
+        return fail(ctx, SHK_ERR_ARG, "unknown k-face scheme, gap scheme or sliding law");
+    ctx->m = *model;
+    fill_phys(ctx);
+    return SHK_OK;
+}
+
+int shk_upload(shk_ctx* ctx, int32_t field, const double* host, size_t count) {
+    if (!ctx || !host) return fail(ctx, SHK_ERR_ARG, "null argument");
+    const size_t want = field_count(ctx, field);
+    if (!want) return fail(ctx, SHK_ERR_ARG, "unknown field id " + std::to_string(field));
+    if (count != want)
+        return fail(ctx, SHK_ERR_ARG, "field " + std::to_string(field) + " expects " + std::to_string(want) +
+                                          " values, got " + std::to_string(count));
+    HIPCHK(hipMemcpyAsync(ctx->F.f[field], host, count * 8, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_download(shk_ctx* ctx, int32_t field, double* host, size_t count) {
+    if (!ctx || !host) return fail(ctx, SHK_ERR_ARG, "null argument");
+    const size_t want = field_count(ctx, field);
+    if (!want) return fail(ctx, SHK_ERR_ARG, "unknown field id " + std::to_string(field));
+    if (count != want)
+        return fail(ctx, SHK_ERR_ARG, "field " + std::to_string(field) + " expects " + std::to_string(want) +
+                                          " values, got " + std::to_string(count));
+    HIPCHK(hipMemcpyAsync(host, ctx->F.f[field], count * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_set_initial_conditions(shk_ctx* ctx) {
+    if (!ctx) return SHK_ERR_ARG;
+    {
+        Prof pr(ctx, P_IC, 0);
+        launch::ic_cells(ctx->P, ctx->F, ctx->st);
+        launch::stagger_b(ctx->P, ctx->F, ctx->st);
+        launch::derived(ctx->G, ctx->P, ctx->F, nullptr, 1, 0, ctx->partials, ctx->sc, ctx->st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->prof) prof_collect(ctx);
+    return SHK_OK;
+}
+
+int shk_solve_elliptic(shk_ctx* ctx, shk_solve_stats* stats) {
+    if (!ctx) return SHK_ERR_ARG;
+    int rc = solve(ctx, 0, 0.0, -1.0, stats);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_solve_parabolic(shk_ctx* ctx, double dt, shk_solve_stats* stats) {
+    if (!ctx || !(dt > 0)) return fail(ctx, SHK_ERR_ARG, "dt must be > 0");
+    int rc = solve(ctx, 1, dt, -1.0, stats);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_update_derived(shk_ctx* ctx) {
+    if (!ctx) return SHK_ERR_ARG;
+    int rc = derived(ctx, nullptr, 0, 0.0);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_picard_iteration(shk_ctx* ctx, double relax_alpha, shk_solve_stats* stats) {
+    if (!ctx) return SHK_ERR_ARG;
+    int rc = solve(ctx, 0, 0.0, relax_alpha, stats);
+    if (rc) return rc;
+    rc = derived(ctx, nullptr, 0, 0.0);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every, double relax_alpha,
+                    int32_t* converged, int32_t* last_iter, int64_t* cg_iters) {
+    if (!ctx || iters < 1 || check_every < 1) return fail(ctx, SHK_ERR_ARG, "iters and check_every must be >= 1");
+    int32_t conv = 0, last = 0;
+    for (int32_t it = 1; it <= iters; ++it) {
+        shk_solve_stats st;
+        int rc = solve(ctx, 0, 0.0, relax_alpha, &st);
+        if (rc) return rc;
+        if (cg_iters) *cg_iters += st.niter;
+        const int check = (it % check_every == 0) || (it == iters);
+        rc = derived(ctx, ctx->h_alt /* previous h */, check, tol);
+        if (rc) return rc;
+        if (check) {
+            rc = read_scalars(ctx);
+            if (rc) return rc;
+            if (ctx->sc_host->picard_converged) {
+                conv = 1;
+                last = it;
+                break;
+            }
+        }
+    }
+    if (!conv) last = iters;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (converged) *converged = conv;
+    if (last_iter) *last_iter = last;
+    return SHK_OK;
+}
+
+int shk_parabolic_step(shk_ctx* ctx, double dt, int64_t* cg_iters) {
+    if (!ctx || !(dt > 0)) return fail(ctx, SHK_ERR_ARG, "dt must be > 0");
+    shk_solve_stats st;
+    int rc = solve(ctx, 1, dt, -1.0, &st);
+    if (rc) return rc;
+    if (cg_iters) *cg_iters += st.niter;
+    rc = derived(ctx, nullptr, 0, 0.0);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_step_b(shk_ctx* ctx, double dt) {
+    if (!ctx) return SHK_ERR_ARG;
+    {
+        Prof pr(ctx, P_GAP, 0);
+        launch::gap(ctx->P, ctx->F, dt, ctx->st);
+    }
+    {
+        Prof pr(ctx, P_STAG, 0);
+        launch::stagger_b(ctx->P, ctx->F, ctx->st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->prof) prof_collect(ctx);
+    return SHK_OK;
+}
+
+int shk_set_moulin_field(shk_ctx* ctx, const double* host, size_t count) {
+    if (!ctx || !host || count != ctx->n) return fail(ctx, SHK_ERR_ARG, "moulin field must hold nx*ny values");
+    if (!ctx->moulin) HIPCHK(hipMalloc((void**)&ctx->moulin, ctx->n * 8));
+    HIPCHK(hipMemcpyAsync(ctx->moulin, host, count * 8, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_update_ieb(shk_ctx* ctx, int32_t mode, double uniform) {
+    if (!ctx || mode < 0 || mode > 2) return fail(ctx, SHK_ERR_ARG, "ieb mode must be 0, 1 or 2");
+    if (mode == 0) return SHK_OK;
+    if (mode == 2 && !ctx->moulin) return fail(ctx, SHK_ERR_STATE, "ieb mode 2 needs shk_set_moulin_field first");
+    {
+        Prof pr(ctx, P_IEB, 0);
+        launch::fill_ieb((int)ctx->n, ctx->F.f[SHK_IEB], uniform, mode == 2 ? ctx->moulin : nullptr, ctx->st);
+    }
+    HIPCHK(hipGetLastError());
+    return SHK_OK;
+}
+
+int shk_step(shk_ctx* ctx, double dt, int32_t ieb_mode, double ieb_uniform, int32_t iters, double tol,
+             int32_t check_every, double relax_alpha, int32_t* converged, int32_t* last_iter, int64_t* cg_iters) {
+    if (!ctx) return SHK_ERR_ARG;
+    int rc = shk_update_ieb(ctx, ieb_mode, ieb_uniform);
+    if (rc) return rc;
+    if (ctx->p.e_v == 0.0) {
+        rc = shk_picard_loop(ctx, iters, tol, check_every, relax_alpha, converged, last_iter, cg_iters);
+    } else {
+        rc = shk_parabolic_step(ctx, dt, cg_iters);
+        if (converged) *converged = 1;
+        if (last_iter) *last_iter = 0;
+    }
+    if (rc) return rc;
+    return shk_step_b(ctx, dt);
+}
+
+int shk_assemble(shk_ctx* ctx, int32_t parabolic, double dt) {
+    if (!ctx) return SHK_ERR_ARG;
+    if (parabolic && !(dt > 0)) return fail(ctx, SHK_ERR_ARG, "dt must be > 0");
+    launch::assemble<double>(ctx->G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st);
+    ctx->have_system = true;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host) {
+    if (!ctx || !x_host || !y_host) return fail(ctx, SHK_ERR_ARG, "null argument");
+    if (!ctx->have_system) return fail(ctx, SHK_ERR_STATE, "no assembled system");
+    HIPCHK(hipMemcpyAsync(ctx->d64, x_host, ctx->n * 8, hipMemcpyHostToDevice, ctx->st));
+    {
+        Prof pr(ctx, P_MATVEC, 5.0 * ctx->n * 8);
+        launch::matvec<double>(ctx->G, ctx->aP, ctx->aE, ctx->aN, ctx->d64, ctx->Ap, ctx->st);
+    }
+    HIPCHK(hipMemcpyAsync(y_host, ctx->Ap, ctx->n * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double* aN, double* aS, double* rhs) {
+    if (!ctx) return SHK_ERR_ARG;
+    if (!ctx->have_system) return fail(ctx, SHK_ERR_STATE, "no assembled system");
+    const size_t n = ctx->n, nx = ctx->g.nx, ny = ctx->g.ny;
+    std::vector<double> e(n), no(n);
+    HIPCHK(hipMemcpyAsync(e.data(), ctx->aE, n * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(no.data(), ctx->aN, n * 8, hipMemcpyDeviceToHost, ctx->st));
+    if (aP) HIPCHK(hipMemcpyAsync(aP, ctx->aP, n * 8, hipMemcpyDeviceToHost, ctx->st));
+    if (rhs) HIPCHK(hipMemcpyAsync(rhs, ctx->rhs, n * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    // Expand the symmetric face layout back into the reference's five arrays.
+    for (size_t iy = 0; iy < ny; ++iy)
+        for (size_t ix = 0; ix < nx; ++ix) {
+            const size_t c = ix + iy * nx;
+            if (aE) aE[c] = e[c];
+            if (aN) aN[c] = no[c];
+            if (aW) aW[c] = ix > 0 ? e[c - 1] : 0.0;
+            if (aS) aS[c] = iy > 0 ? no[c - nx] : 0.0;
+        }
+    return SHK_OK;
+}
+
+int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch) {
+    if (!ctx || reps < 1 || !mean_ms || !bytes_per_launch) return fail(ctx, SHK_ERR_ARG, "bad argument");
+    if (!ctx->have_system) return fail(ctx, SHK_ERR_STATE, "no assembled system");
+    const Geom& G = ctx->G;
+    const double n = (double)ctx->n;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    double bytes = 0;
+    auto one = [&]() {
+        switch (kernel) {
+            case 0: launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, ctx->h_alt, ctx->Ap, ctx->st); break;
+            case 1: launch::matvec<float>(G, ctx->aP32, ctx->aE32, ctx->aN32, ctx->x32, ctx->Ap32, ctx->st); break;
+            default: break;
+        }
+    };
+    if (kernel == 0) bytes = 5 * n * 8;
+    else if (kernel == 1) bytes = 5 * n * 4;
+    else {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        return fail(ctx, SHK_ERR_ARG, "unknown kernel id");
+    }
+    one();  // warm
+    HIPCHK(hipEventRecord(a, ctx->st));
+    for (int i = 0; i < reps; ++i) one();
+    HIPCHK(hipEventRecord(b, ctx->st));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *mean_ms = ms / reps;
+    *bytes_per_launch = bytes;
+    return SHK_OK;
+}
+
+int shk_profile_enable(shk_ctx* ctx, int32_t on) {
+    if (!ctx) return SHK_ERR_ARG;
+    if (!on) prof_collect(ctx);
+    ctx->prof = on != 0;
+    return SHK_OK;
+}
+
+int shk_profile_read(shk_ctx* ctx, int32_t max_entries, const char** names, int64_t* counts, double* total_ms,
+                     double* bytes, int32_t* n_entries) {
+    if (!ctx || !n_entries) return SHK_ERR_ARG;
+    prof_collect(ctx);
+    int k = 0;
+    for (int i = 0; i < P_NUM && k < max_entries; ++i, ++k) {
+        if (names) names[k] = ctx->prof_tab[i].name;
+        if (counts) counts[k] = ctx->prof_tab[i].count;
+        if (total_ms) total_ms[k] = ctx->prof_tab[i].ms;
+        if (bytes) bytes[k] = ctx->prof_tab[i].bytes;
+    }
+    *n_entries = k;
+    return SHK_OK;
+}
+
+int shk_profile_reset(shk_ctx* ctx) {
+    if (!ctx) return SHK_ERR_ARG;
+    prof_collect(ctx);
+    for (auto& e : ctx->prof_tab) { e.count = 0; e.ms = 0; e.bytes = 0; }
+    return SHK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Internal declarations shared by the kernel and API translation units of
+// libshakti_hip.so.  Not part of the public ABI (include/shakti_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "shakti_hip.h"
+
+namespace shk {
+
+constexpr double GROUNDED = 0.0, OCEAN = 1.0, LAND = 2.0, OTHER_BASIN = 3.0;  // src/mask.jl:21-41
+constexpr double EPS64 = 2.220446049250313e-16;                                // eps(Float64)
+
+constexpr int WAVE = 64;         // CDNA wavefront width
+constexpr int WAVES = 4;         // waves per workgroup, stacked along y
+constexpr int MAX_CHEB = 32;     // highest supported Chebyshev degree
+constexpr int MAX_LANCZOS = 64;  // highest supported bound-estimate step count
+
+// A canonicalised exponent (src/model_parameters.jl:122-134): integer-valued
+// exponents take the integer power path.
+struct Expo {
+    int is_int;
+    int k;
+    double e;
+};
+
+// Device copy of the State: one pointer per field, in shk_field order.
+struct Fields {
+    double* f[SHK_NUM_FIELDS];
+};
+
+// Scalar state of the solver, resident in device memory.  Written only by the
+// single finishing workgroup of a reduction kernel; read by every later kernel.
+struct Scalars {
+    // Krylov CG recurrence (see k_cg_* kernels)
+    double gamma, pNorm2, alpha, beta, rNorm, rNorm0, eps_stop, pAp, atol, rtol;
+    int32_t iter, itmax, done, solved, zero_curv, tired;
+    // Chebyshev preconditioner: persistent bounds and this solve's coefficients
+    double lmin, lmax;
+    double cheb_alpha[MAX_CHEB], cheb_beta[MAX_CHEB];
+    int32_t bounds_warning, cheb_degree;
+    // Lanczos-from-CG bound estimate
+    double lz_gamma;
+    double lz_alpha[MAX_LANCZOS], lz_beta[MAX_LANCZOS];
+    double lz_lmin, lz_lmax;
+    // Picard convergence check
+    double dmax, hmax, picard_tol;
+    int32_t picard_converged, picard_nan;
+    // mixed-precision outer loop
+    double ir_rnorm, ir_rnorm0;
+    // block-completion counter of the running reduction kernel
+    uint32_t counter;
+    uint32_t pad;
+};
+
+// Launch geometry of the row-marching kernels: workgroups of WAVE x WAVES
+// threads; each wave walks R consecutive rows of one 64-wide column strip.
+struct Geom {
+    int nx, ny, R;
+    dim3 grid, block;
+    int nblocks;
+};
+
+Geom make_geom(int nx, int ny, int R_override = 0);
+
+// Physical constants and scheme switches handed to every physics kernel.
+struct Phys {
+    int nx, ny;
+    double dx, dy, dx2, dy2;
+    double rho_w, rho_sw, rho_i, g, nu, n, omega, L, br, lr, ct, cw, p_atm, b_min, b_max, e_v;
+    Expo n_exp, n1_exp, inv_n_exp;
+    int with_sensible, with_open_by_sliding, kfs, gap_scheme, sliding_law;
+    double C;
+};
+
+// Kernel launchers (shk_kernels.hip).  All are asynchronous on `st`.
+namespace launch {
+void ic_cells(const Phys& P, Fields F, hipStream_t st);
+void stagger_b(const Phys& P, Fields F, hipStream_t st);
+// mode 0: refresh after a solve (pw from h); mode 1: initial conditions (pw as stored).
+// check != 0 also reduces max|h - h_prev| and max|h| and sets the Picard flag.
+void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,
+             Scalars* sc, hipStream_t st);
+void gap(const Phys& P, Fields F, double dt, hipStream_t st);
+void fill_ieb(int n, double* ieb, double uniform, const double* moulin, hipStream_t st);
+void zero(double* p, size_t n, hipStream_t st);
+
+template <class T>
+void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
+              hipStream_t st);
+template <class T>
+void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st);
+
+// Lanczos-from-CG bound estimate and the Chebyshev coefficients.
+template <class T>
+void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Scalars* sc, hipStream_t st);
+template <class T>
+void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* p_old, T* p_new,
+                    T* Ap, int k, double* partials, Scalars* sc, hipStream_t st);
+template <class T>
+void lanczos_step_b(const Geom& G, const T* Ap, T* r, int k, double* partials, Scalars* sc, hipStream_t st);
+void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStream_t st);
+
+// Preconditioned CG pieces (mode 0: set-up, mode 1: iteration).
+template <class T>
+void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* b, const T* x0, T* r, T* z, T* xacc,
+             T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st);
+template <class T>
+void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* s_in, const T* pc_in,
+               T* s_out, T* pc_out, T* y, int k, int degree, int mode, double* partials, Scalars* sc,
+               hipStream_t st);
+template <class T>
+void cheb_degree1(const Geom& G, const T* aP, const T* r, T* y, int mode, double* partials, Scalars* sc,
+                  hipStream_t st);
+template <class T>
+void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
+               double* partials, Scalars* sc, hipStream_t st);
+template <class T>
+void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
+               Scalars* sc, hipStream_t st);
+void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st);
+}  // namespace launch
+
+}  // namespace shk

@@ -1,0 +1,1146 @@
+// gfx950 kernels of libshakti_hip.so.
+//
+// Memory layout: every field is column-major with ix fastest (reference
+// src/state.jl:13-57): centre (nx, ny) at ix + iy*nx, x-faces (nx+1, ny) at
+// ix + iy*(nx+1), y-faces (nx, ny+1) at ix + iy*nx.
+//
+// The linear system is stored in the symmetric face layout: aP (diagonal),
+// aE (coupling of cell (ix,iy) to (ix+1,iy)) and aN (coupling to (ix,iy+1)).
+// The reference's aW(ix,iy) equals aE(ix-1,iy) bit for bit and aS(ix,iy)
+// equals aN(ix,iy-1) (both faces evaluate the same commutative K_face
+// expression, and every coupling to a non-GROUNDED cell is zero on both
+// sides), so the operator apply reads 3 coefficient arrays instead of 5.
+//
+// Stencil kernels march: each wave owns a 64-wide column strip and walks R
+// rows upwards, keeping the operand of rows iy-1, iy, iy+1 and aN(iy-1) in
+// registers, so every array is read from HBM once per launch (plus one halo
+// row per R rows).  Workgroups are renumbered so that the blocks one XCD
+// receives (the hardware deals workgroups round-robin over the 8 XCDs) form
+// one contiguous band of rows, keeping shared halo rows in that XCD's L2.
+//
+// Reductions (dot products, maxima) are deterministic: a fixed wave-shuffle
+// tree, a fixed cross-wave order, one partial per workgroup, and the last
+// workgroup to finish (agent-scope fence + atomic counter) sums the partials
+// in a fixed order and updates the device-resident solver scalars.
+//
+// Floating-point expressions keep the reference's evaluation order and the
+// library is built with -ffp-contract=off, so the FP64 path rounds exactly as
+// the Julia source is written (only pow() may differ in the last ulp).
+
+#include <math.h>
+
+#include "shk_internal.hpp"
+
+namespace shk {
+
+// --------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double jl_pow_int(double x, int n) {
+    // Julia's Base.:^(::Float64, ::Integer) (power by squaring with an
+    // error-compensated product, n == 3 as x*x*x); n == 2 reduces to x*x.
+    if (n == 2) return x * x;
+    if (n == 3) return x * x * x;
+    if (n == 1) return x;
+    if (n == 0) return 1.0;
+    if (n < 0) return 1.0 / jl_pow_int(x, -n);
+    double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+    while (n > 1) {
+        if (n & 1) {
+            double err = y * xnlo + x * ynlo;
+            double yy = x * y;
+            ynlo = fma(x, y, -yy);
+            y = yy;
+            ynlo += err;
+        }
+        double err = x * 2.0 * xnlo;
+        double xx = x * x;
+        xnlo = fma(x, x, -xx);
+        x = xx;
+        xnlo += err;
+        n >>= 1;
+    }
+    double err = y * xnlo + x * ynlo;
+    if (isfinite(x) && isfinite(err)) return fma(x, y, err);
+    return x * y;
+}
+
+__device__ __forceinline__ double cpow(double x, const Expo& e) { return e.is_int ? jl_pow_int(x, e.k) : pow(x, e.e); }
+
+__device__ __forceinline__ double jl_sign(double x) { return (x == 0.0 || isnan(x)) ? x : copysign(1.0, x); }
+__device__ __forceinline__ double jl_boolmul(bool t, double x) { return t ? x : copysign(0.0, x); }
+__device__ __forceinline__ double jl_min(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a < b) return a;
+    if (b < a) return b;
+    return signbit(a) ? a : b;
+}
+__device__ __forceinline__ double jl_max0(double v) { return isnan(v) ? v : (v > 0.0 ? v : 0.0); }
+__device__ __forceinline__ double jl_clamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+
+__device__ __forceinline__ size_t cidx(int ix, int iy, int nx) { return (size_t)ix + (size_t)iy * (size_t)nx; }
+__device__ __forceinline__ size_t xidx(int ix, int iy, int nx) { return (size_t)ix + (size_t)iy * (size_t)(nx + 1); }
+
+// Virtual workgroup id: XCD k (hardware id % 8) receives one contiguous range.
+__device__ __forceinline__ void virtual_block(int& vbx, int& vby) {
+    const unsigned nb = gridDim.x * gridDim.y;
+    const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
+    const unsigned q = nb / 8, r = nb % 8, xcd = bid % 8, local = bid / 8;
+    const unsigned v = xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+    vbx = (int)(v % gridDim.x);
+    vby = (int)(v / gridDim.x);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, WAVE);
+        v = (isnan(o) || o > v) ? o : v;
+    }
+    return v;
+}
+
+// Sum (or max, with MAX) of up to two values over the workgroup; result valid
+// in thread 0.  Fixed order, so deterministic.
+template <int NV, bool MAX>
+__device__ __forceinline__ void block_reduce(double (&v)[NV]) {
+    __shared__ double red[WAVES][2];
+    const int lane = threadIdx.x, w = threadIdx.y;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = MAX ? wave_max(v[i]) : wave_sum(v[i]);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[w][i] = v[i];
+    __syncthreads();
+    if (lane == 0 && w == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double a = red[0][i];
+            for (int k = 1; k < WAVES; ++k) {
+                double b = red[k][i];
+                a = MAX ? ((isnan(b) || b > a) ? b : a) : a + b;
+            }
+            v[i] = a;
+        }
+    }
+    __syncthreads();
+}
+
+// Grid-wide reduction with last-workgroup finalisation.  Every workgroup
+// calls it once with its per-thread contributions; the last one to arrive
+// runs fin(total0, total1) in thread (0,0).  partials must hold 2*nblocks.
+template <int NV, bool MAX, class Fin>
+__device__ __forceinline__ void grid_reduce(double (&v)[NV], double* partials, Scalars* sc, Fin fin) {
+    __shared__ bool am_last;
+    const unsigned nb = gridDim.x * gridDim.y;
+    const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
+    block_reduce<NV, MAX>(v);
+    const int tid = threadIdx.x + threadIdx.y * WAVE;
+    if (tid == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) partials[i * nb + bid] = v[i];
+        __threadfence();
+        unsigned t = atomicAdd(&sc->counter, 1u);
+        am_last = (t == nb - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double a = 0.0;
+        bool first = true;
+        for (unsigned k = tid; k < nb; k += WAVE * WAVES) {
+            const double b = partials[i * nb + k];
+            if (first) { a = b; first = false; }
+            else a = MAX ? ((isnan(b) || b > a) ? b : a) : a + b;
+        }
+        acc[i] = a;
+    }
+    block_reduce<NV, MAX>(acc);
+    if (tid == 0) {
+        fin(acc[0], NV > 1 ? acc[NV - 1] : 0.0);
+        sc->counter = 0;
+    }
+}
+
+// --------------------------------------------------------- field kernels
+
+struct FV {  // named views of Fields
+    double *h, *pw, *po, *b, *beta, *abs_ub, *mdot, *shear, *potential, *sensible, *Re, *K, *G, *zb, *zs, *H, *ieb,
+        *lambda, *A_visc, *N, *mask;
+    double *dhdx, *q_x, *Re_x, *b_x, *ub_x, *taub_x, *dpwdx, *valid_x;
+    double *dhdy, *q_y, *Re_y, *b_y, *ub_y, *taub_y, *dpwdy, *valid_y;
+    __device__ __forceinline__ explicit FV(const Fields& F) {
+        double** p = (double**)&h;
+#pragma unroll
+        for (int i = 0; i < SHK_NUM_FIELDS; ++i) p[i] = F.f[i];
+    }
+};
+
+// Initial conditions, pointwise part (src/initial_conditions.jl:44-96 with
+// src/mask.jl:62-150, src/static_fields.jl:8-74, src/gap_height.jl:47-73):
+// face masks, lambda, b zeroed off GROUNDED, H, beta, masked ub, abs_ub, po,
+// the initial pw (po/2 or the Dirichlet value) and h from pw.
+__global__ __launch_bounds__(256) void k_ic_cells(Phys P, Fields F) {
+    const int nx = P.nx, ny = P.ny;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    FV s(F);
+    const size_t c = cidx(ix, iy, nx);
+    const double m = s.mask[c];
+    // valid_x on the west face (and the east boundary face), valid_y likewise
+    {
+        double v = 1.0;
+        if (ix > 0) v = (s.mask[c - 1] != OTHER_BASIN && m != OTHER_BASIN) ? 1.0 : 0.0;
+        s.valid_x[xidx(ix, iy, nx)] = v;
+        if (ix == nx - 1) s.valid_x[xidx(nx, iy, nx)] = 1.0;
+        double w = 1.0;
+        if (iy > 0) w = (s.mask[c - nx] != OTHER_BASIN && m != OTHER_BASIN) ? 1.0 : 0.0;
+        s.valid_y[cidx(ix, iy, nx)] = w;
+        if (iy == ny - 1) s.valid_y[cidx(ix, ny, nx)] = 1.0;
+    }
+    s.lambda[c] = 1.5 * s.A_visc[c];
+    double b = s.b[c];
+    if (m != GROUNDED) b = 0.0;
+    s.b[c] = b;
+    const double H = s.zs[c] - (s.zb[c] + b);
+    s.H[c] = H;
+    if (P.with_open_by_sliding) s.beta[c] = jl_max0((P.br - b) / P.lr);
+    // apply_mask_to_sliding!: a face keeps ub only between two GROUNDED cells
+    auto ubx_masked = [&](int fx) -> double {
+        if (fx == 0 || fx == nx) return 0.0;
+        return (s.mask[cidx(fx - 1, iy, nx)] == GROUNDED && s.mask[cidx(fx, iy, nx)] == GROUNDED)
+                   ? s.ub_x[xidx(fx, iy, nx)] : 0.0;
+    };
+    auto uby_masked = [&](int fy) -> double {
+        if (fy == 0 || fy == ny) return 0.0;
+        return (s.mask[cidx(ix, fy - 1, nx)] == GROUNDED && s.mask[cidx(ix, fy, nx)] == GROUNDED)
+                   ? s.ub_y[cidx(ix, fy, nx)] : 0.0;
+    };
+    const double uw = ubx_masked(ix), ue = ubx_masked(ix + 1), us = uby_masked(iy), un = uby_masked(iy + 1);
+    const double ubx = (uw + ue) / 2.0, uby = (us + un) / 2.0;
+    s.abs_ub[c] = sqrt(ubx * ubx + uby * uby);
+    // Each face is written by exactly one thread (the cell east/north of it;
+    // the last column/row also writes the boundary face).  A neighbour may
+    // read a face before or after it is overwritten: masking is idempotent
+    // (a face keeps its value or becomes 0), so both reads give the same.
+    s.ub_x[xidx(ix, iy, nx)] = uw;
+    if (ix == nx - 1) s.ub_x[xidx(nx, iy, nx)] = 0.0;
+    s.ub_y[cidx(ix, iy, nx)] = us;
+    if (iy == ny - 1) s.ub_y[cidx(ix, ny, nx)] = 0.0;
+    const double po = P.rho_i * P.g * H;
+    s.po[c] = po;
+    double pw = po / 2.0;
+    if (m == OCEAN) pw = P.p_atm - P.rho_sw * P.g * jl_min(s.zb[c], 0.0);
+    else if (m == LAND || m == OTHER_BASIN) pw = P.p_atm;
+    s.pw[c] = pw;
+    s.h[c] = pw / (P.rho_w * P.g) + s.zb[c];
+}
+
+// b_x, b_y from b (src/gap_height.jl:75-115); launched over the union of the
+// two face grids, (nx+1) x (ny+1).
+__global__ __launch_bounds__(256) void k_stagger_b(Phys P, Fields F) {
+    const int nx = P.nx, ny = P.ny;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    FV s(F);
+    if (ix <= nx && iy < ny) {
+        double v;
+        if (ix == 0) v = s.b[cidx(0, iy, nx)];
+        else if (ix == nx) v = s.b[cidx(nx - 1, iy, nx)];
+        else v = (s.b[cidx(ix, iy, nx)] + s.b[cidx(ix - 1, iy, nx)]) / 2.0;
+        s.b_x[xidx(ix, iy, nx)] = v;
+    }
+    if (ix < nx && iy <= ny) {
+        double v;
+        if (iy == 0) v = s.b[cidx(ix, 0, nx)];
+        else if (iy == ny) v = s.b[cidx(ix, ny - 1, nx)];
+        else v = (s.b[cidx(ix, iy, nx)] + s.b[cidx(ix, iy - 1, nx)]) / 2.0;
+        s.b_y[cidx(ix, iy, nx)] = v;
+    }
+}
+
+// Face flux and Reynolds number, exact quadratic (src/water_flux.jl:192-216).
+__device__ __forceinline__ void qre(const Phys& P, double bf, double grad, double& q, double& Re) {
+    const double G = fabs(grad);
+    const double D = (bf * bf * bf * P.g * G) / (12.0 * (P.nu * P.nu));
+    const double R = 2.0 * D / (1.0 + sqrt(1.0 + 4.0 * P.omega * D));
+    Re = R;
+    q = -jl_sign(grad) * P.nu * R;
+}
+
+// RegularizedCoulomb face stress (src/sliding_law.jl:213-249).
+__device__ __forceinline__ double taub_face(const Phys& P, double Nf, double abs_v, double lf, double ub) {
+    if (abs_v > 0.0)
+        return Nf * P.C * cpow(abs_v / (abs_v + cpow(fabs(Nf), P.n_exp) * lf), P.inv_n_exp) * (ub / abs_v);
+    return 0.0;
+}
+
+// The refresh after a linear solve, fused into one pass (the reference runs
+// 9 launches: src/elliptic_solver.jl:193-206).  Each thread owns one cell and
+// writes its west/south faces (and the east/north boundary faces at the last
+// column/row); the east/north face values a cell also needs for mdot and Re
+// are recomputed from the same inputs, so every face is evaluated by the
+// same expression on both sides.  mode 1 is the initial-condition variant
+// (pw read as stored instead of computed from h, src/initial_conditions.jl:81-124).
+// With check, also reduces max|h - h_prev| and max|h| for the Picard test
+// (src/elliptic_solver.jl:153-169).
+template <int MODE, bool CHECK>
+__global__ __launch_bounds__(256) void k_derived(Phys P, Fields F, const double* __restrict__ h_prev, int R,
+                                                 double* partials, Scalars* sc) {
+    const int nx = P.nx, ny = P.ny;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const int ix = vbx * WAVE + threadIdx.x;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
+    FV s(F);
+    const double rg = P.rho_w * P.g;
+    double dmax = 0.0, hmax = 0.0;
+    bool any_nan = false;
+    if (ix < nx) {
+        for (int iy = iy0; iy < iy1; ++iy) {
+            const size_t c = cidx(ix, iy, nx);
+            auto PW = [&](size_t k) -> double { return MODE == 1 ? s.pw[k] : rg * (s.h[k] - s.zb[k]); };
+            auto NN = [&](size_t k) -> double { return jl_boolmul(s.mask[k] == GROUNDED, s.po[k] - PW(k)); };
+            const double hc = s.h[c], pwc = PW(c), Nc = NN(c);
+            // ---- x faces: west (fx = ix) and east (fx = ix+1)
+            double qx[2], dhx[2], dpx[2], rex[2], tbx[2], ubx[2];
+            for (int side = 0; side < 2; ++side) {
+                const int fx = ix + side;
+                const size_t k = xidx(fx, iy, nx);
+                double dh, dp;
+                if (fx > 0 && fx < nx) {
+                    const size_t cw = cidx(fx - 1, iy, nx), ce = cidx(fx, iy, nx);
+                    const double v = s.valid_x[k];
+                    dh = ((s.h[ce] - s.h[cw]) / P.dx) * v;
+                    dp = ((PW(ce) - PW(cw)) / P.dx) * v;
+                } else {
+                    dh = s.dhdx[k];
+                    dp = s.dpwdx[k];
+                }
+                double q, Re;
+                qre(P, s.b_x[k], dh, q, Re);
+                double tb;
+                if (P.sliding_law == 0) {
+                    double Nf, av, lf;
+                    if (fx == 0) { Nf = NN(cidx(0, iy, nx)); av = s.abs_ub[cidx(0, iy, nx)]; lf = s.lambda[cidx(0, iy, nx)]; }
+                    else if (fx == nx) { const size_t cc = cidx(nx - 1, iy, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
+                    else {
+                        const size_t cw = cidx(fx - 1, iy, nx), ce = cidx(fx, iy, nx);
+                        Nf = (NN(ce) + NN(cw)) / 2.0;
+                        lf = (s.lambda[ce] + s.lambda[cw]) / 2.0;
+                        av = (s.abs_ub[ce] + s.abs_ub[cw]) / 2.0;
+                    }
+                    tb = taub_face(P, Nf, av, lf, s.ub_x[k]);
+                } else {
+                    tb = s.taub_x[k];
+                }
+                qx[side] = q; dhx[side] = dh; dpx[side] = dp; rex[side] = Re; tbx[side] = tb; ubx[side] = s.ub_x[k];
+            }
+            // ---- y faces: south (fy = iy) and north (fy = iy+1)
+            double qy[2], dhy[2], dpy[2], rey[2], tby[2], uby[2];
+            for (int side = 0; side < 2; ++side) {
+                const int fy = iy + side;
+                const size_t k = cidx(ix, fy, nx);
+                double dh, dp;
+                if (fy > 0 && fy < ny) {
+                    const size_t cs = cidx(ix, fy - 1, nx), cn = cidx(ix, fy, nx);
+                    const double v = s.valid_y[k];
+                    dh = ((s.h[cn] - s.h[cs]) / P.dy) * v;
+                    dp = ((PW(cn) - PW(cs)) / P.dy) * v;
+                } else {
+                    dh = s.dhdy[k];
+                    dp = s.dpwdy[k];
+                }
+                double q, Re;
+                qre(P, s.b_y[k], dh, q, Re);
+                double tb;
+                if (P.sliding_law == 0) {
+                    double Nf, av, lf;
+                    if (fy == 0) { const size_t cc = cidx(ix, 0, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
+                    else if (fy == ny) { const size_t cc = cidx(ix, ny - 1, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
+                    else {
+                        const size_t cs = cidx(ix, fy - 1, nx), cn = cidx(ix, fy, nx);
+                        Nf = (NN(cn) + NN(cs)) / 2.0;
+                        lf = (s.lambda[cn] + s.lambda[cs]) / 2.0;
+                        av = (s.abs_ub[cn] + s.abs_ub[cs]) / 2.0;
+                    }
+                    tb = taub_face(P, Nf, av, lf, s.ub_y[k]);
+                } else {
+                    tb = s.taub_y[k];
+                }
+                qy[side] = q; dhy[side] = dh; dpy[side] = dp; rey[side] = Re; tby[side] = tb; uby[side] = s.ub_y[k];
+            }
+            // ---- writes: own west/south faces, plus boundary faces at the edge
+            {
+                const size_t kw = xidx(ix, iy, nx), ks = cidx(ix, iy, nx);
+                if (ix > 0) { s.dhdx[kw] = dhx[0]; s.dpwdx[kw] = dpx[0]; }
+                s.q_x[kw] = qx[0]; s.Re_x[kw] = rex[0];
+                if (P.sliding_law == 0) s.taub_x[kw] = tbx[0];
+                if (ix == nx - 1) {
+                    const size_t ke = xidx(nx, iy, nx);
+                    s.q_x[ke] = qx[1]; s.Re_x[ke] = rex[1];
+                    if (P.sliding_law == 0) s.taub_x[ke] = tbx[1];
+                }
+                if (iy > 0) { s.dhdy[ks] = dhy[0]; s.dpwdy[ks] = dpy[0]; }
+                s.q_y[ks] = qy[0]; s.Re_y[ks] = rey[0];
+                if (P.sliding_law == 0) s.taub_y[ks] = tby[0];
+                if (iy == ny - 1) {
+                    const size_t kn = cidx(ix, ny, nx);
+                    s.q_y[kn] = qy[1]; s.Re_y[kn] = rey[1];
+                    if (P.sliding_law == 0) s.taub_y[kn] = tby[1];
+                }
+            }
+            if (MODE == 0) s.pw[c] = pwc;
+            s.N[c] = Nc;
+            // ---- Re (src/water_flux.jl:107-112), mdot (src/melt_rate.jl:106-137), K (:221-226)
+            const double Re_c = (rex[0] + rex[1] + rey[0] + rey[1]) / 4.0;
+            s.Re[c] = Re_c;
+            const double sh = fabs((ubx[1] * tbx[1] + ubx[0] * tbx[0]) / 2.0 + (uby[1] * tby[1] + uby[0] * tby[0]) / 2.0);
+            const double pot = fabs((qx[1] * dhx[1] + qx[0] * dhx[0]) / 2.0 + (qy[1] * dhy[1] + qy[0] * dhy[0]) / 2.0);
+            s.shear[c] = sh;
+            s.potential[c] = pot;
+            const double Linv = 1.0 / P.L;
+            if (P.with_sensible) {
+                const double sens = (qx[1] * dpx[1] + qx[0] * dpx[0]) / 2.0 + (qy[1] * dpy[1] + qy[0] * dpy[0]) / 2.0;
+                s.sensible[c] = sens;
+                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot + P.ct * P.cw * P.rho_w * sens);
+            } else {
+                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot);
+            }
+            const double bc = s.b[c];
+            s.K[c] = (bc * bc * bc * P.g) / (12.0 * P.nu * (1.0 + P.omega * Re_c));
+            if (CHECK) {
+                const double d = fabs(hc - h_prev[c]), a = fabs(hc);
+                any_nan |= isnan(d) || isnan(a);
+                dmax = d > dmax ? d : dmax;
+                hmax = a > hmax ? a : hmax;
+            }
+        }
+    }
+    if (CHECK) {
+        double v[2] = {any_nan ? NAN : dmax, hmax};
+        grid_reduce<2, true>(v, partials, sc, [&](double dm, double hm) {
+            sc->dmax = dm;
+            sc->hmax = hm;
+            sc->picard_nan = isnan(dm) || isnan(hm);
+            sc->picard_converged = (!isnan(dm) && !isnan(hm) && dm / (hm + EPS64) < sc->picard_tol) ? 1 : 0;
+        });
+    }
+}
+
+// compute_b! (src/gap_height.jl:117-135) then beta (src/gap_height.jl:47-52)
+__global__ __launch_bounds__(256) void k_gap(Phys P, Fields F, double dt) {
+    const int n = P.nx * P.ny;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    FV s(F);
+    double b = s.b[i];
+    if (s.mask[i] == GROUNDED) {
+        const double N = s.N[i];
+        const double Np = cpow(fabs(N), P.n1_exp);
+        double v;
+        if (P.gap_scheme == 0)
+            v = (b + dt * (s.mdot[i] / P.rho_i + s.beta[i] * s.abs_ub[i])) / (1.0 + dt * s.A_visc[i] * Np * N);
+        else
+            v = b + dt * (s.mdot[i] / P.rho_i + s.beta[i] * s.abs_ub[i] - s.A_visc[i] * Np * N * b);
+        b = jl_clamp(v, P.b_min, P.b_max);
+        s.b[i] = b;
+    }
+    if (P.with_open_by_sliding) s.beta[i] = jl_max0((P.br - b) / P.lr);
+}
+
+__global__ __launch_bounds__(256) void k_fill_ieb(int n, double* ieb, double u, const double* moulin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ieb[i] = moulin ? u + moulin[i] : u;
+}
+
+// ------------------------------------------------------------- assembly
+
+// src/linear_solver.jl:13-27
+__device__ __forceinline__ double dirichlet_head(const Phys& P, double m, double zb) {
+    if (m == OCEAN) {
+        const double pw_bc = P.p_atm - P.rho_sw * P.g * jl_min(zb, 0.0);
+        return pw_bc / (P.rho_w * P.g) + zb;
+    }
+    return P.p_atm / (P.rho_w * P.g) + zb;
+}
+
+// src/k_face_scheme.jl:32-61
+__device__ __forceinline__ double bK(const Phys& P, double K1, double K2, double m2) {
+    if (m2 == OTHER_BASIN) return 0.0;
+    if (m2 == OCEAN || m2 == LAND) return K1;
+    if (P.kfs == 0) return (K1 + K2) / 2.0;
+    return (2.0 * K1 * K2) / (K1 + K2 + EPS64);
+}
+
+// update_MFLS_kernel! / update_MFLS_parabolic_kernel! (src/linear_solver.jl:438-610)
+// fused with the K_face scheme and Dirichlet folding, writing the symmetric
+// layout (aP, aE, aN) and rhs.  The Jacobi diagonal is aP itself
+// (update_diag_precond!, src/linear_solver.jl:697-700).
+template <bool PARABOLIC, class T>
+__global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, int R, T* __restrict__ aP,
+                                                  T* __restrict__ aE, T* __restrict__ aN, T* __restrict__ rhs) {
+    const int nx = P.nx, ny = P.ny;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const int ix = vbx * WAVE + threadIdx.x;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
+    if (ix >= nx) return;
+    FV s(F);
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const size_t c = cidx(ix, iy, nx);
+        const double m = s.mask[c];
+        if (m == OCEAN || m == LAND) {
+            aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
+            rhs[c] = dirichlet_head(P, m, s.zb[c]);
+            continue;
+        }
+        if (m == OTHER_BASIN) {
+            aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
+            rhs[c] = s.h[c];
+            continue;
+        }
+        const double Kc = s.K[c];
+        double mE = 0, mW = 0, mN = 0, mS = 0;
+        double fE = 0.0, fW = 0.0, fN = 0.0, fS = 0.0;
+        if (ix < nx - 1) { mE = s.mask[c + 1]; fE = bK(P, Kc, s.K[c + 1], mE) / P.dx2; }
+        if (ix > 0) { mW = s.mask[c - 1]; fW = bK(P, Kc, s.K[c - 1], mW) / P.dx2; }
+        if (iy < ny - 1) { mN = s.mask[c + nx]; fN = bK(P, Kc, s.K[c + nx], mN) / P.dy2; }
+        if (iy > 0) { mS = s.mask[c - nx]; fS = bK(P, Kc, s.K[c - nx], mS) / P.dy2; }
+        const double Nc = s.N[c], bc = s.b[c], Ac = s.A_visc[c];
+        const double Np = cpow(fabs(Nc), P.n1_exp);
+        const double diag = PARABOLIC ? (fE + fW + fN + fS) + P.e_v / dt
+                                      : (fE + fW + fN + fS) + P.n * P.rho_w * P.g * Ac * Np * bc;
+        double drhs = 0.0;
+        double cE = 0.0, cN = 0.0;
+        if (ix < nx - 1) { if (mE == OCEAN || mE == LAND) drhs += fE * dirichlet_head(P, mE, s.zb[c + 1]); else cE = fE; }
+        if (ix > 0) { if (mW == OCEAN || mW == LAND) drhs += fW * dirichlet_head(P, mW, s.zb[c - 1]); }
+        if (iy < ny - 1) { if (mN == OCEAN || mN == LAND) drhs += fN * dirichlet_head(P, mN, s.zb[c + nx]); else cN = fN; }
+        if (iy > 0) { if (mS == OCEAN || mS == LAND) drhs += fS * dirichlet_head(P, mS, s.zb[c - nx]); }
+        // a coupling is kept only between two GROUNDED cells (the neighbour's
+        // own row is an identity row otherwise, with its coupling zero)
+        if (mE != GROUNDED) cE = 0.0;
+        if (mN != GROUNDED) cN = 0.0;
+        double r;
+        if (PARABOLIC)
+            r = s.mdot[c] * (1.0 / P.rho_w - 1.0 / P.rho_i) - s.beta[c] * s.abs_ub[c] + Ac * Np * Nc * bc + s.ieb[c] +
+                (P.e_v / dt) * s.h[c] + drhs;
+        else
+            r = s.mdot[c] * (1.0 / P.rho_w - 1.0 / P.rho_i) - s.beta[c] * s.abs_ub[c] + Ac * Np * Nc * bc +
+                Ac * Np * (P.n * P.rho_w * P.g * s.h[c]) * bc + s.ieb[c] + drhs;
+        aP[c] = (T)diag;
+        aE[c] = (T)cE;
+        aN[c] = (T)cN;
+        rhs[c] = (T)r;
+    }
+}
+
+// ------------------------------------------------------ marching stencil
+
+// y = A u at every cell of this wave's strip, u given by an operand functor
+// (so u can be a stored vector or an expression of several), with the
+// epilogue epi(c, Au, u_c) run per cell.  Matches stencil_matvec_kernel!
+// (src/linear_solver.jl:616-633): aP*u, then minus the E, W, N, S terms in
+// that order, each only where the neighbour exists.
+template <class T, class Opd, class Epi>
+__device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, const T* __restrict__ aP,
+                                      const T* __restrict__ aE, const T* __restrict__ aN, Opd opd, Epi epi) {
+    const int ix = vbx * WAVE + threadIdx.x;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
+    if (ix >= nx || iy0 >= ny) return;
+    T um = 0, aNm = 0;
+    if (iy0 > 0) {
+        um = opd(cidx(ix, iy0 - 1, nx));
+        aNm = aN[cidx(ix, iy0 - 1, nx)];
+    }
+    T uc = opd(cidx(ix, iy0, nx));
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const size_t c = cidx(ix, iy, nx);
+        const T up = (iy + 1 < ny) ? opd(c + nx) : T(0);
+        const T aPc = aP[c], aEc = aE[c], aNc = aN[c];
+        T y = aPc * uc;
+        if (ix < nx - 1) y -= aEc * opd(c + 1);
+        if (ix > 0) y -= aE[c - 1] * opd(c - 1);
+        if (iy < ny - 1) y -= aNc * up;
+        if (iy > 0) y -= aNm * um;
+        epi(c, y, uc);
+        um = uc;
+        uc = up;
+        aNm = aNc;
+    }
+}
+
+// pointwise walk over the same geometry
+template <class Epi>
+__device__ __forceinline__ void walk(int nx, int ny, int R, int vbx, int vby, Epi epi) {
+    const int ix = vbx * WAVE + threadIdx.x;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
+    if (ix >= nx) return;
+    for (int iy = iy0; iy < iy1; ++iy) epi(cidx(ix, iy, nx));
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_matvec(int nx, int ny, int R, const T* __restrict__ aP,
+                                                const T* __restrict__ aE, const T* __restrict__ aN,
+                                                const T* __restrict__ x, T* __restrict__ y) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> T { return x[k]; },
+             [&](size_t c, T Au, T) { y[c] = Au; });
+}
+
+// ------------------------------------------- Lanczos bound estimate (a9)
+
+// estimate_eigenvalue_bounds (src/preconditioner.jl:89-134): r = rhs, p = r,
+// gamma = r'r (x0 = 0).  p_zero is zeroed so step 0 can form p = r + 0*p.
+template <class T>
+__global__ __launch_bounds__(256) void k_lz_init(int nx, int ny, int R, const T* __restrict__ rhs,
+                                                 T* __restrict__ r, T* __restrict__ p_zero, double* partials,
+                                                 Scalars* sc) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    double acc = 0.0;
+    walk(nx, ny, R, vbx, vby, [&](size_t c) {
+        const T v = rhs[c];
+        r[c] = v;
+        p_zero[c] = 0;
+        acc += (double)v * (double)v;
+    });
+    double v[1] = {acc};
+    grid_reduce<1, false>(v, partials, sc, [&](double g, double) {
+        sc->lz_gamma = g;
+        sc->beta = 0.0;
+    });
+}
+
+// step k: p = r + beta_{k-1} p_old (beta = 0 at k = 0), Ap = (A p) ./ d,
+// alpha_k = gamma / p'Ap.
+template <class T>
+__global__ __launch_bounds__(256) void k_lz_step_a(int nx, int ny, int R, const T* __restrict__ aP,
+                                                   const T* __restrict__ aE, const T* __restrict__ aN,
+                                                   const T* __restrict__ r, const T* __restrict__ p_old,
+                                                   T* __restrict__ p_new, T* __restrict__ Ap, int k,
+                                                   double* partials, Scalars* sc) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const T beta = (T)sc->beta;
+    double acc = 0.0;
+    march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return r[j] + beta * p_old[j]; },
+             [&](size_t c, T Au, T u) {
+                 const T a = Au / aP[c];
+                 p_new[c] = u;
+                 Ap[c] = a;
+                 acc += (double)u * (double)a;
+             });
+    double v[1] = {acc};
+    grid_reduce<1, false>(v, partials, sc, [&](double pAp, double) { sc->lz_alpha[k] = sc->lz_gamma / pAp; });
+}
+
+// step k (k < nsteps-1): r -= alpha_k Ap; beta_k = r'r / gamma.
+template <class T>
+__global__ __launch_bounds__(256) void k_lz_step_b(int nx, int ny, int R, const T* __restrict__ Ap,
+                                                   T* __restrict__ r, int k, double* partials, Scalars* sc) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const T alpha = (T)sc->lz_alpha[k];
+    double acc = 0.0;
+    walk(nx, ny, R, vbx, vby, [&](size_t c) {
+        const T v = r[c] - alpha * Ap[c];
+        r[c] = v;
+        acc += (double)v * (double)v;
+    });
+    double v[1] = {acc};
+    grid_reduce<1, false>(v, partials, sc, [&](double gn, double) {
+        const double b = gn / sc->lz_gamma;
+        sc->lz_beta[k] = b;
+        sc->beta = b;
+        sc->lz_gamma = gn;
+    });
+}
+
+// Sturm count: eigenvalues of the tridiagonal (a, e2 = e^2) below x.
+__device__ int sturm_count(int m, const double* a, const double* e2, double x) {
+    int cnt = 0;
+    double q = a[0] - x;
+    if (q < 0) cnt++;
+    for (int i = 1; i < m; ++i) {
+        if (q == 0.0) q = 1e-300;
+        q = a[i] - x - e2[i - 1] / q;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+
+// One wave: builds the Lanczos tridiagonal from the CG coefficients
+// (src/preconditioner.jl:115-123), finds its extreme eigenvalues by Sturm
+// multisection (64 points per round; textbook bisection, Golub & Van Loan
+// 8.4.1; stands in for LAPACK eigvals(SymTridiagonal) at :125), scales by
+// 0.9 / 1.1, and adopts the bounds only if finite, positive and non-empty
+// (update_chebyshev_bounds!, :197-233).  Then precomputes this solve's
+// Chebyshev alpha/beta sequence (ldiv!, :239-264).
+__global__ __launch_bounds__(64) void k_cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc) {
+    __shared__ double a[MAX_LANCZOS], e2[MAX_LANCZOS];
+    __shared__ int bad;
+    __shared__ double lo_s, hi_s;
+    const int t = threadIdx.x;
+    if (use_estimate) {
+        if (t == 0) {
+            bad = 0;
+            a[0] = 1.0 / sc->lz_alpha[0];
+            for (int k = 1; k < nsteps; ++k) a[k] = 1.0 / sc->lz_alpha[k] + sc->lz_beta[k - 1] / sc->lz_alpha[k - 1];
+            double lo = INFINITY, hi = -INFINITY;
+            for (int k = 0; k < nsteps; ++k) {
+                const double ek = k < nsteps - 1 ? sqrt(sc->lz_beta[k]) / sc->lz_alpha[k] : 0.0;
+                if (k < nsteps - 1) {
+                    e2[k] = ek * ek;
+                    if (!isfinite(ek)) bad = 1;
+                }
+                if (!isfinite(a[k])) bad = 1;
+                double rr = 0.0;
+                if (k > 0) rr += fabs(sqrt(sc->lz_beta[k - 1]) / sc->lz_alpha[k - 1]);
+                if (k < nsteps - 1) rr += fabs(ek);
+                lo = fmin(lo, a[k] - rr);
+                hi = fmax(hi, a[k] + rr);
+            }
+            const double span = fmax(fabs(lo), fabs(hi));
+            lo_s = lo - (1e-14 * span + 1e-300);
+            hi_s = hi + (1e-14 * span + 1e-300);
+        }
+        __syncthreads();
+        double res[2] = {NAN, NAN};
+        if (!bad) {
+            for (int which = 0; which < 2; ++which) {
+                // smallest: first x with count >= 1; largest: first x with count >= m
+                const int target = which == 0 ? 1 : nsteps;
+                double lo = lo_s, hi = hi_s;
+                for (int round = 0; round < 24; ++round) {
+                    const double x = lo + (hi - lo) * (double)(t + 1) / 65.0;
+                    const int ok = sturm_count(nsteps, a, e2, x) >= target;
+                    // first lane whose point satisfies the count
+                    const unsigned long long mask = __ballot(ok);
+                    double nlo, nhi;
+                    if (mask == 0ull) { nlo = lo + (hi - lo) * 64.0 / 65.0; nhi = hi; }
+                    else {
+                        const int j = __ffsll((long long)mask) - 1;
+                        nlo = j == 0 ? lo : lo + (hi - lo) * (double)j / 65.0;
+                        nhi = lo + (hi - lo) * (double)(j + 1) / 65.0;
+                    }
+                    if (!(nhi - nlo < hi - lo)) break;
+                    lo = nlo;
+                    hi = nhi;
+                }
+                res[which] = 0.5 * (lo + hi);
+            }
+        }
+        if (t == 0) {
+            const double lmin = res[0] * 0.9, lmax = res[1] * 1.1;
+            sc->lz_lmin = lmin;
+            sc->lz_lmax = lmax;
+            if (isfinite(lmin) && isfinite(lmax) && lmin > 0 && lmax > lmin) {
+                sc->lmin = lmin;
+                sc->lmax = lmax;
+                sc->bounds_warning = 0;
+            } else {
+                sc->bounds_warning = 1;
+            }
+        }
+    }
+    if (t == 0) {
+        const double d_mid = (sc->lmax + sc->lmin) / 2.0, c_rad = (sc->lmax - sc->lmin) / 2.0;
+        double alpha = 1.0 / d_mid;
+        sc->cheb_alpha[0] = alpha;
+        sc->cheb_beta[0] = 0.0;
+        double alpha_prev = alpha;
+        for (int k = 1; k < degree; ++k) {
+            const double tt = c_rad * alpha_prev / 2.0;
+            const double beta = tt * tt;
+            alpha = 1.0 / (d_mid - beta / alpha_prev);
+            sc->cheb_alpha[k] = alpha;
+            sc->cheb_beta[k] = beta;
+            alpha_prev = alpha;
+        }
+        sc->cheb_degree = degree;
+    }
+}
+
+// ------------------------------------------------------------------ CG
+
+// Finalisers of the preconditioned inner product gamma = r'z.
+// mode 0: initial (Krylov cg! set-up), mode 1: per-iteration update.
+__device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
+    if (mode == 0) {
+        const double rn = sqrt(g);
+        sc->gamma = g;
+        sc->rNorm0 = rn;
+        sc->rNorm = rn;
+        sc->pNorm2 = g;
+        sc->beta = 0.0;
+        sc->iter = 0;
+        sc->zero_curv = 0;
+        sc->tired = 0;
+        sc->eps_stop = sc->atol + sc->rtol * rn;
+        sc->pAp = 0.0;
+        sc->solved = (g == 0.0) || (rn <= sc->eps_stop);
+        sc->done = sc->solved;
+    } else {
+        const double rn = sqrt(g);
+        sc->rNorm = rn;
+        const int solved = (rn <= sc->eps_stop) || (rn + 1.0 <= 1.0);
+        sc->solved = solved;
+        if (!solved) {
+            const double beta = g / sc->gamma;
+            sc->pNorm2 = g + beta * beta * sc->pNorm2;
+            sc->gamma = g;
+            sc->beta = beta;
+        }
+        sc->iter += 1;
+        sc->tired = sc->iter >= sc->itmax;
+        sc->done = solved || sc->tired;
+    }
+}
+
+// Set-up: r = b - A x0 (x0 = h, the warm start; x0 == nullptr means x0 = 0),
+// the accumulated correction x = 0, p_zero = 0; with Jacobi also z = r ./ d
+// and gamma = r'z.  (Krylov cg! warm start: r = b - A*dx, x = 0, x += dx at exit.)
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_init(int nx, int ny, int R, const T* __restrict__ aP,
+                                                 const T* __restrict__ aE, const T* __restrict__ aN,
+                                                 const T* __restrict__ b, const T* __restrict__ x0,
+                                                 T* __restrict__ r, T* __restrict__ z, T* __restrict__ xacc,
+                                                 T* __restrict__ p_zero, int jacobi, double* partials,
+                                                 Scalars* sc) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    double acc = 0.0;
+    if (x0) {
+        march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> T { return x0[k]; },
+                 [&](size_t c, T Au, T) {
+                     const T rv = b[c] - Au;
+                     r[c] = rv;
+                     xacc[c] = 0;
+                     p_zero[c] = 0;
+                     if (jacobi) {
+                         const T zv = rv / aP[c];
+                         z[c] = zv;
+                         acc += (double)rv * (double)zv;
+                     }
+                 });
+    } else {
+        walk(nx, ny, R, vbx, vby, [&](size_t c) {
+            const T rv = b[c];
+            r[c] = rv;
+            xacc[c] = 0;
+            p_zero[c] = 0;
+            if (jacobi) {
+                const T zv = rv / aP[c];
+                z[c] = zv;
+                acc += (double)rv * (double)zv;
+            }
+        });
+    }
+    if (jacobi) {
+        double v[1] = {acc};
+        grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, 0); });
+    }
+}
+
+// One Chebyshev sweep of ldiv!(y, P, r) (src/preconditioner.jl:239-267),
+// pass k = 1 .. degree-1 (the reference's final sweep only updates its
+// internal residual, which the returned y never reads, so it is skipped):
+//   k == 1: operand p0 = s0 = r ./ d (formed on the fly), then
+//           s1 = s0 - a0 (A p0)./d, p1 = s1 + b1 p0, y = a0 p0 + a1 p1
+//   k >= 2: s_k = s_{k-1} - a_{k-1} (A p_{k-1})./d, p_k = s_k + b_k p_{k-1},
+//           y += a_k p_k
+// The last pass (k == degree-1) stores only y and finishes gamma = r'y.
+template <class T>
+__global__ __launch_bounds__(256) void k_cheb_pass(int nx, int ny, int R, const T* __restrict__ aP,
+                                                   const T* __restrict__ aE, const T* __restrict__ aN,
+                                                   const T* __restrict__ r, const T* __restrict__ s_in,
+                                                   const T* __restrict__ pc_in, T* __restrict__ s_out,
+                                                   T* __restrict__ pc_out, T* __restrict__ y, int k, int degree,
+                                                   int mode, double* partials, Scalars* sc) {
+    if (sc->done) return;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const T a_prev = (T)sc->cheb_alpha[k - 1], a_k = (T)sc->cheb_alpha[k], b_k = (T)sc->cheb_beta[k];
+    const bool last = (k == degree - 1);
+    double acc = 0.0;
+    if (k == 1) {
+        march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return r[j] / aP[j]; },
+                 [&](size_t c, T Au, T p0) {
+                     const T s1 = p0 - a_prev * (Au / aP[c]);
+                     const T p1 = s1 + b_k * p0;
+                     const T yv = a_prev * p0 + a_k * p1;
+                     if (!last) {
+                         s_out[c] = s1;
+                         pc_out[c] = p1;
+                     } else {
+                         acc += (double)r[c] * (double)yv;
+                     }
+                     y[c] = yv;
+                 });
+    } else {
+        march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return pc_in[j]; },
+                 [&](size_t c, T Au, T pk) {
+                     const T sk = s_in[c] - a_prev * (Au / aP[c]);
+                     const T pn = sk + b_k * pk;
+                     const T yv = y[c] + a_k * pn;
+                     if (!last) {
+                         s_out[c] = sk;
+                         pc_out[c] = pn;
+                     } else {
+                         acc += (double)r[c] * (double)yv;
+                     }
+                     y[c] = yv;
+                 });
+    }
+    if (last) {
+        double v[1] = {acc};
+        grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, mode); });
+    }
+}
+
+// Chebyshev of degree 1: y = a0 * (r ./ d), gamma = r'y.
+template <class T>
+__global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const T* __restrict__ aP,
+                                                   const T* __restrict__ r, T* __restrict__ y, int mode,
+                                                   double* partials, Scalars* sc) {
+    if (sc->done) return;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const T a0 = (T)sc->cheb_alpha[0];
+    double acc = 0.0;
+    walk(nx, ny, R, vbx, vby, [&](size_t c) {
+        const T rv = r[c];
+        const T yv = a0 * (rv / aP[c]);
+        y[c] = yv;
+        acc += (double)rv * (double)yv;
+    });
+    double v[1] = {acc};
+    grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, mode); });
+}
+
+// p = z + beta*p_old (formed at the stencil points), Ap = A p, p'Ap; then
+// Krylov's zero-curvature test and alpha = gamma / p'Ap.
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const T* __restrict__ aP,
+                                                   const T* __restrict__ aE, const T* __restrict__ aN,
+                                                   const T* __restrict__ z, const T* __restrict__ p_old,
+                                                   T* __restrict__ p_new, T* __restrict__ Ap, double* partials,
+                                                   Scalars* sc) {
+    if (sc->done) return;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const T beta = (T)sc->beta;
+    double acc = 0.0;
+    march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return z[j] + beta * p_old[j]; },
+             [&](size_t c, T Au, T u) {
+                 p_new[c] = u;
+                 Ap[c] = Au;
+                 acc += (double)u * (double)Au;
+             });
+    double v[1] = {acc};
+    grid_reduce<1, false>(v, partials, sc, [&](double pAp, double) {
+        sc->pAp = pAp;
+        if (pAp <= EPS64 * sc->pNorm2 && fabs(pAp) <= EPS64 * sc->pNorm2) {
+            sc->zero_curv = 1;
+            sc->done = 1;
+        } else {
+            sc->alpha = sc->gamma / pAp;
+        }
+    });
+}
+
+// x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const T* __restrict__ aP,
+                                                   const T* __restrict__ p, const T* __restrict__ Ap,
+                                                   T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                   int jacobi, double* partials, Scalars* sc) {
+    if (sc->done) return;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const T alpha = (T)sc->alpha;
+    double acc = 0.0;
+    walk(nx, ny, R, vbx, vby, [&](size_t c) {
+        x[c] = x[c] + alpha * p[c];
+        const T rv = r[c] - alpha * Ap[c];
+        r[c] = rv;
+        if (jacobi) {
+            const T zv = rv / aP[c];
+            z[c] = zv;
+            acc += (double)rv * (double)zv;
+        }
+    });
+    if (jacobi) {
+        double v[1] = {acc};
+        grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, 1); });
+    }
+}
+
+// h_new = dx + h_prev (Krylov's warm-start epilogue, then s.h .= ws.x);
+// optional under-relaxation h = a h + (1-a) h_prev (src/elliptic_solver.jl:42-46).
+__global__ __launch_bounds__(256) void k_cg_finish(int n, const double* __restrict__ x,
+                                                   const double* __restrict__ h_prev, double* __restrict__ h_new,
+                                                   double relax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double h = x[i] + h_prev[i];
+    if (relax > 0.0) h = relax * h + (1.0 - relax) * h_prev[i];
+    h_new[i] = h;
+}
+
+__global__ __launch_bounds__(256) void k_zero(size_t n, double* p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline dim3 cells_grid(int nx, int ny) { return dim3((nx + 63) / 64, (ny + 3) / 4); }
+
+Geom make_geom(int nx, int ny, int R_override) {
+    Geom G;
+    G.nx = nx;
+    G.ny = ny;
+    int R;
+    if (R_override > 0) R = R_override;
+    else {
+        // aim for >= 2048 waves (8 per CU on 256 CUs), at most 32 rows per wave
+        const long waves_per_row_strip = (nx + WAVE - 1) / WAVE;
+        long r = ((long)ny * waves_per_row_strip) / 2048;
+        R = (int)(r < 1 ? 1 : (r > 32 ? 32 : r));
+    }
+    G.R = R;
+    G.block = dim3(WAVE, WAVES);
+    G.grid = dim3((nx + WAVE - 1) / WAVE, (ny + WAVES * R - 1) / (WAVES * R));
+    G.nblocks = G.grid.x * G.grid.y;
+    return G;
+}
+
+namespace launch {
+
+void ic_cells(const Phys& P, Fields F, hipStream_t st) {
+    hipLaunchKernelGGL(k_ic_cells, cells_grid(P.nx, P.ny), dim3(64, 4), 0, st, P, F);
+}
+void stagger_b(const Phys& P, Fields F, hipStream_t st) {
+    hipLaunchKernelGGL(k_stagger_b, cells_grid(P.nx + 1, P.ny + 1), dim3(64, 4), 0, st, P, F);
+}
+void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,
+             Scalars* sc, hipStream_t st) {
+    if (mode == 0 && check)
+        hipLaunchKernelGGL((k_derived<0, true>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
+    else if (mode == 0)
+        hipLaunchKernelGGL((k_derived<0, false>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
+    else
+        hipLaunchKernelGGL((k_derived<1, false>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
+}
+void gap(const Phys& P, Fields F, double dt, hipStream_t st) {
+    const int n = P.nx * P.ny;
+    hipLaunchKernelGGL(k_gap, dim3((n + 255) / 256), dim3(256), 0, st, P, F, dt);
+}
+void fill_ieb(int n, double* ieb, double u, const double* moulin, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_ieb, dim3((n + 255) / 256), dim3(256), 0, st, n, ieb, u, moulin);
+}
+void zero(double* p, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, p);
+}
+
+template <class T>
+void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
+              hipStream_t st) {
+    if (parabolic)
+        hipLaunchKernelGGL((k_assemble<true, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs);
+    else
+        hipLaunchKernelGGL((k_assemble<false, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs);
+}
+template <class T>
+void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st) {
+    hipLaunchKernelGGL(k_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, x, y);
+}
+template <class T>
+void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_lz_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, rhs, r, p0, partials, sc);
+}
+template <class T>
+void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* p_old, T* p_new,
+                    T* Ap, int k, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_lz_step_a<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, r, p_old, p_new, Ap, k,
+                       partials, sc);
+}
+template <class T>
+void lanczos_step_b(const Geom& G, const T* Ap, T* r, int k, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_lz_step_b<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, Ap, r, k, partials, sc);
+}
+void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_cheb_bounds, dim3(1), dim3(64), 0, st, nsteps, degree, use_estimate, sc);
+}
+template <class T>
+void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* b, const T* x0, T* r, T* z, T* xacc,
+             T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x0, r, z, xacc, p_zero,
+                       jacobi, partials, sc);
+}
+template <class T>
+void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* s_in, const T* pc_in,
+               T* s_out, T* pc_out, T* y, int k, int degree, int mode, double* partials, Scalars* sc,
+               hipStream_t st) {
+    hipLaunchKernelGGL(k_cheb_pass<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, r, s_in, pc_in, s_out,
+                       pc_out, y, k, degree, mode, partials, sc);
+}
+template <class T>
+void cheb_degree1(const Geom& G, const T* aP, const T* r, T* y, int mode, double* partials, Scalars* sc,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_cheb_deg1<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, r, y, mode, partials, sc);
+}
+template <class T>
+void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
+               double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old, p_new, Ap,
+                       partials, sc);
+}
+template <class T>
+void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
+               Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_update<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, p, Ap, x, r, z, jacobi,
+                       partials, sc);
+}
+void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, x, h_prev, h_new, relax);
+}
+
+#define SHK_INST(T)                                                                                               \
+    template void assemble<T>(const Geom&, const Phys&, Fields, int, double, T*, T*, T*, T*, hipStream_t);        \
+    template void matvec<T>(const Geom&, const T*, const T*, const T*, const T*, T*, hipStream_t);                \
+    template void lanczos_init<T>(const Geom&, const T*, T*, T*, double*, Scalars*, hipStream_t);                 \
+    template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, int,   \
+                                    double*, Scalars*, hipStream_t);                                              \
+    template void lanczos_step_b<T>(const Geom&, const T*, T*, int, double*, Scalars*, hipStream_t);              \
+    template void cg_init<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, T*, T*, int,  \
+                             double*, Scalars*, hipStream_t);                                                     \
+    template void cheb_pass<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, T*, T*,   \
+                               T*, int, int, int, double*, Scalars*, hipStream_t);                                \
+    template void cheb_degree1<T>(const Geom&, const T*, const T*, T*, int, double*, Scalars*, hipStream_t);      \
+    template void cg_matvec<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, double*,    \
+                               Scalars*, hipStream_t);                                                            \
+    template void cg_update<T>(const Geom&, const T*, const T*, const T*, T*, T*, T*, int, double*, Scalars*,     \
+                               hipStream_t);
+SHK_INST(double)
+SHK_INST(float)
+#undef SHK_INST
+
+}  // namespace launch
+}  // namespace shk

@@ -1,0 +1,122 @@
+"""ctypes binding of libshakti_hip.so (include/shakti_hip.h).
+
+The library is loaded from this package's directory.  There is no fallback:
+if the shared object is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libshakti_hip.so"
+
+SHK_OK = 0
+ERRORS = {-1: "SHK_ERR_ARG", -2: "SHK_ERR_HIP", -3: "SHK_ERR_NUMERIC", -4: "SHK_ERR_STATE", -5: "SHK_ERR_NODEVICE"}
+
+FIELD_NAMES = ["h", "pw", "po", "b", "beta", "abs_ub", "mdot", "shear", "potential", "sensible", "Re", "K", "G", "zb",
+               "zs", "H", "ieb", "lambda", "A_visc", "N", "mask",
+               "dhdx", "q_x", "Re_x", "b_x", "ub_x", "taub_x", "dpwdx", "valid_x",
+               "dhdy", "q_y", "Re_y", "b_y", "ub_y", "taub_y", "dpwdy", "valid_y"]
+FIELD_ID = {n: i for i, n in enumerate(FIELD_NAMES)}
+N_CENTER, N_XFACE = 21, 8
+
+# Every exported entry point of include/shakti_hip.h; a load fails loudly if one is missing.
+EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "shk_set_solver_cfg", "shk_set_params",
+           "shk_set_model", "shk_upload", "shk_download", "shk_set_initial_conditions", "shk_solve_elliptic",
+           "shk_solve_parabolic", "shk_update_derived", "shk_picard_iteration", "shk_picard_loop",
+           "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
+           "shk_apply_operator", "shk_download_system", "shk_time_kernel", "shk_profile_enable", "shk_profile_read",
+           "shk_profile_reset"]
+
+
+class ShkGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("lx", C.c_double), ("ly", C.c_double), ("dx", C.c_double),
+                ("dy", C.c_double), ("dx2", C.c_double), ("dy2", C.c_double)]
+
+
+class ShkParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ["rho_w", "rho_sw", "rho_i", "g", "nu", "n", "omega", "L", "br", "lr", "ct",
+                                          "cw", "p_atm", "b_min", "b_max", "e_v"]]
+
+
+class ShkModel(C.Structure):
+    _fields_ = [("kfs", C.c_int32), ("gap_scheme", C.c_int32), ("sliding_law", C.c_int32),
+                ("sliding_C", C.c_double)]
+
+
+class ShkSolverCfg(C.Structure):
+    _fields_ = [("precond", C.c_int32), ("cheb_degree", C.c_int32), ("cheb_nsteps", C.c_int32), ("form", C.c_int32),
+                ("itmax", C.c_int32), ("check_every", C.c_int32), ("atol", C.c_double), ("rtol", C.c_double),
+                ("inner_rtol", C.c_double)]
+
+
+class ShkSolveStats(C.Structure):
+    _fields_ = [("niter", C.c_int32), ("solved", C.c_int32), ("zero_curvature", C.c_int32),
+                ("bounds_warning", C.c_int32), ("refinements", C.c_int32), ("pad", C.c_int32),
+                ("rnorm0", C.c_double), ("rnorm", C.c_double), ("lambda_min", C.c_double),
+                ("lambda_max", C.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+
+
+class ShaktiHIPError(RuntimeError):
+    """A nonzero status from the C ABI (the reference raises ErrorException)."""
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libshakti_hip.so and declare its signatures (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ShaktiHIPError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(str(LIB_PATH))
+    for name in EXPORTS:
+        if not hasattr(L, name):
+            raise ShaktiHIPError(f"{LIB_PATH} does not export {name}")
+    vp, dp, i32, i64, sz, dbl = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int64, C.c_size_t, C.c_double
+    L.shk_abi_version.restype = i32
+    L.shk_last_error.argtypes = [vp]
+    L.shk_last_error.restype = C.c_char_p
+    L.shk_create.argtypes = [C.POINTER(ShkGrid), C.POINTER(ShkParams), C.POINTER(ShkModel),
+                             C.POINTER(ShkSolverCfg), C.POINTER(vp)]
+    L.shk_destroy.argtypes = [vp]
+    L.shk_set_solver_cfg.argtypes = [vp, C.POINTER(ShkSolverCfg)]
+    L.shk_set_params.argtypes = [vp, C.POINTER(ShkParams)]
+    L.shk_set_model.argtypes = [vp, C.POINTER(ShkModel)]
+    L.shk_upload.argtypes = [vp, i32, dp, sz]
+    L.shk_download.argtypes = [vp, i32, dp, sz]
+    L.shk_set_initial_conditions.argtypes = [vp]
+    L.shk_solve_elliptic.argtypes = [vp, C.POINTER(ShkSolveStats)]
+    L.shk_solve_parabolic.argtypes = [vp, dbl, C.POINTER(ShkSolveStats)]
+    L.shk_update_derived.argtypes = [vp]
+    L.shk_picard_iteration.argtypes = [vp, dbl, C.POINTER(ShkSolveStats)]
+    L.shk_picard_loop.argtypes = [vp, i32, dbl, i32, dbl, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+    L.shk_parabolic_step.argtypes = [vp, dbl, C.POINTER(i64)]
+    L.shk_step_b.argtypes = [vp, dbl]
+    L.shk_set_moulin_field.argtypes = [vp, dp, sz]
+    L.shk_update_ieb.argtypes = [vp, i32, dbl]
+    L.shk_step.argtypes = [vp, dbl, i32, dbl, i32, dbl, i32, dbl, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+    L.shk_assemble.argtypes = [vp, i32, dbl]
+    L.shk_apply_operator.argtypes = [vp, dp, dp]
+    L.shk_download_system.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.shk_time_kernel.argtypes = [vp, i32, i32, dp, dp]
+    L.shk_profile_enable.argtypes = [vp, i32]
+    L.shk_profile_read.argtypes = [vp, i32, C.POINTER(C.c_char_p), C.POINTER(i64), dp, dp, C.POINTER(i32)]
+    L.shk_profile_reset.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None, what: str = "") -> None:
+    if rc != SHK_OK:
+        msg = ""
+        if ctx is not None:
+            raw = load().shk_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise ShaktiHIPError(f"{what}: {ERRORS.get(rc, rc)} {msg}".strip())

@@ -1,0 +1,523 @@
+"""Host-side mirror of Shakti.jl's hydraulic-head interface over libshakti_hip.so.
+
+Names, argument meaning and error behaviour follow the reference (Julia's
+trailing ``!`` is dropped): ``State(g)`` allocates a device-resident State,
+``set_initial_conditions`` fills it, ``CGIterativeSolver(g,
+HIPMatrixFreeLinearSystem; chebyshev_degree=...)`` is the new linear-solver
+subtype that ``PicardSolver`` / ``Simulation`` / ``run`` drive unchanged.
+Every computation runs in the HIP library; this module only moves arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+import weakref
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from . import _lib as L
+
+GROUNDED, OCEAN, LAND, OTHER_BASIN = 0.0, 1.0, 2.0, 3.0  # src/mask.jl:21-41
+SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)            # Krylov's default atol/rtol
+
+
+# ------------------------------------------------------------------ data model
+
+class Grid:
+    """src/grid.jl:26-35: nx*ny cells over lx*ly, dx = lx/(nx-1)."""
+
+    def __init__(self, nx: int, ny: int, lx: float, ly: float):
+        self.nx, self.ny = int(nx), int(ny)
+        self.lx, self.ly = float(lx), float(ly)
+        self.dx = self.lx / (self.nx - 1)
+        self.dy = self.ly / (self.ny - 1)
+        self.x = np.arange(self.nx) * self.dx
+        self.y = np.arange(self.ny) * self.dy
+        self.dx2 = self.dx ** 2
+        self.dy2 = self.dy ** 2
+
+    def c(self) -> L.ShkGrid:
+        return L.ShkGrid(self.nx, self.ny, self.lx, self.ly, self.dx, self.dy, self.dx2, self.dy2)
+
+
+@dataclass
+class ModelParameters:
+    """src/model_parameters.jl:55-84 (keyword constructor defaults)."""
+    rho_w: float = 1000.0
+    rho_sw: float = 1027.0
+    rho_i: float = 910.0
+    g: float = 9.81
+    nu: float = 1.787e-6
+    n: float = 3.0
+    omega: float = 1e-4
+    L: float = 334e3
+    br: float = 0.05
+    lr: float = 2.0
+    ct: float = 7.5e-8
+    cw: float = 4.22e3
+    p_atm: float = 0.0
+    b_min: float = 0.0
+    b_max: float = float("inf")
+    e_v: float = 0.0
+
+    def c(self) -> L.ShkParams:
+        return L.ShkParams(*[float(getattr(self, f.name)) for f in fields(self)])
+
+
+class AbstractKFaceScheme:
+    code = -1
+
+
+class Arithmetic(AbstractKFaceScheme):
+    """src/k_face_scheme.jl:15."""
+    code = 0
+
+
+class Harmonic(AbstractKFaceScheme):
+    """src/k_face_scheme.jl:23."""
+    code = 1
+
+
+class AbstractSlidingLaw:
+    pass
+
+
+@dataclass
+class RegularizedCoulombSlidingLaw(AbstractSlidingLaw):
+    """src/sliding_law.jl:33-35."""
+    C: float
+
+
+class PrescribedSlidingLaw(AbstractSlidingLaw):
+    """src/sliding_law.jl:45: taub fixed at set-up."""
+
+
+class AbstractMeltInput:
+    def value(self, t: float):
+        return None
+
+
+class ConstantMeltInput(AbstractMeltInput):
+    """src/melt_input.jl:22,30: ieb never changes."""
+
+
+@dataclass
+class SeasonalMeltInput(AbstractMeltInput):
+    """src/melt_input.jl:49-94: uniform ieb following a cosine melt season."""
+    t_start: float = 0.4
+    t_end: float = 0.7
+    amplitude: float = 492.75
+    offset: float = 493.75
+    i_min: float = 1.0
+    seconds_per_year: float = 365 * 86400.0
+
+    def value(self, t: float) -> float:
+        period = self.t_end - self.t_start
+        omega = 2 * math.pi / period
+        yf = math.fmod(t / self.seconds_per_year, 1.0)
+        if yf < 0:
+            yf += 1.0
+        if self.t_start <= yf <= self.t_start + period:
+            i_ma = self.offset - self.amplitude * math.cos(omega * (yf - self.t_start))
+        else:
+            i_ma = self.i_min
+        return i_ma / self.seconds_per_year
+
+
+class SeasonalMoulinMeltInput(SeasonalMeltInput):
+    """A SeasonalMeltInput plus a static field of point moulins added on top
+    (ieb = uniform(t) + moulin).  No counterpart in the reference, whose
+    update_ieb! overwrites ieb uniformly (src/melt_input.jl:92)."""
+
+    def __init__(self, moulin: np.ndarray, **kw):
+        super().__init__(**kw)
+        self.moulin = np.asfortranarray(np.asarray(moulin, dtype=np.float64))
+
+
+# ------------------------------------------------------------------ State
+
+def _finalize_ctx(handle):
+    if handle.value:
+        L.load().shk_destroy(handle)
+        handle.value = None
+
+
+class State:
+    """A device-resident State (src/state.jl:13-137).  Field reads download a
+    Fortran-ordered numpy copy (``state.h[ix, iy]``); ``state.set(name, a)``
+    uploads.  The context created here also holds the solver workspace."""
+
+    def __init__(self, g: Grid, p: ModelParameters | None = None):
+        self.grid = g
+        lib = L.load()
+        self._ctx = C.c_void_p()
+        cfg = _solver_cfg(None)
+        m = L.ShkModel(0, 0, 0, 0.5)
+        L.check(lib.shk_create(C.byref(g.c()), C.byref((p or ModelParameters()).c()), C.byref(m), C.byref(cfg),
+                               C.byref(self._ctx)), None, "shk_create")
+        weakref.finalize(self, _finalize_ctx, self._ctx)
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def shape_of(self, name: str):
+        i = L.FIELD_ID[name]
+        nx, ny = self.grid.nx, self.grid.ny
+        if i < L.N_CENTER:
+            return (nx, ny)
+        if i < L.N_CENTER + L.N_XFACE:
+            return (nx + 1, ny)
+        return (nx, ny + 1)
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape_of(name), order="F")
+        L.check(L.load().shk_download(self._ctx, L.FIELD_ID[name], out.ctypes.data_as(C.POINTER(C.c_double)),
+                                      out.size), self._ctx, f"download {name}")
+        return out
+
+    def set(self, name: str, value) -> None:
+        shape = self.shape_of(name)
+        a = np.asfortranarray(np.broadcast_to(np.asarray(value, dtype=np.float64), shape))
+        L.check(L.load().shk_upload(self._ctx, L.FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_double)), a.size),
+                self._ctx, f"upload {name}")
+
+    def __getattr__(self, name):
+        if name in L.FIELD_ID:
+            return self.get(name)
+        raise AttributeError(name)
+
+    def to_dict(self) -> dict:
+        return {n: self.get(n) for n in L.FIELD_NAMES}
+
+    # internal: push the per-call choices before an entry point
+    def _bind(self, p: ModelParameters | None = None, kfs: AbstractKFaceScheme | None = None,
+              sl: AbstractSlidingLaw | None = None, gap_scheme: int | None = None):
+        lib = L.load()
+        if p is not None:
+            L.check(lib.shk_set_params(self._ctx, C.byref(p.c())), self._ctx, "shk_set_params")
+        if kfs is not None or sl is not None or gap_scheme is not None:
+            m = getattr(self, "_model", L.ShkModel(0, 0, 0, 0.5))
+            if kfs is not None:
+                m.kfs = kfs.code
+            if sl is not None:
+                if isinstance(sl, RegularizedCoulombSlidingLaw):
+                    m.sliding_law, m.sliding_C = 0, float(sl.C)
+                elif isinstance(sl, PrescribedSlidingLaw):
+                    m.sliding_law = 1
+                else:
+                    raise L.ShaktiHIPError(f"unsupported sliding law {type(sl).__name__}")
+            if gap_scheme is not None:
+                m.gap_scheme = gap_scheme
+            self._model = m
+            L.check(lib.shk_set_model(self._ctx, C.byref(m)), self._ctx, "shk_set_model")
+
+
+def set_initial_conditions(s: State, g: Grid, p: ModelParameters, sl: AbstractSlidingLaw, mask, A_visc, zb, zs, b, G,
+                           ub_x, ub_y, ieb, taub_x, taub_y) -> State:
+    """set_initial_conditions! (src/initial_conditions.jl:14-126), on the device."""
+    s._bind(p=p, sl=sl)
+    for name, val in [("mask", mask), ("A_visc", A_visc), ("zb", zb), ("zs", zs), ("b", b), ("G", G),
+                      ("ub_x", ub_x), ("ub_y", ub_y), ("ieb", ieb), ("taub_x", taub_x), ("taub_y", taub_y)]:
+        s.set(name, val)
+    L.check(L.load().shk_set_initial_conditions(s.ctx), s.ctx, "shk_set_initial_conditions")
+    return s
+
+
+# ------------------------------------------------------------- linear solver
+
+class AbstractLinearSystem:
+    pass
+
+
+class HIPMatrixFreeLinearSystem(AbstractLinearSystem):
+    """Device-resident matrix-free system in the symmetric face layout
+    (aP, aE, aN, rhs); the counterpart of MatrixFreeLinearSystem
+    (src/linear_solver.jl:91-121)."""
+
+
+class MatrixFreeLinearSystem(HIPMatrixFreeLinearSystem):
+    """Alias so reference scripts that name MatrixFreeLinearSystem run here."""
+
+
+class SparseAssembledLinearSystem(AbstractLinearSystem):
+    """CPU-only in the reference; refused here as it is refused on GPU backends."""
+
+
+@dataclass
+class _Stats:
+    niter: int = 0
+    solved: bool = False
+    zero_curvature: bool = False
+    rnorm0: float = 0.0
+    rnorm: float = 0.0
+    refinements: int = 0
+
+
+class _Workspace:
+    def __init__(self):
+        self.stats = _Stats()
+
+
+@dataclass
+class ChebyshevPreconditioner:
+    """The bounds the last solve used (src/preconditioner.jl:160-186)."""
+    degree: int
+    nsteps_estimate: int = 15
+    lambda_min: float = 1.0
+    lambda_max: float = 1.0
+
+
+class AbstractLinearSolver:
+    pass
+
+
+class AbstractIterativeSolver(AbstractLinearSolver):
+    pass
+
+
+FORMS = {"fp64": 0, "fp32_refine": 1}
+
+
+class CGIterativeSolver(AbstractIterativeSolver):
+    """CGIterativeSolver(g, HIPMatrixFreeLinearSystem; chebyshev_degree,
+    chebyshev_nsteps_estimate) (src/linear_solver.jl:832-843) on the GPU.
+    ``atol``/``rtol``/``itmax`` are Krylov's cg! stopping controls (defaults
+    sqrt(eps), sqrt(eps), 2n); ``check_every`` is how many CG iterations are
+    queued between host looks at the device-side convergence flag."""
+
+    def __init__(self, g: Grid, system=HIPMatrixFreeLinearSystem, chebyshev_degree: int | None = None,
+                 chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
+                 rtol: float = SQRT_EPS, itmax: int = 0, form: str = "fp64", check_every: int = 8,
+                 inner_rtol: float = 1e-3):
+        if system is SparseAssembledLinearSystem:
+            raise L.ShaktiHIPError("CGIterativeSolver(g, SparseAssembledLinearSystem) is CPU-only; use "
+                                   "HIPMatrixFreeLinearSystem on the GPU.")
+        if not (isinstance(system, type) and issubclass(system, HIPMatrixFreeLinearSystem)):
+            raise L.ShaktiHIPError(f"unsupported linear system {system}")
+        if amg:
+            raise L.ShaktiHIPError("AMGPreconditioner is CPU/SparseMatrixCSC-only; use chebyshev_degree here.")
+        if chebyshev_degree is not None and chebyshev_degree < 1:
+            raise L.ShaktiHIPError(f"ChebyshevPreconditioner: degree must be >= 1 (got {chebyshev_degree})")
+        if chebyshev_nsteps_estimate < 2:
+            raise L.ShaktiHIPError(
+                f"ChebyshevPreconditioner: nsteps_estimate must be >= 2 (got {chebyshev_nsteps_estimate})")
+        if form not in FORMS:
+            raise L.ShaktiHIPError(f"unknown solver form {form!r}")
+        self.grid = g
+        self.chebyshev_degree = chebyshev_degree
+        self.nsteps_estimate = chebyshev_nsteps_estimate
+        self.atol, self.rtol, self.itmax = float(atol), float(rtol), int(itmax)
+        self.form = form
+        self.check_every = int(check_every)
+        self.inner_rtol = float(inner_rtol)
+        self.ws = _Workspace()
+        self.precond = None if chebyshev_degree is None else ChebyshevPreconditioner(chebyshev_degree,
+                                                                                     chebyshev_nsteps_estimate)
+        self.last_stats = L.ShkSolveStats()
+
+    def cfg(self) -> L.ShkSolverCfg:
+        return _solver_cfg(self)
+
+    def _record(self, st: L.ShkSolveStats):
+        self.last_stats = st
+        self.ws.stats = _Stats(st.niter, bool(st.solved), bool(st.zero_curvature), st.rnorm0, st.rnorm,
+                               st.refinements)
+        if self.precond is not None:
+            self.precond.lambda_min, self.precond.lambda_max = st.lambda_min, st.lambda_max
+            if st.bounds_warning:
+                import warnings
+                warnings.warn("ChebyshevPreconditioner: eigenvalue bound estimate was invalid; reusing previous "
+                              "bounds for this solve")
+
+
+def _solver_cfg(ls: CGIterativeSolver | None) -> L.ShkSolverCfg:
+    if ls is None:
+        return L.ShkSolverCfg(0, 4, 15, 0, 0, 8, SQRT_EPS, SQRT_EPS, 1e-3)
+    cheb = ls.chebyshev_degree is not None
+    return L.ShkSolverCfg(1 if cheb else 0, ls.chebyshev_degree or 1, ls.nsteps_estimate, FORMS[ls.form], ls.itmax,
+                          ls.check_every, ls.atol, ls.rtol, ls.inner_rtol)
+
+
+def _bind_solver(s: State, ls: CGIterativeSolver):
+    L.check(L.load().shk_set_solver_cfg(s.ctx, C.byref(ls.cfg())), s.ctx, "shk_set_solver_cfg")
+
+
+def solve_elliptic_linear_system(ls: CGIterativeSolver, s: State, g: Grid, p: ModelParameters,
+                                 kfs: AbstractKFaceScheme) -> None:
+    """src/linear_solver.jl:915-925: assemble, precondition, PCG warm-started from h; s.h updated."""
+    s._bind(p=p, kfs=kfs)
+    _bind_solver(s, ls)
+    st = L.ShkSolveStats()
+    L.check(L.load().shk_solve_elliptic(s.ctx, C.byref(st)), s.ctx, "shk_solve_elliptic")
+    ls._record(st)
+
+
+def solve_parabolic_linear_system(ls: CGIterativeSolver, s: State, g: Grid, p: ModelParameters,
+                                  kfs: AbstractKFaceScheme, dt: float) -> None:
+    """src/linear_solver.jl:935-945."""
+    s._bind(p=p, kfs=kfs)
+    _bind_solver(s, ls)
+    st = L.ShkSolveStats()
+    L.check(L.load().shk_solve_parabolic(s.ctx, float(dt), C.byref(st)), s.ctx, "shk_solve_parabolic")
+    ls._record(st)
+
+
+def update_derived(s: State, p: ModelParameters, sl: AbstractSlidingLaw) -> None:
+    """The refresh after a solve (src/elliptic_solver.jl:193-206)."""
+    s._bind(p=p, sl=sl)
+    L.check(L.load().shk_update_derived(s.ctx), s.ctx, "shk_update_derived")
+
+
+# ------------------------------------------------------------------ Picard
+
+class PicardSolver:
+    """src/elliptic_solver.jl:59-105."""
+
+    def __init__(self, iters: int, tol: float, ls: AbstractLinearSolver, g: Grid, alpha: float | None = None,
+                 check_every: int = 1):
+        if not isinstance(ls, CGIterativeSolver):
+            raise L.ShaktiHIPError("PicardSolver here needs a CGIterativeSolver over HIPMatrixFreeLinearSystem")
+        self.iters, self.tol, self.ls = int(iters), float(tol), ls
+        self.alpha = alpha
+        self.check_every = int(check_every)
+        self.converged = False
+        self.last_iter = 0
+        self.cg_iters = 0
+
+
+def Picard_loop(ps: PicardSolver, s: State, g: Grid, p: ModelParameters, kfs: AbstractKFaceScheme,
+                sl: AbstractSlidingLaw) -> None:
+    """src/elliptic_solver.jl:138-176 (each iteration: :187-208)."""
+    s._bind(p=p, kfs=kfs, sl=sl)
+    _bind_solver(s, ps.ls)
+    conv, last, cgi = C.c_int32(), C.c_int32(), C.c_int64(0)
+    L.check(L.load().shk_picard_loop(s.ctx, ps.iters, ps.tol, ps.check_every,
+                                     -1.0 if ps.alpha is None else float(ps.alpha), C.byref(conv), C.byref(last),
+                                     C.byref(cgi)), s.ctx, "shk_picard_loop")
+    ps.converged, ps.last_iter, ps.cg_iters = bool(conv.value), last.value, cgi.value
+
+
+def elliptic_solver(ps: PicardSolver, s: State, g: Grid, p: ModelParameters, kfs: AbstractKFaceScheme,
+                    sl: AbstractSlidingLaw) -> None:
+    """src/elliptic_solver.jl:127-129."""
+    Picard_loop(ps, s, g, p, kfs, sl)
+
+
+def parabolic_solver(ls: CGIterativeSolver, s: State, g: Grid, p: ModelParameters, kfs: AbstractKFaceScheme,
+                     sl: AbstractSlidingLaw, dt: float) -> None:
+    """src/parabolic_solver.jl:16-38."""
+    s._bind(p=p, kfs=kfs, sl=sl)
+    _bind_solver(s, ls)
+    cgi = C.c_int64(0)
+    L.check(L.load().shk_parabolic_step(s.ctx, float(dt), C.byref(cgi)), s.ctx, "shk_parabolic_step")
+    ls.ws.stats.niter = cgi.value
+
+
+# -------------------------------------------------------------- Simulation
+
+class EllipticHeadScheme:
+    def __init__(self, ps: PicardSolver):
+        self.ps = ps
+
+
+class ParabolicHeadScheme:
+    def __init__(self, ls: AbstractLinearSolver):
+        self.ls = ls
+
+
+class Simulation:
+    """src/simulation.jl:105-184 (observers are not part of this library:
+    tracked_obs must be empty)."""
+
+    def __init__(self, grid: Grid, state: State, tsteps: int, dt: float, p: ModelParameters, gap_scheme_choice: str,
+                 tracked_obs, mi: AbstractMeltInput, sl: AbstractSlidingLaw, ps: PicardSolver | None = None,
+                 ls: AbstractLinearSolver | None = None, k_face_choice: str = "arithmetic", verbose: bool = False):
+        if tracked_obs:
+            raise L.ShaktiHIPError("observers are outside this library; pass tracked_obs=[]")
+        if p.e_v == 0:
+            if ps is None:
+                raise L.ShaktiHIPError("ps (a PicardSolver) must be provided when p.e_v == 0 (elliptic head scheme)")
+            self.hs = EllipticHeadScheme(ps)
+        else:
+            if ls is None:
+                raise L.ShaktiHIPError("ls (an AbstractLinearSolver) must be provided when p.e_v != 0 "
+                                       "(parabolic head scheme)")
+            self.hs = ParabolicHeadScheme(ls)
+        if gap_scheme_choice not in ("explicit", "implicit"):
+            raise L.ShaktiHIPError(f'Unknown gap_scheme_choice: "{gap_scheme_choice}" '
+                                   '(expected "explicit" or "implicit")')
+        if k_face_choice == "arithmetic":
+            self.kfs = Arithmetic()
+        elif k_face_choice == "harmonic":
+            self.kfs = Harmonic()
+        else:
+            raise L.ShaktiHIPError(f'Unknown k_face_choice: "{k_face_choice}" (expected "arithmetic" or "harmonic")')
+        self.grid, self.state, self.tsteps, self.dt, self.p = grid, state, int(tsteps), float(dt), p
+        self.gap_scheme = 0 if gap_scheme_choice == "implicit" else 1
+        self.mi, self.sl, self.verbose = mi, sl, verbose
+        self.total_time = 0.0
+        if isinstance(mi, SeasonalMoulinMeltInput):
+            a = mi.moulin
+            L.check(L.load().shk_set_moulin_field(state.ctx, a.ctypes.data_as(C.POINTER(C.c_double)), a.size),
+                    state.ctx, "shk_set_moulin_field")
+        self.step_times: list[float] = []
+        self.picard_log: list[tuple] = []
+
+
+def picard_status(hs):
+    """src/run.jl:73-74."""
+    if isinstance(hs, EllipticHeadScheme):
+        return hs.ps.converged, hs.ps.last_iter
+    return None, None
+
+
+def step(sim: Simulation) -> None:
+    """src/run.jl:82-88: update_ieb!, step_h!, step_b!."""
+    s = sim.state
+    s._bind(p=sim.p, kfs=sim.kfs, sl=sim.sl, gap_scheme=sim.gap_scheme)
+    mi = sim.mi
+    mode = 0
+    u = 0.0
+    if isinstance(mi, SeasonalMoulinMeltInput):
+        mode, u = 2, mi.value(sim.total_time)
+    elif isinstance(mi, SeasonalMeltInput):
+        mode, u = 1, mi.value(sim.total_time)
+    conv, last, cgi = C.c_int32(), C.c_int32(), C.c_int64(0)
+    if isinstance(sim.hs, EllipticHeadScheme):
+        ps = sim.hs.ps
+        _bind_solver(s, ps.ls)
+        L.check(L.load().shk_step(s.ctx, sim.dt, mode, u, ps.iters, ps.tol, ps.check_every,
+                                  -1.0 if ps.alpha is None else float(ps.alpha), C.byref(conv), C.byref(last),
+                                  C.byref(cgi)), s.ctx, "shk_step")
+        ps.converged, ps.last_iter, ps.cg_iters = bool(conv.value), last.value, cgi.value
+    else:
+        _bind_solver(s, sim.hs.ls)
+        L.check(L.load().shk_step(s.ctx, sim.dt, mode, u, 1, 0.0, 1, -1.0, C.byref(conv), C.byref(last),
+                                  C.byref(cgi)), s.ctx, "shk_step")
+        sim.hs.ls.ws.stats.niter = cgi.value
+
+
+def run(sim: Simulation) -> None:
+    """src/run.jl:21-63 without observers/checkpoints: steps 1..tsteps,
+    total_time += dt after each, the per-step report line of :51."""
+    sim.total_time = 0.0
+    for t in range(1, sim.tsteps + 1):
+        t0 = time.perf_counter()
+        step(sim)
+        dt_wall = time.perf_counter() - t0
+        sim.total_time += sim.dt
+        sim.step_times.append(dt_wall)
+        conv, last = picard_status(sim.hs)
+        sim.picard_log.append((conv, last))
+        if sim.verbose:
+            c = "missing" if conv is None else str(conv).lower()
+            li = "missing" if last is None else last
+            print(f"{t} / {sim.tsteps} completed in {round(dt_wall, 4)}s. Picard converged: {c} in {li} iterations",
+                  flush=True)
+
+
+def abi_version() -> int:
+    return L.load().shk_abi_version()

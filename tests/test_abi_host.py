@@ -1,0 +1,91 @@
+"""C-ABI library and host-logic checks that need no GPU."""
+import ctypes as C
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "shakti_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(shk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(hip):
+    from shakti_hip import _lib as L
+    lib = C.CDLL(str(L.LIB_PATH))
+    declared = _declared_symbols()
+    assert len(declared) >= 25
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(L.EXPORTS) == declared
+
+
+def test_abi_version_and_struct_sizes(hip):
+    from shakti_hip import _lib as L
+    assert L.load().shk_abi_version() == 1
+    assert C.sizeof(L.ShkGrid) == 8 + 6 * 8
+    assert C.sizeof(L.ShkParams) == 16 * 8
+    assert C.sizeof(L.ShkSolverCfg) == 6 * 4 + 3 * 8
+    assert C.sizeof(L.ShkSolveStats) == 6 * 4 + 4 * 8
+
+
+def test_null_and_bad_arguments_are_errors(hip):
+    from shakti_hip import _lib as L
+    lib = L.load()
+    assert lib.shk_destroy(None) == -1
+    out = C.c_void_p()
+    assert lib.shk_create(None, None, None, None, C.byref(out)) == -1
+    g = L.ShkGrid(1, 5, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+    p = L.ShkParams()
+    m = L.ShkModel()
+    cfg = L.ShkSolverCfg(0, 4, 15, 0, 0, 8, 0.0, 1e-8, 1e-3)
+    assert lib.shk_create(C.byref(g), C.byref(p), C.byref(m), C.byref(cfg), C.byref(out)) == -1
+
+
+def test_reference_constructor_errors(hip):
+    """ErrorException contract of the reference constructors
+    (src/linear_solver.jl:804-834, src/preconditioner.jl:181-182,
+    src/simulation.jl:113-145), raised before any device call."""
+    H = hip
+    g = H.Grid(6, 6, 1e3, 1e3)
+    with pytest.raises(H.ShaktiHIPError):
+        H.CGIterativeSolver(g, H.SparseAssembledLinearSystem)
+    with pytest.raises(H.ShaktiHIPError):
+        H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, amg=True)
+    with pytest.raises(H.ShaktiHIPError):
+        H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, chebyshev_degree=0)
+    with pytest.raises(H.ShaktiHIPError):
+        H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, chebyshev_nsteps_estimate=1)
+    ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, chebyshev_degree=4)
+    assert ls.precond.lambda_min == 1.0 and ls.precond.lambda_max == 1.0
+    ps = H.PicardSolver(500, 1e-6, ls, g)
+    assert ps.converged is False and ps.last_iter == 0
+
+
+def test_seasonal_melt_input_host_value(hip, oracle):
+    H = hip
+    mi = H.SeasonalMeltInput()
+    for t in [0.0, 0.45 * 365 * 86400, 0.69 * 365 * 86400, 1.5 * 365 * 86400]:
+        assert mi.value(t) == pytest.approx(oracle.seasonal_ieb(t), rel=1e-14)
+
+
+def test_synthetic_inputs_deterministic(hip):
+    from shakti_hip import synthetic as S
+    g1, r1 = S.glacier(128, 32, bed_amp=10.0, seed=11)
+    g2, r2 = S.glacier(128, 32, bed_amp=10.0, seed=11)
+    for k in r1:
+        assert np.array_equal(r1[k], r2[k])
+    assert abs(r1["zb"].std() - 10.0) < 1e-9 and abs(r1["zb"].mean()) < 1e-9
+    assert np.all(r1["mask"][0] == H_OCEAN) and np.all(r1["mask"][1:] == 0.0)
+    m = S.moulin_field(g1, r1["mask"], n=20, seed=3)
+    assert np.count_nonzero(m) == 20
+    q = m[m > 0] * g1.dx * g1.dy
+    assert np.all((q >= 1.0) & (q <= 10.0))
+
+
+H_OCEAN = 1.0

@@ -1,0 +1,189 @@
+"""HIP library vs the CPU oracle on identical inputs (runs on an MI355X).
+
+Tolerances: masks and masked face fields bit-exact; FP64 fields after the
+initial conditions and derived-field refresh within 1e-13 of the field's
+maximum (pow() may differ by an ulp); heads after a solve within 1e-9
+relative and gap heights within 1e-8 relative, per step (the stated bar)."""
+import numpy as np
+import pytest
+
+from helpers import FIELDS_EXACT, field_errors, hip_state, oracle_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+ALL = None
+
+
+def _fixture(O):
+    g, p, m, raw = O.test_fixture_6x6()
+    return (g.nx, g.ny, g.lx, g.ly), raw
+
+
+def test_initial_conditions_6x6(oracle, hip):
+    g_args, raw = _fixture(oracle)
+    _, _, _, os_ = oracle_state(oracle, g_args, raw, C=0.25)
+    _, _, hs = hip_state(hip, g_args, raw, C=0.25)
+    from shakti_hip._lib import FIELD_NAMES
+    for n in FIELDS_EXACT:
+        assert np.array_equal(hs.get(n), getattr(os_, n)), n
+    errs = field_errors(hs, os_, [n for n in FIELD_NAMES if n not in FIELDS_EXACT])
+    bad = {k: v for k, v in errs.items() if v > 1e-13}
+    assert not bad, bad
+
+
+def test_initial_conditions_synthetic(oracle, hip):
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(96, 40, bed_amp=10.0, seed=3)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    _, _, _, os_ = oracle_state(oracle, g_args, raw)
+    _, _, hs = hip_state(hip, g_args, raw)
+    from shakti_hip._lib import FIELD_NAMES
+    for n in FIELDS_EXACT:
+        assert np.array_equal(hs.get(n), getattr(os_, n)), n
+    errs = field_errors(hs, os_, [n for n in FIELD_NAMES if n not in FIELDS_EXACT])
+    bad = {k: v for k, v in errs.items() if v > 1e-13}
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("kfs", [0, 1])
+@pytest.mark.parametrize("parabolic", [False, True])
+def test_assembly_matches_oracle(oracle, hip, kfs, parabolic):
+    """Perturbed mid-solve state (h += 0.01*(1:n)), derived refresh, then the
+    matrix-free assembly: all five coefficient arrays and rhs."""
+    g_args, raw = _fixture(oracle)
+    p_kw = {"e_v": 1e-3} if parabolic else {}
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, p_kw=p_kw, kfs=kfs, C=0.25)
+    hg, hp, hs = hip_state(hip, g_args, raw, p_kw=p_kw, C=0.25)
+    dh = 0.01 * np.arange(1, g.nx * g.ny + 1, dtype=np.float64).reshape((g.nx, g.ny), order="F")
+    os_.h += dh
+    oracle.update_derived(os_, g, p, m)
+    hs.set("h", hs.get("h") + dh)
+    hip.update_derived(hs, hp, hip.RegularizedCoulombSlidingLaw(0.25))
+    from shakti_hip._lib import FIELD_NAMES
+    errs = field_errors(hs, os_, [n for n in FIELD_NAMES if n not in FIELDS_EXACT])
+    assert max(errs.values()) < 1e-13, errs
+    co = oracle.assemble_mfls(os_, g, p, kfs, parabolic, 60.0)
+    import ctypes as C
+    from shakti_hip import _lib as L
+    hs._bind(p=hp, kfs=[hip.Arithmetic(), hip.Harmonic()][kfs])
+    L.check(L.load().shk_assemble(hs.ctx, int(parabolic), 60.0), hs.ctx, "assemble")
+    arrs = [np.zeros((g.nx, g.ny), order="F") for _ in range(6)]
+    L.check(L.load().shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
+    for name, a, b in zip(["aP", "aE", "aW", "aN", "aS", "rhs"], arrs, co):
+        assert rel_err(a, b) < 1e-14, (name, rel_err(a, b))
+
+
+def test_operator_apply_matches_oracle(oracle, hip):
+    """y = A x for a random x on a synthetic 300x70 glacier (ragged against
+    the 64-wide strips), against the oracle's 5-array matvec."""
+    from shakti_hip import synthetic as S
+    import ctypes as C
+    from shakti_hip import _lib as L
+    gg, raw = S.glacier(300, 70, bed_amp=10.0, seed=5)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, kfs=1)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    co = oracle.assemble_mfls(os_, g, p, 1)
+    hs._bind(p=hp, kfs=hip.Harmonic())
+    L.check(L.load().shk_assemble(hs.ctx, 0, 0.0), hs.ctx)
+    rng = np.random.default_rng(0)
+    x = np.asfortranarray(rng.standard_normal((g.nx, g.ny)))
+    y = np.zeros_like(x)
+    L.check(L.load().shk_apply_operator(hs.ctx, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                        y.ctypes.data_as(C.POINTER(C.c_double))), hs.ctx)
+    y_ref = oracle.matvec(co, x)
+    assert rel_err(y, y_ref) < 1e-15
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "chebyshev"])
+def test_single_solve_vs_cholesky_6x6(oracle, hip, precond):
+    g_args, raw = _fixture(oracle)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, C=0.25)
+    hg, hp, hs = hip_state(hip, g_args, raw, C=0.25)
+    sv = oracle.Solver(g, "cholesky")
+    sv.solve_elliptic(os_, p, 0)
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem,
+                               chebyshev_degree=4 if precond == "chebyshev" else None, atol=0.0, rtol=1e-14)
+    hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Arithmetic())
+    assert ls.ws.stats.niter > 0
+    assert rel_err(hs.get("h"), os_.h) < 1e-12
+
+
+def test_chebyshev_bounds_and_iterations_match_oracle_cg(oracle, hip):
+    """Same Lanczos bounds (to rounding) and the same CG iteration count as the
+    oracle's restated Krylov CG with the Chebyshev preconditioner."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(128, 48, bed_amp=10.0, seed=2)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, kfs=1)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    co = oracle.assemble_mfls(os_, g, p, 1)
+    x_ref, st_ref, bounds = oracle.cg_solve(co, co[5], os_.h, precond="chebyshev", atol=0.0, rtol=1e-10)
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-10,
+                               check_every=1)
+    hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+    assert abs(ls.precond.lambda_min - bounds[0]) <= 1e-6 * abs(bounds[0])
+    assert abs(ls.precond.lambda_max - bounds[1]) <= 1e-6 * abs(bounds[1])
+    assert abs(ls.ws.stats.niter - st_ref.niter) <= 1, (ls.ws.stats.niter, st_ref.niter)
+    assert rel_err(hs.get("h"), x_ref) < 1e-9
+
+
+def test_picard_loop_6x6_matches_cholesky_oracle(oracle, hip):
+    g_args, raw = _fixture(oracle)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, C=0.25)
+    hg, hp, hs = hip_state(hip, g_args, raw, C=0.25)
+    sv = oracle.Solver(g, "cholesky")
+    conv, last, _ = oracle.picard_loop(sv, os_, g, p, m, 500, 1e-6)
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-13)
+    ps = hip.PicardSolver(500, 1e-6, ls, hg)
+    hip.elliptic_solver(ps, hs, hg, hp, hip.Arithmetic(), hip.RegularizedCoulombSlidingLaw(0.25))
+    assert ps.converged == conv and ps.last_iter == last
+    assert rel_err(hs.get("h"), os_.h) < 1e-9
+    for n in ["N", "mdot", "K", "q_x", "Re"]:
+        assert rel_err(hs.get(n), getattr(os_, n)) < 1e-8, n
+
+
+def test_config1_trajectory_48_steps(oracle, hip):
+    """64x32, Cholesky oracle vs HIP Chebyshev(4)-PCG, 48 hourly steps: per
+    step head <= 1e-9 and gap height <= 1e-8 relative, identical Picard
+    counts, masks bit-exact."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(64, 32, bed_amp=10.0, seed=1)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    sv = oracle.Solver(g, "cholesky")
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-13)
+    ps = hip.PicardSolver(500, 1e-6, ls, hg)
+    sim = hip.Simulation(hg, hs, 48, 3600.0, hp, "implicit", [], hip.ConstantMeltInput(),
+                         hip.RegularizedCoulombSlidingLaw(0.5), ps=ps)
+    worst_h = worst_b = 0.0
+    for t in range(48):
+        conv, last, _ = oracle.step(sv, os_, g, p, m, 3600.0)
+        hip.step(sim)
+        sim.total_time += sim.dt
+        assert ps.converged == conv and ps.last_iter == last, (t, ps.last_iter, last)
+        eh, eb = rel_err(hs.get("h"), os_.h), rel_err(hs.get("b"), os_.b)
+        worst_h, worst_b = max(worst_h, eh), max(worst_b, eb)
+        assert eh <= 1e-9 and eb <= 1e-8, (t, eh, eb)
+    for n in FIELDS_EXACT:
+        assert np.array_equal(hs.get(n), getattr(os_, n)), n
+    print(f"config-1 worst per-step errors: h {worst_h:.2e}  b {worst_b:.2e}")
+
+
+def test_parabolic_steps_match_oracle(oracle, hip):
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(80, 24, bed_amp=20.0, seed=4)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    p_kw = {"e_v": 1e-3}
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, p_kw=p_kw)
+    hg, hp, hs = hip_state(hip, g_args, raw, p_kw=p_kw)
+    sv = oracle.Solver(g, "cholesky")
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-13)
+    sim = hip.Simulation(hg, hs, 10, 600.0, hp, "implicit", [], hip.ConstantMeltInput(),
+                         hip.RegularizedCoulombSlidingLaw(0.5), ls=ls)
+    for t in range(10):
+        oracle.step(sv, os_, g, p, m, 600.0)
+        hip.step(sim)
+        assert rel_err(hs.get("h"), os_.h) <= 1e-9, t
+        assert rel_err(hs.get("b"), os_.b) <= 1e-8, t
