@@ -193,8 +193,11 @@ int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double
 int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch);
 
 /* Per-kernel timing collected with HIP events on the context's stream while
- * enabled: names[i] (static strings), launch counts and total milliseconds. */
-int shk_profile_enable(shk_ctx* ctx, int32_t on);
+ * enabled: names[i] (static strings), launch counts, total milliseconds and
+ * algorithmic bytes.  mask: 0 off, 1 every kernel, otherwise a bit set of
+ * kernel ids (bit i = entry i of shk_profile_read) so that only the kernels of
+ * interest carry event records. */
+int shk_profile_enable(shk_ctx* ctx, int32_t mask);
 int shk_profile_read(shk_ctx* ctx, int32_t max_entries, const char** names, int64_t* counts, double* total_ms,
                      double* bytes, int32_t* n_entries);
 int shk_profile_reset(shk_ctx* ctx);

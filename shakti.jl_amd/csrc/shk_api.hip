@@ -9,6 +9,7 @@
 // `check_every` iterations.
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
@@ -68,11 +69,21 @@ struct shk_ctx {
     int system_parabolic = 0;
     double system_dt = 0.0;
 
-    // profiling
+    // profiling (HIP events around the launches whose id bit is set in prof_mask)
     bool prof = false;
+    uint32_t prof_mask = 0xffffffffu;
     std::vector<ProfEntry> prof_tab;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
+
+    // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)
+    struct GraphEntry {
+        int key[4];
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    bool use_graphs = true;
 
     std::string err;
 };
@@ -113,7 +124,7 @@ struct Prof {
     double bytes;
     hipEvent_t a = nullptr;
     Prof(shk_ctx* c_, int id_, double bytes_) : c(c_), id(id_), bytes(bytes_) {
-        if (c->prof) {
+        if (c->prof && ((c->prof_mask >> id) & 1u)) {
             a = take_event(c);
             if (a) hipEventRecord(a, c->st);
         }
@@ -231,8 +242,10 @@ void cheb_apply(shk_ctx* c, const Work<T>& w, int degree, int mode) {
         T* s_out = w.s;
         T* pc_out = (k % 2) ? w.cA : w.cB;
         const bool last = k == degree - 1;
-        // bytes: k==1 reads r, aP, aE, aN; later passes read aP, aE, aN, pc, s, y; the last adds r
-        double bytes = k == 1 ? (last ? 6 : 7) * nb : (last ? 7 : 9) * nb;
+        // algorithmic bytes (each array once): pass 1 reads r, aP, aE, aN and writes s, pc, y
+        // (y only when last); later passes read aP, aE, aN, pc, s, y and write s, pc, y (the
+        // last reads r as well and writes y only)
+        double bytes = k == 1 ? (last ? 5 : 7) * nb : (last ? 8 : 9) * nb;
         Prof pr(c, P_CHEB, bytes);
         launch::cheb_pass<T>(G, w.aP, w.aE, w.aN, w.r, s_in, pc_in, s_out, pc_out, w.z, k, degree, mode,
                              c->partials, c->sc, c->st);
@@ -285,13 +298,15 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                            ctx->st);
     }
     if (!jacobi) cheb_apply<T>(ctx, w, degree, 0);
-    T* p_old = w.pA;
-    T* p_new = w.pB;
-    long queued = 0;
-    for (;;) {
-        for (int k = 0; k < check_every; ++k) {
+    // an even batch returns the p ping-pong buffers to their starting roles,
+    // so one captured batch can be replayed
+    const int batch = check_every + (check_every & 1);
+    auto enqueue_batch = [&]() {
+        T* p_old = w.pA;
+        T* p_new = w.pB;
+        for (int k = 0; k < batch; ++k) {
             {
-                Prof pr(ctx, P_MV, 6 * nb);
+                Prof pr(ctx, P_MV, 7 * nb);  // reads aP, aE, aN, z, p_old; writes p_new, Ap
                 launch::cg_matvec<T>(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap, ctx->partials, ctx->sc, ctx->st);
             }
             {
@@ -302,13 +317,34 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             T* t = p_old;
             p_old = p_new;
             p_new = t;
-            ++queued;
         }
+    };
+    hipGraphExec_t exec = nullptr;
+    if (ctx->use_graphs && !ctx->prof) {
+        const int key[4] = {(int)sizeof(T), jacobi, degree, batch};
+        for (auto& e : ctx->graphs)
+            if (!memcmp(e.key, key, sizeof(key))) exec = e.exec;
+        if (!exec) {
+            shk_ctx::GraphEntry e;
+            memcpy(e.key, key, sizeof(key));
+            HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+            enqueue_batch();
+            HIPCHK(hipStreamEndCapture(ctx->st, &e.graph));
+            HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+            ctx->graphs.push_back(e);
+            exec = e.exec;
+        }
+    }
+    long queued = 0;
+    for (;;) {
+        if (exec) HIPCHK(hipGraphLaunch(exec, ctx->st));
+        else enqueue_batch();
+        queued += batch;
         if (ctx->prof) prof_collect(ctx);
         rc = read_scalars(ctx);
         if (rc) return rc;
         if (ctx->sc_host->done) break;
-        if (queued > (long)itmax + check_every) break;  // cannot happen: tired sets done
+        if (queued > (long)itmax + batch) break;  // cannot happen: tired sets done
     }
     return SHK_OK;
 }
@@ -409,6 +445,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
         return SHK_ERR_NODEVICE;
     }
     HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    ctx->use_graphs = getenv("SHK_NO_GRAPHS") == nullptr;
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;
@@ -460,6 +497,10 @@ int shk_destroy(shk_ctx* ctx) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ctx->sc_host) hipHostFree(ctx->sc_host);
+    for (auto& e : ctx->graphs) {
+        hipGraphExecDestroy(e.exec);
+        hipGraphDestroy(e.graph);
+    }
     for (auto& pe : ctx->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
     if (ctx->st) hipStreamDestroy(ctx->st);
@@ -745,10 +786,11 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     return SHK_OK;
 }
 
-int shk_profile_enable(shk_ctx* ctx, int32_t on) {
+int shk_profile_enable(shk_ctx* ctx, int32_t mask) {
     if (!ctx) return SHK_ERR_ARG;
-    if (!on) prof_collect(ctx);
-    ctx->prof = on != 0;
+    if (!mask) prof_collect(ctx);
+    ctx->prof = mask != 0;
+    ctx->prof_mask = mask == 1 ? 0xffffffffu : (uint32_t)mask;
     return SHK_OK;
 }
 

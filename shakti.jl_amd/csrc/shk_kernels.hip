@@ -130,42 +130,19 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Grid-wide reduction with last-workgroup finalisation.  Every workgroup
-// calls it once with its per-thread contributions; the last one to arrive
-// runs fin(total0, total1) in thread (0,0).  partials must hold 2*nblocks.
-template <int NV, bool MAX, class Fin>
-__device__ __forceinline__ void grid_reduce(double (&v)[NV], double* partials, Scalars* sc, Fin fin) {
-    __shared__ bool am_last;
+// Per-workgroup partial of a grid-wide reduction: thread (0,0) stores the
+// workgroup's total(s) at partials[i*nblocks + block].  The finishing kernel
+// k_finish (one workgroup, launched right after on the same stream) sums the
+// partials in a fixed order; the kernel boundary orders the two, so no device
+// fence (an L2 write-back on this chip) is needed.
+template <int NV, bool MAX>
+__device__ __forceinline__ void block_partial(double (&v)[NV], double* partials) {
     const unsigned nb = gridDim.x * gridDim.y;
     const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
     block_reduce<NV, MAX>(v);
-    const int tid = threadIdx.x + threadIdx.y * WAVE;
-    if (tid == 0) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) partials[i * nb + bid] = v[i];
-        __threadfence();
-        unsigned t = atomicAdd(&sc->counter, 1u);
-        am_last = (t == nb - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double a = 0.0;
-        bool first = true;
-        for (unsigned k = tid; k < nb; k += WAVE * WAVES) {
-            const double b = partials[i * nb + k];
-            if (first) { a = b; first = false; }
-            else a = MAX ? ((isnan(b) || b > a) ? b : a) : a + b;
-        }
-        acc[i] = a;
-    }
-    block_reduce<NV, MAX>(acc);
-    if (tid == 0) {
-        fin(acc[0], NV > 1 ? acc[NV - 1] : 0.0);
-        sc->counter = 0;
     }
 }
 
@@ -428,12 +405,7 @@ __global__ __launch_bounds__(256) void k_derived(Phys P, Fields F, const double*
     }
     if (CHECK) {
         double v[2] = {any_nan ? NAN : dmax, hmax};
-        grid_reduce<2, true>(v, partials, sc, [&](double dm, double hm) {
-            sc->dmax = dm;
-            sc->hmax = hm;
-            sc->picard_nan = isnan(dm) || isnan(hm);
-            sc->picard_converged = (!isnan(dm) && !isnan(hm) && dm / (hm + EPS64) < sc->picard_tol) ? 1 : 0;
-        });
+        block_partial<2, true>(v, partials);
     }
 }
 
@@ -620,10 +592,7 @@ __global__ __launch_bounds__(256) void k_lz_init(int nx, int ny, int R, const T*
         acc += (double)v * (double)v;
     });
     double v[1] = {acc};
-    grid_reduce<1, false>(v, partials, sc, [&](double g, double) {
-        sc->lz_gamma = g;
-        sc->beta = 0.0;
-    });
+    block_partial<1, false>(v, partials);
 }
 
 // step k: p = r + beta_{k-1} p_old (beta = 0 at k = 0), Ap = (A p) ./ d,
@@ -646,7 +615,7 @@ __global__ __launch_bounds__(256) void k_lz_step_a(int nx, int ny, int R, const 
                  acc += (double)u * (double)a;
              });
     double v[1] = {acc};
-    grid_reduce<1, false>(v, partials, sc, [&](double pAp, double) { sc->lz_alpha[k] = sc->lz_gamma / pAp; });
+    block_partial<1, false>(v, partials);
 }
 
 // step k (k < nsteps-1): r -= alpha_k Ap; beta_k = r'r / gamma.
@@ -663,12 +632,7 @@ __global__ __launch_bounds__(256) void k_lz_step_b(int nx, int ny, int R, const 
         acc += (double)v * (double)v;
     });
     double v[1] = {acc};
-    grid_reduce<1, false>(v, partials, sc, [&](double gn, double) {
-        const double b = gn / sc->lz_gamma;
-        sc->lz_beta[k] = b;
-        sc->beta = b;
-        sc->lz_gamma = gn;
-    });
+    block_partial<1, false>(v, partials);
 }
 
 // Sturm count: eigenvalues of the tridiagonal (a, e2 = e^2) below x.
@@ -853,7 +817,7 @@ __global__ __launch_bounds__(256) void k_cg_init(int nx, int ny, int R, const T*
     }
     if (jacobi) {
         double v[1] = {acc};
-        grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, 0); });
+        block_partial<1, false>(v, partials);
     }
 }
 
@@ -909,7 +873,7 @@ __global__ __launch_bounds__(256) void k_cheb_pass(int nx, int ny, int R, const 
     }
     if (last) {
         double v[1] = {acc};
-        grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, mode); });
+        block_partial<1, false>(v, partials);
     }
 }
 
@@ -930,7 +894,7 @@ __global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const 
         acc += (double)rv * (double)yv;
     });
     double v[1] = {acc};
-    grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, mode); });
+    block_partial<1, false>(v, partials);
 }
 
 // p = z + beta*p_old (formed at the stencil points), Ap = A p, p'Ap; then
@@ -953,15 +917,7 @@ __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const 
                  acc += (double)u * (double)Au;
              });
     double v[1] = {acc};
-    grid_reduce<1, false>(v, partials, sc, [&](double pAp, double) {
-        sc->pAp = pAp;
-        if (pAp <= EPS64 * sc->pNorm2 && fabs(pAp) <= EPS64 * sc->pNorm2) {
-            sc->zero_curv = 1;
-            sc->done = 1;
-        } else {
-            sc->alpha = sc->gamma / pAp;
-        }
-    });
+    block_partial<1, false>(v, partials);
 }
 
 // x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
@@ -987,7 +943,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
     });
     if (jacobi) {
         double v[1] = {acc};
-        grid_reduce<1, false>(v, partials, sc, [&](double g, double) { fin_gamma(sc, g, 1); });
+        block_partial<1, false>(v, partials);
     }
 }
 
@@ -1006,6 +962,85 @@ __global__ __launch_bounds__(256) void k_cg_finish(int n, const double* __restri
 __global__ __launch_bounds__(256) void k_zero(size_t n, double* p) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------------------------- finishing kernel
+
+enum FinMode {
+    FIN_GAMMA_INIT = 0,  // gamma = r'z at CG set-up
+    FIN_GAMMA_ITER = 1,  // gamma_next = r'z after an update (stopping test, beta)
+    FIN_PAP = 2,         // p'Ap: zero-curvature test, alpha
+    FIN_LZ_GAMMA0 = 3,   // Lanczos: gamma_0 = r'r
+    FIN_LZ_ALPHA = 4,    // Lanczos: alpha_k = gamma / p'(D^-1 A)p
+    FIN_LZ_BETA = 5,     // Lanczos: beta_k = r'r / gamma
+    FIN_PICARD = 6,      // Picard: max|h - h_prev|, max|h| -> converged flag
+};
+
+// One workgroup: sums (or maximises) the nb partials of the preceding
+// reduction kernel in a fixed order and updates the solver scalars.
+// Modes that follow an iteration kernel do nothing once the solve is done
+// (that kernel returned early and wrote no partials).
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partials, int nb, int mode, int k,
+                                                Scalars* sc) {
+    if ((mode == FIN_GAMMA_ITER || mode == FIN_PAP) && sc->done) return;
+    const bool is_max = mode == FIN_PICARD;
+    const int nv = is_max ? 2 : 1;
+    const int tid = threadIdx.x + threadIdx.y * WAVE;
+    double acc[2] = {0.0, 0.0};
+    for (int i = 0; i < nv; ++i) {
+        double a = 0.0;
+        bool first = true;
+        for (int j = tid; j < nb; j += WAVE * WAVES) {
+            const double b = partials[(size_t)i * nb + j];
+            if (first) { a = b; first = false; }
+            else a = is_max ? ((isnan(b) || b > a) ? b : a) : a + b;
+        }
+        acc[i] = a;
+    }
+    if (is_max) block_reduce<2, true>(acc);
+    else {
+        double one[1] = {acc[0]};
+        block_reduce<1, false>(one);
+        acc[0] = one[0];
+    }
+    if (tid != 0) return;
+    const double v0 = acc[0], v1 = acc[1];
+    switch (mode) {
+        case FIN_GAMMA_INIT: fin_gamma(sc, v0, 0); break;
+        case FIN_GAMMA_ITER: fin_gamma(sc, v0, 1); break;
+        case FIN_PAP:
+            sc->pAp = v0;
+            if (v0 <= EPS64 * sc->pNorm2 && fabs(v0) <= EPS64 * sc->pNorm2) {
+                sc->zero_curv = 1;
+                sc->done = 1;
+            } else {
+                sc->alpha = sc->gamma / v0;
+            }
+            break;
+        case FIN_LZ_GAMMA0:
+            sc->lz_gamma = v0;
+            sc->beta = 0.0;
+            break;
+        case FIN_LZ_ALPHA: sc->lz_alpha[k] = sc->lz_gamma / v0; break;
+        case FIN_LZ_BETA: {
+            const double b = v0 / sc->lz_gamma;
+            sc->lz_beta[k] = b;
+            sc->beta = b;
+            sc->lz_gamma = v0;
+            break;
+        }
+        case FIN_PICARD:
+            sc->dmax = v0;
+            sc->hmax = v1;
+            sc->picard_nan = isnan(v0) || isnan(v1);
+            sc->picard_converged = (!isnan(v0) && !isnan(v1) && v0 / (v1 + EPS64) < sc->picard_tol) ? 1 : 0;
+            break;
+        default: break;
+    }
+}
+
+static inline void finish(const Geom& G, double* partials, int mode, int k, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, G.nblocks, mode, k, sc);
 }
 
 // ------------------------------------------------------------ launchers
@@ -1041,8 +1076,10 @@ void stagger_b(const Phys& P, Fields F, hipStream_t st) {
 }
 void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,
              Scalars* sc, hipStream_t st) {
-    if (mode == 0 && check)
+    if (mode == 0 && check) {
         hipLaunchKernelGGL((k_derived<0, true>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
+        finish(G, partials, FIN_PICARD, 0, sc, st);
+    }
     else if (mode == 0)
         hipLaunchKernelGGL((k_derived<0, false>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
     else
@@ -1074,16 +1111,19 @@ void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T*
 template <class T>
 void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_lz_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, rhs, r, p0, partials, sc);
+    finish(G, partials, FIN_LZ_GAMMA0, 0, sc, st);
 }
 template <class T>
 void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* p_old, T* p_new,
                     T* Ap, int k, double* partials, Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_lz_step_a<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, r, p_old, p_new, Ap, k,
                        partials, sc);
+    finish(G, partials, FIN_LZ_ALPHA, k, sc, st);
 }
 template <class T>
 void lanczos_step_b(const Geom& G, const T* Ap, T* r, int k, double* partials, Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_lz_step_b<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, Ap, r, k, partials, sc);
+    finish(G, partials, FIN_LZ_BETA, k, sc, st);
 }
 void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_cheb_bounds, dim3(1), dim3(64), 0, st, nsteps, degree, use_estimate, sc);
@@ -1093,6 +1133,7 @@ void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* b, c
              T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x0, r, z, xacc, p_zero,
                        jacobi, partials, sc);
+    if (jacobi) finish(G, partials, FIN_GAMMA_INIT, 0, sc, st);
 }
 template <class T>
 void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* s_in, const T* pc_in,
@@ -1100,23 +1141,27 @@ void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r,
                hipStream_t st) {
     hipLaunchKernelGGL(k_cheb_pass<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, r, s_in, pc_in, s_out,
                        pc_out, y, k, degree, mode, partials, sc);
+    if (k == degree - 1) finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
 void cheb_degree1(const Geom& G, const T* aP, const T* r, T* y, int mode, double* partials, Scalars* sc,
                   hipStream_t st) {
     hipLaunchKernelGGL(k_cheb_deg1<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, r, y, mode, partials, sc);
+    finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
 void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
                double* partials, Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old, p_new, Ap,
                        partials, sc);
+    finish(G, partials, FIN_PAP, 0, sc, st);
 }
 template <class T>
 void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_update<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, p, Ap, x, r, z, jacobi,
                        partials, sc);
+    if (jacobi) finish(G, partials, FIN_GAMMA_ITER, 0, sc, st);
 }
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, x, h_prev, h_new, relax);

@@ -192,6 +192,37 @@ class State:
     def to_dict(self) -> dict:
         return {n: self.get(n) for n in L.FIELD_NAMES}
 
+    # per-kernel HIP-event timing inside the library (see shk_profile_*)
+    PROFILE_IDS = ["assemble", "lanczos", "cheb_bounds", "cg_init", "cheb_pass", "cg_matvec", "cg_update",
+                   "cg_finish", "derived", "gap", "stagger_b", "ieb", "ic", "matvec", "ir"]
+
+    def profile_enable(self, kernels=None) -> None:
+        """kernels: None for all, [] to switch off, or a list of kernel names."""
+        if kernels is None:
+            mask = 1
+        else:
+            mask = 0
+            for k in kernels:
+                mask |= 1 << self.PROFILE_IDS.index(k)
+            if mask == 1:  # kernel 0 alone would read as "all"; widen to a bit that is never used
+                mask |= 1 << 30
+        L.check(L.load().shk_profile_enable(self._ctx, mask), self._ctx, "shk_profile_enable")
+
+    def profile_read(self) -> dict:
+        n = 32
+        names = (C.c_char_p * n)()
+        counts = (C.c_int64 * n)()
+        ms = (C.c_double * n)()
+        by = (C.c_double * n)()
+        k = C.c_int32()
+        L.check(L.load().shk_profile_read(self._ctx, n, names, counts, ms, by, C.byref(k)), self._ctx,
+                "shk_profile_read")
+        return {names[i].decode(): {"count": counts[i], "ms": ms[i], "bytes": by[i]} for i in range(k.value)
+                if counts[i] > 0}
+
+    def profile_reset(self) -> None:
+        L.check(L.load().shk_profile_reset(self._ctx), self._ctx, "shk_profile_reset")
+
     # internal: push the per-call choices before an entry point
     def _bind(self, p: ModelParameters | None = None, kfs: AbstractKFaceScheme | None = None,
               sl: AbstractSlidingLaw | None = None, gap_scheme: int | None = None):
