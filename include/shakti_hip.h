@@ -189,7 +189,8 @@ int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double
 /* Benchmark helpers: time `reps` back-to-back launches of one kernel on the
  * context's stream with HIP events; returns the mean duration in ms and the
  * algorithmic bytes one launch moves.  kernel: 0 FP64 operator apply,
- * 1 FP32 operator apply, 2 FP64 Chebyshev sweep, 3 derived-field refresh. */
+ * 1 FP32 operator apply, 2 plain FP64 copy of nx*ny values (the streaming
+ * reference point). */
 int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch);
 
 /* Per-kernel timing collected with HIP events on the context's stream while

@@ -55,11 +55,11 @@ struct shk_ctx {
     // linear system (FP64) and workspace
     double *aP = nullptr, *aE = nullptr, *aN = nullptr, *rhs = nullptr;
     double *x = nullptr, *r = nullptr, *z = nullptr, *pA = nullptr, *pB = nullptr, *Ap = nullptr;
-    double *s = nullptr, *cA = nullptr, *cB = nullptr;
+    double *s = nullptr, *cA = nullptr, *cB = nullptr, *r2 = nullptr;
     // FP32 copies for the mixed-precision form
     float *aP32 = nullptr, *aE32 = nullptr, *aN32 = nullptr;
     float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *pA32 = nullptr, *pB32 = nullptr,
-          *Ap32 = nullptr, *s32 = nullptr, *cA32 = nullptr, *cB32 = nullptr;
+          *Ap32 = nullptr, *s32 = nullptr, *cA32 = nullptr, *cB32 = nullptr, *r2_32 = nullptr;
     double* d64 = nullptr;  // FP64 correction / residual scratch
 
     double* partials = nullptr;
@@ -84,6 +84,7 @@ struct shk_ctx {
     };
     std::vector<GraphEntry> graphs;
     bool use_graphs = true;
+    bool use_fused = true;
 
     std::string err;
 };
@@ -92,8 +93,10 @@ namespace {
 
 const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_init",     "cheb_pass",
                             "cg_matvec",  "cg_update",    "cg_finish",   "derived",     "gap",
-                            "stagger_b",  "ieb",          "ic",          "matvec",      "ir"};
-enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR, P_NUM };
+                            "stagger_b",  "ieb",          "ic",          "matvec",      "ir",
+                            "cg_update_cheb"};
+enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR,
+              P_FUSED, P_NUM };
 
 int fail(shk_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -223,7 +226,7 @@ int read_scalars(shk_ctx* ctx) {
 template <class T>
 struct Work {
     const T *aP, *aE, *aN, *b;
-    T *x, *r, *z, *pA, *pB, *Ap, *s, *cA, *cB;
+    T *x, *r, *z, *pA, *pB, *Ap, *s, *cA, *cB, *r2;
 };
 
 // Chebyshev application z = P^-1 r (passes 1..degree-1), finishing gamma.
@@ -301,19 +304,33 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     // an even batch returns the p ping-pong buffers to their starting roles,
     // so one captured batch can be replayed
     const int batch = check_every + (check_every & 1);
+    const bool fused = !jacobi && degree >= 2 && degree <= 8 && ctx->use_fused;
     auto enqueue_batch = [&]() {
         T* p_old = w.pA;
         T* p_new = w.pB;
+        T* r_cur = w.r;
+        T* r_nxt = w.r2;
         for (int k = 0; k < batch; ++k) {
             {
                 Prof pr(ctx, P_MV, 7 * nb);  // reads aP, aE, aN, z, p_old; writes p_new, Ap
                 launch::cg_matvec<T>(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap, ctx->partials, ctx->sc, ctx->st);
             }
-            {
-                Prof pr(ctx, P_UPD, (jacobi ? 8 : 6) * nb);
-                launch::cg_update<T>(G, w.aP, p_new, w.Ap, w.x, w.r, w.z, jacobi, ctx->partials, ctx->sc, ctx->st);
+            if (fused) {
+                // reads aP, aE, aN, p, Ap, x, r_old; writes x, r_new, z
+                Prof pr(ctx, P_FUSED, 10 * nb);
+                launch::cg_update_cheb<T>(G, degree, w.aP, w.aE, w.aN, p_new, w.Ap, r_cur, r_nxt, w.x, w.z,
+                                          ctx->partials, ctx->sc, ctx->st);
+                T* t = r_cur;
+                r_cur = r_nxt;
+                r_nxt = t;
+            } else {
+                {
+                    Prof pr(ctx, P_UPD, (jacobi ? 8 : 6) * nb);
+                    launch::cg_update<T>(G, w.aP, p_new, w.Ap, w.x, w.r, w.z, jacobi, ctx->partials, ctx->sc,
+                                         ctx->st);
+                }
+                if (!jacobi) cheb_apply<T>(ctx, w, degree, 1);
             }
-            if (!jacobi) cheb_apply<T>(ctx, w, degree, 1);
             T* t = p_old;
             p_old = p_new;
             p_new = t;
@@ -321,7 +338,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     };
     hipGraphExec_t exec = nullptr;
     if (ctx->use_graphs && !ctx->prof) {
-        const int key[4] = {(int)sizeof(T), jacobi, degree, batch};
+        const int key[4] = {(int)sizeof(T), jacobi, degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
             if (!memcmp(e.key, key, sizeof(key))) exec = e.exec;
         if (!exec) {
@@ -362,7 +379,7 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     ctx->system_parabolic = parabolic;
     ctx->system_dt = dt;
     Work<double> w{ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
-                   ctx->s, ctx->cA, ctx->cB};
+                   ctx->s, ctx->cA, ctx->cB, ctx->r2};
     const int cheb = cfg.precond == SHK_PRECOND_CHEBYSHEV;
     if (cheb) cheb_refresh<double>(ctx, w, cfg.cheb_nsteps, cfg.cheb_degree);
     int rc = pcg<double>(ctx, w, ctx->F.f[SHK_H], cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol,
@@ -446,6 +463,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     }
     HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
     ctx->use_graphs = getenv("SHK_NO_GRAPHS") == nullptr;
+    ctx->use_fused = getenv("SHK_NO_FUSED") == nullptr;
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;
@@ -463,15 +481,17 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
         HIPCHK(hipStreamSynchronize(ctx->st));
     }
     double** d64[] = {&ctx->aP, &ctx->aE, &ctx->aN, &ctx->rhs, &ctx->x, &ctx->r, &ctx->z, &ctx->pA,
-                      &ctx->pB, &ctx->Ap, &ctx->s, &ctx->cA, &ctx->cB, &ctx->d64};
+                      &ctx->pB, &ctx->Ap, &ctx->s, &ctx->cA, &ctx->cB, &ctx->d64, &ctx->r2};
     for (auto pp : d64)
         if ((rc = dalloc(ctx, pp, ctx->n))) return rc;
     float** d32[] = {&ctx->aP32, &ctx->aE32, &ctx->aN32, &ctx->b32, &ctx->x32, &ctx->r32, &ctx->z32,
-                     &ctx->pA32, &ctx->pB32, &ctx->Ap32, &ctx->s32, &ctx->cA32, &ctx->cB32};
+                     &ctx->pA32, &ctx->pB32, &ctx->Ap32, &ctx->s32, &ctx->cA32, &ctx->cB32, &ctx->r2_32};
     for (auto pp : d32)
         if ((rc = dalloc(ctx, pp, ctx->n))) return rc;
     Geom cells = make_geom((int)nx, (int)ny, 1);
-    const size_t nparts = 2 * (size_t)std::max(ctx->G.nblocks, cells.nblocks) + 64;
+    size_t maxblocks = (size_t)std::max(ctx->G.nblocks, cells.nblocks);
+    for (int H = 1; H < 8; ++H) maxblocks = std::max(maxblocks, (size_t)fused_geom(ctx->G, H).nblocks);
+    const size_t nparts = 2 * maxblocks + 64;
     if ((rc = dalloc(ctx, &ctx->partials, nparts))) return rc;
     HIPCHK(hipMalloc((void**)&ctx->sc, sizeof(Scalars)));
     HIPCHK(hipHostMalloc((void**)&ctx->sc_host, sizeof(Scalars), hipHostMallocDefault));
@@ -490,7 +510,7 @@ int shk_destroy(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     if (ctx->st) hipStreamSynchronize(ctx->st);
     for (int f = 0; f < SHK_NUM_FIELDS; ++f) hipFree(ctx->F.f[f]);
-    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
+    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->r2, ctx->r2_32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
                     ctx->pA, ctx->pB, ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->d64, ctx->aP32, ctx->aE32,
                     ctx->aN32, ctx->b32, ctx->x32, ctx->r32, ctx->z32, ctx->pA32, ctx->pB32, ctx->Ap32,
                     ctx->s32, ctx->cA32, ctx->cB32, ctx->partials, ctx->sc};
@@ -762,11 +782,13 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
         switch (kernel) {
             case 0: launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, ctx->h_alt, ctx->Ap, ctx->st); break;
             case 1: launch::matvec<float>(G, ctx->aP32, ctx->aE32, ctx->aN32, ctx->x32, ctx->Ap32, ctx->st); break;
+            case 2: launch::copy(ctx->h_alt, ctx->d64, ctx->n, ctx->st); break;
             default: break;
         }
     };
     if (kernel == 0) bytes = 5 * n * 8;
     else if (kernel == 1) bytes = 5 * n * 4;
+    else if (kernel == 2) bytes = 2 * n * 8;
     else {
         hipEventDestroy(a);
         hipEventDestroy(b);

@@ -63,6 +63,7 @@ struct Geom {
 };
 
 Geom make_geom(int nx, int ny, int R_override = 0);
+Geom fused_geom(const Geom& G, int H);
 
 // Physical constants and scheme switches handed to every physics kernel.
 struct Phys {
@@ -85,6 +86,7 @@ void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int m
 void gap(const Phys& P, Fields F, double dt, hipStream_t st);
 void fill_ieb(int n, double* ieb, double uniform, const double* moulin, hipStream_t st);
 void zero(double* p, size_t n, hipStream_t st);
+void copy(const double* a, double* b, size_t n, hipStream_t st);
 
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
@@ -119,6 +121,11 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
 template <class T>
 void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st);
+// Fused CG update + Chebyshev application (2 <= degree <= 8); r is ping-ponged.
+// Returns -1 for an unsupported degree.
+template <class T>
+int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* p, const T* Ap,
+                   const T* r_old, T* r_new, T* x, T* z, double* partials, Scalars* sc, hipStream_t st);
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st);
 }  // namespace launch
 

@@ -28,6 +28,7 @@
 // the Julia source is written (only pow() may differ in the last ulp).
 
 #include <math.h>
+#include <stdlib.h>
 
 #include "shk_internal.hpp"
 
@@ -524,30 +525,47 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, i
 // (so u can be a stored vector or an expression of several), with the
 // epilogue epi(c, Au, u_c) run per cell.  Matches stencil_matvec_kernel!
 // (src/linear_solver.jl:616-633): aP*u, then minus the E, W, N, S terms in
-// that order, each only where the neighbour exists.
+// that order, each only where the neighbour exists.  The operand is formed
+// once per cell: rows iy-1/iy+1 come from rolling registers, the E/W
+// neighbours from wave shuffles (only the two strip-edge lanes evaluate the
+// operand of the neighbouring strip).  All 64 lanes stay active so the
+// shuffles are well defined; lanes past the east edge only skip their stores.
 template <class T, class Opd, class Epi>
 __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, const T* __restrict__ aP,
                                       const T* __restrict__ aE, const T* __restrict__ aN, Opd opd, Epi epi) {
-    const int ix = vbx * WAVE + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int ix = vbx * WAVE + lane;
     const int iy0 = (vby * WAVES + threadIdx.y) * R;
     const int iy1 = min(iy0 + R, ny);
-    if (ix >= nx || iy0 >= ny) return;
+    if (iy0 >= ny) return;  // uniform over the wave
+    const bool inx = ix < nx;
+    const int ixc = inx ? ix : nx - 1;  // clamped column for loads of idle lanes
     T um = 0, aNm = 0;
     if (iy0 > 0) {
-        um = opd(cidx(ix, iy0 - 1, nx));
-        aNm = aN[cidx(ix, iy0 - 1, nx)];
+        um = opd(cidx(ixc, iy0 - 1, nx));
+        aNm = aN[cidx(ixc, iy0 - 1, nx)];
     }
-    T uc = opd(cidx(ix, iy0, nx));
+    T uc = opd(cidx(ixc, iy0, nx));
+    T aEw = 0;
+#pragma unroll 2
     for (int iy = iy0; iy < iy1; ++iy) {
-        const size_t c = cidx(ix, iy, nx);
+        const size_t c = cidx(ixc, iy, nx);
         const T up = (iy + 1 < ny) ? opd(c + nx) : T(0);
         const T aPc = aP[c], aEc = aE[c], aNc = aN[c];
+        T ue = __shfl_down(uc, 1, WAVE);
+        T uw = __shfl_up(uc, 1, WAVE);
+        aEw = __shfl_up(aEc, 1, WAVE);
+        if (lane == WAVE - 1 && ix < nx - 1) ue = opd(c + 1);
+        if (lane == 0 && ix > 0) {
+            uw = opd(c - 1);
+            aEw = aE[c - 1];
+        }
         T y = aPc * uc;
-        if (ix < nx - 1) y -= aEc * opd(c + 1);
-        if (ix > 0) y -= aE[c - 1] * opd(c - 1);
+        if (ix < nx - 1) y -= aEc * ue;
+        if (ix > 0) y -= aEw * uw;
         if (iy < ny - 1) y -= aNc * up;
         if (iy > 0) y -= aNm * um;
-        epi(c, y, uc);
+        if (inx) epi(c, y, uc);
         um = uc;
         uc = up;
         aNm = aNc;
@@ -959,9 +977,167 @@ __global__ __launch_bounds__(256) void k_cg_finish(int n, const double* __restri
     h_new[i] = h;
 }
 
+// Plain streaming copy (reference point for bandwidth and counter scaling).
+__global__ __launch_bounds__(256) void k_copy(size_t n, const double* __restrict__ a, double* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 __global__ __launch_bounds__(256) void k_zero(size_t n, double* p) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------- fused CG update + Chebyshev (H = degree-1)
+
+// One CG iteration's second half in a single pass:
+//   x += alpha p;  r_new = r_old - alpha Ap;  z = P^-1 r_new (all degree-1
+//   Chebyshev sweeps of ldiv!, src/preconditioner.jl:239-267);  partial of r_new'z.
+// Each wave owns a 64-lane strip of columns that overlaps its neighbours by
+// H columns on each side; the H sweeps run as a pipeline of stages that
+// trail each other by one row (stage k works on row j-k while row j is
+// loaded), y-neighbours come from rolling registers and x-neighbours from wave
+// shuffles, so the intermediate sweep vectors never touch memory.  A stage
+// value is correct on lanes [k, 64-k) and the strip stores lanes [H, 64-H).
+// Each quantity is the same expression, in the same order, as in the
+// unfused passes (k_cg_update + k_cheb_pass), so the results are identical.
+// Reads r_old/Ap/aP/aE/aN with the halo; writes x, r_new, z on owned cells
+// only (r is ping-ponged), so no workgroup reads what another writes.
+// Wave-wide lane shifts through DPP (GFX9 wave_shr:1 / wave_shl:1): the value
+// of lane i-1 (from the west) or lane i+1 (from the east); lanes without a
+// source receive 0.
+__device__ __forceinline__ int dpp_west_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_east_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ double from_west(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_west_i((int)(b & 0xffffffffll)), hi = dpp_west_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_east(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_east_i((int)(b & 0xffffffffll)), hi = dpp_east_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float from_west(float v) { return __int_as_float(dpp_west_i(__float_as_int(v))); }
+__device__ __forceinline__ float from_east(float v) { return __int_as_float(dpp_east_i(__float_as_int(v))); }
+
+template <class T, int H>
+__global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, const T* __restrict__ aP,
+                                                        const T* __restrict__ aE, const T* __restrict__ aN,
+                                                        const T* __restrict__ p, const T* __restrict__ Ap,
+                                                        const T* __restrict__ r_old, T* __restrict__ r_new,
+                                                        T* __restrict__ x, T* __restrict__ z, double* partials,
+                                                        Scalars* sc) {
+    if (sc->done) return;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const int lane = threadIdx.x;
+    const int ix = vbx * (WAVE - 2 * H) - H + lane;
+    const bool inx = ix >= 0 && ix < nx;
+    const bool own_x = inx && lane >= H && lane < WAVE - H;
+    const bool has_e = ix < nx - 1, has_w = ix > 0;
+    const int ixc = ix < 0 ? 0 : (ix >= nx ? nx - 1 : ix);  // clamped column: every lane loads a valid address
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
+    const T alpha = (T)sc->alpha;
+    T ca[H + 1], cb[H + 1];
+#pragma unroll
+    for (int k = 0; k <= H; ++k) {
+        ca[k] = (T)sc->cheb_alpha[k];
+        cb[k] = (T)sc->cheb_beta[k];
+    }
+    T wP[H + 1], wE[H + 1], wW[H + 1], wN[H + 2], wR[H + 1];
+    T pw[H][3];
+    T sw[H][2], yw[H][2];
+#pragma unroll
+    for (int k = 0; k <= H; ++k) { wP[k] = 1; wE[k] = 0; wW[k] = 0; wR[k] = 0; }
+#pragma unroll
+    for (int k = 0; k < H + 2; ++k) wN[k] = 0;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        pw[k][0] = pw[k][1] = pw[k][2] = 0;
+        sw[k][0] = sw[k][1] = 0;
+        yw[k][0] = yw[k][1] = 0;
+    }
+    double acc = 0.0;
+    const int j0 = max(iy0 - H, 0);
+    const int j1 = (iy0 < ny) ? iy1 + H : j0;  // exclusive
+    const T* pP = aP + ixc;
+    const T* pE = aE + ixc;
+    const T* pN = aN + ixc;
+    const T* pR = r_old + ixc;
+    const T* pA = Ap + ixc;
+    auto row_ofs = [&](int j) -> size_t { return (size_t)(j < ny ? j : ny - 1) * (size_t)nx; };
+    T nP = 1, nE = 0, nN = 0, nR = 0, nA = 0;  // prefetched inputs of row j
+    if (j0 < j1) {
+        const size_t o0 = row_ofs(j0);
+        nP = pP[o0]; nE = pE[o0]; nN = pN[o0]; nR = pR[o0]; nA = pA[o0];
+    }
+    // unrolled so that the window shifts become register renames
+#pragma unroll 4
+    for (int j = j0; j < j1; ++j) {
+#pragma unroll
+        for (int k = H; k > 0; --k) { wP[k] = wP[k - 1]; wE[k] = wE[k - 1]; wW[k] = wW[k - 1]; wR[k] = wR[k - 1]; }
+#pragma unroll
+        for (int k = H + 1; k > 0; --k) wN[k] = wN[k - 1];
+        // ---- stage 0 at row j (inputs prefetched); issue the loads of row j+1 first
+        const bool live = inx && j < ny;
+        const T dP = live ? nP : T(1), dE = live ? nE : T(0), dN = live ? nN : T(0);
+        const T r0 = nR, a0 = nA;
+        {
+            const size_t o1 = row_ofs(j + 1);
+            nP = pP[o1]; nE = pE[o1]; nN = pN[o1]; nR = pR[o1]; nA = pA[o1];
+        }
+        const T rr = live ? r0 - alpha * a0 : T(0);
+        if (own_x && j >= iy0 && j < iy1) {
+            const size_t c = cidx(ix, j, nx);
+            x[c] = x[c] + alpha * p[c];
+            r_new[c] = rr;
+        }
+        const T pv = rr / dP;
+        const T dW = from_west(dE);
+        wP[0] = dP; wE[0] = dE; wW[0] = dW; wN[0] = dN; wR[0] = rr;
+        T p_in = pv, s_in = pv, y_in = ca[0] * pv;
+        // ---- stages 1..H: stage k at row o = j - k
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            pw[k - 1][2] = pw[k - 1][1];
+            pw[k - 1][1] = pw[k - 1][0];
+            pw[k - 1][0] = p_in;
+            sw[k - 1][1] = sw[k - 1][0];
+            sw[k - 1][0] = s_in;
+            yw[k - 1][1] = yw[k - 1][0];
+            yw[k - 1][0] = y_in;
+            const int o = j - k;
+            const T pc = pw[k - 1][1], pn = pw[k - 1][0], ps = pw[k - 1][2];
+            const T pe = from_east(pc);
+            const T pwst = from_west(pc);
+            // absent neighbours contribute no term (y - 0 == y exactly)
+            const T tE = has_e ? wE[k] * pe : T(0);
+            const T tW = has_w ? wW[k] * pwst : T(0);
+            const T tN = (o < ny - 1) ? wN[k] * pn : T(0);
+            const T tS = (o > 0) ? wN[k + 1] * ps : T(0);
+            T Au = wP[k] * pc;
+            Au -= tE;
+            Au -= tW;
+            Au -= tN;
+            Au -= tS;
+            const T sk = sw[k - 1][1] - ca[k - 1] * (Au / wP[k]);
+            const T pk = sk + cb[k] * pc;
+            const T yk = yw[k - 1][1] + ca[k] * pk;
+            p_in = pk;
+            s_in = sk;
+            y_in = yk;
+            if (k == H) {
+                if (own_x && o >= iy0 && o < iy1) {
+                    z[cidx(ix, o, nx)] = yk;
+                    acc += (double)wR[k] * (double)yk;
+                }
+            }
+        }
+    }
+    double v[1] = {acc};
+    block_partial<1, false>(v, partials);
 }
 
 // ------------------------------------------------------- finishing kernel
@@ -1054,16 +1230,36 @@ Geom make_geom(int nx, int ny, int R_override) {
     int R;
     if (R_override > 0) R = R_override;
     else {
-        // aim for >= 2048 waves (8 per CU on 256 CUs), at most 32 rows per wave
+        // aim for >= 8192 waves (32 per CU on 256 CUs, enough loads in
+        // flight to cover HBM latency), at most 32 rows per wave
         const long waves_per_row_strip = (nx + WAVE - 1) / WAVE;
-        long r = ((long)ny * waves_per_row_strip) / 2048;
+        long r = ((long)ny * waves_per_row_strip) / 8192;
         R = (int)(r < 1 ? 1 : (r > 32 ? 32 : r));
+        const char* env = getenv("SHK_MARCH_R");
+        if (env && atoi(env) > 0) R = atoi(env);
     }
     G.R = R;
     G.block = dim3(WAVE, WAVES);
     G.grid = dim3((nx + WAVE - 1) / WAVE, (ny + WAVES * R - 1) / (WAVES * R));
     G.nblocks = G.grid.x * G.grid.y;
     return G;
+}
+
+// Geometry of the fused update+Chebyshev kernel: strips of 64-2H owned
+// columns, R owned rows per wave.
+Geom fused_geom(const Geom& G, int H) {
+    Geom F = G;
+    const int stride = WAVE - 2 * H;
+    const int gx = (G.nx + stride - 1) / stride;
+    // each wave recomputes 2H rows of warm-up/drain around its R owned rows
+    int R = 2 * H + 2;
+    if (R > 32) R = 32;
+    const char* env = getenv("SHK_FUSED_R");
+    if (env && atoi(env) > 0) R = atoi(env);
+    F.R = R;
+    F.grid = dim3(gx, (G.ny + WAVES * R - 1) / (WAVES * R));
+    F.nblocks = F.grid.x * F.grid.y;
+    return F;
 }
 
 namespace launch {
@@ -1091,6 +1287,9 @@ void gap(const Phys& P, Fields F, double dt, hipStream_t st) {
 }
 void fill_ieb(int n, double* ieb, double u, const double* moulin, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_ieb, dim3((n + 255) / 256), dim3(256), 0, st, n, ieb, u, moulin);
+}
+void copy(const double* a, double* b, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b);
 }
 void zero(double* p, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, p);
@@ -1163,6 +1362,31 @@ void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, 
                        partials, sc);
     if (jacobi) finish(G, partials, FIN_GAMMA_ITER, 0, sc, st);
 }
+template <class T>
+int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* p, const T* Ap,
+                   const T* r_old, T* r_new, T* x, T* z, double* partials, Scalars* sc, hipStream_t st) {
+    const int H = degree - 1;
+    const Geom F = fused_geom(G, H);
+#define SHK_FUSED(HH)                                                                                             \
+    case HH:                                                                                                      \
+        hipLaunchKernelGGL((k_cg_update_cheb<T, HH>), F.grid, F.block, 0, st, G.nx, G.ny, F.R, aP, aE, aN, p, Ap, \
+                           r_old, r_new, x, z, partials, sc);                                                     \
+        break;
+    switch (H) {
+        SHK_FUSED(1)
+        SHK_FUSED(2)
+        SHK_FUSED(3)
+        SHK_FUSED(4)
+        SHK_FUSED(5)
+        SHK_FUSED(6)
+        SHK_FUSED(7)
+        default: return -1;
+    }
+#undef SHK_FUSED
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, F.nblocks, (int)FIN_GAMMA_ITER, 0, sc);
+    return 0;
+}
+
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, x, h_prev, h_new, relax);
 }
@@ -1182,7 +1406,9 @@ void cg_finish(int n, const double* x, const double* h_prev, double* h_new, doub
     template void cg_matvec<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, double*,    \
                                Scalars*, hipStream_t);                                                            \
     template void cg_update<T>(const Geom&, const T*, const T*, const T*, T*, T*, T*, int, double*, Scalars*,     \
-                               hipStream_t);
+                               hipStream_t);                                                                      \
+    template int cg_update_cheb<T>(const Geom&, int, const T*, const T*, const T*, const T*, const T*, const T*,  \
+                                   T*, T*, T*, double*, Scalars*, hipStream_t);
 SHK_INST(double)
 SHK_INST(float)
 #undef SHK_INST

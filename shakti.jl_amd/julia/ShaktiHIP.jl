@@ -1,0 +1,166 @@
+"""
+    ShaktiHIP
+
+Shakti.jl plug-in that solves the hydraulic-head system on an MI355X through
+`libshakti_hip.so` (C ABI in `include/shakti_hip.h`).  It adds new subtypes of
+Shakti's `AbstractLinearSystem` / `AbstractIterativeSolver` and the two solve
+methods the reference dispatches on (src/linear_solver.jl:915-945), so
+`PicardSolver`, `EllipticHeadScheme` / `ParabolicHeadScheme` and
+`Simulation` / `run!` drive it unchanged:
+
+    ls = ShaktiHIP.HIPPCGSolver(grid; chebyshev_degree = 4, rtol = 1e-11, atol = 0.0)
+    ps = PicardSolver(500, 1e-6, ls, grid)
+    sim = Simulation(grid, state, tsteps, dt, p, "implicit", String[], mi, sl; ps = ps)
+    run!(sim)
+
+Each solve uploads the assembly inputs of the host State (mask, zb, h, K,
+A_visc, N, b, mdot, beta, abs_ub, ieb), assembles and runs the PCG on the GPU,
+and downloads `h`, exactly the contract of `solve_elliptic_linear_system!`
+(overwrite `s.h`, everything else untouched).  `ls.stats.niter` mirrors
+Krylov's `ws.stats.niter`.
+"""
+module ShaktiHIP
+
+using Shakti
+import Shakti: solve_elliptic_linear_system!, solve_parabolic_linear_system!
+
+const libshk = joinpath(@__DIR__, "..", "shakti_hip", "libshakti_hip.so")
+
+# ---- C structs (layout of include/shakti_hip.h) -------------------------------
+
+struct ShkGrid
+    nx::Int32; ny::Int32; lx::Float64; ly::Float64; dx::Float64; dy::Float64; dx2::Float64; dy2::Float64
+end
+struct ShkParams
+    rho_w::Float64; rho_sw::Float64; rho_i::Float64; g::Float64; nu::Float64; n::Float64; omega::Float64
+    L::Float64; br::Float64; lr::Float64; ct::Float64; cw::Float64; p_atm::Float64; b_min::Float64
+    b_max::Float64; e_v::Float64
+end
+struct ShkModel
+    kfs::Int32; gap_scheme::Int32; sliding_law::Int32; sliding_C::Float64
+end
+struct ShkSolverCfg
+    precond::Int32; cheb_degree::Int32; cheb_nsteps::Int32; form::Int32; itmax::Int32; check_every::Int32
+    atol::Float64; rtol::Float64; inner_rtol::Float64
+end
+mutable struct ShkSolveStats
+    niter::Int32; solved::Int32; zero_curvature::Int32; bounds_warning::Int32; refinements::Int32; pad::Int32
+    rnorm0::Float64; rnorm::Float64; lambda_min::Float64; lambda_max::Float64
+    ShkSolveStats() = new(0, 0, 0, 0, 0, 0, 0.0, 0.0, 1.0, 1.0)
+end
+
+# field ids (enum shk_field)
+const SHK_H, SHK_B, SHK_MDOT, SHK_K, SHK_ZB, SHK_IEB, SHK_A_VISC, SHK_N, SHK_MASK = 0, 3, 6, 11, 13, 16, 18, 19, 20
+const SHK_BETA, SHK_ABS_UB = 4, 5
+
+function check(rc, ctx, what)
+    if rc != 0
+        msg = ctx == C_NULL ? "" : unsafe_string(ccall((:shk_last_error, libshk), Cstring, (Ptr{Cvoid},), ctx))
+        error("$what failed ($rc): $msg")   # the reference's ErrorException contract
+    end
+end
+
+# ---- the new linear-system / solver subtypes ----------------------------------
+
+"The device-resident matrix-free system (aP, aE, aN, rhs in the symmetric face layout)."
+struct HIPMatrixFreeLinearSystem <: Shakti.AbstractLinearSystem end
+
+mutable struct HIPStats
+    niter::Int
+end
+
+"""
+    HIPPCGSolver(g; chebyshev_degree = nothing, chebyshev_nsteps_estimate = 15,
+                 atol = sqrt(eps()), rtol = sqrt(eps()), itmax = 0, check_every = 8)
+
+CG on the GPU with Jacobi (`chebyshev_degree === nothing`) or Chebyshev
+preconditioning; same options and errors as `CGIterativeSolver(g,
+MatrixFreeLinearSystem; ...)` (src/linear_solver.jl:832-843).
+"""
+mutable struct HIPPCGSolver <: Shakti.AbstractIterativeSolver
+    ctx::Ptr{Cvoid}
+    grid::Shakti.Grid
+    cfg::ShkSolverCfg
+    stats::HIPStats
+end
+
+function HIPPCGSolver(g::Shakti.Grid; chebyshev_degree::Union{Nothing, Int} = nothing,
+                      chebyshev_nsteps_estimate::Int = 15, amg::Bool = false, atol = sqrt(eps(Float64)),
+                      rtol = sqrt(eps(Float64)), itmax::Int = 0, check_every::Int = 8)
+    amg && error("AMGPreconditioner is CPU/SparseMatrixCSC-only; use chebyshev_degree with HIPPCGSolver.")
+    chebyshev_degree !== nothing && chebyshev_degree < 1 &&
+        error("ChebyshevPreconditioner: degree must be >= 1 (got $chebyshev_degree)")
+    chebyshev_nsteps_estimate >= 2 ||
+        error("ChebyshevPreconditioner: nsteps_estimate must be >= 2 (got $chebyshev_nsteps_estimate)")
+    cfg = ShkSolverCfg(chebyshev_degree === nothing ? 0 : 1, something(chebyshev_degree, 1),
+                       chebyshev_nsteps_estimate, 0, itmax, check_every, atol, rtol, 1e-3)
+    sg = Ref(ShkGrid(g.nx, g.ny, g.lx, g.ly, g.dx, g.dy, g.dx2, g.dy2))
+    sp = Ref(params_c(Shakti.ModelParameters()))
+    sm = Ref(ShkModel(0, 0, 0, 0.5))
+    sc = Ref(cfg)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:shk_create, libshk), Cint,
+                (Ref{ShkGrid}, Ref{ShkParams}, Ref{ShkModel}, Ref{ShkSolverCfg}, Ref{Ptr{Cvoid}}),
+                sg, sp, sm, sc, out), C_NULL, "shk_create")
+    ls = HIPPCGSolver(out[], g, cfg, HIPStats(0))
+    finalizer(x -> x.ctx != C_NULL && ccall((:shk_destroy, libshk), Cint, (Ptr{Cvoid},), x.ctx), ls)
+    return ls
+end
+
+params_c(p) = ShkParams(p.rho_w, p.rho_sw, p.rho_i, p.g, p.nu, p.n, p.omega, p.L, p.br, p.lr, p.ct, p.cw,
+                        p.p_atm, p.b_min, p.b_max, p.e_v)
+kfs_code(::Shakti.Arithmetic) = Int32(0)
+kfs_code(::Shakti.Harmonic) = Int32(1)
+
+function upload(ls, field, a::AbstractArray{Float64})
+    h = Array(a)  # device arrays of other backends come back to the host first
+    GC.@preserve h check(ccall((:shk_upload, libshk), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Csize_t),
+                               ls.ctx, field, h, length(h)), ls.ctx, "shk_upload")
+end
+
+function download!(a::AbstractArray{Float64}, ls, field)
+    h = Array{Float64}(undef, size(a))
+    GC.@preserve h check(ccall((:shk_download, libshk), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Csize_t),
+                               ls.ctx, field, h, length(h)), ls.ctx, "shk_download")
+    copyto!(a, h)
+end
+
+function sync_inputs!(ls::HIPPCGSolver, s, p, kfs)
+    check(ccall((:shk_set_params, libshk), Cint, (Ptr{Cvoid}, Ref{ShkParams}), ls.ctx, Ref(params_c(p))),
+          ls.ctx, "shk_set_params")
+    check(ccall((:shk_set_model, libshk), Cint, (Ptr{Cvoid}, Ref{ShkModel}), ls.ctx,
+                Ref(ShkModel(kfs_code(kfs), 0, 0, 0.5))), ls.ctx, "shk_set_model")
+    check(ccall((:shk_set_solver_cfg, libshk), Cint, (Ptr{Cvoid}, Ref{ShkSolverCfg}), ls.ctx, Ref(ls.cfg)),
+          ls.ctx, "shk_set_solver_cfg")
+    for (f, a) in ((SHK_MASK, s.mask), (SHK_ZB, s.zb), (SHK_H, s.h), (SHK_K, s.K), (SHK_A_VISC, s.A_visc),
+                   (SHK_N, s.N), (SHK_B, s.b), (SHK_MDOT, s.mdot), (SHK_BETA, s.beta), (SHK_ABS_UB, s.abs_ub),
+                   (SHK_IEB, s.ieb))
+        upload(ls, f, a)
+    end
+end
+
+"solve_elliptic_linear_system! for the GPU solver (same contract as src/linear_solver.jl:915-925)."
+function solve_elliptic_linear_system!(ls::HIPPCGSolver, s::Shakti.State, g::Shakti.Grid,
+                                       p::Shakti.ModelParameters, kfs::Shakti.AbstractKFaceScheme)
+    sync_inputs!(ls, s, p, kfs)
+    st = ShkSolveStats()
+    check(ccall((:shk_solve_elliptic, libshk), Cint, (Ptr{Cvoid}, Ref{ShkSolveStats}), ls.ctx, st), ls.ctx,
+          "shk_solve_elliptic")
+    ls.stats.niter = st.niter
+    download!(s.h, ls, SHK_H)
+    return nothing
+end
+
+"solve_parabolic_linear_system! for the GPU solver (src/linear_solver.jl:935-945)."
+function solve_parabolic_linear_system!(ls::HIPPCGSolver, s::Shakti.State, g::Shakti.Grid,
+                                        p::Shakti.ModelParameters, kfs::Shakti.AbstractKFaceScheme, dt)
+    sync_inputs!(ls, s, p, kfs)
+    st = ShkSolveStats()
+    check(ccall((:shk_solve_parabolic, libshk), Cint, (Ptr{Cvoid}, Float64, Ref{ShkSolveStats}), ls.ctx,
+                Float64(dt), st), ls.ctx, "shk_solve_parabolic")
+    ls.stats.niter = st.niter
+    download!(s.h, ls, SHK_H)
+    return nothing
+end
+
+end # module
