@@ -66,6 +66,8 @@ typedef struct {
 
 #define SHK_PRECOND_JACOBI 0    /* CGIterativeSolver(g, MatrixFreeLinearSystem) */
 #define SHK_PRECOND_CHEBYSHEV 1 /* ... ; chebyshev_degree = k */
+#define SHK_PRECOND_MULTIGRID 2 /* geometric multigrid V-cycle (FP32; only with SHK_FORM_FP32_REFINE);
+                                   no reference counterpart (the reference's AMG is CPU-only) */
 
 #define SHK_FORM_FP64 0         /* plain FP64 PCG */
 #define SHK_FORM_FP32_REFINE 1  /* FP32 inner PCG inside FP64 iterative refinement */

@@ -55,11 +55,11 @@ struct shk_ctx {
     // linear system (FP64) and workspace
     double *aP = nullptr, *aE = nullptr, *aN = nullptr, *rhs = nullptr;
     double *x = nullptr, *r = nullptr, *z = nullptr, *pA = nullptr, *pB = nullptr, *Ap = nullptr;
-    double *s = nullptr, *cA = nullptr, *cB = nullptr, *r2 = nullptr;
+    double *s = nullptr, *cA = nullptr, *cB = nullptr, *r2 = nullptr, *cP = nullptr;
     // FP32 copies for the mixed-precision form
     float *aP32 = nullptr, *aE32 = nullptr, *aN32 = nullptr;
     float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *pA32 = nullptr, *pB32 = nullptr,
-          *Ap32 = nullptr, *s32 = nullptr, *cA32 = nullptr, *cB32 = nullptr, *r2_32 = nullptr;
+          *Ap32 = nullptr, *s32 = nullptr, *cA32 = nullptr, *cB32 = nullptr, *r2_32 = nullptr, *dg32 = nullptr;
     double* d64 = nullptr;  // FP64 correction / residual scratch
 
     double* partials = nullptr;
@@ -75,6 +75,17 @@ struct shk_ctx {
     std::vector<ProfEntry> prof_tab;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
+
+    // geometric multigrid levels (FP32), allocated on first use
+    struct MGLevel {
+        int nx, ny;
+        float *c, *e, *n, *D, *b, *xa, *xb;
+        bool owned;
+    };
+    std::vector<MGLevel> mg;
+    float* mg_inv = nullptr;
+    double* mg_work = nullptr;
+    float mg_omega = 0.8f, mg_scale = 1.5f;  // damped-Jacobi weight, coarse-correction scale (measured best)
 
     // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)
     struct GraphEntry {
@@ -94,9 +105,9 @@ namespace {
 const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_init",     "cheb_pass",
                             "cg_matvec",  "cg_update",    "cg_finish",   "derived",     "gap",
                             "stagger_b",  "ieb",          "ic",          "matvec",      "ir",
-                            "cg_update_cheb"};
+                            "cg_update_cheb", "mg_down",  "mg_up",       "mg_coarse",   "mg_setup"};
 enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR,
-              P_FUSED, P_NUM };
+              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NUM };
 
 int fail(shk_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -192,8 +203,11 @@ size_t field_count(const shk_ctx* c, int f) {
 }
 
 int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
-    if (cfg->precond != SHK_PRECOND_JACOBI && cfg->precond != SHK_PRECOND_CHEBYSHEV)
-        return fail(ctx, SHK_ERR_ARG, "precond must be SHK_PRECOND_JACOBI or SHK_PRECOND_CHEBYSHEV");
+    if (cfg->precond != SHK_PRECOND_JACOBI && cfg->precond != SHK_PRECOND_CHEBYSHEV &&
+        cfg->precond != SHK_PRECOND_MULTIGRID)
+        return fail(ctx, SHK_ERR_ARG, "unknown preconditioner");
+    if (cfg->precond == SHK_PRECOND_MULTIGRID && cfg->form != SHK_FORM_FP32_REFINE)
+        return fail(ctx, SHK_ERR_ARG, "the multigrid preconditioner runs in FP32: use SHK_FORM_FP32_REFINE");
     if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_degree < 1 || cfg->cheb_degree > MAX_CHEB))
         return fail(ctx, SHK_ERR_ARG, "ChebyshevPreconditioner: degree must be >= 1 (and <= 32)");
     if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_nsteps < 2 || cfg->cheb_nsteps > MAX_LANCZOS))
@@ -223,9 +237,12 @@ int read_scalars(shk_ctx* ctx) {
     return SHK_OK;
 }
 
+// Arrays of one linear solve: the stencil coefficients (aP is the diagonal
+// in the FP64 form and the flux-form reaction c in the FP32 form), the Jacobi
+// diagonal dg, the right-hand side and the CG / Chebyshev workspace.
 template <class T>
 struct Work {
-    const T *aP, *aE, *aN, *b;
+    const T *aP, *aE, *aN, *dg, *b;
     T *x, *r, *z, *pA, *pB, *Ap, *s, *cA, *cB, *r2;
 };
 
@@ -236,7 +253,7 @@ void cheb_apply(shk_ctx* c, const Work<T>& w, int degree, int mode) {
     const double nb = (double)c->n * sizeof(T);
     if (degree == 1) {
         Prof pr(c, P_CHEB, 3 * nb);
-        launch::cheb_degree1<T>(G, w.aP, w.r, w.z, mode, c->partials, c->sc, c->st);
+        launch::cheb_degree1<T>(G, w.dg, w.r, w.z, mode, c->partials, c->sc, c->st);
         return;
     }
     const T* s_in = nullptr;
@@ -250,7 +267,7 @@ void cheb_apply(shk_ctx* c, const Work<T>& w, int degree, int mode) {
         // last reads r as well and writes y only)
         double bytes = k == 1 ? (last ? 5 : 7) * nb : (last ? 8 : 9) * nb;
         Prof pr(c, P_CHEB, bytes);
-        launch::cheb_pass<T>(G, w.aP, w.aE, w.aN, w.r, s_in, pc_in, s_out, pc_out, w.z, k, degree, mode,
+        launch::cheb_pass<T>(G, w.aP, w.aE, w.aN, w.dg, w.r, s_in, pc_in, s_out, pc_out, w.z, k, degree, mode,
                              c->partials, c->sc, c->st);
         s_in = s_out;
         pc_in = pc_out;
@@ -271,7 +288,8 @@ void cheb_refresh(shk_ctx* c, const Work<T>& w, int nsteps, int degree) {
     for (int k = 0; k < nsteps; ++k) {
         {
             Prof pr(c, P_LZ, 7 * nb);
-            launch::lanczos_step_a<T>(G, w.aP, w.aE, w.aN, w.r, p_old, p_new, w.Ap, k, c->partials, c->sc, c->st);
+            launch::lanczos_step_a<T>(G, w.aP, w.aE, w.aN, w.dg, w.r, p_old, p_new, w.Ap, k, c->partials, c->sc,
+                                      c->st);
         }
         if (k < nsteps - 1) {
             Prof pr(c, P_LZ, 3 * nb);
@@ -283,6 +301,77 @@ void cheb_refresh(shk_ctx* c, const Work<T>& w, int nsteps, int degree) {
     }
     Prof pr(c, P_CB, 0);
     launch::cheb_bounds(nsteps, degree, 1, c->sc, c->st);
+}
+
+// Multigrid hierarchy for the FP32 flux-form system in w (level 0 shares its
+// arrays); coarsens 2x2 until at most 64 cells remain (at least once), then
+// inverts the coarsest operator densely.
+int mg_setup(shk_ctx* ctx, const Work<float>& w) {
+    const int nx0 = ctx->g.nx, ny0 = ctx->g.ny;
+    if (ctx->mg.empty()) {
+        shk_ctx::MGLevel L0{nx0, ny0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, false};
+        HIPCHK(hipMalloc((void**)&L0.xa, (size_t)nx0 * ny0 * sizeof(float)));
+        ctx->mg.push_back(L0);
+        int nx = nx0, ny = ny0;
+        while (((long)nx * ny > 64 || ctx->mg.size() == 1) && (long)nx * ny > 1) {
+            nx = (nx + 1) / 2;
+            ny = (ny + 1) / 2;
+            shk_ctx::MGLevel L{nx, ny, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, true};
+            const size_t n = (size_t)nx * ny * sizeof(float);
+            float** ps[] = {&L.c, &L.e, &L.n, &L.D, &L.b, &L.xa, &L.xb};
+            for (auto pp : ps) HIPCHK(hipMalloc((void**)pp, n));
+            ctx->mg.push_back(L);
+        }
+        const size_t nc = (size_t)ctx->mg.back().nx * ctx->mg.back().ny;
+        HIPCHK(hipMalloc((void**)&ctx->mg_inv, nc * nc * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&ctx->mg_work, 2 * nc * nc * sizeof(double)));
+        if (const char* e = getenv("SHK_MG_OMEGA")) ctx->mg_omega = (float)atof(e);
+        if (const char* e = getenv("SHK_MG_SCALE")) ctx->mg_scale = (float)atof(e);
+    }
+    auto& L0 = ctx->mg[0];
+    L0.c = (float*)w.aP;
+    L0.e = (float*)w.aE;
+    L0.n = (float*)w.aN;
+    L0.D = (float*)w.dg;
+    for (size_t l = 1; l < ctx->mg.size(); ++l) {
+        auto& f = ctx->mg[l - 1];
+        auto& c = ctx->mg[l];
+        launch::mg_coarsen(f.nx, f.ny, c.nx, c.ny, f.c, f.e, f.n, c.c, c.e, c.n, ctx->st);
+        launch::mg_diag(c.nx, c.ny, c.c, c.e, c.n, c.D, ctx->st);
+    }
+    auto& Lc = ctx->mg.back();
+    Prof pr(ctx, P_MGS, 0);
+    launch::mg_dense_inverse(Lc.nx, Lc.ny, Lc.c, Lc.e, Lc.n, ctx->mg_work, ctx->mg_inv, ctx->st);
+    return SHK_OK;
+}
+
+// z = V(r): one V-cycle (one damped-Jacobi sweep down and up on every level,
+// dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1 iteration).
+void mg_apply(shk_ctx* ctx, const float* r, float* z, int mode) {
+    auto& mg = ctx->mg;
+    const size_t L = mg.size() - 1;
+    for (size_t l = 0; l < L; ++l) {
+        const auto& f = mg[l];
+        const float* b = l == 0 ? r : f.b;
+        // reads b, c, aE, aN, D; writes x and the coarse rhs
+        Prof pr(ctx, P_MGD, 4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny));
+        launch::mg_down(f.nx, f.ny, mg[l + 1].nx, mg[l + 1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, b, f.xa,
+                        mg[l + 1].b, ctx->st);
+    }
+    {
+        const double nc = (double)mg[L].nx * mg[L].ny;
+        Prof pr(ctx, P_MGC, 4.0 * (nc * nc + 2 * nc));
+        launch::mg_coarse(mg[L].nx * mg[L].ny, ctx->mg_inv, mg[L].b, mg[L].xb, ctx->st);
+    }
+    for (size_t l = L; l-- > 0;) {
+        const auto& f = mg[l];
+        const float* b = l == 0 ? r : f.b;
+        float* out = l == 0 ? z : f.xb;
+        // reads b, c, aE, aN, D, x and the coarse correction; writes the result
+        Prof pr(ctx, P_MGU, 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny));
+        launch::mg_up(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, b, f.xa,
+                      mg[l + 1].xb, out, l == 0 ? mode : -1, ctx->partials, ctx->sc, ctx->st);
+    }
 }
 
 // Preconditioned CG on A x = b from x0 (nullptr: zero), Krylov cg! semantics.
@@ -297,14 +386,19 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     if (rc) return rc;
     {
         Prof pr(ctx, P_INIT, (x0 ? 8 : 5) * nb);
-        launch::cg_init<T>(G, w.aP, w.aE, w.aN, w.b, x0, w.r, w.z, w.x, w.pA, jacobi, ctx->partials, ctx->sc,
+        launch::cg_init<T>(G, w.aP, w.aE, w.aN, w.dg, w.b, x0, w.r, w.z, w.x, w.pA, jacobi, ctx->partials, ctx->sc,
                            ctx->st);
     }
-    if (!jacobi) cheb_apply<T>(ctx, w, degree, 0);
+    const bool multigrid = precond == SHK_PRECOND_MULTIGRID;
+    if (multigrid) {
+        if constexpr (sizeof(T) == 4) mg_apply(ctx, w.r, w.z, 0);
+    } else if (!jacobi) {
+        cheb_apply<T>(ctx, w, degree, 0);
+    }
     // an even batch returns the p ping-pong buffers to their starting roles,
     // so one captured batch can be replayed
     const int batch = check_every + (check_every & 1);
-    const bool fused = !jacobi && degree >= 2 && degree <= 8 && ctx->use_fused;
+    const bool fused = precond == SHK_PRECOND_CHEBYSHEV && degree >= 2 && degree <= 8 && ctx->use_fused;
     auto enqueue_batch = [&]() {
         T* p_old = w.pA;
         T* p_new = w.pB;
@@ -318,7 +412,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             if (fused) {
                 // reads aP, aE, aN, p, Ap, x, r_old; writes x, r_new, z
                 Prof pr(ctx, P_FUSED, 10 * nb);
-                launch::cg_update_cheb<T>(G, degree, w.aP, w.aE, w.aN, p_new, w.Ap, r_cur, r_nxt, w.x, w.z,
+                launch::cg_update_cheb<T>(G, degree, w.aP, w.aE, w.aN, w.dg, p_new, w.Ap, r_cur, r_nxt, w.x, w.z,
                                           ctx->partials, ctx->sc, ctx->st);
                 T* t = r_cur;
                 r_cur = r_nxt;
@@ -326,10 +420,14 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             } else {
                 {
                     Prof pr(ctx, P_UPD, (jacobi ? 8 : 6) * nb);
-                    launch::cg_update<T>(G, w.aP, p_new, w.Ap, w.x, w.r, w.z, jacobi, ctx->partials, ctx->sc,
+                    launch::cg_update<T>(G, w.dg, p_new, w.Ap, w.x, w.r, w.z, jacobi, ctx->partials, ctx->sc,
                                          ctx->st);
                 }
-                if (!jacobi) cheb_apply<T>(ctx, w, degree, 1);
+                if (multigrid) {
+                    if constexpr (sizeof(T) == 4) mg_apply(ctx, w.r, w.z, 1);
+                } else if (!jacobi) {
+                    cheb_apply<T>(ctx, w, degree, 1);
+                }
             }
             T* t = p_old;
             p_old = p_new;
@@ -338,7 +436,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     };
     hipGraphExec_t exec = nullptr;
     if (ctx->use_graphs && !ctx->prof) {
-        const int key[4] = {(int)sizeof(T), jacobi, degree * (fused ? 100 : 1), batch};
+        const int key[4] = {(int)sizeof(T), precond, degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
             if (!memcmp(e.key, key, sizeof(key))) exec = e.exec;
         if (!exec) {
@@ -366,7 +464,101 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     return SHK_OK;
 }
 
+// FP32-inner / FP64-refinement solve (SHK_FORM_FP32_REFINE).  The FP64
+// residual r = b - A h is measured in the Jacobi-weighted norm sqrt(r'D^-1 r);
+// each refinement solves A d = r in FP32 (FP32 operator and Chebyshev sweeps,
+// FP64 dot products) to inner_rtol and adds d to the FP64 iterate, until the
+// FP64 norm meets atol + rtol * initial (the same relative test as the FP64 form).
+int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_solve_stats* stats) {
+    const Geom& G = ctx->G;
+    const size_t n = ctx->n;
+    const double nb = (double)n * 8;
+    const shk_solver_cfg& cfg = ctx->cfg;
+    const int itmax = cfg.itmax > 0 ? cfg.itmax : (int)(2 * n < 2147483647 ? 2 * n : 2147483647);
+    {
+        Prof pr(ctx, P_ASM, 17 * nb);
+        launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st,
+                                 ctx->cP);
+    }
+    ctx->have_system = true;
+    {
+        // FP32 flux-form system: c (in aP32), aE, aN, and the Jacobi diagonal
+        Prof pr(ctx, P_IR, 5 * 12.0 * n);
+        launch::to_f32(n, ctx->cP, ctx->aP32, ctx->st);
+        launch::to_f32(n, ctx->aE, ctx->aE32, ctx->st);
+        launch::to_f32(n, ctx->aN, ctx->aN32, ctx->st);
+        launch::to_f32(n, ctx->aP, ctx->dg32, ctx->st);
+        launch::to_f32(n, ctx->rhs, ctx->b32, ctx->st);
+    }
+    // b32 holds the FP32 rhs for the bound estimate, then each FP32 residual:
+    // it is the right-hand side of every inner solve
+    float* r_in = ctx->b32;
+    Work<float> wi{ctx->aP32, ctx->aE32, ctx->aN32, ctx->dg32, r_in, ctx->x32, ctx->r32, ctx->z32, ctx->pA32,
+                   ctx->pB32, ctx->Ap32, ctx->s32, ctx->cA32, ctx->cB32, ctx->r2_32};
+    const int cheb = cfg.precond == SHK_PRECOND_CHEBYSHEV;
+    // bounds of the FP32 operator, estimated from the FP32 rhs (scale-free)
+    if (cheb) cheb_refresh<float>(ctx, wi, cfg.cheb_nsteps, cfg.cheb_degree);
+    if (cfg.precond == SHK_PRECOND_MULTIGRID) {
+        int rc0 = mg_setup(ctx, wi);
+        if (rc0) return rc0;
+    }
+    double* x = ctx->h_alt;  // the FP64 iterate, started from h
+    double* h_prev = ctx->F.f[SHK_H];
+    HIPCHK(hipMemcpyAsync(x, h_prev, n * 8, hipMemcpyDeviceToDevice, ctx->st));
+    int rc = reset_scalars(ctx, 0.0, 0.0, itmax);
+    if (rc) return rc;
+    ctx->sc_host->ir_atol = cfg.atol;
+    ctx->sc_host->ir_rtol = cfg.rtol;
+    HIPCHK(hipMemcpyAsync(&ctx->sc->ir_atol, &ctx->sc_host->ir_atol, 2 * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->st));
+    int refinements = 0;
+    long inner = 0;
+    double rnorm0 = 0.0, rnorm = 0.0;
+    int solved = 0, zc = 0;
+    for (int k = 0; k < 64; ++k) {
+        {
+            Prof pr(ctx, P_IR, 8.5 * nb);
+            launch::ir_residual(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, x, r_in, k == 0, ctx->partials, ctx->sc,
+                                ctx->st);
+        }
+        rc = read_scalars(ctx);
+        if (rc) return rc;
+        if (k == 0) rnorm0 = ctx->sc_host->ir_rnorm0;
+        rnorm = ctx->sc_host->ir_rnorm;
+        if (ctx->sc_host->ir_done) { solved = 1; break; }
+        rc = pcg<float>(ctx, wi, nullptr, cfg.precond, cfg.cheb_degree, cfg.check_every, 0.0, cfg.inner_rtol,
+                        itmax);
+        if (rc) return rc;
+        inner += ctx->sc_host->iter;
+        zc |= ctx->sc_host->zero_curv;
+        {
+            Prof pr(ctx, P_IR, 20.0 * n);
+            launch::ir_correct(n, x, ctx->x32, ctx->st);
+        }
+        ++refinements;
+        if (ctx->sc_host->iter == 0) break;  // the inner solve can make no progress
+    }
+    if (relax_alpha > 0.0) launch::relax(n, x, h_prev, relax_alpha, ctx->st);
+    ctx->F.f[SHK_H] = x;
+    ctx->h_alt = h_prev;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->niter = (int32_t)inner;
+        stats->solved = solved;
+        stats->zero_curvature = zc;
+        stats->bounds_warning = cheb ? ctx->sc_host->bounds_warning : 0;
+        stats->refinements = refinements;
+        stats->rnorm0 = rnorm0;
+        stats->rnorm = rnorm;
+        stats->lambda_min = ctx->sc_host->lmin;
+        stats->lambda_max = ctx->sc_host->lmax;
+    }
+    if (ctx->prof) prof_collect(ctx);
+    return SHK_OK;
+}
+
 int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats* stats) {
+    if (ctx->cfg.form == SHK_FORM_FP32_REFINE) return solve_ir(ctx, parabolic, dt, relax, stats);
     const Geom& G = ctx->G;
     const double nb = (double)ctx->n * 8;
     const shk_solver_cfg& cfg = ctx->cfg;
@@ -378,8 +570,8 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     ctx->have_system = true;
     ctx->system_parabolic = parabolic;
     ctx->system_dt = dt;
-    Work<double> w{ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
-                   ctx->s, ctx->cA, ctx->cB, ctx->r2};
+    Work<double> w{ctx->aP, ctx->aE, ctx->aN, ctx->aP, ctx->rhs, ctx->x, ctx->r, ctx->z, ctx->pA, ctx->pB,
+                   ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->r2};
     const int cheb = cfg.precond == SHK_PRECOND_CHEBYSHEV;
     if (cheb) cheb_refresh<double>(ctx, w, cfg.cheb_nsteps, cfg.cheb_degree);
     int rc = pcg<double>(ctx, w, ctx->F.f[SHK_H], cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol,
@@ -481,11 +673,12 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
         HIPCHK(hipStreamSynchronize(ctx->st));
     }
     double** d64[] = {&ctx->aP, &ctx->aE, &ctx->aN, &ctx->rhs, &ctx->x, &ctx->r, &ctx->z, &ctx->pA,
-                      &ctx->pB, &ctx->Ap, &ctx->s, &ctx->cA, &ctx->cB, &ctx->d64, &ctx->r2};
+                      &ctx->pB, &ctx->Ap, &ctx->s, &ctx->cA, &ctx->cB, &ctx->d64, &ctx->r2, &ctx->cP};
     for (auto pp : d64)
         if ((rc = dalloc(ctx, pp, ctx->n))) return rc;
     float** d32[] = {&ctx->aP32, &ctx->aE32, &ctx->aN32, &ctx->b32, &ctx->x32, &ctx->r32, &ctx->z32,
-                     &ctx->pA32, &ctx->pB32, &ctx->Ap32, &ctx->s32, &ctx->cA32, &ctx->cB32, &ctx->r2_32};
+                     &ctx->pA32, &ctx->pB32, &ctx->Ap32, &ctx->s32, &ctx->cA32, &ctx->cB32, &ctx->r2_32,
+                     &ctx->dg32};
     for (auto pp : d32)
         if ((rc = dalloc(ctx, pp, ctx->n))) return rc;
     Geom cells = make_geom((int)nx, (int)ny, 1);
@@ -510,7 +703,7 @@ int shk_destroy(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     if (ctx->st) hipStreamSynchronize(ctx->st);
     for (int f = 0; f < SHK_NUM_FIELDS; ++f) hipFree(ctx->F.f[f]);
-    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->r2, ctx->r2_32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
+    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->r2, ctx->r2_32, ctx->cP, ctx->dg32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
                     ctx->pA, ctx->pB, ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->d64, ctx->aP32, ctx->aE32,
                     ctx->aN32, ctx->b32, ctx->x32, ctx->r32, ctx->z32, ctx->pA32, ctx->pB32, ctx->Ap32,
                     ctx->s32, ctx->cA32, ctx->cB32, ctx->partials, ctx->sc};
@@ -521,6 +714,15 @@ int shk_destroy(shk_ctx* ctx) {
         hipGraphExecDestroy(e.exec);
         hipGraphDestroy(e.graph);
     }
+    for (auto& L : ctx->mg) {
+        hipFree(L.xa);
+        if (L.owned) {
+            float* ps[] = {L.c, L.e, L.n, L.D, L.b, L.xb};
+            for (float* p : ps) hipFree(p);
+        }
+    }
+    if (ctx->mg_inv) hipFree(ctx->mg_inv);
+    if (ctx->mg_work) hipFree(ctx->mg_work);
     for (auto& pe : ctx->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
     if (ctx->st) hipStreamDestroy(ctx->st);

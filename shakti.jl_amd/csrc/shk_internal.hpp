@@ -47,8 +47,9 @@ struct Scalars {
     // Picard convergence check
     double dmax, hmax, picard_tol;
     int32_t picard_converged, picard_nan;
-    // mixed-precision outer loop
-    double ir_rnorm, ir_rnorm0;
+    // mixed-precision outer loop (FP64 residual in the Jacobi-weighted norm)
+    double ir_rnorm, ir_rnorm0, ir_atol, ir_rtol;
+    int32_t ir_done, ir_pad;
     // block-completion counter of the running reduction kernel
     uint32_t counter;
     uint32_t pad;
@@ -90,7 +91,7 @@ void copy(const double* a, double* b, size_t n, hipStream_t st);
 
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
-              hipStream_t st);
+              hipStream_t st, double* cP = nullptr);
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st);
 
@@ -98,35 +99,59 @@ void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T*
 template <class T>
 void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Scalars* sc, hipStream_t st);
 template <class T>
-void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* p_old, T* p_new,
-                    T* Ap, int k, double* partials, Scalars* sc, hipStream_t st);
+void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* r, const T* p_old,
+                    T* p_new, T* Ap, int k, double* partials, Scalars* sc, hipStream_t st);
 template <class T>
 void lanczos_step_b(const Geom& G, const T* Ap, T* r, int k, double* partials, Scalars* sc, hipStream_t st);
 void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStream_t st);
 
 // Preconditioned CG pieces (mode 0: set-up, mode 1: iteration).
 template <class T>
-void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* b, const T* x0, T* r, T* z, T* xacc,
-             T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st);
+void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* b, const T* x0, T* r, T* z,
+             T* xacc, T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st);
 template <class T>
-void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* s_in, const T* pc_in,
-               T* s_out, T* pc_out, T* y, int k, int degree, int mode, double* partials, Scalars* sc,
+void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* r, const T* s_in,
+               const T* pc_in, T* s_out, T* pc_out, T* y, int k, int degree, int mode, double* partials, Scalars* sc,
                hipStream_t st);
 template <class T>
-void cheb_degree1(const Geom& G, const T* aP, const T* r, T* y, int mode, double* partials, Scalars* sc,
+void cheb_degree1(const Geom& G, const T* dg, const T* r, T* y, int mode, double* partials, Scalars* sc,
                   hipStream_t st);
 template <class T>
 void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
                double* partials, Scalars* sc, hipStream_t st);
 template <class T>
-void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
+void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st);
 // Fused CG update + Chebyshev application (2 <= degree <= 8); r is ping-ponged.
 // Returns -1 for an unsupported degree.
 template <class T>
-int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* p, const T* Ap,
-                   const T* r_old, T* r_new, T* x, T* z, double* partials, Scalars* sc, hipStream_t st);
+int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* dg, const T* p,
+                   const T* Ap, const T* r_old, T* r_new, T* x, T* z, double* partials, Scalars* sc,
+                   hipStream_t st);
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st);
+
+// Geometric multigrid on FP32 flux-form levels (see shk_kernels.hip).
+// dot_mode: -1 no dot, 0 CG set-up gamma, 1 CG iteration gamma.
+void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float* ef, const float* nf, float* cc,
+                float* ec, float* nc, hipStream_t st);
+void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st);
+void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
+             const float* D, const float* b, float* x, float* bc, hipStream_t st);
+void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
+           const float* D, const float* b, const float* x, const float* xc, float* out, int dot_mode,
+           double* partials, Scalars* sc, hipStream_t st);
+void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
+                      hipStream_t st);
+void mg_coarse(int n, const float* inv, const float* b, float* x, hipStream_t st);
+
+// Mixed precision: FP32 copies of the coefficients; the FP64 residual
+// r = b - A x with its Jacobi-weighted norm (first != 0 records the initial
+// norm) and its FP32 copy; the FP64 correction x += d.
+void to_f32(size_t n, const double* a, float* b, hipStream_t st);
+void ir_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
+                 const double* x, float* r32, int first, double* partials, Scalars* sc, hipStream_t st);
+void ir_correct(size_t n, double* x, const float* d, hipStream_t st);
+void relax(size_t n, double* h, const double* h_prev, double alpha, hipStream_t st);
 }  // namespace launch
 
 }  // namespace shk

@@ -460,9 +460,13 @@ __device__ __forceinline__ double bK(const Phys& P, double K1, double K2, double
 // fused with the K_face scheme and Dirichlet folding, writing the symmetric
 // layout (aP, aE, aN) and rhs.  The Jacobi diagonal is aP itself
 // (update_diag_precond!, src/linear_solver.jl:697-700).
+// cP (optional) receives the diagonal minus the kept couplings -- the creep
+// (or storage) reaction plus the conductances of faces to Dirichlet cells --
+// summed without cancellation, for the FP32 flux-form operator.
 template <bool PARABOLIC, class T>
 __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, int R, T* __restrict__ aP,
-                                                  T* __restrict__ aE, T* __restrict__ aN, T* __restrict__ rhs) {
+                                                  T* __restrict__ aE, T* __restrict__ aN, T* __restrict__ rhs,
+                                                  double* __restrict__ cP) {
     const int nx = P.nx, ny = P.ny;
     int vbx, vby;
     virtual_block(vbx, vby);
@@ -477,11 +481,13 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, i
         if (m == OCEAN || m == LAND) {
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
             rhs[c] = dirichlet_head(P, m, s.zb[c]);
+            if (cP) cP[c] = 1.0;
             continue;
         }
         if (m == OTHER_BASIN) {
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
             rhs[c] = s.h[c];
+            if (cP) cP[c] = 1.0;
             continue;
         }
         const double Kc = s.K[c];
@@ -505,6 +511,14 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, i
         // own row is an identity row otherwise, with its coupling zero)
         if (mE != GROUNDED) cE = 0.0;
         if (mN != GROUNDED) cN = 0.0;
+        if (cP) {
+            double c0 = PARABOLIC ? P.e_v / dt : P.n * P.rho_w * P.g * Ac * Np * bc;
+            if (ix < nx - 1 && (mE == OCEAN || mE == LAND)) c0 += fE;
+            if (ix > 0 && (mW == OCEAN || mW == LAND)) c0 += fW;
+            if (iy < ny - 1 && (mN == OCEAN || mN == LAND)) c0 += fN;
+            if (iy > 0 && (mS == OCEAN || mS == LAND)) c0 += fS;
+            cP[c] = c0;
+        }
         double r;
         if (PARABOLIC)
             r = s.mdot[c] * (1.0 / P.rho_w - 1.0 / P.rho_i) - s.beta[c] * s.abs_ub[c] + Ac * Np * Nc * bc + s.ieb[c] +
@@ -530,6 +544,9 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, i
 // neighbours from wave shuffles (only the two strip-edge lanes evaluate the
 // operand of the neighbouring strip).  All 64 lanes stay active so the
 // shuffles are well defined; lanes past the east edge only skip their stores.
+// FP32 systems are applied in flux form, c*u + sum_f a_f (u - u_f), with aP
+// holding c = diagonal minus kept couplings (assembled exactly in FP64): for
+// smooth u the FP32 diagonal form would subtract nearly equal numbers.
 template <class T, class Opd, class Epi>
 __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, const T* __restrict__ aP,
                                       const T* __restrict__ aE, const T* __restrict__ aN, Opd opd, Epi epi) {
@@ -560,11 +577,21 @@ __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, c
             uw = opd(c - 1);
             aEw = aE[c - 1];
         }
-        T y = aPc * uc;
-        if (ix < nx - 1) y -= aEc * ue;
-        if (ix > 0) y -= aEw * uw;
-        if (iy < ny - 1) y -= aNc * up;
-        if (iy > 0) y -= aNm * um;
+        T y;
+        if (sizeof(T) == 4) {
+            // FP32: flux form c*u + sum a_f (u - u_nb), aP holding c (no cancellation for smooth u)
+            y = aPc * uc;
+            if (ix < nx - 1) y += aEc * (uc - ue);
+            if (ix > 0) y += aEw * (uc - uw);
+            if (iy < ny - 1) y += aNc * (uc - up);
+            if (iy > 0) y += aNm * (uc - um);
+        } else {
+            y = aPc * uc;
+            if (ix < nx - 1) y -= aEc * ue;
+            if (ix > 0) y -= aEw * uw;
+            if (iy < ny - 1) y -= aNc * up;
+            if (iy > 0) y -= aNm * um;
+        }
         if (inx) epi(c, y, uc);
         um = uc;
         uc = up;
@@ -618,16 +645,16 @@ __global__ __launch_bounds__(256) void k_lz_init(int nx, int ny, int R, const T*
 template <class T>
 __global__ __launch_bounds__(256) void k_lz_step_a(int nx, int ny, int R, const T* __restrict__ aP,
                                                    const T* __restrict__ aE, const T* __restrict__ aN,
-                                                   const T* __restrict__ r, const T* __restrict__ p_old,
-                                                   T* __restrict__ p_new, T* __restrict__ Ap, int k,
-                                                   double* partials, Scalars* sc) {
+                                                   const T* __restrict__ dg, const T* __restrict__ r,
+                                                   const T* __restrict__ p_old, T* __restrict__ p_new,
+                                                   T* __restrict__ Ap, int k, double* partials, Scalars* sc) {
     int vbx, vby;
     virtual_block(vbx, vby);
     const T beta = (T)sc->beta;
     double acc = 0.0;
     march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return r[j] + beta * p_old[j]; },
              [&](size_t c, T Au, T u) {
-                 const T a = Au / aP[c];
+                 const T a = Au / dg[c];
                  p_new[c] = u;
                  Ap[c] = a;
                  acc += (double)u * (double)a;
@@ -800,7 +827,8 @@ __device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_init(int nx, int ny, int R, const T* __restrict__ aP,
                                                  const T* __restrict__ aE, const T* __restrict__ aN,
-                                                 const T* __restrict__ b, const T* __restrict__ x0,
+                                                 const T* __restrict__ dg, const T* __restrict__ b,
+                                                 const T* __restrict__ x0,
                                                  T* __restrict__ r, T* __restrict__ z, T* __restrict__ xacc,
                                                  T* __restrict__ p_zero, int jacobi, double* partials,
                                                  Scalars* sc) {
@@ -815,7 +843,7 @@ __global__ __launch_bounds__(256) void k_cg_init(int nx, int ny, int R, const T*
                      xacc[c] = 0;
                      p_zero[c] = 0;
                      if (jacobi) {
-                         const T zv = rv / aP[c];
+                         const T zv = rv / dg[c];
                          z[c] = zv;
                          acc += (double)rv * (double)zv;
                      }
@@ -827,7 +855,7 @@ __global__ __launch_bounds__(256) void k_cg_init(int nx, int ny, int R, const T*
             xacc[c] = 0;
             p_zero[c] = 0;
             if (jacobi) {
-                const T zv = rv / aP[c];
+                const T zv = rv / dg[c];
                 z[c] = zv;
                 acc += (double)rv * (double)zv;
             }
@@ -850,7 +878,8 @@ __global__ __launch_bounds__(256) void k_cg_init(int nx, int ny, int R, const T*
 template <class T>
 __global__ __launch_bounds__(256) void k_cheb_pass(int nx, int ny, int R, const T* __restrict__ aP,
                                                    const T* __restrict__ aE, const T* __restrict__ aN,
-                                                   const T* __restrict__ r, const T* __restrict__ s_in,
+                                                   const T* __restrict__ dg, const T* __restrict__ r,
+                                                   const T* __restrict__ s_in,
                                                    const T* __restrict__ pc_in, T* __restrict__ s_out,
                                                    T* __restrict__ pc_out, T* __restrict__ y, int k, int degree,
                                                    int mode, double* partials, Scalars* sc) {
@@ -861,9 +890,9 @@ __global__ __launch_bounds__(256) void k_cheb_pass(int nx, int ny, int R, const 
     const bool last = (k == degree - 1);
     double acc = 0.0;
     if (k == 1) {
-        march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return r[j] / aP[j]; },
+        march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return r[j] / dg[j]; },
                  [&](size_t c, T Au, T p0) {
-                     const T s1 = p0 - a_prev * (Au / aP[c]);
+                     const T s1 = p0 - a_prev * (Au / dg[c]);
                      const T p1 = s1 + b_k * p0;
                      const T yv = a_prev * p0 + a_k * p1;
                      if (!last) {
@@ -877,7 +906,7 @@ __global__ __launch_bounds__(256) void k_cheb_pass(int nx, int ny, int R, const 
     } else {
         march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return pc_in[j]; },
                  [&](size_t c, T Au, T pk) {
-                     const T sk = s_in[c] - a_prev * (Au / aP[c]);
+                     const T sk = s_in[c] - a_prev * (Au / dg[c]);
                      const T pn = sk + b_k * pk;
                      const T yv = y[c] + a_k * pn;
                      if (!last) {
@@ -897,7 +926,7 @@ __global__ __launch_bounds__(256) void k_cheb_pass(int nx, int ny, int R, const 
 
 // Chebyshev of degree 1: y = a0 * (r ./ d), gamma = r'y.
 template <class T>
-__global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const T* __restrict__ aP,
+__global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const T* __restrict__ dg,
                                                    const T* __restrict__ r, T* __restrict__ y, int mode,
                                                    double* partials, Scalars* sc) {
     if (sc->done) return;
@@ -907,7 +936,7 @@ __global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const 
     double acc = 0.0;
     walk(nx, ny, R, vbx, vby, [&](size_t c) {
         const T rv = r[c];
-        const T yv = a0 * (rv / aP[c]);
+        const T yv = a0 * (rv / dg[c]);
         y[c] = yv;
         acc += (double)rv * (double)yv;
     });
@@ -940,7 +969,7 @@ __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const 
 
 // x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
 template <class T>
-__global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const T* __restrict__ aP,
+__global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const T* __restrict__ dg,
                                                    const T* __restrict__ p, const T* __restrict__ Ap,
                                                    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                    int jacobi, double* partials, Scalars* sc) {
@@ -954,7 +983,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
         const T rv = r[c] - alpha * Ap[c];
         r[c] = rv;
         if (jacobi) {
-            const T zv = rv / aP[c];
+            const T zv = rv / dg[c];
             z[c] = zv;
             acc += (double)rv * (double)zv;
         }
@@ -1024,7 +1053,8 @@ __device__ __forceinline__ float from_east(float v) { return __int_as_float(dpp_
 template <class T, int H>
 __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, const T* __restrict__ aP,
                                                         const T* __restrict__ aE, const T* __restrict__ aN,
-                                                        const T* __restrict__ p, const T* __restrict__ Ap,
+                                                        const T* __restrict__ dg, const T* __restrict__ p,
+                                                        const T* __restrict__ Ap,
                                                         const T* __restrict__ r_old, T* __restrict__ r_new,
                                                         T* __restrict__ x, T* __restrict__ z, double* partials,
                                                         Scalars* sc) {
@@ -1046,11 +1076,12 @@ __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, c
         ca[k] = (T)sc->cheb_alpha[k];
         cb[k] = (T)sc->cheb_beta[k];
     }
-    T wP[H + 1], wE[H + 1], wW[H + 1], wN[H + 2], wR[H + 1];
+    constexpr bool FLUX = sizeof(T) == 4;  // FP32: flux form, diagonal from dg
+    T wP[H + 1], wE[H + 1], wW[H + 1], wN[H + 2], wR[H + 1], wD[H + 1];
     T pw[H][3];
     T sw[H][2], yw[H][2];
 #pragma unroll
-    for (int k = 0; k <= H; ++k) { wP[k] = 1; wE[k] = 0; wW[k] = 0; wR[k] = 0; }
+    for (int k = 0; k <= H; ++k) { wP[k] = 1; wE[k] = 0; wW[k] = 0; wR[k] = 0; wD[k] = 1; }
 #pragma unroll
     for (int k = 0; k < H + 2; ++k) wN[k] = 0;
 #pragma unroll
@@ -1067,26 +1098,32 @@ __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, c
     const T* pN = aN + ixc;
     const T* pR = r_old + ixc;
     const T* pA = Ap + ixc;
+    const T* pD = dg + ixc;
     auto row_ofs = [&](int j) -> size_t { return (size_t)(j < ny ? j : ny - 1) * (size_t)nx; };
-    T nP = 1, nE = 0, nN = 0, nR = 0, nA = 0;  // prefetched inputs of row j
+    T nP = 1, nE = 0, nN = 0, nR = 0, nA = 0, nD = 1;  // prefetched inputs of row j
     if (j0 < j1) {
         const size_t o0 = row_ofs(j0);
         nP = pP[o0]; nE = pE[o0]; nN = pN[o0]; nR = pR[o0]; nA = pA[o0];
+        if (FLUX) nD = pD[o0];
     }
     // unrolled so that the window shifts become register renames
 #pragma unroll 4
     for (int j = j0; j < j1; ++j) {
 #pragma unroll
-        for (int k = H; k > 0; --k) { wP[k] = wP[k - 1]; wE[k] = wE[k - 1]; wW[k] = wW[k - 1]; wR[k] = wR[k - 1]; }
+        for (int k = H; k > 0; --k) {
+            wP[k] = wP[k - 1]; wE[k] = wE[k - 1]; wW[k] = wW[k - 1]; wR[k] = wR[k - 1]; wD[k] = wD[k - 1];
+        }
 #pragma unroll
         for (int k = H + 1; k > 0; --k) wN[k] = wN[k - 1];
         // ---- stage 0 at row j (inputs prefetched); issue the loads of row j+1 first
         const bool live = inx && j < ny;
         const T dP = live ? nP : T(1), dE = live ? nE : T(0), dN = live ? nN : T(0);
+        const T dD = FLUX ? (live ? nD : T(1)) : dP;
         const T r0 = nR, a0 = nA;
         {
             const size_t o1 = row_ofs(j + 1);
             nP = pP[o1]; nE = pE[o1]; nN = pN[o1]; nR = pR[o1]; nA = pA[o1];
+            if (FLUX) nD = pD[o1];
         }
         const T rr = live ? r0 - alpha * a0 : T(0);
         if (own_x && j >= iy0 && j < iy1) {
@@ -1094,9 +1131,9 @@ __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, c
             x[c] = x[c] + alpha * p[c];
             r_new[c] = rr;
         }
-        const T pv = rr / dP;
+        const T pv = rr / dD;
         const T dW = from_west(dE);
-        wP[0] = dP; wE[0] = dE; wW[0] = dW; wN[0] = dN; wR[0] = rr;
+        wP[0] = dP; wE[0] = dE; wW[0] = dW; wN[0] = dN; wR[0] = rr; wD[0] = dD;
         T p_in = pv, s_in = pv, y_in = ca[0] * pv;
         // ---- stages 1..H: stage k at row o = j - k
 #pragma unroll
@@ -1118,11 +1155,18 @@ __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, c
             const T tN = (o < ny - 1) ? wN[k] * pn : T(0);
             const T tS = (o > 0) ? wN[k + 1] * ps : T(0);
             T Au = wP[k] * pc;
-            Au -= tE;
-            Au -= tW;
-            Au -= tN;
-            Au -= tS;
-            const T sk = sw[k - 1][1] - ca[k - 1] * (Au / wP[k]);
+            if (FLUX) {
+                Au += has_e ? wE[k] * (pc - pe) : T(0);
+                Au += has_w ? wW[k] * (pc - pwst) : T(0);
+                Au += (o < ny - 1) ? wN[k] * (pc - pn) : T(0);
+                Au += (o > 0) ? wN[k + 1] * (pc - ps) : T(0);
+            } else {
+                Au -= tE;
+                Au -= tW;
+                Au -= tN;
+                Au -= tS;
+            }
+            const T sk = sw[k - 1][1] - ca[k - 1] * (Au / wD[k]);
             const T pk = sk + cb[k] * pc;
             const T yk = yw[k - 1][1] + ca[k] * pk;
             p_in = pk;
@@ -1140,6 +1184,221 @@ __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, c
     block_partial<1, false>(v, partials);
 }
 
+// --------------------------------------------------- mixed precision pieces
+
+__global__ __launch_bounds__(256) void k_to_f32(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+// FP64 residual of the FP64 system, r32 = (float) r, and the partial of
+// r' D^-1 r (D = aP) for the refinement's stopping test.
+__global__ __launch_bounds__(256) void k_ir_residual(int nx, int ny, int R, const double* __restrict__ aP,
+                                                     const double* __restrict__ aE, const double* __restrict__ aN,
+                                                     const double* __restrict__ b, const double* __restrict__ x,
+                                                     float* __restrict__ r32, double* partials) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    double acc = 0.0;
+    march<double>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> double { return x[k]; },
+                  [&](size_t c, double Au, double) {
+                      const double r = b[c] - Au;
+                      r32[c] = (float)r;
+                      acc += r * r / aP[c];
+                  });
+    double v[1] = {acc};
+    block_partial<1, false>(v, partials);
+}
+
+__global__ __launch_bounds__(256) void k_ir_correct(size_t n, double* __restrict__ x, const float* __restrict__ d) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + (double)d[i];
+}
+
+__global__ __launch_bounds__(256) void k_relax(size_t n, double* __restrict__ h, const double* __restrict__ hp,
+                                               double a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) h[i] = a * h[i] + (1.0 - a) * hp[i];
+}
+
+// ------------------------------------------------- geometric multigrid (FP32)
+
+// Cell-centred multigrid on the flux-form operator A u = c u + sum_f a_f (u - u_f).
+// Coarse cell (I,J) aggregates fine cells (2I..2I+1, 2J..2J+1); with
+// piecewise-constant prolongation P and restriction P^T the Galerkin operator
+// P^T A P is again a 5-point flux-form operator: c sums over the block, a
+// coarse face sums the fine faces it replaces, faces inside the block cancel.
+
+__global__ __launch_bounds__(256) void k_mg_coarsen(int nxf, int nyf, int nxc, int nyc,
+                                                    const float* __restrict__ cf, const float* __restrict__ ef,
+                                                    const float* __restrict__ nf, float* __restrict__ cc,
+                                                    float* __restrict__ ec, float* __restrict__ nc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= nxc || J >= nyc) return;
+    const int x0 = 2 * I, y0 = 2 * J;
+    const int x1 = min(x0 + 1, nxf - 1), y1 = min(y0 + 1, nyf - 1);
+    float c = 0.f, e = 0.f, n = 0.f;
+    for (int y = y0; y <= y1; ++y)
+        for (int x = x0; x <= x1; ++x) c += cf[(size_t)x + (size_t)y * nxf];
+    for (int y = y0; y <= y1; ++y) e += ef[(size_t)x1 + (size_t)y * nxf];  // fine faces on the block's east side
+    for (int x = x0; x <= x1; ++x) n += nf[(size_t)x + (size_t)y1 * nxf];  // ... and north side
+    cc[(size_t)I + (size_t)J * nxc] = c;
+    ec[(size_t)I + (size_t)J * nxc] = (I < nxc - 1) ? e : 0.f;
+    nc[(size_t)I + (size_t)J * nxc] = (J < nyc - 1) ? n : 0.f;
+}
+
+// Jacobi diagonal of a flux-form level: c + aE + aW + aN + aS.
+__global__ __launch_bounds__(256) void k_mg_diag(int nx, int ny, const float* __restrict__ c,
+                                                 const float* __restrict__ e, const float* __restrict__ n,
+                                                 float* __restrict__ D) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    const size_t k = (size_t)ix + (size_t)iy * nx;
+    float d = c[k] + e[k] + n[k];
+    if (ix > 0) d += e[k - 1];
+    if (iy > 0) d += n[k - nx];
+    D[k] = d;
+}
+
+// Flux-form residual helper: (A u)_k with u given by a functor of the index.
+template <class U>
+__device__ __forceinline__ float flux_apply(int nx, int ny, int ix, int iy, size_t k, const float* __restrict__ c,
+                                            const float* __restrict__ e, const float* __restrict__ n, U u) {
+    const float uc = u(k);
+    float y = c[k] * uc;
+    if (ix < nx - 1) y += e[k] * (uc - u(k + 1));
+    if (ix > 0) y += e[k - 1] * (uc - u(k - 1));
+    if (iy < ny - 1) y += n[k] * (uc - u(k + nx));
+    if (iy > 0) y += n[k - nx] * (uc - u(k - nx));
+    return y;
+}
+
+// Down-leg of the V-cycle on one level: one damped-Jacobi sweep from zero,
+// x = omega b / D, stored for the up-leg, and the restricted residual
+// bc = P^T (b - A x).  One thread per coarse cell.
+__global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int nyc, float omega,
+                                                 const float* __restrict__ c, const float* __restrict__ e,
+                                                 const float* __restrict__ n, const float* __restrict__ D,
+                                                 const float* __restrict__ b, float* __restrict__ x,
+                                                 float* __restrict__ bc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= nxc || J >= nyc) return;
+    auto xs = [&](size_t k) -> float { return omega * b[k] / D[k]; };
+    float acc = 0.f;
+    const int x0 = 2 * I, y0 = 2 * J;
+    const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
+    for (int iy = y0; iy <= y1; ++iy)
+        for (int ix = x0; ix <= x1; ++ix) {
+            const size_t k = (size_t)ix + (size_t)iy * nx;
+            const float xv = xs(k);
+            x[k] = xv;
+            acc += b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+        }
+    bc[(size_t)I + (size_t)J * nxc] = acc;
+}
+
+// Up-leg on one level: add the (scaled) coarse correction, one damped-Jacobi
+// sweep, write the level's result.  On the finest level also the partial of
+// r'z (r = b, z = the result) for CG.
+__global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float omega, float scale,
+                                               const float* __restrict__ c, const float* __restrict__ e,
+                                               const float* __restrict__ n, const float* __restrict__ D,
+                                               const float* __restrict__ b, const float* __restrict__ x,
+                                               const float* __restrict__ xc, float* __restrict__ out, int with_dot,
+                                               double* partials) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    double acc = 0.0;
+    if (ix < nx && iy < ny) {
+        auto xs = [&](size_t k) -> float {
+            const int kx = (int)(k % (size_t)nx), ky = (int)(k / (size_t)nx);
+            return x[k] + scale * xc[(size_t)(kx >> 1) + (size_t)(ky >> 1) * nxc];
+        };
+        const size_t k = (size_t)ix + (size_t)iy * nx;
+        const float u = xs(k);
+        const float z = u + omega * (b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs)) / D[k];
+        out[k] = z;
+        acc = (double)b[k] * (double)z;
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+    }
+}
+
+// Coarsest level: assemble the dense matrix (FP64, in `work`) and invert it
+// in place by Gauss-Jordan elimination (SPD, so no pivoting); the inverse is
+// stored in FP32.  One workgroup; n <= 512.
+__global__ __launch_bounds__(256) void k_mg_dense_inverse(int nx, int ny, const float* __restrict__ c,
+                                                          const float* __restrict__ e,
+                                                          const float* __restrict__ nn, double* __restrict__ work,
+                                                          float* __restrict__ inv) {
+    const int n = nx * ny;
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nt = blockDim.x * blockDim.y;
+    // [A | I] with A in work[0 .. n*n) and I in work[n*n .. 2n*n), row-major
+    double* A = work;
+    double* B = work + (size_t)n * n;
+    for (int t = tid; t < n * n; t += nt) {
+        A[t] = 0.0;
+        B[t] = (t / n == t % n) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const int k = ix + iy * nx;
+                double d = c[k];
+                if (ix < nx - 1) { d += e[k]; A[(size_t)k * n + k + 1] = -(double)e[k]; }
+                if (ix > 0) { d += e[k - 1]; A[(size_t)k * n + k - 1] = -(double)e[k - 1]; }
+                if (iy < ny - 1) { d += nn[k]; A[(size_t)k * n + k + nx] = -(double)nn[k]; }
+                if (iy > 0) { d += nn[k - nx]; A[(size_t)k * n + k - nx] = -(double)nn[k - nx]; }
+                A[(size_t)k * n + k] = d;
+            }
+    }
+    __syncthreads();
+    __shared__ double piv;
+    for (int k = 0; k < n; ++k) {
+        if (tid == 0) piv = A[(size_t)k * n + k];
+        __syncthreads();
+        const double ip = 1.0 / piv;
+        for (int j = tid; j < n; j += nt) {
+            A[(size_t)k * n + j] *= ip;
+            B[(size_t)k * n + j] *= ip;
+        }
+        __syncthreads();
+        for (int t = tid; t < n * n; t += nt) {
+            const int i = t / n, j = t % n;
+            if (i == k) continue;
+            const double f = A[(size_t)i * n + k];
+            if (j != k) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+            B[(size_t)i * n + j] -= f * B[(size_t)k * n + j];
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += nt)
+            if (i != k) A[(size_t)i * n + k] = 0.0;
+        __syncthreads();
+    }
+    for (int t = tid; t < n * n; t += nt) inv[t] = (float)B[t];
+}
+
+// Coarsest-level solve x = A^-1 b with the stored inverse (one workgroup).
+__global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* __restrict__ inv, const float* __restrict__ b,
+                                                   float* __restrict__ x) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nt = blockDim.x * blockDim.y;
+    // the inverse of the SPD coarse operator is symmetric: read it by columns so
+    // that neighbouring threads load neighbouring words
+    for (int i = tid; i < n; i += nt) {
+        float acc = 0.f;
+        for (int j = 0; j < n; ++j) acc += inv[(size_t)j * n + i] * b[j];
+        x[i] = acc;
+    }
+}
+
 // ------------------------------------------------------- finishing kernel
 
 enum FinMode {
@@ -1150,6 +1409,8 @@ enum FinMode {
     FIN_LZ_ALPHA = 4,    // Lanczos: alpha_k = gamma / p'(D^-1 A)p
     FIN_LZ_BETA = 5,     // Lanczos: beta_k = r'r / gamma
     FIN_PICARD = 6,      // Picard: max|h - h_prev|, max|h| -> converged flag
+    FIN_IR_FIRST = 7,    // refinement: initial FP64 residual norm
+    FIN_IR = 8,          // refinement: FP64 residual norm -> done flag
 };
 
 // One workgroup: sums (or maximises) the nb partials of the preceding
@@ -1203,6 +1464,14 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
             sc->lz_beta[k] = b;
             sc->beta = b;
             sc->lz_gamma = v0;
+            break;
+        }
+        case FIN_IR_FIRST:
+        case FIN_IR: {
+            const double rn = sqrt(v0);
+            sc->ir_rnorm = rn;
+            if (mode == FIN_IR_FIRST) sc->ir_rnorm0 = rn;
+            sc->ir_done = (rn <= sc->ir_atol + sc->ir_rtol * sc->ir_rnorm0) || (rn + 1.0 <= 1.0);
             break;
         }
         case FIN_PICARD:
@@ -1288,6 +1557,20 @@ void gap(const Phys& P, Fields F, double dt, hipStream_t st) {
 void fill_ieb(int n, double* ieb, double u, const double* moulin, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_ieb, dim3((n + 255) / 256), dim3(256), 0, st, n, ieb, u, moulin);
 }
+void to_f32(size_t n, const double* a, float* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b);
+}
+void ir_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
+                 const double* x, float* r32, int first, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_ir_residual, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, r32, partials);
+    finish(G, partials, first ? FIN_IR_FIRST : FIN_IR, 0, sc, st);
+}
+void ir_correct(size_t n, double* x, const float* d, hipStream_t st) {
+    hipLaunchKernelGGL(k_ir_correct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, x, d);
+}
+void relax(size_t n, double* h, const double* hp, double a, hipStream_t st) {
+    hipLaunchKernelGGL(k_relax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, h, hp, a);
+}
 void copy(const double* a, double* b, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b);
 }
@@ -1297,11 +1580,11 @@ void zero(double* p, size_t n, hipStream_t st) {
 
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
-              hipStream_t st) {
+              hipStream_t st, double* cP) {
     if (parabolic)
-        hipLaunchKernelGGL((k_assemble<true, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs);
+        hipLaunchKernelGGL((k_assemble<true, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs, cP);
     else
-        hipLaunchKernelGGL((k_assemble<false, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs);
+        hipLaunchKernelGGL((k_assemble<false, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs, cP);
 }
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st) {
@@ -1313,10 +1596,10 @@ void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Sc
     finish(G, partials, FIN_LZ_GAMMA0, 0, sc, st);
 }
 template <class T>
-void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* p_old, T* p_new,
-                    T* Ap, int k, double* partials, Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_lz_step_a<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, r, p_old, p_new, Ap, k,
-                       partials, sc);
+void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* r, const T* p_old,
+                    T* p_new, T* Ap, int k, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_lz_step_a<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, r, p_old, p_new, Ap,
+                       k, partials, sc);
     finish(G, partials, FIN_LZ_ALPHA, k, sc, st);
 }
 template <class T>
@@ -1328,24 +1611,24 @@ void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStrea
     hipLaunchKernelGGL(k_cheb_bounds, dim3(1), dim3(64), 0, st, nsteps, degree, use_estimate, sc);
 }
 template <class T>
-void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* b, const T* x0, T* r, T* z, T* xacc,
-             T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x0, r, z, xacc, p_zero,
-                       jacobi, partials, sc);
+void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* b, const T* x0, T* r, T* z,
+             T* xacc, T* p_zero, int jacobi, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, b, x0, r, z, xacc,
+                       p_zero, jacobi, partials, sc);
     if (jacobi) finish(G, partials, FIN_GAMMA_INIT, 0, sc, st);
 }
 template <class T>
-void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* r, const T* s_in, const T* pc_in,
-               T* s_out, T* pc_out, T* y, int k, int degree, int mode, double* partials, Scalars* sc,
+void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* r, const T* s_in,
+               const T* pc_in, T* s_out, T* pc_out, T* y, int k, int degree, int mode, double* partials, Scalars* sc,
                hipStream_t st) {
-    hipLaunchKernelGGL(k_cheb_pass<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, r, s_in, pc_in, s_out,
-                       pc_out, y, k, degree, mode, partials, sc);
+    hipLaunchKernelGGL(k_cheb_pass<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, r, s_in, pc_in,
+                       s_out, pc_out, y, k, degree, mode, partials, sc);
     if (k == degree - 1) finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
-void cheb_degree1(const Geom& G, const T* aP, const T* r, T* y, int mode, double* partials, Scalars* sc,
+void cheb_degree1(const Geom& G, const T* dg, const T* r, T* y, int mode, double* partials, Scalars* sc,
                   hipStream_t st) {
-    hipLaunchKernelGGL(k_cheb_deg1<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, r, y, mode, partials, sc);
+    hipLaunchKernelGGL(k_cheb_deg1<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, dg, r, y, mode, partials, sc);
     finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
@@ -1356,21 +1639,22 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
     finish(G, partials, FIN_PAP, 0, sc, st);
 }
 template <class T>
-void cg_update(const Geom& G, const T* aP, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
+void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_update<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, p, Ap, x, r, z, jacobi,
+    hipLaunchKernelGGL(k_cg_update<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, dg, p, Ap, x, r, z, jacobi,
                        partials, sc);
     if (jacobi) finish(G, partials, FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
-int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* p, const T* Ap,
-                   const T* r_old, T* r_new, T* x, T* z, double* partials, Scalars* sc, hipStream_t st) {
+int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* dg, const T* p,
+                   const T* Ap, const T* r_old, T* r_new, T* x, T* z, double* partials, Scalars* sc,
+                   hipStream_t st) {
     const int H = degree - 1;
     const Geom F = fused_geom(G, H);
 #define SHK_FUSED(HH)                                                                                             \
     case HH:                                                                                                      \
-        hipLaunchKernelGGL((k_cg_update_cheb<T, HH>), F.grid, F.block, 0, st, G.nx, G.ny, F.R, aP, aE, aN, p, Ap, \
-                           r_old, r_new, x, z, partials, sc);                                                     \
+        hipLaunchKernelGGL((k_cg_update_cheb<T, HH>), F.grid, F.block, 0, st, G.nx, G.ny, F.R, aP, aE, aN, dg, p, \
+                           Ap, r_old, r_new, x, z, partials, sc);                                                 \
         break;
     switch (H) {
         SHK_FUSED(1)
@@ -1387,28 +1671,60 @@ int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T*
     return 0;
 }
 
+void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float* ef, const float* nf, float* cc,
+                float* ec, float* nc, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_coarsen, cells_grid(nxc, nyc), dim3(64, 4), 0, st, nxf, nyf, nxc, nyc, cf, ef, nf, cc,
+                       ec, nc);
+}
+void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_diag, cells_grid(nx, ny), dim3(64, 4), 0, st, nx, ny, c, e, n, D);
+}
+void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
+             const float* D, const float* b, float* x, float* bc, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_down, cells_grid(nxc, nyc), dim3(64, 4), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D, b,
+                       x, bc);
+}
+void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
+           const float* D, const float* b, const float* x, const float* xc, float* out, int dot_mode,
+           double* partials, Scalars* sc, hipStream_t st) {
+    const dim3 grid = cells_grid(nx, ny);
+    hipLaunchKernelGGL(k_mg_up, grid, dim3(64, 4), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc, out,
+                       dot_mode >= 0 ? 1 : 0, partials);
+    if (dot_mode >= 0)
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
+                           dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
+}
+void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_dense_inverse, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, work, inv);
+}
+void mg_coarse(int n, const float* inv, const float* b, float* x, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(64, 4), 0, st, n, inv, b, x);
+}
+
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, x, h_prev, h_new, relax);
 }
 
 #define SHK_INST(T)                                                                                               \
-    template void assemble<T>(const Geom&, const Phys&, Fields, int, double, T*, T*, T*, T*, hipStream_t);        \
+    template void assemble<T>(const Geom&, const Phys&, Fields, int, double, T*, T*, T*, T*, hipStream_t,         \
+                              double*);                                                                           \
     template void matvec<T>(const Geom&, const T*, const T*, const T*, const T*, T*, hipStream_t);                \
     template void lanczos_init<T>(const Geom&, const T*, T*, T*, double*, Scalars*, hipStream_t);                 \
-    template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, int,   \
-                                    double*, Scalars*, hipStream_t);                                              \
+    template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, T*,  \
+                                    T*, int, double*, Scalars*, hipStream_t);                                     \
     template void lanczos_step_b<T>(const Geom&, const T*, T*, int, double*, Scalars*, hipStream_t);              \
-    template void cg_init<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, T*, T*, int,  \
-                             double*, Scalars*, hipStream_t);                                                     \
-    template void cheb_pass<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, T*, T*,   \
-                               T*, int, int, int, double*, Scalars*, hipStream_t);                                \
+    template void cg_init<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, T*, T*, T*,  \
+                             T*, int, double*, Scalars*, hipStream_t);                                            \
+    template void cheb_pass<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, const T*, \
+                               T*, T*, T*, int, int, int, double*, Scalars*, hipStream_t);                        \
     template void cheb_degree1<T>(const Geom&, const T*, const T*, T*, int, double*, Scalars*, hipStream_t);      \
     template void cg_matvec<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, T*, T*, double*,    \
                                Scalars*, hipStream_t);                                                            \
     template void cg_update<T>(const Geom&, const T*, const T*, const T*, T*, T*, T*, int, double*, Scalars*,     \
                                hipStream_t);                                                                      \
     template int cg_update_cheb<T>(const Geom&, int, const T*, const T*, const T*, const T*, const T*, const T*,  \
-                                   T*, T*, T*, double*, Scalars*, hipStream_t);
+                                   const T*, T*, T*, T*, double*, Scalars*, hipStream_t);
 SHK_INST(double)
 SHK_INST(float)
 #undef SHK_INST

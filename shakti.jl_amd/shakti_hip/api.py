@@ -194,7 +194,8 @@ class State:
 
     # per-kernel HIP-event timing inside the library (see shk_profile_*)
     PROFILE_IDS = ["assemble", "lanczos", "cheb_bounds", "cg_init", "cheb_pass", "cg_matvec", "cg_update",
-                   "cg_finish", "derived", "gap", "stagger_b", "ieb", "ic", "matvec", "ir", "cg_update_cheb"]
+                   "cg_finish", "derived", "gap", "stagger_b", "ieb", "ic", "matvec", "ir", "cg_update_cheb",
+                   "mg_down", "mg_up", "mg_coarse", "mg_setup"]
 
     def profile_enable(self, kernels=None) -> None:
         """kernels: None for all, [] to switch off, or a list of kernel names."""
@@ -322,7 +323,7 @@ class CGIterativeSolver(AbstractIterativeSolver):
     def __init__(self, g: Grid, system=HIPMatrixFreeLinearSystem, chebyshev_degree: int | None = None,
                  chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
                  rtol: float = SQRT_EPS, itmax: int = 0, form: str = "fp64", check_every: int = 8,
-                 inner_rtol: float = 1e-3):
+                 inner_rtol: float = 1e-3, multigrid: bool = False):
         if system is SparseAssembledLinearSystem:
             raise L.ShaktiHIPError("CGIterativeSolver(g, SparseAssembledLinearSystem) is CPU-only; use "
                                    "HIPMatrixFreeLinearSystem on the GPU.")
@@ -337,6 +338,10 @@ class CGIterativeSolver(AbstractIterativeSolver):
                 f"ChebyshevPreconditioner: nsteps_estimate must be >= 2 (got {chebyshev_nsteps_estimate})")
         if form not in FORMS:
             raise L.ShaktiHIPError(f"unknown solver form {form!r}")
+        if multigrid and (form != "fp32_refine" or chebyshev_degree is not None):
+            raise L.ShaktiHIPError("multigrid=True is the FP32 inner preconditioner of form='fp32_refine' "
+                                   "(and replaces chebyshev_degree)")
+        self.multigrid = bool(multigrid)
         self.grid = g
         self.chebyshev_degree = chebyshev_degree
         self.nsteps_estimate = chebyshev_nsteps_estimate
@@ -368,7 +373,8 @@ def _solver_cfg(ls: CGIterativeSolver | None) -> L.ShkSolverCfg:
     if ls is None:
         return L.ShkSolverCfg(0, 4, 15, 0, 0, 8, SQRT_EPS, SQRT_EPS, 1e-3)
     cheb = ls.chebyshev_degree is not None
-    return L.ShkSolverCfg(1 if cheb else 0, ls.chebyshev_degree or 1, ls.nsteps_estimate, FORMS[ls.form], ls.itmax,
+    precond = 2 if ls.multigrid else (1 if cheb else 0)
+    return L.ShkSolverCfg(precond, ls.chebyshev_degree or 1, ls.nsteps_estimate, FORMS[ls.form], ls.itmax,
                           ls.check_every, ls.atol, ls.rtol, ls.inner_rtol)
 
 

@@ -214,3 +214,78 @@ def test_fused_update_chebyshev_matches_unfused(oracle, hip, degree, monkeypatch
     co = oracle.assemble_mfls(os_, g, p, 1)
     x_ref, st_ref, _ = oracle.cg_solve(co, co[5], os_.h, precond="chebyshev", degree=degree, atol=0.0, rtol=1e-12)
     assert rel_err(out[True][0], x_ref) < 1e-10
+
+
+@pytest.mark.parametrize("kfs", [0, 1])
+def test_fp32_refinement_solve_matches_fp64(oracle, hip, kfs):
+    """The FP32-inner / FP64-refinement form reaches the same FP64 residual
+    test as the FP64 form: heads agree to the solve tolerance."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(257, 65, bed_amp=30.0, seed=8)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    kf = [hip.Arithmetic(), hip.Harmonic()][kfs]
+    res = {}
+    for form in ("fp64", "fp32_refine"):
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-11,
+                                   form=form, inner_rtol=1e-4)
+        hip.solve_elliptic_linear_system(ls, hs, hg, hp, kf)
+        res[form] = (hs.get("h"), ls.ws.stats)
+    st = res["fp32_refine"][1]
+    assert st.solved and st.refinements >= 2
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, kfs=kfs)
+    co = oracle.assemble_mfls(os_, g, p, kfs)
+    x_ref = oracle.cholesky_solve(co, co[5]) if g.nx * g.ny < 20000 else None
+    e = rel_err(res["fp32_refine"][0], res["fp64"][0])
+    assert e < 1e-9, e
+    if x_ref is not None:
+        assert rel_err(res["fp32_refine"][0], x_ref) < 1e-9
+
+
+def test_fp32_refinement_picard_trajectory(oracle, hip):
+    """Config-1 style run with the precision form: Picard converges every step
+    and h, b stay within 1e-7 / 1e-6 of the Cholesky oracle."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(64, 32, bed_amp=10.0, seed=1)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    sv = oracle.Solver(g, "cholesky")
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-8,
+                               form="fp32_refine", inner_rtol=1e-3)
+    ps = hip.PicardSolver(500, 1e-6, ls, hg)
+    sim = hip.Simulation(hg, hs, 12, 3600.0, hp, "implicit", [], hip.ConstantMeltInput(),
+                         hip.RegularizedCoulombSlidingLaw(0.5), ps=ps)
+    for t in range(12):
+        oracle.step(sv, os_, g, p, m, 3600.0)
+        hip.step(sim)
+        assert ps.converged
+        assert rel_err(hs.get("h"), os_.h) <= 1e-7, t
+        assert rel_err(hs.get("b"), os_.b) <= 1e-6, t
+
+
+@pytest.mark.parametrize("shape", [(6, 6), (257, 65), (301, 77)])
+def test_multigrid_fp32_refinement_solve(oracle, hip, shape):
+    """FP32 multigrid-preconditioned CG inside FP64 refinement solves to the
+    same FP64 residual test as the FP64 Chebyshev form, in far fewer
+    iterations (grids ragged against the 2x2 coarsening)."""
+    from shakti_hip import synthetic as S
+    nx, ny = shape
+    if shape == (6, 6):
+        g0, p0, m0, raw = oracle.test_fixture_6x6()
+        g_args = (6, 6, g0.lx, g0.ly)
+    else:
+        gg, raw = S.glacier(nx, ny, bed_amp=30.0, seed=9)
+        g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    res = {}
+    for name, kw in [("fp64", dict(chebyshev_degree=4)),
+                     ("mg", dict(form="fp32_refine", multigrid=True, inner_rtol=1e-4))]:
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, rtol=1e-11, **kw)
+        hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+        res[name] = (hs.get("h"), ls.ws.stats)
+    st = res["mg"][1]
+    assert st.solved
+    assert rel_err(res["mg"][0], res["fp64"][0]) < 1e-9
+    if shape != (6, 6):
+        assert st.niter < res["fp64"][1].niter / 3, (st.niter, res["fp64"][1].niter)
