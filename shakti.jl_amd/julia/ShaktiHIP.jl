@@ -71,11 +71,17 @@ end
 
 """
     HIPPCGSolver(g; chebyshev_degree = nothing, chebyshev_nsteps_estimate = 15,
-                 atol = sqrt(eps()), rtol = sqrt(eps()), itmax = 0, check_every = 8)
+                 atol = sqrt(eps()), rtol = sqrt(eps()), itmax = 0, check_every = 8,
+                 form = :fp64, inner_rtol = 1e-3, multigrid = false)
 
 CG on the GPU with Jacobi (`chebyshev_degree === nothing`) or Chebyshev
 preconditioning; same options and errors as `CGIterativeSolver(g,
 MatrixFreeLinearSystem; ...)` (src/linear_solver.jl:832-843).
+
+`form = :fp32_refine` solves in FP32 inside an FP64 iterative-refinement loop
+that stops on the same FP64 residual test; `multigrid = true` (only with
+`:fp32_refine` and no Chebyshev degree) preconditions the FP32 solves with the
+geometric-multigrid V-cycle.
 """
 mutable struct HIPPCGSolver <: Shakti.AbstractIterativeSolver
     ctx::Ptr{Cvoid}
@@ -86,14 +92,19 @@ end
 
 function HIPPCGSolver(g::Shakti.Grid; chebyshev_degree::Union{Nothing, Int} = nothing,
                       chebyshev_nsteps_estimate::Int = 15, amg::Bool = false, atol = sqrt(eps(Float64)),
-                      rtol = sqrt(eps(Float64)), itmax::Int = 0, check_every::Int = 8)
+                      rtol = sqrt(eps(Float64)), itmax::Int = 0, check_every::Int = 8,
+                      form::Symbol = :fp64, inner_rtol = 1e-3, multigrid::Bool = false)
     amg && error("AMGPreconditioner is CPU/SparseMatrixCSC-only; use chebyshev_degree with HIPPCGSolver.")
+    form in (:fp64, :fp32_refine) || error("HIPPCGSolver: unknown form $form")
+    multigrid && (form != :fp32_refine || chebyshev_degree !== nothing) &&
+        error("HIPPCGSolver: multigrid = true needs form = :fp32_refine and no chebyshev_degree")
     chebyshev_degree !== nothing && chebyshev_degree < 1 &&
         error("ChebyshevPreconditioner: degree must be >= 1 (got $chebyshev_degree)")
     chebyshev_nsteps_estimate >= 2 ||
         error("ChebyshevPreconditioner: nsteps_estimate must be >= 2 (got $chebyshev_nsteps_estimate)")
-    cfg = ShkSolverCfg(chebyshev_degree === nothing ? 0 : 1, something(chebyshev_degree, 1),
-                       chebyshev_nsteps_estimate, 0, itmax, check_every, atol, rtol, 1e-3)
+    precond = multigrid ? 2 : (chebyshev_degree === nothing ? 0 : 1)
+    cfg = ShkSolverCfg(precond, something(chebyshev_degree, 1), chebyshev_nsteps_estimate,
+                       form == :fp32_refine ? 1 : 0, itmax, check_every, atol, rtol, inner_rtol)
     sg = Ref(ShkGrid(g.nx, g.ny, g.lx, g.ly, g.dx, g.dy, g.dx2, g.dy2))
     sp = Ref(params_c(Shakti.ModelParameters()))
     sm = Ref(ShkModel(0, 0, 0, 0.5))
