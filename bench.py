@@ -53,9 +53,12 @@ def gpu_run(args, g, raw, moulin, form):
     s = H.State(g, p)
     H.set_initial_conditions(s, g, p, sl, **raw)
     # fp64: the reference's Chebyshev(4)-PCG in FP64; fp32_refine: FP32 Chebyshev(4)-PCG inside FP64
-    # refinement; fp32_mg: FP32 multigrid-PCG inside FP64 refinement
+    # refinement; fp32_mg: FP32 multigrid-PCG inside FP64 refinement; fp64_mg: FP64 CG preconditioned
+    # by the FP32 multigrid V-cycle
     if form == "fp64":
         kw = dict(chebyshev_degree=4, rtol=args.rtol, form="fp64")
+    elif form == "fp64_mg":
+        kw = dict(multigrid=True, rtol=args.mg_rtol, form="fp64")
     elif form == "fp32_refine":
         kw = dict(chebyshev_degree=4, rtol=args.ir_rtol, form="fp32_refine", inner_rtol=args.inner_rtol)
     else:
@@ -158,7 +161,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--rtol", type=float, default=1e-11)
     ap.add_argument("--check-every", type=int, default=32)
-    ap.add_argument("--forms", default="fp64,fp32_mg",
+    ap.add_argument("--forms", default="fp64,fp32_mg,fp64_mg",
                     help="comma-separated solver forms to run from the same initial state")
     ap.add_argument("--ir-rtol", type=float, default=1e-8,
                     help="fp32_refine: FP64 residual reduction per linear solve")
@@ -166,6 +169,8 @@ def main():
                     help="fp32_refine: residual reduction of each FP32 inner solve")
     ap.add_argument("--mg-inner-rtol", type=float, default=1e-4,
                     help="fp32_mg: residual reduction of each FP32 multigrid-PCG inner solve")
+    ap.add_argument("--mg-rtol", type=float, default=1e-10,
+                    help="fp64_mg: reduction of the multigrid-preconditioned residual norm per linear solve")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--max-picard", type=int, default=500,
                     help="Picard iteration cap per step (500 as in the reference configs; lower only for profiling)")

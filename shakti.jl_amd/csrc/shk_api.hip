@@ -206,8 +206,6 @@ int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
     if (cfg->precond != SHK_PRECOND_JACOBI && cfg->precond != SHK_PRECOND_CHEBYSHEV &&
         cfg->precond != SHK_PRECOND_MULTIGRID)
         return fail(ctx, SHK_ERR_ARG, "unknown preconditioner");
-    if (cfg->precond == SHK_PRECOND_MULTIGRID && cfg->form != SHK_FORM_FP32_REFINE)
-        return fail(ctx, SHK_ERR_ARG, "the multigrid preconditioner runs in FP32: use SHK_FORM_FP32_REFINE");
     if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_degree < 1 || cfg->cheb_degree > MAX_CHEB))
         return fail(ctx, SHK_ERR_ARG, "ChebyshevPreconditioner: degree must be >= 1 (and <= 32)");
     if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_nsteps < 2 || cfg->cheb_nsteps > MAX_LANCZOS))
@@ -347,16 +345,24 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 
 // z = V(r): one V-cycle (one damped-Jacobi sweep down and up on every level,
 // dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1 iteration).
-void mg_apply(shk_ctx* ctx, const float* r, float* z, int mode) {
+// With T = double the finest level reads the FP64 residual and writes the FP64
+// z that FP64 CG uses; every level computes in FP32.
+template <class T>
+void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
     auto& mg = ctx->mg;
     const size_t L = mg.size() - 1;
+    const double tb = sizeof(T) - 4.0;  // extra bytes per cell of an FP64 rhs / result
     for (size_t l = 0; l < L; ++l) {
         const auto& f = mg[l];
-        const float* b = l == 0 ? r : f.b;
         // reads b, c, aE, aN, D; writes x and the coarse rhs
-        Prof pr(ctx, P_MGD, 4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny));
-        launch::mg_down(f.nx, f.ny, mg[l + 1].nx, mg[l + 1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, b, f.xa,
-                        mg[l + 1].b, ctx->st);
+        Prof pr(ctx, P_MGD, 4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) +
+                                (l == 0 ? tb * f.nx * f.ny : 0.0));
+        if (l == 0)
+            launch::mg_down<T>(f.nx, f.ny, mg[1].nx, mg[1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, r, f.xa, mg[1].b,
+                               ctx->st);
+        else
+            launch::mg_down<float>(f.nx, f.ny, mg[l + 1].nx, mg[l + 1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, f.b,
+                                   f.xa, mg[l + 1].b, ctx->st);
     }
     {
         const double nc = (double)mg[L].nx * mg[L].ny;
@@ -365,12 +371,15 @@ void mg_apply(shk_ctx* ctx, const float* r, float* z, int mode) {
     }
     for (size_t l = L; l-- > 0;) {
         const auto& f = mg[l];
-        const float* b = l == 0 ? r : f.b;
-        float* out = l == 0 ? z : f.xb;
         // reads b, c, aE, aN, D, x and the coarse correction; writes the result
-        Prof pr(ctx, P_MGU, 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny));
-        launch::mg_up(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, b, f.xa,
-                      mg[l + 1].xb, out, l == 0 ? mode : -1, ctx->partials, ctx->sc, ctx->st);
+        Prof pr(ctx, P_MGU, 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) +
+                                (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
+        if (l == 0)
+            launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, r, f.xa,
+                             mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->st);
+        else
+            launch::mg_up<float>(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, f.b,
+                                 f.xa, mg[l + 1].xb, f.xb, -1, ctx->partials, ctx->sc, ctx->st);
     }
 }
 
@@ -391,7 +400,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     }
     const bool multigrid = precond == SHK_PRECOND_MULTIGRID;
     if (multigrid) {
-        if constexpr (sizeof(T) == 4) mg_apply(ctx, w.r, w.z, 0);
+        mg_apply<T>(ctx, w.r, w.z, 0);
     } else if (!jacobi) {
         cheb_apply<T>(ctx, w, degree, 0);
     }
@@ -424,7 +433,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                                          ctx->st);
                 }
                 if (multigrid) {
-                    if constexpr (sizeof(T) == 4) mg_apply(ctx, w.r, w.z, 1);
+                    mg_apply<T>(ctx, w.r, w.z, 1);
                 } else if (!jacobi) {
                     cheb_apply<T>(ctx, w, degree, 1);
                 }
@@ -563,9 +572,11 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     const double nb = (double)ctx->n * 8;
     const shk_solver_cfg& cfg = ctx->cfg;
     const int itmax = cfg.itmax > 0 ? cfg.itmax : (int)(2 * ctx->n < 2147483647 ? 2 * ctx->n : 2147483647);
+    const bool multigrid = cfg.precond == SHK_PRECOND_MULTIGRID;
     {
-        Prof pr(ctx, P_ASM, 17 * nb);
-        launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st);
+        Prof pr(ctx, P_ASM, (multigrid ? 18 : 17) * nb);
+        launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st,
+                                 multigrid ? ctx->cP : nullptr);
     }
     ctx->have_system = true;
     ctx->system_parabolic = parabolic;
@@ -574,6 +585,21 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
                    ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->r2};
     const int cheb = cfg.precond == SHK_PRECOND_CHEBYSHEV;
     if (cheb) cheb_refresh<double>(ctx, w, cfg.cheb_nsteps, cfg.cheb_degree);
+    if (multigrid) {
+        // FP64 CG preconditioned by the FP32 V-cycle: the hierarchy is built
+        // from the FP32 flux form of this system
+        {
+            Prof pr(ctx, P_IR, 4 * 12.0 * ctx->n);
+            launch::to_f32(ctx->n, ctx->cP, ctx->aP32, ctx->st);
+            launch::to_f32(ctx->n, ctx->aE, ctx->aE32, ctx->st);
+            launch::to_f32(ctx->n, ctx->aN, ctx->aN32, ctx->st);
+            launch::to_f32(ctx->n, ctx->aP, ctx->dg32, ctx->st);
+        }
+        Work<float> wi{ctx->aP32, ctx->aE32, ctx->aN32, ctx->dg32, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        int rc0 = mg_setup(ctx, wi);
+        if (rc0) return rc0;
+    }
     int rc = pcg<double>(ctx, w, ctx->F.f[SHK_H], cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol,
                          itmax);
     if (rc) return rc;

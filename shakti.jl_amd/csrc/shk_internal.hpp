@@ -135,10 +135,12 @@ void cg_finish(int n, const double* x, const double* h_prev, double* h_new, doub
 void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float* ef, const float* nf, float* cc,
                 float* ec, float* nc, hipStream_t st);
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st);
+template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
-             const float* D, const float* b, float* x, float* bc, hipStream_t st);
+             const float* D, const TB* b, float* x, float* bc, hipStream_t st);
+template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
-           const float* D, const float* b, const float* x, const float* xc, float* out, int dot_mode,
+           const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st);
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
                       hipStream_t st);

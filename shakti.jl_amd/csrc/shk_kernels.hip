@@ -1278,15 +1278,19 @@ __device__ __forceinline__ float flux_apply(int nx, int ny, int ix, int iy, size
 // Down-leg of the V-cycle on one level: one damped-Jacobi sweep from zero,
 // x = omega b / D, stored for the up-leg, and the restricted residual
 // bc = P^T (b - A x).  One thread per coarse cell.
+// TB is the type of the level's right-hand side: FP32 inside the hierarchy,
+// FP64 on the finest level when the V-cycle preconditions FP64 CG (the
+// arithmetic stays FP32 either way).
+template <class TB>
 __global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int nyc, float omega,
                                                  const float* __restrict__ c, const float* __restrict__ e,
                                                  const float* __restrict__ n, const float* __restrict__ D,
-                                                 const float* __restrict__ b, float* __restrict__ x,
+                                                 const TB* __restrict__ b, float* __restrict__ x,
                                                  float* __restrict__ bc) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     if (I >= nxc || J >= nyc) return;
-    auto xs = [&](size_t k) -> float { return omega * b[k] / D[k]; };
+    auto xs = [&](size_t k) -> float { return omega * (float)b[k] / D[k]; };
     float acc = 0.f;
     const int x0 = 2 * I, y0 = 2 * J;
     const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
@@ -1295,7 +1299,7 @@ __global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int ny
             const size_t k = (size_t)ix + (size_t)iy * nx;
             const float xv = xs(k);
             x[k] = xv;
-            acc += b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+            acc += (float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
         }
     bc[(size_t)I + (size_t)J * nxc] = acc;
 }
@@ -1303,11 +1307,12 @@ __global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int ny
 // Up-leg on one level: add the (scaled) coarse correction, one damped-Jacobi
 // sweep, write the level's result.  On the finest level also the partial of
 // r'z (r = b, z = the result) for CG.
+template <class TB>
 __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float omega, float scale,
                                                const float* __restrict__ c, const float* __restrict__ e,
                                                const float* __restrict__ n, const float* __restrict__ D,
-                                               const float* __restrict__ b, const float* __restrict__ x,
-                                               const float* __restrict__ xc, float* __restrict__ out, int with_dot,
+                                               const TB* __restrict__ b, const float* __restrict__ x,
+                                               const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
                                                double* partials) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1319,8 +1324,8 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
         };
         const size_t k = (size_t)ix + (size_t)iy * nx;
         const float u = xs(k);
-        const float z = u + omega * (b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs)) / D[k];
-        out[k] = z;
+        const float z = u + omega * ((float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs)) / D[k];
+        out[k] = (TB)z;
         acc = (double)b[k] * (double)z;
     }
     if (with_dot) {
@@ -1679,21 +1684,31 @@ void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_diag, cells_grid(nx, ny), dim3(64, 4), 0, st, nx, ny, c, e, n, D);
 }
+template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
-             const float* D, const float* b, float* x, float* bc, hipStream_t st) {
-    hipLaunchKernelGGL(k_mg_down, cells_grid(nxc, nyc), dim3(64, 4), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D, b,
-                       x, bc);
+             const float* D, const TB* b, float* x, float* bc, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_down<TB>, cells_grid(nxc, nyc), dim3(64, 4), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D,
+                       b, x, bc);
 }
+template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
-           const float* D, const float* b, const float* x, const float* xc, float* out, int dot_mode,
+           const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st) {
     const dim3 grid = cells_grid(nx, ny);
-    hipLaunchKernelGGL(k_mg_up, grid, dim3(64, 4), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc, out,
+    hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(64, 4), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc, out,
                        dot_mode >= 0 ? 1 : 0, partials);
     if (dot_mode >= 0)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
 }
+template void mg_down<float>(int, int, int, int, float, const float*, const float*, const float*, const float*,
+                             const float*, float*, float*, hipStream_t);
+template void mg_down<double>(int, int, int, int, float, const float*, const float*, const float*, const float*,
+                              const double*, float*, float*, hipStream_t);
+template void mg_up<float>(int, int, int, float, float, const float*, const float*, const float*, const float*,
+                           const float*, const float*, const float*, float*, int, double*, Scalars*, hipStream_t);
+template void mg_up<double>(int, int, int, float, float, const float*, const float*, const float*, const float*,
+                            const double*, const float*, const float*, double*, int, double*, Scalars*, hipStream_t);
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
                       hipStream_t st) {
     hipLaunchKernelGGL(k_mg_dense_inverse, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, work, inv);

@@ -318,7 +318,12 @@ class CGIterativeSolver(AbstractIterativeSolver):
     chebyshev_nsteps_estimate) (src/linear_solver.jl:832-843) on the GPU.
     ``atol``/``rtol``/``itmax`` are Krylov's cg! stopping controls (defaults
     sqrt(eps), sqrt(eps), 2n); ``check_every`` is how many CG iterations are
-    queued between host looks at the device-side convergence flag."""
+    queued between host looks at the device-side convergence flag.
+
+    ``multigrid=True`` preconditions with the FP32 geometric-multigrid
+    V-cycle: with ``form="fp64"`` it preconditions FP64 CG (residuals, dots
+    and updates stay FP64); with ``form="fp32_refine"`` it preconditions the
+    FP32 inner solves of the FP64 refinement loop."""
 
     def __init__(self, g: Grid, system=HIPMatrixFreeLinearSystem, chebyshev_degree: int | None = None,
                  chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
@@ -338,9 +343,8 @@ class CGIterativeSolver(AbstractIterativeSolver):
                 f"ChebyshevPreconditioner: nsteps_estimate must be >= 2 (got {chebyshev_nsteps_estimate})")
         if form not in FORMS:
             raise L.ShaktiHIPError(f"unknown solver form {form!r}")
-        if multigrid and (form != "fp32_refine" or chebyshev_degree is not None):
-            raise L.ShaktiHIPError("multigrid=True is the FP32 inner preconditioner of form='fp32_refine' "
-                                   "(and replaces chebyshev_degree)")
+        if multigrid and chebyshev_degree is not None:
+            raise L.ShaktiHIPError("multigrid=True replaces chebyshev_degree: give one of them")
         self.multigrid = bool(multigrid)
         self.grid = g
         self.chebyshev_degree = chebyshev_degree

@@ -266,9 +266,10 @@ def test_fp32_refinement_picard_trajectory(oracle, hip):
 
 @pytest.mark.parametrize("shape", [(6, 6), (257, 65), (301, 77)])
 def test_multigrid_fp32_refinement_solve(oracle, hip, shape):
-    """FP32 multigrid-preconditioned CG inside FP64 refinement solves to the
-    same FP64 residual test as the FP64 Chebyshev form, in far fewer
-    iterations (grids ragged against the 2x2 coarsening)."""
+    """FP32 multigrid-preconditioned CG inside FP64 refinement, and FP64 CG
+    preconditioned by the FP32 V-cycle, solve to within 1e-9 of the FP64
+    Chebyshev form in far fewer iterations (grids ragged against the 2x2
+    coarsening)."""
     from shakti_hip import synthetic as S
     nx, ny = shape
     if shape == (6, 6):
@@ -279,13 +280,15 @@ def test_multigrid_fp32_refinement_solve(oracle, hip, shape):
         g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
     res = {}
     for name, kw in [("fp64", dict(chebyshev_degree=4)),
-                     ("mg", dict(form="fp32_refine", multigrid=True, inner_rtol=1e-4))]:
+                     ("mg", dict(form="fp32_refine", multigrid=True, inner_rtol=1e-4)),
+                     ("fp64_mg", dict(form="fp64", multigrid=True))]:
         hg, hp, hs = hip_state(hip, g_args, raw)
         ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, rtol=1e-11, **kw)
         hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
         res[name] = (hs.get("h"), ls.ws.stats)
-    st = res["mg"][1]
-    assert st.solved
-    assert rel_err(res["mg"][0], res["fp64"][0]) < 1e-9
-    if shape != (6, 6):
-        assert st.niter < res["fp64"][1].niter / 3, (st.niter, res["fp64"][1].niter)
+    for name in ("mg", "fp64_mg"):
+        st = res[name][1]
+        assert st.solved, name
+        assert rel_err(res[name][0], res["fp64"][0]) < 1e-9, name
+        if shape != (6, 6):
+            assert st.niter < res["fp64"][1].niter / 3, (name, st.niter, res["fp64"][1].niter)
