@@ -66,11 +66,13 @@ typedef struct {
 
 #define SHK_PRECOND_JACOBI 0    /* CGIterativeSolver(g, MatrixFreeLinearSystem) */
 #define SHK_PRECOND_CHEBYSHEV 1 /* ... ; chebyshev_degree = k */
-#define SHK_PRECOND_MULTIGRID 2 /* geometric multigrid V-cycle (FP32; only with SHK_FORM_FP32_REFINE);
-                                   no reference counterpart (the reference's AMG is CPU-only) */
+#define SHK_PRECOND_MULTIGRID 2 /* geometric multigrid V-cycle in FP32: with SHK_FORM_FP64 it preconditions
+                                   FP64 CG, with SHK_FORM_FP32_REFINE the FP32 inner solves; no reference
+                                   counterpart (the reference's AMG is CPU-only) */
 
 #define SHK_FORM_FP64 0         /* plain FP64 PCG */
 #define SHK_FORM_FP32_REFINE 1  /* FP32 inner PCG inside FP64 iterative refinement */
+#define SHK_FORM_LEARNED 2      /* FP64 PCG warm-started by the attached U-Net (shakti_learned.h) */
 
 /* Linear-solver configuration -- reference CGIterativeSolver(g,
  * MatrixFreeLinearSystem; chebyshev_degree, chebyshev_nsteps_estimate)
@@ -93,7 +95,7 @@ typedef struct {
     int32_t zero_curvature; /* 1 when p'Ap vanished (Krylov's zero-curvature exit) */
     int32_t bounds_warning; /* 1 when the Chebyshev bound estimate was rejected (previous bounds kept) */
     int32_t refinements;    /* FP32_REFINE outer iterations */
-    int32_t pad;
+    int32_t fallback;       /* LEARNED: 1 when the network's warm start was rejected (CG started from h) */
     double rnorm0, rnorm;   /* preconditioned residual norm at start / end */
     double lambda_min, lambda_max; /* Chebyshev bounds in use */
 } shk_solve_stats;

@@ -16,9 +16,19 @@
 #include <string>
 #include <vector>
 
+#include "shakti_learned.h"
 #include "shk_internal.hpp"
 
 using namespace shk;
+
+// U-Net entry points used by the solver (shk_nn.hip)
+namespace shknn {
+int forward(shk_unet* net, const void* x, int B, int H, int W);
+float* output(shk_unet* net);
+void set_stream(shk_unet* net, hipStream_t st);
+bool fits(const shk_unet* net, int H, int W);
+const char* unet_error(const shk_unet* net);
+}  // namespace shknn
 
 namespace {
 
@@ -97,6 +107,16 @@ struct shk_ctx {
     bool use_graphs = true;
     bool use_fused = true;
 
+    // learned warm start and training-pair recording: the network input is
+    // the grid padded with zero cells to W8 x H8 (multiples of 64 and 32)
+    shk_unet* net = nullptr;
+    int W8 = 0, H8 = 0;
+    void* nn_x = nullptr;  // BF16 network input [H8][W8][8]
+    double *nn_e = nullptr, *nn_Ae = nullptr, *hw = nullptr;
+    void* rec_x = nullptr;
+    float* rec_t = nullptr;
+    int rec_cap = 0, rec_count = 0;
+
     std::string err;
 };
 
@@ -105,9 +125,10 @@ namespace {
 const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_init",     "cheb_pass",
                             "cg_matvec",  "cg_update",    "cg_finish",   "derived",     "gap",
                             "stagger_b",  "ieb",          "ic",          "matvec",      "ir",
-                            "cg_update_cheb", "mg_down",  "mg_up",       "mg_coarse",   "mg_setup"};
+                            "cg_update_cheb", "mg_down",  "mg_up",       "mg_coarse",   "mg_setup",
+                            "nn_forward", "nn_warm_start"};
 enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR,
-              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NUM };
+              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NNF, P_NNW, P_NUM };
 
 int fail(shk_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -210,7 +231,7 @@ int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
         return fail(ctx, SHK_ERR_ARG, "ChebyshevPreconditioner: degree must be >= 1 (and <= 32)");
     if (cfg->precond == SHK_PRECOND_CHEBYSHEV && (cfg->cheb_nsteps < 2 || cfg->cheb_nsteps > MAX_LANCZOS))
         return fail(ctx, SHK_ERR_ARG, "ChebyshevPreconditioner: nsteps_estimate must be >= 2 (and <= 64)");
-    if (cfg->form != SHK_FORM_FP64 && cfg->form != SHK_FORM_FP32_REFINE)
+    if (cfg->form != SHK_FORM_FP64 && cfg->form != SHK_FORM_FP32_REFINE && cfg->form != SHK_FORM_LEARNED)
         return fail(ctx, SHK_ERR_ARG, "unknown solver form");
     if (cfg->check_every < 1) return fail(ctx, SHK_ERR_ARG, "check_every must be >= 1");
     if (!(cfg->atol >= 0) || !(cfg->rtol >= 0)) return fail(ctx, SHK_ERR_ARG, "atol/rtol must be >= 0");
@@ -567,16 +588,28 @@ int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_sol
 }
 
 int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats* stats) {
+    {
+        // the stopping test refers to the CG start's own residual unless the
+        // learned warm start below sets the reference to r(h)
+        ctx->sc_host->rnorm0_ref = 0.0;
+        HIPCHK(hipMemcpyAsync(&ctx->sc->rnorm0_ref, &ctx->sc_host->rnorm0_ref, sizeof(double),
+                              hipMemcpyHostToDevice, ctx->st));
+    }
     if (ctx->cfg.form == SHK_FORM_FP32_REFINE) return solve_ir(ctx, parabolic, dt, relax, stats);
     const Geom& G = ctx->G;
     const double nb = (double)ctx->n * 8;
     const shk_solver_cfg& cfg = ctx->cfg;
     const int itmax = cfg.itmax > 0 ? cfg.itmax : (int)(2 * ctx->n < 2147483647 ? 2 * ctx->n : 2147483647);
     const bool multigrid = cfg.precond == SHK_PRECOND_MULTIGRID;
+    const bool learned = cfg.form == SHK_FORM_LEARNED;
+    if (learned && !ctx->net)
+        return fail(ctx, SHK_ERR_STATE, "form SHK_FORM_LEARNED needs a network: call shk_attach_unet first");
+    const bool record = ctx->rec_x && ctx->rec_count < ctx->rec_cap;
+    const bool need_c = multigrid || learned || record;
     {
-        Prof pr(ctx, P_ASM, (multigrid ? 18 : 17) * nb);
+        Prof pr(ctx, P_ASM, (need_c ? 18 : 17) * nb);
         launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st,
-                                 multigrid ? ctx->cP : nullptr);
+                                 need_c ? ctx->cP : nullptr);
     }
     ctx->have_system = true;
     ctx->system_parabolic = parabolic;
@@ -600,20 +633,68 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
         int rc0 = mg_setup(ctx, wi);
         if (rc0) return rc0;
     }
-    int rc = pcg<double>(ctx, w, ctx->F.f[SHK_H], cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol,
-                         itmax);
+    const double* x0 = ctx->F.f[SHK_H];
+    const size_t slot = (size_t)ctx->rec_count * ctx->W8 * ctx->H8;
+    if (learned || record) {
+        int rc0 = reset_scalars(ctx, cfg.atol, cfg.rtol, itmax);
+        if (rc0) return rc0;
+        void* xin = record ? (void*)((char*)ctx->rec_x + slot * 16) : ctx->nn_x;
+        {
+            // r(h), its scale, and the network input
+            Prof pr(ctx, P_NNW, 0);
+            launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, x0, w.Ap, ctx->st);
+            launch::nn_resid_scale(ctx->n, ctx->aP, ctx->rhs, w.Ap, w.r, ctx->partials, ctx->sc, ctx->st);
+            launch::nn_features(ctx->g.nx, ctx->g.ny, ctx->W8, ctx->H8, ctx->aP, ctx->aE, ctx->aN, ctx->cP, w.r,
+                                ctx->sc, xin, ctx->st);
+        }
+        if (learned) {
+            // the preconditioned norm of r(h): the stopping test's reference,
+            // so a better start means fewer iterations, not a tighter target
+            if (multigrid) mg_apply<double>(ctx, w.r, w.z, 0);
+            else if (cheb) cheb_apply<double>(ctx, w, cfg.cheb_degree, 0);
+            launch::nn_ref(ctx->sc, (multigrid || cheb) ? 0 : 1, (double)ctx->n, ctx->st);
+        }
+        if (learned) {
+            {
+                Prof pr(ctx, P_NNF, 0);
+                shknn::set_stream(ctx->net, ctx->st);
+                if (shknn::forward(ctx->net, xin, 1, ctx->H8, ctx->W8))
+                    return fail(ctx, SHK_ERR_STATE, std::string("network: ") + shknn::unet_error(ctx->net));
+            }
+            Prof pr(ctx, P_NNW, 0);
+            launch::nn_correction(ctx->g.nx, ctx->g.ny, ctx->W8, shknn::output(ctx->net), ctx->aP, ctx->aE, ctx->aN,
+                                  ctx->sc, ctx->nn_e, ctx->st);
+            launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, ctx->nn_e, ctx->nn_Ae, ctx->st);
+            launch::nn_line_search(ctx->n, ctx->nn_e, w.r, ctx->nn_Ae, ctx->partials, ctx->sc, ctx->st);
+            launch::nn_warm(ctx->n, x0, ctx->nn_e, ctx->sc, ctx->hw, ctx->st);
+            x0 = ctx->hw;
+        }
+    }
+    int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol, itmax);
     if (rc) return rc;
-    // h_new = x + h_prev into the alternate buffer, then swap so F.f[SHK_H] is current
+    // h_new = x + x0 into the alternate buffer, then swap so F.f[SHK_H] is current
     double* h_prev = ctx->F.f[SHK_H];
     {
         Prof pr(ctx, P_FIN, 3 * nb);
-        launch::cg_finish((int)ctx->n, ctx->x, h_prev, ctx->h_alt, relax, ctx->st);
+        if (x0 == h_prev) {
+            launch::cg_finish((int)ctx->n, ctx->x, h_prev, ctx->h_alt, relax, ctx->st);
+        } else {
+            launch::cg_finish((int)ctx->n, ctx->x, x0, ctx->h_alt, 0.0, ctx->st);
+            if (relax > 0.0) launch::relax(ctx->n, ctx->h_alt, h_prev, relax, ctx->st);
+        }
     }
     ctx->F.f[SHK_H] = ctx->h_alt;
     ctx->h_alt = h_prev;
+    if (record) {
+        launch::nn_target(ctx->g.nx, ctx->g.ny, ctx->W8, ctx->H8, ctx->aP, ctx->F.f[SHK_H], h_prev, ctx->sc,
+                          ctx->rec_t + slot, ctx->st);
+        ctx->rec_count += 1;
+        HIPCHK(hipStreamSynchronize(ctx->st));
+    }
     const Scalars* h = ctx->sc_host;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
+        stats->fallback = learned ? h->nn_fallback : 0;
         stats->niter = h->iter;
         stats->solved = h->solved;
         stats->zero_curvature = h->zero_curv;
@@ -688,6 +769,8 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     ctx->nyf = nx * (ny + 1);
     fill_phys(ctx);
     ctx->G = make_geom((int)nx, (int)ny);
+    ctx->W8 = (int)((nx + 63) / 64 * 64);
+    ctx->H8 = (int)((ny + 31) / 32 * 32);
     for (int f = 0; f < SHK_NUM_FIELDS; ++f)
         if ((rc = dalloc(ctx, &ctx->F.f[f], field_count(ctx, f)))) return rc;
     if ((rc = dalloc(ctx, &ctx->h_alt, ctx->n))) return rc;
@@ -749,6 +832,10 @@ int shk_destroy(shk_ctx* ctx) {
     }
     if (ctx->mg_inv) hipFree(ctx->mg_inv);
     if (ctx->mg_work) hipFree(ctx->mg_work);
+    if (ctx->net) shknn::set_stream(ctx->net, nullptr);
+    void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw};
+    for (void* p : nnp)
+        if (p) hipFree(p);
     for (auto& pe : ctx->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
     if (ctx->st) hipStreamDestroy(ctx->st);
@@ -1063,6 +1150,60 @@ int shk_profile_reset(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     prof_collect(ctx);
     for (auto& e : ctx->prof_tab) { e.count = 0; e.ms = 0; e.bytes = 0; }
+    return SHK_OK;
+}
+
+// ---- learned warm start (shakti_learned.h) ----
+
+static int nn_buffers(shk_ctx* ctx) {
+    if (!ctx->nn_e) {
+        HIPCHK(hipMalloc((void**)&ctx->nn_e, ctx->n * 8 + 256));
+        HIPCHK(hipMalloc((void**)&ctx->nn_Ae, ctx->n * 8 + 256));
+        HIPCHK(hipMalloc((void**)&ctx->hw, ctx->n * 8 + 256));
+        HIPCHK(hipMalloc(&ctx->nn_x, (size_t)ctx->W8 * ctx->H8 * 16 + 256));
+    }
+    return SHK_OK;
+}
+
+int shk_attach_unet(shk_ctx* ctx, shk_unet* net) {
+    if (!ctx) return SHK_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->net) shknn::set_stream(ctx->net, nullptr);
+    ctx->net = nullptr;
+    if (!net) return SHK_OK;
+    if (!shknn::fits(net, ctx->H8, ctx->W8))
+        return fail(ctx, SHK_ERR_ARG,
+                    "network capacity too small for one image of this grid (padded to " + std::to_string(ctx->H8) +
+                        " x " + std::to_string(ctx->W8) + ")");
+    int rc = nn_buffers(ctx);
+    if (rc) return rc;
+    ctx->net = net;
+    shknn::set_stream(net, ctx->st);
+    return SHK_OK;
+}
+
+int shk_record_pairs(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity) {
+    if (!ctx) return SHK_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (!x_dev) {
+        ctx->rec_x = nullptr;
+        ctx->rec_t = nullptr;
+        ctx->rec_cap = ctx->rec_count = 0;
+        return SHK_OK;
+    }
+    if (!t_dev || capacity < 1) return fail(ctx, SHK_ERR_ARG, "record buffers need a target buffer and capacity >= 1");
+    int rc = nn_buffers(ctx);
+    if (rc) return rc;
+    ctx->rec_x = x_dev;
+    ctx->rec_t = t_dev;
+    ctx->rec_cap = capacity;
+    ctx->rec_count = 0;
+    return SHK_OK;
+}
+
+int shk_recorded_pairs(shk_ctx* ctx, int32_t* recorded) {
+    if (!ctx || !recorded) return SHK_ERR_ARG;
+    *recorded = ctx->rec_count;
     return SHK_OK;
 }
 

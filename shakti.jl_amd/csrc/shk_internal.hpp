@@ -50,6 +50,11 @@ struct Scalars {
     // mixed-precision outer loop (FP64 residual in the Jacobi-weighted norm)
     double ir_rnorm, ir_rnorm0, ir_atol, ir_rtol;
     int32_t ir_done, ir_pad;
+    // learned warm start: image scale, energy-minimising step, and the
+    // preconditioned residual norm at h that the stopping test refers to
+    // (0: use the CG start's own norm, as Krylov does)
+    double nn_scale, nn_gamma, rnorm0_ref;
+    int32_t nn_fallback, nn_pad;
     // block-completion counter of the running reduction kernel
     uint32_t counter;
     uint32_t pad;
@@ -150,6 +155,19 @@ void mg_coarse(int n, const float* inv, const float* b, float* x, hipStream_t st
 // r = b - A x with its Jacobi-weighted norm (first != 0 records the initial
 // norm) and its FP32 copy; the FP64 correction x += d.
 void to_f32(size_t n, const double* a, float* b, hipStream_t st);
+// learned warm start (network input / output on the image padded to W8 x H8)
+void nn_resid_scale(size_t n, const double* aP, const double* rhs, const double* Ap, double* r, double* partials,
+                    Scalars* sc, hipStream_t st);
+void nn_features(int nx, int ny, int W8, int H8, const double* aP, const double* aE, const double* aN,
+                 const double* cP, const double* r, const Scalars* sc, void* x, hipStream_t st);
+void nn_correction(int nx, int ny, int W8, const float* N, const double* aP, const double* aE, const double* aN,
+                   const Scalars* sc, double* e, hipStream_t st);
+void nn_line_search(size_t n, const double* e, const double* r, const double* Ae, double* partials, Scalars* sc,
+                    hipStream_t st);
+void nn_warm(size_t n, const double* h, const double* e, const Scalars* sc, double* x0, hipStream_t st);
+void nn_target(int nx, int ny, int W8, int H8, const double* aP, const double* h_new, const double* h_prev,
+               const Scalars* sc, float* t, hipStream_t st);
+void nn_ref(Scalars* sc, int mode, double n, hipStream_t st);
 void ir_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
                  const double* x, float* r32, int first, double* partials, Scalars* sc, hipStream_t st);
 void ir_correct(size_t n, double* x, const float* d, hipStream_t st);

@@ -792,15 +792,16 @@ __global__ __launch_bounds__(64) void k_cheb_bounds(int nsteps, int degree, int 
 __device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
     if (mode == 0) {
         const double rn = sqrt(g);
+        const double ref = sc->rnorm0_ref > 0.0 ? sc->rnorm0_ref : rn;
         sc->gamma = g;
-        sc->rNorm0 = rn;
+        sc->rNorm0 = ref;
         sc->rNorm = rn;
         sc->pNorm2 = g;
         sc->beta = 0.0;
         sc->iter = 0;
         sc->zero_curv = 0;
         sc->tired = 0;
-        sc->eps_stop = sc->atol + sc->rtol * rn;
+        sc->eps_stop = sc->atol + sc->rtol * ref;
         sc->pAp = 0.0;
         sc->solved = (g == 0.0) || (rn <= sc->eps_stop);
         sc->done = sc->solved;
@@ -1404,6 +1405,116 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* __restric
     }
 }
 
+// ------------------------------------------------ learned warm start (FP64)
+// The network sees the Picard iteration's system Jacobi-scaled: with
+// D = diag(aP), r^ = D^-1/2 r / s and A^ = D^-1/2 A D^-1/2 (unit diagonal,
+// couplings aF / sqrt(aP aP_nb)), the correction e solves A^ (D^1/2 e / s) = r^.
+// The image is the grid padded with zero cells to W8 x H8.
+
+typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
+
+// r = rhs - Ap and the partials of sum r^2 / aP; grid-stride over n cells.
+__global__ __launch_bounds__(256) void k_nn_resid(size_t n, const double* __restrict__ aP,
+                                                  const double* __restrict__ rhs, const double* __restrict__ Ap,
+                                                  double* __restrict__ r, double* partials) {
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * WAVE * WAVES + tid; i < n; i += stride) {
+        const double v = rhs[i] - Ap[i];
+        r[i] = v;
+        acc += v * v / aP[i];
+    }
+    double a[1] = {acc};
+    block_partial<1, false>(a, partials);
+}
+
+// Network input (8 BF16 channels per pixel): r^, 2x the four Jacobi-scaled
+// couplings (E, W, N, S), c/aP, 0, 0; zero outside the grid.
+__global__ __launch_bounds__(256) void k_nn_features(int nx, int ny, int W8, int H8, const double* __restrict__ aP,
+                                                     const double* __restrict__ aE, const double* __restrict__ aN,
+                                                     const double* __restrict__ cP, const double* __restrict__ r,
+                                                     const Scalars* __restrict__ sc, nn_bf16x8* __restrict__ x) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)W8 * H8) return;
+    const int ix = (int)(p % W8), iy = (int)(p / W8);
+    nn_bf16x8 o = {};
+    if (ix < nx && iy < ny) {
+        const size_t k = (size_t)ix + (size_t)iy * nx;
+        const double d = aP[k], sd = sqrt(d);
+        const double s = sc->nn_scale;
+        o[0] = (__bf16)(s > 0.0 ? r[k] / (sd * s) : 0.0);
+        o[1] = (__bf16)(ix < nx - 1 ? 2.0 * aE[k] / sqrt(d * aP[k + 1]) : 0.0);
+        o[2] = (__bf16)(ix > 0 ? 2.0 * aE[k - 1] / sqrt(d * aP[k - 1]) : 0.0);
+        o[3] = (__bf16)(iy < ny - 1 ? 2.0 * aN[k] / sqrt(d * aP[k + nx]) : 0.0);
+        o[4] = (__bf16)(iy > 0 ? 2.0 * aN[k - nx] / sqrt(d * aP[k - nx]) : 0.0);
+        o[5] = (__bf16)(cP[k] / d);
+    }
+    x[p] = o;
+}
+
+// e = s N / sqrt(aP) on cells with at least one coupling (0 on identity and
+// isolated rows, whose solution the warm start must not move).
+__global__ __launch_bounds__(256) void k_nn_correction(int nx, int ny, int W8, const float* __restrict__ N,
+                                                       const double* __restrict__ aP, const double* __restrict__ aE,
+                                                       const double* __restrict__ aN, const Scalars* __restrict__ sc,
+                                                       double* __restrict__ e) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (size_t)nx * ny) return;
+    const int ix = (int)(k % nx), iy = (int)(k / nx);
+    const bool coupled =
+        aE[k] != 0.0 || aN[k] != 0.0 || (ix > 0 && aE[k - 1] != 0.0) || (iy > 0 && aN[k - nx] != 0.0);
+    e[k] = coupled ? sc->nn_scale * (double)N[(size_t)ix + (size_t)iy * W8] / sqrt(aP[k]) : 0.0;
+}
+
+// Partials of e'r and e'Ae for the energy-minimising step.
+__global__ __launch_bounds__(256) void k_nn_ls(size_t n, const double* __restrict__ e, const double* __restrict__ r,
+                                               const double* __restrict__ Ae, double* partials) {
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    double a0 = 0.0, a1 = 0.0;
+    for (size_t i = (size_t)blockIdx.x * WAVE * WAVES + tid; i < n; i += stride) {
+        a0 += e[i] * r[i];
+        a1 += e[i] * Ae[i];
+    }
+    double a[2] = {a0, a1};
+    block_partial<2, false>(a, partials);
+}
+
+// x0 = h + gamma e (x0 = h exactly when the step was rejected).
+__global__ __launch_bounds__(256) void k_nn_warm(size_t n, const double* __restrict__ h,
+                                                 const double* __restrict__ e, const Scalars* __restrict__ sc,
+                                                 double* __restrict__ x0) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double g = sc->nn_gamma;
+    x0[i] = g != 0.0 ? h[i] + g * e[i] : h[i];
+}
+
+// Training target sqrt(aP) (h_new - h_prev) / s on the padded image.
+__global__ __launch_bounds__(256) void k_nn_target(int nx, int ny, int W8, int H8, const double* __restrict__ aP,
+                                                   const double* __restrict__ h_new,
+                                                   const double* __restrict__ h_prev,
+                                                   const Scalars* __restrict__ sc, float* __restrict__ t) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)W8 * H8) return;
+    const int ix = (int)(p % W8), iy = (int)(p / W8);
+    float v = 0.f;
+    const double s = sc->nn_scale;
+    if (ix < nx && iy < ny && s > 0.0) {
+        const size_t k = (size_t)ix + (size_t)iy * nx;
+        v = (float)(sqrt(aP[k]) * (h_new[k] - h_prev[k]) / s);
+    }
+    t[p] = v;
+}
+
+// Reference norm of the stopping test: sqrt(r'z) of the preconditioner just
+// applied to r(h) (mode 0), or the Jacobi norm s sqrt(n) (mode 1).
+__global__ void k_nn_ref(Scalars* sc, int mode, double n) {
+    if (threadIdx.x != 0) return;
+    sc->rnorm0_ref = mode == 0 ? sqrt(sc->gamma) : sc->nn_scale * sqrt(n);
+}
+
 // ------------------------------------------------------- finishing kernel
 
 enum FinMode {
@@ -1416,6 +1527,8 @@ enum FinMode {
     FIN_PICARD = 6,      // Picard: max|h - h_prev|, max|h| -> converged flag
     FIN_IR_FIRST = 7,    // refinement: initial FP64 residual norm
     FIN_IR = 8,          // refinement: FP64 residual norm -> done flag
+    FIN_NN_SCALE = 9,    // learned: image scale s = sqrt(sum r^2/aP / k)
+    FIN_NN_LS = 10,      // learned: gamma = e'r / e'Ae (0 when not a descent step)
 };
 
 // One workgroup: sums (or maximises) the nb partials of the preceding
@@ -1426,7 +1539,7 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
                                                 Scalars* sc) {
     if ((mode == FIN_GAMMA_ITER || mode == FIN_PAP) && sc->done) return;
     const bool is_max = mode == FIN_PICARD;
-    const int nv = is_max ? 2 : 1;
+    const int nv = (is_max || mode == FIN_NN_LS) ? 2 : 1;
     const int tid = threadIdx.x + threadIdx.y * WAVE;
     double acc[2] = {0.0, 0.0};
     for (int i = 0; i < nv; ++i) {
@@ -1440,6 +1553,7 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
         acc[i] = a;
     }
     if (is_max) block_reduce<2, true>(acc);
+    else if (nv == 2) block_reduce<2, false>(acc);
     else {
         double one[1] = {acc[0]};
         block_reduce<1, false>(one);
@@ -1477,6 +1591,14 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
             sc->ir_rnorm = rn;
             if (mode == FIN_IR_FIRST) sc->ir_rnorm0 = rn;
             sc->ir_done = (rn <= sc->ir_atol + sc->ir_rtol * sc->ir_rnorm0) || (rn + 1.0 <= 1.0);
+            break;
+        }
+        case FIN_NN_SCALE: sc->nn_scale = sqrt(v0 / (double)k); break;
+        case FIN_NN_LS: {
+            const double g = v0 / v1;
+            const bool ok = v1 > 0.0 && v0 > 0.0 && isfinite(g);
+            sc->nn_gamma = ok ? g : 0.0;
+            sc->nn_fallback = ok ? 0 : 1;
             break;
         }
         case FIN_PICARD:
@@ -1715,6 +1837,47 @@ void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const floa
 }
 void mg_coarse(int n, const float* inv, const float* b, float* x, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(64, 4), 0, st, n, inv, b, x);
+}
+
+static inline unsigned nn_blocks(size_t n) {
+    size_t b = (n + WAVE * WAVES - 1) / (WAVE * WAVES);
+    return (unsigned)(b > 1024 ? 1024 : (b < 1 ? 1 : b));
+}
+void nn_resid_scale(size_t n, const double* aP, const double* rhs, const double* Ap, double* r, double* partials,
+                    Scalars* sc, hipStream_t st) {
+    const unsigned nb = nn_blocks(n);
+    hipLaunchKernelGGL(k_nn_resid, dim3(nb), dim3(WAVE, WAVES), 0, st, n, aP, rhs, Ap, r, partials);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_NN_SCALE, (int)n, sc);
+}
+void nn_features(int nx, int ny, int W8, int H8, const double* aP, const double* aE, const double* aN,
+                 const double* cP, const double* r, const Scalars* sc, void* x, hipStream_t st) {
+    const size_t np = (size_t)W8 * H8;
+    hipLaunchKernelGGL(k_nn_features, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, nx, ny, W8, H8, aP, aE,
+                       aN, cP, r, sc, (nn_bf16x8*)x);
+}
+void nn_correction(int nx, int ny, int W8, const float* N, const double* aP, const double* aE, const double* aN,
+                   const Scalars* sc, double* e, hipStream_t st) {
+    const size_t n = (size_t)nx * ny;
+    hipLaunchKernelGGL(k_nn_correction, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nx, ny, W8, N, aP, aE,
+                       aN, sc, e);
+}
+void nn_line_search(size_t n, const double* e, const double* r, const double* Ae, double* partials, Scalars* sc,
+                    hipStream_t st) {
+    const unsigned nb = nn_blocks(n);
+    hipLaunchKernelGGL(k_nn_ls, dim3(nb), dim3(WAVE, WAVES), 0, st, n, e, r, Ae, partials);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_NN_LS, 0, sc);
+}
+void nn_warm(size_t n, const double* h, const double* e, const Scalars* sc, double* x0, hipStream_t st) {
+    hipLaunchKernelGGL(k_nn_warm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, h, e, sc, x0);
+}
+void nn_target(int nx, int ny, int W8, int H8, const double* aP, const double* h_new, const double* h_prev,
+               const Scalars* sc, float* t, hipStream_t st) {
+    const size_t np = (size_t)W8 * H8;
+    hipLaunchKernelGGL(k_nn_target, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, nx, ny, W8, H8, aP, h_new,
+                       h_prev, sc, t);
+}
+void nn_ref(Scalars* sc, int mode, double n, hipStream_t st) {
+    hipLaunchKernelGGL(k_nn_ref, dim3(1), dim3(64), 0, st, sc, mode, n);
 }
 
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st) {

@@ -44,7 +44,7 @@ struct ShkSolverCfg
     atol::Float64; rtol::Float64; inner_rtol::Float64
 end
 mutable struct ShkSolveStats
-    niter::Int32; solved::Int32; zero_curvature::Int32; bounds_warning::Int32; refinements::Int32; pad::Int32
+    niter::Int32; solved::Int32; zero_curvature::Int32; bounds_warning::Int32; refinements::Int32; fallback::Int32
     rnorm0::Float64; rnorm::Float64; lambda_min::Float64; lambda_max::Float64
     ShkSolveStats() = new(0, 0, 0, 0, 0, 0, 0.0, 0.0, 1.0, 1.0)
 end

@@ -28,6 +28,12 @@ EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "sh
            "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
            "shk_apply_operator", "shk_download_system", "shk_time_kernel", "shk_profile_enable", "shk_profile_read",
            "shk_profile_reset"]
+# ... and of include/shakti_learned.h
+EXPORTS_LEARNED = ["shk_unet_create", "shk_unet_destroy", "shk_unet_last_error", "shk_unet_num_params",
+                   "shk_unet_layer_info", "shk_unet_init_params", "shk_unet_set_params", "shk_unet_get_params",
+                   "shk_unet_get_grads", "shk_unet_forward", "shk_unet_loss", "shk_unet_backward", "shk_unet_adam",
+                   "shk_unet_train_step", "shk_unet_download_tensor", "shk_dev_alloc", "shk_dev_free",
+                   "shk_dev_copy", "shk_attach_unet", "shk_record_pairs", "shk_recorded_pairs"]
 
 
 class ShkGrid(C.Structure):
@@ -53,12 +59,30 @@ class ShkSolverCfg(C.Structure):
 
 class ShkSolveStats(C.Structure):
     _fields_ = [("niter", C.c_int32), ("solved", C.c_int32), ("zero_curvature", C.c_int32),
-                ("bounds_warning", C.c_int32), ("refinements", C.c_int32), ("pad", C.c_int32),
+                ("bounds_warning", C.c_int32), ("refinements", C.c_int32), ("fallback", C.c_int32),
                 ("rnorm0", C.c_double), ("rnorm", C.c_double), ("lambda_min", C.c_double),
                 ("lambda_max", C.c_double)]
 
     def as_dict(self):
-        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class ShkUnetCfg(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("max_h", C.c_int32), ("max_w", C.c_int32), ("leaky_slope", C.c_float)]
+
+
+# enum shk_unet_tensor of include/shakti_learned.h: (name, channels, level)
+UNET_TENSORS = [("x", 8, 0), ("a0", 16, 0), ("s0", 16, 0), ("p0", 16, 1), ("a1", 32, 1), ("s1", 32, 1),
+                ("p1", 32, 2), ("a2", 64, 2), ("s2", 64, 2), ("p2", 64, 3), ("a3", 128, 3), ("t3", 128, 3),
+                ("u2", 64, 2), ("v2", 64, 2), ("u1", 32, 1), ("v1", 32, 1), ("u0", 16, 0), ("v0", 16, 0),
+                ("out", 1, 0), ("dout", 1, 0), ("dv0", 16, 0), ("du0", 16, 0), ("dcat0", 48, 0), ("dv1", 32, 1),
+                ("du1", 32, 1), ("dcat1", 96, 1), ("dv2", 64, 2), ("du2", 64, 2), ("dcat2", 192, 2),
+                ("dt3", 128, 3), ("da3", 128, 3), ("dp2", 64, 3), ("ds2", 64, 2), ("da2", 64, 2), ("dp1", 32, 2),
+                ("ds1", 32, 1), ("da1", 32, 1), ("dp0", 16, 1), ("ds0", 16, 0), ("da0", 16, 0)]
+UNET_TENSOR_ID = {n: i for i, (n, _, _) in enumerate(UNET_TENSORS)}
+# enum shk_unet_layer
+UNET_LAYERS = ["enc0a", "enc0b", "enc1a", "enc1b", "enc2a", "enc2b", "enc3a", "enc3b", "dec2a", "dec2b", "dec1a",
+               "dec1b", "dec0a", "dec0b", "head"]
 
 
 class ShaktiHIPError(RuntimeError):
@@ -76,7 +100,7 @@ def load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ShaktiHIPError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
     L = C.CDLL(str(LIB_PATH))
-    for name in EXPORTS:
+    for name in EXPORTS + EXPORTS_LEARNED:
         if not hasattr(L, name):
             raise ShaktiHIPError(f"{LIB_PATH} does not export {name}")
     vp, dp, i32, i64, sz, dbl = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int64, C.c_size_t, C.c_double
@@ -109,8 +133,41 @@ def load() -> C.CDLL:
     L.shk_profile_enable.argtypes = [vp, i32]
     L.shk_profile_read.argtypes = [vp, i32, C.POINTER(C.c_char_p), C.POINTER(i64), dp, dp, C.POINTER(i32)]
     L.shk_profile_reset.argtypes = [vp]
+    fp = C.POINTER(C.c_float)
+    L.shk_unet_create.argtypes = [C.POINTER(ShkUnetCfg), C.POINTER(vp)]
+    L.shk_unet_destroy.argtypes = [vp]
+    L.shk_unet_last_error.argtypes = [vp]
+    L.shk_unet_last_error.restype = C.c_char_p
+    L.shk_unet_num_params.restype = i64
+    L.shk_unet_layer_info.argtypes = [i32, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32)]
+    L.shk_unet_init_params.argtypes = [vp, C.c_uint64]
+    L.shk_unet_set_params.argtypes = [vp, fp, i64]
+    L.shk_unet_get_params.argtypes = [vp, fp, i64]
+    L.shk_unet_get_grads.argtypes = [vp, fp, i64]
+    L.shk_unet_forward.argtypes = [vp, vp, i32, i32, i32, vp]
+    L.shk_unet_loss.argtypes = [vp, vp, i32, dp]
+    L.shk_unet_backward.argtypes = [vp, vp]
+    L.shk_unet_adam.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.shk_unet_train_step.argtypes = [vp, vp, vp, i32, i32, i32, C.c_float, dp]
+    L.shk_unet_download_tensor.argtypes = [vp, i32, fp, i64]
+    L.shk_dev_alloc.argtypes = [sz, C.POINTER(vp)]
+    L.shk_dev_free.argtypes = [vp]
+    L.shk_dev_copy.argtypes = [vp, vp, sz, i32]
+    L.shk_attach_unet.argtypes = [vp, vp]
+    L.shk_record_pairs.argtypes = [vp, vp, vp, i32]
+    L.shk_recorded_pairs.argtypes = [vp, C.POINTER(i32)]
     _lib = L
     return L
+
+
+def check_net(rc: int, net=None, what: str = "") -> None:
+    """Like check() for a network handle (shk_unet_last_error)."""
+    if rc != SHK_OK:
+        msg = ""
+        if net is not None:
+            raw = load().shk_unet_last_error(net)
+            msg = raw.decode() if raw else ""
+        raise ShaktiHIPError(f"{what}: {ERRORS.get(rc, rc)} {msg}".strip())
 
 
 def check(rc: int, ctx=None, what: str = "") -> None:

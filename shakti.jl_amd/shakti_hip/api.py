@@ -195,7 +195,7 @@ class State:
     # per-kernel HIP-event timing inside the library (see shk_profile_*)
     PROFILE_IDS = ["assemble", "lanczos", "cheb_bounds", "cg_init", "cheb_pass", "cg_matvec", "cg_update",
                    "cg_finish", "derived", "gap", "stagger_b", "ieb", "ic", "matvec", "ir", "cg_update_cheb",
-                   "mg_down", "mg_up", "mg_coarse", "mg_setup"]
+                   "mg_down", "mg_up", "mg_coarse", "mg_setup", "nn_forward", "nn_warm_start"]
 
     def profile_enable(self, kernels=None) -> None:
         """kernels: None for all, [] to switch off, or a list of kernel names."""
@@ -286,6 +286,7 @@ class _Stats:
     rnorm0: float = 0.0
     rnorm: float = 0.0
     refinements: int = 0
+    fallback: bool = False
 
 
 class _Workspace:
@@ -310,7 +311,7 @@ class AbstractIterativeSolver(AbstractLinearSolver):
     pass
 
 
-FORMS = {"fp64": 0, "fp32_refine": 1}
+FORMS = {"fp64": 0, "fp32_refine": 1, "learned": 2}
 
 
 class CGIterativeSolver(AbstractIterativeSolver):
@@ -323,12 +324,17 @@ class CGIterativeSolver(AbstractIterativeSolver):
     ``multigrid=True`` preconditions with the FP32 geometric-multigrid
     V-cycle: with ``form="fp64"`` it preconditions FP64 CG (residuals, dots
     and updates stay FP64); with ``form="fp32_refine"`` it preconditions the
-    FP32 inner solves of the FP64 refinement loop."""
+    FP32 inner solves of the FP64 refinement loop.
+
+    ``form="learned"`` with ``network=`` a trained ``learned.UNet`` runs FP64
+    CG (with the chosen preconditioner) from the network's warm start; the
+    stopping test still refers to the residual at the Picard iterate h, and
+    a warm start that is not a descent step is dropped (``ws.stats.fallback``)."""
 
     def __init__(self, g: Grid, system=HIPMatrixFreeLinearSystem, chebyshev_degree: int | None = None,
                  chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
                  rtol: float = SQRT_EPS, itmax: int = 0, form: str = "fp64", check_every: int = 8,
-                 inner_rtol: float = 1e-3, multigrid: bool = False):
+                 inner_rtol: float = 1e-3, multigrid: bool = False, network=None):
         if system is SparseAssembledLinearSystem:
             raise L.ShaktiHIPError("CGIterativeSolver(g, SparseAssembledLinearSystem) is CPU-only; use "
                                    "HIPMatrixFreeLinearSystem on the GPU.")
@@ -345,6 +351,10 @@ class CGIterativeSolver(AbstractIterativeSolver):
             raise L.ShaktiHIPError(f"unknown solver form {form!r}")
         if multigrid and chebyshev_degree is not None:
             raise L.ShaktiHIPError("multigrid=True replaces chebyshev_degree: give one of them")
+        if (form == "learned") != (network is not None):
+            raise L.ShaktiHIPError("form='learned' takes the trained network (shakti_hip.learned.UNet) as "
+                                   "network=, and only that form does")
+        self.network = network
         self.multigrid = bool(multigrid)
         self.grid = g
         self.chebyshev_degree = chebyshev_degree
@@ -364,7 +374,7 @@ class CGIterativeSolver(AbstractIterativeSolver):
     def _record(self, st: L.ShkSolveStats):
         self.last_stats = st
         self.ws.stats = _Stats(st.niter, bool(st.solved), bool(st.zero_curvature), st.rnorm0, st.rnorm,
-                               st.refinements)
+                               st.refinements, bool(st.fallback))
         if self.precond is not None:
             self.precond.lambda_min, self.precond.lambda_max = st.lambda_min, st.lambda_max
             if st.bounds_warning:
@@ -384,6 +394,12 @@ def _solver_cfg(ls: CGIterativeSolver | None) -> L.ShkSolverCfg:
 
 def _bind_solver(s: State, ls: CGIterativeSolver):
     L.check(L.load().shk_set_solver_cfg(s.ctx, C.byref(ls.cfg())), s.ctx, "shk_set_solver_cfg")
+    net = getattr(ls, "network", None)
+    handle = net.handle if net is not None else None
+    if getattr(s, "_attached", None) != handle:
+        L.check(L.load().shk_attach_unet(s.ctx, handle), s.ctx, "shk_attach_unet")
+        s._attached = handle
+        s._attached_net = net  # the context holds the handle: keep the network alive
 
 
 def solve_elliptic_linear_system(ls: CGIterativeSolver, s: State, g: Grid, p: ModelParameters,

@@ -9,8 +9,8 @@ import pytest
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 
 
-def _declared_symbols():
-    text = (ROOT / "include" / "shakti_hip.h").read_text()
+def _declared_symbols(header="shakti_hip.h"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(shk_[a-z0-9_]+)\s*\(", text)))
 
@@ -20,9 +20,33 @@ def test_library_exports_every_declared_symbol(hip):
     lib = C.CDLL(str(L.LIB_PATH))
     declared = _declared_symbols()
     assert len(declared) >= 25
-    missing = [s for s in declared if not hasattr(lib, s)]
+    learned = _declared_symbols("shakti_learned.h")
+    assert len(learned) >= 18
+    headers = sorted(p.name for p in (ROOT / "include").glob("*.h"))
+    assert headers == ["shakti_hip.h", "shakti_learned.h"]
+    missing = [s for s in declared + learned if not hasattr(lib, s)]
     assert not missing, missing
     assert sorted(L.EXPORTS) == declared
+    assert sorted(L.EXPORTS_LEARNED) == learned
+
+
+def test_unet_parameter_layout(hip):
+    """Parameter count and per-layer offsets of the 16/32/64/128 U-Net (no device needed)."""
+    from shakti_hip import _lib as L
+    lib = L.load()
+    chans = [(8, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (192, 64), (64, 64),
+             (96, 32), (32, 32), (48, 16), (16, 16)]
+    off = 0
+    for l, (ci, co) in enumerate(chans):
+        o, a, b = C.c_int64(), C.c_int32(), C.c_int32()
+        assert lib.shk_unet_layer_info(l, C.byref(o), C.byref(a), C.byref(b)) == 0
+        assert (o.value, a.value, b.value) == (off, ci, co)
+        off += co * 9 * ci + co
+    o, a, b = C.c_int64(), C.c_int32(), C.c_int32()
+    assert lib.shk_unet_layer_info(14, C.byref(o), C.byref(a), C.byref(b)) == 0
+    assert (o.value, a.value, b.value) == (off, 16, 1)
+    assert lib.shk_unet_num_params() == off + 17
+    assert lib.shk_unet_layer_info(15, None, None, None) == -1
 
 
 def test_abi_version_and_struct_sizes(hip):
