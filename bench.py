@@ -54,11 +54,17 @@ def gpu_run(args, g, raw, moulin, form):
     H.set_initial_conditions(s, g, p, sl, **raw)
     # fp64: the reference's Chebyshev(4)-PCG in FP64; fp32_refine: FP32 Chebyshev(4)-PCG inside FP64
     # refinement; fp32_mg: FP32 multigrid-PCG inside FP64 refinement; fp64_mg: FP64 CG preconditioned
-    # by the FP32 multigrid V-cycle
+    # by the FP32 multigrid V-cycle; learned_mg: fp64_mg from the U-Net's warm start
     if form == "fp64":
         kw = dict(chebyshev_degree=4, rtol=args.rtol, form="fp64")
     elif form == "fp64_mg":
         kw = dict(multigrid=True, rtol=args.mg_rtol, form="fp64")
+    elif form == "learned_mg":
+        # fp64_mg warm-started by the trained U-Net (its own forward kernels, BF16 on MFMA)
+        from shakti_hip import learned as LN
+        H8, W8 = LN.canvas_shape(g.nx, g.ny)
+        net = LN.UNet.load(args.weights, 1, H8, W8)
+        kw = dict(multigrid=True, rtol=args.mg_rtol, form="learned", network=net)
     elif form == "fp32_refine":
         kw = dict(chebyshev_degree=4, rtol=args.ir_rtol, form="fp32_refine", inner_rtol=args.inner_rtol)
     else:
@@ -161,7 +167,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--rtol", type=float, default=1e-11)
     ap.add_argument("--check-every", type=int, default=32)
-    ap.add_argument("--forms", default="fp64,fp32_mg,fp64_mg",
+    ap.add_argument("--forms", default="fp64,fp32_mg,fp64_mg,learned_mg",
                     help="comma-separated solver forms to run from the same initial state")
     ap.add_argument("--ir-rtol", type=float, default=1e-8,
                     help="fp32_refine: FP64 residual reduction per linear solve")
@@ -171,6 +177,9 @@ def main():
                     help="fp32_mg: residual reduction of each FP32 multigrid-PCG inner solve")
     ap.add_argument("--mg-rtol", type=float, default=1e-10,
                     help="fp64_mg: reduction of the multigrid-preconditioned residual norm per linear solve")
+    ap.add_argument("--weights", default=str(ROOT / "shakti.jl_amd" / "shakti_hip" / "weights" /
+                                             "unet_16_32_64_128.npz"),
+                    help="learned_mg: trained U-Net parameters")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--max-picard", type=int, default=500,
                     help="Picard iteration cap per step (500 as in the reference configs; lower only for profiling)")

@@ -59,7 +59,11 @@ typedef struct {
     int32_t max_batch;    /* 1 .. 256 */
     int32_t max_h, max_w; /* multiples of 32 and 64 */
     float leaky_slope;    /* LeakyReLU slope for negative inputs (0.1) */
+    int32_t loss_kind;    /* SHK_LOSS_L2 or SHK_LOSS_ENERGY (shk_unet_loss) */
 } shk_unet_cfg;
+
+#define SHK_LOSS_L2 0     /* 1 - (N.t)^2 / (|N|^2 |t|^2) */
+#define SHK_LOSS_ENERGY 1 /* 1 - (N.r)^2 / (E(N) E(t)), E the energy of the input's operator */
 
 /* Layers, in parameter order. */
 enum shk_unet_layer {
@@ -107,10 +111,15 @@ int shk_unet_get_grads(shk_unet* net, float* host, int64_t n);
  * NULL; the output stays in the network's SHK_T_OUT tensor). */
 int shk_unet_forward(shk_unet* net, const void* x_dev, int32_t B, int32_t H, int32_t W, float* y_dev);
 
-/* Scale-free loss of the last forward output against target t (FP32
- * [B][H][W]): the mean over images of 1 - (N.t)^2 / (|N|^2 |t|^2), i.e. the
- * relative squared error left after the best scaling of N.  Writes the loss
- * to *loss and, when with_grad, its gradient to the SHK_T_DOUT tensor. */
+/* Scale-free loss of the last forward output N against target t (FP32
+ * [B][H][W]), averaged over images.  SHK_LOSS_L2: 1 - (N.t)^2 / (|N|^2 |t|^2),
+ * the relative squared error left after the best scaling of N.
+ * SHK_LOSS_ENERGY: the same in the energy norm of the Jacobi-scaled operator
+ * that the last forward's input carries, evaluated in flux form,
+ * E(u) = sum_cells c u^2 + sum_faces a_f (u_i - u_j)^2, so
+ * 1 - (N.r)^2 / (E(N) E(t)) with r = input channel 0 (t solves A t = r).
+ * Writes the loss to *loss and, when with_grad, its gradient to the
+ * SHK_T_DOUT tensor. */
 int shk_unet_loss(shk_unet* net, const float* t_dev, int32_t with_grad, double* loss);
 
 /* Backward pass of the last forward from dout_dev (FP32 [B][H][W]; NULL: use

@@ -217,6 +217,50 @@ def loss(out, t):
     return Lb.mean(), (grad / B).reshape(out.shape)
 
 
+def _nb(u, dy, dx):
+    """u at (y+dy, x+dx), zero outside the image (u: [B, H, W])."""
+    out = np.zeros_like(u)
+    B, H, W = u.shape
+    ys, ye = max(0, -dy), min(H, H - dy)
+    xs, xe = max(0, -dx), min(W, W - dx)
+    out[:, ys:ye, xs:xe] = u[:, ys + dy:ye + dy, xs + dx:xe + dx]
+    return out
+
+
+def energy(u, x):
+    """E(u) per image: sum c u^2 + sum over east / north faces a (u - u_nb)^2,
+    with c = ch5, a_E = ch1 / 2, a_N = ch3 / 2 of the input x [B, H, W, 8]."""
+    c, aE, aN = x[..., 5], 0.5 * x[..., 1], 0.5 * x[..., 3]
+    dE = u - _nb(u, 0, 1)
+    dN = u - _nb(u, 1, 0)
+    return (c * u * u + aE * dE * dE + aN * dN * dN).reshape(u.shape[0], -1).sum(1)
+
+
+def apply_scaled(u, x):
+    """A^ u in flux form: c u + sum_faces a_f (u - u_nb), west / south
+    couplings from ch2 / ch4."""
+    c = x[..., 5]
+    out = c * u
+    for ch, dy, dx in ((1, 0, 1), (2, 0, -1), (3, 1, 0), (4, -1, 0)):
+        out = out + 0.5 * x[..., ch] * (u - _nb(u, dy, dx))
+    return out
+
+
+def loss_energy(out, x, t):
+    """Mean over images of 1 - (N.r)^2 / (E(N) E(t)), r = x[..., 0], and its
+    gradient (dE/dN = 2 A^ N)."""
+    B = out.shape[0]
+    N = out.astype(np.float64)
+    r = x[..., 0]
+    nr = (N * r).reshape(B, -1).sum(1)
+    en, et = energy(N, x), energy(t.astype(np.float64), x)
+    Lb = 1.0 - nr * nr / (en * et)
+    k1 = -2.0 * nr / (en * et)
+    k2 = 2.0 * nr * nr / (en * en * et)
+    grad = k1[:, None, None] * r + k2[:, None, None] * apply_scaled(N, x)
+    return Lb.mean(), grad / B
+
+
 # ---------------------------------------------------- warm-start features
 
 def canvas_shape(nx: int, ny: int):

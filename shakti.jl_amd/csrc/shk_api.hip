@@ -26,6 +26,7 @@ namespace shknn {
 int forward(shk_unet* net, const void* x, int B, int H, int W);
 float* output(shk_unet* net);
 void set_stream(shk_unet* net, hipStream_t st);
+void release_stream(shk_unet* net, hipStream_t st);
 bool fits(const shk_unet* net, int H, int W);
 const char* unet_error(const shk_unet* net);
 }  // namespace shknn
@@ -832,7 +833,7 @@ int shk_destroy(shk_ctx* ctx) {
     }
     if (ctx->mg_inv) hipFree(ctx->mg_inv);
     if (ctx->mg_work) hipFree(ctx->mg_work);
-    if (ctx->net) shknn::set_stream(ctx->net, nullptr);
+    if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
     void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw};
     for (void* p : nnp)
         if (p) hipFree(p);
@@ -1168,7 +1169,7 @@ static int nn_buffers(shk_ctx* ctx) {
 int shk_attach_unet(shk_ctx* ctx, shk_unet* net) {
     if (!ctx) return SHK_ERR_ARG;
     HIPCHK(hipStreamSynchronize(ctx->st));
-    if (ctx->net) shknn::set_stream(ctx->net, nullptr);
+    if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
     ctx->net = nullptr;
     if (!net) return SHK_OK;
     if (!shknn::fits(net, ctx->H8, ctx->W8))

@@ -499,6 +499,86 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ N, 
     dN[i] = (float)(coef[b * 3 + 1] * (double)t[i] + coef[b * 3 + 2] * (double)N[i]);
 }
 
+// Energy-norm variant of the scale-free loss.  The input image carries the
+// Jacobi-scaled operator: r^ = ch0, east / north couplings a = ch1 / 2, ch3 / 2,
+// west / south ch2 / 2, ch4 / 2, and the reaction c = ch5.  In flux form the
+// energy of u is a sum of non-negative terms,
+//   E(u) = sum_cells c u^2 + sum_faces a_f (u_i - u_j)^2,
+// so it stays accurate with BF16 couplings.  Per image:
+//   L_b = 1 - (N.r^)^2 / (E(N) E(t)),
+// the relative energy-norm error left after the best scaling of N when t
+// solves A^ t = r^.  part[b][blk] = {N.r^, E(N), E(t)}.
+__global__ __launch_bounds__(256) void k_eloss_sums(const float* __restrict__ N, const bf16* __restrict__ x,
+                                                    const float* __restrict__ t, int H, int W, long nps,
+                                                    double* __restrict__ part) {
+    __shared__ double red[256][3];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.y;
+    const long HW = (long)H * W;
+    const long p_begin = (long)blockIdx.x * nps;
+    const long p_end = p_begin + nps < HW ? p_begin + nps : HW;
+    const float* Nb = N + (size_t)b * HW;
+    const float* tb = t + (size_t)b * HW;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (long p = p_begin + tid; p < p_end; p += 256) {
+        const int yy = (int)(p / W), xx = (int)(p - (long)yy * W);
+        const bf16x8 f = *(const bf16x8*)(x + ((size_t)b * HW + p) * 8);
+        const double r = (float)f[0], aE = 0.5f * (float)f[1], aN = 0.5f * (float)f[3], c = (float)f[5];
+        const double n = Nb[p], tv = tb[p];
+        const double nE = xx + 1 < W ? Nb[p + 1] : 0.0, tE = xx + 1 < W ? tb[p + 1] : 0.0;
+        const double nN = yy + 1 < H ? Nb[p + W] : 0.0, tN = yy + 1 < H ? tb[p + W] : 0.0;
+        s0 += n * r;
+        s1 += c * n * n + aE * (n - nE) * (n - nE) + aN * (n - nN) * (n - nN);
+        s2 += c * tv * tv + aE * (tv - tE) * (tv - tE) + aN * (tv - tN) * (tv - tN);
+    }
+    red[tid][0] = s0;
+    red[tid][1] = s1;
+    red[tid][2] = s2;
+    __syncthreads();
+    if (tid < 3) {
+        double s = 0;
+        for (int l = 0; l < 256; ++l) s += red[l][tid];
+        part[((size_t)b * gridDim.x + blockIdx.x) * 3 + tid] = s;
+    }
+}
+
+// coef[b] = {L_b, k1, k2}: dL/dN = k1 r^ + k2 (A^ N), with A^ N in flux form.
+__global__ void k_eloss_finish(const double* __restrict__ part, int nblk, int B, double* __restrict__ coef) {
+    const int b = threadIdx.x;
+    if (b >= B) return;
+    double nr = 0, en = 0, et = 0;
+    for (int k = 0; k < nblk; ++k) {
+        nr += part[((size_t)b * nblk + k) * 3 + 0];
+        en += part[((size_t)b * nblk + k) * 3 + 1];
+        et += part[((size_t)b * nblk + k) * 3 + 2];
+    }
+    const double tiny = 1e-300;
+    const double d = en * et + tiny;
+    coef[b * 3 + 0] = 1.0 - nr * nr / d;
+    coef[b * 3 + 1] = -2.0 * nr / (d * B);
+    coef[b * 3 + 2] = 2.0 * nr * nr / ((en + tiny) * d * B);
+}
+
+__global__ __launch_bounds__(256) void k_eloss_grad(const float* __restrict__ N, const bf16* __restrict__ x,
+                                                    const double* __restrict__ coef, int H, int W, int B,
+                                                    float* __restrict__ dN) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long HW = (long)H * W;
+    if (i >= HW * B) return;
+    const int b = (int)(i / HW);
+    const long p = i - (long)b * HW;
+    const int yy = (int)(p / W), xx = (int)(p - (long)yy * W);
+    const float* Nb = N + (size_t)b * HW;
+    const bf16x8 f = *(const bf16x8*)(x + (size_t)i * 8);
+    const double n = Nb[p];
+    double an = (double)(float)f[5] * n;
+    if (xx + 1 < W) an += 0.5 * (double)(float)f[1] * (n - Nb[p + 1]);
+    if (xx > 0) an += 0.5 * (double)(float)f[2] * (n - Nb[p - 1]);
+    if (yy + 1 < H) an += 0.5 * (double)(float)f[3] * (n - Nb[p + W]);
+    if (yy > 0) an += 0.5 * (double)(float)f[4] * (n - Nb[p - W]);
+    dN[i] = (float)(coef[b * 3 + 1] * (double)(float)f[0] + coef[b * 3 + 2] * an);
+}
+
 // Adam on the FP32 master weights (bias-corrected step sizes from the host).
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ th, const float* __restrict__ g,
                                               float* __restrict__ m, float* __restrict__ v, long n, float lr,
@@ -768,6 +848,10 @@ int forward(shk_unet* net, const void* x, int B, int H, int W) {
 
 float* output(shk_unet* net) { return (float*)net->t[SHK_T_OUT]; }
 void set_stream(shk_unet* net, hipStream_t st) { net->st = st ? st : net->own; }
+// back to the network's own stream, if it still runs on st
+void release_stream(shk_unet* net, hipStream_t st) {
+    if (net->st == st) net->st = net->own;
+}
 bool fits(const shk_unet* net, int H, int W) {
     return net->cfg.max_batch >= 1 && H <= net->cfg.max_h && W <= net->cfg.max_w;
 }
@@ -844,12 +928,23 @@ int loss(shk_unet* net, const float* tgt, int with_grad, double* out) {
     if (nblk > 64) nblk = 64;
     const long nps = (HW + nblk - 1) / nblk;
     hipStream_t st = net->st;
-    hipLaunchKernelGGL(k_loss_sums, dim3((unsigned)nblk, B), dim3(256), 0, st, (const float*)net->t[SHK_T_OUT],
-                       tgt, HW, nps, net->lpart);
-    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(256), 0, st, net->lpart, (int)nblk, B, net->lcoef);
-    if (with_grad)
-        hipLaunchKernelGGL(k_loss_grad, dim3(blocks(HW * B)), dim3(256), 0, st, (const float*)net->t[SHK_T_OUT],
-                           tgt, net->lcoef, HW, B, (float*)net->t[SHK_T_DOUT]);
+    if (net->cfg.loss_kind == 1) {
+        const bf16* x = (const bf16*)net->x_in;
+        hipLaunchKernelGGL(k_eloss_sums, dim3((unsigned)nblk, B), dim3(256), 0, st,
+                           (const float*)net->t[SHK_T_OUT], x, tgt, net->H, net->W, nps, net->lpart);
+        hipLaunchKernelGGL(k_eloss_finish, dim3(1), dim3(256), 0, st, net->lpart, (int)nblk, B, net->lcoef);
+        if (with_grad)
+            hipLaunchKernelGGL(k_eloss_grad, dim3(blocks(HW * B)), dim3(256), 0, st,
+                               (const float*)net->t[SHK_T_OUT], x, net->lcoef, net->H, net->W, B,
+                               (float*)net->t[SHK_T_DOUT]);
+    } else {
+        hipLaunchKernelGGL(k_loss_sums, dim3((unsigned)nblk, B), dim3(256), 0, st,
+                           (const float*)net->t[SHK_T_OUT], tgt, HW, nps, net->lpart);
+        hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(256), 0, st, net->lpart, (int)nblk, B, net->lcoef);
+        if (with_grad)
+            hipLaunchKernelGGL(k_loss_grad, dim3(blocks(HW * B)), dim3(256), 0, st,
+                               (const float*)net->t[SHK_T_OUT], tgt, net->lcoef, HW, B, (float*)net->t[SHK_T_DOUT]);
+    }
     NNCHK(hipGetLastError());
     std::vector<double> c(3 * B);
     NNCHK(hipMemcpyAsync(c.data(), net->lcoef, 3 * B * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -905,7 +1000,7 @@ int shk_unet_create(const shk_unet_cfg* cfg, shk_unet** out) {
     if (!cfg || !out) return SHK_ERR_ARG;
     *out = nullptr;
     if (cfg->max_batch < 1 || cfg->max_batch > 256 || cfg->max_h < 32 || cfg->max_w < 64 || cfg->max_h % 32 ||
-        cfg->max_w % 64 ||
+        cfg->max_w % 64 || cfg->loss_kind < 0 || cfg->loss_kind > 1 ||
         !(cfg->leaky_slope >= 0.f && cfg->leaky_slope < 1.f))
         return SHK_ERR_ARG;
     int ndev = 0;

@@ -68,7 +68,8 @@ class ShkSolveStats(C.Structure):
 
 
 class ShkUnetCfg(C.Structure):
-    _fields_ = [("max_batch", C.c_int32), ("max_h", C.c_int32), ("max_w", C.c_int32), ("leaky_slope", C.c_float)]
+    _fields_ = [("max_batch", C.c_int32), ("max_h", C.c_int32), ("max_w", C.c_int32), ("leaky_slope", C.c_float),
+                ("loss_kind", C.c_int32)]
 
 
 # enum shk_unet_tensor of include/shakti_learned.h: (name, channels, level)

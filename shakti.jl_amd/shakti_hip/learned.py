@@ -116,11 +116,13 @@ class UNet:
     """A network with buffers for up to ``max_batch`` images of
     ``max_h`` x ``max_w`` (multiples of 32 and 64)."""
 
+    LOSSES = {"l2": 0, "energy": 1}
+
     def __init__(self, max_batch: int = 1, max_h: int = 32, max_w: int = 64, slope: float = 0.1,
-                 seed: int | None = 0, params: np.ndarray | None = None):
+                 seed: int | None = 0, params: np.ndarray | None = None, loss: str = "l2"):
         lib = L.load()
         self._h = C.c_void_p()
-        cfg = L.ShkUnetCfg(int(max_batch), int(max_h), int(max_w), float(slope))
+        cfg = L.ShkUnetCfg(int(max_batch), int(max_h), int(max_w), float(slope), self.LOSSES[loss])
         L.check_net(lib.shk_unet_create(C.byref(cfg), C.byref(self._h)), None, "shk_unet_create")
         weakref.finalize(self, _finalize_net, self._h)
         self.max_batch, self.max_h, self.max_w, self.slope = max_batch, max_h, max_w, slope

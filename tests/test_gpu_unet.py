@@ -34,7 +34,7 @@ def _close_f32(gpu, ref, name, rel=1e-4):
 def run(hip):
     from shakti_hip import _lib as L
     from shakti_hip import learned as LN
-    net = LN.UNet(max_batch=B, max_h=H, max_w=W, slope=SLOPE, seed=7)
+    net = LN.UNet(max_batch=B, max_h=H, max_w=W, slope=SLOPE, seed=7, loss="l2")
     rng = np.random.default_rng(11)
     print(f"U-Net {B}x{H}x{W}: forward", flush=True)
     xn = rng.normal(size=(B, H, W, 8))
@@ -129,6 +129,28 @@ def test_backward_every_tensor_and_gradient_matches_oracle(run):
         _close_bf16(bw["d" + deep], U.leaky_grad(fw[deep], U.bf16_round(ref_deep), SLOPE), "d" + deep)
         ref_skip = U.bf16_round(dcat[..., cd:]) + U.pool_bwd(f64(bw["dp" + str(lvl)]))
         _close_bf16(bw["d" + skip], U.leaky_grad(fw[skip], U.bf16_round(ref_skip), SLOPE), "d" + skip)
+
+
+def test_energy_loss_and_its_gradient_match_oracle(hip):
+    """The energy-norm loss (flux form over the input's couplings) against the
+    oracle on the BF16 input the kernels read."""
+    from shakti_hip import learned as LN
+    net = LN.UNet(max_batch=B, max_h=H, max_w=W, slope=SLOPE, seed=9, loss="energy")
+    rng = np.random.default_rng(13)
+    x = np.zeros((B, H, W, 8))
+    aE = rng.uniform(0, 1, (B, H, W)); aE[:, :, -1] = 0
+    aN = rng.uniform(0, 1, (B, H, W)); aN[:, -1, :] = 0
+    x[..., 1], x[..., 3] = aE, aN
+    x[..., 2], x[..., 4] = U._nb(aE, 0, -1), U._nb(aN, -1, 0)
+    x[..., 5] = rng.uniform(1e-3, 0.1, (B, H, W))
+    x[..., 0] = rng.normal(size=(B, H, W))
+    xd = LN.DeviceArray.from_numpy(x, "bf16")
+    out = net.forward(xd).download().astype(np.float64)
+    t = rng.normal(size=(B, H, W))
+    lv = net.loss(LN.DeviceArray.from_numpy(t, "f32"), with_grad=True)
+    ref, g = U.loss_energy(out, U.bf16_round(x), t.astype(np.float32).astype(np.float64))
+    assert abs(lv - ref) <= 1e-9
+    _close_f32(net.tensor("dout", B, H, W), g, "energy dout", rel=1e-5)
 
 
 def test_whole_gradient_close_to_fp64_network(run):

@@ -149,3 +149,37 @@ def test_features_match_a_direct_loop():
                    cP[i, j] / d, 0, 0]
             np.testing.assert_allclose(f[j, i], exp, rtol=1e-13, atol=1e-15)
     assert not f[ny:].any() and not f[:, nx:].any()
+
+
+def _consistent_features(rng, B, H, W):
+    """Random Jacobi-scaled couplings with the west / south channels equal to
+    the neighbours' east / north ones, as the solver writes them."""
+    x = np.zeros((B, H, W, 8))
+    aE = rng.uniform(0, 1, (B, H, W)); aE[:, :, -1] = 0
+    aN = rng.uniform(0, 1, (B, H, W)); aN[:, -1, :] = 0
+    x[..., 1], x[..., 3] = aE, aN
+    x[..., 2] = U._nb(aE, 0, -1)
+    x[..., 4] = U._nb(aN, -1, 0)
+    x[..., 5] = rng.uniform(1e-3, 0.1, (B, H, W))
+    return x
+
+
+def test_energy_loss_against_closed_form_and_finite_differences():
+    rng = np.random.default_rng(6)
+    B, H, W = 2, 6, 9
+    x = _consistent_features(rng, B, H, W)
+    t = rng.normal(size=(B, H, W))
+    x[..., 0] = U.apply_scaled(t, x)  # r = A^ t
+    # E(u) = u . A^ u (flux form == operator form)
+    assert np.allclose(U.energy(t, x), (t * U.apply_scaled(t, x)).reshape(B, -1).sum(1), rtol=1e-13)
+    # zero for any multiple of the solution, positive otherwise
+    assert abs(U.loss_energy(3.0 * t, x, t)[0]) < 1e-13
+    N = rng.normal(size=(B, H, W))
+    lv, g = U.loss_energy(N, x, t)
+    assert 0 < lv < 1
+    eps = 1e-6
+    for k in range(4):
+        v = rng.normal(size=N.shape)
+        v /= np.linalg.norm(v)
+        fd = (U.loss_energy(N + eps * v, x, t)[0] - U.loss_energy(N - eps * v, x, t)[0]) / (2 * eps)
+        assert abs(fd - (g * v).sum()) <= 1e-7 * max(1e-3, abs(fd))
