@@ -97,6 +97,7 @@ struct shk_ctx {
     float* mg_inv = nullptr;
     double* mg_work = nullptr;
     float mg_omega = 0.8f, mg_scale = 1.5f;  // damped-Jacobi weight, coarse-correction scale (measured best)
+    size_t mg_tail_first = 1;                 // first level handled by the single-workgroup tail kernel
 
     // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)
     struct GraphEntry {
@@ -126,10 +127,11 @@ namespace {
 const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_init",     "cheb_pass",
                             "cg_matvec",  "cg_update",    "cg_finish",   "derived",     "gap",
                             "stagger_b",  "ieb",          "ic",          "matvec",      "ir",
-                            "cg_update_cheb", "mg_down",  "mg_up",       "mg_coarse",   "mg_setup",
-                            "nn_forward", "nn_warm_start"};
+                            "cg_update_cheb", "mg_down",  "mg_up",       "mg_tail",     "mg_setup",
+                            "nn_forward", "nn_warm_start", "mg_down_fine", "mg_up_fine"};
+// mg_down / mg_up: the coarser levels; *_fine: the finest level
 enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR,
-              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NNF, P_NNW, P_NUM };
+              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NNF, P_NNW, P_MGD0, P_MGU0, P_NUM };
 
 int fail(shk_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -342,6 +344,23 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
             for (auto pp : ps) HIPCHK(hipMalloc((void**)pp, n));
             ctx->mg.push_back(L);
         }
+        ctx->mg_tail_first = ctx->mg.size() - 1;
+        for (size_t l = 1; l < ctx->mg.size(); ++l)
+            if ((long)ctx->mg[l].nx * ctx->mg[l].ny <= MG_TAIL_CELLS) {
+                ctx->mg_tail_first = l;
+                break;
+            }
+        // the tail keeps 2 floats per cell of all its levels in LDS: stay within 56 KB
+        // (degenerate shapes such as n x 1 halve only once per level)
+        auto tail_cells = [&](size_t f) {
+            size_t c = 0;
+            for (size_t l = f; l < ctx->mg.size(); ++l) c += (size_t)ctx->mg[l].nx * ctx->mg[l].ny;
+            return c;
+        };
+        while (ctx->mg_tail_first + 1 < ctx->mg.size() && 8 * tail_cells(ctx->mg_tail_first) > 56 * 1024)
+            ++ctx->mg_tail_first;
+        if (ctx->mg.size() - ctx->mg_tail_first > (size_t)MG_MAX_TAIL)
+            return fail(ctx, SHK_ERR_STATE, "multigrid hierarchy deeper than the tail kernel holds");
         const size_t nc = (size_t)ctx->mg.back().nx * ctx->mg.back().ny;
         HIPCHK(hipMalloc((void**)&ctx->mg_inv, nc * nc * sizeof(float)));
         HIPCHK(hipMalloc((void**)&ctx->mg_work, 2 * nc * nc * sizeof(double)));
@@ -368,34 +387,46 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 // z = V(r): one V-cycle (one damped-Jacobi sweep down and up on every level,
 // dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1 iteration).
 // With T = double the finest level reads the FP64 residual and writes the FP64
-// z that FP64 CG uses; every level computes in FP32.
+// z that FP64 CG uses; every level computes in FP32.  The large levels run
+// as tiled kernels, the levels below MG_TAIL_CELLS cells in one workgroup.
 template <class T>
 void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
     auto& mg = ctx->mg;
     const size_t L = mg.size() - 1;
-    const double tb = sizeof(T) - 4.0;  // extra bytes per cell of an FP64 rhs / result
-    for (size_t l = 0; l < L; ++l) {
+    const size_t first = ctx->mg_tail_first;  // >= 1
+    const double tb = sizeof(T) - 4.0;        // extra bytes per cell of an FP64 rhs / result
+    for (size_t l = 0; l < first; ++l) {
         const auto& f = mg[l];
         // reads b, c, aE, aN, D; writes x and the coarse rhs
-        Prof pr(ctx, P_MGD, 4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) +
-                                (l == 0 ? tb * f.nx * f.ny : 0.0));
+        Prof pr(ctx, l == 0 ? P_MGD0 : P_MGD,
+                4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? tb * f.nx * f.ny : 0.0));
         if (l == 0)
             launch::mg_down<T>(f.nx, f.ny, mg[1].nx, mg[1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, r, f.xa, mg[1].b,
-                               ctx->st);
+                               ctx->sc, ctx->st);
         else
             launch::mg_down<float>(f.nx, f.ny, mg[l + 1].nx, mg[l + 1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, f.b,
-                                   f.xa, mg[l + 1].b, ctx->st);
+                                   f.xa, mg[l + 1].b, ctx->sc, ctx->st);
     }
     {
-        const double nc = (double)mg[L].nx * mg[L].ny;
-        Prof pr(ctx, P_MGC, 4.0 * (nc * nc + 2 * nc));
-        launch::mg_coarse(mg[L].nx * mg[L].ny, ctx->mg_inv, mg[L].b, mg[L].xb, ctx->st);
+        MGTail Tl{};
+        Tl.nlev = (int)(L - first + 1);
+        Tl.omega = ctx->mg_omega;
+        Tl.scale = ctx->mg_scale;
+        Tl.inv = ctx->mg_inv;
+        double bytes = 0.0;
+        for (size_t l = first; l <= L; ++l) {
+            const auto& v = mg[l];
+            Tl.L[l - first] = MGLevelDev{v.nx, v.ny, v.c, v.e, v.n, v.D, v.b, v.xa, v.xb};
+            bytes += 4.0 * 13.0 * v.nx * v.ny;
+        }
+        Prof pr(ctx, P_MGC, bytes);
+        launch::mg_tail(Tl, ctx->sc, ctx->st);
     }
-    for (size_t l = L; l-- > 0;) {
+    for (size_t l = first; l-- > 0;) {
         const auto& f = mg[l];
         // reads b, c, aE, aN, D, x and the coarse correction; writes the result
-        Prof pr(ctx, P_MGU, 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) +
-                                (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
+        Prof pr(ctx, l == 0 ? P_MGU0 : P_MGU,
+                4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
         if (l == 0)
             launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, r, f.xa,
                              mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->st);

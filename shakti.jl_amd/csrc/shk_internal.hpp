@@ -60,6 +60,23 @@ struct Scalars {
     uint32_t pad;
 };
 
+// One level of the multigrid hierarchy as the device sees it, and the small
+// levels handled by the single-workgroup tail kernel (level 0 of T is the
+// first level below MG_TAIL_CELLS cells, the last one is the coarsest).
+struct MGLevelDev {
+    int nx, ny;
+    const float *c, *e, *n, *D;
+    float *b, *xa, *xb;
+};
+constexpr int MG_MAX_TAIL = 12;
+constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; all tail levels (~5.5k cells) in 44 KB of LDS
+struct MGTail {
+    int nlev;
+    float omega, scale;
+    const float* inv;
+    MGLevelDev L[MG_MAX_TAIL];
+};
+
 // Launch geometry of the row-marching kernels: workgroups of WAVE x WAVES
 // threads; each wave walks R consecutive rows of one 64-wide column strip.
 struct Geom {
@@ -142,14 +159,14 @@ void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st);
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
-             const float* D, const TB* b, float* x, float* bc, hipStream_t st);
+             const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
 template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st);
+void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st);
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
                       hipStream_t st);
-void mg_coarse(int n, const float* inv, const float* b, float* x, hipStream_t st);
 
 // Mixed precision: FP32 copies of the coefficients; the FP64 residual
 // r = b - A x with its Jacobi-weighted norm (first != 0 records the initial

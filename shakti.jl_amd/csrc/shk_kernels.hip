@@ -1276,63 +1276,206 @@ __device__ __forceinline__ float flux_apply(int nx, int ny, int ix, int iy, size
     return y;
 }
 
-// Down-leg of the V-cycle on one level: one damped-Jacobi sweep from zero,
-// x = omega b / D, stored for the up-leg, and the restricted residual
-// bc = P^T (b - A x).  One thread per coarse cell.
-// TB is the type of the level's right-hand side: FP32 inside the hierarchy,
-// FP64 on the finest level when the V-cycle preconditions FP64 CG (the
-// arithmetic stays FP32 either way).
+// The large levels of the V-cycle work on tiles of MG_TX x MG_TY fine cells
+// (even-aligned, so each tile restricts onto whole coarse cells): the values
+// the 5-point stencil needs at the neighbours are staged once per tile, with
+// a one-cell halo, in LDS.  Workgroups of 64 x 4 threads, 4 rows per thread.
+constexpr int MG_TX = 64, MG_TY = 16, MG_HX = MG_TX + 2, MG_HY = MG_TY + 2;
+
+// Down-leg on one level: one damped-Jacobi sweep from zero, x = omega b / D
+// (stored for the up-leg), and the restricted residual bc = P^T (b - A x).
+// TB is the right-hand side's type: FP64 on the finest level when the
+// V-cycle preconditions FP64 CG, FP32 inside the hierarchy; the arithmetic
+// is FP32.  Skipped once the CG it serves has converged.
 template <class TB>
 __global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int nyc, float omega,
                                                  const float* __restrict__ c, const float* __restrict__ e,
                                                  const float* __restrict__ n, const float* __restrict__ D,
                                                  const TB* __restrict__ b, float* __restrict__ x,
-                                                 float* __restrict__ bc) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    if (I >= nxc || J >= nyc) return;
-    auto xs = [&](size_t k) -> float { return omega * (float)b[k] / D[k]; };
-    float acc = 0.f;
-    const int x0 = 2 * I, y0 = 2 * J;
-    const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
-    for (int iy = y0; iy <= y1; ++iy)
-        for (int ix = x0; ix <= x1; ++ix) {
-            const size_t k = (size_t)ix + (size_t)iy * nx;
-            const float xv = xs(k);
-            x[k] = xv;
-            acc += (float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+                                                 float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ float xs[MG_HY][MG_HX];
+    __shared__ float rs[MG_TY][MG_TX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
+    const int x0 = blockIdx.x * MG_TX, y0 = blockIdx.y * MG_TY;
+    for (int i = tid; i < MG_HX * MG_HY; i += WAVE * WAVES) {
+        const int hy = i / MG_HX, hx = i - hy * MG_HX;
+        const int gx = x0 + hx - 1, gy = y0 + hy - 1;
+        float v = 0.f;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+            const size_t k = (size_t)gx + (size_t)gy * nx;
+            v = omega * (float)b[k] / D[k];
         }
-    bc[(size_t)I + (size_t)J * nxc] = acc;
+        xs[hy][hx] = v;
+    }
+    __syncthreads();
+    const int gx = x0 + tx;
+#pragma unroll
+    for (int j = 0; j < MG_TY / WAVES; ++j) {
+        const int fy = ty + WAVES * j, gy = y0 + fy;
+        float res = 0.f;
+        if (gx < nx && gy < ny) {
+            const size_t k = (size_t)gx + (size_t)gy * nx;
+            const float u = xs[fy + 1][tx + 1];
+            float a = c[k] * u;
+            if (gx < nx - 1) a += e[k] * (u - xs[fy + 1][tx + 2]);
+            if (gx > 0) a += e[k - 1] * (u - xs[fy + 1][tx]);
+            if (gy < ny - 1) a += n[k] * (u - xs[fy + 2][tx + 1]);
+            if (gy > 0) a += n[k - nx] * (u - xs[fy][tx + 1]);
+            res = (float)b[k] - a;
+            x[k] = u;
+        }
+        rs[fy][tx] = res;
+    }
+    __syncthreads();
+    // one thread per coarse cell of the tile (32 x 8)
+    const int cx = tid % (MG_TX / 2), cy = tid / (MG_TX / 2);
+    const int I = x0 / 2 + cx, J = y0 / 2 + cy;
+    if (I < nxc && J < nyc)
+        bc[(size_t)I + (size_t)J * nxc] =
+            (rs[2 * cy][2 * cx] + rs[2 * cy][2 * cx + 1]) + (rs[2 * cy + 1][2 * cx] + rs[2 * cy + 1][2 * cx + 1]);
 }
 
-// Up-leg on one level: add the (scaled) coarse correction, one damped-Jacobi
-// sweep, write the level's result.  On the finest level also the partial of
-// r'z (r = b, z = the result) for CG.
+// Up-leg on one level: u = x + scale * P xc (the scaled coarse correction),
+// one damped-Jacobi sweep z = u + omega (b - A u) / D, written to out; on the
+// finest level also the partial of b'z (= r'z for CG).
 template <class TB>
 __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float omega, float scale,
                                                const float* __restrict__ c, const float* __restrict__ e,
                                                const float* __restrict__ n, const float* __restrict__ D,
                                                const TB* __restrict__ b, const float* __restrict__ x,
                                                const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
-                                               double* partials) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+                                               double* partials, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ float us[MG_HY][MG_HX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
+    const int x0 = blockIdx.x * MG_TX, y0 = blockIdx.y * MG_TY;
+    for (int i = tid; i < MG_HX * MG_HY; i += WAVE * WAVES) {
+        const int hy = i / MG_HX, hx = i - hy * MG_HX;
+        const int gx = x0 + hx - 1, gy = y0 + hy - 1;
+        float v = 0.f;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+            v = x[(size_t)gx + (size_t)gy * nx] + scale * xc[(size_t)(gx >> 1) + (size_t)(gy >> 1) * nxc];
+        us[hy][hx] = v;
+    }
+    __syncthreads();
+    const int gx = x0 + tx;
     double acc = 0.0;
-    if (ix < nx && iy < ny) {
-        auto xs = [&](size_t k) -> float {
-            const int kx = (int)(k % (size_t)nx), ky = (int)(k / (size_t)nx);
-            return x[k] + scale * xc[(size_t)(kx >> 1) + (size_t)(ky >> 1) * nxc];
-        };
-        const size_t k = (size_t)ix + (size_t)iy * nx;
-        const float u = xs(k);
-        const float z = u + omega * ((float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs)) / D[k];
-        out[k] = (TB)z;
-        acc = (double)b[k] * (double)z;
+#pragma unroll
+    for (int j = 0; j < MG_TY / WAVES; ++j) {
+        const int fy = ty + WAVES * j, gy = y0 + fy;
+        if (gx < nx && gy < ny) {
+            const size_t k = (size_t)gx + (size_t)gy * nx;
+            const float u = us[fy + 1][tx + 1];
+            float a = c[k] * u;
+            if (gx < nx - 1) a += e[k] * (u - us[fy + 1][tx + 2]);
+            if (gx > 0) a += e[k - 1] * (u - us[fy + 1][tx]);
+            if (gy < ny - 1) a += n[k] * (u - us[fy + 2][tx + 1]);
+            if (gy > 0) a += n[k - nx] * (u - us[fy][tx + 1]);
+            const TB bk = b[k];
+            const float z = u + omega * ((float)bk - a) / D[k];
+            out[k] = (TB)z;
+            acc += (double)bk * (double)z;
+        }
     }
     if (with_dot) {
         double v[1] = {acc};
         block_partial<1, false>(v, partials);
     }
+}
+
+// The small levels of the V-cycle, from the first level of at most
+// MG_TAIL_CELLS cells down to the coarsest, in one workgroup: down-legs, the
+// dense coarsest solve, up-legs.  Every level's b and x live in LDS (2 floats
+// per cell of all tail levels, dynamic shared memory; x is overwritten by the
+// level's result after a barrier); the coefficients are read from global
+// memory (L2-resident).  The first tail level's b comes from global memory
+// and its result goes back there.  With at most MG_TAIL_CELLS cells per level
+// each thread owns at most 4 cells of a level.
+__global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    extern __shared__ float lds[];
+    __shared__ float red[1024];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const float omega = T.omega, scale = T.scale;
+    int off[MG_MAX_TAIL + 1];
+    off[0] = 0;
+    for (int l = 0; l < T.nlev; ++l) off[l + 1] = off[l] + T.L[l].nx * T.L[l].ny;
+    const int tot = off[T.nlev];
+    float* Bs = lds;        // right-hand sides
+    float* Xs = lds + tot;  // smoothed x, then u, then the level's result
+    for (int k = tid; k < off[1]; k += nt) Bs[k] = T.L[0].b[k];
+    __syncthreads();
+    for (int l = 0; l + 1 < T.nlev; ++l) {
+        const MGLevelDev F = T.L[l], C = T.L[l + 1];
+        const float* b = Bs + off[l];
+        float* x = Xs + off[l];
+        float* bc = Bs + off[l + 1];
+        auto xs = [&](size_t k) -> float { return omega * b[k] / F.D[k]; };
+        for (int q = tid; q < C.nx * C.ny; q += nt) {
+            const int I = q % C.nx, J = q / C.nx;
+            const int fx0 = 2 * I, fy0 = 2 * J;
+            const int fx1 = min(fx0 + 1, F.nx - 1), fy1 = min(fy0 + 1, F.ny - 1);
+            float acc = 0.f;
+            for (int iy = fy0; iy <= fy1; ++iy)
+                for (int ix = fx0; ix <= fx1; ++ix) {
+                    const size_t k = (size_t)ix + (size_t)iy * F.nx;
+                    x[k] = xs(k);
+                    acc += b[k] - flux_apply(F.nx, F.ny, ix, iy, k, F.c, F.e, F.n, xs);
+                }
+            bc[q] = acc;
+        }
+        __syncthreads();
+    }
+    {
+        // coarsest: x = inv b with the symmetric inverse read by columns;
+        // 16 threads per row, 4 products each, summed in a fixed order
+        const int lc = T.nlev - 1;
+        const int ncell = T.L[lc].nx * T.L[lc].ny;
+        const float* b = Bs + off[lc];
+        const int i = tid >> 4, part = tid & 15;
+        float acc = 0.f;
+        if (i < ncell)
+            for (int j = part * 4; j < part * 4 + 4 && j < ncell; ++j) acc += T.inv[(size_t)j * ncell + i] * b[j];
+        red[tid] = acc;
+        __syncthreads();
+        if (part == 0 && i < ncell) {
+            float s = 0.f;
+            for (int q = 0; q < 16; ++q) s += red[tid + q];
+            Xs[off[lc] + i] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = T.nlev - 2; l >= 0; --l) {
+        const MGLevelDev F = T.L[l], C = T.L[l + 1];
+        const int nf = F.nx * F.ny;
+        const float* b = Bs + off[l];
+        float* x = Xs + off[l];
+        const float* zc = Xs + off[l + 1];
+        for (int k = tid; k < nf; k += nt) {
+            const int ix = k % F.nx, iy = k / F.nx;
+            x[k] = x[k] + scale * zc[(ix >> 1) + (iy >> 1) * C.nx];
+        }
+        __syncthreads();
+        auto us = [&](size_t k) -> float { return x[k]; };
+        float z[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = tid + j * nt;
+            if (k < nf) {
+                const int ix = k % F.nx, iy = k / F.nx;
+                z[j] = x[k] + omega * (b[k] - flux_apply(F.nx, F.ny, ix, iy, (size_t)k, F.c, F.e, F.n, us)) / F.D[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = tid + j * nt;
+            if (k < nf) x[k] = z[j];
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < off[1]; k += nt) T.L[0].xb[k] = Xs[k];
 }
 
 // Coarsest level: assemble the dense matrix (FP64, in `work`) and invert it
@@ -1389,20 +1532,6 @@ __global__ __launch_bounds__(256) void k_mg_dense_inverse(int nx, int ny, const 
         __syncthreads();
     }
     for (int t = tid; t < n * n; t += nt) inv[t] = (float)B[t];
-}
-
-// Coarsest-level solve x = A^-1 b with the stored inverse (one workgroup).
-__global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* __restrict__ inv, const float* __restrict__ b,
-                                                   float* __restrict__ x) {
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    const int nt = blockDim.x * blockDim.y;
-    // the inverse of the SPD coarse operator is symmetric: read it by columns so
-    // that neighbouring threads load neighbouring words
-    for (int i = tid; i < n; i += nt) {
-        float acc = 0.f;
-        for (int j = 0; j < n; ++j) acc += inv[(size_t)j * n + i] * b[j];
-        x[i] = acc;
-    }
 }
 
 // ------------------------------------------------ learned warm start (FP64)
@@ -1808,35 +1937,38 @@ void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, flo
 }
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
-             const float* D, const TB* b, float* x, float* bc, hipStream_t st) {
-    hipLaunchKernelGGL(k_mg_down<TB>, cells_grid(nxc, nyc), dim3(64, 4), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D,
-                       b, x, bc);
+             const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+    const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
+    hipLaunchKernelGGL(k_mg_down<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D, b, x, bc,
+                       sc);
 }
 template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st) {
-    const dim3 grid = cells_grid(nx, ny);
-    hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(64, 4), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc, out,
-                       dot_mode >= 0 ? 1 : 0, partials);
+    const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
+    hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
+                       out, dot_mode >= 0 ? 1 : 0, partials, sc);
     if (dot_mode >= 0)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
 }
 template void mg_down<float>(int, int, int, int, float, const float*, const float*, const float*, const float*,
-                             const float*, float*, float*, hipStream_t);
+                             const float*, float*, float*, const Scalars*, hipStream_t);
 template void mg_down<double>(int, int, int, int, float, const float*, const float*, const float*, const float*,
-                              const double*, float*, float*, hipStream_t);
+                              const double*, float*, float*, const Scalars*, hipStream_t);
 template void mg_up<float>(int, int, int, float, float, const float*, const float*, const float*, const float*,
                            const float*, const float*, const float*, float*, int, double*, Scalars*, hipStream_t);
 template void mg_up<double>(int, int, int, float, float, const float*, const float*, const float*, const float*,
                             const double*, const float*, const float*, double*, int, double*, Scalars*, hipStream_t);
+void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st) {
+    size_t cells = 0;
+    for (int l = 0; l < T.nlev; ++l) cells += (size_t)T.L[l].nx * T.L[l].ny;
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 2 * cells * sizeof(float), st, T, sc);
+}
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
                       hipStream_t st) {
     hipLaunchKernelGGL(k_mg_dense_inverse, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, work, inv);
-}
-void mg_coarse(int n, const float* inv, const float* b, float* x, hipStream_t st) {
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(64, 4), 0, st, n, inv, b, x);
 }
 
 static inline unsigned nn_blocks(size_t n) {

@@ -195,7 +195,8 @@ class State:
     # per-kernel HIP-event timing inside the library (see shk_profile_*)
     PROFILE_IDS = ["assemble", "lanczos", "cheb_bounds", "cg_init", "cheb_pass", "cg_matvec", "cg_update",
                    "cg_finish", "derived", "gap", "stagger_b", "ieb", "ic", "matvec", "ir", "cg_update_cheb",
-                   "mg_down", "mg_up", "mg_coarse", "mg_setup", "nn_forward", "nn_warm_start"]
+                   "mg_down", "mg_up", "mg_tail", "mg_setup", "nn_forward", "nn_warm_start", "mg_down_fine",
+                   "mg_up_fine"]
 
     def profile_enable(self, kernels=None) -> None:
         """kernels: None for all, [] to switch off, or a list of kernel names."""
