@@ -10,6 +10,16 @@ baseline plus a 1e-7 m/s one-day sinusoid) plus 20 seeded moulins of
 relative tolerance with Chebyshev(4)-preconditioned CG (every solve to a
 relative preconditioned residual of 1e-11), then the gap step.
 
+With ``--workload parabolic`` the run is the parabolic head scheme instead:
+8192 x 2048 cells, bed amplitude 100 m plus a seeded 2 km-wide trough,
+e_v = 1e-3, dt = 600 s, one backward-Euler solve per timestep.
+
+``--forms`` picks the solver forms run from the same initial state (plain
+FP64 Chebyshev(4)-PCG; FP32 inner solves with FP64 refinement; FP64 CG with
+the FP32 multigrid V-cycle; the same warm-started by the U-Net); ``value`` is
+the fastest form's time and every other form is reported beside it with its
+per-field error against the FP64 form.
+
 A step is one whole timestep.  W untimed warm-up steps (the first solves the
 initial-condition transient), then K timed steps bracketed by device
 synchronisation.  The JSON line reports seconds per timestep, the roofline of
@@ -38,17 +48,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(nx, ny, bed_amp, seed):
+def make_inputs(nx, ny, bed_amp, seed, trough=0.0, trough_depth=0.0):
     from shakti_hip import synthetic as S
-    g, raw = S.glacier(nx, ny, bed_amp=bed_amp, seed=seed)
+    g, raw = S.glacier(nx, ny, bed_amp=bed_amp, seed=seed, trough_width=trough, trough_depth=trough_depth)
     moulin = S.moulin_field(g, raw["mask"], n=20, seed=seed)
     return g, raw, moulin
+
+
+class _ParabolicCounts:
+    """PicardSolver-like view of a parabolic run: one linear solve per step."""
+
+    def __init__(self, ls):
+        self.ls = ls
+        self.converged = True
+        self.last_iter = 1
+
+    @property
+    def cg_iters(self):
+        return self.ls.ws.stats.niter
 
 
 def gpu_run(args, g, raw, moulin, form):
     import shakti_hip as H
     from shakti_hip import synthetic as S
-    p = H.ModelParameters()
+    parabolic = args.workload == "parabolic"
+    p = H.ModelParameters(e_v=args.e_v) if parabolic else H.ModelParameters()
     sl = H.RegularizedCoulombSlidingLaw(0.5)
     s = H.State(g, p)
     H.set_initial_conditions(s, g, p, sl, **raw)
@@ -70,10 +94,16 @@ def gpu_run(args, g, raw, moulin, form):
     else:
         kw = dict(multigrid=True, rtol=args.ir_rtol, form="fp32_refine", inner_rtol=args.mg_inner_rtol)
     ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, atol=0.0, check_every=args.check_every, **kw)
-    ps = H.PicardSolver(args.max_picard, 1e-6, ls, g)
     mi = H.SeasonalMoulinMeltInput(moulin, **S.SEASONAL_C2)
-    sim = H.Simulation(g, s, args.warmup + args.steps, 3600.0, p, "implicit", [], mi, sl, ps=ps,
-                       k_face_choice="harmonic")
+    if parabolic:
+        # ParabolicHeadScheme: one backward-Euler solve per step (src/parabolic_solver.jl:16-38)
+        ps = _ParabolicCounts(ls)
+        sim = H.Simulation(g, s, args.warmup + args.steps, args.dt, p, "implicit", [], mi, sl, ls=ls,
+                           k_face_choice="harmonic")
+    else:
+        ps = H.PicardSolver(args.max_picard, 1e-6, ls, g)
+        sim = H.Simulation(g, s, args.warmup + args.steps, args.dt, p, "implicit", [], mi, sl, ps=ps,
+                           k_face_choice="harmonic")
     for w in range(args.warmup):
         t0 = time.perf_counter()
         H.step(sim)
@@ -127,7 +157,8 @@ def cpu_baseline(args, g, raw, moulin, picard_per_step, cg_per_step):
     import oracle as O
     O.build()
     og = O.Grid(g.nx, g.ny, g.lx, g.ly)
-    p = O.ModelParameters()
+    par = args.workload == "parabolic"
+    p = O.ModelParameters(e_v=args.e_v) if par else O.ModelParameters()
     m = O.Model(kfs=1, sliding_C=0.5)
     s = O.State(og)
     O.set_initial_conditions(s, og, p, m, **raw)
@@ -135,13 +166,20 @@ def cpu_baseline(args, g, raw, moulin, picard_per_step, cg_per_step):
     S = args.cpu_iters
     sv1 = O.Solver(og, "cg", precond="chebyshev", degree=4, atol=0.0, rtol=1e-30, itmax=1)
     h0 = s.h.copy()
+
+    def solve(sv):
+        if par:
+            sv.solve_parabolic(s, p, 1, args.dt)
+        else:
+            sv.solve_elliptic(s, p, 1)
+
     t0 = time.perf_counter()
-    sv1.solve_elliptic(s, p, 1)
+    solve(sv1)
     t1 = time.perf_counter() - t0
     s.h[...] = h0
     svS = O.Solver(og, "cg", precond="chebyshev", degree=4, atol=0.0, rtol=1e-30, itmax=S + 1)
     t0 = time.perf_counter()
-    svS.solve_elliptic(s, p, 1)
+    solve(svS)
     tS = time.perf_counter() - t0
     t0 = time.perf_counter()
     O.update_derived(s, og, p, m)
@@ -151,8 +189,9 @@ def cpu_baseline(args, g, raw, moulin, picard_per_step, cg_per_step):
     est = cg_per_step * per_cg + picard_per_step * per_picard
     sample = (f"oracle on {g.nx}x{g.ny}: 1 solve capped at 1 CG iteration ({t1:.2f} s), 1 capped at {S + 1} "
               f"({tS:.2f} s), 1 derived refresh ({td:.2f} s); {per_cg * 1e3:.1f} ms per Chebyshev(4)-PCG "
-              f"iteration, {per_picard:.2f} s per Picard iteration outside CG; scaled by the GPU run's "
-              f"{picard_per_step:.1f} Picard and {cg_per_step:.0f} CG iterations per step")
+              f"iteration, {per_picard:.2f} s per {'solve' if par else 'Picard iteration'} outside CG; scaled "
+              f"by the GPU run's {picard_per_step:.1f} {'solves' if par else 'Picard iterations'} and "
+              f"{cg_per_step:.0f} CG iterations per step")
     return {"value": est, "unit": "s/timestep", "cores": O.num_threads(), "kind": "port", "sample": sample}
 
 
@@ -161,9 +200,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nx", type=int, default=4096)
-    ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--bed-amp", type=float, default=50.0)
+    ap.add_argument("--workload", choices=["elliptic", "parabolic"], default="elliptic",
+                    help="elliptic: Picard to 1e-6 per step (4096x1024 defaults); parabolic: one backward-Euler "
+                         "solve per step, e_v 1e-3, dt 600 s, 8192x2048, bed amplitude 100 m and a 2 km trough")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--bed-amp", type=float, default=None)
+    ap.add_argument("--dt", type=float, default=None)
+    ap.add_argument("--e-v", type=float, default=1e-3)
+    ap.add_argument("--trough-depth", type=float, default=200.0,
+                    help="parabolic workload: depth (m) of the seeded 2 km-wide trough")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--rtol", type=float, default=1e-11)
     ap.add_argument("--check-every", type=int, default=32)
@@ -185,13 +231,19 @@ def main():
                     help="Picard iteration cap per step (500 as in the reference configs; lower only for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    par = args.workload == "parabolic"
+    args.nx = args.nx or (8192 if par else 4096)
+    args.ny = args.ny or (2048 if par else 1024)
+    args.bed_amp = args.bed_amp if args.bed_amp is not None else (100.0 if par else 50.0)
+    args.dt = args.dt or (600.0 if par else 3600.0)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the head solve is not sharded across GPUs "
               "(see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     log(f"[bench] building inputs {args.nx}x{args.ny}, bed amplitude {args.bed_amp} m, seed {args.seed}")
-    g, raw, moulin = make_inputs(args.nx, args.ny, args.bed_amp, args.seed)
+    g, raw, moulin = make_inputs(args.nx, args.ny, args.bed_amp, args.seed, trough=2000.0 if par else 0.0,
+                                 trough_depth=args.trough_depth if par else 0.0)
     forms = args.forms.split(",")
     runs = {}
     for form in forms:
@@ -268,9 +320,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m, harmonic "
-                               f"K_face, Chebyshev(4)-PCG rtol {args.rtol}, Picard tol 1e-6, implicit gap, "
-                               "seasonal + 20 moulins, dt 3600 s",
+        "config": {"workload": (f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m, harmonic "
+                                f"K_face, Chebyshev(4)-PCG rtol {args.rtol}, Picard tol 1e-6, implicit gap, "
+                                "seasonal + 20 moulins, dt 3600 s") if not par else
+                               (f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m + 2 km trough "
+                                f"{args.trough_depth} m deep, parabolic scheme e_v {args.e_v}, one backward-Euler "
+                                f"solve per step, harmonic K_face, implicit gap, seasonal + 20 moulins, "
+                                f"dt {args.dt} s"),
                    "form": best, "nx": args.nx, "ny": args.ny, "cells": args.nx * args.ny},
         "forms": form_summary,
         "timesteps_per_s": round(1.0 / s_per_step, 4),
