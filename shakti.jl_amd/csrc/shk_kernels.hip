@@ -599,6 +599,85 @@ __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, c
     }
 }
 
+// Two adjacent columns per lane (128-column strips, 16 B loads and stores) for
+// the FP64 operator: the same arithmetic in the same order as march(), so
+// results are identical.  opd2(c) gives the operand at columns (c, c+1),
+// opd1(c) one column (the strip-edge neighbours).  Even nx only (every lane's
+// pair is inside the grid or entirely past its east edge).  NT selects
+// non-temporal loads and stores for the streamed arrays: grids whose arrays
+// exceed the 256 MB Infinity Cache stream them from HBM anyway, and the hint
+// keeps them from evicting what is reused.
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ d2v ld2(const double* p) {
+    if constexpr (NT) return __builtin_nontemporal_load((const d2v*)p);
+    else return *(const d2v*)p;
+}
+template <bool NT>
+__device__ __forceinline__ void st2(double* p, d2v v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, (d2v*)p);
+    else *(d2v*)p = v;
+}
+
+template <bool NT, class Opd2, class Opd1, class Epi2>
+__device__ __forceinline__ void march2(int nx, int ny, int R, int vbx, int vby, const double* __restrict__ aP,
+                                       const double* __restrict__ aE, const double* __restrict__ aN, Opd2 opd2,
+                                       Opd1 opd1, Epi2 epi2) {
+    const int lane = threadIdx.x;
+    const int ix = vbx * 2 * WAVE + 2 * lane;  // columns ix, ix + 1
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
+    if (iy0 >= ny) return;  // uniform over the wave
+    const bool inx = ix < nx;
+    const int ixc = inx ? ix : nx - 2;  // clamped pair for loads of idle lanes
+    d2v um = {0.0, 0.0}, aNm = {0.0, 0.0};
+    if (iy0 > 0) {
+        um = opd2(cidx(ixc, iy0 - 1, nx));
+        aNm = ld2<NT>(aN + cidx(ixc, iy0 - 1, nx));
+    }
+    d2v uc = opd2(cidx(ixc, iy0, nx));
+#pragma unroll 2
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const size_t c = cidx(ixc, iy, nx);
+        d2v up = {0.0, 0.0};
+        if (iy + 1 < ny) up = opd2(c + nx);
+        const d2v aPc = ld2<NT>(aP + c), aEc = ld2<NT>(aE + c), aNc = ld2<NT>(aN + c);
+        double ue = __shfl_down(uc.x, 1, WAVE);   // column ix + 2
+        double uw = __shfl_up(uc.y, 1, WAVE);     // column ix - 1
+        double aEw = __shfl_up(aEc.y, 1, WAVE);   // coupling (ix - 1, ix)
+        if (lane == WAVE - 1 && ix + 2 < nx) ue = opd1(c + 2);
+        if (lane == 0 && ix > 0) {
+            uw = opd1(c - 1);
+            aEw = aE[c - 1];
+        }
+        double y0 = aPc.x * uc.x;
+        if (ix < nx - 1) y0 -= aEc.x * uc.y;
+        if (ix > 0) y0 -= aEw * uw;
+        if (iy < ny - 1) y0 -= aNc.x * up.x;
+        if (iy > 0) y0 -= aNm.x * um.x;
+        double y1 = aPc.y * uc.y;
+        if (ix + 1 < nx - 1) y1 -= aEc.y * ue;
+        y1 -= aEc.x * uc.x;
+        if (iy < ny - 1) y1 -= aNc.y * up.y;
+        if (iy > 0) y1 -= aNm.y * um.y;
+        if (inx) epi2(c, d2v{y0, y1}, uc);
+        um = uc;
+        uc = up;
+        aNm = aNc;
+    }
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_matvec2(int nx, int ny, int R, const double* __restrict__ aP,
+                                                 const double* __restrict__ aE, const double* __restrict__ aN,
+                                                 const double* __restrict__ x, double* __restrict__ y) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    // the operand is re-read by the strip-edge lanes of the neighbouring strips: plain loads
+    march2<NT>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> d2v { return ld2<false>(x + k); },
+               [&](size_t k) -> double { return x[k]; }, [&](size_t c, d2v Au, d2v) { st2<NT>(y + c, Au); });
+}
+
 // pointwise walk over the same geometry
 template <class Epi>
 __device__ __forceinline__ void walk(int nx, int ny, int R, int vbx, int vby, Epi epi) {
@@ -968,6 +1047,32 @@ __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const 
     block_partial<1, false>(v, partials);
 }
 
+// k_cg_matvec on two columns per lane (FP64, even nx).
+template <bool NT>
+__global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const double* __restrict__ aP,
+                                                    const double* __restrict__ aE, const double* __restrict__ aN,
+                                                    const double* __restrict__ z, const double* __restrict__ p_old,
+                                                    double* __restrict__ p_new, double* __restrict__ Ap,
+                                                    double* partials, Scalars* sc) {
+    if (sc->done) return;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const double beta = sc->beta;
+    double acc = 0.0;
+    march2<NT>(
+        nx, ny, R, vbx, vby, aP, aE, aN,
+        [&](size_t j) -> d2v { return ld2<false>(z + j) + beta * ld2<false>(p_old + j); },
+        [&](size_t j) -> double { return z[j] + beta * p_old[j]; },
+        [&](size_t c, d2v Au, d2v u) {
+            st2<NT>(p_new + c, u);
+            st2<NT>(Ap + c, Au);
+            acc += u.x * Au.x;
+            acc += u.y * Au.y;
+        });
+    double v[1] = {acc};
+    block_partial<1, false>(v, partials);
+}
+
 // x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const T* __restrict__ dg,
@@ -1051,6 +1156,9 @@ __device__ __forceinline__ double from_east(double v) {
 __device__ __forceinline__ float from_west(float v) { return __int_as_float(dpp_west_i(__float_as_int(v))); }
 __device__ __forceinline__ float from_east(float v) { return __int_as_float(dpp_east_i(__float_as_int(v))); }
 
+// (Non-temporal hints were measured here as well: at 8192 x 2048 they slowed
+// this kernel from 320 to 452 us, the overlapping strips re-reading halo
+// columns the hint had let go.)
 template <class T, int H>
 __global__ __launch_bounds__(256) void k_cg_update_cheb(int nx, int ny, int R, const T* __restrict__ aP,
                                                         const T* __restrict__ aE, const T* __restrict__ aN,
@@ -1770,6 +1878,31 @@ Geom make_geom(int nx, int ny, int R_override) {
     return G;
 }
 
+// Geometry of the two-columns-per-lane FP64 stencil kernels: 128-column
+// strips, R chosen for >= 8192 waves as in make_geom.
+// Used where the arrays stream from HBM (stream_nt): inside the Infinity
+// Cache the one-column kernels measured faster (6.4 against 5.8 TB/s for the
+// operator at 4096 x 1024).
+bool wide_ok(const Geom& G) {
+    static const bool off = getenv("SHK_NO_WIDE") != nullptr;
+    return !off && G.nx % 2 == 0 && G.nx >= 2 && stream_nt(G);
+}
+Geom wide_geom(const Geom& G) {
+    Geom W = G;
+    const long strips = (G.nx + 2 * WAVE - 1) / (2 * WAVE);
+    long r = ((long)G.ny * strips) / 8192;
+    W.R = (int)(r < 1 ? 1 : (r > 32 ? 32 : r));
+    W.grid = dim3((unsigned)strips, (G.ny + WAVES * W.R - 1) / (WAVES * W.R));
+    W.nblocks = W.grid.x * W.grid.y;
+    return W;
+}
+// Non-temporal streaming once the operator's arrays (40 B per cell) exceed
+// the 256 MB Infinity Cache.
+bool stream_nt(const Geom& G) {
+    static const bool off = getenv("SHK_NO_NT") != nullptr;
+    return !off && (double)G.nx * G.ny * 40.0 > 256.0 * 1024 * 1024;
+}
+
 // Geometry of the fused update+Chebyshev kernel: strips of 64-2H owned
 // columns, R owned rows per wave.
 Geom fused_geom(const Geom& G, int H) {
@@ -1844,6 +1977,16 @@ void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, 
 }
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st) {
+    if constexpr (sizeof(T) == 8) {
+        if (wide_ok(G)) {
+            const Geom W = wide_geom(G);
+            if (stream_nt(G))
+                hipLaunchKernelGGL(k_matvec2<true>, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, x, y);
+            else
+                hipLaunchKernelGGL(k_matvec2<false>, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, x, y);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, x, y);
 }
 template <class T>
@@ -1890,6 +2033,19 @@ void cheb_degree1(const Geom& G, const T* dg, const T* r, T* y, int mode, double
 template <class T>
 void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
                double* partials, Scalars* sc, hipStream_t st) {
+    if constexpr (sizeof(T) == 8) {
+        if (wide_ok(G)) {
+            const Geom W = wide_geom(G);
+            if (stream_nt(G))
+                hipLaunchKernelGGL(k_cg_matvec2<true>, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
+                                   p_old, p_new, Ap, partials, sc);
+            else
+                hipLaunchKernelGGL(k_cg_matvec2<false>, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
+                                   p_old, p_new, Ap, partials, sc);
+            finish(W, partials, FIN_PAP, 0, sc, st);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_cg_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old, p_new, Ap,
                        partials, sc);
     finish(G, partials, FIN_PAP, 0, sc, st);

@@ -292,3 +292,46 @@ def test_multigrid_fp32_refinement_solve(oracle, hip, shape):
         assert rel_err(res[name][0], res["fp64"][0]) < 1e-9, name
         if shape != (6, 6):
             assert st.niter < res["fp64"][1].niter / 3, (name, st.niter, res["fp64"][1].niter)
+
+
+def test_streaming_kernels_on_a_grid_beyond_the_infinity_cache(hip):
+    """At 4096x2048 (8.4 M cells, 336 MB per operator apply) the FP64 operator
+    and CG kernels switch to the two-columns-per-lane, non-temporal variants.
+    The operator apply equals the reference's 5-array matvec bit for bit
+    (same products, same order), and the plain Chebyshev form (fused
+    non-temporal kernel) and the multigrid form reach the same head."""
+    from shakti_hip import synthetic as S
+    import ctypes as C
+    from shakti_hip import _lib as L
+    gg, raw = S.glacier(4096, 2048, bed_amp=10.0, seed=12)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    print("4096x2048: operator apply", flush=True)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    hs._bind(p=hp, kfs=hip.Harmonic())
+    L.check(L.load().shk_assemble(hs.ctx, 0, 0.0), hs.ctx)
+    n = (gg.nx, gg.ny)
+    arrs = [np.zeros(n, order="F") for _ in range(6)]
+    L.check(L.load().shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
+    aP, aE, aW, aN, aS, _ = arrs
+    rng = np.random.default_rng(3)
+    x = np.asfortranarray(rng.standard_normal(n))
+    y = np.zeros_like(x)
+    L.check(L.load().shk_apply_operator(hs.ctx, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                        y.ctypes.data_as(C.POINTER(C.c_double))), hs.ctx)
+    ref = aP * x
+    ref[:-1] -= aE[:-1] * x[1:]
+    ref[1:] -= aW[1:] * x[:-1]
+    ref[:, :-1] -= aN[:, :-1] * x[:, 1:]
+    ref[:, 1:] -= aS[:, 1:] * x[:, :-1]
+    assert np.array_equal(y, ref)
+    del hs
+    res = {}
+    for name, kw in [("cheb", dict(chebyshev_degree=4)), ("mg", dict(multigrid=True))]:
+        print(f"4096x2048: solve {name}", flush=True)
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, rtol=1e-11, check_every=32, **kw)
+        hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+        assert ls.ws.stats.solved
+        res[name] = hs.get("h")
+        del hs
+    assert rel_err(res["cheb"], res["mg"]) < 1e-9
