@@ -144,6 +144,13 @@ def gpu_run(args, g, raw, moulin, form):
     op["frac"] = op["achieved_GBs"] / HBM_PEAK_GBS
     L.check(L.load().shk_time_kernel(s.ctx, 2, 50, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
     op["copy_reference_GBs"] = nbytes.value / (mean_ms.value * 1e-3) / 1e9
+    # cold: each launch alone after a 1 GiB read has flushed L2 and the Infinity Cache
+    L.check(L.load().shk_time_kernel(s.ctx, 3, 10, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
+    op["cold_mean_ms"] = mean_ms.value
+    op["cold_GBs"] = nbytes.value / (mean_ms.value * 1e-3) / 1e9
+    op["cold_frac"] = op["cold_GBs"] / HBM_PEAK_GBS
+    L.check(L.load().shk_time_kernel(s.ctx, 4, 10, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
+    op["cold_copy_reference_GBs"] = nbytes.value / (mean_ms.value * 1e-3) / 1e9
     fields = {f: s.get(f) for f in ("h", "b", "N", "q_x", "q_y", "mdot", "K")}
     return dict(times=times, picard=picard, cg=cg, conv=conv, prof=prof, prof_step=prof_step,
                 prof_cg=ps.cg_iters, operator=op, fields=fields, state=s)

@@ -1127,27 +1127,53 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     double bytes = 0;
     auto one = [&]() {
         switch (kernel) {
-            case 0: launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, ctx->h_alt, ctx->Ap, ctx->st); break;
+            case 0:
+            case 3: launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, ctx->h_alt, ctx->Ap, ctx->st); break;
             case 1: launch::matvec<float>(G, ctx->aP32, ctx->aE32, ctx->aN32, ctx->x32, ctx->Ap32, ctx->st); break;
-            case 2: launch::copy(ctx->h_alt, ctx->d64, ctx->n, ctx->st); break;
+            case 2:
+            case 4: launch::copy(ctx->h_alt, ctx->d64, ctx->n, ctx->st); break;
             default: break;
         }
     };
-    if (kernel == 0) bytes = 5 * n * 8;
+    if (kernel == 0 || kernel == 3) bytes = 5 * n * 8;
     else if (kernel == 1) bytes = 5 * n * 4;
-    else if (kernel == 2) bytes = 2 * n * 8;
+    else if (kernel == 2 || kernel == 4) bytes = 2 * n * 8;
     else {
         hipEventDestroy(a);
         hipEventDestroy(b);
         return fail(ctx, SHK_ERR_ARG, "unknown kernel id");
     }
     one();  // warm
-    HIPCHK(hipEventRecord(a, ctx->st));
-    for (int i = 0; i < reps; ++i) one();
-    HIPCHK(hipEventRecord(b, ctx->st));
-    HIPCHK(hipEventSynchronize(b));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    if (kernel >= 3) {
+        // cold: stream a 1 GiB read before every launch, so L2 and the 256 MB
+        // Infinity Cache hold only clean lines of unrelated data; each launch
+        // is timed alone
+        const size_t nf = (size_t)1 << 27;
+        double* flush = nullptr;
+        double* sink = nullptr;
+        HIPCHK(hipMalloc((void**)&flush, nf * 8));
+        HIPCHK(hipMalloc((void**)&sink, 4096 * 8));
+        HIPCHK(hipMemsetAsync(flush, 0, nf * 8, ctx->st));
+        for (int i = 0; i < reps; ++i) {
+            launch::read_flush(nf, flush, sink, ctx->st);
+            HIPCHK(hipEventRecord(a, ctx->st));
+            one();
+            HIPCHK(hipEventRecord(b, ctx->st));
+            HIPCHK(hipEventSynchronize(b));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, a, b));
+            ms += t;
+        }
+        hipFree(flush);
+        hipFree(sink);
+    } else {
+        HIPCHK(hipEventRecord(a, ctx->st));
+        for (int i = 0; i < reps; ++i) one();
+        HIPCHK(hipEventRecord(b, ctx->st));
+        HIPCHK(hipEventSynchronize(b));
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+    }
     hipEventDestroy(a);
     hipEventDestroy(b);
     *mean_ms = ms / reps;

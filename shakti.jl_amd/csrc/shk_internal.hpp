@@ -69,6 +69,7 @@ struct MGLevelDev {
     float *b, *xa, *xb;
 };
 constexpr int MG_MAX_TAIL = 12;
+constexpr long MG_FLAT_CELLS = 65536;  // middle levels up to this size use the single-phase kernels
 constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; all tail levels (~5.5k cells) in 44 KB of LDS
 struct MGTail {
     int nlev;
@@ -175,6 +176,7 @@ void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const floa
 // r = b - A x with its Jacobi-weighted norm (first != 0 records the initial
 // norm) and its FP32 copy; the FP64 correction x += d.
 void to_f32(size_t n, const double* a, float* b, hipStream_t st);
+void read_flush(size_t n, const double* a, double* sink, hipStream_t st);
 // learned warm start (network input / output on the image padded to W8 x H8)
 void nn_resid_scale(size_t n, const double* aP, const double* rhs, const double* Ap, double* r, double* partials,
                     Scalars* sc, hipStream_t st);

@@ -1118,6 +1118,21 @@ __global__ __launch_bounds__(256) void k_copy(size_t n, const double* __restrict
     if (i < n) b[i] = a[i];
 }
 
+// Cache flush for cold timings: a grid-stride read of n doubles, one sum per
+// workgroup into sink (so the loads cannot be dropped).
+__global__ __launch_bounds__(256) void k_read_flush(size_t n, const double* __restrict__ a, double* __restrict__ sink) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) s += a[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < 256; ++k) t += red[k];
+        sink[blockIdx.x] = t;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_zero(size_t n, double* p) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -1490,6 +1505,51 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
         double v[1] = {acc};
         block_partial<1, false>(v, partials);
     }
+}
+
+// The same two legs for the middle levels (at most MG_FLAT_CELLS cells) in a
+// single phase, one thread per coarse cell (down) or fine cell (up), reading
+// the neighbours' values directly: these levels are too small to fill the
+// chip, and each barrier-separated phase of the tiled kernels adds a global
+// memory round trip to their latency.
+__global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, int nyc, float omega,
+                                                      const float* __restrict__ c, const float* __restrict__ e,
+                                                      const float* __restrict__ n, const float* __restrict__ D,
+                                                      const float* __restrict__ b, float* __restrict__ x,
+                                                      float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= nxc || J >= nyc) return;
+    auto xs = [&](size_t k) -> float { return omega * b[k] / D[k]; };
+    const int x0 = 2 * I, y0 = 2 * J;
+    const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
+    float acc = 0.f;
+    for (int iy = y0; iy <= y1; ++iy)
+        for (int ix = x0; ix <= x1; ++ix) {
+            const size_t k = (size_t)ix + (size_t)iy * nx;
+            x[k] = xs(k);
+            acc += b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+        }
+    bc[(size_t)I + (size_t)J * nxc] = acc;
+}
+__global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, float omega, float scale,
+                                                    const float* __restrict__ c, const float* __restrict__ e,
+                                                    const float* __restrict__ n, const float* __restrict__ D,
+                                                    const float* __restrict__ b, const float* __restrict__ x,
+                                                    const float* __restrict__ xc, float* __restrict__ out,
+                                                    const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    auto us = [&](size_t k) -> float {
+        const int kx = (int)(k % (size_t)nx), ky = (int)(k / (size_t)nx);
+        return x[k] + scale * xc[(size_t)(kx >> 1) + (size_t)(ky >> 1) * nxc];
+    };
+    const size_t k = (size_t)ix + (size_t)iy * nx;
+    const float u = us(k);
+    out[k] = u + omega * (b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, us)) / D[k];
 }
 
 // The small levels of the V-cycle, from the first level of at most
@@ -2094,6 +2154,13 @@ void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, flo
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
              const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+    if constexpr (sizeof(TB) == 4) {
+        if ((long)nx * ny <= MG_FLAT_CELLS) {
+            hipLaunchKernelGGL(k_mg_down_flat, cells_grid(nxc, nyc), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc,
+                               omega, c, e, n, D, b, x, bc, sc);
+            return;
+        }
+    }
     const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
     hipLaunchKernelGGL(k_mg_down<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D, b, x, bc,
                        sc);
@@ -2102,6 +2169,13 @@ template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st) {
+    if constexpr (sizeof(TB) == 4) {
+        if ((long)nx * ny <= MG_FLAT_CELLS && dot_mode < 0) {
+            hipLaunchKernelGGL(k_mg_up_flat, cells_grid(nx, ny), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale,
+                               c, e, n, D, b, x, xc, out, sc);
+            return;
+        }
+    }
     const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
     hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
                        out, dot_mode >= 0 ? 1 : 0, partials, sc);
@@ -2168,6 +2242,9 @@ void nn_ref(Scalars* sc, int mode, double n, hipStream_t st) {
     hipLaunchKernelGGL(k_nn_ref, dim3(1), dim3(64), 0, st, sc, mode, n);
 }
 
+void read_flush(size_t n, const double* a, double* sink, hipStream_t st) {
+    hipLaunchKernelGGL(k_read_flush, dim3(4096), dim3(256), 0, st, n, a, sink);
+}
 void cg_finish(int n, const double* x, const double* h_prev, double* h_new, double relax, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, x, h_prev, h_new, relax);
 }
