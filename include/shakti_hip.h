@@ -95,7 +95,8 @@ typedef struct {
     int32_t zero_curvature; /* 1 when p'Ap vanished (Krylov's zero-curvature exit) */
     int32_t bounds_warning; /* 1 when the Chebyshev bound estimate was rejected (previous bounds kept) */
     int32_t refinements;    /* FP32_REFINE outer iterations */
-    int32_t fallback;       /* LEARNED: 1 when the network's warm start was rejected (CG started from h) */
+    int32_t fallback;       /* LEARNED: bit 0 when the network's warm start was rejected (CG started
+                               from h); bit 1 when the learned preconditioner stalled (plain PCG after) */
     double rnorm0, rnorm;   /* preconditioned residual norm at start / end */
     double lambda_min, lambda_max; /* Chebyshev bounds in use */
 } shk_solve_stats;

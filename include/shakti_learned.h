@@ -155,6 +155,22 @@ int shk_dev_copy(void* dst, const void* src, size_t bytes, int32_t kind);
  * used elsewhere while attached. */
 int shk_attach_unet(shk_ctx* ctx, shk_unet* net);
 
+/* Learned preconditioner inside flexible CG (form SHK_FORM_LEARNED with the
+ * multigrid or Chebyshev preconditioner; no counterpart in the reference,
+ * whose CG takes a fixed preconditioner).  every = 0: the network gives the
+ * warm start only (default).  every >= 2 (rounded up to even): the solve
+ * queues batches of `every` CG iterations; the first iteration of each batch
+ * preconditions with z = g e + P(r - g A e), e the network's correction for
+ * r, g = e'r / e'Ae and P the fixed preconditioner, and that iteration and
+ * the next take the flexible beta = -z'Ap / p'Ap.  A learned batch whose
+ * final preconditioned residual norm is not below stall_ratio times the one
+ * at its start ends the learned batches: the solve goes on as plain PCG and
+ * shk_solve_stats.fallback gets bit 1 (value 2).  The stopping test is the
+ * fixed preconditioner's, checked on the iterations that use it.
+ * every in {0} U [2, 4096], stall_ratio in [0, 1000]; 0 makes every learned
+ * batch count as stalled (the first one ends them). */
+int shk_set_learned_precond(shk_ctx* ctx, int32_t every, double stall_ratio);
+
 /* Record training pairs: from now on every elliptic solve of the context
  * writes, into slot k = (number recorded so far), the network input of its
  * Picard iteration (BF16 [H8][W8][8] at x_dev + k*H8*W8*8) and the target
@@ -163,6 +179,17 @@ int shk_attach_unet(shk_ctx* ctx, shk_unet* net);
  * and 64.  x_dev == NULL stops recording; shk_recorded_pairs returns the
  * number of pairs written since recording started. */
 int shk_record_pairs(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity);
+
+/* Record training pairs for the learned preconditioner: from now on every
+ * learned CG iteration (form SHK_FORM_LEARNED with shk_set_learned_precond
+ * every >= 2) writes its network input (the Jacobi-scaled residual r_k /
+ * (sqrt(aP) s_k) with the coupling channels) into the next slot, and when the
+ * solve has converged the target sqrt(aP) (x_final - x_k) / s_k, the
+ * remaining error of that iteration.  Same layout and capacity rules as
+ * shk_record_pairs; the library holds capacity x n doubles of iterates until
+ * recording stops.  Solves run uncaptured while recording.  Only one kind of
+ * recording is active at a time; x_dev == NULL stops either. */
+int shk_record_cg_pairs(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity);
 int shk_recorded_pairs(shk_ctx* ctx, int32_t* recorded);
 
 #ifdef __cplusplus

@@ -104,6 +104,7 @@ struct shk_ctx {
         int key[4];
         hipGraph_t graph;
         hipGraphExec_t exec;
+        const shk_unet* net = nullptr;  // set for a batch that runs the attached network
     };
     std::vector<GraphEntry> graphs;
     bool use_graphs = true;
@@ -118,6 +119,16 @@ struct shk_ctx {
     void* rec_x = nullptr;
     float* rec_t = nullptr;
     int rec_cap = 0, rec_count = 0;
+    // rec_kind 0: one pair per Picard iteration; 1: one pair per learned CG
+    // iteration, whose target needs the iterate there (rec_snap, n per slot)
+    // and the image scale (rec_scale)
+    int rec_kind = 0;
+    double *rec_snap = nullptr, *rec_scale = nullptr;
+    // learned preconditioner inside flexible CG: one iteration in nn_every
+    // (0: warm start only); a learned batch whose residual norm does not fall
+    // below nn_stall x its value at the batch start ends the learned batches
+    int nn_every = 0;
+    double nn_stall = 1.0;
 
     std::string err;
 };
@@ -128,10 +139,10 @@ const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_ini
                             "cg_matvec",  "cg_update",    "cg_finish",   "derived",     "gap",
                             "stagger_b",  "ieb",          "ic",          "matvec",      "ir",
                             "cg_update_cheb", "mg_down",  "mg_up",       "mg_tail",     "mg_setup",
-                            "nn_forward", "nn_warm_start", "mg_down_fine", "mg_up_fine"};
+                            "nn_forward", "nn_warm_start", "mg_down_fine", "mg_up_fine", "nn_precond"};
 // mg_down / mg_up: the coarser levels; *_fine: the finest level
 enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR,
-              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NNF, P_NNW, P_MGD0, P_MGU0, P_NUM };
+              P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NNF, P_NNW, P_MGD0, P_MGU0, P_NNP, P_NUM };
 
 int fail(shk_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -436,11 +447,71 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
     }
 }
 
+// Learned preconditioner (FP64 CG only): z = gamma e + P(r - gamma A e), with
+// e the network's correction for r, gamma = e'r / e'Ae its energy-minimising
+// step and P the fixed preconditioner; then gamma = r'z and the flexible-CG
+// beta = -z'Ap / p'Ap.  r - gamma A e goes through w.r2 (unused unless the
+// fused Chebyshev kernel runs, which learned batches do not use).
+static int nn_precond(shk_ctx* ctx, const Work<double>& w, int precond, int degree) {
+    const size_t n = ctx->n;
+    {
+        Prof pr(ctx, P_NNP, 0);
+        launch::nn_rscale(n, ctx->aP, w.r, ctx->partials, ctx->sc, ctx->st);
+        launch::nn_features(ctx->g.nx, ctx->g.ny, ctx->W8, ctx->H8, ctx->aP, ctx->aE, ctx->aN, ctx->cP, w.r, ctx->sc,
+                            1, ctx->nn_x, ctx->st);
+    }
+    if (ctx->rec_kind == 1 && ctx->rec_x && ctx->rec_count < ctx->rec_cap) {
+        // training pair of this iteration: its input now, its target (the
+        // remaining error) once the solve has converged
+        const size_t k = (size_t)ctx->rec_count++;
+        const size_t img = (size_t)ctx->W8 * ctx->H8;
+        HIPCHK(hipMemcpyAsync((char*)ctx->rec_x + k * img * 16, ctx->nn_x, img * 16, hipMemcpyDeviceToDevice,
+                              ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->rec_snap + k * n, w.x, n * 8, hipMemcpyDeviceToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->rec_scale + k, &ctx->sc->nn_scale_pc, 8, hipMemcpyDeviceToDevice, ctx->st));
+    }
+    {
+        Prof pr(ctx, P_NNF, 0);
+        if (shknn::forward(ctx->net, ctx->nn_x, 1, ctx->H8, ctx->W8))
+            return fail(ctx, SHK_ERR_STATE, std::string("network: ") + shknn::unet_error(ctx->net));
+    }
+    {
+        Prof pr(ctx, P_NNP, 0);
+        launch::nn_correction(ctx->g.nx, ctx->g.ny, ctx->W8, shknn::output(ctx->net), ctx->aP, ctx->aE, ctx->aN,
+                              ctx->sc, 1, ctx->nn_e, ctx->st);
+        launch::matvec<double>(ctx->G, ctx->aP, ctx->aE, ctx->aN, ctx->nn_e, ctx->nn_Ae, ctx->st);
+        launch::nn_line_search(n, ctx->nn_e, w.r, ctx->nn_Ae, ctx->partials, ctx->sc, 1, ctx->st);
+        launch::nn_r1(n, w.r, ctx->nn_Ae, ctx->sc, w.r2, ctx->st);
+    }
+    if (precond == SHK_PRECOND_MULTIGRID) {
+        mg_apply<double>(ctx, w.r2, w.z, -1);
+    } else {
+        Work<double> w1 = w;
+        w1.r = w.r2;
+        cheb_apply<double>(ctx, w1, degree, -1);
+    }
+    Prof pr(ctx, P_NNP, 0);
+    launch::nn_combine(n, w.r, ctx->nn_e, w.z, ctx->partials, ctx->sc, ctx->st);
+    launch::fcg_beta(n, w.z, w.Ap, ctx->partials, ctx->sc, ctx->st);
+    return SHK_OK;
+}
+
 // Preconditioned CG on A x = b from x0 (nullptr: zero), Krylov cg! semantics.
 // The accumulated correction is left in w.x.  Returns SHK_OK or an error.
+//
+// nn_every > 0 (FP64 with an attached network, multigrid or Chebyshev):
+// flexible CG.  Each batch of nn_every iterations starts with one iteration
+// whose preconditioner is the learned one; that iteration and the next take
+// the flexible (truncated, one-vector) beta = -z'Ap / p'Ap, which keeps p
+// A-orthogonal to the previous direction when the preconditioner changes;
+// the others keep Krylov's beta, which equals it for a fixed preconditioner.
+// The first batch is plain PCG and sets the reference reduction rho_ref =
+// (residual norm at its end) / (at its start).  A learned batch whose own
+// reduction is not below nn_stall * rho_ref ends the learned batches: the
+// solve goes on as plain PCG from where it is (*nn_stalled = 1).
 template <class T>
 int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, int check_every, double atol,
-        double rtol, int itmax) {
+        double rtol, int itmax, int nn_every = 0, int* nn_stalled = nullptr) {
     const Geom& G = ctx->G;
     const double nb = (double)ctx->n * sizeof(T);
     const int jacobi = precond == SHK_PRECOND_JACOBI;
@@ -459,8 +530,15 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     }
     // an even batch returns the p ping-pong buffers to their starting roles,
     // so one captured batch can be replayed
-    const int batch = check_every + (check_every & 1);
-    const bool fused = precond == SHK_PRECOND_CHEBYSHEV && degree >= 2 && degree <= 8 && ctx->use_fused;
+    const bool nn = nn_every > 0;
+    if constexpr (sizeof(T) != 8) {
+        if (nn) return fail(ctx, SHK_ERR_STATE, "the learned preconditioner runs in FP64 CG only");
+    }
+    const int every = nn_every + (nn_every & 1);
+    const int batch = nn ? every : check_every + (check_every & 1);
+    const bool fused = precond == SHK_PRECOND_CHEBYSHEV && degree >= 2 && degree <= 8 && ctx->use_fused && !nn;
+    bool learned_batch = false;
+    int rc_nn = SHK_OK;
     auto enqueue_batch = [&]() {
         T* p_old = w.pA;
         T* p_new = w.pB;
@@ -485,10 +563,20 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                     launch::cg_update<T>(G, w.dg, p_new, w.Ap, w.x, w.r, w.z, jacobi, ctx->partials, ctx->sc,
                                          ctx->st);
                 }
-                if (multigrid) {
-                    mg_apply<T>(ctx, w.r, w.z, 1);
-                } else if (!jacobi) {
-                    cheb_apply<T>(ctx, w, degree, 1);
+                if constexpr (sizeof(T) == 8) {
+                    if (learned_batch && k == 0) {
+                        if (!rc_nn) rc_nn = nn_precond(ctx, w, precond, degree);
+                    } else {
+                        if (multigrid) mg_apply<T>(ctx, w.r, w.z, 1);
+                        else if (!jacobi) cheb_apply<T>(ctx, w, degree, 1);
+                        if (learned_batch && k == 1) {
+                            Prof pr(ctx, P_NNP, 0);
+                            launch::fcg_beta(ctx->n, w.z, w.Ap, ctx->partials, ctx->sc, ctx->st);
+                        }
+                    }
+                } else {
+                    if (multigrid) mg_apply<T>(ctx, w.r, w.z, 1);
+                    else if (!jacobi) cheb_apply<T>(ctx, w, degree, 1);
                 }
             }
             T* t = p_old;
@@ -496,32 +584,73 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             p_new = t;
         }
     };
-    hipGraphExec_t exec = nullptr;
-    if (ctx->use_graphs && !ctx->prof) {
-        const int key[4] = {(int)sizeof(T), precond, degree * (fused ? 100 : 1), batch};
+    // one graph per batch kind; the learned batch's graph holds the network's
+    // kernels too (they run on ctx->st), so it is keyed on the network and
+    // its image
+    auto batch_graph = [&](hipGraphExec_t* out) -> int {
+        *out = nullptr;
+        if (!ctx->use_graphs || ctx->prof || (ctx->rec_kind == 1 && ctx->rec_x)) return SHK_OK;
+        const int key[4] = {(int)sizeof(T), precond + (learned_batch ? 16 : 0), degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
-            if (!memcmp(e.key, key, sizeof(key))) exec = e.exec;
-        if (!exec) {
-            shk_ctx::GraphEntry e;
-            memcpy(e.key, key, sizeof(key));
-            HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-            enqueue_batch();
-            HIPCHK(hipStreamEndCapture(ctx->st, &e.graph));
-            HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
-            ctx->graphs.push_back(e);
-            exec = e.exec;
+            if (!memcmp(e.key, key, sizeof(key)) && (!learned_batch || e.net == ctx->net)) *out = e.exec;
+        if (*out) return SHK_OK;
+        shk_ctx::GraphEntry e;
+        memcpy(e.key, key, sizeof(key));
+        e.net = learned_batch ? ctx->net : nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+        enqueue_batch();
+        HIPCHK(hipStreamEndCapture(ctx->st, &e.graph));
+        if (rc_nn) {
+            hipGraphDestroy(e.graph);
+            return rc_nn;
         }
+        HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+        ctx->graphs.push_back(e);
+        *out = e.exec;
+        return SHK_OK;
+    };
+    hipGraphExec_t exec = nullptr;
+    rc = batch_graph(&exec);
+    if (rc) return rc;
+    if (nn) {
+        // the residual norm the first learned batch is measured against (and
+        // no network run when the start already meets the test)
+        rc = read_scalars(ctx);
+        if (rc) return rc;
+        if (ctx->sc_host->done) return SHK_OK;
     }
+    double rn_start = ctx->sc_host->rNorm, rho_ref = 0.0;
+    bool plain_done = false;
     long queued = 0;
     for (;;) {
         if (exec) HIPCHK(hipGraphLaunch(exec, ctx->st));
         else enqueue_batch();
+        if (rc_nn) return rc_nn;
         queued += batch;
         if (ctx->prof) prof_collect(ctx);
         rc = read_scalars(ctx);
         if (rc) return rc;
         if (ctx->sc_host->done) break;
         if (queued > (long)itmax + batch) break;  // cannot happen: tired sets done
+        if (nn && !plain_done) {
+            // the plain batch's reduction is the reference; learned batches follow
+            plain_done = true;
+            rho_ref = ctx->sc_host->rNorm / rn_start;
+            learned_batch = true;
+            rc = batch_graph(&exec);
+            if (rc) return rc;
+        } else if (learned_batch) {
+            const double rho = ctx->sc_host->rNorm / rn_start;
+            if (!(rho < ctx->nn_stall * rho_ref)) {
+                // stalled: plain PCG from here on (the learned iteration was the
+                // first of the batch, so no later iteration needs the flexible beta)
+                learned_batch = false;
+                if (nn_stalled) *nn_stalled = 1;
+                rc = batch_graph(&exec);
+                if (rc) return rc;
+            }
+        }
+        rn_start = ctx->sc_host->rNorm;
     }
     return SHK_OK;
 }
@@ -636,7 +765,13 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     const bool learned = cfg.form == SHK_FORM_LEARNED;
     if (learned && !ctx->net)
         return fail(ctx, SHK_ERR_STATE, "form SHK_FORM_LEARNED needs a network: call shk_attach_unet first");
-    const bool record = ctx->rec_x && ctx->rec_count < ctx->rec_cap;
+    const bool record = ctx->rec_kind == 0 && ctx->rec_x && ctx->rec_count < ctx->rec_cap;
+    const bool cheb_pc = cfg.precond == SHK_PRECOND_CHEBYSHEV;
+    if (ctx->rec_kind == 1 && ctx->rec_x && !(learned && ctx->nn_every > 0 && (multigrid || cheb_pc)))
+        return fail(ctx, SHK_ERR_STATE,
+                    "recording CG-iteration pairs needs SHK_FORM_LEARNED with a learned preconditioner period "
+                    "and the multigrid or Chebyshev preconditioner");
+    const int rec_first = ctx->rec_count;
     const bool need_c = multigrid || learned || record;
     {
         Prof pr(ctx, P_ASM, (need_c ? 18 : 17) * nb);
@@ -677,7 +812,7 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
             launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, x0, w.Ap, ctx->st);
             launch::nn_resid_scale(ctx->n, ctx->aP, ctx->rhs, w.Ap, w.r, ctx->partials, ctx->sc, ctx->st);
             launch::nn_features(ctx->g.nx, ctx->g.ny, ctx->W8, ctx->H8, ctx->aP, ctx->aE, ctx->aN, ctx->cP, w.r,
-                                ctx->sc, xin, ctx->st);
+                                ctx->sc, 0, xin, ctx->st);
         }
         if (learned) {
             // the preconditioned norm of r(h): the stopping test's reference,
@@ -695,15 +830,27 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
             }
             Prof pr(ctx, P_NNW, 0);
             launch::nn_correction(ctx->g.nx, ctx->g.ny, ctx->W8, shknn::output(ctx->net), ctx->aP, ctx->aE, ctx->aN,
-                                  ctx->sc, ctx->nn_e, ctx->st);
+                                  ctx->sc, 0, ctx->nn_e, ctx->st);
             launch::matvec<double>(G, ctx->aP, ctx->aE, ctx->aN, ctx->nn_e, ctx->nn_Ae, ctx->st);
-            launch::nn_line_search(ctx->n, ctx->nn_e, w.r, ctx->nn_Ae, ctx->partials, ctx->sc, ctx->st);
+            launch::nn_line_search(ctx->n, ctx->nn_e, w.r, ctx->nn_Ae, ctx->partials, ctx->sc, 0, ctx->st);
             launch::nn_warm(ctx->n, x0, ctx->nn_e, ctx->sc, ctx->hw, ctx->st);
             x0 = ctx->hw;
         }
     }
-    int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol, itmax);
+    // the learned preconditioner needs a fixed preconditioner to compose with
+    const int nn_every = (learned && (multigrid || cheb)) ? ctx->nn_every : 0;
+    int stalled = 0;
+    int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol, itmax,
+                         nn_every, &stalled);
     if (rc) return rc;
+    if (ctx->rec_kind == 1 && ctx->rec_x) {
+        // targets of the pairs this solve recorded: sqrt(aP) (x_final - x_k) / s_k
+        const size_t img = (size_t)ctx->W8 * ctx->H8;
+        for (int k = rec_first; k < ctx->rec_count; ++k)
+            launch::nn_target_cg(ctx->g.nx, ctx->g.ny, ctx->W8, ctx->H8, ctx->aP, ctx->x, ctx->rec_snap + (size_t)k * ctx->n,
+                                 ctx->rec_scale + k, ctx->rec_t + (size_t)k * img, ctx->st);
+        HIPCHK(hipStreamSynchronize(ctx->st));
+    }
     // h_new = x + x0 into the alternate buffer, then swap so F.f[SHK_H] is current
     double* h_prev = ctx->F.f[SHK_H];
     {
@@ -726,7 +873,7 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     const Scalars* h = ctx->sc_host;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
-        stats->fallback = learned ? h->nn_fallback : 0;
+        stats->fallback = learned ? (h->nn_fallback | (stalled ? 2 : 0)) : 0;
         stats->niter = h->iter;
         stats->solved = h->solved;
         stats->zero_curvature = h->zero_curv;
@@ -865,7 +1012,7 @@ int shk_destroy(shk_ctx* ctx) {
     if (ctx->mg_inv) hipFree(ctx->mg_inv);
     if (ctx->mg_work) hipFree(ctx->mg_work);
     if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
-    void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw};
+    void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw, ctx->rec_snap, ctx->rec_scale};
     for (void* p : nnp)
         if (p) hipFree(p);
     for (auto& pe : ctx->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
@@ -1228,6 +1375,17 @@ int shk_attach_unet(shk_ctx* ctx, shk_unet* net) {
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
     ctx->net = nullptr;
+    // captured learned batches hold the previous network's buffers
+    for (size_t i = 0; i < ctx->graphs.size();) {
+        auto& e = ctx->graphs[i];
+        if (e.net) {
+            hipGraphExecDestroy(e.exec);
+            hipGraphDestroy(e.graph);
+            ctx->graphs.erase(ctx->graphs.begin() + i);
+        } else {
+            ++i;
+        }
+    }
     if (!net) return SHK_OK;
     if (!shknn::fits(net, ctx->H8, ctx->W8))
         return fail(ctx, SHK_ERR_ARG,
@@ -1240,23 +1398,53 @@ int shk_attach_unet(shk_ctx* ctx, shk_unet* net) {
     return SHK_OK;
 }
 
-int shk_record_pairs(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity) {
+int shk_set_learned_precond(shk_ctx* ctx, int32_t every, double stall_ratio) {
+    if (!ctx) return SHK_ERR_ARG;
+    if (every < 0 || every == 1 || every > 4096)
+        return fail(ctx, SHK_ERR_ARG, "learned preconditioner period must be 0 (off) or 2..4096 iterations");
+    if (!(stall_ratio >= 0.0) || !(stall_ratio <= 1e3))
+        return fail(ctx, SHK_ERR_ARG, "stall ratio must lie in [0, 1000]");
+    ctx->nn_every = every;
+    ctx->nn_stall = stall_ratio;
+    return SHK_OK;
+}
+
+static void rec_release(shk_ctx* ctx) {
+    if (ctx->rec_snap) hipFree(ctx->rec_snap);
+    if (ctx->rec_scale) hipFree(ctx->rec_scale);
+    ctx->rec_snap = ctx->rec_scale = nullptr;
+    ctx->rec_x = nullptr;
+    ctx->rec_t = nullptr;
+    ctx->rec_cap = ctx->rec_count = 0;
+    ctx->rec_kind = 0;
+}
+
+static int record_start(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity, int kind) {
     if (!ctx) return SHK_ERR_ARG;
     HIPCHK(hipStreamSynchronize(ctx->st));
-    if (!x_dev) {
-        ctx->rec_x = nullptr;
-        ctx->rec_t = nullptr;
-        ctx->rec_cap = ctx->rec_count = 0;
-        return SHK_OK;
-    }
+    rec_release(ctx);
+    if (!x_dev) return SHK_OK;
     if (!t_dev || capacity < 1) return fail(ctx, SHK_ERR_ARG, "record buffers need a target buffer and capacity >= 1");
     int rc = nn_buffers(ctx);
     if (rc) return rc;
+    if (kind == 1) {
+        HIPCHK(hipMalloc((void**)&ctx->rec_snap, (size_t)capacity * ctx->n * 8 + 256));
+        HIPCHK(hipMalloc((void**)&ctx->rec_scale, (size_t)capacity * 8 + 256));
+    }
     ctx->rec_x = x_dev;
     ctx->rec_t = t_dev;
     ctx->rec_cap = capacity;
     ctx->rec_count = 0;
+    ctx->rec_kind = kind;
     return SHK_OK;
+}
+
+int shk_record_pairs(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity) {
+    return record_start(ctx, x_dev, t_dev, capacity, 0);
+}
+
+int shk_record_cg_pairs(shk_ctx* ctx, void* x_dev, float* t_dev, int32_t capacity) {
+    return record_start(ctx, x_dev, t_dev, capacity, 1);
 }
 
 int shk_recorded_pairs(shk_ctx* ctx, int32_t* recorded) {

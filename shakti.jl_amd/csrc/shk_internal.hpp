@@ -55,6 +55,8 @@ struct Scalars {
     // (0: use the CG start's own norm, as Krylov does)
     double nn_scale, nn_gamma, rnorm0_ref;
     int32_t nn_fallback, nn_pad;
+    // learned preconditioner inside flexible CG: image scale and step
+    double nn_scale_pc, nn_gamma_pc;
     // block-completion counter of the running reduction kernel
     uint32_t counter;
     uint32_t pad;
@@ -180,16 +182,25 @@ void read_flush(size_t n, const double* a, double* sink, hipStream_t st);
 // learned warm start (network input / output on the image padded to W8 x H8)
 void nn_resid_scale(size_t n, const double* aP, const double* rhs, const double* Ap, double* r, double* partials,
                     Scalars* sc, hipStream_t st);
+// slot 0: the warm start's scale and step; slot 1: the preconditioner's
 void nn_features(int nx, int ny, int W8, int H8, const double* aP, const double* aE, const double* aN,
-                 const double* cP, const double* r, const Scalars* sc, void* x, hipStream_t st);
+                 const double* cP, const double* r, const Scalars* sc, int slot, void* x, hipStream_t st);
 void nn_correction(int nx, int ny, int W8, const float* N, const double* aP, const double* aE, const double* aN,
-                   const Scalars* sc, double* e, hipStream_t st);
+                   const Scalars* sc, int slot, double* e, hipStream_t st);
 void nn_line_search(size_t n, const double* e, const double* r, const double* Ae, double* partials, Scalars* sc,
-                    hipStream_t st);
+                    int slot, hipStream_t st);
 void nn_warm(size_t n, const double* h, const double* e, const Scalars* sc, double* x0, hipStream_t st);
 void nn_target(int nx, int ny, int W8, int H8, const double* aP, const double* h_new, const double* h_prev,
                const Scalars* sc, float* t, hipStream_t st);
 void nn_ref(Scalars* sc, int mode, double n, hipStream_t st);
+// learned preconditioner inside flexible CG
+void nn_target_cg(int nx, int ny, int W8, int H8, const double* aP, const double* x_final, const double* x_k,
+                  const double* s_k, float* t, hipStream_t st);
+void nn_rscale(size_t n, const double* aP, const double* r, double* partials, Scalars* sc, hipStream_t st);
+void nn_r1(size_t n, const double* r, const double* Ae, const Scalars* sc, double* r1, hipStream_t st);
+void nn_combine(size_t n, const double* r, const double* e, double* z, double* partials, Scalars* sc,
+                hipStream_t st);
+void fcg_beta(size_t n, const double* z, const double* Ap, double* partials, Scalars* sc, hipStream_t st);
 void ir_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
                  const double* x, float* r32, int first, double* partials, Scalars* sc, hipStream_t st);
 void ir_correct(size_t n, double* x, const float* d, hipStream_t st);

@@ -1731,7 +1731,8 @@ __global__ __launch_bounds__(256) void k_nn_resid(size_t n, const double* __rest
 __global__ __launch_bounds__(256) void k_nn_features(int nx, int ny, int W8, int H8, const double* __restrict__ aP,
                                                      const double* __restrict__ aE, const double* __restrict__ aN,
                                                      const double* __restrict__ cP, const double* __restrict__ r,
-                                                     const Scalars* __restrict__ sc, nn_bf16x8* __restrict__ x) {
+                                                     const Scalars* __restrict__ sc, int slot,
+                                                     nn_bf16x8* __restrict__ x) {
     const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= (size_t)W8 * H8) return;
     const int ix = (int)(p % W8), iy = (int)(p / W8);
@@ -1739,7 +1740,7 @@ __global__ __launch_bounds__(256) void k_nn_features(int nx, int ny, int W8, int
     if (ix < nx && iy < ny) {
         const size_t k = (size_t)ix + (size_t)iy * nx;
         const double d = aP[k], sd = sqrt(d);
-        const double s = sc->nn_scale;
+        const double s = slot ? sc->nn_scale_pc : sc->nn_scale;
         o[0] = (__bf16)(s > 0.0 ? r[k] / (sd * s) : 0.0);
         o[1] = (__bf16)(ix < nx - 1 ? 2.0 * aE[k] / sqrt(d * aP[k + 1]) : 0.0);
         o[2] = (__bf16)(ix > 0 ? 2.0 * aE[k - 1] / sqrt(d * aP[k - 1]) : 0.0);
@@ -1751,17 +1752,19 @@ __global__ __launch_bounds__(256) void k_nn_features(int nx, int ny, int W8, int
 }
 
 // e = s N / sqrt(aP) on cells with at least one coupling (0 on identity and
-// isolated rows, whose solution the warm start must not move).
+// isolated rows, whose solution the warm start must not move).  slot 0 takes
+// the warm start's scale, 1 the preconditioner's.
 __global__ __launch_bounds__(256) void k_nn_correction(int nx, int ny, int W8, const float* __restrict__ N,
                                                        const double* __restrict__ aP, const double* __restrict__ aE,
                                                        const double* __restrict__ aN, const Scalars* __restrict__ sc,
-                                                       double* __restrict__ e) {
+                                                       int slot, double* __restrict__ e) {
     const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= (size_t)nx * ny) return;
     const int ix = (int)(k % nx), iy = (int)(k / nx);
     const bool coupled =
         aE[k] != 0.0 || aN[k] != 0.0 || (ix > 0 && aE[k - 1] != 0.0) || (iy > 0 && aN[k - nx] != 0.0);
-    e[k] = coupled ? sc->nn_scale * (double)N[(size_t)ix + (size_t)iy * W8] / sqrt(aP[k]) : 0.0;
+    const double s = slot ? sc->nn_scale_pc : sc->nn_scale;
+    e[k] = coupled ? s * (double)N[(size_t)ix + (size_t)iy * W8] / sqrt(aP[k]) : 0.0;
 }
 
 // Partials of e'r and e'Ae for the energy-minimising step.
@@ -1805,11 +1808,82 @@ __global__ __launch_bounds__(256) void k_nn_target(int nx, int ny, int W8, int H
     t[p] = v;
 }
 
+// Training target of a learned CG iteration: sqrt(aP) (x_final - x_k) / s_k,
+// the remaining error at iteration k in the network's scaling.
+__global__ __launch_bounds__(256) void k_nn_target_cg(int nx, int ny, int W8, int H8, const double* __restrict__ aP,
+                                                      const double* __restrict__ x_final,
+                                                      const double* __restrict__ x_k,
+                                                      const double* __restrict__ s_k, float* __restrict__ t) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)W8 * H8) return;
+    const int ix = (int)(p % W8), iy = (int)(p / W8);
+    float v = 0.f;
+    const double s = *s_k;
+    if (ix < nx && iy < ny && s > 0.0) {
+        const size_t k = (size_t)ix + (size_t)iy * nx;
+        v = (float)(sqrt(aP[k]) * (x_final[k] - x_k[k]) / s);
+    }
+    t[p] = v;
+}
+
 // Reference norm of the stopping test: sqrt(r'z) of the preconditioner just
 // applied to r(h) (mode 0), or the Jacobi norm s sqrt(n) (mode 1).
 __global__ void k_nn_ref(Scalars* sc, int mode, double n) {
     if (threadIdx.x != 0) return;
     sc->rnorm0_ref = mode == 0 ? sqrt(sc->gamma) : sc->nn_scale * sqrt(n);
+}
+
+// Learned preconditioner inside flexible CG (one iteration in m):
+//   s = rms of r / sqrt(aP);  e = network correction for r;  gamma = e'r / e'Ae;
+//   z = gamma e + P(r - gamma A e)   (P: the V-cycle or the Chebyshev polynomial).
+// A rejected step (gamma = 0) leaves z = P(r).  These kernels return at once
+// after convergence, like the CG kernels.
+__global__ __launch_bounds__(256) void k_nn_rscale(size_t n, const double* __restrict__ aP,
+                                                   const double* __restrict__ r, double* partials,
+                                                   const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * WAVE * WAVES + tid; i < n; i += stride) acc += r[i] * r[i] / aP[i];
+    double a[1] = {acc};
+    block_partial<1, false>(a, partials);
+}
+__global__ __launch_bounds__(256) void k_nn_r1(size_t n, const double* __restrict__ r, const double* __restrict__ Ae,
+                                               const Scalars* __restrict__ sc, double* __restrict__ r1) {
+    if (sc->done) return;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double g = sc->nn_gamma_pc;
+    r1[i] = g != 0.0 ? r[i] - g * Ae[i] : r[i];
+}
+__global__ __launch_bounds__(256) void k_nn_combine(size_t n, const double* __restrict__ r,
+                                                    const double* __restrict__ e, double* __restrict__ z,
+                                                    double* partials, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    const double g = sc->nn_gamma_pc;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * WAVE * WAVES + tid; i < n; i += stride) {
+        const double zi = g != 0.0 ? z[i] + g * e[i] : z[i];
+        z[i] = zi;
+        acc += r[i] * zi;
+    }
+    double a[1] = {acc};
+    block_partial<1, false>(a, partials);
+}
+// Partial of a'b (the flexible-CG beta numerator z'Ap).
+__global__ __launch_bounds__(256) void k_dot_partial(size_t n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* partials,
+                                                     const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * WAVE * WAVES + tid; i < n; i += stride) acc += a[i] * b[i];
+    double v[1] = {acc};
+    block_partial<1, false>(v, partials);
 }
 
 // ------------------------------------------------------- finishing kernel
@@ -1826,6 +1900,10 @@ enum FinMode {
     FIN_IR = 8,          // refinement: FP64 residual norm -> done flag
     FIN_NN_SCALE = 9,    // learned: image scale s = sqrt(sum r^2/aP / k)
     FIN_NN_LS = 10,      // learned: gamma = e'r / e'Ae (0 when not a descent step)
+    FIN_FCG_BETA = 11,   // flexible CG (truncation 1): beta = -z'Ap / p'Ap
+    FIN_GAMMA_NN = 12,   // gamma = r'z after the learned preconditioner (no convergence test)
+    FIN_NN_LS_PC = 13,   // learned preconditioner: gamma_pc = e'r / e'Ae (0 when e'Ae <= 0)
+    FIN_NN_SCALE_PC = 14,  // learned preconditioner: image scale of r
 };
 
 // One workgroup: sums (or maximises) the nb partials of the preceding
@@ -1834,9 +1912,12 @@ enum FinMode {
 // (that kernel returned early and wrote no partials).
 __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partials, int nb, int mode, int k,
                                                 Scalars* sc) {
-    if ((mode == FIN_GAMMA_ITER || mode == FIN_PAP) && sc->done) return;
+    if ((mode == FIN_GAMMA_ITER || mode == FIN_PAP || mode == FIN_FCG_BETA || mode == FIN_GAMMA_NN ||
+         mode == FIN_NN_LS_PC || mode == FIN_NN_SCALE_PC) &&
+        sc->done)
+        return;
     const bool is_max = mode == FIN_PICARD;
-    const int nv = (is_max || mode == FIN_NN_LS) ? 2 : 1;
+    const int nv = (is_max || mode == FIN_NN_LS || mode == FIN_NN_LS_PC) ? 2 : 1;
     const int tid = threadIdx.x + threadIdx.y * WAVE;
     double acc[2] = {0.0, 0.0};
     for (int i = 0; i < nv; ++i) {
@@ -1891,6 +1972,28 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
             break;
         }
         case FIN_NN_SCALE: sc->nn_scale = sqrt(v0 / (double)k); break;
+        case FIN_FCG_BETA: sc->beta = -v0 / sc->pAp; break;
+        case FIN_NN_SCALE_PC: sc->nn_scale_pc = sqrt(v0 / (double)k); break;
+        case FIN_NN_LS_PC: {
+            // any finite step along e lowers the energy by (e'r)^2 / e'Ae
+            const double g = v0 / v1;
+            sc->nn_gamma_pc = (v1 > 0.0 && isfinite(g)) ? g : 0.0;
+            break;
+        }
+        case FIN_GAMMA_NN: {
+            // r'z of the composite preconditioner: alpha = r'z / p'Ap is still
+            // the exact line search along p (r is orthogonal to the previous p),
+            // whatever its sign; the convergence test waits for the next
+            // iteration, whose norm is the fixed preconditioner's
+            const double beta = v0 / sc->gamma;
+            sc->pNorm2 = fabs(v0) + beta * beta * sc->pNorm2;
+            sc->gamma = v0;
+            sc->beta = beta;
+            sc->iter += 1;
+            sc->tired = sc->iter >= sc->itmax;
+            sc->done = sc->tired;
+            break;
+        }
         case FIN_NN_LS: {
             const double g = v0 / v1;
             const bool ok = v1 > 0.0 && v0 > 0.0 && isfinite(g);
@@ -2082,13 +2185,13 @@ void cheb_pass(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg
                hipStream_t st) {
     hipLaunchKernelGGL(k_cheb_pass<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, r, s_in, pc_in,
                        s_out, pc_out, y, k, degree, mode, partials, sc);
-    if (k == degree - 1) finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
+    if (k == degree - 1 && mode >= 0) finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
 void cheb_degree1(const Geom& G, const T* dg, const T* r, T* y, int mode, double* partials, Scalars* sc,
                   hipStream_t st) {
     hipLaunchKernelGGL(k_cheb_deg1<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, dg, r, y, mode, partials, sc);
-    finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
+    if (mode >= 0) finish(G, partials, mode == 0 ? FIN_GAMMA_INIT : FIN_GAMMA_ITER, 0, sc, st);
 }
 template <class T>
 void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
@@ -2212,22 +2315,23 @@ void nn_resid_scale(size_t n, const double* aP, const double* rhs, const double*
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_NN_SCALE, (int)n, sc);
 }
 void nn_features(int nx, int ny, int W8, int H8, const double* aP, const double* aE, const double* aN,
-                 const double* cP, const double* r, const Scalars* sc, void* x, hipStream_t st) {
+                 const double* cP, const double* r, const Scalars* sc, int slot, void* x, hipStream_t st) {
     const size_t np = (size_t)W8 * H8;
     hipLaunchKernelGGL(k_nn_features, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, nx, ny, W8, H8, aP, aE,
-                       aN, cP, r, sc, (nn_bf16x8*)x);
+                       aN, cP, r, sc, slot, (nn_bf16x8*)x);
 }
 void nn_correction(int nx, int ny, int W8, const float* N, const double* aP, const double* aE, const double* aN,
-                   const Scalars* sc, double* e, hipStream_t st) {
+                   const Scalars* sc, int slot, double* e, hipStream_t st) {
     const size_t n = (size_t)nx * ny;
     hipLaunchKernelGGL(k_nn_correction, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nx, ny, W8, N, aP, aE,
-                       aN, sc, e);
+                       aN, sc, slot, e);
 }
 void nn_line_search(size_t n, const double* e, const double* r, const double* Ae, double* partials, Scalars* sc,
-                    hipStream_t st) {
+                    int slot, hipStream_t st) {
     const unsigned nb = nn_blocks(n);
     hipLaunchKernelGGL(k_nn_ls, dim3(nb), dim3(WAVE, WAVES), 0, st, n, e, r, Ae, partials);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_NN_LS, 0, sc);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb,
+                       (int)(slot ? FIN_NN_LS_PC : FIN_NN_LS), 0, sc);
 }
 void nn_warm(size_t n, const double* h, const double* e, const Scalars* sc, double* x0, hipStream_t st) {
     hipLaunchKernelGGL(k_nn_warm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, h, e, sc, x0);
@@ -2237,6 +2341,32 @@ void nn_target(int nx, int ny, int W8, int H8, const double* aP, const double* h
     const size_t np = (size_t)W8 * H8;
     hipLaunchKernelGGL(k_nn_target, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, nx, ny, W8, H8, aP, h_new,
                        h_prev, sc, t);
+}
+void nn_target_cg(int nx, int ny, int W8, int H8, const double* aP, const double* x_final, const double* x_k,
+                  const double* s_k, float* t, hipStream_t st) {
+    const size_t np = (size_t)W8 * H8;
+    hipLaunchKernelGGL(k_nn_target_cg, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, nx, ny, W8, H8, aP,
+                       x_final, x_k, s_k, t);
+}
+void nn_rscale(size_t n, const double* aP, const double* r, double* partials, Scalars* sc, hipStream_t st) {
+    const unsigned nb = nn_blocks(n);
+    hipLaunchKernelGGL(k_nn_rscale, dim3(nb), dim3(WAVE, WAVES), 0, st, n, aP, r, partials, sc);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_NN_SCALE_PC, (int)n,
+                       sc);
+}
+void nn_r1(size_t n, const double* r, const double* Ae, const Scalars* sc, double* r1, hipStream_t st) {
+    hipLaunchKernelGGL(k_nn_r1, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, r, Ae, sc, r1);
+}
+void nn_combine(size_t n, const double* r, const double* e, double* z, double* partials, Scalars* sc,
+                hipStream_t st) {
+    const unsigned nb = nn_blocks(n);
+    hipLaunchKernelGGL(k_nn_combine, dim3(nb), dim3(WAVE, WAVES), 0, st, n, r, e, z, partials, sc);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_GAMMA_NN, 0, sc);
+}
+void fcg_beta(size_t n, const double* z, const double* Ap, double* partials, Scalars* sc, hipStream_t st) {
+    const unsigned nb = nn_blocks(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(WAVE, WAVES), 0, st, n, z, Ap, partials, sc);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)nb, (int)FIN_FCG_BETA, 0, sc);
 }
 void nn_ref(Scalars* sc, int mode, double n, hipStream_t st) {
     hipLaunchKernelGGL(k_nn_ref, dim3(1), dim3(64), 0, st, sc, mode, n);

@@ -33,7 +33,8 @@ EXPORTS_LEARNED = ["shk_unet_create", "shk_unet_destroy", "shk_unet_last_error",
                    "shk_unet_layer_info", "shk_unet_init_params", "shk_unet_set_params", "shk_unet_get_params",
                    "shk_unet_get_grads", "shk_unet_forward", "shk_unet_loss", "shk_unet_backward", "shk_unet_adam",
                    "shk_unet_train_step", "shk_unet_download_tensor", "shk_dev_alloc", "shk_dev_free",
-                   "shk_dev_copy", "shk_attach_unet", "shk_record_pairs", "shk_recorded_pairs"]
+                   "shk_dev_copy", "shk_attach_unet", "shk_set_learned_precond", "shk_record_pairs",
+                   "shk_record_cg_pairs", "shk_recorded_pairs"]
 
 
 class ShkGrid(C.Structure):
@@ -155,7 +156,9 @@ def load() -> C.CDLL:
     L.shk_dev_free.argtypes = [vp]
     L.shk_dev_copy.argtypes = [vp, vp, sz, i32]
     L.shk_attach_unet.argtypes = [vp, vp]
+    L.shk_set_learned_precond.argtypes = [vp, i32, dbl]
     L.shk_record_pairs.argtypes = [vp, vp, vp, i32]
+    L.shk_record_cg_pairs.argtypes = [vp, vp, vp, i32]
     L.shk_recorded_pairs.argtypes = [vp, C.POINTER(i32)]
     _lib = L
     return L

@@ -196,7 +196,7 @@ class State:
     PROFILE_IDS = ["assemble", "lanczos", "cheb_bounds", "cg_init", "cheb_pass", "cg_matvec", "cg_update",
                    "cg_finish", "derived", "gap", "stagger_b", "ieb", "ic", "matvec", "ir", "cg_update_cheb",
                    "mg_down", "mg_up", "mg_tail", "mg_setup", "nn_forward", "nn_warm_start", "mg_down_fine",
-                   "mg_up_fine"]
+                   "mg_up_fine", "nn_precond"]
 
     def profile_enable(self, kernels=None) -> None:
         """kernels: None for all, [] to switch off, or a list of kernel names."""
@@ -287,7 +287,7 @@ class _Stats:
     rnorm0: float = 0.0
     rnorm: float = 0.0
     refinements: int = 0
-    fallback: bool = False
+    fallback: int = 0  # bit 0: warm start rejected; bit 1: learned preconditioner stalled
 
 
 class _Workspace:
@@ -330,12 +330,19 @@ class CGIterativeSolver(AbstractIterativeSolver):
     ``form="learned"`` with ``network=`` a trained ``learned.UNet`` runs FP64
     CG (with the chosen preconditioner) from the network's warm start; the
     stopping test still refers to the residual at the Picard iterate h, and
-    a warm start that is not a descent step is dropped (``ws.stats.fallback``)."""
+    a warm start that is not a descent step is dropped (``ws.stats.fallback``
+    bit 0).  ``learned_precond_every=m`` (m >= 2) also makes the network a
+    preconditioner inside flexible CG: one iteration in m preconditions with
+    the network's correction composed with the fixed preconditioner; a
+    batch of m iterations that does not lower the residual norm below
+    ``stall_ratio`` times its start value ends that, and the solve goes on as
+    plain PCG (``ws.stats.fallback`` bit 1)."""
 
     def __init__(self, g: Grid, system=HIPMatrixFreeLinearSystem, chebyshev_degree: int | None = None,
                  chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
                  rtol: float = SQRT_EPS, itmax: int = 0, form: str = "fp64", check_every: int = 8,
-                 inner_rtol: float = 1e-3, multigrid: bool = False, network=None):
+                 inner_rtol: float = 1e-3, multigrid: bool = False, network=None,
+                 learned_precond_every: int = 0, stall_ratio: float = 1.0):
         if system is SparseAssembledLinearSystem:
             raise L.ShaktiHIPError("CGIterativeSolver(g, SparseAssembledLinearSystem) is CPU-only; use "
                                    "HIPMatrixFreeLinearSystem on the GPU.")
@@ -355,7 +362,13 @@ class CGIterativeSolver(AbstractIterativeSolver):
         if (form == "learned") != (network is not None):
             raise L.ShaktiHIPError("form='learned' takes the trained network (shakti_hip.learned.UNet) as "
                                    "network=, and only that form does")
+        if learned_precond_every and form != "learned":
+            raise L.ShaktiHIPError("learned_precond_every needs form='learned' and its network")
+        if learned_precond_every and not multigrid and chebyshev_degree is None:
+            raise L.ShaktiHIPError("the learned preconditioner composes with multigrid=True or chebyshev_degree")
         self.network = network
+        self.learned_precond_every = int(learned_precond_every)
+        self.stall_ratio = float(stall_ratio)
         self.multigrid = bool(multigrid)
         self.grid = g
         self.chebyshev_degree = chebyshev_degree
@@ -375,7 +388,7 @@ class CGIterativeSolver(AbstractIterativeSolver):
     def _record(self, st: L.ShkSolveStats):
         self.last_stats = st
         self.ws.stats = _Stats(st.niter, bool(st.solved), bool(st.zero_curvature), st.rnorm0, st.rnorm,
-                               st.refinements, bool(st.fallback))
+                               st.refinements, int(st.fallback))
         if self.precond is not None:
             self.precond.lambda_min, self.precond.lambda_max = st.lambda_min, st.lambda_max
             if st.bounds_warning:
@@ -401,6 +414,8 @@ def _bind_solver(s: State, ls: CGIterativeSolver):
         L.check(L.load().shk_attach_unet(s.ctx, handle), s.ctx, "shk_attach_unet")
         s._attached = handle
         s._attached_net = net  # the context holds the handle: keep the network alive
+    L.check(L.load().shk_set_learned_precond(s.ctx, getattr(ls, "learned_precond_every", 0),
+                                             getattr(ls, "stall_ratio", 1.0)), s.ctx, "shk_set_learned_precond")
 
 
 def solve_elliptic_linear_system(ls: CGIterativeSolver, s: State, g: Grid, p: ModelParameters,

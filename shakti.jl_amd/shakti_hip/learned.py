@@ -207,13 +207,23 @@ class UNet:
 class PairRecorder:
     """Makes the State's elliptic solves fill device buffers with (network
     input, target) pairs: x bf16 [capacity, H8, W8, 8], t f32 [capacity, H8,
-    W8], written from entry ``offset`` on, at most ``capacity`` pairs."""
+    W8], written from entry ``offset`` on, at most ``capacity`` pairs.
 
-    def __init__(self, state, x: DeviceArray, t: DeviceArray, offset: int, capacity: int):
+    kind "picard": one pair per Picard iteration (input: the residual at h;
+    target: the scaled correction h_new - h).  kind "cg": one pair per
+    learned CG iteration of a learned-preconditioner solve (input: that
+    iteration's residual; target: its remaining error)."""
+
+    KINDS = ("picard", "cg")
+
+    def __init__(self, state, x: DeviceArray, t: DeviceArray, offset: int, capacity: int, kind: str = "picard"):
         if offset + capacity > x.shape[0] or offset + capacity > t.shape[0]:
             raise ValueError("recording past the end of the buffers")
+        if kind not in self.KINDS:
+            raise ValueError(f"unknown pair kind {kind!r}")
         self.state = state
-        L.check(L.load().shk_record_pairs(state.ctx, x.at(offset), t.at(offset), capacity), state.ctx, "record")
+        start = L.load().shk_record_pairs if kind == "picard" else L.load().shk_record_cg_pairs
+        L.check(start(state.ctx, x.at(offset), t.at(offset), capacity), state.ctx, "record")
 
     @property
     def count(self) -> int:
@@ -239,11 +249,15 @@ def glacier_variant(k: int, nx: int, ny: int, seed0: int = 1000):
 
 
 def generate_pairs(n_glaciers: int = 64, nx: int = 512, ny: int = 128, per_glacier: int = 320,
-                   max_steps: int = 200, seed0: int = 1000, rtol: float = 1e-10, log=print):
-    """Run the plain solver (FP64 CG with the multigrid preconditioner) over
-    seeded glaciers and record one pair per Picard iteration.  Returns the
-    device buffers (x, t) and the number of pairs of each glacier (glacier k
-    holds entries k*per_glacier ...)."""
+                   max_steps: int = 200, seed0: int = 1000, rtol: float = 1e-10, log=print, kind: str = "picard",
+                   precond: str = "multigrid", every: int = 16):
+    """Run the plain solver (FP64 CG with the multigrid preconditioner, or
+    ``precond="chebyshev"``: Chebyshev(4)) over seeded glaciers and record
+    one pair per Picard iteration (kind "picard"), or one pair per
+    ``every``-th CG iteration (kind "cg": the learned-preconditioner solve
+    with a network whose output is zero, which takes plain PCG's steps).
+    Returns the device buffers (x, t) and the number of pairs of each
+    glacier (glacier k holds entries k*per_glacier ...)."""
     from . import api as H
     from . import synthetic as S
     H8, W8 = canvas_shape(nx, ny)
@@ -251,6 +265,11 @@ def generate_pairs(n_glaciers: int = 64, nx: int = 512, ny: int = 128, per_glaci
     x = DeviceArray((cap, H8, W8, 8), "bf16")
     t = DeviceArray((cap, H8, W8), "f32")
     counts = []
+    kw = dict(multigrid=True) if precond == "multigrid" else dict(chebyshev_degree=4)
+    if kind == "cg":
+        zero = UNet(1, H8, W8, seed=None)
+        zero.set_params(np.zeros(zero.n_params, dtype=np.float32))
+        kw.update(form="learned", network=zero, learned_precond_every=every, stall_ratio=1e3)
     t0 = time.perf_counter()
     for k in range(n_glaciers):
         g, raw, moulin = glacier_variant(k, nx, ny, seed0)
@@ -258,12 +277,11 @@ def generate_pairs(n_glaciers: int = 64, nx: int = 512, ny: int = 128, per_glaci
         sl = H.RegularizedCoulombSlidingLaw(0.5)
         s = H.State(g, p)
         H.set_initial_conditions(s, g, p, sl, **raw)
-        ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=rtol,
-                                 check_every=16)
+        ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, atol=0.0, rtol=rtol, check_every=16, **kw)
         ps = H.PicardSolver(500, 1e-6, ls, g)
         mi = H.SeasonalMoulinMeltInput(moulin, **S.SEASONAL_C2)
         sim = H.Simulation(g, s, max_steps, 3600.0, p, "implicit", [], mi, sl, ps=ps, k_face_choice="harmonic")
-        rec = PairRecorder(s, x, t, k * per_glacier, per_glacier)
+        rec = PairRecorder(s, x, t, k * per_glacier, per_glacier, kind)
         steps = 0
         while rec.count < per_glacier and steps < max_steps:
             H.step(sim)

@@ -30,16 +30,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_steps(g, raw, moulin, form: str, net, steps: int, precond: str = "multigrid", rtol: float = 1e-10):
+def run_steps(g, raw, moulin, form: str, net, steps: int, precond: str = "multigrid", rtol: float = 1e-10,
+              every: int = 0):
     """Timesteps of the 1024x256-style workload on glacier (g, raw); returns
-    per-step (seconds, Picard iterations, CG iterations) and the final h."""
+    per-step (seconds, Picard iterations, CG iterations) and the final h.
+    every > 0 (form "learned"): the network also preconditions one CG
+    iteration in `every`."""
     p = H.ModelParameters()
     sl = H.RegularizedCoulombSlidingLaw(0.5)
     s = H.State(g, p)
     H.set_initial_conditions(s, g, p, sl, **raw)
     kw = dict(multigrid=True) if precond == "multigrid" else dict(chebyshev_degree=4)
     ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, atol=0.0, rtol=rtol, check_every=16, form=form,
-                             network=net if form == "learned" else None, **kw)
+                             network=net if form == "learned" else None,
+                             learned_precond_every=every if form == "learned" else 0, **kw)
     ps = H.PicardSolver(500, 1e-6, ls, g)
     mi = H.SeasonalMoulinMeltInput(moulin, **S.SEASONAL_C2)
     sim = H.Simulation(g, s, steps, 3600.0, p, "implicit", [], mi, sl, ps=ps, k_face_choice="harmonic")
@@ -52,9 +56,10 @@ def run_steps(g, raw, moulin, form: str, net, steps: int, precond: str = "multig
     return out, s.get("h")
 
 
-def gain(net_path, nx: int, ny: int, seeds, steps: int, precond: str = "multigrid"):
+def gain(net_path, nx: int, ny: int, seeds, steps: int, precond: str = "multigrid", every: int = 0):
     """CG iterations and seconds per timestep with and without the learned
-    warm start, on glaciers the network never saw."""
+    warm start (and, every > 0, the learned preconditioner), on glaciers the
+    network never saw."""
     H8, W8 = LN.canvas_shape(nx, ny)
     net = LN.UNet.load(net_path, 1, H8, W8)
     rows = []
@@ -63,14 +68,14 @@ def gain(net_path, nx: int, ny: int, seeds, steps: int, precond: str = "multigri
         moulin = S.moulin_field(g, raw["mask"], n=20, seed=sd)
         res = {}
         for form in ("fp64", "learned"):
-            res[form] = run_steps(g, raw, moulin, form, net, steps, precond)
+            res[form] = run_steps(g, raw, moulin, form, net, steps, precond, every=every)
         base, lrn = res["fp64"][0], res["learned"][0]
         h_err = float(np.abs(res["learned"][1] - res["fp64"][1]).max() / np.abs(res["fp64"][1]).max())
         rows.append({"seed": sd, "cg_plain": sum(r[2] for r in base), "cg_learned": sum(r[2] for r in lrn),
                      "picard_plain": sum(r[1] for r in base), "picard_learned": sum(r[1] for r in lrn),
                      "s_plain": sum(r[0] for r in base[1:]), "s_learned": sum(r[0] for r in lrn[1:]),
                      "h_rel_diff": h_err})
-        log(f"[gain] {nx}x{ny} seed {sd} ({precond}): CG {rows[-1]['cg_plain']} -> {rows[-1]['cg_learned']}, "
+        log(f"[gain] {nx}x{ny} seed {sd} ({precond}, every {every}): CG {rows[-1]['cg_plain']} -> {rows[-1]['cg_learned']}, "
             f"Picard {rows[-1]['picard_plain']} -> {rows[-1]['picard_learned']}, h diff {h_err:.1e}")
     return rows
 
@@ -91,10 +96,16 @@ def main(argv=None):
     ap.add_argument("--out", default=str(LN.DEFAULT_WEIGHTS))
     ap.add_argument("--gain-steps", type=int, default=4)
     ap.add_argument("--gain-seeds", default="5001,5002")
+    ap.add_argument("--kind", choices=["picard", "cg"], default="picard",
+                    help="pairs of Picard corrections (warm start) or of CG iterations' remaining errors "
+                         "(learned preconditioner)")
+    ap.add_argument("--precond", choices=["multigrid", "chebyshev"], default="multigrid")
+    ap.add_argument("--every", type=int, default=16, help="CG pairs: one pair per this many CG iterations")
     args = ap.parse_args(argv)
 
     t0 = time.perf_counter()
-    x, t, counts = LN.generate_pairs(args.glaciers, args.nx, args.ny, args.per_glacier, log=log)
+    x, t, counts = LN.generate_pairs(args.glaciers, args.nx, args.ny, args.per_glacier, log=log, kind=args.kind,
+                                     precond=args.precond, every=args.every)
     t_pairs = time.perf_counter() - t0
     ntr = args.glaciers - args.val_glaciers
     idx_tr = np.concatenate([np.arange(k * args.per_glacier, k * args.per_glacier + counts[k]) for k in range(ntr)])
@@ -108,15 +119,17 @@ def main(argv=None):
                          idx_val=idx_va, log=log, log_every=max(1, args.iters // 10))
     t_train = time.perf_counter() - t1
     meta = dict(glaciers=args.glaciers, per_glacier=args.per_glacier, nx=args.nx, ny=args.ny, iters=args.iters,
-                batch=args.batch, lr=args.lr, seed=args.seed, loss=args.loss, val_loss=val[-1][1])
+                batch=args.batch, lr=args.lr, seed=args.seed, loss=args.loss, val_loss=val[-1][1], kind=args.kind,
+                precond=args.precond, every=args.every)
     net.save(args.out, **meta)
     log(f"[train] saved {args.out}")
-    rows = gain(args.out, args.nx, args.ny, [int(v) for v in args.gain_seeds.split(",")], args.gain_steps)
+    rows = gain(args.out, args.nx, args.ny, [int(v) for v in args.gain_seeds.split(",")], args.gain_steps,
+                args.precond, args.every if args.kind == "cg" else 0)
     print(json.dumps({
         "pairs": int(sum(counts)), "pairs_per_glacier": counts[:4], "pair_seconds": round(t_pairs, 1),
         "train_seconds": round(t_train, 1), "train_iters": args.iters, "batch": args.batch,
         "train_loss_first_last": [float(np.mean(hist[:50])), float(np.mean(hist[-50:]))],
-        "val_loss_untrained": val0, "val_loss": val, "gain_512x128_multigrid": rows,
+        "val_loss_untrained": val0, "val_loss": val, f"gain_{args.nx}x{args.ny}_{args.precond}": rows,
     }), flush=True)
 
 
