@@ -92,9 +92,12 @@ struct shk_ctx {
         int nx, ny;
         float *c, *e, *n, *D, *b, *xa, *xb;
         bool owned;
+        bool packed = false;  // c, e, n, D live in mg_pack (tail levels)
     };
     std::vector<MGLevel> mg;
-    float* mg_inv = nullptr;
+    float* mg_pack = nullptr;  // the tail's [c | e | n | D] (stride mg_stride) and the coarsest inverse
+    int mg_stride = 0;
+    float* mg_inv = nullptr;   // inside mg_pack
     double* mg_work = nullptr;
     float mg_omega = 0.8f, mg_scale = 1.5f;  // damped-Jacobi weight, coarse-correction scale (measured best)
     size_t mg_tail_first = 1;                 // first level handled by the single-workgroup tail kernel
@@ -361,19 +364,51 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
                 ctx->mg_tail_first = l;
                 break;
             }
-        // the tail keeps 2 floats per cell of all its levels in LDS: stay within 56 KB
-        // (degenerate shapes such as n x 1 halve only once per level)
+        // the tail keeps 6 floats per cell of all its levels and the coarsest
+        // inverse in LDS: stay within what one workgroup may hold (160 KB on
+        // gfx950), less the kernel's 4 KB of static LDS (degenerate shapes
+        // such as n x 1 halve only once per level)
         auto tail_cells = [&](size_t f) {
             size_t c = 0;
             for (size_t l = f; l < ctx->mg.size(); ++l) c += (size_t)ctx->mg[l].nx * ctx->mg[l].ny;
             return c;
         };
-        while (ctx->mg_tail_first + 1 < ctx->mg.size() && 8 * tail_cells(ctx->mg_tail_first) > 56 * 1024)
+        int dev = 0, lds_max = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t ncl = (size_t)ctx->mg.back().nx * ctx->mg.back().ny;
+        const size_t budget = lds_max > 4096 ? (size_t)lds_max - 4096 : 0;
+        while (ctx->mg_tail_first + 1 < ctx->mg.size() &&
+               launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl) > budget)
             ++ctx->mg_tail_first;
+        // above 64 KB the kernel must opt in; without it, stay within 64 KB
+        const size_t dflt = 65536 - 4096;
+        if (launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl) > dflt &&
+            launch::mg_tail_reserve(launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl)))
+            while (ctx->mg_tail_first + 1 < ctx->mg.size() &&
+                   launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl) > dflt)
+                ++ctx->mg_tail_first;
         if (ctx->mg.size() - ctx->mg_tail_first > (size_t)MG_MAX_TAIL)
             return fail(ctx, SHK_ERR_STATE, "multigrid hierarchy deeper than the tail kernel holds");
         const size_t nc = (size_t)ctx->mg.back().nx * ctx->mg.back().ny;
-        HIPCHK(hipMalloc((void**)&ctx->mg_inv, nc * nc * sizeof(float)));
+        // the tail levels' coefficients and the inverse are written straight
+        // into the block the tail kernel copies to LDS
+        const size_t tp = launch::mg_tail_stride(tail_cells(ctx->mg_tail_first));
+        ctx->mg_stride = (int)tp;
+        HIPCHK(hipMalloc((void**)&ctx->mg_pack, (4 * tp + nc * nc + 4) * sizeof(float) + 256));
+        HIPCHK(hipMemsetAsync(ctx->mg_pack, 0, (4 * tp + nc * nc + 4) * sizeof(float), ctx->st));
+        size_t o = 0;
+        for (size_t l = ctx->mg_tail_first; l < ctx->mg.size(); ++l) {
+            auto& L = ctx->mg[l];
+            float** ps[] = {&L.c, &L.e, &L.n, &L.D};
+            for (int a = 0; a < 4; ++a) {
+                HIPCHK(hipFree(*ps[a]));
+                *ps[a] = ctx->mg_pack + a * tp + o;
+            }
+            L.packed = true;
+            o += (size_t)L.nx * L.ny;
+        }
+        ctx->mg_inv = ctx->mg_pack + 4 * tp;
         HIPCHK(hipMalloc((void**)&ctx->mg_work, 2 * nc * nc * sizeof(double)));
         if (const char* e = getenv("SHK_MG_OMEGA")) ctx->mg_omega = (float)atof(e);
         if (const char* e = getenv("SHK_MG_SCALE")) ctx->mg_scale = (float)atof(e);
@@ -423,6 +458,8 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
         Tl.nlev = (int)(L - first + 1);
         Tl.omega = ctx->mg_omega;
         Tl.scale = ctx->mg_scale;
+        Tl.pack = ctx->mg_pack;
+        Tl.stride = ctx->mg_stride;
         Tl.inv = ctx->mg_inv;
         double bytes = 0.0;
         for (size_t l = first; l <= L; ++l) {
@@ -1005,11 +1042,15 @@ int shk_destroy(shk_ctx* ctx) {
     for (auto& L : ctx->mg) {
         hipFree(L.xa);
         if (L.owned) {
-            float* ps[] = {L.c, L.e, L.n, L.D, L.b, L.xb};
+            float* ps[] = {L.b, L.xb};
             for (float* p : ps) hipFree(p);
+            if (!L.packed) {
+                float* qs[] = {L.c, L.e, L.n, L.D};
+                for (float* p : qs) hipFree(p);
+            }
         }
     }
-    if (ctx->mg_inv) hipFree(ctx->mg_inv);
+    if (ctx->mg_pack) hipFree(ctx->mg_pack);  // holds mg_inv
     if (ctx->mg_work) hipFree(ctx->mg_work);
     if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
     void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw, ctx->rec_snap, ctx->rec_scale};

@@ -57,7 +57,7 @@ struct Scalars {
     int32_t nn_fallback, nn_pad;
     // learned preconditioner inside flexible CG: image scale and step
     double nn_scale_pc, nn_gamma_pc;
-    // block-completion counter of the running reduction kernel
+    // unused (reductions finish in a separate kernel); reset with the rest
     uint32_t counter;
     uint32_t pad;
 };
@@ -71,12 +71,19 @@ struct MGLevelDev {
     float *b, *xa, *xb;
 };
 constexpr int MG_MAX_TAIL = 12;
-constexpr long MG_FLAT_CELLS = 65536;  // middle levels up to this size use the single-phase kernels
-constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; all tail levels (~5.5k cells) in 44 KB of LDS
+// levels up to this size use the single-phase kernels (faster than the tiled
+// ones from 1M cells down; the tiled down-leg is faster on a 4M-cell level)
+constexpr long MG_FLAT_CELLS = 1L << 20;
+constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; levels of ~5.5k cells take ~150 KB of LDS
 struct MGTail {
     int nlev;
     float omega, scale;
     const float* inv;
+    // c, e, n, D of every tail level and the coarsest inverse, packed in the
+    // order the kernel keeps them in LDS: [c | e | n | D] with a stride of
+    // `stride` floats (the tail's cell count rounded up to 4), then the inverse
+    const float* pack;
+    int stride;
     MGLevelDev L[MG_MAX_TAIL];
 };
 
@@ -171,6 +178,11 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st);
 void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st);
+// dynamic LDS bytes of the tail kernel (rhs, x, c, e, n, D per cell and the
+// coarsest inverse), and the opt-in for more than the default 64 KB (0 = ok)
+size_t mg_tail_stride(size_t tail_cells);
+size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells);
+int mg_tail_reserve(size_t bytes);
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
                       hipStream_t st);
 

@@ -19,9 +19,10 @@
 // one contiguous band of rows, keeping shared halo rows in that XCD's L2.
 //
 // Reductions (dot products, maxima) are deterministic: a fixed wave-shuffle
-// tree, a fixed cross-wave order, one partial per workgroup, and the last
-// workgroup to finish (agent-scope fence + atomic counter) sums the partials
-// in a fixed order and updates the device-resident solver scalars.
+// tree, a fixed cross-wave order, one partial per workgroup, and a
+// one-workgroup finishing kernel launched right after on the same stream sums
+// the partials in a fixed order and updates the device-resident solver
+// scalars.
 //
 // Floating-point expressions keep the reference's evaluation order and the
 // library is built with -ffp-contract=off, so the FP64 path rounds exactly as
@@ -1421,15 +1422,32 @@ __global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int ny
     __shared__ float rs[MG_TY][MG_TX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
     const int x0 = blockIdx.x * MG_TX, y0 = blockIdx.y * MG_TY;
-    for (int i = tid; i < MG_HX * MG_HY; i += WAVE * WAVES) {
-        const int hy = i / MG_HX, hx = i - hy * MG_HX;
-        const int gx = x0 + hx - 1, gy = y0 + hy - 1;
-        float v = 0.f;
-        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
-            const size_t k = (size_t)gx + (size_t)gy * nx;
-            v = omega * (float)b[k] / D[k];
+    {
+        // the halo tile: every load of a thread issued before the first use
+        constexpr int NH = (MG_HX * MG_HY + WAVE * WAVES - 1) / (WAVE * WAVES);
+        TB bv[NH];
+        float dv[NH];
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            const int i = tid + j * WAVE * WAVES;
+            const int hy = i / MG_HX, hx = i - hy * MG_HX;
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1;
+            bv[j] = 0;
+            dv[j] = 1.f;
+            if (i < MG_HX * MG_HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+                const size_t k = (size_t)gx + (size_t)gy * nx;
+                bv[j] = b[k];
+                dv[j] = D[k];
+            }
         }
-        xs[hy][hx] = v;
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            const int i = tid + j * WAVE * WAVES;
+            if (i < MG_HX * MG_HY) {
+                const int hy = i / MG_HX, hx = i - hy * MG_HX;
+                xs[hy][hx] = omega * (float)bv[j] / dv[j];
+            }
+        }
     }
     __syncthreads();
     const int gx = x0 + tx;
@@ -1473,13 +1491,30 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
     __shared__ float us[MG_HY][MG_HX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
     const int x0 = blockIdx.x * MG_TX, y0 = blockIdx.y * MG_TY;
-    for (int i = tid; i < MG_HX * MG_HY; i += WAVE * WAVES) {
-        const int hy = i / MG_HX, hx = i - hy * MG_HX;
-        const int gx = x0 + hx - 1, gy = y0 + hy - 1;
-        float v = 0.f;
-        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-            v = x[(size_t)gx + (size_t)gy * nx] + scale * xc[(size_t)(gx >> 1) + (size_t)(gy >> 1) * nxc];
-        us[hy][hx] = v;
+    {
+        // the halo tile: every load of a thread issued before the first use
+        constexpr int NH = (MG_HX * MG_HY + WAVE * WAVES - 1) / (WAVE * WAVES);
+        float xv[NH], cv[NH];
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            const int i = tid + j * WAVE * WAVES;
+            const int hy = i / MG_HX, hx = i - hy * MG_HX;
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1;
+            xv[j] = 0.f;
+            cv[j] = 0.f;
+            if (i < MG_HX * MG_HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+                xv[j] = x[(size_t)gx + (size_t)gy * nx];
+                cv[j] = xc[(size_t)(gx >> 1) + (size_t)(gy >> 1) * nxc];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            const int i = tid + j * WAVE * WAVES;
+            if (i < MG_HX * MG_HY) {
+                const int hy = i / MG_HX, hx = i - hy * MG_HX;
+                us[hy][hx] = xv[j] + scale * cv[j];
+            }
+        }
     }
     __syncthreads();
     const int gx = x0 + tx;
@@ -1512,16 +1547,17 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
 // the neighbours' values directly: these levels are too small to fill the
 // chip, and each barrier-separated phase of the tiled kernels adds a global
 // memory round trip to their latency.
+template <class TB>
 __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, int nyc, float omega,
                                                       const float* __restrict__ c, const float* __restrict__ e,
                                                       const float* __restrict__ n, const float* __restrict__ D,
-                                                      const float* __restrict__ b, float* __restrict__ x,
+                                                      const TB* __restrict__ b, float* __restrict__ x,
                                                       float* __restrict__ bc, const Scalars* __restrict__ sc) {
     if (sc->done) return;
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     if (I >= nxc || J >= nyc) return;
-    auto xs = [&](size_t k) -> float { return omega * b[k] / D[k]; };
+    auto xs = [&](size_t k) -> float { return omega * (float)b[k] / D[k]; };
     const int x0 = 2 * I, y0 = 2 * J;
     const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
     float acc = 0.f;
@@ -1529,67 +1565,127 @@ __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, i
         for (int ix = x0; ix <= x1; ++ix) {
             const size_t k = (size_t)ix + (size_t)iy * nx;
             x[k] = xs(k);
-            acc += b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+            acc += (float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
         }
     bc[(size_t)I + (size_t)J * nxc] = acc;
 }
+// with_dot: also the partial of b'z (the finest level of FP64 CG's V-cycle)
+template <class TB>
 __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, float omega, float scale,
                                                     const float* __restrict__ c, const float* __restrict__ e,
                                                     const float* __restrict__ n, const float* __restrict__ D,
-                                                    const float* __restrict__ b, const float* __restrict__ x,
-                                                    const float* __restrict__ xc, float* __restrict__ out,
-                                                    const Scalars* __restrict__ sc) {
+                                                    const TB* __restrict__ b, const float* __restrict__ x,
+                                                    const float* __restrict__ xc, TB* __restrict__ out,
+                                                    int with_dot, double* partials, const Scalars* __restrict__ sc) {
     if (sc->done) return;
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-    if (ix >= nx || iy >= ny) return;
-    auto us = [&](size_t k) -> float {
-        const int kx = (int)(k % (size_t)nx), ky = (int)(k / (size_t)nx);
-        return x[k] + scale * xc[(size_t)(kx >> 1) + (size_t)(ky >> 1) * nxc];
-    };
-    const size_t k = (size_t)ix + (size_t)iy * nx;
-    const float u = us(k);
-    out[k] = u + omega * (b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, us)) / D[k];
+    double acc = 0.0;
+    if (ix < nx && iy < ny) {
+        auto uat = [&](int jx, int jy) -> float {
+            return x[(size_t)jx + (size_t)jy * nx] + scale * xc[(size_t)(jx >> 1) + (size_t)(jy >> 1) * nxc];
+        };
+        const size_t k = (size_t)ix + (size_t)iy * nx;
+        const float u = uat(ix, iy);
+        float a = c[k] * u;
+        if (ix < nx - 1) a += e[k] * (u - uat(ix + 1, iy));
+        if (ix > 0) a += e[k - 1] * (u - uat(ix - 1, iy));
+        if (iy < ny - 1) a += n[k] * (u - uat(ix, iy + 1));
+        if (iy > 0) a += n[k - nx] * (u - uat(ix, iy - 1));
+        const TB bk = b[k];
+        const float z = u + omega * ((float)bk - a) / D[k];
+        out[k] = (TB)z;
+        acc = (double)bk * (double)z;
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+    }
 }
 
 // The small levels of the V-cycle, from the first level of at most
 // MG_TAIL_CELLS cells down to the coarsest, in one workgroup: down-legs, the
-// dense coarsest solve, up-legs.  Every level's b and x live in LDS (2 floats
-// per cell of all tail levels, dynamic shared memory; x is overwritten by the
-// level's result after a barrier); the coefficients are read from global
-// memory (L2-resident).  The first tail level's b comes from global memory
+// dense coarsest solve, up-legs.  Everything lives in LDS (dynamic shared
+// memory): every level's b and x (x is overwritten by the level's result
+// after a barrier), its c, e, n, D and the coarsest inverse, 6 floats per
+// cell plus the inverse.  The first tail level's b comes from global memory
 // and its result goes back there.  With at most MG_TAIL_CELLS cells per level
 // each thread owns at most 4 cells of a level.
 __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __restrict__ sc) {
     if (sc->done) return;
-    extern __shared__ float lds[];
+    extern __shared__ float4 lds4[];
+    float* lds = (float*)lds4;
     __shared__ float red[1024];
     const int tid = threadIdx.x, nt = blockDim.x;
     const float omega = T.omega, scale = T.scale;
     int off[MG_MAX_TAIL + 1];
     off[0] = 0;
     for (int l = 0; l < T.nlev; ++l) off[l + 1] = off[l] + T.L[l].nx * T.L[l].ny;
-    const int tot = off[T.nlev];
-    float* Bs = lds;        // right-hand sides
-    float* Xs = lds + tot;  // smoothed x, then u, then the level's result
-    for (int k = tid; k < off[1]; k += nt) Bs[k] = T.L[0].b[k];
+    const int tp = T.stride;  // >= off[nlev], a multiple of 4
+    const int lc = T.nlev - 1;
+    const int ncell = T.L[lc].nx * T.L[lc].ny;
+    // LDS: rhs, x, the four coefficient arrays of every tail level, and the
+    // coarsest inverse.  The last five come packed from set-up in this very
+    // layout and are copied with 16 B loads, all of a thread's loads in
+    // flight at once (one memory round trip); every later phase reads LDS.
+    float* Bs = lds;           // right-hand sides
+    float* Xs = lds + tp;      // smoothed x, then u, then the level's result
+    float* Cs = lds + 2 * tp;  // reaction c
+    float* Es = lds + 3 * tp;  // east couplings
+    float* Ns = lds + 4 * tp;  // north couplings
+    float* Ds = lds + 5 * tp;  // Jacobi diagonal
+    float* Is = lds + 6 * tp;  // coarsest inverse, ncell x ncell
+    {
+        constexpr int UB = 4, UP = 8;
+        const float* b0 = T.L[0].b;
+        const int nb = off[1];
+        float vb[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int k = tid + u * nt;
+            vb[u] = k < nb ? b0[k] : 0.f;
+        }
+        const float4* src = (const float4*)T.pack;
+        float4* dst = lds4 + (2 * tp) / 4;
+        const int n4 = (4 * tp + ncell * ncell + 3) / 4;
+        for (int base = tid; base < n4; base += UP * nt) {
+            float4 v[UP];
+#pragma unroll
+            for (int u = 0; u < UP; ++u) {
+                const int i = base + u * nt;
+                if (i < n4) v[u] = src[i];
+            }
+#pragma unroll
+            for (int u = 0; u < UP; ++u) {
+                const int i = base + u * nt;
+                if (i < n4) dst[i] = v[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int k = tid + u * nt;
+            if (k < nb) Bs[k] = vb[u];
+        }
+        for (int k = tid + UB * nt; k < nb; k += nt) Bs[k] = b0[k];
+    }
     __syncthreads();
     for (int l = 0; l + 1 < T.nlev; ++l) {
-        const MGLevelDev F = T.L[l], C = T.L[l + 1];
+        const int fnx = T.L[l].nx, fny = T.L[l].ny, cnx = T.L[l + 1].nx, cny = T.L[l + 1].ny;
         const float* b = Bs + off[l];
+        const float *c = Cs + off[l], *e = Es + off[l], *n = Ns + off[l], *D = Ds + off[l];
         float* x = Xs + off[l];
         float* bc = Bs + off[l + 1];
-        auto xs = [&](size_t k) -> float { return omega * b[k] / F.D[k]; };
-        for (int q = tid; q < C.nx * C.ny; q += nt) {
-            const int I = q % C.nx, J = q / C.nx;
+        auto xs = [&](size_t k) -> float { return omega * b[k] / D[k]; };
+        for (int q = tid; q < cnx * cny; q += nt) {
+            const int I = q % cnx, J = q / cnx;
             const int fx0 = 2 * I, fy0 = 2 * J;
-            const int fx1 = min(fx0 + 1, F.nx - 1), fy1 = min(fy0 + 1, F.ny - 1);
+            const int fx1 = min(fx0 + 1, fnx - 1), fy1 = min(fy0 + 1, fny - 1);
             float acc = 0.f;
             for (int iy = fy0; iy <= fy1; ++iy)
                 for (int ix = fx0; ix <= fx1; ++ix) {
-                    const size_t k = (size_t)ix + (size_t)iy * F.nx;
+                    const size_t k = (size_t)ix + (size_t)iy * fnx;
                     x[k] = xs(k);
-                    acc += b[k] - flux_apply(F.nx, F.ny, ix, iy, k, F.c, F.e, F.n, xs);
+                    acc += b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xs);
                 }
             bc[q] = acc;
         }
@@ -1598,13 +1694,11 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
     {
         // coarsest: x = inv b with the symmetric inverse read by columns;
         // 16 threads per row, 4 products each, summed in a fixed order
-        const int lc = T.nlev - 1;
-        const int ncell = T.L[lc].nx * T.L[lc].ny;
         const float* b = Bs + off[lc];
         const int i = tid >> 4, part = tid & 15;
         float acc = 0.f;
         if (i < ncell)
-            for (int j = part * 4; j < part * 4 + 4 && j < ncell; ++j) acc += T.inv[(size_t)j * ncell + i] * b[j];
+            for (int j = part * 4; j < part * 4 + 4 && j < ncell; ++j) acc += Is[j * ncell + i] * b[j];
         red[tid] = acc;
         __syncthreads();
         if (part == 0 && i < ncell) {
@@ -1615,14 +1709,15 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         __syncthreads();
     }
     for (int l = T.nlev - 2; l >= 0; --l) {
-        const MGLevelDev F = T.L[l], C = T.L[l + 1];
-        const int nf = F.nx * F.ny;
+        const int fnx = T.L[l].nx, fny = T.L[l].ny, cnx = T.L[l + 1].nx;
+        const int nf = fnx * fny;
         const float* b = Bs + off[l];
+        const float *c = Cs + off[l], *e = Es + off[l], *n = Ns + off[l], *D = Ds + off[l];
         float* x = Xs + off[l];
         const float* zc = Xs + off[l + 1];
         for (int k = tid; k < nf; k += nt) {
-            const int ix = k % F.nx, iy = k / F.nx;
-            x[k] = x[k] + scale * zc[(ix >> 1) + (iy >> 1) * C.nx];
+            const int ix = k % fnx, iy = k / fnx;
+            x[k] = x[k] + scale * zc[(ix >> 1) + (iy >> 1) * cnx];
         }
         __syncthreads();
         auto us = [&](size_t k) -> float { return x[k]; };
@@ -1631,8 +1726,8 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         for (int j = 0; j < 4; ++j) {
             const int k = tid + j * nt;
             if (k < nf) {
-                const int ix = k % F.nx, iy = k / F.nx;
-                z[j] = x[k] + omega * (b[k] - flux_apply(F.nx, F.ny, ix, iy, (size_t)k, F.c, F.e, F.n, us)) / F.D[k];
+                const int ix = k % fnx, iy = k / fnx;
+                z[j] = x[k] + omega * (b[k] - flux_apply(fnx, fny, ix, iy, (size_t)k, c, e, n, us)) / D[k];
             }
         }
         __syncthreads();
@@ -1644,6 +1739,49 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         __syncthreads();
     }
     for (int k = tid; k < off[1]; k += nt) T.L[0].xb[k] = Xs[k];
+}
+
+// Coarsest level of at most 64 cells: the same inverse, in place in LDS
+// (Gauss-Jordan without pivoting, the matrix being SPD):
+//   p = A_kk;  A_kk = 1/p;  A_kj /= p (j != k);  A_ij -= A_ik A_kj (i, j != k);
+//   A_ik = -A_ik / p (i != k).
+__global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, const float* __restrict__ c,
+                                                              const float* __restrict__ e,
+                                                              const float* __restrict__ nn,
+                                                              float* __restrict__ inv) {
+    __shared__ double A[64 * 64];
+    __shared__ double colk[64];
+    const int n = nx * ny;
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nt = blockDim.x * blockDim.y;
+    for (int t = tid; t < n * n; t += nt) A[t] = 0.0;
+    __syncthreads();
+    for (int k = tid; k < n; k += nt) {
+        const int ix = k % nx, iy = k / nx;
+        double d = c[k];
+        if (ix < nx - 1) { d += e[k]; A[k * n + k + 1] = -(double)e[k]; }
+        if (ix > 0) { d += e[k - 1]; A[k * n + k - 1] = -(double)e[k - 1]; }
+        if (iy < ny - 1) { d += nn[k]; A[k * n + k + nx] = -(double)nn[k]; }
+        if (iy > 0) { d += nn[k - nx]; A[k * n + k - nx] = -(double)nn[k - nx]; }
+        A[k * n + k] = d;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double p = A[k * n + k];
+        const double ip = 1.0 / p;
+        // the pivot column before this step, then the scaled pivot row
+        for (int i = tid; i < n; i += nt) colk[i] = A[i * n + k];
+        __syncthreads();
+        for (int j = tid; j < n; j += nt) A[k * n + j] = (j == k) ? ip : A[k * n + j] * ip;
+        __syncthreads();
+        for (int t = tid; t < n * n; t += nt) {
+            const int i = t / n, j = t % n;
+            if (i == k) continue;
+            A[t] = (j == k) ? -colk[i] * ip : A[t] - colk[i] * A[k * n + j];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < n * n; t += nt) inv[t] = (float)A[t];
 }
 
 // Coarsest level: assemble the dense matrix (FP64, in `work`) and invert it
@@ -2254,15 +2392,22 @@ void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_diag, cells_grid(nx, ny), dim3(64, 4), 0, st, nx, ny, c, e, n, D);
 }
+// Largest level (cells) that runs the single-phase kernels; SHK_MG_FLAT
+// overrides MG_FLAT_CELLS for A/B measurements.
+static long mg_flat_cells() {
+    static const long v = [] {
+        const char* env = getenv("SHK_MG_FLAT");
+        return env ? atol(env) : MG_FLAT_CELLS;
+    }();
+    return v;
+}
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
              const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
-    if constexpr (sizeof(TB) == 4) {
-        if ((long)nx * ny <= MG_FLAT_CELLS) {
-            hipLaunchKernelGGL(k_mg_down_flat, cells_grid(nxc, nyc), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc,
-                               omega, c, e, n, D, b, x, bc, sc);
-            return;
-        }
+    if ((long)nx * ny <= mg_flat_cells()) {
+        hipLaunchKernelGGL(k_mg_down_flat<TB>, cells_grid(nxc, nyc), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc,
+                           omega, c, e, n, D, b, x, bc, sc);
+        return;
     }
     const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
     hipLaunchKernelGGL(k_mg_down<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D, b, x, bc,
@@ -2272,16 +2417,14 @@ template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st) {
-    if constexpr (sizeof(TB) == 4) {
-        if ((long)nx * ny <= MG_FLAT_CELLS && dot_mode < 0) {
-            hipLaunchKernelGGL(k_mg_up_flat, cells_grid(nx, ny), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale,
-                               c, e, n, D, b, x, xc, out, sc);
-            return;
-        }
-    }
-    const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
-    hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
-                       out, dot_mode >= 0 ? 1 : 0, partials, sc);
+    const bool flat = (long)nx * ny <= mg_flat_cells();
+    const dim3 grid = flat ? cells_grid(nx, ny) : dim3((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
+    if (flat)
+        hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,
+                           b, x, xc, out, dot_mode >= 0 ? 1 : 0, partials, sc);
+    else
+        hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x,
+                           xc, out, dot_mode >= 0 ? 1 : 0, partials, sc);
     if (dot_mode >= 0)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
@@ -2294,14 +2437,30 @@ template void mg_up<float>(int, int, int, float, float, const float*, const floa
                            const float*, const float*, const float*, float*, int, double*, Scalars*, hipStream_t);
 template void mg_up<double>(int, int, int, float, float, const float*, const float*, const float*, const float*,
                             const double*, const float*, const float*, double*, int, double*, Scalars*, hipStream_t);
+size_t mg_tail_stride(size_t tail_cells) { return (tail_cells + 3) & ~(size_t)3; }
+size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells) {
+    // rhs, x and the packed block (c, e, n, D, inverse; rounded up to 16 B)
+    return (2 * mg_tail_stride(tail_cells) + ((4 * mg_tail_stride(tail_cells) + coarsest_cells * coarsest_cells + 3) & ~(size_t)3)) *
+           sizeof(float);
+}
+int mg_tail_reserve(size_t bytes) {
+    return hipFuncSetAttribute((const void*)k_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+                   hipSuccess
+               ? 0
+               : -1;
+}
 void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st) {
     size_t cells = 0;
     for (int l = 0; l < T.nlev; ++l) cells += (size_t)T.L[l].nx * T.L[l].ny;
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 2 * cells * sizeof(float), st, T, sc);
+    const size_t nc = (size_t)T.L[T.nlev - 1].nx * T.L[T.nlev - 1].ny;
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), mg_tail_lds(cells, nc), st, T, sc);
 }
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
                       hipStream_t st) {
-    hipLaunchKernelGGL(k_mg_dense_inverse, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, work, inv);
+    if (nx * ny <= 64)
+        hipLaunchKernelGGL(k_mg_dense_inverse_lds, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, inv);
+    else
+        hipLaunchKernelGGL(k_mg_dense_inverse, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, work, inv);
 }
 
 static inline unsigned nn_blocks(size_t n) {
