@@ -359,8 +359,10 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
             ctx->mg.push_back(L);
         }
         ctx->mg_tail_first = ctx->mg.size() - 1;
+        long tail_cells_max = MG_TAIL_CELLS;  // SHK_MG_TAIL: A/B override, at most MG_TAIL_CELLS
+        if (const char* env = getenv("SHK_MG_TAIL")) tail_cells_max = std::min(atol(env), MG_TAIL_CELLS);
         for (size_t l = 1; l < ctx->mg.size(); ++l)
-            if ((long)ctx->mg[l].nx * ctx->mg[l].ny <= MG_TAIL_CELLS) {
+            if ((long)ctx->mg[l].nx * ctx->mg[l].ny <= tail_cells_max) {
                 ctx->mg_tail_first = l;
                 break;
             }

@@ -1400,6 +1400,11 @@ __device__ __forceinline__ float flux_apply(int nx, int ny, int ix, int iy, size
     return y;
 }
 
+// FP32 divide through the hardware reciprocal (about 1 ulp) for the V-cycle's
+// Jacobi sweeps: the V-cycle only preconditions CG, so its rounding changes
+// how fast CG converges, never the solution CG converges to.
+__device__ __forceinline__ float mg_div(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
+
 // The large levels of the V-cycle work on tiles of MG_TX x MG_TY fine cells
 // (even-aligned, so each tile restricts onto whole coarse cells): the values
 // the 5-point stencil needs at the neighbours are staged once per tile, with
@@ -1445,7 +1450,7 @@ __global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int ny
             const int i = tid + j * WAVE * WAVES;
             if (i < MG_HX * MG_HY) {
                 const int hy = i / MG_HX, hx = i - hy * MG_HX;
-                xs[hy][hx] = omega * (float)bv[j] / dv[j];
+                xs[hy][hx] = mg_div(omega * (float)bv[j], dv[j]);
             }
         }
     }
@@ -1531,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
             if (gy < ny - 1) a += n[k] * (u - us[fy + 2][tx + 1]);
             if (gy > 0) a += n[k - nx] * (u - us[fy][tx + 1]);
             const TB bk = b[k];
-            const float z = u + omega * ((float)bk - a) / D[k];
+            const float z = u + mg_div(omega * ((float)bk - a), D[k]);
             out[k] = (TB)z;
             acc += (double)bk * (double)z;
         }
@@ -1557,7 +1562,7 @@ __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, i
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     if (I >= nxc || J >= nyc) return;
-    auto xs = [&](size_t k) -> float { return omega * (float)b[k] / D[k]; };
+    auto xs = [&](size_t k) -> float { return mg_div(omega * (float)b[k], D[k]); };
     const int x0 = 2 * I, y0 = 2 * J;
     const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
     float acc = 0.f;
@@ -1593,7 +1598,7 @@ __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, flo
         if (iy < ny - 1) a += n[k] * (u - uat(ix, iy + 1));
         if (iy > 0) a += n[k - nx] * (u - uat(ix, iy - 1));
         const TB bk = b[k];
-        const float z = u + omega * ((float)bk - a) / D[k];
+        const float z = u + mg_div(omega * ((float)bk - a), D[k]);
         out[k] = (TB)z;
         acc = (double)bk * (double)z;
     }
@@ -1675,7 +1680,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         const float *c = Cs + off[l], *e = Es + off[l], *n = Ns + off[l], *D = Ds + off[l];
         float* x = Xs + off[l];
         float* bc = Bs + off[l + 1];
-        auto xs = [&](size_t k) -> float { return omega * b[k] / D[k]; };
+        auto xs = [&](size_t k) -> float { return mg_div(omega * b[k], D[k]); };
         for (int q = tid; q < cnx * cny; q += nt) {
             const int I = q % cnx, J = q / cnx;
             const int fx0 = 2 * I, fy0 = 2 * J;
@@ -1727,7 +1732,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
             const int k = tid + j * nt;
             if (k < nf) {
                 const int ix = k % fnx, iy = k / fnx;
-                z[j] = x[k] + omega * (b[k] - flux_apply(fnx, fny, ix, iy, (size_t)k, c, e, n, us)) / D[k];
+                z[j] = x[k] + mg_div(omega * (b[k] - flux_apply(fnx, fny, ix, iy, (size_t)k, c, e, n, us)), D[k]);
             }
         }
         __syncthreads();
