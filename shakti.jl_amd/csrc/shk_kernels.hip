@@ -1563,14 +1563,17 @@ __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, i
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     if (I >= nxc || J >= nyc) return;
     auto xs = [&](size_t k) -> float { return mg_div(omega * (float)b[k], D[k]); };
-    const int x0 = 2 * I, y0 = 2 * J;
-    const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
     float acc = 0.f;
-    for (int iy = y0; iy <= y1; ++iy)
-        for (int ix = x0; ix <= x1; ++ix) {
-            const size_t k = (size_t)ix + (size_t)iy * nx;
-            x[k] = xs(k);
-            acc += (float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int ix = 2 * I + dx, iy = 2 * J + dy;
+            if (ix < nx && iy < ny) {
+                const size_t k = (size_t)ix + (size_t)iy * nx;
+                x[k] = xs(k);
+                acc += (float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
+            }
         }
     bc[(size_t)I + (size_t)J * nxc] = acc;
 }
@@ -1617,7 +1620,7 @@ __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, flo
 // and its result goes back there.  With at most MG_TAIL_CELLS cells per level
 // each thread owns at most 4 cells of a level.
 __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __restrict__ sc) {
-    if (sc->done) return;
+    const int done = sc->done;  // read together with the loads below
     extern __shared__ float4 lds4[];
     float* lds = (float*)lds4;
     __shared__ float red[1024];
@@ -1646,25 +1649,30 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         const int nb = off[1];
         float vb[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            const int k = tid + u * nt;
-            vb[u] = k < nb ? b0[k] : 0.f;
-        }
-        const float4* src = (const float4*)T.pack;
-        float4* dst = lds4 + (2 * tp) / 4;
+        for (int u = 0; u < UB; ++u) vb[u] = b0[min(tid + u * nt, nb - 1)];
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4* src = (const f4*)T.pack;
+        f4* dst = (f4*)(lds4 + (2 * tp) / 4);
         const int n4 = (4 * tp + ncell * ncell + 3) / 4;
-        for (int base = tid; base < n4; base += UP * nt) {
-            float4 v[UP];
+        // unconditional loads (index clamped into the block), all issued
+        // before any of them is used: the empty asm statements take every
+        // loaded value, so the compiler cannot move a load down to its
+        // (conditional) store; the done flag arrives with them
+        for (int base = tid;; base += UP * nt) {
+            f4 v[UP];
 #pragma unroll
-            for (int u = 0; u < UP; ++u) {
-                const int i = base + u * nt;
-                if (i < n4) v[u] = src[i];
-            }
+            for (int u = 0; u < UP; ++u) v[u] = src[min(base + u * nt, n4 - 1)];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) asm volatile("" : : "v"(vb[u]));
+#pragma unroll
+            for (int u = 0; u < UP; ++u) asm volatile("" : : "v"(v[u]));
+            if (done) return;  // uniform: every thread leaves here, before any barrier
 #pragma unroll
             for (int u = 0; u < UP; ++u) {
                 const int i = base + u * nt;
                 if (i < n4) dst[i] = v[u];
             }
+            if (base + UP * nt >= n4) break;
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
@@ -1683,14 +1691,17 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         auto xs = [&](size_t k) -> float { return mg_div(omega * b[k], D[k]); };
         for (int q = tid; q < cnx * cny; q += nt) {
             const int I = q % cnx, J = q / cnx;
-            const int fx0 = 2 * I, fy0 = 2 * J;
-            const int fx1 = min(fx0 + 1, fnx - 1), fy1 = min(fy0 + 1, fny - 1);
             float acc = 0.f;
-            for (int iy = fy0; iy <= fy1; ++iy)
-                for (int ix = fx0; ix <= fx1; ++ix) {
-                    const size_t k = (size_t)ix + (size_t)iy * fnx;
-                    x[k] = xs(k);
-                    acc += b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xs);
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int ix = 2 * I + dx, iy = 2 * J + dy;
+                    if (ix < fnx && iy < fny) {
+                        const size_t k = (size_t)ix + (size_t)iy * fnx;
+                        x[k] = xs(k);
+                        acc += b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xs);
+                    }
                 }
             bc[q] = acc;
         }
