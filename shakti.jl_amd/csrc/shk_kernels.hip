@@ -155,11 +155,18 @@ struct FV {  // named views of Fields
         *lambda, *A_visc, *N, *mask;
     double *dhdx, *q_x, *Re_x, *b_x, *ub_x, *taub_x, *dpwdx, *valid_x;
     double *dhdy, *q_y, *Re_y, *b_y, *ub_y, *taub_y, *dpwdy, *valid_y;
-    __device__ __forceinline__ explicit FV(const Fields& F) {
-        double** p = (double**)&h;
-#pragma unroll
-        for (int i = 0; i < SHK_NUM_FIELDS; ++i) p[i] = F.f[i];
-    }
+    // member-wise from the kernel argument, so the pointers stay in scalar
+    // registers (a cast-and-store loop would place the struct in scratch)
+    __device__ __forceinline__ explicit FV(const Fields& F)
+        : h(F.f[SHK_H]), pw(F.f[SHK_PW]), po(F.f[SHK_PO]), b(F.f[SHK_B]), beta(F.f[SHK_BETA]),
+          abs_ub(F.f[SHK_ABS_UB]), mdot(F.f[SHK_MDOT]), shear(F.f[SHK_SHEAR]), potential(F.f[SHK_POTENTIAL]),
+          sensible(F.f[SHK_SENSIBLE]), Re(F.f[SHK_RE]), K(F.f[SHK_K]), G(F.f[SHK_G]), zb(F.f[SHK_ZB]),
+          zs(F.f[SHK_ZS]), H(F.f[SHK_HICE]), ieb(F.f[SHK_IEB]), lambda(F.f[SHK_LAMBDA]), A_visc(F.f[SHK_A_VISC]),
+          N(F.f[SHK_N]), mask(F.f[SHK_MASK]), dhdx(F.f[SHK_DHDX]), q_x(F.f[SHK_Q_X]), Re_x(F.f[SHK_RE_X]),
+          b_x(F.f[SHK_B_X]), ub_x(F.f[SHK_UB_X]), taub_x(F.f[SHK_TAUB_X]), dpwdx(F.f[SHK_DPWDX]),
+          valid_x(F.f[SHK_VALID_X]), dhdy(F.f[SHK_DHDY]), q_y(F.f[SHK_Q_Y]), Re_y(F.f[SHK_RE_Y]),
+          b_y(F.f[SHK_B_Y]), ub_y(F.f[SHK_UB_Y]), taub_y(F.f[SHK_TAUB_Y]), dpwdy(F.f[SHK_DPWDY]),
+          valid_y(F.f[SHK_VALID_Y]) {}
 };
 
 // Initial conditions, pointwise part (src/initial_conditions.jl:44-96 with
@@ -262,147 +269,147 @@ __device__ __forceinline__ double taub_face(const Phys& P, double Nf, double abs
     return 0.0;
 }
 
-// The refresh after a linear solve, fused into one pass (the reference runs
-// 9 launches: src/elliptic_solver.jl:193-206).  Each thread owns one cell and
-// writes its west/south faces (and the east/north boundary faces at the last
-// column/row); the east/north face values a cell also needs for mdot and Re
-// are recomputed from the same inputs, so every face is evaluated by the
-// same expression on both sides.  mode 1 is the initial-condition variant
-// (pw read as stored instead of computed from h, src/initial_conditions.jl:81-124).
-// With check, also reduces max|h - h_prev| and max|h| for the Picard test
+// The refresh after a linear solve (the reference runs 9 launches:
+// src/elliptic_solver.jl:193-206) in two passes.  k_derived_faces evaluates
+// every face once, one thread per (x face, y face) pair of the union grid
+// (nx+1) x (ny+1): gradients (interior faces), flux, Reynolds number and the
+// sliding stress, written to the face arrays.  k_derived_cells then reads the
+// four faces of each cell for Re, the melt-rate terms and K, and writes pw
+// and N.  Every value is the same expression of the same inputs as in the
+// reference.  mode 1 is the initial-condition variant (pw read as stored
+// instead of computed from h, src/initial_conditions.jl:81-124).  With check,
+// the cell pass also reduces max|h - h_prev| and max|h| for the Picard test
 // (src/elliptic_solver.jl:153-169).
-template <int MODE, bool CHECK>
-__global__ __launch_bounds__(256) void k_derived(Phys P, Fields F, const double* __restrict__ h_prev, int R,
-                                                 double* partials, Scalars* sc) {
+template <int MODE>
+__global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
     const int nx = P.nx, ny = P.ny;
-    int vbx, vby;
-    virtual_block(vbx, vby);
-    const int ix = vbx * WAVE + threadIdx.x;
-    const int iy0 = (vby * WAVES + threadIdx.y) * R;
-    const int iy1 = min(iy0 + R, ny);
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    FV s(F);
+    const double rg = P.rho_w * P.g;
+    auto PW = [&](size_t k) -> double { return MODE == 1 ? s.pw[k] : rg * (s.h[k] - s.zb[k]); };
+    auto NN = [&](size_t k) -> double { return jl_boolmul(s.mask[k] == GROUNDED, s.po[k] - PW(k)); };
+    // x face fx = ix of row iy
+    if (ix <= nx && iy < ny) {
+        const int fx = ix;
+        const size_t k = xidx(fx, iy, nx);
+        double dh, dp;
+        const bool inner = fx > 0 && fx < nx;
+        if (inner) {
+            const size_t cw = cidx(fx - 1, iy, nx), ce = cidx(fx, iy, nx);
+            const double v = s.valid_x[k];
+            dh = ((s.h[ce] - s.h[cw]) / P.dx) * v;
+            dp = ((PW(ce) - PW(cw)) / P.dx) * v;
+        } else {
+            dh = s.dhdx[k];
+            dp = s.dpwdx[k];
+        }
+        double q, Re;
+        qre(P, s.b_x[k], dh, q, Re);
+        if (inner) {
+            s.dhdx[k] = dh;
+            s.dpwdx[k] = dp;
+        }
+        s.q_x[k] = q;
+        s.Re_x[k] = Re;
+        if (P.sliding_law == 0) {
+            double Nf, av, lf;
+            if (fx == 0) { Nf = NN(cidx(0, iy, nx)); av = s.abs_ub[cidx(0, iy, nx)]; lf = s.lambda[cidx(0, iy, nx)]; }
+            else if (fx == nx) { const size_t cc = cidx(nx - 1, iy, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
+            else {
+                const size_t cw = cidx(fx - 1, iy, nx), ce = cidx(fx, iy, nx);
+                Nf = (NN(ce) + NN(cw)) / 2.0;
+                lf = (s.lambda[ce] + s.lambda[cw]) / 2.0;
+                av = (s.abs_ub[ce] + s.abs_ub[cw]) / 2.0;
+            }
+            s.taub_x[k] = taub_face(P, Nf, av, lf, s.ub_x[k]);
+        }
+    }
+    // y face fy = iy of column ix
+    if (ix < nx && iy <= ny) {
+        const int fy = iy;
+        const size_t k = cidx(ix, fy, nx);
+        double dh, dp;
+        const bool inner = fy > 0 && fy < ny;
+        if (inner) {
+            const size_t cs = cidx(ix, fy - 1, nx), cn = cidx(ix, fy, nx);
+            const double v = s.valid_y[k];
+            dh = ((s.h[cn] - s.h[cs]) / P.dy) * v;
+            dp = ((PW(cn) - PW(cs)) / P.dy) * v;
+        } else {
+            dh = s.dhdy[k];
+            dp = s.dpwdy[k];
+        }
+        double q, Re;
+        qre(P, s.b_y[k], dh, q, Re);
+        if (inner) {
+            s.dhdy[k] = dh;
+            s.dpwdy[k] = dp;
+        }
+        s.q_y[k] = q;
+        s.Re_y[k] = Re;
+        if (P.sliding_law == 0) {
+            double Nf, av, lf;
+            if (fy == 0) { const size_t cc = cidx(ix, 0, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
+            else if (fy == ny) { const size_t cc = cidx(ix, ny - 1, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
+            else {
+                const size_t cs = cidx(ix, fy - 1, nx), cn = cidx(ix, fy, nx);
+                Nf = (NN(cn) + NN(cs)) / 2.0;
+                lf = (s.lambda[cn] + s.lambda[cs]) / 2.0;
+                av = (s.abs_ub[cn] + s.abs_ub[cs]) / 2.0;
+            }
+            s.taub_y[k] = taub_face(P, Nf, av, lf, s.ub_y[k]);
+        }
+    }
+}
+
+template <int MODE, bool CHECK>
+__global__ __launch_bounds__(256) void k_derived_cells(Phys P, Fields F, const double* __restrict__ h_prev,
+                                                       double* partials) {
+    const int nx = P.nx, ny = P.ny;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
     FV s(F);
     const double rg = P.rho_w * P.g;
     double dmax = 0.0, hmax = 0.0;
     bool any_nan = false;
-    if (ix < nx) {
-        for (int iy = iy0; iy < iy1; ++iy) {
-            const size_t c = cidx(ix, iy, nx);
-            auto PW = [&](size_t k) -> double { return MODE == 1 ? s.pw[k] : rg * (s.h[k] - s.zb[k]); };
-            auto NN = [&](size_t k) -> double { return jl_boolmul(s.mask[k] == GROUNDED, s.po[k] - PW(k)); };
-            const double hc = s.h[c], pwc = PW(c), Nc = NN(c);
-            // ---- x faces: west (fx = ix) and east (fx = ix+1)
-            double qx[2], dhx[2], dpx[2], rex[2], tbx[2], ubx[2];
-            for (int side = 0; side < 2; ++side) {
-                const int fx = ix + side;
-                const size_t k = xidx(fx, iy, nx);
-                double dh, dp;
-                if (fx > 0 && fx < nx) {
-                    const size_t cw = cidx(fx - 1, iy, nx), ce = cidx(fx, iy, nx);
-                    const double v = s.valid_x[k];
-                    dh = ((s.h[ce] - s.h[cw]) / P.dx) * v;
-                    dp = ((PW(ce) - PW(cw)) / P.dx) * v;
-                } else {
-                    dh = s.dhdx[k];
-                    dp = s.dpwdx[k];
-                }
-                double q, Re;
-                qre(P, s.b_x[k], dh, q, Re);
-                double tb;
-                if (P.sliding_law == 0) {
-                    double Nf, av, lf;
-                    if (fx == 0) { Nf = NN(cidx(0, iy, nx)); av = s.abs_ub[cidx(0, iy, nx)]; lf = s.lambda[cidx(0, iy, nx)]; }
-                    else if (fx == nx) { const size_t cc = cidx(nx - 1, iy, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
-                    else {
-                        const size_t cw = cidx(fx - 1, iy, nx), ce = cidx(fx, iy, nx);
-                        Nf = (NN(ce) + NN(cw)) / 2.0;
-                        lf = (s.lambda[ce] + s.lambda[cw]) / 2.0;
-                        av = (s.abs_ub[ce] + s.abs_ub[cw]) / 2.0;
-                    }
-                    tb = taub_face(P, Nf, av, lf, s.ub_x[k]);
-                } else {
-                    tb = s.taub_x[k];
-                }
-                qx[side] = q; dhx[side] = dh; dpx[side] = dp; rex[side] = Re; tbx[side] = tb; ubx[side] = s.ub_x[k];
-            }
-            // ---- y faces: south (fy = iy) and north (fy = iy+1)
-            double qy[2], dhy[2], dpy[2], rey[2], tby[2], uby[2];
-            for (int side = 0; side < 2; ++side) {
-                const int fy = iy + side;
-                const size_t k = cidx(ix, fy, nx);
-                double dh, dp;
-                if (fy > 0 && fy < ny) {
-                    const size_t cs = cidx(ix, fy - 1, nx), cn = cidx(ix, fy, nx);
-                    const double v = s.valid_y[k];
-                    dh = ((s.h[cn] - s.h[cs]) / P.dy) * v;
-                    dp = ((PW(cn) - PW(cs)) / P.dy) * v;
-                } else {
-                    dh = s.dhdy[k];
-                    dp = s.dpwdy[k];
-                }
-                double q, Re;
-                qre(P, s.b_y[k], dh, q, Re);
-                double tb;
-                if (P.sliding_law == 0) {
-                    double Nf, av, lf;
-                    if (fy == 0) { const size_t cc = cidx(ix, 0, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
-                    else if (fy == ny) { const size_t cc = cidx(ix, ny - 1, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
-                    else {
-                        const size_t cs = cidx(ix, fy - 1, nx), cn = cidx(ix, fy, nx);
-                        Nf = (NN(cn) + NN(cs)) / 2.0;
-                        lf = (s.lambda[cn] + s.lambda[cs]) / 2.0;
-                        av = (s.abs_ub[cn] + s.abs_ub[cs]) / 2.0;
-                    }
-                    tb = taub_face(P, Nf, av, lf, s.ub_y[k]);
-                } else {
-                    tb = s.taub_y[k];
-                }
-                qy[side] = q; dhy[side] = dh; dpy[side] = dp; rey[side] = Re; tby[side] = tb; uby[side] = s.ub_y[k];
-            }
-            // ---- writes: own west/south faces, plus boundary faces at the edge
-            {
-                const size_t kw = xidx(ix, iy, nx), ks = cidx(ix, iy, nx);
-                if (ix > 0) { s.dhdx[kw] = dhx[0]; s.dpwdx[kw] = dpx[0]; }
-                s.q_x[kw] = qx[0]; s.Re_x[kw] = rex[0];
-                if (P.sliding_law == 0) s.taub_x[kw] = tbx[0];
-                if (ix == nx - 1) {
-                    const size_t ke = xidx(nx, iy, nx);
-                    s.q_x[ke] = qx[1]; s.Re_x[ke] = rex[1];
-                    if (P.sliding_law == 0) s.taub_x[ke] = tbx[1];
-                }
-                if (iy > 0) { s.dhdy[ks] = dhy[0]; s.dpwdy[ks] = dpy[0]; }
-                s.q_y[ks] = qy[0]; s.Re_y[ks] = rey[0];
-                if (P.sliding_law == 0) s.taub_y[ks] = tby[0];
-                if (iy == ny - 1) {
-                    const size_t kn = cidx(ix, ny, nx);
-                    s.q_y[kn] = qy[1]; s.Re_y[kn] = rey[1];
-                    if (P.sliding_law == 0) s.taub_y[kn] = tby[1];
-                }
-            }
-            if (MODE == 0) s.pw[c] = pwc;
-            s.N[c] = Nc;
-            // ---- Re (src/water_flux.jl:107-112), mdot (src/melt_rate.jl:106-137), K (:221-226)
-            const double Re_c = (rex[0] + rex[1] + rey[0] + rey[1]) / 4.0;
-            s.Re[c] = Re_c;
-            const double sh = fabs((ubx[1] * tbx[1] + ubx[0] * tbx[0]) / 2.0 + (uby[1] * tby[1] + uby[0] * tby[0]) / 2.0);
-            const double pot = fabs((qx[1] * dhx[1] + qx[0] * dhx[0]) / 2.0 + (qy[1] * dhy[1] + qy[0] * dhy[0]) / 2.0);
-            s.shear[c] = sh;
-            s.potential[c] = pot;
-            const double Linv = 1.0 / P.L;
-            if (P.with_sensible) {
-                const double sens = (qx[1] * dpx[1] + qx[0] * dpx[0]) / 2.0 + (qy[1] * dpy[1] + qy[0] * dpy[0]) / 2.0;
-                s.sensible[c] = sens;
-                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot + P.ct * P.cw * P.rho_w * sens);
-            } else {
-                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot);
-            }
-            const double bc = s.b[c];
-            s.K[c] = (bc * bc * bc * P.g) / (12.0 * P.nu * (1.0 + P.omega * Re_c));
-            if (CHECK) {
-                const double d = fabs(hc - h_prev[c]), a = fabs(hc);
-                any_nan |= isnan(d) || isnan(a);
-                dmax = d > dmax ? d : dmax;
-                hmax = a > hmax ? a : hmax;
-            }
+    if (ix < nx && iy < ny) {
+        const size_t c = cidx(ix, iy, nx);
+        const double hc = s.h[c];
+        const double pwc = MODE == 1 ? s.pw[c] : rg * (hc - s.zb[c]);
+        const double Nc = jl_boolmul(s.mask[c] == GROUNDED, s.po[c] - pwc);
+        const size_t kw = xidx(ix, iy, nx), ke = xidx(ix + 1, iy, nx);
+        const size_t ks = cidx(ix, iy, nx), kn = cidx(ix, iy + 1, nx);
+        const double qx0 = s.q_x[kw], qx1 = s.q_x[ke], qy0 = s.q_y[ks], qy1 = s.q_y[kn];
+        const double rex0 = s.Re_x[kw], rex1 = s.Re_x[ke], rey0 = s.Re_y[ks], rey1 = s.Re_y[kn];
+        const double dhx0 = s.dhdx[kw], dhx1 = s.dhdx[ke], dhy0 = s.dhdy[ks], dhy1 = s.dhdy[kn];
+        const double tbx0 = s.taub_x[kw], tbx1 = s.taub_x[ke], tby0 = s.taub_y[ks], tby1 = s.taub_y[kn];
+        const double ubx0 = s.ub_x[kw], ubx1 = s.ub_x[ke], uby0 = s.ub_y[ks], uby1 = s.ub_y[kn];
+        if (MODE == 0) s.pw[c] = pwc;
+        s.N[c] = Nc;
+        // ---- Re (src/water_flux.jl:107-112), mdot (src/melt_rate.jl:106-137), K (:221-226)
+        const double Re_c = (rex0 + rex1 + rey0 + rey1) / 4.0;
+        s.Re[c] = Re_c;
+        const double sh = fabs((ubx1 * tbx1 + ubx0 * tbx0) / 2.0 + (uby1 * tby1 + uby0 * tby0) / 2.0);
+        const double pot = fabs((qx1 * dhx1 + qx0 * dhx0) / 2.0 + (qy1 * dhy1 + qy0 * dhy0) / 2.0);
+        s.shear[c] = sh;
+        s.potential[c] = pot;
+        const double Linv = 1.0 / P.L;
+        if (P.with_sensible) {
+            const double dpx0 = s.dpwdx[kw], dpx1 = s.dpwdx[ke], dpy0 = s.dpwdy[ks], dpy1 = s.dpwdy[kn];
+            const double sens = (qx1 * dpx1 + qx0 * dpx0) / 2.0 + (qy1 * dpy1 + qy0 * dpy0) / 2.0;
+            s.sensible[c] = sens;
+            s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot + P.ct * P.cw * P.rho_w * sens);
+        } else {
+            s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot);
+        }
+        const double bc = s.b[c];
+        s.K[c] = (bc * bc * bc * P.g) / (12.0 * P.nu * (1.0 + P.omega * Re_c));
+        if (CHECK) {
+            const double d = fabs(hc - h_prev[c]), a = fabs(hc);
+            any_nan |= isnan(d) || isnan(a);
+            dmax = d > dmax ? d : dmax;
+            hmax = a > hmax ? a : hmax;
         }
     }
     if (CHECK) {
@@ -2247,14 +2254,18 @@ void stagger_b(const Phys& P, Fields F, hipStream_t st) {
 }
 void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,
              Scalars* sc, hipStream_t st) {
+    const dim3 fgrid = cells_grid(G.nx + 1, G.ny + 1), cgrid = cells_grid(G.nx, G.ny), blk(WAVE, WAVES);
+    if (mode == 0) hipLaunchKernelGGL((k_derived_faces<0>), fgrid, blk, 0, st, P, F);
+    else hipLaunchKernelGGL((k_derived_faces<1>), fgrid, blk, 0, st, P, F);
     if (mode == 0 && check) {
-        hipLaunchKernelGGL((k_derived<0, true>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
-        finish(G, partials, FIN_PICARD, 0, sc, st);
+        hipLaunchKernelGGL((k_derived_cells<0, true>), cgrid, blk, 0, st, P, F, h_prev, partials);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(cgrid.x * cgrid.y),
+                           (int)FIN_PICARD, 0, sc);
+    } else if (mode == 0) {
+        hipLaunchKernelGGL((k_derived_cells<0, false>), cgrid, blk, 0, st, P, F, h_prev, partials);
+    } else {
+        hipLaunchKernelGGL((k_derived_cells<1, false>), cgrid, blk, 0, st, P, F, h_prev, partials);
     }
-    else if (mode == 0)
-        hipLaunchKernelGGL((k_derived<0, false>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
-    else
-        hipLaunchKernelGGL((k_derived<1, false>), G.grid, G.block, 0, st, P, F, h_prev, G.R, partials, sc);
 }
 void gap(const Phys& P, Fields F, double dt, hipStream_t st) {
     const int n = P.nx * P.ny;
