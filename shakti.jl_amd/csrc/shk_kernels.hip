@@ -471,32 +471,30 @@ __device__ __forceinline__ double bK(const Phys& P, double K1, double K2, double
 // cP (optional) receives the diagonal minus the kept couplings -- the creep
 // (or storage) reaction plus the conductances of faces to Dirichlet cells --
 // summed without cancellation, for the FP32 flux-form operator.
+// One thread per cell (a plain 2D grid of 64 x 4 workgroups).
 template <bool PARABOLIC, class T>
-__global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, int R, T* __restrict__ aP,
+__global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T* __restrict__ aP,
                                                   T* __restrict__ aE, T* __restrict__ aN, T* __restrict__ rhs,
                                                   double* __restrict__ cP) {
     const int nx = P.nx, ny = P.ny;
-    int vbx, vby;
-    virtual_block(vbx, vby);
-    const int ix = vbx * WAVE + threadIdx.x;
-    const int iy0 = (vby * WAVES + threadIdx.y) * R;
-    const int iy1 = min(iy0 + R, ny);
-    if (ix >= nx) return;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
     FV s(F);
-    for (int iy = iy0; iy < iy1; ++iy) {
+    {
         const size_t c = cidx(ix, iy, nx);
         const double m = s.mask[c];
         if (m == OCEAN || m == LAND) {
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
             rhs[c] = dirichlet_head(P, m, s.zb[c]);
             if (cP) cP[c] = 1.0;
-            continue;
+            return;
         }
         if (m == OTHER_BASIN) {
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
             rhs[c] = s.h[c];
             if (cP) cP[c] = 1.0;
-            continue;
+            return;
         }
         const double Kc = s.K[c];
         double mE = 0, mW = 0, mN = 0, mS = 0;
@@ -2298,10 +2296,11 @@ void zero(double* p, size_t n, hipStream_t st) {
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
               hipStream_t st, double* cP) {
+    const dim3 grid = cells_grid(G.nx, G.ny), blk(WAVE, WAVES);
     if (parabolic)
-        hipLaunchKernelGGL((k_assemble<true, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs, cP);
+        hipLaunchKernelGGL((k_assemble<true, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP);
     else
-        hipLaunchKernelGGL((k_assemble<false, T>), G.grid, G.block, 0, st, P, F, dt, G.R, aP, aE, aN, rhs, cP);
+        hipLaunchKernelGGL((k_assemble<false, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP);
 }
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st) {
