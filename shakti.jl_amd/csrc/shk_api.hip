@@ -74,6 +74,7 @@ struct shk_ctx {
     double* d64 = nullptr;  // FP64 correction / residual scratch
 
     double* partials = nullptr;
+    double* partials2 = nullptr;  // p'Ap partials while r'z partials are pending (deferred finishing)
     Scalars* sc = nullptr;       // device
     Scalars* sc_host = nullptr;  // pinned host mirror
     bool have_system = false;
@@ -112,6 +113,9 @@ struct shk_ctx {
     std::vector<GraphEntry> graphs;
     bool use_graphs = true;
     bool use_fused = true;
+    bool use_defer = true;  // FP64 CG + V-cycle: finishing kernels folded into the next kernel
+    bool use_adaptive = true;  // batch sizes from the observed convergence rate
+    int last_niter = 0;        // CG iterations of the previous solve
 
     // learned warm start and training-pair recording: the network input is
     // the grid padded with zero cells to W8 x H8 (multiples of 64 and 32)
@@ -263,6 +267,7 @@ int reset_scalars(shk_ctx* ctx, double atol, double rtol, int itmax) {
     h->done = 0; h->solved = 0; h->zero_curv = 0; h->tired = 0; h->iter = 0;
     h->atol = atol; h->rtol = rtol; h->itmax = itmax;
     h->counter = 0;
+    h->gpend = 0;
     HIPCHK(hipMemcpyAsync(ctx->sc, h, sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
     return SHK_OK;
 }
@@ -433,7 +438,8 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 }
 
 // z = V(r): one V-cycle (one damped-Jacobi sweep down and up on every level,
-// dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1 iteration).
+// dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1
+// iteration; 2 leaves the partials pending for the next CG matvec; -1 none).
 // With T = double the finest level reads the FP64 residual and writes the FP64
 // z that FP64 CG uses; every level computes in FP32.  The large levels run
 // as tiled kernels, the levels below MG_TAIL_CELLS cells in one workgroup.
@@ -574,16 +580,50 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         if (nn) return fail(ctx, SHK_ERR_STATE, "the learned preconditioner runs in FP64 CG only");
     }
     const int every = nn_every + (nn_every & 1);
-    const int batch = nn ? every : check_every + (check_every & 1);
+    const int bmax = nn ? every : check_every + (check_every & 1);
+    // plain solves size each batch to the iterations the convergence rate so
+    // far predicts (a power of two, 2..bmax), so few iterations are queued
+    // past convergence; the first batch follows the previous solve's count
+    const bool adaptive = !nn && ctx->use_adaptive;
+    auto pow2_batch = [&](double need) {
+        int b = 2;
+        while (b < bmax && b < need) b *= 2;
+        return b < bmax ? b : bmax;
+    };
+    int batch = (adaptive && ctx->last_niter > 0) ? pow2_batch(ctx->last_niter + 1) : bmax;
     const bool fused = precond == SHK_PRECOND_CHEBYSHEV && degree >= 2 && degree <= 8 && ctx->use_fused && !nn;
     bool learned_batch = false;
     int rc_nn = SHK_OK;
+    // FP64 CG with the V-cycle (no learned preconditioner): the two finishing
+    // kernels of an iteration are folded into the matvec and the update
+    const bool defer = sizeof(T) == 8 && multigrid && !nn && ctx->use_defer;
+    const int nba = multigrid ? launch::mg_up_nblocks(ctx->mg[0].nx, ctx->mg[0].ny) : 0;
     auto enqueue_batch = [&]() {
         T* p_old = w.pA;
         T* p_new = w.pB;
         T* r_cur = w.r;
         T* r_nxt = w.r2;
         for (int k = 0; k < batch; ++k) {
+            if constexpr (sizeof(T) == 8) {
+                if (defer) {
+                    int nbb;
+                    {
+                        Prof pr(ctx, P_MV, 7 * nb);  // reads aP, aE, aN, z, p_old; writes p_new, Ap
+                        nbb = launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap, ctx->partials2,
+                                                         ctx->sc, ctx->partials, nba, ctx->st);
+                    }
+                    {
+                        Prof pr(ctx, P_UPD, 6 * nb);  // reads x, p, r, Ap; writes x, r
+                        launch::cg_update_deferred(G, p_new, w.Ap, w.x, w.r, ctx->sc, ctx->partials, nba,
+                                                   ctx->partials2, nbb, ctx->st);
+                    }
+                    mg_apply<T>(ctx, w.r, w.z, 2);
+                    T* t = p_old;
+                    p_old = p_new;
+                    p_new = t;
+                    continue;
+                }
+            }
             {
                 Prof pr(ctx, P_MV, 7 * nb);  // reads aP, aE, aN, z, p_old; writes p_new, Ap
                 launch::cg_matvec<T>(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap, ctx->partials, ctx->sc, ctx->st);
@@ -629,7 +669,8 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     auto batch_graph = [&](hipGraphExec_t* out) -> int {
         *out = nullptr;
         if (!ctx->use_graphs || ctx->prof || (ctx->rec_kind == 1 && ctx->rec_x)) return SHK_OK;
-        const int key[4] = {(int)sizeof(T), precond + (learned_batch ? 16 : 0), degree * (fused ? 100 : 1), batch};
+        const int key[4] = {(int)sizeof(T), precond + (learned_batch ? 16 : 0) + (defer ? 32 : 0),
+                            degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
             if (!memcmp(e.key, key, sizeof(key)) && (!learned_batch || e.net == ctx->net)) *out = e.exec;
         if (*out) return SHK_OK;
@@ -671,6 +712,22 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         if (rc) return rc;
         if (ctx->sc_host->done) break;
         if (queued > (long)itmax + batch) break;  // cannot happen: tired sets done
+        if (adaptive) {
+            const Scalars* h = ctx->sc_host;
+            const double prev = queued == batch ? h->rNorm0 : rn_start;
+            const double ratio = h->rNorm / prev;
+            double need = bmax;
+            if (ratio > 0.0 && ratio < 1.0 && h->rNorm > 0.0 && h->eps_stop > 0.0) {
+                const double per_it = log(ratio) / batch;  // log of the mean reduction per iteration
+                need = ceil(log(h->eps_stop / h->rNorm) / per_it) + (defer ? 1 : 0);
+            }
+            const int nb2 = pow2_batch(need);
+            if (nb2 != batch) {
+                batch = nb2;
+                rc = batch_graph(&exec);
+                if (rc) return rc;
+            }
+        }
         if (nn && !plain_done) {
             // the plain batch's reduction is the reference; learned batches follow
             plain_done = true;
@@ -691,6 +748,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         }
         rn_start = ctx->sc_host->rNorm;
     }
+    ctx->last_niter = ctx->sc_host->iter;
     return SHK_OK;
 }
 
@@ -981,6 +1039,8 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
     ctx->use_graphs = getenv("SHK_NO_GRAPHS") == nullptr;
     ctx->use_fused = getenv("SHK_NO_FUSED") == nullptr;
+    ctx->use_defer = getenv("SHK_NO_DEFER") == nullptr;
+    ctx->use_adaptive = getenv("SHK_FIXED_BATCH") == nullptr;
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;
@@ -1013,6 +1073,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     for (int H = 1; H < 8; ++H) maxblocks = std::max(maxblocks, (size_t)fused_geom(ctx->G, H).nblocks);
     const size_t nparts = 2 * maxblocks + 64;
     if ((rc = dalloc(ctx, &ctx->partials, nparts))) return rc;
+    if ((rc = dalloc(ctx, &ctx->partials2, nparts))) return rc;
     HIPCHK(hipMalloc((void**)&ctx->sc, sizeof(Scalars)));
     HIPCHK(hipHostMalloc((void**)&ctx->sc_host, sizeof(Scalars), hipHostMallocDefault));
     memset(ctx->sc_host, 0, sizeof(Scalars));
@@ -1033,7 +1094,7 @@ int shk_destroy(shk_ctx* ctx) {
     void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->r2, ctx->r2_32, ctx->cP, ctx->dg32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
                     ctx->pA, ctx->pB, ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->d64, ctx->aP32, ctx->aE32,
                     ctx->aN32, ctx->b32, ctx->x32, ctx->r32, ctx->z32, ctx->pA32, ctx->pB32, ctx->Ap32,
-                    ctx->s32, ctx->cA32, ctx->cB32, ctx->partials, ctx->sc};
+                    ctx->s32, ctx->cA32, ctx->cB32, ctx->partials, ctx->partials2, ctx->sc};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ctx->sc_host) hipHostFree(ctx->sc_host);

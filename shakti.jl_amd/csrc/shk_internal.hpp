@@ -57,6 +57,9 @@ struct Scalars {
     int32_t nn_fallback, nn_pad;
     // learned preconditioner inside flexible CG: image scale and step
     double nn_scale_pc, nn_gamma_pc;
+    // r'z partials of the last preconditioner not yet finished (deferred
+    // finishing: the next CG matvec sums them)
+    int32_t gpend, gpad;
     // unused (reductions finish in a separate kernel); reset with the rest
     uint32_t counter;
     uint32_t pad;
@@ -154,6 +157,15 @@ void cheb_degree1(const Geom& G, const T* dg, const T* r, T* y, int mode, double
 template <class T>
 void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
                double* partials, Scalars* sc, hipStream_t st);
+// FP64 CG with the finishing kernels folded into the next kernel: the matvec
+// sums the pending r'z partials (pa), writes p'Ap partials to pb and returns
+// their count; the update sums both.
+int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
+                       const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
+                       int nba, hipStream_t st);
+void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
+                        const double* pa, int nba, const double* pb, int nbb, hipStream_t st);
+int mg_up_nblocks(int nx, int ny);
 template <class T>
 void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st);

@@ -148,6 +148,37 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double* partials)
     }
 }
 
+// The finishing kernel's sum of nb partials (k_finish, one value), taken by
+// every workgroup of a consumer kernel: the same per-thread strides and the
+// same tree, so every workgroup gets k_finish's value bit for bit.  All
+// threads of the workgroup must call it; all receive the total.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int nb) {
+    __shared__ double total;
+    const int tid = threadIdx.x + threadIdx.y * WAVE;
+    double a = 0.0;
+    bool first = true;
+    for (int j = tid; j < nb; j += WAVE * WAVES) {
+        const double b = p[j];
+        if (first) {
+            a = b;
+            first = false;
+        } else {
+            a = a + b;
+        }
+    }
+    double one[1] = {a};
+    block_reduce<1, false>(one);
+    if (tid == 0) total = one[0];
+    __syncthreads();
+    const double t = total;
+    __syncthreads();  // a later call rewrites total
+    return t;
+}
+
+__device__ __forceinline__ bool lead_thread() {
+    return blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+}
+
 // --------------------------------------------------------- field kernels
 
 struct FV {  // named views of Fields
@@ -888,6 +919,7 @@ __device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
         sc->tired = 0;
         sc->eps_stop = sc->atol + sc->rtol * ref;
         sc->pAp = 0.0;
+        sc->gpend = 0;
         sc->solved = (g == 0.0) || (rn <= sc->eps_stop);
         sc->done = sc->solved;
     } else {
@@ -1032,16 +1064,48 @@ __global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const 
 
 // p = z + beta*p_old (formed at the stencil points), Ap = A p, p'Ap; then
 // Krylov's zero-curvature test and alpha = gamma / p'Ap.
-template <class T>
+//
+// Deferred finish (FP64 CG with the V-cycle, DEFER): the preconditioner of
+// the previous iteration left its r'z partials (pa, nba) without a finishing
+// kernel (gpend set).  Every workgroup sums them as k_finish would and takes
+// the same stop test and beta (Krylov's cg!, as in fin_gamma); workgroup 0
+// stores rNorm, solved, iter, tired, done, beta and pNorm2.  gamma, which the
+// other workgroups read, is stored by the update kernel that follows.
+template <bool DEFER>
+__device__ __forceinline__ bool deferred_gamma(Scalars* sc, const double* __restrict__ pa, int nba, double& beta) {
+    if (!DEFER || !sc->gpend) return false;
+    const double g = sum_partials(pa, nba);
+    const double rn = sqrt(g);
+    const int solved = (rn <= sc->eps_stop) || (rn + 1.0 <= 1.0);
+    const double b = g / sc->gamma;
+    if (lead_thread()) {
+        sc->rNorm = rn;
+        sc->solved = solved;
+        if (!solved) {
+            sc->pNorm2 = g + b * b * sc->pNorm2;
+            sc->beta = b;
+        }
+        const int it = sc->iter + 1;
+        sc->iter = it;
+        sc->tired = it >= sc->itmax;
+        sc->done = solved || it >= sc->itmax;
+    }
+    beta = b;
+    return solved;
+}
+
+template <class T, bool DEFER>
 __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const T* __restrict__ aP,
                                                    const T* __restrict__ aE, const T* __restrict__ aN,
                                                    const T* __restrict__ z, const T* __restrict__ p_old,
                                                    T* __restrict__ p_new, T* __restrict__ Ap, double* partials,
-                                                   Scalars* sc) {
+                                                   Scalars* sc, const double* __restrict__ pa, int nba) {
     if (sc->done) return;
     int vbx, vby;
     virtual_block(vbx, vby);
-    const T beta = (T)sc->beta;
+    double b = sc->beta;
+    if (deferred_gamma<DEFER>(sc, pa, nba, b)) return;
+    const T beta = (T)b;
     double acc = 0.0;
     march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return z[j] + beta * p_old[j]; },
              [&](size_t c, T Au, T u) {
@@ -1054,16 +1118,18 @@ __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const 
 }
 
 // k_cg_matvec on two columns per lane (FP64, even nx).
-template <bool NT>
+template <bool NT, bool DEFER>
 __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const double* __restrict__ aP,
                                                     const double* __restrict__ aE, const double* __restrict__ aN,
                                                     const double* __restrict__ z, const double* __restrict__ p_old,
                                                     double* __restrict__ p_new, double* __restrict__ Ap,
-                                                    double* partials, Scalars* sc) {
+                                                    double* partials, Scalars* sc, const double* __restrict__ pa,
+                                                    int nba) {
     if (sc->done) return;
     int vbx, vby;
     virtual_block(vbx, vby);
-    const double beta = sc->beta;
+    double beta = sc->beta;
+    if (deferred_gamma<DEFER>(sc, pa, nba, beta)) return;
     double acc = 0.0;
     march2<NT>(
         nx, ny, R, vbx, vby, aP, aE, aN,
@@ -1080,15 +1146,39 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
 }
 
 // x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
-template <class T>
+// DEFER: every workgroup sums p'Ap (pb, nbb) and r'z (pa, nba) as k_finish
+// would, applies the zero-curvature test and alpha = r'z / p'Ap (FIN_PAP);
+// workgroup 0 stores pAp, gamma, alpha (or zero_curv and done).
+template <class T, bool DEFER>
 __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const T* __restrict__ dg,
                                                    const T* __restrict__ p, const T* __restrict__ Ap,
                                                    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                   int jacobi, double* partials, Scalars* sc) {
+                                                   int jacobi, double* partials, Scalars* sc,
+                                                   const double* __restrict__ pa, int nba,
+                                                   const double* __restrict__ pb, int nbb) {
     if (sc->done) return;
     int vbx, vby;
     virtual_block(vbx, vby);
-    const T alpha = (T)sc->alpha;
+    double al = sc->alpha;
+    if constexpr (DEFER) {
+        const double pAp = sum_partials(pb, nbb);
+        const double g = sum_partials(pa, nba);
+        const double pn2 = sc->pNorm2;
+        const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
+        al = g / pAp;
+        if (lead_thread()) {
+            sc->pAp = pAp;
+            sc->gamma = g;
+            if (zc) {
+                sc->zero_curv = 1;
+                sc->done = 1;
+            } else {
+                sc->alpha = al;
+            }
+        }
+        if (zc) return;
+    }
+    const T alpha = (T)al;
     double acc = 0.0;
     walk(nx, ny, R, vbx, vby, [&](size_t c) {
         x[c] = x[c] + alpha * p[c];
@@ -1496,7 +1586,7 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
                                                const float* __restrict__ n, const float* __restrict__ D,
                                                const TB* __restrict__ b, const float* __restrict__ x,
                                                const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
-                                               double* partials, const Scalars* __restrict__ sc) {
+                                               double* partials, Scalars* __restrict__ sc) {
     if (sc->done) return;
     __shared__ float us[MG_HY][MG_HX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
@@ -1549,6 +1639,7 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
     if (with_dot) {
         double v[1] = {acc};
         block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
     }
 }
 
@@ -1589,7 +1680,7 @@ __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, flo
                                                     const float* __restrict__ n, const float* __restrict__ D,
                                                     const TB* __restrict__ b, const float* __restrict__ x,
                                                     const float* __restrict__ xc, TB* __restrict__ out,
-                                                    int with_dot, double* partials, const Scalars* __restrict__ sc) {
+                                                    int with_dot, double* partials, Scalars* __restrict__ sc) {
     if (sc->done) return;
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1613,6 +1704,7 @@ __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, flo
     if (with_dot) {
         double v[1] = {acc};
         block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
     }
 }
 
@@ -2364,25 +2456,47 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
         if (wide_ok(G)) {
             const Geom W = wide_geom(G);
             if (stream_nt(G))
-                hipLaunchKernelGGL(k_cg_matvec2<true>, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
-                                   p_old, p_new, Ap, partials, sc);
+                hipLaunchKernelGGL((k_cg_matvec2<true, false>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
+                                   z, p_old, p_new, Ap, partials, sc, nullptr, 0);
             else
-                hipLaunchKernelGGL(k_cg_matvec2<false>, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
-                                   p_old, p_new, Ap, partials, sc);
+                hipLaunchKernelGGL((k_cg_matvec2<false, false>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE,
+                                   aN, z, p_old, p_new, Ap, partials, sc, nullptr, 0);
             finish(W, partials, FIN_PAP, 0, sc, st);
             return;
         }
     }
-    hipLaunchKernelGGL(k_cg_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old, p_new, Ap,
-                       partials, sc);
+    hipLaunchKernelGGL((k_cg_matvec<T, false>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old, p_new,
+                       Ap, partials, sc, nullptr, 0);
     finish(G, partials, FIN_PAP, 0, sc, st);
 }
 template <class T>
 void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_update<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, dg, p, Ap, x, r, z, jacobi,
-                       partials, sc);
+    hipLaunchKernelGGL((k_cg_update<T, false>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, dg, p, Ap, x, r, z, jacobi,
+                       partials, sc, nullptr, 0, nullptr, 0);
     if (jacobi) finish(G, partials, FIN_GAMMA_ITER, 0, sc, st);
+}
+int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
+                       const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
+                       int nba, hipStream_t st) {
+    if (wide_ok(G)) {
+        const Geom W = wide_geom(G);
+        if (stream_nt(G))
+            hipLaunchKernelGGL((k_cg_matvec2<true, true>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
+                               p_old, p_new, Ap, pb, sc, pa, nba);
+        else
+            hipLaunchKernelGGL((k_cg_matvec2<false, true>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
+                               p_old, p_new, Ap, pb, sc, pa, nba);
+        return W.nblocks;
+    }
+    hipLaunchKernelGGL((k_cg_matvec<double, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old,
+                       p_new, Ap, pb, sc, pa, nba);
+    return G.nblocks;
+}
+void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
+                        const double* pa, int nba, const double* pb, int nbb, hipStream_t st) {
+    hipLaunchKernelGGL((k_cg_update<double, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, nullptr, p, Ap, x, r,
+                       nullptr, 0, nullptr, sc, pa, nba, pb, nbb);
 }
 template <class T>
 int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* dg, const T* p,
@@ -2427,6 +2541,11 @@ static long mg_flat_cells() {
     }();
     return v;
 }
+// Workgroups (and so r'z partials) of the up-leg on an nx x ny level.
+int mg_up_nblocks(int nx, int ny) {
+    if ((long)nx * ny <= mg_flat_cells()) return (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y);
+    return ((nx + MG_TX - 1) / MG_TX) * ((ny + MG_TY - 1) / MG_TY);
+}
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
              const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
@@ -2445,13 +2564,15 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
            double* partials, Scalars* sc, hipStream_t st) {
     const bool flat = (long)nx * ny <= mg_flat_cells();
     const dim3 grid = flat ? cells_grid(nx, ny) : dim3((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
+    // dot_mode 2: the partials of r'z stay pending for the next CG matvec
+    const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
     if (flat)
         hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,
-                           b, x, xc, out, dot_mode >= 0 ? 1 : 0, partials, sc);
+                           b, x, xc, out, with_dot, partials, sc);
     else
         hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x,
-                           xc, out, dot_mode >= 0 ? 1 : 0, partials, sc);
-    if (dot_mode >= 0)
+                           xc, out, with_dot, partials, sc);
+    if (dot_mode == 0 || dot_mode == 1)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
 }
