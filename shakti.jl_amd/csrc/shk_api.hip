@@ -441,8 +441,9 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 // dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1
 // iteration; 2 leaves the partials pending for the next CG matvec; -1 none).
 // With T = double the finest level reads the FP64 residual and writes the FP64
-// z that FP64 CG uses; every level computes in FP32.  The large levels run
-// as tiled kernels, the levels below MG_TAIL_CELLS cells in one workgroup.
+// z that FP64 CG uses; every level computes in FP32.  Levels above
+// MG_FLAT_CELLS cells run marching kernels, the middle levels single-phase
+// kernels, the levels below MG_TAIL_CELLS cells one workgroup.
 template <class T>
 void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
     auto& mg = ctx->mg;
@@ -1383,16 +1384,32 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
             case 1: launch::matvec<float>(G, ctx->aP32, ctx->aE32, ctx->aN32, ctx->x32, ctx->Ap32, ctx->st); break;
             case 2:
             case 4: launch::copy(ctx->h_alt, ctx->d64, ctx->n, ctx->st); break;
+            case 5:
+                launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
+                                           ctx->partials2, ctx->sc, ctx->partials, 0, ctx->st);
+                break;
             default: break;
         }
     };
     if (kernel == 0 || kernel == 3) bytes = 5 * n * 8;
     else if (kernel == 1) bytes = 5 * n * 4;
     else if (kernel == 2 || kernel == 4) bytes = 2 * n * 8;
+    else if (kernel == 5) bytes = 7 * n * 8;
     else {
         hipEventDestroy(a);
         hipEventDestroy(b);
         return fail(ctx, SHK_ERR_ARG, "unknown kernel id");
+    }
+    Scalars saved{};
+    if (kernel == 5) {
+        // the CG matvec runs only while the solve is live: clear done and the
+        // pending flag for the timing, restore the scalars afterwards
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        HIPCHK(hipMemcpy(&saved, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        Scalars live = saved;
+        live.done = 0;
+        live.gpend = 0;
+        HIPCHK(hipMemcpy(ctx->sc, &live, sizeof(Scalars), hipMemcpyHostToDevice));
     }
     one();  // warm
     float ms = 0;
@@ -1427,6 +1444,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
+    if (kernel == 5) HIPCHK(hipMemcpy(ctx->sc, &saved, sizeof(Scalars), hipMemcpyHostToDevice));
     *mean_ms = ms / reps;
     *bytes_per_launch = bytes;
     return SHK_OK;

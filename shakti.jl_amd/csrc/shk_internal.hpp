@@ -74,8 +74,8 @@ struct MGLevelDev {
     float *b, *xa, *xb;
 };
 constexpr int MG_MAX_TAIL = 12;
-// levels up to this size use the single-phase kernels (faster than the tiled
-// ones from 1M cells down; the tiled down-leg is faster on a 4M-cell level)
+// levels up to this size use the single-phase kernels, larger ones the
+// marching kernels
 constexpr long MG_FLAT_CELLS = 1L << 20;
 constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; levels of ~5.5k cells take ~150 KB of LDS
 struct MGTail {

@@ -1500,140 +1500,132 @@ __device__ __forceinline__ float flux_apply(int nx, int ny, int ix, int iy, size
 // how fast CG converges, never the solution CG converges to.
 __device__ __forceinline__ float mg_div(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
 
-// The large levels of the V-cycle work on tiles of MG_TX x MG_TY fine cells
-// (even-aligned, so each tile restricts onto whole coarse cells): the values
-// the 5-point stencil needs at the neighbours are staged once per tile, with
-// a one-cell halo, in LDS.  Workgroups of 64 x 4 threads, 4 rows per thread.
-constexpr int MG_TX = 64, MG_TY = 16, MG_HX = MG_TX + 2, MG_HY = MG_TY + 2;
-
-// Down-leg on one level: one damped-Jacobi sweep from zero, x = omega b / D
-// (stored for the up-leg), and the restricted residual bc = P^T (b - A x).
-// TB is the right-hand side's type: FP64 on the finest level when the
-// V-cycle preconditions FP64 CG, FP32 inside the hierarchy; the arithmetic
-// is FP32.  Skipped once the CG it serves has converged.
+// The two legs for the large levels as marching kernels (the layout of the
+// CG kernels: one wave per 64-column strip walking R rows, R even so a wave
+// owns whole row pairs; rows iy-1, iy, iy+1 in rolling registers, E/W
+// neighbours by lane shuffles, the strip-edge lanes evaluating the
+// neighbouring strip's value).  Every array is read once per launch plus a
+// halo row per R rows; the arithmetic is that of the single-phase kernels
+// below, in the same order.
+//
+// Down-leg: x = omega b / D, r = b - A x; the restriction sums the 2 x 2
+// residuals as (r00 + r01) + (r10 + r11): horizontal pairs by a shuffle from
+// the odd lane, the even row's pair kept in a register until the odd row.
 template <class TB>
-__global__ __launch_bounds__(256) void k_mg_down(int nx, int ny, int nxc, int nyc, float omega,
-                                                 const float* __restrict__ c, const float* __restrict__ e,
-                                                 const float* __restrict__ n, const float* __restrict__ D,
-                                                 const TB* __restrict__ b, float* __restrict__ x,
-                                                 float* __restrict__ bc, const Scalars* __restrict__ sc) {
+__global__ __launch_bounds__(256) void k_mg_down_march(int nx, int ny, int nxc, int R, float omega,
+                                                       const float* __restrict__ c, const float* __restrict__ e,
+                                                       const float* __restrict__ n, const float* __restrict__ D,
+                                                       const TB* __restrict__ b, float* __restrict__ x,
+                                                       float* __restrict__ bc, const Scalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ float xs[MG_HY][MG_HX];
-    __shared__ float rs[MG_TY][MG_TX];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
-    const int x0 = blockIdx.x * MG_TX, y0 = blockIdx.y * MG_TY;
-    {
-        // the halo tile: every load of a thread issued before the first use
-        constexpr int NH = (MG_HX * MG_HY + WAVE * WAVES - 1) / (WAVE * WAVES);
-        TB bv[NH];
-        float dv[NH];
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            const int i = tid + j * WAVE * WAVES;
-            const int hy = i / MG_HX, hx = i - hy * MG_HX;
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1;
-            bv[j] = 0;
-            dv[j] = 1.f;
-            if (i < MG_HX * MG_HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
-                const size_t k = (size_t)gx + (size_t)gy * nx;
-                bv[j] = b[k];
-                dv[j] = D[k];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            const int i = tid + j * WAVE * WAVES;
-            if (i < MG_HX * MG_HY) {
-                const int hy = i / MG_HX, hx = i - hy * MG_HX;
-                xs[hy][hx] = mg_div(omega * (float)bv[j], dv[j]);
-            }
-        }
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const int lane = threadIdx.x;
+    const int ix = vbx * WAVE + lane;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;  // even
+    const int iy1 = min(iy0 + R, ny);
+    if (iy0 >= ny) return;  // uniform over the wave; no barriers follow
+    const bool inx = ix < nx;
+    const int ixc = inx ? ix : nx - 1;  // clamped column for the loads of idle lanes
+    const bool even = (lane & 1) == 0;
+    auto xs = [&](size_t k) -> float { return mg_div(omega * (float)b[k], D[k]); };
+    float um = 0.f, nm = 0.f;
+    if (iy0 > 0) {
+        um = xs(cidx(ixc, iy0 - 1, nx));
+        nm = n[cidx(ixc, iy0 - 1, nx)];
     }
-    __syncthreads();
-    const int gx = x0 + tx;
-#pragma unroll
-    for (int j = 0; j < MG_TY / WAVES; ++j) {
-        const int fy = ty + WAVES * j, gy = y0 + fy;
-        float res = 0.f;
-        if (gx < nx && gy < ny) {
-            const size_t k = (size_t)gx + (size_t)gy * nx;
-            const float u = xs[fy + 1][tx + 1];
-            float a = c[k] * u;
-            if (gx < nx - 1) a += e[k] * (u - xs[fy + 1][tx + 2]);
-            if (gx > 0) a += e[k - 1] * (u - xs[fy + 1][tx]);
-            if (gy < ny - 1) a += n[k] * (u - xs[fy + 2][tx + 1]);
-            if (gy > 0) a += n[k - nx] * (u - xs[fy][tx + 1]);
-            res = (float)b[k] - a;
-            x[k] = u;
+    float uc = xs(cidx(ixc, iy0, nx));
+    float hrow = 0.f;  // the even row's horizontal pair sum
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const size_t k = cidx(ixc, iy, nx);
+        const float up = (iy + 1 < ny) ? xs(k + nx) : 0.f;
+        const float cc = c[k], ec = e[k], nc = n[k];
+        const float bk = (float)b[k];
+        float ue = __shfl_down(uc, 1, WAVE);
+        float uw = __shfl_up(uc, 1, WAVE);
+        float ew = __shfl_up(ec, 1, WAVE);
+        if (lane == WAVE - 1 && ix < nx - 1) ue = xs(k + 1);
+        if (lane == 0 && ix > 0) {
+            uw = xs(k - 1);
+            ew = e[k - 1];
         }
-        rs[fy][tx] = res;
+        float a = cc * uc;
+        if (ix < nx - 1) a += ec * (uc - ue);
+        if (ix > 0) a += ew * (uc - uw);
+        if (iy < ny - 1) a += nc * (uc - up);
+        if (iy > 0) a += nm * (uc - um);
+        const float res = inx ? bk - a : 0.f;
+        if (inx) x[k] = uc;
+        const float hs = res + __shfl_down(res, 1, WAVE);  // meaningful on even lanes
+        if ((iy & 1) == 0) {
+            hrow = hs;
+        } else if (even && inx) {
+            bc[(size_t)(ix >> 1) + (size_t)(iy >> 1) * nxc] = hrow + hs;
+        }
+        um = uc;
+        uc = up;
+        nm = nc;
     }
-    __syncthreads();
-    // one thread per coarse cell of the tile (32 x 8)
-    const int cx = tid % (MG_TX / 2), cy = tid / (MG_TX / 2);
-    const int I = x0 / 2 + cx, J = y0 / 2 + cy;
-    if (I < nxc && J < nyc)
-        bc[(size_t)I + (size_t)J * nxc] =
-            (rs[2 * cy][2 * cx] + rs[2 * cy][2 * cx + 1]) + (rs[2 * cy + 1][2 * cx] + rs[2 * cy + 1][2 * cx + 1]);
+    // odd ny: the last (even) row has no partner row; its coarse cells take
+    // the even row's sums plus the zero residuals of the missing row
+    if ((iy1 & 1) == 1 && even && inx) bc[(size_t)(ix >> 1) + (size_t)((iy1 - 1) >> 1) * nxc] = hrow + 0.f;
 }
 
-// Up-leg on one level: u = x + scale * P xc (the scaled coarse correction),
-// one damped-Jacobi sweep z = u + omega (b - A u) / D, written to out; on the
-// finest level also the partial of b'z (= r'z for CG).
+// Up-leg: u = x + scale P xc, z = u + omega (b - A u) / D (written to out);
+// with_dot: the partial of b'z (2: left pending for the next CG matvec).
 template <class TB>
-__global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float omega, float scale,
-                                               const float* __restrict__ c, const float* __restrict__ e,
-                                               const float* __restrict__ n, const float* __restrict__ D,
-                                               const TB* __restrict__ b, const float* __restrict__ x,
-                                               const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
-                                               double* partials, Scalars* __restrict__ sc) {
+__global__ __launch_bounds__(256) void k_mg_up_march(int nx, int ny, int nxc, int R, float omega, float scale,
+                                                     const float* __restrict__ c, const float* __restrict__ e,
+                                                     const float* __restrict__ n, const float* __restrict__ D,
+                                                     const TB* __restrict__ b, const float* __restrict__ x,
+                                                     const float* __restrict__ xc, TB* __restrict__ out,
+                                                     int with_dot, double* partials, Scalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ float us[MG_HY][MG_HX];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * WAVE;
-    const int x0 = blockIdx.x * MG_TX, y0 = blockIdx.y * MG_TY;
-    {
-        // the halo tile: every load of a thread issued before the first use
-        constexpr int NH = (MG_HX * MG_HY + WAVE * WAVES - 1) / (WAVE * WAVES);
-        float xv[NH], cv[NH];
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            const int i = tid + j * WAVE * WAVES;
-            const int hy = i / MG_HX, hx = i - hy * MG_HX;
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1;
-            xv[j] = 0.f;
-            cv[j] = 0.f;
-            if (i < MG_HX * MG_HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
-                xv[j] = x[(size_t)gx + (size_t)gy * nx];
-                cv[j] = xc[(size_t)(gx >> 1) + (size_t)(gy >> 1) * nxc];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            const int i = tid + j * WAVE * WAVES;
-            if (i < MG_HX * MG_HY) {
-                const int hy = i / MG_HX, hx = i - hy * MG_HX;
-                us[hy][hx] = xv[j] + scale * cv[j];
-            }
-        }
-    }
-    __syncthreads();
-    const int gx = x0 + tx;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const int lane = threadIdx.x;
+    const int ix = vbx * WAVE + lane;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int iy1 = min(iy0 + R, ny);
     double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < MG_TY / WAVES; ++j) {
-        const int fy = ty + WAVES * j, gy = y0 + fy;
-        if (gx < nx && gy < ny) {
-            const size_t k = (size_t)gx + (size_t)gy * nx;
-            const float u = us[fy + 1][tx + 1];
-            float a = c[k] * u;
-            if (gx < nx - 1) a += e[k] * (u - us[fy + 1][tx + 2]);
-            if (gx > 0) a += e[k - 1] * (u - us[fy + 1][tx]);
-            if (gy < ny - 1) a += n[k] * (u - us[fy + 2][tx + 1]);
-            if (gy > 0) a += n[k - nx] * (u - us[fy][tx + 1]);
+    if (iy0 < ny) {  // uniform over the wave (the partial below needs every wave)
+        const bool inx = ix < nx;
+        const int ixc = inx ? ix : nx - 1;
+        auto us = [&](int jx, int jy) -> float {
+            return x[(size_t)jx + (size_t)jy * nx] + scale * xc[(size_t)(jx >> 1) + (size_t)(jy >> 1) * nxc];
+        };
+        float um = 0.f, nm = 0.f;
+        if (iy0 > 0) {
+            um = us(ixc, iy0 - 1);
+            nm = n[cidx(ixc, iy0 - 1, nx)];
+        }
+        float uc = us(ixc, iy0);
+        for (int iy = iy0; iy < iy1; ++iy) {
+            const size_t k = cidx(ixc, iy, nx);
+            const float up = (iy + 1 < ny) ? us(ixc, iy + 1) : 0.f;
+            const float cc = c[k], ec = e[k], nc = n[k], dk = D[k];
             const TB bk = b[k];
-            const float z = u + mg_div(omega * ((float)bk - a), D[k]);
-            out[k] = (TB)z;
-            acc += (double)bk * (double)z;
+            float ue = __shfl_down(uc, 1, WAVE);
+            float uw = __shfl_up(uc, 1, WAVE);
+            float ew = __shfl_up(ec, 1, WAVE);
+            if (lane == WAVE - 1 && ix < nx - 1) ue = us(ix + 1, iy);
+            if (lane == 0 && ix > 0) {
+                uw = us(ix - 1, iy);
+                ew = e[k - 1];
+            }
+            float a = cc * uc;
+            if (ix < nx - 1) a += ec * (uc - ue);
+            if (ix > 0) a += ew * (uc - uw);
+            if (iy < ny - 1) a += nc * (uc - up);
+            if (iy > 0) a += nm * (uc - um);
+            const float z = uc + mg_div(omega * ((float)bk - a), dk);
+            if (inx) {
+                out[k] = (TB)z;
+                acc += (double)bk * (double)z;
+            }
+            um = uc;
+            uc = up;
+            nm = nc;
         }
     }
     if (with_dot) {
@@ -1643,11 +1635,10 @@ __global__ __launch_bounds__(256) void k_mg_up(int nx, int ny, int nxc, float om
     }
 }
 
-// The same two legs for the middle levels (at most MG_FLAT_CELLS cells) in a
-// single phase, one thread per coarse cell (down) or fine cell (up), reading
-// the neighbours' values directly: these levels are too small to fill the
-// chip, and each barrier-separated phase of the tiled kernels adds a global
-// memory round trip to their latency.
+// The same two legs for the middle levels (at most MG_FLAT_CELLS cells), one
+// thread per coarse cell (down) or fine cell (up), reading the neighbours'
+// values directly: at these sizes a marching wave would have few rows to
+// amortise its halo and start-up over.
 template <class TB>
 __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, int nyc, float omega,
                                                       const float* __restrict__ c, const float* __restrict__ e,
@@ -1661,16 +1652,30 @@ __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, i
     auto xs = [&](size_t k) -> float { return mg_div(omega * (float)b[k], D[k]); };
     float acc = 0.f;
 #pragma unroll
-    for (int dy = 0; dy < 2; ++dy)
+    for (int dy = 0; dy < 2; ++dy) {
+        float xv[2] = {0.f, 0.f};
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
             const int ix = 2 * I + dx, iy = 2 * J + dy;
             if (ix < nx && iy < ny) {
                 const size_t k = (size_t)ix + (size_t)iy * nx;
-                x[k] = xs(k);
+                xv[dx] = xs(k);
                 acc += (float)b[k] - flux_apply(nx, ny, ix, iy, k, c, e, n, xs);
             }
         }
+        // the pair of x values as one 8 B store when it is aligned (even nx),
+        // so each cache line is written whole
+        const int iy = 2 * J + dy;
+        if (iy < ny) {
+            const size_t k0 = (size_t)(2 * I) + (size_t)iy * nx;
+            if ((nx & 1) == 0) {
+                *(float2*)(x + k0) = make_float2(xv[0], xv[1]);
+            } else {
+                x[k0] = xv[0];
+                if (2 * I + 1 < nx) x[k0 + 1] = xv[1];
+            }
+        }
+    }
     bc[(size_t)I + (size_t)J * nxc] = acc;
 }
 // with_dot: also the partial of b'z (the finest level of FP64 CG's V-cycle)
@@ -2541,10 +2546,16 @@ static long mg_flat_cells() {
     }();
     return v;
 }
+// Levels above MG_FLAT_CELLS run the marching kernels on the CG geometry
+// with R even.
+static Geom mg_march_geom(int nx, int ny) {
+    const Geom G = make_geom(nx, ny);
+    return G.R % 2 == 0 ? G : make_geom(nx, ny, G.R + 1);
+}
 // Workgroups (and so r'z partials) of the up-leg on an nx x ny level.
 int mg_up_nblocks(int nx, int ny) {
     if ((long)nx * ny <= mg_flat_cells()) return (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y);
-    return ((nx + MG_TX - 1) / MG_TX) * ((ny + MG_TY - 1) / MG_TY);
+    return mg_march_geom(nx, ny).nblocks;
 }
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
@@ -2554,8 +2565,8 @@ void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, cons
                            omega, c, e, n, D, b, x, bc, sc);
         return;
     }
-    const dim3 grid((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
-    hipLaunchKernelGGL(k_mg_down<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc, omega, c, e, n, D, b, x, bc,
+    const Geom M = mg_march_geom(nx, ny);
+    hipLaunchKernelGGL(k_mg_down_march<TB>, M.grid, M.block, 0, st, nx, ny, nxc, M.R, omega, c, e, n, D, b, x, bc,
                        sc);
 }
 template <class TB>
@@ -2563,15 +2574,16 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st) {
     const bool flat = (long)nx * ny <= mg_flat_cells();
-    const dim3 grid = flat ? cells_grid(nx, ny) : dim3((nx + MG_TX - 1) / MG_TX, (ny + MG_TY - 1) / MG_TY);
+    const Geom M = flat ? Geom{} : mg_march_geom(nx, ny);
+    const dim3 grid = flat ? cells_grid(nx, ny) : M.grid;
     // dot_mode 2: the partials of r'z stay pending for the next CG matvec
     const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
     if (flat)
         hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,
                            b, x, xc, out, with_dot, partials, sc);
     else
-        hipLaunchKernelGGL(k_mg_up<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x,
-                           xc, out, with_dot, partials, sc);
+        hipLaunchKernelGGL(k_mg_up_march<TB>, grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b,
+                           x, xc, out, with_dot, partials, sc);
     if (dot_mode == 0 || dot_mode == 1)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
