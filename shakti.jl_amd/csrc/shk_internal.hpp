@@ -75,8 +75,8 @@ struct MGLevelDev {
 };
 constexpr int MG_MAX_TAIL = 12;
 // levels up to this size use the single-phase kernels, larger ones the
-// marching kernels
-constexpr long MG_FLAT_CELLS = 1L << 20;
+// marching kernels (measured faster down to 16k-cell levels at 4096 x 1024)
+constexpr long MG_FLAT_CELLS = 4096;
 constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; levels of ~5.5k cells take ~150 KB of LDS
 struct MGTail {
     int nlev;

@@ -1635,10 +1635,9 @@ __global__ __launch_bounds__(256) void k_mg_up_march(int nx, int ny, int nxc, in
     }
 }
 
-// The same two legs for the middle levels (at most MG_FLAT_CELLS cells), one
-// thread per coarse cell (down) or fine cell (up), reading the neighbours'
-// values directly: at these sizes a marching wave would have few rows to
-// amortise its halo and start-up over.
+// The same two legs for small levels (at most MG_FLAT_CELLS cells) outside
+// the tail kernel, one thread per coarse cell (down) or fine cell (up),
+// reading the neighbours' values directly.
 template <class TB>
 __global__ __launch_bounds__(256) void k_mg_down_flat(int nx, int ny, int nxc, int nyc, float omega,
                                                       const float* __restrict__ c, const float* __restrict__ e,
