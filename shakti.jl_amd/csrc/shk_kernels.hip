@@ -1862,6 +1862,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
 // (Gauss-Jordan without pivoting, the matrix being SPD):
 //   p = A_kk;  A_kk = 1/p;  A_kj /= p (j != k);  A_ij -= A_ik A_kj (i, j != k);
 //   A_ik = -A_ik / p (i != k).
+// Workgroup 64 x 4: thread (tx, ty) owns column j = tx of rows ty, ty+4, ...
 __global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, const float* __restrict__ c,
                                                               const float* __restrict__ e,
                                                               const float* __restrict__ nn,
@@ -1869,11 +1870,13 @@ __global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, co
     __shared__ double A[64 * 64];
     __shared__ double colk[64];
     const int n = nx * ny;
+    const int j = threadIdx.x, ty = threadIdx.y;
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    const int nt = blockDim.x * blockDim.y;
-    for (int t = tid; t < n * n; t += nt) A[t] = 0.0;
+    for (int i = ty; i < n; i += 4)
+        if (j < n) A[i * n + j] = 0.0;
     __syncthreads();
-    for (int k = tid; k < n; k += nt) {
+    if (tid < n) {
+        const int k = tid;
         const int ix = k % nx, iy = k / nx;
         double d = c[k];
         if (ix < nx - 1) { d += e[k]; A[k * n + k + 1] = -(double)e[k]; }
@@ -1884,21 +1887,23 @@ __global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, co
     }
     __syncthreads();
     for (int k = 0; k < n; ++k) {
-        const double p = A[k * n + k];
-        const double ip = 1.0 / p;
+        const double ip = 1.0 / A[k * n + k];
         // the pivot column before this step, then the scaled pivot row
-        for (int i = tid; i < n; i += nt) colk[i] = A[i * n + k];
+        if (ty == 0 && j < n) colk[j] = A[j * n + k];
         __syncthreads();
-        for (int j = tid; j < n; j += nt) A[k * n + j] = (j == k) ? ip : A[k * n + j] * ip;
+        if (ty == 1 && j < n) A[k * n + j] = (j == k) ? ip : A[k * n + j] * ip;
         __syncthreads();
-        for (int t = tid; t < n * n; t += nt) {
-            const int i = t / n, j = t % n;
-            if (i == k) continue;
-            A[t] = (j == k) ? -colk[i] * ip : A[t] - colk[i] * A[k * n + j];
+        if (j < n) {
+            const double akj = A[k * n + j];
+            for (int i = ty; i < n; i += 4) {
+                if (i == k) continue;
+                A[i * n + j] = (j == k) ? -colk[i] * ip : A[i * n + j] - colk[i] * akj;
+            }
         }
         __syncthreads();
     }
-    for (int t = tid; t < n * n; t += nt) inv[t] = (float)A[t];
+    for (int i = ty; i < n; i += 4)
+        if (j < n) inv[i * n + j] = (float)A[i * n + j];
 }
 
 // Coarsest level: assemble the dense matrix (FP64, in `work`) and invert it
