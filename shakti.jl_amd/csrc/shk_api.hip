@@ -100,7 +100,11 @@ struct shk_ctx {
     int mg_stride = 0;
     float* mg_inv = nullptr;   // inside mg_pack
     double* mg_work = nullptr;
-    float mg_omega = 0.8f, mg_scale = 1.5f;  // damped-Jacobi weight, coarse-correction scale (measured best)
+    // damped-Jacobi weight and coarse-correction scale.  The symmetric V-cycle
+    // stays positive definite for omega < 2 / lambda_max(D^-1 A) (>= 1 here)
+    // and scale < 2.  omega 0.95 (against 0.8) cut CG iterations 10-16% on
+    // every grid measured; larger scales helped some systems and hurt others
+    float mg_omega = 0.95f, mg_scale = 1.5f;
     size_t mg_tail_first = 1;                 // first level handled by the single-workgroup tail kernel
 
     // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)

@@ -192,7 +192,9 @@ def test_recorded_cg_pairs_hold_residual_and_remaining_error(oracle, hip, precon
     g, p, m, os_ = oracle_state(oracle, g_args, raw, kfs=1)
     aP, aE, aW, aN, aS, rhs = oracle.assemble_mfls(os_, g, p, 1)
     xh, th = x.download(), t.download()
-    for k in range(0, n, max(1, n // 4)):
+    # pairs of the first half of the solve: their remaining error stands well
+    # above the rounding of x_final - x_k (near convergence it does not)
+    for k in range(0, max(1, n // 2), max(1, n // 8)):
         f0 = xh[k, :ny, :nx, 0].T.astype(np.float64)
         e = th[k, :ny, :nx].T.astype(np.float64) / np.sqrt(aP)
         Ae = aP * e

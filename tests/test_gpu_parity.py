@@ -287,6 +287,7 @@ def test_multigrid_fp32_refinement_solve(oracle, hip, shape):
         ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, rtol=1e-11, **kw)
         hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
         res[name] = (hs.get("h"), ls.ws.stats)
+    print(f"{shape}: CG iterations " + ", ".join(f"{k} {v[1].niter}" for k, v in res.items()), flush=True)
     for name in ("mg", "fp64_mg"):
         st = res[name][1]
         assert st.solved, name
