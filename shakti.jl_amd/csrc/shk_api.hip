@@ -1418,9 +1418,9 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     one();  // warm
     float ms = 0;
     if (kernel >= 3) {
-        // cold: stream a 1 GiB read before every launch, so L2 and the 256 MB
-        // Infinity Cache hold only clean lines of unrelated data; each launch
-        // is timed alone
+        // cold (3, 4, 5): stream a 1 GiB read before every launch, so L2 and
+        // the 256 MB Infinity Cache hold only clean lines of unrelated data;
+        // each launch is timed alone
         const size_t nf = (size_t)1 << 27;
         double* flush = nullptr;
         double* sink = nullptr;
