@@ -99,7 +99,6 @@ struct shk_ctx {
     float* mg_pack = nullptr;  // the tail's [c | e | n | D] (stride mg_stride) and the coarsest inverse
     int mg_stride = 0;
     float* mg_inv = nullptr;   // inside mg_pack
-    double* mg_work = nullptr;
     // damped-Jacobi weight and coarse-correction scale.  The symmetric V-cycle
     // stays positive definite for omega < 2 / lambda_max(D^-1 A) (>= 1 here)
     // and scale < 2.  omega 0.95 (against 0.8) cut CG iterations 10-16% on
@@ -420,7 +419,6 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
             o += (size_t)L.nx * L.ny;
         }
         ctx->mg_inv = ctx->mg_pack + 4 * tp;
-        HIPCHK(hipMalloc((void**)&ctx->mg_work, 2 * nc * nc * sizeof(double)));
         if (const char* e = getenv("SHK_MG_OMEGA")) ctx->mg_omega = (float)atof(e);
         if (const char* e = getenv("SHK_MG_SCALE")) ctx->mg_scale = (float)atof(e);
     }
@@ -437,7 +435,7 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
     }
     auto& Lc = ctx->mg.back();
     Prof pr(ctx, P_MGS, 0);
-    launch::mg_dense_inverse(Lc.nx, Lc.ny, Lc.c, Lc.e, Lc.n, ctx->mg_work, ctx->mg_inv, ctx->st);
+    launch::mg_dense_inverse(Lc.nx, Lc.ny, Lc.c, Lc.e, Lc.n, ctx->mg_inv, ctx->st);
     return SHK_OK;
 }
 
@@ -1119,7 +1117,6 @@ int shk_destroy(shk_ctx* ctx) {
         }
     }
     if (ctx->mg_pack) hipFree(ctx->mg_pack);  // holds mg_inv
-    if (ctx->mg_work) hipFree(ctx->mg_work);
     if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
     void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw, ctx->rec_snap, ctx->rec_scale};
     for (void* p : nnp)

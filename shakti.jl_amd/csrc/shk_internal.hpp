@@ -195,8 +195,7 @@ void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st);
 size_t mg_tail_stride(size_t tail_cells);
 size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells);
 int mg_tail_reserve(size_t bytes);
-void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
-                      hipStream_t st);
+void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, float* inv, hipStream_t st);
 
 // Mixed precision: FP32 copies of the coefficients; the FP64 residual
 // r = b - A x with its Jacobi-weighted norm (first != 0 records the initial

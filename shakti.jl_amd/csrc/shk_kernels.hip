@@ -1858,8 +1858,9 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
     for (int k = tid; k < off[1]; k += nt) T.L[0].xb[k] = Xs[k];
 }
 
-// Coarsest level of at most 64 cells: the same inverse, in place in LDS
-// (Gauss-Jordan without pivoting, the matrix being SPD):
+// Coarsest level (at most 64 cells): assemble the dense FP64 matrix and invert
+// it in place in LDS (Gauss-Jordan without pivoting, the matrix being SPD);
+// the inverse is stored in FP32:
 //   p = A_kk;  A_kk = 1/p;  A_kj /= p (j != k);  A_ij -= A_ik A_kj (i, j != k);
 //   A_ik = -A_ik / p (i != k).
 // Workgroup 64 x 4: thread (tx, ty) owns column j = tx of rows ty, ty+4, ...
@@ -1906,61 +1907,6 @@ __global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, co
         if (j < n) inv[i * n + j] = (float)A[i * n + j];
 }
 
-// Coarsest level: assemble the dense matrix (FP64, in `work`) and invert it
-// in place by Gauss-Jordan elimination (SPD, so no pivoting); the inverse is
-// stored in FP32.  One workgroup; n <= 512.
-__global__ __launch_bounds__(256) void k_mg_dense_inverse(int nx, int ny, const float* __restrict__ c,
-                                                          const float* __restrict__ e,
-                                                          const float* __restrict__ nn, double* __restrict__ work,
-                                                          float* __restrict__ inv) {
-    const int n = nx * ny;
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    const int nt = blockDim.x * blockDim.y;
-    // [A | I] with A in work[0 .. n*n) and I in work[n*n .. 2n*n), row-major
-    double* A = work;
-    double* B = work + (size_t)n * n;
-    for (int t = tid; t < n * n; t += nt) {
-        A[t] = 0.0;
-        B[t] = (t / n == t % n) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        for (int iy = 0; iy < ny; ++iy)
-            for (int ix = 0; ix < nx; ++ix) {
-                const int k = ix + iy * nx;
-                double d = c[k];
-                if (ix < nx - 1) { d += e[k]; A[(size_t)k * n + k + 1] = -(double)e[k]; }
-                if (ix > 0) { d += e[k - 1]; A[(size_t)k * n + k - 1] = -(double)e[k - 1]; }
-                if (iy < ny - 1) { d += nn[k]; A[(size_t)k * n + k + nx] = -(double)nn[k]; }
-                if (iy > 0) { d += nn[k - nx]; A[(size_t)k * n + k - nx] = -(double)nn[k - nx]; }
-                A[(size_t)k * n + k] = d;
-            }
-    }
-    __syncthreads();
-    __shared__ double piv;
-    for (int k = 0; k < n; ++k) {
-        if (tid == 0) piv = A[(size_t)k * n + k];
-        __syncthreads();
-        const double ip = 1.0 / piv;
-        for (int j = tid; j < n; j += nt) {
-            A[(size_t)k * n + j] *= ip;
-            B[(size_t)k * n + j] *= ip;
-        }
-        __syncthreads();
-        for (int t = tid; t < n * n; t += nt) {
-            const int i = t / n, j = t % n;
-            if (i == k) continue;
-            const double f = A[(size_t)i * n + k];
-            if (j != k) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
-            B[(size_t)i * n + j] -= f * B[(size_t)k * n + j];
-        }
-        __syncthreads();
-        for (int i = tid; i < n; i += nt)
-            if (i != k) A[(size_t)i * n + k] = 0.0;
-        __syncthreads();
-    }
-    for (int t = tid; t < n * n; t += nt) inv[t] = (float)B[t];
-}
 
 // ------------------------------------------------ learned warm start (FP64)
 // The network sees the Picard iteration's system Jacobi-scaled: with
@@ -2618,12 +2564,8 @@ void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st) {
     const size_t nc = (size_t)T.L[T.nlev - 1].nx * T.L[T.nlev - 1].ny;
     hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), mg_tail_lds(cells, nc), st, T, sc);
 }
-void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, double* work, float* inv,
-                      hipStream_t st) {
-    if (nx * ny <= 64)
-        hipLaunchKernelGGL(k_mg_dense_inverse_lds, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, inv);
-    else
-        hipLaunchKernelGGL(k_mg_dense_inverse, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, work, inv);
+void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, float* inv, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_dense_inverse_lds, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, inv);
 }
 
 static inline unsigned nn_blocks(size_t n) {
