@@ -79,9 +79,10 @@ preconditioning; same options and errors as `CGIterativeSolver(g,
 MatrixFreeLinearSystem; ...)` (src/linear_solver.jl:832-843).
 
 `form = :fp32_refine` solves in FP32 inside an FP64 iterative-refinement loop
-that stops on the same FP64 residual test; `multigrid = true` (only with
-`:fp32_refine` and no Chebyshev degree) preconditions the FP32 solves with the
-geometric-multigrid V-cycle.
+that stops on the same FP64 residual test; `multigrid = true` (no Chebyshev
+degree) preconditions with the FP32 geometric-multigrid V-cycle: the FP64 CG
+itself with `:fp64` (the fastest form), the FP32 inner solves with
+`:fp32_refine`.
 """
 mutable struct HIPPCGSolver <: Shakti.AbstractIterativeSolver
     ctx::Ptr{Cvoid}
@@ -96,8 +97,8 @@ function HIPPCGSolver(g::Shakti.Grid; chebyshev_degree::Union{Nothing, Int} = no
                       form::Symbol = :fp64, inner_rtol = 1e-3, multigrid::Bool = false)
     amg && error("AMGPreconditioner is CPU/SparseMatrixCSC-only; use chebyshev_degree with HIPPCGSolver.")
     form in (:fp64, :fp32_refine) || error("HIPPCGSolver: unknown form $form")
-    multigrid && (form != :fp32_refine || chebyshev_degree !== nothing) &&
-        error("HIPPCGSolver: multigrid = true needs form = :fp32_refine and no chebyshev_degree")
+    multigrid && chebyshev_degree !== nothing &&
+        error("HIPPCGSolver: multigrid = true replaces chebyshev_degree: give one of them")
     chebyshev_degree !== nothing && chebyshev_degree < 1 &&
         error("ChebyshevPreconditioner: degree must be >= 1 (got $chebyshev_degree)")
     chebyshev_nsteps_estimate >= 2 ||
