@@ -296,7 +296,8 @@ def main():
     mean_ms = event_mean_ms
     bytes_per_launch = d["bytes"] / d["count"]
     how = "profiled step, each launch bracketed by HIP events on the library stream"
-    if dom_name == "cg_matvec" and "cg_matvec_mean_ms" in r["operator"]:
+    fp64_cg = best in ("fp64", "fp64_mg", "learned_mg")  # the forms whose CG matvec is the FP64 kernel
+    if dom_name == "cg_matvec" and fp64_cg and "cg_matvec_mean_ms" in r["operator"]:
         # each launch alone between HIP events on the library stream, after a
         # 1 GiB read (the cache state the solve gives it): comparable with the
         # profiler's mean over the solve's active launches

@@ -101,6 +101,7 @@ struct Geom {
 Geom make_geom(int nx, int ny, int R_override = 0);
 bool wide_ok(const Geom& G);
 Geom wide_geom(const Geom& G);
+bool cg_wide_ok(const Geom& G);
 bool stream_nt(const Geom& G);
 Geom fused_geom(const Geom& G, int H);
 

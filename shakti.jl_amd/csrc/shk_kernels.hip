@@ -644,6 +644,12 @@ __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, c
 // non-temporal loads and stores for the streamed arrays: grids whose arrays
 // exceed the 256 MB Infinity Cache stream them from HBM anyway, and the hint
 // keeps them from evicting what is reused.
+//
+// Tiles (wide_tile): a workgroup's four waves take four adjacent strips over
+// the same R rows, so each row of each array is read as 512 contiguous
+// columns (4 KB).  From HBM this streams faster than four row blocks of one
+// strip: the CG matvec at 4096 x 1024 took 49 against 51 us (57 us for the
+// one-column kernel) with its inputs evicted from the caches.
 typedef double d2v __attribute__((ext_vector_type(2)));
 template <bool NT>
 __device__ __forceinline__ d2v ld2(const double* p) {
@@ -657,14 +663,13 @@ __device__ __forceinline__ void st2(double* p, d2v v) {
 }
 
 template <bool NT, class Opd2, class Opd1, class Epi2>
-__device__ __forceinline__ void march2(int nx, int ny, int R, int vbx, int vby, const double* __restrict__ aP,
+__device__ __forceinline__ void march2(int nx, int ny, int R, int strip, int iy0, const double* __restrict__ aP,
                                        const double* __restrict__ aE, const double* __restrict__ aN, Opd2 opd2,
                                        Opd1 opd1, Epi2 epi2) {
     const int lane = threadIdx.x;
-    const int ix = vbx * 2 * WAVE + 2 * lane;  // columns ix, ix + 1
-    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    const int ix = strip * 2 * WAVE + 2 * lane;  // columns ix, ix + 1
     const int iy1 = min(iy0 + R, ny);
-    if (iy0 >= ny) return;  // uniform over the wave
+    if (iy0 >= ny || strip * 2 * WAVE >= nx) return;  // uniform over the wave
     const bool inx = ix < nx;
     const int ixc = inx ? ix : nx - 2;  // clamped pair for loads of idle lanes
     d2v um = {0.0, 0.0}, aNm = {0.0, 0.0};
@@ -704,14 +709,22 @@ __device__ __forceinline__ void march2(int nx, int ny, int R, int vbx, int vby, 
     }
 }
 
+// Strip and first row of this wave in the wide kernels' tiling (wide_geom).
+__device__ __forceinline__ void wide_tile(int R, int& strip, int& iy0) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    strip = vbx * WAVES + threadIdx.y;
+    iy0 = vby * R;
+}
+
 template <bool NT>
 __global__ __launch_bounds__(256) void k_matvec2(int nx, int ny, int R, const double* __restrict__ aP,
                                                  const double* __restrict__ aE, const double* __restrict__ aN,
                                                  const double* __restrict__ x, double* __restrict__ y) {
-    int vbx, vby;
-    virtual_block(vbx, vby);
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
     // the operand is re-read by the strip-edge lanes of the neighbouring strips: plain loads
-    march2<NT>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> d2v { return ld2<false>(x + k); },
+    march2<NT>(nx, ny, R, strip, iy0, aP, aE, aN, [&](size_t k) -> d2v { return ld2<false>(x + k); },
                [&](size_t k) -> double { return x[k]; }, [&](size_t c, d2v Au, d2v) { st2<NT>(y + c, Au); });
 }
 
@@ -1126,13 +1139,13 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
                                                     double* partials, Scalars* sc, const double* __restrict__ pa,
                                                     int nba) {
     if (sc->done) return;
-    int vbx, vby;
-    virtual_block(vbx, vby);
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
     double beta = sc->beta;
     if (deferred_gamma<DEFER>(sc, pa, nba, beta)) return;
     double acc = 0.0;
     march2<NT>(
-        nx, ny, R, vbx, vby, aP, aE, aN,
+        nx, ny, R, strip, iy0, aP, aE, aN,
         [&](size_t j) -> d2v { return ld2<false>(z + j) + beta * ld2<false>(p_old + j); },
         [&](size_t j) -> double { return z[j] + beta * p_old[j]; },
         [&](size_t c, d2v Au, d2v u) {
@@ -1193,6 +1206,50 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
     if (jacobi) {
         double v[1] = {acc};
         block_partial<1, false>(v, partials);
+    }
+}
+
+// k_cg_update<double, true> (no Jacobi) over the flat cell index: two cells per
+// lane (16 B accesses), grid-stride.  Pointwise work streams faster flat than
+// in the stencil kernels' strips: 36 against 42 us at 4096 x 1024 with the
+// vectors evicted from the caches.  (64, 4) workgroups, as sum_partials needs.
+__global__ __launch_bounds__(256) void k_cg_update_flat(size_t n, const double* __restrict__ p,
+                                                        const double* __restrict__ Ap, double* __restrict__ x,
+                                                        double* __restrict__ r, Scalars* sc,
+                                                        const double* __restrict__ pa, int nba,
+                                                        const double* __restrict__ pb, int nbb) {
+    if (sc->done) return;
+    const double pAp = sum_partials(pb, nbb);
+    const double g = sum_partials(pa, nba);
+    const double pn2 = sc->pNorm2;
+    const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
+    const double alpha = g / pAp;
+    if (lead_thread()) {
+        sc->pAp = pAp;
+        sc->gamma = g;
+        if (zc) {
+            sc->zero_curv = 1;
+            sc->done = 1;
+        } else {
+            sc->alpha = alpha;
+        }
+    }
+    if (zc) return;
+    const size_t tid = threadIdx.x + (size_t)threadIdx.y * WAVE;
+    for (size_t i = ((size_t)blockIdx.x * (WAVE * WAVES) + tid) * 2; i < n; i += (size_t)gridDim.x * 2 * WAVE * WAVES) {
+        if (i + 1 < n) {
+            const d2v pv = *(const d2v*)(p + i), av = *(const d2v*)(Ap + i);
+            d2v xv = *(const d2v*)(x + i), rv = *(const d2v*)(r + i);
+            xv.x = xv.x + alpha * pv.x;
+            xv.y = xv.y + alpha * pv.y;
+            rv.x = rv.x - alpha * av.x;
+            rv.y = rv.y - alpha * av.y;
+            *(d2v*)(x + i) = xv;
+            *(d2v*)(r + i) = rv;
+        } else {
+            x[i] = x[i] + alpha * p[i];
+            r[i] = r[i] - alpha * Ap[i];
+        }
     }
 }
 
@@ -1622,6 +1679,172 @@ __global__ __launch_bounds__(256) void k_mg_up_march(int nx, int ny, int nxc, in
             if (inx) {
                 out[k] = (TB)z;
                 acc += (double)bk * (double)z;
+            }
+            um = uc;
+            uc = up;
+            nm = nc;
+        }
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
+    }
+}
+
+// The two legs on two adjacent columns per lane (even nx): lane l owns
+// columns 2l, 2l+1 of its 128-column strip, which is coarse column l, so the
+// restriction's horizontal pair sum stays in the lane and the prolongation
+// reads one coarse value per lane and row.  Tiles as the wide CG matvec
+// (wide_tile, R even); the per-cell arithmetic and its order are those of
+// the one-column kernels, so x, bc and z are identical to theirs.
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <class TB>
+struct Pair;
+template <>
+struct Pair<float> {
+    typedef f2v type;
+};
+template <>
+struct Pair<double> {
+    typedef d2v type;
+};
+
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_down_march2(int nx, int ny, int nxc, int R, float omega,
+                                                        const float* __restrict__ c, const float* __restrict__ e,
+                                                        const float* __restrict__ n, const float* __restrict__ D,
+                                                        const TB* __restrict__ b, float* __restrict__ x,
+                                                        float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    if (sc->done) return;
+    int strip, iy0;
+    wide_tile(R, strip, iy0);  // iy0 even
+    const int lane = threadIdx.x;
+    const int ix = strip * 2 * WAVE + 2 * lane;  // columns ix, ix + 1
+    const int iy1 = min(iy0 + R, ny);
+    if (iy0 >= ny || strip * 2 * WAVE >= nx) return;  // uniform over the wave; no barriers follow
+    const bool inx = ix < nx;
+    const int ixc = inx ? ix : nx - 2;  // clamped pair for the loads of idle lanes
+    auto xs2 = [&](size_t k) -> f2v {
+        const TB2 bv = *(const TB2*)(b + k);
+        const f2v dv = *(const f2v*)(D + k);
+        return f2v{mg_div(omega * (float)bv.x, dv.x), mg_div(omega * (float)bv.y, dv.y)};
+    };
+    auto xs1 = [&](size_t k) -> float { return mg_div(omega * (float)b[k], D[k]); };
+    f2v um = {0.f, 0.f}, nm = {0.f, 0.f};
+    if (iy0 > 0) {
+        um = xs2(cidx(ixc, iy0 - 1, nx));
+        nm = *(const f2v*)(n + cidx(ixc, iy0 - 1, nx));
+    }
+    f2v uc = xs2(cidx(ixc, iy0, nx));
+    float hrow = 0.f;  // the even row's pair sum
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const size_t k = cidx(ixc, iy, nx);
+        f2v up = {0.f, 0.f};
+        if (iy + 1 < ny) up = xs2(k + nx);
+        const f2v cc = *(const f2v*)(c + k), ec = *(const f2v*)(e + k), nc = *(const f2v*)(n + k);
+        const TB2 bk = *(const TB2*)(b + k);
+        float ue = __shfl_down(uc.x, 1, WAVE);  // column ix + 2
+        float uw = __shfl_up(uc.y, 1, WAVE);    // column ix - 1
+        float ew = __shfl_up(ec.y, 1, WAVE);    // coupling (ix - 1, ix)
+        if (lane == WAVE - 1 && ix + 2 < nx) ue = xs1(k + 2);
+        if (lane == 0 && ix > 0) {
+            uw = xs1(k - 1);
+            ew = e[k - 1];
+        }
+        float a0 = cc.x * uc.x;
+        a0 += ec.x * (uc.x - uc.y);  // ix < nx - 1 for every active pair
+        if (ix > 0) a0 += ew * (uc.x - uw);
+        if (iy < ny - 1) a0 += nc.x * (uc.x - up.x);
+        if (iy > 0) a0 += nm.x * (uc.x - um.x);
+        float a1 = cc.y * uc.y;
+        if (ix + 1 < nx - 1) a1 += ec.y * (uc.y - ue);
+        a1 += ec.x * (uc.y - uc.x);
+        if (iy < ny - 1) a1 += nc.y * (uc.y - up.y);
+        if (iy > 0) a1 += nm.y * (uc.y - um.y);
+        const float r0 = inx ? (float)bk.x - a0 : 0.f, r1 = inx ? (float)bk.y - a1 : 0.f;
+        if (inx) *(f2v*)(x + k) = uc;
+        const float hs = r0 + r1;
+        if ((iy & 1) == 0) {
+            hrow = hs;
+        } else if (inx) {
+            bc[(size_t)(ix >> 1) + (size_t)(iy >> 1) * nxc] = hrow + hs;
+        }
+        um = uc;
+        uc = up;
+        nm = nc;
+    }
+    // odd ny: the last (even) row has no partner row
+    if ((iy1 & 1) == 1 && inx) bc[(size_t)(ix >> 1) + (size_t)((iy1 - 1) >> 1) * nxc] = hrow + 0.f;
+}
+
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_up_march2(int nx, int ny, int nxc, int R, float omega, float scale,
+                                                      const float* __restrict__ c, const float* __restrict__ e,
+                                                      const float* __restrict__ n, const float* __restrict__ D,
+                                                      const TB* __restrict__ b, const float* __restrict__ x,
+                                                      const float* __restrict__ xc, TB* __restrict__ out,
+                                                      int with_dot, double* partials, Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    if (sc->done) return;
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
+    const int lane = threadIdx.x;
+    const int ix = strip * 2 * WAVE + 2 * lane;
+    const int iy1 = min(iy0 + R, ny);
+    double acc = 0.0;
+    if (iy0 < ny && strip * 2 * WAVE < nx) {  // uniform over the wave (the partial below needs every wave)
+        const bool inx = ix < nx;
+        const int ixc = inx ? ix : nx - 2;
+        auto us2 = [&](int jx, int jy) -> f2v {
+            const f2v xv = *(const f2v*)(x + (size_t)jx + (size_t)jy * nx);
+            const float cv = xc[(size_t)(jx >> 1) + (size_t)(jy >> 1) * nxc];
+            return f2v{xv.x + scale * cv, xv.y + scale * cv};
+        };
+        auto us1 = [&](int jx, int jy) -> float {
+            return x[(size_t)jx + (size_t)jy * nx] + scale * xc[(size_t)(jx >> 1) + (size_t)(jy >> 1) * nxc];
+        };
+        f2v um = {0.f, 0.f}, nm = {0.f, 0.f};
+        if (iy0 > 0) {
+            um = us2(ixc, iy0 - 1);
+            nm = *(const f2v*)(n + cidx(ixc, iy0 - 1, nx));
+        }
+        f2v uc = us2(ixc, iy0);
+        for (int iy = iy0; iy < iy1; ++iy) {
+            const size_t k = cidx(ixc, iy, nx);
+            f2v up = {0.f, 0.f};
+            if (iy + 1 < ny) up = us2(ixc, iy + 1);
+            const f2v cc = *(const f2v*)(c + k), ec = *(const f2v*)(e + k), nc = *(const f2v*)(n + k),
+                      dk = *(const f2v*)(D + k);
+            const TB2 bk = *(const TB2*)(b + k);
+            float ue = __shfl_down(uc.x, 1, WAVE);
+            float uw = __shfl_up(uc.y, 1, WAVE);
+            float ew = __shfl_up(ec.y, 1, WAVE);
+            if (lane == WAVE - 1 && ix + 2 < nx) ue = us1(ix + 2, iy);
+            if (lane == 0 && ix > 0) {
+                uw = us1(ix - 1, iy);
+                ew = e[k - 1];
+            }
+            float a0 = cc.x * uc.x;
+            a0 += ec.x * (uc.x - uc.y);
+            if (ix > 0) a0 += ew * (uc.x - uw);
+            if (iy < ny - 1) a0 += nc.x * (uc.x - up.x);
+            if (iy > 0) a0 += nm.x * (uc.x - um.x);
+            float a1 = cc.y * uc.y;
+            if (ix + 1 < nx - 1) a1 += ec.y * (uc.y - ue);
+            a1 += ec.x * (uc.y - uc.x);
+            if (iy < ny - 1) a1 += nc.y * (uc.y - up.y);
+            if (iy > 0) a1 += nm.y * (uc.y - um.y);
+            const float z0 = uc.x + mg_div(omega * ((float)bk.x - a0), dk.x);
+            const float z1 = uc.y + mg_div(omega * ((float)bk.y - a1), dk.y);
+            if (inx) {
+                TB2 zv;
+                zv.x = (TB)z0;
+                zv.y = (TB)z1;
+                *(TB2*)(out + k) = zv;
+                acc += (double)bk.x * (double)z0;
+                acc += (double)bk.y * (double)z1;
             }
             um = uc;
             uc = up;
@@ -2252,18 +2475,26 @@ Geom make_geom(int nx, int ny, int R_override) {
 // Used where the arrays stream from HBM (stream_nt): inside the Infinity
 // Cache the one-column kernels measured faster (6.4 against 5.8 TB/s for the
 // operator at 4096 x 1024).
-bool wide_ok(const Geom& G) {
+static bool wide_off() {
     static const bool off = getenv("SHK_NO_WIDE") != nullptr;
-    return !off && G.nx % 2 == 0 && G.nx >= 2 && stream_nt(G);
+    return off;
 }
+bool wide_ok(const Geom& G) { return !wide_off() && G.nx % 2 == 0 && G.nx >= 2 && stream_nt(G); }
+// Tiles of four 128-column strips by R rows (wide_tile); R for about 4096
+// waves, at most 8 (at 4096 x 1024, R = 8 measured faster than 4 or 16).
 Geom wide_geom(const Geom& G) {
     Geom W = G;
     const long strips = (G.nx + 2 * WAVE - 1) / (2 * WAVE);
-    long r = ((long)G.ny * strips) / 8192;
-    W.R = (int)(r < 1 ? 1 : (r > 32 ? 32 : r));
-    W.grid = dim3((unsigned)strips, (G.ny + WAVES * W.R - 1) / (WAVES * W.R));
+    long r = ((long)G.ny * strips) / 4096;
+    W.R = (int)(r < 1 ? 1 : (r > 8 ? 8 : r));
+    W.grid = dim3((unsigned)((strips + WAVES - 1) / WAVES), (unsigned)((G.ny + W.R - 1) / W.R));
     W.nblocks = W.grid.x * W.grid.y;
     return W;
+}
+// The CG matvec (seven arrays per cell) also takes the wide kernel on large
+// grids inside the Infinity Cache: 41 against 45 us at 4096 x 1024.
+bool cg_wide_ok(const Geom& G) {
+    return !wide_off() && G.nx % 2 == 0 && G.nx >= 2 && (stream_nt(G) || (long)G.nx * G.ny >= (1L << 21));
 }
 // Non-temporal streaming once the operator's arrays (40 B per cell) exceed
 // the 256 MB Infinity Cache.
@@ -2408,7 +2639,7 @@ template <class T>
 void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z, const T* p_old, T* p_new, T* Ap,
                double* partials, Scalars* sc, hipStream_t st) {
     if constexpr (sizeof(T) == 8) {
-        if (wide_ok(G)) {
+        if (cg_wide_ok(G)) {
             const Geom W = wide_geom(G);
             if (stream_nt(G))
                 hipLaunchKernelGGL((k_cg_matvec2<true, false>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
@@ -2434,7 +2665,7 @@ void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, 
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
                        int nba, hipStream_t st) {
-    if (wide_ok(G)) {
+    if (cg_wide_ok(G)) {
         const Geom W = wide_geom(G);
         if (stream_nt(G))
             hipLaunchKernelGGL((k_cg_matvec2<true, true>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
@@ -2450,8 +2681,12 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
 }
 void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
                         const double* pa, int nba, const double* pb, int nbb, hipStream_t st) {
-    hipLaunchKernelGGL((k_cg_update<double, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, nullptr, p, Ap, x, r,
-                       nullptr, 0, nullptr, sc, pa, nba, pb, nbb);
+    // 2048 workgroups (one per 8 waves of the chip): each re-sums the partials
+    // first, so more workgroups cost more than their better streaming gains
+    // (in the solve at 4096 x 1024: 36.9 us at 2048, 38.9 at 4096, 44.2 at 8192)
+    const size_t n = (size_t)G.nx * G.ny;
+    const unsigned nblk = (unsigned)std::min<size_t>(2048, (n + 511) / 512);
+    hipLaunchKernelGGL(k_cg_update_flat, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, p, Ap, x, r, sc, pa, nba, pb, nbb);
 }
 template <class T>
 int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* dg, const T* p,
@@ -2502,10 +2737,26 @@ static Geom mg_march_geom(int nx, int ny) {
     const Geom G = make_geom(nx, ny);
     return G.R % 2 == 0 ? G : make_geom(nx, ny, G.R + 1);
 }
+// Marching levels of even width take the two-column legs: tiles as the wide
+// CG matvec, R even and chosen for about 8192 waves (at 4096 x 1024, in the
+// solve: 30.6 / 28.6 us down / up, against 34.8 / 30.3 for 4096 waves and
+// 32.5 / 30.0 for the one-column legs).  Same results as the one-column legs;
+// also faster where the one-column grid leaves a partial round of
+// workgroups (4160 x 1024: 37.5 against 45 us for the down-leg, caches cold).
+static bool mg_wide(int nx) { return !wide_off() && nx % 2 == 0; }
+static Geom mg_wide_geom(int nx, int ny) {
+    Geom W = make_geom(nx, ny);
+    const long strips = (nx + 2 * WAVE - 1) / (2 * WAVE);
+    long r = ((long)ny * strips) / 8192;
+    W.R = (int)(r < 2 ? 2 : (r > 8 ? 8 : r + (r & 1)));
+    W.grid = dim3((unsigned)((strips + WAVES - 1) / WAVES), (unsigned)((ny + W.R - 1) / W.R));
+    W.nblocks = W.grid.x * W.grid.y;
+    return W;
+}
 // Workgroups (and so r'z partials) of the up-leg on an nx x ny level.
 int mg_up_nblocks(int nx, int ny) {
     if ((long)nx * ny <= mg_flat_cells()) return (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y);
-    return mg_march_geom(nx, ny).nblocks;
+    return mg_wide(nx) ? mg_wide_geom(nx, ny).nblocks : mg_march_geom(nx, ny).nblocks;
 }
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
@@ -2513,6 +2764,12 @@ void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, cons
     if ((long)nx * ny <= mg_flat_cells()) {
         hipLaunchKernelGGL(k_mg_down_flat<TB>, cells_grid(nxc, nyc), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc,
                            omega, c, e, n, D, b, x, bc, sc);
+        return;
+    }
+    if (mg_wide(nx)) {
+        const Geom W = mg_wide_geom(nx, ny);
+        hipLaunchKernelGGL(k_mg_down_march2<TB>, W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b, x,
+                           bc, sc);
         return;
     }
     const Geom M = mg_march_geom(nx, ny);
@@ -2524,13 +2781,17 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, hipStream_t st) {
     const bool flat = (long)nx * ny <= mg_flat_cells();
-    const Geom M = flat ? Geom{} : mg_march_geom(nx, ny);
+    const bool wide = !flat && mg_wide(nx);
+    const Geom M = flat ? Geom{} : (wide ? mg_wide_geom(nx, ny) : mg_march_geom(nx, ny));
     const dim3 grid = flat ? cells_grid(nx, ny) : M.grid;
     // dot_mode 2: the partials of r'z stay pending for the next CG matvec
     const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
     if (flat)
         hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,
                            b, x, xc, out, with_dot, partials, sc);
+    else if (wide)
+        hipLaunchKernelGGL(k_mg_up_march2<TB>, grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b,
+                           x, xc, out, with_dot, partials, sc);
     else
         hipLaunchKernelGGL(k_mg_up_march<TB>, grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b,
                            x, xc, out, with_dot, partials, sc);
