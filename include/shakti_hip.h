@@ -187,6 +187,13 @@ int shk_step(shk_ctx* ctx, double dt, int32_t ieb_mode, double ieb_uniform, int3
  * nx*ny doubles. */
 int shk_assemble(shk_ctx* ctx, int32_t parabolic, double dt);
 int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host);
+/* One application z = M r of the multigrid preconditioner (the configured
+ * one must be SHK_PRECOND_MULTIGRID) built from the last assembly, as the
+ * solve applies it: one V-cycle of the FP32 flux-form hierarchy with an FP64
+ * r and z.  Host buffers of nx*ny doubles.  No counterpart in the reference
+ * (whose preconditioners are Jacobi, Chebyshev and CPU-side AMG); for
+ * callers that run their own Krylov loop, and for checking the V-cycle. */
+int shk_apply_precond(shk_ctx* ctx, const double* r_host, double* z_host);
 /* Download the last assembly in the reference's MatrixFreeLinearSystem layout
  * (aP, aE, aW, aN, aS, rhs; each nx*ny). Any pointer may be NULL. */
 int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double* aN, double* aS, double* rhs);

@@ -87,6 +87,7 @@ struct MGTail {
     // `stride` floats (the tail's cell count rounded up to 4), then the inverse
     const float* pack;
     int stride;
+    int rb;  // 1: red-black Gauss-Seidel smoothing, 0: damped Jacobi
     MGLevelDev L[MG_MAX_TAIL];
 };
 
@@ -166,7 +167,7 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
                        int nba, hipStream_t st);
 void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
                         const double* pa, int nba, const double* pb, int nbb, hipStream_t st);
-int mg_up_nblocks(int nx, int ny);
+int mg_up_nblocks(int nx, int ny, int rb);
 template <class T>
 void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st);
@@ -185,11 +186,11 @@ void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st);
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
-             const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
+             const float* D, const TB* b, float* x, float* bc, const Scalars* sc, int rb, hipStream_t st);
 template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
-           double* partials, Scalars* sc, hipStream_t st);
+           double* partials, Scalars* sc, int rb, hipStream_t st);
 void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st);
 // dynamic LDS bytes of the tail kernel (rhs, x, c, e, n, D per cell and the
 // coarsest inverse), and the opt-in for more than the default 64 KB (0 = ok)

@@ -1557,6 +1557,62 @@ __device__ __forceinline__ float flux_apply(int nx, int ny, int ix, int iy, size
 // how fast CG converges, never the solution CG converges to.
 __device__ __forceinline__ float mg_div(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
 
+// ---- red-black Gauss-Seidel smoothing (the default V-cycle smoother) ----
+// A cell (ix, iy) is red when ix + iy is even.  Down-leg from x = 0: red
+// cells take x = omega b / D, then black cells x = omega (b + sum_f a_f x_f) / D
+// from their (red) neighbours, b - A x at a black cell with x_b = 0 being
+// b + sum_f a_f x_f in the flux form.  Up-leg, the adjoint order: black cells
+// u += omega (b - A u) / D, then red cells the same with the updated black
+// neighbours.  Per cell the terms are taken in a fixed order (W, E, N, S for
+// the black down-leg value, E, W, N, S for the flux form), and every kernel
+// forms a value through these helpers, so a lane that recomputes a
+// neighbouring strip's cell gets the owner's bits.  An absent face passes
+// zero coupling and zero value.
+__device__ __forceinline__ float rb_black0(float b, float d, float omega, float aW, float xW, float aE, float xE,
+                                           float aN, float xN, float aS, float xS) {
+    float s = b;
+    s += aW * xW;
+    s += aE * xE;
+    s += aN * xN;
+    s += aS * xS;
+    return mg_div(omega * s, d);
+}
+// u + omega (b - A u) / D with A u in the flux form (terms E, W, N, S)
+__device__ __forceinline__ float rb_relax(float b, float c, float d, float omega, float u, bool hE, float aE,
+                                          float uE, bool hW, float aW, float uW, bool hN, float aN, float uN, bool hS,
+                                          float aS, float uS) {
+    float a = c * u;
+    if (hE) a += aE * (u - uE);
+    if (hW) a += aW * (u - uW);
+    if (hN) a += aN * (u - uN);
+    if (hS) a += aS * (u - uS);
+    return u + mg_div(omega * (b - a), d);
+}
+// The down-leg's x at any cell, from global memory (strip-edge lanes, flat kernels).
+template <class TB>
+__device__ __forceinline__ float rb_down_x(int nx, int ny, int jx, int jy, float omega, const float* __restrict__ e,
+                                           const float* __restrict__ n, const float* __restrict__ D,
+                                           const TB* __restrict__ b) {
+    const size_t k = (size_t)jx + (size_t)jy * nx;
+    if (((jx + jy) & 1) == 0) return mg_div(omega * (float)b[k], D[k]);
+    const bool hW = jx > 0, hE = jx < nx - 1, hN = jy < ny - 1, hS = jy > 0;
+    auto red = [&](size_t q) -> float { return mg_div(omega * (float)b[q], D[q]); };
+    return rb_black0((float)b[k], D[k], omega, hW ? e[k - 1] : 0.f, hW ? red(k - 1) : 0.f, hE ? e[k] : 0.f,
+                     hE ? red(k + 1) : 0.f, hN ? n[k] : 0.f, hN ? red(k + nx) : 0.f, hS ? n[k - nx] : 0.f,
+                     hS ? red(k - nx) : 0.f);
+}
+// The up-leg's black-updated value at a black cell, u given by a functor of (jx, jy).
+template <class TB, class U>
+__device__ __forceinline__ float rb_up_black(int nx, int ny, int jx, int jy, float omega, const float* __restrict__ c,
+                                             const float* __restrict__ e, const float* __restrict__ n,
+                                             const float* __restrict__ D, const TB* __restrict__ b, U u) {
+    const size_t k = (size_t)jx + (size_t)jy * nx;
+    const bool hE = jx < nx - 1, hW = jx > 0, hN = jy < ny - 1, hS = jy > 0;
+    return rb_relax((float)b[k], c[k], D[k], omega, u(jx, jy), hE, hE ? e[k] : 0.f, hE ? u(jx + 1, jy) : 0.f, hW,
+                    hW ? e[k - 1] : 0.f, hW ? u(jx - 1, jy) : 0.f, hN, hN ? n[k] : 0.f, hN ? u(jx, jy + 1) : 0.f,
+                    hS, hS ? n[k - nx] : 0.f, hS ? u(jx, jy - 1) : 0.f);
+}
+
 // The two legs for the large levels as marching kernels (the layout of the
 // CG kernels: one wave per 64-column strip walking R rows, R even so a wave
 // owns whole row pairs; rows iy-1, iy, iy+1 in rolling registers, E/W
@@ -1858,6 +1914,304 @@ __global__ __launch_bounds__(256) void k_mg_up_march2(int nx, int ny, int nxc, i
     }
 }
 
+// Red-black legs on two columns per lane (even nx).  A wave covers 128
+// columns of which it owns the middle 124 (RB_OWN): lanes 0 and 63 are halo
+// lanes that compute the neighbouring strips' two nearest columns by the
+// same code, so every value an owned cell needs comes by shuffle (the halo
+// lanes' own outer neighbours are unknown; nothing owned depends on them).
+// Lane l holds columns ix, ix + 1 with ix = 124 strip - 2 + 2 l; in row j
+// its red cell is column ix when j is even, ix + 1 when j is odd.
+// Workgroups of four side-by-side strips over R rows (wide_tile), R even.
+constexpr int RB_OWN = 2 * (WAVE - 2);
+
+// Down-leg from x = 0: red, then black from its red neighbours; the wave runs
+// two rows ahead (step t forms the red values of row t+2, the full x of row
+// t+1 and the residual and restriction of row t, as k_mg_down_march2).
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, int R, float omega,
+                                                     const float* __restrict__ c, const float* __restrict__ e,
+                                                     const float* __restrict__ n, const float* __restrict__ D,
+                                                     const TB* __restrict__ b, float* __restrict__ x,
+                                                     float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    if (sc->done) return;
+    int strip, iy0;
+    wide_tile(R, strip, iy0);  // iy0 even
+    const int lane = threadIdx.x;
+    const int ix = strip * RB_OWN - 2 + 2 * lane;
+    const int iy1 = min(iy0 + R, ny);
+    if (iy0 >= ny || strip * RB_OWN >= nx) return;  // uniform over the wave; no barriers follow
+    const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
+    const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);  // clamped pair for the loads of lanes outside
+    const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;         // faces west of ix, east of ix + 1
+    const f2v z2 = {0.f, 0.f};
+    auto red_row = [&](int j, f2v& bj, f2v& dj) -> f2v {
+        f2v r = z2;
+        bj = z2;
+        dj = z2;
+        if (j < 0 || j >= ny) return r;
+        const size_t k = cidx(ixc, j, nx);
+        const TB2 bv = *(const TB2*)(b + k);
+        bj = f2v{(float)bv.x, (float)bv.y};  // b is used in FP32 only
+        dj = *(const f2v*)(D + k);
+        if ((j & 1) == 0) r.x = mg_div(omega * bj.x, dj.x);
+        else r.y = mg_div(omega * bj.y, dj.y);
+        return r;
+    };
+    auto full_row = [&](int j, f2v rm, f2v rc, f2v rp, f2v bj, f2v dj, f2v ej, f2v nj, f2v nm) -> f2v {
+        if (j < 0 || j >= ny) return z2;
+        const bool hN = j < ny - 1, hS = j > 0;
+        f2v v = rc;
+        if ((j & 1) == 0) {  // black: column ix + 1
+            const float xE = __shfl_down(rc.x, 1, WAVE);
+            v.y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f, hN ? nj.y : 0.f,
+                            hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
+        } else {  // black: column ix
+            const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+            v.x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y, hN ? nj.x : 0.f,
+                            hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
+        }
+        return v;
+    };
+    f2v b0, b1, d0, d1;
+    f2v r0 = red_row(iy0 - 2, b0, d0);
+    f2v r1 = red_row(iy0 - 1, b1, d1);
+    f2v e0 = z2, n0 = z2, nm = z2, X0 = z2, Xm = z2;
+    if (iy0 >= 2) n0 = *(const f2v*)(n + cidx(ixc, iy0 - 2, nx));
+    float hrow = 0.f;  // the even row's pair sum
+    for (int t = iy0 - 2; t < iy1; ++t) {
+        f2v b2, d2;
+        const f2v r2 = red_row(t + 2, b2, d2);
+        f2v e1 = z2, n1 = z2;
+        if (t + 1 >= 0 && t + 1 < ny) {
+            const size_t k1 = cidx(ixc, t + 1, nx);
+            e1 = *(const f2v*)(e + k1);
+            n1 = *(const f2v*)(n + k1);
+        }
+        const f2v X1 = full_row(t + 1, r0, r1, r2, b1, d1, e1, n1, n0);
+        if (t >= iy0) {
+            const size_t k = cidx(ixc, t, nx);
+            const f2v cc = *(const f2v*)(c + k);
+            const float ue = __shfl_down(X0.x, 1, WAVE);  // column ix + 2
+            const float uw = __shfl_up(X0.y, 1, WAVE);    // column ix - 1
+            const float ew = __shfl_up(e0.y, 1, WAVE);    // coupling (ix - 1, ix)
+            float a0 = cc.x * X0.x;
+            a0 += e0.x * (X0.x - X0.y);  // ix < nx - 1 for every owned pair
+            if (hW0) a0 += ew * (X0.x - uw);
+            if (t < ny - 1) a0 += n0.x * (X0.x - X1.x);
+            if (t > 0) a0 += nm.x * (X0.x - Xm.x);
+            float a1 = cc.y * X0.y;
+            if (hE1) a1 += e0.y * (X0.y - ue);
+            a1 += e0.x * (X0.y - X0.x);
+            if (t < ny - 1) a1 += n0.y * (X0.y - X1.y);
+            if (t > 0) a1 += nm.y * (X0.y - Xm.y);
+            const float q0 = own ? b0.x - a0 : 0.f, q1 = own ? b0.y - a1 : 0.f;
+            if (own) *(f2v*)(x + k) = X0;
+            const float hs = q0 + q1;
+            if ((t & 1) == 0) {
+                hrow = hs;
+            } else if (own) {
+                bc[(size_t)(ix >> 1) + (size_t)(t >> 1) * nxc] = hrow + hs;
+            }
+        }
+        Xm = X0;
+        X0 = X1;
+        r0 = r1;
+        r1 = r2;
+        b0 = b1;
+        b1 = b2;
+        d1 = d2;
+        nm = n0;
+        n0 = n1;
+        e0 = e1;
+    }
+    // odd ny: the last (even) row has no partner row
+    if ((iy1 & 1) == 1 && own) bc[(size_t)(ix >> 1) + (size_t)((iy1 - 1) >> 1) * nxc] = hrow + 0.f;
+}
+
+// Up-leg: u = x + scale P xc; black cells relax from u, then red cells from
+// the relaxed black neighbours.  Step t reads u of row t+2, relaxes the
+// black cell of row t+1 and the red cell of row t, and writes row t.
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int R, float omega, float scale,
+                                                   const float* __restrict__ c, const float* __restrict__ e,
+                                                   const float* __restrict__ n, const float* __restrict__ D,
+                                                   const TB* __restrict__ b, const float* __restrict__ x,
+                                                   const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
+                                                   double* partials, Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    if (sc->done) return;
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
+    const int lane = threadIdx.x;
+    const int ix = strip * RB_OWN - 2 + 2 * lane;
+    const int iy1 = min(iy0 + R, ny);
+    double acc = 0.0;
+    if (iy0 < ny && strip * RB_OWN < nx) {  // uniform over the wave (the partial below needs every wave)
+        const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
+        const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
+        const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+        const f2v z2 = {0.f, 0.f};
+        auto u_row = [&](int j) -> f2v {
+            if (j < 0 || j >= ny) return z2;
+            const f2v xv = *(const f2v*)(x + cidx(ixc, j, nx));
+            const float cv = xc[(size_t)(ixc >> 1) + (size_t)(j >> 1) * nxc];
+            return f2v{xv.x + scale * cv, xv.y + scale * cv};
+        };
+        auto black_row = [&](int j, f2v um, f2v uc, f2v up, f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) -> f2v {
+            if (j < 0 || j >= ny) return z2;
+            const bool hN = j < ny - 1, hS = j > 0;
+            f2v v = uc;
+            if ((j & 1) == 0) {  // black: column ix + 1
+                const float uE = __shfl_down(uc.x, 1, WAVE);
+                v.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x,
+                               uc.x, hN, hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
+            } else {  // black: column ix
+                const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                v.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f,
+                               hW0 ? uW : 0.f, hN, hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f,
+                               hS ? um.x : 0.f);
+            }
+            return v;
+        };
+        auto red_relax = [&](int j, f2v vm, f2v vc, f2v vp, f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) -> f2v {
+            const bool hN = j < ny - 1, hS = j > 0;
+            f2v z = vc;
+            if ((j & 1) == 0) {  // red: column ix
+                const float vW = __shfl_up(vc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                z.x = rb_relax(bj.x, cj.x, dj.x, omega, vc.x, true, ej.x, vc.y, hW0, hW0 ? aW : 0.f,
+                               hW0 ? vW : 0.f, hN, hN ? nj.x : 0.f, hN ? vp.x : 0.f, hS, hS ? nm.x : 0.f,
+                               hS ? vm.x : 0.f);
+            } else {  // red: column ix + 1
+                const float vE = __shfl_down(vc.x, 1, WAVE);
+                z.y = rb_relax(bj.y, cj.y, dj.y, omega, vc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? vE : 0.f, true, ej.x,
+                               vc.x, hN, hN ? nj.y : 0.f, hN ? vp.y : 0.f, hS, hS ? nm.y : 0.f, hS ? vm.y : 0.f);
+            }
+            return z;
+        };
+        f2v u0 = u_row(iy0 - 2), u1 = u_row(iy0 - 1);
+        f2v vm = z2, v0 = z2;
+        TB2 b0;  // the FP64 (or FP32) rhs of row t, for the dot product
+        b0.x = (TB)0;
+        b0.y = (TB)0;
+        f2v c0 = z2, d0 = z2, e0 = z2, n0 = z2, nm = z2;
+        if (iy0 >= 2) n0 = *(const f2v*)(n + cidx(ixc, iy0 - 2, nx));
+        for (int t = iy0 - 2; t < iy1; ++t) {
+            const f2v u2 = u_row(t + 2);
+            TB2 b1 = b0;
+            f2v c1 = z2, d1 = z2, e1 = z2, n1 = z2;
+            if (t + 1 >= 0 && t + 1 < ny) {
+                const size_t k1 = cidx(ixc, t + 1, nx);
+                b1 = *(const TB2*)(b + k1);
+                c1 = *(const f2v*)(c + k1);
+                d1 = *(const f2v*)(D + k1);
+                e1 = *(const f2v*)(e + k1);
+                n1 = *(const f2v*)(n + k1);
+            }
+            const f2v v1 = black_row(t + 1, u0, u1, u2, f2v{(float)b1.x, (float)b1.y}, c1, d1, e1, n1, n0);
+            if (t >= iy0) {
+                const f2v z = red_relax(t, vm, v0, v1, f2v{(float)b0.x, (float)b0.y}, c0, d0, e0, n0, nm);
+                if (own) {
+                    TB2 zv;
+                    zv.x = (TB)z.x;
+                    zv.y = (TB)z.y;
+                    *(TB2*)(out + cidx(ix, t, nx)) = zv;
+                    acc += (double)b0.x * (double)z.x;
+                    acc += (double)b0.y * (double)z.y;
+                }
+            }
+            u0 = u1;
+            u1 = u2;
+            vm = v0;
+            v0 = v1;
+            b0 = b1;
+            c0 = c1;
+            d0 = d1;
+            e0 = e1;
+            nm = n0;
+            n0 = n1;
+        }
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
+    }
+}
+
+// Red-black legs for the small levels and for levels of odd width: one
+// thread per coarse cell (down) or fine cell (up), every neighbour's value
+// formed from global memory with the helpers above.
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_down_rb_flat(int nx, int ny, int nxc, int nyc, float omega,
+                                                         const float* __restrict__ c, const float* __restrict__ e,
+                                                         const float* __restrict__ n, const float* __restrict__ D,
+                                                         const TB* __restrict__ b, float* __restrict__ x,
+                                                         float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= nxc || J >= nyc) return;
+    auto xs = [&](int jx, int jy) -> float { return rb_down_x<TB>(nx, ny, jx, jy, omega, e, n, D, b); };
+    float acc = 0.f;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int ix = 2 * I + dx, iy = 2 * J + dy;
+            if (ix < nx && iy < ny) {
+                const size_t k = (size_t)ix + (size_t)iy * nx;
+                const float u = xs(ix, iy);
+                x[k] = u;
+                float a = c[k] * u;
+                if (ix < nx - 1) a += e[k] * (u - xs(ix + 1, iy));
+                if (ix > 0) a += e[k - 1] * (u - xs(ix - 1, iy));
+                if (iy < ny - 1) a += n[k] * (u - xs(ix, iy + 1));
+                if (iy > 0) a += n[k - nx] * (u - xs(ix, iy - 1));
+                acc += (float)b[k] - a;
+            }
+        }
+    bc[(size_t)I + (size_t)J * nxc] = acc;
+}
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_up_rb_flat(int nx, int ny, int nxc, float omega, float scale,
+                                                       const float* __restrict__ c, const float* __restrict__ e,
+                                                       const float* __restrict__ n, const float* __restrict__ D,
+                                                       const TB* __restrict__ b, const float* __restrict__ x,
+                                                       const float* __restrict__ xc, TB* __restrict__ out,
+                                                       int with_dot, double* partials, Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    double acc = 0.0;
+    if (ix < nx && iy < ny) {
+        auto uat = [&](int jx, int jy) -> float {
+            return x[(size_t)jx + (size_t)jy * nx] + scale * xc[(size_t)(jx >> 1) + (size_t)(jy >> 1) * nxc];
+        };
+        auto vat = [&](int jx, int jy) -> float {  // a black neighbour, relaxed
+            return rb_up_black<TB>(nx, ny, jx, jy, omega, c, e, n, D, b, uat);
+        };
+        const size_t k = (size_t)ix + (size_t)iy * nx;
+        float z;
+        if ((ix + iy) & 1) {
+            z = rb_up_black<TB>(nx, ny, ix, iy, omega, c, e, n, D, b, uat);
+        } else {
+            const bool hE = ix < nx - 1, hW = ix > 0, hN = iy < ny - 1, hS = iy > 0;
+            z = rb_relax((float)b[k], c[k], D[k], omega, uat(ix, iy), hE, hE ? e[k] : 0.f, hE ? vat(ix + 1, iy) : 0.f,
+                         hW, hW ? e[k - 1] : 0.f, hW ? vat(ix - 1, iy) : 0.f, hN, hN ? n[k] : 0.f,
+                         hN ? vat(ix, iy + 1) : 0.f, hS, hS ? n[k - nx] : 0.f, hS ? vat(ix, iy - 1) : 0.f);
+        }
+        const TB bk = b[k];
+        out[k] = (TB)z;
+        acc = (double)bk * (double)z;
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
+    }
+}
+
 // The same two legs for small levels (at most MG_FLAT_CELLS cells) outside
 // the tail kernel, one thread per coarse cell (down) or fine cell (up),
 // reading the neighbours' values directly.
@@ -2013,6 +2367,39 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         float* x = Xs + off[l];
         float* bc = Bs + off[l + 1];
         auto xs = [&](size_t k) -> float { return mg_div(omega * b[k], D[k]); };
+        if (T.rb) {
+            // red-black: every cell's x first (black from its red neighbours), then the residuals
+            for (int k = tid; k < fnx * fny; k += nt) {
+                const int ix = k % fnx, iy = k / fnx;
+                if (((ix + iy) & 1) == 0) {
+                    x[k] = xs(k);
+                } else {
+                    const bool hW = ix > 0, hE = ix < fnx - 1, hN = iy < fny - 1, hS = iy > 0;
+                    x[k] = rb_black0(b[k], D[k], omega, hW ? e[k - 1] : 0.f, hW ? xs(k - 1) : 0.f, hE ? e[k] : 0.f,
+                                     hE ? xs(k + 1) : 0.f, hN ? n[k] : 0.f, hN ? xs(k + fnx) : 0.f,
+                                     hS ? n[k - fnx] : 0.f, hS ? xs(k - fnx) : 0.f);
+                }
+            }
+            __syncthreads();
+            auto xk = [&](size_t k) -> float { return x[k]; };
+            for (int q = tid; q < cnx * cny; q += nt) {
+                const int I = q % cnx, J = q / cnx;
+                float acc = 0.f;
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int ix = 2 * I + dx, iy = 2 * J + dy;
+                        if (ix < fnx && iy < fny) {
+                            const size_t k = (size_t)ix + (size_t)iy * fnx;
+                            acc += b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xk);
+                        }
+                    }
+                bc[q] = acc;
+            }
+            __syncthreads();
+            continue;
+        }
         for (int q = tid; q < cnx * cny; q += nt) {
             const int I = q % cnx, J = q / cnx;
             float acc = 0.f;
@@ -2061,6 +2448,22 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         }
         __syncthreads();
         auto us = [&](size_t k) -> float { return x[k]; };
+        if (T.rb) {
+            // black cells relax from u in place (they read only red neighbours),
+            // then red cells from the relaxed black ones
+            for (int colour = 1; colour >= 0; --colour) {
+                for (int k = tid; k < nf; k += nt) {
+                    const int ix = k % fnx, iy = k / fnx;
+                    if (((ix + iy) & 1) != colour) continue;
+                    const bool hE = ix < fnx - 1, hW = ix > 0, hN = iy < fny - 1, hS = iy > 0;
+                    x[k] = rb_relax(b[k], c[k], D[k], omega, x[k], hE, hE ? e[k] : 0.f, hE ? x[k + 1] : 0.f, hW,
+                                    hW ? e[k - 1] : 0.f, hW ? x[k - 1] : 0.f, hN, hN ? n[k] : 0.f,
+                                    hN ? x[k + fnx] : 0.f, hS, hS ? n[k - fnx] : 0.f, hS ? x[k - fnx] : 0.f);
+                }
+                __syncthreads();
+            }
+            continue;
+        }
         float z[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -2753,14 +3156,40 @@ static Geom mg_wide_geom(int nx, int ny) {
     W.nblocks = W.grid.x * W.grid.y;
     return W;
 }
+// Geometry of the red-black two-column legs: strips of RB_OWN owned columns,
+// R even, chosen for about 8192 waves.
+static Geom mg_rb_geom(int nx, int ny) {
+    Geom W = make_geom(nx, ny);
+    const long strips = (nx + RB_OWN - 1) / RB_OWN;
+    long r = ((long)ny * strips) / 8192;
+    W.R = (int)(r < 2 ? 2 : (r > 8 ? 8 : r + (r & 1)));
+    W.grid = dim3((unsigned)((strips + WAVES - 1) / WAVES), (unsigned)((ny + W.R - 1) / W.R));
+    W.nblocks = W.grid.x * W.grid.y;
+    return W;
+}
+// Red-black smoothing: the two-column marching legs on levels of even width
+// above MG_FLAT_CELLS cells, the flat legs elsewhere (small or odd-width levels).
+static bool mg_rb_flat(int nx, int ny) { return (long)nx * ny <= mg_flat_cells() || nx % 2 != 0; }
 // Workgroups (and so r'z partials) of the up-leg on an nx x ny level.
-int mg_up_nblocks(int nx, int ny) {
+int mg_up_nblocks(int nx, int ny, int rb) {
+    if (rb) return mg_rb_flat(nx, ny) ? (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y) : mg_rb_geom(nx, ny).nblocks;
     if ((long)nx * ny <= mg_flat_cells()) return (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y);
     return mg_wide(nx) ? mg_wide_geom(nx, ny).nblocks : mg_march_geom(nx, ny).nblocks;
 }
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
-             const float* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+             const float* D, const TB* b, float* x, float* bc, const Scalars* sc, int rb, hipStream_t st) {
+    if (rb) {
+        if (mg_rb_flat(nx, ny)) {
+            hipLaunchKernelGGL(k_mg_down_rb_flat<TB>, cells_grid(nxc, nyc), dim3(WAVE, WAVES), 0, st, nx, ny, nxc,
+                               nyc, omega, c, e, n, D, b, x, bc, sc);
+        } else {
+            const Geom W = mg_rb_geom(nx, ny);
+            hipLaunchKernelGGL(k_mg_down_rb2<TB>, W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b, x,
+                               bc, sc);
+        }
+        return;
+    }
     if ((long)nx * ny <= mg_flat_cells()) {
         hipLaunchKernelGGL(k_mg_down_flat<TB>, cells_grid(nxc, nyc), dim3(WAVE, WAVES), 0, st, nx, ny, nxc, nyc,
                            omega, c, e, n, D, b, x, bc, sc);
@@ -2779,14 +3208,20 @@ void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, cons
 template <class TB>
 void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, const float* e, const float* n,
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
-           double* partials, Scalars* sc, hipStream_t st) {
-    const bool flat = (long)nx * ny <= mg_flat_cells();
-    const bool wide = !flat && mg_wide(nx);
-    const Geom M = flat ? Geom{} : (wide ? mg_wide_geom(nx, ny) : mg_march_geom(nx, ny));
+           double* partials, Scalars* sc, int rb, hipStream_t st) {
+    const bool flat = rb ? mg_rb_flat(nx, ny) : (long)nx * ny <= mg_flat_cells();
+    const bool wide = !flat && (rb || mg_wide(nx));
+    const Geom M = flat ? Geom{} : (rb ? mg_rb_geom(nx, ny) : (wide ? mg_wide_geom(nx, ny) : mg_march_geom(nx, ny)));
     const dim3 grid = flat ? cells_grid(nx, ny) : M.grid;
     // dot_mode 2: the partials of r'z stay pending for the next CG matvec
     const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
-    if (flat)
+    if (rb && flat)
+        hipLaunchKernelGGL(k_mg_up_rb_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n,
+                           D, b, x, xc, out, with_dot, partials, sc);
+    else if (rb)
+        hipLaunchKernelGGL(k_mg_up_rb2<TB>, grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b, x,
+                           xc, out, with_dot, partials, sc);
+    else if (flat)
         hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,
                            b, x, xc, out, with_dot, partials, sc);
     else if (wide)
@@ -2800,13 +3235,15 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
 }
 template void mg_down<float>(int, int, int, int, float, const float*, const float*, const float*, const float*,
-                             const float*, float*, float*, const Scalars*, hipStream_t);
+                             const float*, float*, float*, const Scalars*, int, hipStream_t);
 template void mg_down<double>(int, int, int, int, float, const float*, const float*, const float*, const float*,
-                              const double*, float*, float*, const Scalars*, hipStream_t);
+                              const double*, float*, float*, const Scalars*, int, hipStream_t);
 template void mg_up<float>(int, int, int, float, float, const float*, const float*, const float*, const float*,
-                           const float*, const float*, const float*, float*, int, double*, Scalars*, hipStream_t);
+                           const float*, const float*, const float*, float*, int, double*, Scalars*, int,
+                           hipStream_t);
 template void mg_up<double>(int, int, int, float, float, const float*, const float*, const float*, const float*,
-                            const double*, const float*, const float*, double*, int, double*, Scalars*, hipStream_t);
+                            const double*, const float*, const float*, double*, int, double*, Scalars*, int,
+                            hipStream_t);
 size_t mg_tail_stride(size_t tail_cells) { return (tail_cells + 3) & ~(size_t)3; }
 size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells) {
     // rhs, x and the packed block (c, e, n, D, inverse; rounded up to 16 B)
