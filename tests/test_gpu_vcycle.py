@@ -33,7 +33,7 @@ def test_vcycle_matches_restatement(hip, shape):
     arrs = [np.zeros((nx, ny), order="F") for _ in range(6)]
     L.check(lib.shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
     aP, aE, _, aN, _, _ = arrs
-    M = VCycle(aP, aE, aN, omega=1.1, scale=1.7)
+    M = VCycle(aP, aE, aN)  # the library's default smoothing weight and coarse scale
     rng = np.random.default_rng(7)
     rs = [np.asfortranarray(rng.standard_normal((nx, ny))) for _ in range(2)]
     zs = []

@@ -104,7 +104,7 @@ def dense_inverse(c, e, n):
 
 
 class VCycle:
-    def __init__(self, aP, aE, aN, omega=1.1, scale=1.7):
+    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8):
         self.omega, self.scale = F32(omega), F32(scale)
         c, e, n, D = fine_level(aP, aE, aN)
         self.levels = [(c, e, n, D)]
