@@ -2139,6 +2139,244 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
     }
 }
 
+// The same two legs for the levels where a wave takes two rows (R = 2):
+// these levels are latency-bound, so every load of the wave's rows (row
+// indices clamped into the grid, rows outside it zeroed after the load) is
+// issued at once, and the done flag is read with them; the arithmetic per
+// row is that of k_mg_down_rb2 / k_mg_up_rb2, in the same order.
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, float omega,
+                                                      const float* __restrict__ c, const float* __restrict__ e,
+                                                      const float* __restrict__ n, const float* __restrict__ D,
+                                                      const TB* __restrict__ b, float* __restrict__ x,
+                                                      float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    const int done = sc->done;  // arrives with the loads below
+    int strip, iy0;
+    wide_tile(2, strip, iy0);  // iy0 even
+    const int lane = threadIdx.x;
+    const int ix = strip * RB_OWN - 2 + 2 * lane;
+    if (iy0 >= ny || strip * RB_OWN >= nx) return;  // uniform over the wave; no barriers follow
+    const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
+    const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
+    const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+    const f2v z2 = {0.f, 0.f};
+    auto row = [&](int j) -> size_t { return cidx(ixc, min(max(j, 0), ny - 1), nx); };
+    // b and D of rows iy0-2 .. iy0+3, n of rows iy0-2 .. iy0+2, e of rows iy0-1 .. iy0+2, c of rows iy0, iy0+1
+    f2v Bv[6], Dv[6], Nv[5], Ev[4], Cv[2];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const size_t k = row(iy0 - 2 + q);
+        const TB2 v = *(const TB2*)(b + k);
+        Bv[q] = f2v{(float)v.x, (float)v.y};
+        Dv[q] = *(const f2v*)(D + k);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Nv[q] = *(const f2v*)(n + row(iy0 - 2 + q));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Ev[q] = *(const f2v*)(e + row(iy0 - 1 + q));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) Cv[q] = *(const f2v*)(c + row(iy0 + q));
+    // every load is issued before the flag decides
+#pragma unroll
+    for (int q = 0; q < 6; ++q) asm volatile("" : : "v"(Bv[q].x), "v"(Bv[q].y), "v"(Dv[q].x), "v"(Dv[q].y));
+#pragma unroll
+    for (int q = 0; q < 5; ++q) asm volatile("" : : "v"(Nv[q].x), "v"(Nv[q].y));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) asm volatile("" : : "v"(Ev[q].x), "v"(Ev[q].y));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) asm volatile("" : : "v"(Cv[q].x), "v"(Cv[q].y));
+    if (done) return;
+    f2v Rv[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int j = iy0 - 2 + q;
+        Rv[q] = z2;
+        if (j >= 0 && j < ny) {
+            if ((j & 1) == 0) Rv[q].x = mg_div(omega * Bv[q].x, Dv[q].x);
+            else Rv[q].y = mg_div(omega * Bv[q].y, Dv[q].y);
+        }
+        if (j < 0 || j >= ny) {
+            Bv[q] = z2;
+            Dv[q] = z2;
+        }
+    }
+    // full x of rows iy0-1 .. iy0+2
+    f2v Xv[4];
+#pragma unroll
+    for (int q = 1; q <= 4; ++q) {
+        const int j = iy0 - 2 + q;
+        f2v v = z2;
+        if (j >= 0 && j < ny) {
+            const f2v rm = Rv[q - 1], rc = Rv[q], rp = Rv[q + 1], ej = Ev[q - 1], nj = Nv[q], nm = Nv[q - 1];
+            const bool hN = j < ny - 1, hS = j > 0;
+            v = rc;
+            if ((j & 1) == 0) {
+                const float xE = __shfl_down(rc.x, 1, WAVE);
+                v.y = rb_black0(Bv[q].y, Dv[q].y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
+                                hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
+            } else {
+                const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                v.x = rb_black0(Bv[q].x, Dv[q].x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
+                                hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
+            }
+        }
+        Xv[q - 1] = v;
+    }
+    float hrow = 0.f;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+        const int t = iy0 + s2;
+        if (t < ny) {
+            const f2v X0 = Xv[s2 + 1], Xm = Xv[s2], X1 = Xv[s2 + 2];
+            const f2v e0 = Ev[s2 + 1], n0 = Nv[s2 + 2], nm = Nv[s2 + 1], cc = Cv[s2], b0 = Bv[s2 + 2];
+            const float ue = __shfl_down(X0.x, 1, WAVE);
+            const float uw = __shfl_up(X0.y, 1, WAVE);
+            const float ew = __shfl_up(e0.y, 1, WAVE);
+            float a0 = cc.x * X0.x;
+            a0 += e0.x * (X0.x - X0.y);
+            if (hW0) a0 += ew * (X0.x - uw);
+            if (t < ny - 1) a0 += n0.x * (X0.x - X1.x);
+            if (t > 0) a0 += nm.x * (X0.x - Xm.x);
+            float a1 = cc.y * X0.y;
+            if (hE1) a1 += e0.y * (X0.y - ue);
+            a1 += e0.x * (X0.y - X0.x);
+            if (t < ny - 1) a1 += n0.y * (X0.y - X1.y);
+            if (t > 0) a1 += nm.y * (X0.y - Xm.y);
+            const float q0 = own ? b0.x - a0 : 0.f, q1 = own ? b0.y - a1 : 0.f;
+            if (own) *(f2v*)(x + cidx(ix, t, nx)) = X0;
+            const float hs = q0 + q1;
+            if (s2 == 0) hrow = hs;
+            else if (own) bc[(size_t)(ix >> 1) + (size_t)(t >> 1) * nxc] = hrow + hs;
+        }
+    }
+    // odd ny: the last (even) row has no partner row
+    if (iy0 + 1 >= ny && own) bc[(size_t)(ix >> 1) + (size_t)(iy0 >> 1) * nxc] = hrow + 0.f;
+}
+
+template <class TB>
+__global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, float omega, float scale,
+                                                    const float* __restrict__ c, const float* __restrict__ e,
+                                                    const float* __restrict__ n, const float* __restrict__ D,
+                                                    const TB* __restrict__ b, const float* __restrict__ x,
+                                                    const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
+                                                    double* partials, Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    const int done = sc->done;  // arrives with the loads below
+    int strip, iy0;
+    wide_tile(2, strip, iy0);
+    const int lane = threadIdx.x;
+    const int ix = strip * RB_OWN - 2 + 2 * lane;
+    double acc = 0.0;
+    const bool active = iy0 < ny && strip * RB_OWN < nx;  // uniform over the wave
+    const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
+    const int iyc = active ? iy0 : 0;  // rows of an inactive wave: any in-grid rows (never used)
+    auto row = [&](int j) -> size_t { return cidx(ixc, min(max(j, 0), ny - 1), nx); };
+    // u (x and xc) of rows iy0-2 .. iy0+3, b c D e of rows iy0-1 .. iy0+2, n of rows iy0-2 .. iy0+2
+    f2v Xv[6], Cx[6];
+    TB2 Bt[4];
+    f2v Cv[4], Dv[4], Ev[4], Nv[5];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int j = min(max(iyc - 2 + q, 0), ny - 1);
+        Xv[q] = *(const f2v*)(x + cidx(ixc, j, nx));
+        const float cv = xc[(size_t)(ixc >> 1) + (size_t)(j >> 1) * nxc];
+        Cx[q] = f2v{cv, cv};
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const size_t k = row(iyc - 1 + q);
+        Bt[q] = *(const TB2*)(b + k);
+        Cv[q] = *(const f2v*)(c + k);
+        Dv[q] = *(const f2v*)(D + k);
+        Ev[q] = *(const f2v*)(e + k);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Nv[q] = *(const f2v*)(n + row(iyc - 2 + q));
+#pragma unroll
+    for (int q = 0; q < 6; ++q) asm volatile("" : : "v"(Xv[q].x), "v"(Xv[q].y), "v"(Cx[q].x));
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        asm volatile("" : : "v"(Bt[q].x), "v"(Bt[q].y), "v"(Cv[q].x), "v"(Cv[q].y), "v"(Dv[q].x), "v"(Dv[q].y),
+                     "v"(Ev[q].x), "v"(Ev[q].y));
+#pragma unroll
+    for (int q = 0; q < 5; ++q) asm volatile("" : : "v"(Nv[q].x), "v"(Nv[q].y));
+    if (done) return;  // uniform over the workgroup, before the partial's barriers
+    if (active) {
+        const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
+        const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+        const f2v z2 = {0.f, 0.f};
+        f2v U[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int j = iy0 - 2 + q;
+            U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * Cx[q].x, Xv[q].y + scale * Cx[q].y} : z2;
+        }
+        // rows iy0-1 .. iy0+2 with their black cell relaxed
+        f2v V[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = iy0 - 1 + q;
+            f2v v = z2;
+            if (j >= 0 && j < ny) {
+                const f2v um = U[q], uc = U[q + 1], up = U[q + 2], cj = Cv[q], dj = Dv[q], ej = Ev[q];
+                const f2v nj = Nv[q + 1], nm = Nv[q];
+                const f2v bj = f2v{(float)Bt[q].x, (float)Bt[q].y};
+                const bool hN = j < ny - 1, hS = j > 0;
+                v = uc;
+                if ((j & 1) == 0) {
+                    const float uE = __shfl_down(uc.x, 1, WAVE);
+                    v.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true,
+                                   ej.x, uc.x, hN, hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f,
+                                   hS ? um.y : 0.f);
+                } else {
+                    const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                    v.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f,
+                                   hW0 ? uW : 0.f, hN, hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f,
+                                   hS ? um.x : 0.f);
+                }
+            }
+            V[q] = v;
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int t = iy0 + s2;
+            if (t < ny) {
+                const f2v vm = V[s2], vc = V[s2 + 1], vp = V[s2 + 2];
+                const f2v cj = Cv[s2 + 1], dj = Dv[s2 + 1], ej = Ev[s2 + 1], nj = Nv[s2 + 2], nm = Nv[s2 + 1];
+                const TB2 bt = Bt[s2 + 1];
+                const f2v bj = f2v{(float)bt.x, (float)bt.y};
+                const bool hN = t < ny - 1, hS = t > 0;
+                f2v z = vc;
+                if ((t & 1) == 0) {
+                    const float vW = __shfl_up(vc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                    z.x = rb_relax(bj.x, cj.x, dj.x, omega, vc.x, true, ej.x, vc.y, hW0, hW0 ? aW : 0.f,
+                                   hW0 ? vW : 0.f, hN, hN ? nj.x : 0.f, hN ? vp.x : 0.f, hS, hS ? nm.x : 0.f,
+                                   hS ? vm.x : 0.f);
+                } else {
+                    const float vE = __shfl_down(vc.x, 1, WAVE);
+                    z.y = rb_relax(bj.y, cj.y, dj.y, omega, vc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? vE : 0.f, true,
+                                   ej.x, vc.x, hN, hN ? nj.y : 0.f, hN ? vp.y : 0.f, hS, hS ? nm.y : 0.f,
+                                   hS ? vm.y : 0.f);
+                }
+                if (own) {
+                    TB2 zv;
+                    zv.x = (TB)z.x;
+                    zv.y = (TB)z.y;
+                    *(TB2*)(out + cidx(ix, t, nx)) = zv;
+                    acc += (double)bt.x * (double)z.x;
+                    acc += (double)bt.y * (double)z.y;
+                }
+            }
+        }
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
+    }
+}
+
 // Red-black legs for the small levels and for levels of odd width: one
 // thread per coarse cell (down) or fine cell (up), every neighbour's value
 // formed from global memory with the helpers above.
@@ -3189,8 +3427,12 @@ void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, cons
                                nyc, omega, c, e, n, D, b, x, bc, sc);
         } else {
             const Geom W = mg_rb_geom(nx, ny);
-            hipLaunchKernelGGL(k_mg_down_rb2<TB>, W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b, x,
-                               bc, sc);
+            if (W.R == 2)
+                hipLaunchKernelGGL(k_mg_down_rb2s<TB>, W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D, b, x,
+                                   bc, sc);
+            else
+                hipLaunchKernelGGL(k_mg_down_rb2<TB>, W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b,
+                                   x, bc, sc);
         }
         return;
     }
@@ -3222,6 +3464,9 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
     if (rb && flat)
         hipLaunchKernelGGL(k_mg_up_rb_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n,
                            D, b, x, xc, out, with_dot, partials, sc);
+    else if (rb && M.R == 2)
+        hipLaunchKernelGGL(k_mg_up_rb2s<TB>, grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
+                           out, with_dot, partials, sc);
     else if (rb)
         hipLaunchKernelGGL(k_mg_up_rb2<TB>, grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b, x,
                            xc, out, with_dot, partials, sc);
