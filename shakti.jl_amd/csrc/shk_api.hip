@@ -100,14 +100,20 @@ struct shk_ctx {
     int mg_stride = 0;
     float* mg_inv = nullptr;   // inside mg_pack
     // Smoother, its weight and the coarse-correction scale.  Red-black
-    // Gauss-Seidel (the default) with over-relaxation 1.2 and scale 1.8: on
-    // the assembled glacier systems 35-40% fewer CG iterations than damped
-    // Jacobi (omega 0.95, scale 1.5; SHK_MG_JACOBI=1) at the same memory
-    // traffic per V-cycle.  Larger scales helped some systems and hurt
-    // others.  The symmetric V-cycle (red-black down, black-red up) stays
-    // positive definite for omega < 2 and scale < 2.
+    // Gauss-Seidel (the default) with over-relaxation 1.2: on the assembled
+    // glacier systems 35-40% fewer CG iterations than damped Jacobi (omega
+    // 0.95, scale 1.5; SHK_MG_JACOBI=1) at the same memory traffic per
+    // V-cycle.  The piecewise-constant coarse correction is scaled up to make
+    // up for the energy it misses, which depends on how diagonally dominant
+    // the system is: 1.8 for the elliptic systems (the reaction term orders
+    // of magnitude below the conductances), 1.0 (no over-correction) for the
+    // backward-Euler parabolic ones, where the storage term dominates (on
+    // the 8192 x 2048 workload, CG iterations per solve for scale 0.8 / 0.9
+    // / 1.0 / 1.1 / 1.2 / 1.8: 50 / 42 / 37 / 41 / 49 / 150).  The symmetric
+    // V-cycle (red-black down, black-red up) stays positive definite for
+    // omega < 2 and scale < 2.
     int mg_rb = 1;
-    float mg_omega = 1.2f, mg_scale = 1.8f;
+    float mg_omega = 1.2f, mg_scale = 1.8f, mg_scale_par = 1.0f;
     size_t mg_tail_first = 1;                 // first level handled by the single-workgroup tail kernel
 
     // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)
@@ -426,10 +432,11 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
         if (getenv("SHK_MG_JACOBI")) {
             ctx->mg_rb = 0;
             ctx->mg_omega = 0.95f;
-            ctx->mg_scale = 1.5f;
+            ctx->mg_scale = ctx->mg_scale_par = 1.5f;
         }
         if (const char* e = getenv("SHK_MG_OMEGA")) ctx->mg_omega = (float)atof(e);
         if (const char* e = getenv("SHK_MG_SCALE")) ctx->mg_scale = (float)atof(e);
+        if (const char* e = getenv("SHK_MG_SCALE_PARABOLIC")) ctx->mg_scale_par = (float)atof(e);
     }
     auto& L0 = ctx->mg[0];
     L0.c = (float*)w.aP;
@@ -458,6 +465,7 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 template <class T>
 void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
     auto& mg = ctx->mg;
+    const float scale = ctx->system_parabolic ? ctx->mg_scale_par : ctx->mg_scale;
     const size_t L = mg.size() - 1;
     const size_t first = ctx->mg_tail_first;  // >= 1
     const double tb = sizeof(T) - 4.0;        // extra bytes per cell of an FP64 rhs / result
@@ -477,7 +485,7 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
         MGTail Tl{};
         Tl.nlev = (int)(L - first + 1);
         Tl.omega = ctx->mg_omega;
-        Tl.scale = ctx->mg_scale;
+        Tl.scale = scale;
         Tl.pack = ctx->mg_pack;
         Tl.stride = ctx->mg_stride;
         Tl.inv = ctx->mg_inv;
@@ -497,10 +505,10 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
         Prof pr(ctx, l == 0 ? P_MGU0 : P_MGU,
                 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
         if (l == 0)
-            launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, r, f.xa,
+            launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, f.c, f.e, f.n, f.D, r, f.xa,
                              mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->mg_rb, ctx->st);
         else
-            launch::mg_up<float>(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, ctx->mg_scale, f.c, f.e, f.n, f.D, f.b,
+            launch::mg_up<float>(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, scale, f.c, f.e, f.n, f.D, f.b,
                                  f.xa, mg[l + 1].xb, f.xb, -1, ctx->partials, ctx->sc, ctx->mg_rb, ctx->st);
     }
 }
@@ -682,7 +690,10 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     auto batch_graph = [&](hipGraphExec_t* out) -> int {
         *out = nullptr;
         if (!ctx->use_graphs || ctx->prof || (ctx->rec_kind == 1 && ctx->rec_x)) return SHK_OK;
-        const int key[4] = {(int)sizeof(T), precond + (learned_batch ? 16 : 0) + (defer ? 32 : 0),
+        // the V-cycle's coarse scale (a kernel argument) differs between
+        // elliptic and parabolic systems: part of the key
+        const int key[4] = {(int)sizeof(T),
+                            precond + (learned_batch ? 16 : 0) + (defer ? 32 : 0) + (ctx->system_parabolic ? 64 : 0),
                             degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
             if (!memcmp(e.key, key, sizeof(key)) && (!learned_batch || e.net == ctx->net)) *out = e.exec;
@@ -782,6 +793,8 @@ int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_sol
                                  ctx->cP);
     }
     ctx->have_system = true;
+    ctx->system_parabolic = parabolic;
+    ctx->system_dt = dt;
     {
         // FP32 flux-form system: c (in aP32), aE, aN, and the Jacobi diagonal
         Prof pr(ctx, P_IR, 5 * 12.0 * n);
@@ -1340,6 +1353,8 @@ int shk_assemble(shk_ctx* ctx, int32_t parabolic, double dt) {
     launch::assemble<double>(ctx->G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st,
                              ctx->cP);
     ctx->have_system = true;
+    ctx->system_parabolic = parabolic;
+    ctx->system_dt = dt;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));
     return SHK_OK;

@@ -2751,24 +2751,37 @@ __global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, co
         A[k * n + k] = d;
     }
     __syncthreads();
+    // the elimination on registers: thread (j, ty) keeps A[i][j] for i = ty + 4m;
+    // each step shares the pivot column and the pivot row through LDS
+    __shared__ double rowk[64];
+    double a[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int i = ty + 4 * m;
+        a[m] = (i < n && j < n) ? A[i * n + j] : 0.0;
+    }
     for (int k = 0; k < n; ++k) {
-        const double ip = 1.0 / A[k * n + k];
-        // the pivot column before this step, then the scaled pivot row
-        if (ty == 0 && j < n) colk[j] = A[j * n + k];
-        __syncthreads();
-        if (ty == 1 && j < n) A[k * n + j] = (j == k) ? ip : A[k * n + j] * ip;
-        __syncthreads();
-        if (j < n) {
-            const double akj = A[k * n + j];
-            for (int i = ty; i < n; i += 4) {
-                if (i == k) continue;
-                A[i * n + j] = (j == k) ? -colk[i] * ip : A[i * n + j] - colk[i] * akj;
-            }
+        const int mk = k >> 2;  // the pivot row's register index in the threads with ty == k % 4
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            if (j == k && ty + 4 * m < n) colk[ty + 4 * m] = a[m];  // the pivot column before this step
+            if (m == mk && ty == (k & 3) && j < n) rowk[j] = a[m];  // the pivot row before this step
         }
         __syncthreads();
+        const double ip = 1.0 / colk[k];
+        const double akj = (j == k) ? ip : rowk[j] * ip;  // the scaled pivot row
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int i = ty + 4 * m;
+            if (i < n && j < n) a[m] = (i == k) ? akj : ((j == k) ? -colk[i] * ip : a[m] - colk[i] * akj);
+        }
+        __syncthreads();  // colk and rowk are rewritten by the next step
     }
-    for (int i = ty; i < n; i += 4)
-        if (j < n) inv[i * n + j] = (float)A[i * n + j];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int i = ty + 4 * m;
+        if (i < n && j < n) inv[i * n + j] = (float)a[m];
+    }
 }
 
 

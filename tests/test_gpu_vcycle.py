@@ -49,3 +49,53 @@ def test_vcycle_matches_restatement(hip, shape):
     a, b = float(np.sum(rs[1] * zs[0])), float(np.sum(rs[0] * zs[1]))
     scale = np.sqrt(float(np.sum(rs[0] * zs[0])) * float(np.sum(rs[1] * zs[1])))
     assert abs(a - b) <= 1e-4 * scale, (a, b)
+
+
+def test_vcycle_parabolic_system(hip):
+    """The backward-Euler system (storage term e_v / dt on the diagonal) takes the
+    unscaled coarse correction (scale 1.0): the same comparison with the restatement."""
+    from shakti_hip import synthetic as S
+    from shakti_hip import api
+    from shakti_hip import _lib as L
+    nx, ny = 520, 130
+    gg, raw = S.glacier(nx, ny, bed_amp=50.0, seed=12)
+    hg, hp, hs = hip_state(hip, (gg.nx, gg.ny, gg.lx, gg.ly), raw, p_kw={"e_v": 1e-3})
+    hs._bind(p=hp, kfs=hip.Harmonic())
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=1e-10)
+    api._bind_solver(hs, ls)
+    lib = L.load()
+    L.check(lib.shk_assemble(hs.ctx, 1, 600.0), hs.ctx)
+    arrs = [np.zeros((nx, ny), order="F") for _ in range(6)]
+    L.check(lib.shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
+    M = VCycle(arrs[0], arrs[1], arrs[3], omega=1.2, scale=1.0)
+    r = np.asfortranarray(np.random.default_rng(8).standard_normal((nx, ny)))
+    z = np.zeros((nx, ny), order="F")
+    L.check(lib.shk_apply_precond(hs.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                  z.ctypes.data_as(C.POINTER(C.c_double))), hs.ctx)
+    zr = M(r)
+    err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+    print(f"parabolic {nx}x{ny}: |z - z_restated| / |z| = {err:.2e}", flush=True)
+    assert err < 5e-4, err
+
+
+def test_parabolic_solve_after_an_elliptic_one_matches_a_fresh_context(hip):
+    """CG batches are replayed from captured HIP graphs, and the V-cycle's coarse
+    scale (a kernel argument) differs between elliptic and parabolic systems: a
+    parabolic multigrid solve after an elliptic one on the same context takes the
+    same CG iterations and gives the same head, bit for bit, as on a fresh context."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(257, 66, bed_amp=50.0, seed=4)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    out = []
+    for warm in (True, False):
+        hg, hp, hs = hip_state(hip, g_args, raw, p_kw={"e_v": 1e-3})
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=1e-10)
+        h0 = hs.get("h")
+        if warm:
+            hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+            hs.set("h", h0)
+        hip.solve_parabolic_linear_system(ls, hs, hg, hp, hip.Harmonic(), 600.0)
+        out.append((ls.last_stats.niter, hs.get("h")))
+    print(f"parabolic CG iterations: after an elliptic solve {out[0][0]}, fresh {out[1][0]}", flush=True)
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
