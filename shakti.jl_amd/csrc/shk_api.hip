@@ -114,6 +114,12 @@ struct shk_ctx {
     // omega < 2 and scale < 2.
     int mg_rb = 1;
     float mg_omega = 1.2f, mg_scale = 1.8f, mg_scale_par = 1.0f;
+    // The finest level's c, aE, aN, D in BF16 for its red-black legs (where
+    // those are the two-column kernels): half the coefficient bytes of the
+    // two full-size V-cycle kernels, at no cost in CG iterations measured
+    // (SHK_MG_FP32_FINE=1 keeps FP32).  The FP32 arrays stay for coarsening.
+    bool mg_bf16 = false;
+    uint16_t *mg_c16 = nullptr, *mg_e16 = nullptr, *mg_n16 = nullptr, *mg_D16 = nullptr;
     size_t mg_tail_first = 1;                 // first level handled by the single-workgroup tail kernel
 
     // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)
@@ -443,6 +449,15 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
     L0.e = (float*)w.aE;
     L0.n = (float*)w.aN;
     L0.D = (float*)w.dg;
+    ctx->mg_bf16 = ctx->mg_rb && launch::mg_bf16_ok(L0.nx, L0.ny) && !getenv("SHK_MG_FP32_FINE");
+    if (ctx->mg_bf16) {
+        const size_t n0 = (size_t)L0.nx * L0.ny;
+        if (!ctx->mg_c16) {
+            uint16_t** ps[] = {&ctx->mg_c16, &ctx->mg_e16, &ctx->mg_n16, &ctx->mg_D16};
+            for (auto pp : ps) HIPCHK(hipMalloc((void**)pp, n0 * sizeof(uint16_t) + 256));
+        }
+        launch::to_bf16x4(n0, L0.c, L0.e, L0.n, L0.D, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16, ctx->mg_D16, ctx->st);
+    }
     for (size_t l = 1; l < ctx->mg.size(); ++l) {
         auto& f = ctx->mg[l - 1];
         auto& c = ctx->mg[l];
@@ -474,7 +489,10 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
         // reads b, c, aE, aN, D; writes x and the coarse rhs
         Prof pr(ctx, l == 0 ? P_MGD0 : P_MGD,
                 4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? tb * f.nx * f.ny : 0.0));
-        if (l == 0)
+        if (l == 0 && ctx->mg_bf16)
+            launch::mg_down_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
+                                    ctx->mg_D16, r, f.xa, mg[1].b, ctx->sc, ctx->st);
+        else if (l == 0)
             launch::mg_down<T>(f.nx, f.ny, mg[1].nx, mg[1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, r, f.xa, mg[1].b,
                                ctx->sc, ctx->mg_rb, ctx->st);
         else
@@ -504,7 +522,10 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
         // reads b, c, aE, aN, D, x and the coarse correction; writes the result
         Prof pr(ctx, l == 0 ? P_MGU0 : P_MGU,
                 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
-        if (l == 0)
+        if (l == 0 && ctx->mg_bf16)
+            launch::mg_up_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
+                                  ctx->mg_D16, r, f.xa, mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->st);
+        else if (l == 0)
             launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, f.c, f.e, f.n, f.D, r, f.xa,
                              mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->mg_rb, ctx->st);
         else
@@ -1139,7 +1160,9 @@ int shk_destroy(shk_ctx* ctx) {
             }
         }
     }
-    if (ctx->mg_pack) hipFree(ctx->mg_pack);  // holds mg_inv
+    if (ctx->mg_pack) hipFree(ctx->mg_pack);
+    for (uint16_t* q : {ctx->mg_c16, ctx->mg_e16, ctx->mg_n16, ctx->mg_D16})
+        if (q) hipFree(q);  // holds mg_inv
     if (ctx->net) shknn::release_stream(ctx->net, ctx->st);
     void* nnp[] = {ctx->nn_x, ctx->nn_e, ctx->nn_Ae, ctx->hw, ctx->rec_snap, ctx->rec_scale};
     for (void* p : nnp)

@@ -192,6 +192,16 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
            const float* D, const TB* b, const float* x, const float* xc, TB* out, int dot_mode,
            double* partials, Scalars* sc, int rb, hipStream_t st);
 void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st);
+bool mg_bf16_ok(int nx, int ny);
+void to_bf16x4(size_t n, const float* a0, const float* a1, const float* a2, const float* a3, uint16_t* o0,
+               uint16_t* o1, uint16_t* o2, uint16_t* o3, hipStream_t st);
+template <class TB>
+void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
+                  const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
+template <class TB>
+void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
+                const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TB* out,
+                int dot_mode, double* partials, Scalars* sc, hipStream_t st);
 // dynamic LDS bytes of the tail kernel (rhs, x, c, e, n, D per cell and the
 // coarsest inverse), and the opt-in for more than the default 64 KB (0 = ok)
 size_t mg_tail_stride(size_t tail_cells);

@@ -1914,6 +1914,15 @@ __global__ __launch_bounds__(256) void k_mg_up_march2(int nx, int ny, int nxc, i
     }
 }
 
+// Coefficient pairs of the two-column red-black legs: FP32, or BF16 (the
+// finest level, rounded to nearest even at set-up) unpacked from one 32-bit
+// load; the element at the lower address is the low half.
+__device__ __forceinline__ f2v ldc2(const float* p, size_t k) { return *(const f2v*)(p + k); }
+__device__ __forceinline__ f2v ldc2(const uint16_t* p, size_t k) {
+    const uint32_t w = *(const uint32_t*)(p + k);
+    return f2v{__uint_as_float(w << 16), __uint_as_float(w & 0xFFFF0000u)};
+}
+
 // Red-black legs on two columns per lane (even nx).  A wave covers 128
 // columns of which it owns the middle 124 (RB_OWN): lanes 0 and 63 are halo
 // lanes that compute the neighbouring strips' two nearest columns by the
@@ -1927,10 +1936,10 @@ constexpr int RB_OWN = 2 * (WAVE - 2);
 // Down-leg from x = 0: red, then black from its red neighbours; the wave runs
 // two rows ahead (step t forms the red values of row t+2, the full x of row
 // t+1 and the residual and restriction of row t, as k_mg_down_march2).
-template <class TB>
+template <class TB, class CT>
 __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, int R, float omega,
-                                                     const float* __restrict__ c, const float* __restrict__ e,
-                                                     const float* __restrict__ n, const float* __restrict__ D,
+                                                     const CT* __restrict__ c, const CT* __restrict__ e,
+                                                     const CT* __restrict__ n, const CT* __restrict__ D,
                                                      const TB* __restrict__ b, float* __restrict__ x,
                                                      float* __restrict__ bc, const Scalars* __restrict__ sc) {
     typedef typename Pair<TB>::type TB2;
@@ -1953,7 +1962,7 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         const size_t k = cidx(ixc, j, nx);
         const TB2 bv = *(const TB2*)(b + k);
         bj = f2v{(float)bv.x, (float)bv.y};  // b is used in FP32 only
-        dj = *(const f2v*)(D + k);
+        dj = ldc2(D, k);
         if ((j & 1) == 0) r.x = mg_div(omega * bj.x, dj.x);
         else r.y = mg_div(omega * bj.y, dj.y);
         return r;
@@ -1977,7 +1986,7 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
     f2v r0 = red_row(iy0 - 2, b0, d0);
     f2v r1 = red_row(iy0 - 1, b1, d1);
     f2v e0 = z2, n0 = z2, nm = z2, X0 = z2, Xm = z2;
-    if (iy0 >= 2) n0 = *(const f2v*)(n + cidx(ixc, iy0 - 2, nx));
+    if (iy0 >= 2) n0 = ldc2(n, cidx(ixc, iy0 - 2, nx));
     float hrow = 0.f;  // the even row's pair sum
     for (int t = iy0 - 2; t < iy1; ++t) {
         f2v b2, d2;
@@ -1985,13 +1994,13 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         f2v e1 = z2, n1 = z2;
         if (t + 1 >= 0 && t + 1 < ny) {
             const size_t k1 = cidx(ixc, t + 1, nx);
-            e1 = *(const f2v*)(e + k1);
-            n1 = *(const f2v*)(n + k1);
+            e1 = ldc2(e, k1);
+            n1 = ldc2(n, k1);
         }
         const f2v X1 = full_row(t + 1, r0, r1, r2, b1, d1, e1, n1, n0);
         if (t >= iy0) {
             const size_t k = cidx(ixc, t, nx);
-            const f2v cc = *(const f2v*)(c + k);
+            const f2v cc = ldc2(c, k);
             const float ue = __shfl_down(X0.x, 1, WAVE);  // column ix + 2
             const float uw = __shfl_up(X0.y, 1, WAVE);    // column ix - 1
             const float ew = __shfl_up(e0.y, 1, WAVE);    // coupling (ix - 1, ix)
@@ -2032,10 +2041,10 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
 // Up-leg: u = x + scale P xc; black cells relax from u, then red cells from
 // the relaxed black neighbours.  Step t reads u of row t+2, relaxes the
 // black cell of row t+1 and the red cell of row t, and writes row t.
-template <class TB>
+template <class TB, class CT>
 __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int R, float omega, float scale,
-                                                   const float* __restrict__ c, const float* __restrict__ e,
-                                                   const float* __restrict__ n, const float* __restrict__ D,
+                                                   const CT* __restrict__ c, const CT* __restrict__ e,
+                                                   const CT* __restrict__ n, const CT* __restrict__ D,
                                                    const TB* __restrict__ b, const float* __restrict__ x,
                                                    const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
                                                    double* partials, Scalars* __restrict__ sc) {
@@ -2095,7 +2104,7 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
         b0.x = (TB)0;
         b0.y = (TB)0;
         f2v c0 = z2, d0 = z2, e0 = z2, n0 = z2, nm = z2;
-        if (iy0 >= 2) n0 = *(const f2v*)(n + cidx(ixc, iy0 - 2, nx));
+        if (iy0 >= 2) n0 = ldc2(n, cidx(ixc, iy0 - 2, nx));
         for (int t = iy0 - 2; t < iy1; ++t) {
             const f2v u2 = u_row(t + 2);
             TB2 b1 = b0;
@@ -2103,10 +2112,10 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
             if (t + 1 >= 0 && t + 1 < ny) {
                 const size_t k1 = cidx(ixc, t + 1, nx);
                 b1 = *(const TB2*)(b + k1);
-                c1 = *(const f2v*)(c + k1);
-                d1 = *(const f2v*)(D + k1);
-                e1 = *(const f2v*)(e + k1);
-                n1 = *(const f2v*)(n + k1);
+                c1 = ldc2(c, k1);
+                d1 = ldc2(D, k1);
+                e1 = ldc2(e, k1);
+                n1 = ldc2(n, k1);
             }
             const f2v v1 = black_row(t + 1, u0, u1, u2, f2v{(float)b1.x, (float)b1.y}, c1, d1, e1, n1, n0);
             if (t >= iy0) {
@@ -2144,10 +2153,10 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
 // indices clamped into the grid, rows outside it zeroed after the load) is
 // issued at once, and the done flag is read with them; the arithmetic per
 // row is that of k_mg_down_rb2 / k_mg_up_rb2, in the same order.
-template <class TB>
+template <class TB, class CT>
 __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, float omega,
-                                                      const float* __restrict__ c, const float* __restrict__ e,
-                                                      const float* __restrict__ n, const float* __restrict__ D,
+                                                      const CT* __restrict__ c, const CT* __restrict__ e,
+                                                      const CT* __restrict__ n, const CT* __restrict__ D,
                                                       const TB* __restrict__ b, float* __restrict__ x,
                                                       float* __restrict__ bc, const Scalars* __restrict__ sc) {
     typedef typename Pair<TB>::type TB2;
@@ -2169,14 +2178,14 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
         const size_t k = row(iy0 - 2 + q);
         const TB2 v = *(const TB2*)(b + k);
         Bv[q] = f2v{(float)v.x, (float)v.y};
-        Dv[q] = *(const f2v*)(D + k);
+        Dv[q] = ldc2(D, k);
     }
 #pragma unroll
-    for (int q = 0; q < 5; ++q) Nv[q] = *(const f2v*)(n + row(iy0 - 2 + q));
+    for (int q = 0; q < 5; ++q) Nv[q] = ldc2(n, row(iy0 - 2 + q));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Ev[q] = *(const f2v*)(e + row(iy0 - 1 + q));
+    for (int q = 0; q < 4; ++q) Ev[q] = ldc2(e, row(iy0 - 1 + q));
 #pragma unroll
-    for (int q = 0; q < 2; ++q) Cv[q] = *(const f2v*)(c + row(iy0 + q));
+    for (int q = 0; q < 2; ++q) Cv[q] = ldc2(c, row(iy0 + q));
     // every load is issued before the flag decides
 #pragma unroll
     for (int q = 0; q < 6; ++q) asm volatile("" : : "v"(Bv[q].x), "v"(Bv[q].y), "v"(Dv[q].x), "v"(Dv[q].y));
@@ -2254,10 +2263,10 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
     if (iy0 + 1 >= ny && own) bc[(size_t)(ix >> 1) + (size_t)(iy0 >> 1) * nxc] = hrow + 0.f;
 }
 
-template <class TB>
+template <class TB, class CT>
 __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, float omega, float scale,
-                                                    const float* __restrict__ c, const float* __restrict__ e,
-                                                    const float* __restrict__ n, const float* __restrict__ D,
+                                                    const CT* __restrict__ c, const CT* __restrict__ e,
+                                                    const CT* __restrict__ n, const CT* __restrict__ D,
                                                     const TB* __restrict__ b, const float* __restrict__ x,
                                                     const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
                                                     double* partials, Scalars* __restrict__ sc) {
@@ -2287,12 +2296,12 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
     for (int q = 0; q < 4; ++q) {
         const size_t k = row(iyc - 1 + q);
         Bt[q] = *(const TB2*)(b + k);
-        Cv[q] = *(const f2v*)(c + k);
-        Dv[q] = *(const f2v*)(D + k);
-        Ev[q] = *(const f2v*)(e + k);
+        Cv[q] = ldc2(c, k);
+        Dv[q] = ldc2(D, k);
+        Ev[q] = ldc2(e, k);
     }
 #pragma unroll
-    for (int q = 0; q < 5; ++q) Nv[q] = *(const f2v*)(n + row(iyc - 2 + q));
+    for (int q = 0; q < 5; ++q) Nv[q] = ldc2(n, row(iyc - 2 + q));
 #pragma unroll
     for (int q = 0; q < 6; ++q) asm volatile("" : : "v"(Xv[q].x), "v"(Xv[q].y), "v"(Cx[q].x));
 #pragma unroll
@@ -3431,6 +3440,68 @@ int mg_up_nblocks(int nx, int ny, int rb) {
     if ((long)nx * ny <= mg_flat_cells()) return (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y);
     return mg_wide(nx) ? mg_wide_geom(nx, ny).nblocks : mg_march_geom(nx, ny).nblocks;
 }
+// The finest level's red-black legs with BF16 coefficients (levels of even
+// width that take the two-column legs; see mg_bf16_ok).
+template <class TB>
+void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
+                  const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+    const Geom W = mg_rb_geom(nx, ny);
+    if (W.R == 2)
+        hipLaunchKernelGGL((k_mg_down_rb2s<TB, uint16_t>), W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D, b,
+                           x, bc, sc);
+    else
+        hipLaunchKernelGGL((k_mg_down_rb2<TB, uint16_t>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n,
+                           D, b, x, bc, sc);
+}
+template <class TB>
+void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
+                const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TB* out,
+                int dot_mode, double* partials, Scalars* sc, hipStream_t st) {
+    const Geom M = mg_rb_geom(nx, ny);
+    const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
+    if (M.R == 2)
+        hipLaunchKernelGGL((k_mg_up_rb2s<TB, uint16_t>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n,
+                           D, b, x, xc, out, with_dot, partials, sc);
+    else
+        hipLaunchKernelGGL((k_mg_up_rb2<TB, uint16_t>), M.grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c,
+                           e, n, D, b, x, xc, out, with_dot, partials, sc);
+    if (dot_mode == 0 || dot_mode == 1)
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)M.nblocks,
+                           dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
+}
+template void mg_down_bf16<float>(int, int, int, float, const uint16_t*, const uint16_t*, const uint16_t*,
+                                  const uint16_t*, const float*, float*, float*, const Scalars*, hipStream_t);
+template void mg_down_bf16<double>(int, int, int, float, const uint16_t*, const uint16_t*, const uint16_t*,
+                                   const uint16_t*, const double*, float*, float*, const Scalars*, hipStream_t);
+template void mg_up_bf16<float>(int, int, int, float, float, const uint16_t*, const uint16_t*, const uint16_t*,
+                                const uint16_t*, const float*, const float*, const float*, float*, int, double*,
+                                Scalars*, hipStream_t);
+template void mg_up_bf16<double>(int, int, int, float, float, const uint16_t*, const uint16_t*, const uint16_t*,
+                                 const uint16_t*, const double*, const float*, const float*, double*, int, double*,
+                                 Scalars*, hipStream_t);
+// Levels whose red-black legs are the two-column kernels (and so can read BF16 coefficients).
+bool mg_bf16_ok(int nx, int ny) { return !mg_rb_flat(nx, ny); }
+// FP32 -> BF16, round to nearest even, four arrays of n values
+__global__ __launch_bounds__(256) void k_to_bf16x4(size_t n, const float* __restrict__ a0, const float* __restrict__ a1,
+                                                   const float* __restrict__ a2, const float* __restrict__ a3,
+                                                   uint16_t* __restrict__ o0, uint16_t* __restrict__ o1,
+                                                   uint16_t* __restrict__ o2, uint16_t* __restrict__ o3) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    auto rn = [](float v) -> uint16_t {
+        const uint32_t u = __float_as_uint(v);
+        return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    };
+    o0[i] = rn(a0[i]);
+    o1[i] = rn(a1[i]);
+    o2[i] = rn(a2[i]);
+    o3[i] = rn(a3[i]);
+}
+void to_bf16x4(size_t n, const float* a0, const float* a1, const float* a2, const float* a3, uint16_t* o0,
+               uint16_t* o1, uint16_t* o2, uint16_t* o3, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_bf16x4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a0, a1, a2, a3, o0, o1,
+                       o2, o3);
+}
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
              const float* D, const TB* b, float* x, float* bc, const Scalars* sc, int rb, hipStream_t st) {
@@ -3441,10 +3512,10 @@ void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, cons
         } else {
             const Geom W = mg_rb_geom(nx, ny);
             if (W.R == 2)
-                hipLaunchKernelGGL(k_mg_down_rb2s<TB>, W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D, b, x,
+                hipLaunchKernelGGL((k_mg_down_rb2s<TB, float>), W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D, b, x,
                                    bc, sc);
             else
-                hipLaunchKernelGGL(k_mg_down_rb2<TB>, W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b,
+                hipLaunchKernelGGL((k_mg_down_rb2<TB, float>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b,
                                    x, bc, sc);
         }
         return;
@@ -3478,10 +3549,10 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
         hipLaunchKernelGGL(k_mg_up_rb_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n,
                            D, b, x, xc, out, with_dot, partials, sc);
     else if (rb && M.R == 2)
-        hipLaunchKernelGGL(k_mg_up_rb2s<TB>, grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
+        hipLaunchKernelGGL((k_mg_up_rb2s<TB, float>), grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
                            out, with_dot, partials, sc);
     else if (rb)
-        hipLaunchKernelGGL(k_mg_up_rb2<TB>, grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b, x,
+        hipLaunchKernelGGL((k_mg_up_rb2<TB, float>), grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b, x,
                            xc, out, with_dot, partials, sc);
     else if (flat)
         hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,

@@ -1,8 +1,9 @@
 """The library's multigrid preconditioner (one V-cycle, shk_apply_precond)
 against the NumPy restatement (tests/vcycle_restate.py) on the same assembled
 system, on shapes that between them run every V-cycle kernel: two-column
-red-black legs with FP64 and FP32 right-hand sides, the flat legs on odd-width
-and small levels, and the LDS tail with the dense coarsest solve.  FP32
+red-black legs with FP64 and FP32 right-hand sides and with BF16 and FP32
+coefficients, the flat legs on odd-width and small levels, and the LDS tail
+with the dense coarsest solve.  FP32
 inside: the two agree to 5e-4 of |z| (rounding order and the hardware
 reciprocal); a wrong neighbour or smoothing order is off by orders of magnitude.
 Symmetry r2'M r1 = r1'M r2 is checked on the library's own output."""
@@ -33,7 +34,9 @@ def test_vcycle_matches_restatement(hip, shape):
     arrs = [np.zeros((nx, ny), order="F") for _ in range(6)]
     L.check(lib.shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
     aP, aE, _, aN, _, _ = arrs
-    M = VCycle(aP, aE, aN)  # the library's default smoothing weight and coarse scale
+    # the library's default smoothing weight and coarse scale; BF16 finest-level
+    # coefficients where the finest level takes the two-column legs
+    M = VCycle(aP, aE, aN, bf16_level0=(nx % 2 == 0 and nx * ny > 4096))
     rng = np.random.default_rng(7)
     rs = [np.asfortranarray(rng.standard_normal((nx, ny))) for _ in range(2)]
     zs = []
@@ -67,7 +70,7 @@ def test_vcycle_parabolic_system(hip):
     L.check(lib.shk_assemble(hs.ctx, 1, 600.0), hs.ctx)
     arrs = [np.zeros((nx, ny), order="F") for _ in range(6)]
     L.check(lib.shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
-    M = VCycle(arrs[0], arrs[1], arrs[3], omega=1.2, scale=1.0)
+    M = VCycle(arrs[0], arrs[1], arrs[3], omega=1.2, scale=1.0, bf16_level0=True)
     r = np.asfortranarray(np.random.default_rng(8).standard_normal((nx, ny)))
     z = np.zeros((nx, ny), order="F")
     L.check(lib.shk_apply_precond(hs.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),

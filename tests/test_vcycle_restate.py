@@ -5,7 +5,9 @@ pinned by what the mathematics fixes on a real assembled glacier system (CPU):
   (a swapped smoothing order or a wrong neighbour breaks the symmetry);
 * PCG with it converges to 1e-8 in a small fraction of Jacobi-PCG's iterations;
 * red-black smoothing needs fewer CG iterations than the same cycle with damped
-  Jacobi smoothing (omega 0.95, scale 1.5), the reason it is the default.
+  Jacobi smoothing (omega 0.95, scale 1.5), the reason it is the default;
+* the finest level's coefficients in BF16 cost at most one iteration, the
+  reason the library stores them so.
 
 The GPU test (test_gpu_vcycle.py) then checks the library's V-cycle against it."""
 import numpy as np
@@ -91,3 +93,6 @@ def test_red_black_cycle_beats_jacobi_pcg_and_the_jacobi_cycle(oracle, system):
     jacobi = _pcg(oracle, co, lambda r: r / co[0])
     assert rb * 10 < jacobi, (rb, jacobi)
     assert rb < 0.9 * jc, (rb, jc)
+    # the finest level's coefficients rounded to BF16 cost at most one iteration
+    rb16 = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True))
+    assert rb16 <= rb + 1, (rb16, rb)

@@ -12,7 +12,10 @@ from a reference file:
   dense inverse (formed in FP64, kept in FP32);
 * smoothing: symmetric red-black Gauss-Seidel with over-relaxation omega (red
   cells ix + iy even): down-leg from x = 0, red then black; up-leg after the
-  coarse correction u = x + scale P xc, black then red.
+  coarse correction u = x + scale P xc, black then red;
+* bf16_level0: the finest level's smoothing and residual use its c, e, n, D
+  rounded to BF16 (round to nearest even), as the library does where the
+  finest level takes the two-column legs; coarsening uses the FP32 values.
 
 Arrays are indexed [ix, iy] with shape (nx, ny); everything is float32.
 """
@@ -39,6 +42,13 @@ def neighbour_sum(e, n, x):
     s[:, :-1] += n[:, :-1] * x[:, 1:]
     s[:, 1:] += n[:, :-1] * x[:, :-1]
     return s
+
+
+def bf16(a):
+    """FP32 -> BF16 -> FP32, round to nearest even."""
+    u = np.asarray(a, dtype=F32).view(np.uint32)
+    u = ((u + np.uint32(0x7FFF) + ((u >> np.uint32(16)) & np.uint32(1))) & np.uint32(0xFFFF0000)).astype(np.uint32)
+    return u.view(F32)
 
 
 def fine_level(aP, aE, aN):
@@ -104,7 +114,7 @@ def dense_inverse(c, e, n):
 
 
 class VCycle:
-    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8):
+    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False):
         self.omega, self.scale = F32(omega), F32(scale)
         c, e, n, D = fine_level(aP, aE, aN)
         self.levels = [(c, e, n, D)]
@@ -114,6 +124,8 @@ class VCycle:
                 break
             cc, ec, nc = coarsen(c, e, n)
             self.levels.append((cc, ec, nc, diag(cc, ec, nc)))
+        if bf16_level0:
+            self.levels[0] = tuple(bf16(a) for a in self.levels[0])
         c, e, n, _ = self.levels[-1]
         self.inv = dense_inverse(c, e, n)
         self.red = [((np.arange(L[0].shape[0])[:, None] + np.arange(L[0].shape[1])[None, :]) % 2) == 0
