@@ -477,8 +477,11 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 // z that FP64 CG uses; every level computes in FP32.  Levels above
 // MG_FLAT_CELLS cells run marching kernels, the middle levels single-phase
 // kernels, the levels below MG_TAIL_CELLS cells one workgroup.
+// zf: where given (FP64 CG with the deferred finish, BF16 finest level), the
+// result goes there in FP32, the precision the V-cycle computes it in; the
+// CG matvec widens it as it loads it, to the values an FP64 z would hold.
 template <class T>
-void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
+void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
     auto& mg = ctx->mg;
     const float scale = ctx->system_parabolic ? ctx->mg_scale_par : ctx->mg_scale;
     const size_t L = mg.size() - 1;
@@ -522,9 +525,14 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode) {
         // reads b, c, aE, aN, D, x and the coarse correction; writes the result
         Prof pr(ctx, l == 0 ? P_MGU0 : P_MGU,
                 4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
-        if (l == 0 && ctx->mg_bf16)
-            launch::mg_up_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
-                                  ctx->mg_D16, r, f.xa, mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->st);
+        if (l == 0 && ctx->mg_bf16 && zf)
+            launch::mg_up_bf16<T, float>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
+                                         ctx->mg_n16, ctx->mg_D16, r, f.xa, mg[1].xb, zf, mode, ctx->partials,
+                                         ctx->sc, ctx->st);
+        else if (l == 0 && ctx->mg_bf16)
+            launch::mg_up_bf16<T, T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
+                                     ctx->mg_n16, ctx->mg_D16, r, f.xa, mg[1].xb, z, mode, ctx->partials, ctx->sc,
+                                     ctx->st);
         else if (l == 0)
             launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, f.c, f.e, f.n, f.D, r, f.xa,
                              mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->mg_rb, ctx->st);
@@ -610,8 +618,14 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                            ctx->st);
     }
     const bool multigrid = precond == SHK_PRECOND_MULTIGRID;
+    // FP64 CG with the V-cycle and the deferred finish: z kept in FP32 (the
+    // precision the V-cycle produces it in) where the finest level is BF16 and
+    // the matvec is the two-column kernel
+    const bool zf32 = sizeof(T) == 8 && multigrid && nn_every == 0 && ctx->use_defer && ctx->mg_bf16 &&
+                      cg_wide_ok(G);
+    float* const zf = zf32 ? ctx->z32 : nullptr;
     if (multigrid) {
-        mg_apply<T>(ctx, w.r, w.z, 0);
+        mg_apply<T>(ctx, w.r, w.z, 0, zf);
     } else if (!jacobi) {
         cheb_apply<T>(ctx, w, degree, 0);
     }
@@ -651,15 +665,18 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                     int nbb;
                     {
                         Prof pr(ctx, P_MV, 7 * nb);  // reads aP, aE, aN, z, p_old; writes p_new, Ap
-                        nbb = launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap, ctx->partials2,
-                                                         ctx->sc, ctx->partials, nba, ctx->st);
+                        nbb = zf32 ? launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, (const float*)zf, p_old, p_new,
+                                                                w.Ap, ctx->partials2, ctx->sc, ctx->partials, nba,
+                                                                ctx->st)
+                                   : launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap,
+                                                                ctx->partials2, ctx->sc, ctx->partials, nba, ctx->st);
                     }
                     {
                         Prof pr(ctx, P_UPD, 6 * nb);  // reads x, p, r, Ap; writes x, r
                         launch::cg_update_deferred(G, p_new, w.Ap, w.x, w.r, ctx->sc, ctx->partials, nba,
                                                    ctx->partials2, nbb, ctx->st);
                     }
-                    mg_apply<T>(ctx, w.r, w.z, 2);
+                    mg_apply<T>(ctx, w.r, w.z, 2, zf);
                     T* t = p_old;
                     p_old = p_new;
                     p_new = t;

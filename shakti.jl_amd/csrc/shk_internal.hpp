@@ -165,6 +165,9 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
                        int nba, hipStream_t st);
+int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const float* z,
+                       const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
+                       int nba, hipStream_t st);
 void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
                         const double* pa, int nba, const double* pb, int nbb, hipStream_t st);
 int mg_up_nblocks(int nx, int ny, int rb);
@@ -198,9 +201,9 @@ void to_bf16x4(size_t n, const float* a0, const float* a1, const float* a2, cons
 template <class TB>
 void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
                   const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
-template <class TB>
+template <class TB, class TO>
 void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
-                const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TB* out,
+                const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TO* out,
                 int dot_mode, double* partials, Scalars* sc, hipStream_t st);
 // dynamic LDS bytes of the tail kernel (rhs, x, c, e, n, D per cell and the
 // coarsest inverse), and the opt-in for more than the default 64 KB (0 = ok)

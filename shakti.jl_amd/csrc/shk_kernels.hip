@@ -651,6 +651,7 @@ __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, c
 // strip: the CG matvec at 4096 x 1024 took 49 against 51 us (57 us for the
 // one-column kernel) with its inputs evicted from the caches.
 typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 template <bool NT>
 __device__ __forceinline__ d2v ld2(const double* p) {
     if constexpr (NT) return __builtin_nontemporal_load((const d2v*)p);
@@ -1130,11 +1131,18 @@ __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const 
     block_partial<1, false>(v, partials);
 }
 
-// k_cg_matvec on two columns per lane (FP64, even nx).
-template <bool NT, bool DEFER>
+// k_cg_matvec on two columns per lane (FP64, even nx).  z is FP64, or the
+// V-cycle's FP32 result (ZT = float), widened as it is loaded: the same
+// values the FP64 copy would hold.
+__device__ __forceinline__ d2v ldz2(const double* p) { return ld2<false>(p); }
+__device__ __forceinline__ d2v ldz2(const float* p) {
+    const f2v v = *(const f2v*)p;
+    return d2v{(double)v.x, (double)v.y};
+}
+template <bool NT, bool DEFER, class ZT = double>
 __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const double* __restrict__ aP,
                                                     const double* __restrict__ aE, const double* __restrict__ aN,
-                                                    const double* __restrict__ z, const double* __restrict__ p_old,
+                                                    const ZT* __restrict__ z, const double* __restrict__ p_old,
                                                     double* __restrict__ p_new, double* __restrict__ Ap,
                                                     double* partials, Scalars* sc, const double* __restrict__ pa,
                                                     int nba) {
@@ -1146,8 +1154,8 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
     double acc = 0.0;
     march2<NT>(
         nx, ny, R, strip, iy0, aP, aE, aN,
-        [&](size_t j) -> d2v { return ld2<false>(z + j) + beta * ld2<false>(p_old + j); },
-        [&](size_t j) -> double { return z[j] + beta * p_old[j]; },
+        [&](size_t j) -> d2v { return ldz2(z + j) + beta * ld2<false>(p_old + j); },
+        [&](size_t j) -> double { return (double)z[j] + beta * p_old[j]; },
         [&](size_t c, d2v Au, d2v u) {
             st2<NT>(p_new + c, u);
             st2<NT>(Ap + c, Au);
@@ -1754,7 +1762,6 @@ __global__ __launch_bounds__(256) void k_mg_up_march(int nx, int ny, int nxc, in
 // reads one coarse value per lane and row.  Tiles as the wide CG matvec
 // (wide_tile, R even); the per-cell arithmetic and its order are those of
 // the one-column kernels, so x, bc and z are identical to theirs.
-typedef float f2v __attribute__((ext_vector_type(2)));
 template <class TB>
 struct Pair;
 template <>
@@ -2041,12 +2048,12 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
 // Up-leg: u = x + scale P xc; black cells relax from u, then red cells from
 // the relaxed black neighbours.  Step t reads u of row t+2, relaxes the
 // black cell of row t+1 and the red cell of row t, and writes row t.
-template <class TB, class CT>
+template <class TB, class CT, class TO>
 __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int R, float omega, float scale,
                                                    const CT* __restrict__ c, const CT* __restrict__ e,
                                                    const CT* __restrict__ n, const CT* __restrict__ D,
                                                    const TB* __restrict__ b, const float* __restrict__ x,
-                                                   const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
+                                                   const float* __restrict__ xc, TO* __restrict__ out, int with_dot,
                                                    double* partials, Scalars* __restrict__ sc) {
     typedef typename Pair<TB>::type TB2;
     if (sc->done) return;
@@ -2121,10 +2128,10 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
             if (t >= iy0) {
                 const f2v z = red_relax(t, vm, v0, v1, f2v{(float)b0.x, (float)b0.y}, c0, d0, e0, n0, nm);
                 if (own) {
-                    TB2 zv;
-                    zv.x = (TB)z.x;
-                    zv.y = (TB)z.y;
-                    *(TB2*)(out + cidx(ix, t, nx)) = zv;
+                    typename Pair<TO>::type zv;
+                    zv.x = (TO)z.x;
+                    zv.y = (TO)z.y;
+                    *(typename Pair<TO>::type*)(out + cidx(ix, t, nx)) = zv;
                     acc += (double)b0.x * (double)z.x;
                     acc += (double)b0.y * (double)z.y;
                 }
@@ -2263,12 +2270,12 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
     if (iy0 + 1 >= ny && own) bc[(size_t)(ix >> 1) + (size_t)(iy0 >> 1) * nxc] = hrow + 0.f;
 }
 
-template <class TB, class CT>
+template <class TB, class CT, class TO>
 __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, float omega, float scale,
                                                     const CT* __restrict__ c, const CT* __restrict__ e,
                                                     const CT* __restrict__ n, const CT* __restrict__ D,
                                                     const TB* __restrict__ b, const float* __restrict__ x,
-                                                    const float* __restrict__ xc, TB* __restrict__ out, int with_dot,
+                                                    const float* __restrict__ xc, TO* __restrict__ out, int with_dot,
                                                     double* partials, Scalars* __restrict__ sc) {
     typedef typename Pair<TB>::type TB2;
     const int done = sc->done;  // arrives with the loads below
@@ -2369,10 +2376,10 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
                                    hS ? vm.y : 0.f);
                 }
                 if (own) {
-                    TB2 zv;
-                    zv.x = (TB)z.x;
-                    zv.y = (TB)z.y;
-                    *(TB2*)(out + cidx(ix, t, nx)) = zv;
+                    typename Pair<TO>::type zv;
+                    zv.x = (TO)z.x;
+                    zv.y = (TO)z.y;
+                    *(typename Pair<TO>::type*)(out + cidx(ix, t, nx)) = zv;
                     acc += (double)bt.x * (double)z.x;
                     acc += (double)bt.y * (double)z.y;
                 }
@@ -3342,6 +3349,19 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
                        p_new, Ap, pb, sc, pa, nba);
     return G.nblocks;
 }
+// The same with the V-cycle's FP32 z (cg_wide_ok grids only).
+int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const float* z,
+                       const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
+                       int nba, hipStream_t st) {
+    const Geom W = wide_geom(G);
+    if (stream_nt(G))
+        hipLaunchKernelGGL((k_cg_matvec2<true, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
+                           p_old, p_new, Ap, pb, sc, pa, nba);
+    else
+        hipLaunchKernelGGL((k_cg_matvec2<false, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
+                           z, p_old, p_new, Ap, pb, sc, pa, nba);
+    return W.nblocks;
+}
 void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
                         const double* pa, int nba, const double* pb, int nbb, hipStream_t st) {
     // 2048 workgroups (one per 8 waves of the chip): each re-sums the partials
@@ -3453,17 +3473,17 @@ void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const
         hipLaunchKernelGGL((k_mg_down_rb2<TB, uint16_t>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n,
                            D, b, x, bc, sc);
 }
-template <class TB>
+template <class TB, class TO>
 void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
-                const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TB* out,
+                const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TO* out,
                 int dot_mode, double* partials, Scalars* sc, hipStream_t st) {
     const Geom M = mg_rb_geom(nx, ny);
     const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
     if (M.R == 2)
-        hipLaunchKernelGGL((k_mg_up_rb2s<TB, uint16_t>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n,
+        hipLaunchKernelGGL((k_mg_up_rb2s<TB, uint16_t, TO>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n,
                            D, b, x, xc, out, with_dot, partials, sc);
     else
-        hipLaunchKernelGGL((k_mg_up_rb2<TB, uint16_t>), M.grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c,
+        hipLaunchKernelGGL((k_mg_up_rb2<TB, uint16_t, TO>), M.grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c,
                            e, n, D, b, x, xc, out, with_dot, partials, sc);
     if (dot_mode == 0 || dot_mode == 1)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)M.nblocks,
@@ -3473,12 +3493,15 @@ template void mg_down_bf16<float>(int, int, int, float, const uint16_t*, const u
                                   const uint16_t*, const float*, float*, float*, const Scalars*, hipStream_t);
 template void mg_down_bf16<double>(int, int, int, float, const uint16_t*, const uint16_t*, const uint16_t*,
                                    const uint16_t*, const double*, float*, float*, const Scalars*, hipStream_t);
-template void mg_up_bf16<float>(int, int, int, float, float, const uint16_t*, const uint16_t*, const uint16_t*,
-                                const uint16_t*, const float*, const float*, const float*, float*, int, double*,
-                                Scalars*, hipStream_t);
-template void mg_up_bf16<double>(int, int, int, float, float, const uint16_t*, const uint16_t*, const uint16_t*,
-                                 const uint16_t*, const double*, const float*, const float*, double*, int, double*,
-                                 Scalars*, hipStream_t);
+template void mg_up_bf16<float, float>(int, int, int, float, float, const uint16_t*, const uint16_t*,
+                                       const uint16_t*, const uint16_t*, const float*, const float*, const float*,
+                                       float*, int, double*, Scalars*, hipStream_t);
+template void mg_up_bf16<double, double>(int, int, int, float, float, const uint16_t*, const uint16_t*,
+                                         const uint16_t*, const uint16_t*, const double*, const float*, const float*,
+                                         double*, int, double*, Scalars*, hipStream_t);
+template void mg_up_bf16<double, float>(int, int, int, float, float, const uint16_t*, const uint16_t*,
+                                        const uint16_t*, const uint16_t*, const double*, const float*, const float*,
+                                        float*, int, double*, Scalars*, hipStream_t);
 // Levels whose red-black legs are the two-column kernels (and so can read BF16 coefficients).
 bool mg_bf16_ok(int nx, int ny) { return !mg_rb_flat(nx, ny); }
 // FP32 -> BF16, round to nearest even, four arrays of n values
@@ -3549,10 +3572,10 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
         hipLaunchKernelGGL(k_mg_up_rb_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n,
                            D, b, x, xc, out, with_dot, partials, sc);
     else if (rb && M.R == 2)
-        hipLaunchKernelGGL((k_mg_up_rb2s<TB, float>), grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
+        hipLaunchKernelGGL((k_mg_up_rb2s<TB, float, TB>), grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n, D, b, x, xc,
                            out, with_dot, partials, sc);
     else if (rb)
-        hipLaunchKernelGGL((k_mg_up_rb2<TB, float>), grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b, x,
+        hipLaunchKernelGGL((k_mg_up_rb2<TB, float, TB>), grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c, e, n, D, b, x,
                            xc, out, with_dot, partials, sc);
     else if (flat)
         hipLaunchKernelGGL(k_mg_up_flat<TB>, grid, dim3(WAVE, WAVES), 0, st, nx, ny, nxc, omega, scale, c, e, n, D,

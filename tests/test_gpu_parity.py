@@ -264,13 +264,15 @@ def test_fp32_refinement_picard_trajectory(oracle, hip):
         assert rel_err(hs.get("b"), os_.b) <= 1e-6, t
 
 
-@pytest.mark.parametrize("shape", [(6, 6), (257, 65), (301, 77), (1283, 1001)])
+@pytest.mark.parametrize("shape", [(6, 6), (257, 65), (301, 77), (1283, 1001), (2048, 1024)])
 def test_multigrid_fp32_refinement_solve(oracle, hip, shape):
     """FP32 multigrid-preconditioned CG inside FP64 refinement, and FP64 CG
     preconditioned by the FP32 V-cycle, solve to within 1e-9 of the FP64
-    Chebyshev form in far fewer iterations (grids ragged against the 2x2
-    coarsening; 1283x1001 has a finest level above 1M cells, which runs the
-    LDS-tiled kernels, the others the single-phase ones)."""
+    Chebyshev form in far fewer iterations.  The first four grids are ragged
+    against the 2x2 coarsening (odd widths take the flat red-black legs);
+    2048x1024 (2 M cells, even) runs the two-column legs with BF16 finest-level
+    coefficients and the FP64 CG with the two-column matvec and the V-cycle's
+    z kept in FP32."""
     from shakti_hip import synthetic as S
     nx, ny = shape
     if shape == (6, 6):
