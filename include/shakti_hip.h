@@ -206,9 +206,10 @@ int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double
  * timed alone after a 1 GiB read has replaced the contents of L2 and the
  * Infinity Cache (allocates that buffer for the duration of the call); 5 the
  * FP64 CG matvec kernel (p = z + beta p_old, Ap, p'Ap partials) on the solve's
- * work vectors, cold like 3 and 4: in the V-cycle-preconditioned solve the
- * V-cycle between two matvecs streams their inputs out of the caches (the
- * device scalars are restored afterwards). */
+ * work vectors, in the variant the V-cycle-preconditioned solve runs (z read
+ * in FP32 where that solve keeps it so: 52 instead of 56 bytes per cell), cold
+ * like 3 and 4: in that solve the V-cycle between two matvecs streams their
+ * inputs out of the caches (the device scalars are restored afterwards). */
 int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch);
 
 /* Per-kernel timing collected with HIP events on the context's stream while

@@ -486,12 +486,15 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
     const float scale = ctx->system_parabolic ? ctx->mg_scale_par : ctx->mg_scale;
     const size_t L = mg.size() - 1;
     const size_t first = ctx->mg_tail_first;  // >= 1
-    const double tb = sizeof(T) - 4.0;        // extra bytes per cell of an FP64 rhs / result
+    // bytes per cell of the finest level: its rhs (T), its c, aE, aN, D (BF16 or
+    // FP32) and its result (zf: FP32, else T); the coarser levels are all FP32
+    const double rb0 = sizeof(T), cb0 = ctx->mg_bf16 ? 8.0 : 16.0, zb0 = zf ? 4.0 : (double)sizeof(T);
     for (size_t l = 0; l < first; ++l) {
         const auto& f = mg[l];
         // reads b, c, aE, aN, D; writes x and the coarse rhs
+        const double cells = (double)f.nx * f.ny, coarse = (double)mg[l + 1].nx * mg[l + 1].ny;
         Prof pr(ctx, l == 0 ? P_MGD0 : P_MGD,
-                4.0 * (6.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? tb * f.nx * f.ny : 0.0));
+                l == 0 ? (rb0 + cb0 + 4.0) * cells + 4.0 * coarse : 4.0 * (6.0 * cells + coarse));
         if (l == 0 && ctx->mg_bf16)
             launch::mg_down_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
                                     ctx->mg_D16, r, f.xa, mg[1].b, ctx->sc, ctx->st);
@@ -523,8 +526,9 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
     for (size_t l = first; l-- > 0;) {
         const auto& f = mg[l];
         // reads b, c, aE, aN, D, x and the coarse correction; writes the result
+        const double cells = (double)f.nx * f.ny, coarse = (double)mg[l + 1].nx * mg[l + 1].ny;
         Prof pr(ctx, l == 0 ? P_MGU0 : P_MGU,
-                4.0 * (7.0 * f.nx * f.ny + (double)mg[l + 1].nx * mg[l + 1].ny) + (l == 0 ? 2 * tb * f.nx * f.ny : 0.0));
+                l == 0 ? (rb0 + cb0 + 4.0 + zb0) * cells + 4.0 * coarse : 4.0 * (7.0 * cells + coarse));
         if (l == 0 && ctx->mg_bf16 && zf)
             launch::mg_up_bf16<T, float>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
                                          ctx->mg_n16, ctx->mg_D16, r, f.xa, mg[1].xb, zf, mode, ctx->partials,
@@ -664,7 +668,8 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                 if (defer) {
                     int nbb;
                     {
-                        Prof pr(ctx, P_MV, 7 * nb);  // reads aP, aE, aN, z, p_old; writes p_new, Ap
+                        // reads aP, aE, aN, z (FP32 or FP64), p_old; writes p_new, Ap
+                        Prof pr(ctx, P_MV, (zf32 ? 6.5 : 7.0) * nb);
                         nbb = zf32 ? launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, (const float*)zf, p_old, p_new,
                                                                 w.Ap, ctx->partials2, ctx->sc, ctx->partials, nba,
                                                                 ctx->st)
@@ -1468,6 +1473,9 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     double bytes = 0;
+    // the variant the V-cycle-preconditioned solve runs: z in FP32 where its
+    // finest level is BF16 and the matvec the two-column kernel
+    const bool z32 = ctx->use_defer && ctx->mg_bf16 && cg_wide_ok(G);
     auto one = [&]() {
         switch (kernel) {
             case 0:
@@ -1476,8 +1484,12 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
             case 2:
             case 4: launch::copy(ctx->h_alt, ctx->d64, ctx->n, ctx->st); break;
             case 5:
-                launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
-                                           ctx->partials2, ctx->sc, ctx->partials, 0, ctx->st);
+                if (z32)
+                    launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, (const float*)ctx->z32, ctx->pA,
+                                               ctx->pB, ctx->Ap, ctx->partials2, ctx->sc, ctx->partials, 0, ctx->st);
+                else
+                    launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
+                                               ctx->partials2, ctx->sc, ctx->partials, 0, ctx->st);
                 break;
             default: break;
         }
@@ -1485,7 +1497,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     if (kernel == 0 || kernel == 3) bytes = 5 * n * 8;
     else if (kernel == 1) bytes = 5 * n * 4;
     else if (kernel == 2 || kernel == 4) bytes = 2 * n * 8;
-    else if (kernel == 5) bytes = 7 * n * 8;
+    else if (kernel == 5) bytes = z32 ? 6.5 * n * 8 : 7 * n * 8;
     else {
         hipEventDestroy(a);
         hipEventDestroy(b);

@@ -102,3 +102,27 @@ def test_parabolic_solve_after_an_elliptic_one_matches_a_fresh_context(hip):
     print(f"parabolic CG iterations: after an elliptic solve {out[0][0]}, fresh {out[1][0]}", flush=True)
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_fp32_z_gives_the_same_iterates(hip, monkeypatch):
+    """FP64 CG with the V-cycle keeps z in FP32 where the deferred finish runs, the
+    finest level is BF16 and the matvec is the two-column kernel (2048 x 1024
+    here): the up-leg stores the FP32 values it computes and the matvec widens
+    them on load, so the CG iterates are those of the FP64-z path
+    (SHK_NO_DEFER=1, which finishes each reduction in its own kernel in the same
+    order), bit for bit: the same iteration count and the same head."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(2048, 1024, bed_amp=50.0, seed=5)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    out = []
+    for no_defer in (False, True):
+        if no_defer:
+            monkeypatch.setenv("SHK_NO_DEFER", "1")
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=1e-10)
+        hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+        out.append((ls.last_stats.niter, hs.get("h")))
+        monkeypatch.delenv("SHK_NO_DEFER", raising=False)
+    print(f"CG iterations: FP32 z {out[0][0]}, FP64 z {out[1][0]}", flush=True)
+    assert out[0][0] == out[1][0] > 0
+    assert np.array_equal(out[0][1], out[1][1])
