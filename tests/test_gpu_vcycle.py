@@ -2,8 +2,10 @@
 against the NumPy restatement (tests/vcycle_restate.py) on the same assembled
 system, on shapes that between them run every V-cycle kernel: two-column
 red-black legs with FP64 and FP32 right-hand sides and with BF16 and FP32
-coefficients, the flat legs on odd-width and small levels, and the LDS tail
-with the dense coarsest solve.  FP32
+coefficients, at two rows per wave and at 4 and 8 (2052 x 1023, 4100 x 1025:
+the finest level's marching legs with their loads issued a step ahead, last
+tiles of 3 and 1 rows), the flat legs on odd-width and small levels, and the
+LDS tail with the dense coarsest solve.  FP32
 inside: the two agree to 5e-4 of |z| (rounding order and the hardware
 reciprocal); a wrong neighbour or smoothing order is off by orders of magnitude.
 Symmetry r2'M r1 = r1'M r2 is checked on the library's own output."""
@@ -18,7 +20,7 @@ from vcycle_restate import VCycle
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(130, 66), (520, 130), (1282, 150), (6, 6)])
+@pytest.mark.parametrize("shape", [(130, 66), (520, 130), (1282, 150), (6, 6), (2052, 1023), (4100, 1025)])
 def test_vcycle_matches_restatement(hip, shape):
     from shakti_hip import synthetic as S
     from shakti_hip import api

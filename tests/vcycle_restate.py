@@ -67,19 +67,23 @@ def fine_level(aP, aE, aN):
 
 
 def coarsen(c, e, n):
+    """2 x 2 aggregation; a block's cells summed as ((c00 + c10) + c01) + c11,
+    cells beyond an odd edge counting as zero."""
     fx, fy = c.shape
     cx, cy = (fx + 1) // 2, (fy + 1) // 2
-    C = np.zeros((cx, cy), F32)
-    Ec = np.zeros((cx, cy), F32)
-    Nc = np.zeros((cx, cy), F32)
-    for J in range(cy):
-        y0, y1 = 2 * J, min(2 * J + 1, fy - 1)
-        for I in range(cx):
-            x0, x1 = 2 * I, min(2 * I + 1, fx - 1)
-            C[I, J] = c[x0:x1 + 1, y0:y1 + 1].sum(dtype=F32)
-            Ec[I, J] = e[x1, y0:y1 + 1].sum(dtype=F32) if I < cx - 1 else F32(0)
-            Nc[I, J] = n[x0:x1 + 1, y1].sum(dtype=F32) if J < cy - 1 else F32(0)
-    return C, Ec, Nc
+
+    def pad(a):
+        p = np.zeros((2 * cx, 2 * cy), F32)
+        p[:fx, :fy] = a
+        return p
+
+    cp, ep, np_ = pad(c), pad(e), pad(n)
+    C = ((cp[0::2, 0::2] + cp[1::2, 0::2]) + cp[0::2, 1::2]) + cp[1::2, 1::2]
+    Ec = ep[1::2, 0::2] + ep[1::2, 1::2]  # faces on the block's east side
+    Ec[-1, :] = 0.0
+    Nc = np_[0::2, 1::2] + np_[1::2, 1::2]  # faces on its north side
+    Nc[:, -1] = 0.0
+    return C.astype(F32), Ec.astype(F32), Nc.astype(F32)
 
 
 def diag(c, e, n):
