@@ -2794,63 +2794,68 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
 }
 
 // Coarsest level (at most 64 cells): assemble the dense FP64 matrix and invert
-// it in place in LDS (Gauss-Jordan without pivoting, the matrix being SPD);
-// the inverse is stored in FP32:
+// it in place (Gauss-Jordan without pivoting, the matrix being SPD); the
+// inverse is stored in FP32:
 //   p = A_kk;  A_kk = 1/p;  A_kj /= p (j != k);  A_ij -= A_ik A_kj (i, j != k);
 //   A_ik = -A_ik / p (i != k).
-// Workgroup 64 x 4: thread (tx, ty) owns column j = tx of rows ty, ty+4, ...
-__global__ __launch_bounds__(256) void k_mg_dense_inverse_lds(int nx, int ny, const float* __restrict__ c,
-                                                              const float* __restrict__ e,
-                                                              const float* __restrict__ nn,
-                                                              float* __restrict__ inv) {
-    __shared__ double A[64 * 64];
-    __shared__ double colk[64];
+// Workgroup 64 x 16: thread (j, ty) keeps A_ij for the rows i = ty + 16 m
+// (m = 0..3) in registers, so each wave holds all 64 columns of its rows and
+// takes the pivot column A_ik from lane k by readlane; the pivot row goes
+// through LDS, double-buffered, with one barrier a step.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long w = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)w, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(w >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__global__ __launch_bounds__(1024) void k_mg_dense_inverse_lds(int nx, int ny, const float* __restrict__ c,
+                                                               const float* __restrict__ e,
+                                                               const float* __restrict__ nn,
+                                                               float* __restrict__ inv) {
+    constexpr int NM = 64;  // the largest coarsest level, and the LDS pitch
+    __shared__ double A[NM * NM];
+    __shared__ double rowk[2][NM];
     const int n = nx * ny;
     const int j = threadIdx.x, ty = threadIdx.y;
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    for (int i = ty; i < n; i += 4)
-        if (j < n) A[i * n + j] = 0.0;
+    const int tid = j + WAVE * ty;
+    for (int q = tid; q < NM * NM; q += WAVE * 16) A[q] = 0.0;
     __syncthreads();
     if (tid < n) {
         const int k = tid;
         const int ix = k % nx, iy = k / nx;
         double d = c[k];
-        if (ix < nx - 1) { d += e[k]; A[k * n + k + 1] = -(double)e[k]; }
-        if (ix > 0) { d += e[k - 1]; A[k * n + k - 1] = -(double)e[k - 1]; }
-        if (iy < ny - 1) { d += nn[k]; A[k * n + k + nx] = -(double)nn[k]; }
-        if (iy > 0) { d += nn[k - nx]; A[k * n + k - nx] = -(double)nn[k - nx]; }
-        A[k * n + k] = d;
+        if (ix < nx - 1) { d += e[k]; A[k * NM + k + 1] = -(double)e[k]; }
+        if (ix > 0) { d += e[k - 1]; A[k * NM + k - 1] = -(double)e[k - 1]; }
+        if (iy < ny - 1) { d += nn[k]; A[k * NM + k + nx] = -(double)nn[k]; }
+        if (iy > 0) { d += nn[k - nx]; A[k * NM + k - nx] = -(double)nn[k - nx]; }
+        A[k * NM + k] = d;
     }
     __syncthreads();
-    // the elimination on registers: thread (j, ty) keeps A[i][j] for i = ty + 4m;
-    // each step shares the pivot column and the pivot row through LDS
-    __shared__ double rowk[64];
-    double a[16];
+    double a[4];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int i = ty + 4 * m;
-        a[m] = (i < n && j < n) ? A[i * n + j] : 0.0;
+    for (int m = 0; m < 4; ++m) {
+        const int i = ty + 16 * m;
+        a[m] = (i < n && j < n) ? A[i * NM + j] : 0.0;
     }
     for (int k = 0; k < n; ++k) {
-        const int mk = k >> 2;  // the pivot row's register index in the threads with ty == k % 4
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            if (j == k && ty + 4 * m < n) colk[ty + 4 * m] = a[m];  // the pivot column before this step
-            if (m == mk && ty == (k & 3) && j < n) rowk[j] = a[m];  // the pivot row before this step
+        double* rb = rowk[k & 1];
+        if (ty == (k & 15)) {  // the pivot row's wave (uniform branch)
+            const int q = k >> 4;
+            rb[j] = q == 0 ? a[0] : (q == 1 ? a[1] : (q == 2 ? a[2] : a[3]));
         }
-        __syncthreads();
-        const double ip = 1.0 / colk[k];
-        const double akj = (j == k) ? ip : rowk[j] * ip;  // the scaled pivot row
+        __syncthreads();  // the other buffer is rewritten only after the next barrier
+        const double ip = 1.0 / rb[k];
+        const double akj = (j == k) ? ip : rb[j] * ip;  // the scaled pivot row
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int i = ty + 4 * m;
-            if (i < n && j < n) a[m] = (i == k) ? akj : ((j == k) ? -colk[i] * ip : a[m] - colk[i] * akj);
+        for (int m = 0; m < 4; ++m) {
+            const int i = ty + 16 * m;
+            const double cik = readlane_f64(a[m], k);  // A_ik before this step
+            if (i < n && j < n) a[m] = (i == k) ? akj : ((j == k) ? -cik * ip : a[m] - cik * akj);
         }
-        __syncthreads();  // colk and rowk are rewritten by the next step
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int i = ty + 4 * m;
+    for (int m = 0; m < 4; ++m) {
+        const int i = ty + 16 * m;
         if (i < n && j < n) inv[i * n + j] = (float)a[m];
     }
 }
@@ -3674,7 +3679,7 @@ void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), mg_tail_lds(cells, nc), st, T, sc);
 }
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, float* inv, hipStream_t st) {
-    hipLaunchKernelGGL(k_mg_dense_inverse_lds, dim3(1), dim3(64, 4), 0, st, nx, ny, c, e, n, inv);
+    hipLaunchKernelGGL(k_mg_dense_inverse_lds, dim3(1), dim3(WAVE, 16), 0, st, nx, ny, c, e, n, inv);
 }
 
 static inline unsigned nn_blocks(size_t n) {
