@@ -2598,6 +2598,17 @@ __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, flo
     }
 }
 
+// k / w for 0 <= k < 2^21 through the FP32 reciprocal of w: (k + 1/2) / w
+// lies at least 1/(2w) from every integer and the two roundings move it by a
+// relative 2^-23 at most, so truncation gives the quotient exactly (checked
+// for every k below 2^21 and every w up to 4096).
+struct FastDiv {
+    int w;
+    float rw;
+    __device__ explicit FastDiv(int w_) : w(w_), rw(1.0f / (float)w_) {}
+    __device__ int quot(int k) const { return (int)(((float)k + 0.5f) * rw); }
+};
+
 // The small levels of the V-cycle, from the first level of at most
 // MG_TAIL_CELLS cells down to the coarsest, in one workgroup: down-legs, the
 // dense coarsest solve, up-legs.  Everything lives in LDS (dynamic shared
@@ -2676,23 +2687,44 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         float* x = Xs + off[l];
         float* bc = Bs + off[l + 1];
         auto xs = [&](size_t k) -> float { return mg_div(omega * b[k], D[k]); };
+        const FastDiv dfx(fnx), dcx(cnx);
         if (T.rb) {
-            // red-black: every cell's x first (black from its red neighbours), then the residuals
-            for (int k = tid; k < fnx * fny; k += nt) {
-                const int ix = k % fnx, iy = k / fnx;
-                if (((ix + iy) & 1) == 0) {
+            if ((fnx & 1) == 0) {
+                // even width: each pass takes only its own colour's cells, hx per
+                // row (red ix = 2 j + (iy & 1), black the other); red x, then black
+                // from the stored red values
+                const int hx = fnx >> 1;
+                const FastDiv dh(hx);
+                for (int p = tid; p < hx * fny; p += nt) {
+                    const int iy = dh.quot(p), k = 2 * (p - iy * hx) + (iy & 1) + iy * fnx;
                     x[k] = xs(k);
-                } else {
+                }
+                __syncthreads();
+                for (int p = tid; p < hx * fny; p += nt) {
+                    const int iy = dh.quot(p), ix = 2 * (p - iy * hx) + ((iy + 1) & 1), k = ix + iy * fnx;
                     const bool hW = ix > 0, hE = ix < fnx - 1, hN = iy < fny - 1, hS = iy > 0;
-                    x[k] = rb_black0(b[k], D[k], omega, hW ? e[k - 1] : 0.f, hW ? xs(k - 1) : 0.f, hE ? e[k] : 0.f,
-                                     hE ? xs(k + 1) : 0.f, hN ? n[k] : 0.f, hN ? xs(k + fnx) : 0.f,
-                                     hS ? n[k - fnx] : 0.f, hS ? xs(k - fnx) : 0.f);
+                    x[k] = rb_black0(b[k], D[k], omega, hW ? e[k - 1] : 0.f, hW ? x[k - 1] : 0.f, hE ? e[k] : 0.f,
+                                     hE ? x[k + 1] : 0.f, hN ? n[k] : 0.f, hN ? x[k + fnx] : 0.f,
+                                     hS ? n[k - fnx] : 0.f, hS ? x[k - fnx] : 0.f);
+                }
+            } else {
+                // red-black: every cell's x (black from its red neighbours' values, recomputed)
+                for (int k = tid; k < fnx * fny; k += nt) {
+                    const int iy = dfx.quot(k), ix = k - iy * fnx;
+                    if (((ix + iy) & 1) == 0) {
+                        x[k] = xs(k);
+                    } else {
+                        const bool hW = ix > 0, hE = ix < fnx - 1, hN = iy < fny - 1, hS = iy > 0;
+                        x[k] = rb_black0(b[k], D[k], omega, hW ? e[k - 1] : 0.f, hW ? xs(k - 1) : 0.f,
+                                         hE ? e[k] : 0.f, hE ? xs(k + 1) : 0.f, hN ? n[k] : 0.f,
+                                         hN ? xs(k + fnx) : 0.f, hS ? n[k - fnx] : 0.f, hS ? xs(k - fnx) : 0.f);
+                    }
                 }
             }
             __syncthreads();
             auto xk = [&](size_t k) -> float { return x[k]; };
             for (int q = tid; q < cnx * cny; q += nt) {
-                const int I = q % cnx, J = q / cnx;
+                const int J = dcx.quot(q), I = q - J * cnx;
                 float acc = 0.f;
 #pragma unroll
                 for (int dy = 0; dy < 2; ++dy)
@@ -2710,7 +2742,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
             continue;
         }
         for (int q = tid; q < cnx * cny; q += nt) {
-            const int I = q % cnx, J = q / cnx;
+            const int J = dcx.quot(q), I = q - J * cnx;
             float acc = 0.f;
 #pragma unroll
             for (int dy = 0; dy < 2; ++dy)
@@ -2751,8 +2783,36 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         const float *c = Cs + off[l], *e = Es + off[l], *n = Ns + off[l], *D = Ds + off[l];
         float* x = Xs + off[l];
         const float* zc = Xs + off[l + 1];
+        const FastDiv dfx(fnx);
+        if (T.rb && (fnx & 1) == 0) {
+            // even width: u = x + scale P xc formed where it is read, black cells
+            // relaxing from u, then red cells (own u, relaxed black neighbours);
+            // each pass takes only its own colour's cells
+            const int hx = fnx >> 1;
+            const FastDiv dh(hx);
+            auto uat = [&](int k, int ix, int iy) -> float { return x[k] + scale * zc[(ix >> 1) + (iy >> 1) * cnx]; };
+            for (int p = tid; p < hx * fny; p += nt) {
+                const int iy = dh.quot(p), ix = 2 * (p - iy * hx) + ((iy + 1) & 1), k = ix + iy * fnx;
+                const bool hE = ix < fnx - 1, hW = ix > 0, hN = iy < fny - 1, hS = iy > 0;
+                x[k] = rb_relax(b[k], c[k], D[k], omega, uat(k, ix, iy), hE, hE ? e[k] : 0.f,
+                                hE ? uat(k + 1, ix + 1, iy) : 0.f, hW, hW ? e[k - 1] : 0.f,
+                                hW ? uat(k - 1, ix - 1, iy) : 0.f, hN, hN ? n[k] : 0.f,
+                                hN ? uat(k + fnx, ix, iy + 1) : 0.f, hS, hS ? n[k - fnx] : 0.f,
+                                hS ? uat(k - fnx, ix, iy - 1) : 0.f);
+            }
+            __syncthreads();
+            for (int p = tid; p < hx * fny; p += nt) {
+                const int iy = dh.quot(p), ix = 2 * (p - iy * hx) + (iy & 1), k = ix + iy * fnx;
+                const bool hE = ix < fnx - 1, hW = ix > 0, hN = iy < fny - 1, hS = iy > 0;
+                x[k] = rb_relax(b[k], c[k], D[k], omega, uat(k, ix, iy), hE, hE ? e[k] : 0.f, hE ? x[k + 1] : 0.f,
+                                hW, hW ? e[k - 1] : 0.f, hW ? x[k - 1] : 0.f, hN, hN ? n[k] : 0.f,
+                                hN ? x[k + fnx] : 0.f, hS, hS ? n[k - fnx] : 0.f, hS ? x[k - fnx] : 0.f);
+            }
+            __syncthreads();
+            continue;
+        }
         for (int k = tid; k < nf; k += nt) {
-            const int ix = k % fnx, iy = k / fnx;
+            const int iy = dfx.quot(k), ix = k - iy * fnx;
             x[k] = x[k] + scale * zc[(ix >> 1) + (iy >> 1) * cnx];
         }
         __syncthreads();
@@ -2762,7 +2822,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
             // then red cells from the relaxed black ones
             for (int colour = 1; colour >= 0; --colour) {
                 for (int k = tid; k < nf; k += nt) {
-                    const int ix = k % fnx, iy = k / fnx;
+                    const int iy = dfx.quot(k), ix = k - iy * fnx;
                     if (((ix + iy) & 1) != colour) continue;
                     const bool hE = ix < fnx - 1, hW = ix > 0, hN = iy < fny - 1, hS = iy > 0;
                     x[k] = rb_relax(b[k], c[k], D[k], omega, x[k], hE, hE ? e[k] : 0.f, hE ? x[k + 1] : 0.f, hW,
@@ -2778,7 +2838,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         for (int j = 0; j < 4; ++j) {
             const int k = tid + j * nt;
             if (k < nf) {
-                const int ix = k % fnx, iy = k / fnx;
+                const int iy = dfx.quot(k), ix = k - iy * fnx;
                 z[j] = x[k] + mg_div(omega * (b[k] - flux_apply(fnx, fny, ix, iy, (size_t)k, c, e, n, us)), D[k]);
             }
         }
