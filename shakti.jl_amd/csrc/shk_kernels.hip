@@ -2601,7 +2601,8 @@ __global__ __launch_bounds__(256) void k_mg_up_flat(int nx, int ny, int nxc, flo
 // k / w for 0 <= k < 2^21 through the FP32 reciprocal of w: (k + 1/2) / w
 // lies at least 1/(2w) from every integer and the two roundings move it by a
 // relative 2^-23 at most, so truncation gives the quotient exactly (checked
-// for every k below 2^21 and every w up to 4096).
+// on the CPU for every w up to 4096, k below 2^16, and for k below 2^21 over a
+// spread of widths).
 struct FastDiv {
     int w;
     float rw;
