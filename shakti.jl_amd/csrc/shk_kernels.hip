@@ -3140,13 +3140,23 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
     const int nv = (is_max || mode == FIN_NN_LS || mode == FIN_NN_LS_PC) ? 2 : 1;
     const int tid = threadIdx.x + threadIdx.y * WAVE;
     double acc[2] = {0.0, 0.0};
+    // each thread takes j = tid, tid + 256, ... in that order; the loads of 16
+    // consecutive terms are issued together (the order of the sum is unchanged)
+    constexpr int UF = 16, ST = WAVE * WAVES;
     for (int i = 0; i < nv; ++i) {
         double a = 0.0;
         bool first = true;
-        for (int j = tid; j < nb; j += WAVE * WAVES) {
-            const double b = partials[(size_t)i * nb + j];
-            if (first) { a = b; first = false; }
-            else a = is_max ? ((isnan(b) || b > a) ? b : a) : a + b;
+        for (int j0 = tid; j0 < nb; j0 += UF * ST) {
+            double v[UF];
+#pragma unroll
+            for (int u = 0; u < UF; ++u) v[u] = partials[(size_t)i * nb + min(j0 + u * ST, nb - 1)];
+#pragma unroll
+            for (int u = 0; u < UF; ++u) {
+                if (j0 + u * ST >= nb) break;
+                const double b = v[u];
+                if (first) { a = b; first = false; }
+                else a = is_max ? ((isnan(b) || b > a) ? b : a) : a + b;
+            }
         }
         acc[i] = a;
     }
