@@ -152,20 +152,35 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double* partials)
 // every workgroup of a consumer kernel: the same per-thread strides and the
 // same tree, so every workgroup gets k_finish's value bit for bit.  All
 // threads of the workgroup must call it; all receive the total.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int nb) {
-    __shared__ double total;
+// thread_sum: this thread's terms j = tid, tid + 256, ... summed in that order,
+// the loads of 16 consecutive terms issued together; block_total: the
+// workgroup's tree over those sums, returned to every thread.  Split so that a
+// kernel needing two totals issues both sets of loads before either tree.
+__device__ __forceinline__ double thread_sum(const double* __restrict__ p, int nb) {
+    constexpr int UF = 16, ST = WAVE * WAVES;
     const int tid = threadIdx.x + threadIdx.y * WAVE;
     double a = 0.0;
     bool first = true;
-    for (int j = tid; j < nb; j += WAVE * WAVES) {
-        const double b = p[j];
-        if (first) {
-            a = b;
-            first = false;
-        } else {
-            a = a + b;
+    for (int j0 = tid; j0 < nb; j0 += UF * ST) {
+        double v[UF];
+#pragma unroll
+        for (int u = 0; u < UF; ++u) v[u] = p[min(j0 + u * ST, nb - 1)];
+#pragma unroll
+        for (int u = 0; u < UF; ++u) {
+            if (j0 + u * ST >= nb) break;
+            if (first) {
+                a = v[u];
+                first = false;
+            } else {
+                a = a + v[u];
+            }
         }
     }
+    return a;
+}
+__device__ __forceinline__ double block_total(double a) {
+    __shared__ double total;
+    const int tid = threadIdx.x + threadIdx.y * WAVE;
     double one[1] = {a};
     block_reduce<1, false>(one);
     if (tid == 0) total = one[0];
@@ -173,6 +188,9 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int
     const double t = total;
     __syncthreads();  // a later call rewrites total
     return t;
+}
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int nb) {
+    return block_total(thread_sum(p, nb));
 }
 
 __device__ __forceinline__ bool lead_thread() {
@@ -1182,8 +1200,9 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
     virtual_block(vbx, vby);
     double al = sc->alpha;
     if constexpr (DEFER) {
-        const double pAp = sum_partials(pb, nbb);
-        const double g = sum_partials(pa, nba);
+        const double tb = thread_sum(pb, nbb), ta = thread_sum(pa, nba);
+        const double pAp = block_total(tb);
+        const double g = block_total(ta);
         const double pn2 = sc->pNorm2;
         const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
         al = g / pAp;
@@ -1227,8 +1246,9 @@ __global__ __launch_bounds__(256) void k_cg_update_flat(size_t n, const double* 
                                                         const double* __restrict__ pa, int nba,
                                                         const double* __restrict__ pb, int nbb) {
     if (sc->done) return;
-    const double pAp = sum_partials(pb, nbb);
-    const double g = sum_partials(pa, nba);
+    const double tb = thread_sum(pb, nbb), ta = thread_sum(pa, nba);
+    const double pAp = block_total(tb);
+    const double g = block_total(ta);
     const double pn2 = sc->pNorm2;
     const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
     const double alpha = g / pAp;
