@@ -37,14 +37,16 @@ namespace shk {
 
 // --------------------------------------------------------------- helpers
 
-__device__ __forceinline__ double jl_pow_int(double x, int n) {
-    // Julia's Base.:^(::Float64, ::Integer) (power by squaring with an
-    // error-compensated product, n == 3 as x*x*x); n == 2 reduces to x*x.
+// Julia's Base.:^(::Float64, ::Integer) (power by squaring with an
+// error-compensated product, n == 3 as x*x*x); n == 2 reduces to x*x.  The
+// body for n >= 0 is not recursive, so the compiler inlines it (a recursive
+// form is a real call, with its stack in scratch memory, in every kernel
+// that uses it).
+__device__ __forceinline__ double jl_pow_nonneg(double x, int n) {
     if (n == 2) return x * x;
     if (n == 3) return x * x * x;
     if (n == 1) return x;
     if (n == 0) return 1.0;
-    if (n < 0) return 1.0 / jl_pow_int(x, -n);
     double y = 1.0, xnlo = 0.0, ynlo = 0.0;
     while (n > 1) {
         if (n & 1) {
@@ -64,6 +66,9 @@ __device__ __forceinline__ double jl_pow_int(double x, int n) {
     double err = y * xnlo + x * ynlo;
     if (isfinite(x) && isfinite(err)) return fma(x, y, err);
     return x * y;
+}
+__device__ __forceinline__ double jl_pow_int(double x, int n) {
+    return n < 0 ? 1.0 / jl_pow_nonneg(x, -n) : jl_pow_nonneg(x, n);
 }
 
 __device__ __forceinline__ double cpow(double x, const Expo& e) { return e.is_int ? jl_pow_int(x, e.k) : pow(x, e.e); }
