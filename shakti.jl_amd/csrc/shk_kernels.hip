@@ -3336,9 +3336,10 @@ bool stream_nt(const Geom& G) {
 // 0.0772 s per step (the finest down-leg 22.0 -> 20.4 us, the matvec
 // 40.7 -> 39.8 us in the solve), although the matvec timed alone with the
 // caches flushed is slightly slower with the hints (4.55 against 4.69 TB/s).
+// (SHK_NO_NT is read at each call: a context created after it is set runs
+// without the hints.)
 static bool cg_stream_nt(const Geom& G) {
-    static const bool off = getenv("SHK_NO_NT") != nullptr;
-    return !off && (double)G.nx * G.ny * 72.0 > 256.0 * 1024 * 1024;
+    return getenv("SHK_NO_NT") == nullptr && (double)G.nx * G.ny * 72.0 > 256.0 * 1024 * 1024;
 }
 
 // Geometry of the fused update+Chebyshev kernel: strips of 64-2H owned
