@@ -205,11 +205,13 @@ int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double
  * reference point); 3 and 4 the same as 0 and 2 but cold: each launch is
  * timed alone after a 1 GiB read has replaced the contents of L2 and the
  * Infinity Cache (allocates that buffer for the duration of the call); 5 the
- * FP64 CG matvec kernel (p = z + beta p_old, Ap, p'Ap partials) on the solve's
- * work vectors, in the variant the V-cycle-preconditioned solve runs (z read
- * in FP32 where that solve keeps it so: 52 instead of 56 bytes per cell), cold
- * like 3 and 4: in that solve the V-cycle between two matvecs streams their
- * inputs out of the caches (the device scalars are restored afterwards). */
+ * FP64 CG matvec kernel (p = z + beta p_old, Ap, p'Ap partials, and the
+ * previous iteration's x += alpha p_old, here with alpha = 0 so x keeps its
+ * values) on the solve's work vectors, in the variant the
+ * V-cycle-preconditioned solve runs (z read in FP32 where that solve keeps it
+ * so: 68 instead of 72 bytes per cell), cold like 3 and 4: in that solve the
+ * V-cycle between two matvecs streams their inputs out of the caches (the
+ * device scalars are restored afterwards). */
 int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch);
 
 /* Per-kernel timing collected with HIP events on the context's stream while

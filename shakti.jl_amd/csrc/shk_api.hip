@@ -668,18 +668,20 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                 if (defer) {
                     int nbb;
                     {
-                        // reads aP, aE, aN, z (FP32 or FP64), p_old; writes p_new, Ap
-                        Prof pr(ctx, P_MV, (zf32 ? 6.5 : 7.0) * nb);
+                        // reads aP, aE, aN, z (FP32 or FP64), p_old, x; writes p_new, Ap, x
+                        // (x += alpha p_old: the previous iteration's step)
+                        Prof pr(ctx, P_MV, (zf32 ? 8.5 : 9.0) * nb);
                         nbb = zf32 ? launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, (const float*)zf, p_old, p_new,
                                                                 w.Ap, ctx->partials2, ctx->sc, ctx->partials, nba,
-                                                                ctx->st)
+                                                                w.x, ctx->st)
                                    : launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap,
-                                                                ctx->partials2, ctx->sc, ctx->partials, nba, ctx->st);
+                                                                ctx->partials2, ctx->sc, ctx->partials, nba, w.x,
+                                                                ctx->st);
                     }
                     {
-                        Prof pr(ctx, P_UPD, 6 * nb);  // reads x, p, r, Ap; writes x, r
-                        launch::cg_update_deferred(G, p_new, w.Ap, w.x, w.r, ctx->sc, ctx->partials, nba,
-                                                   ctx->partials2, nbb, ctx->st);
+                        Prof pr(ctx, P_UPD, 3 * nb);  // reads r, Ap; writes r
+                        launch::cg_update_deferred(G, w.Ap, w.r, ctx->sc, ctx->partials, nba, ctx->partials2, nbb,
+                                                   ctx->st);
                     }
                     mg_apply<T>(ctx, w.r, w.z, 2, zf);
                     T* t = p_old;
@@ -1486,10 +1488,11 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
             case 5:
                 if (z32)
                     launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, (const float*)ctx->z32, ctx->pA,
-                                               ctx->pB, ctx->Ap, ctx->partials2, ctx->sc, ctx->partials, 0, ctx->st);
+                                               ctx->pB, ctx->Ap, ctx->partials2, ctx->sc, ctx->partials, 0, ctx->x,
+                                               ctx->st);
                 else
                     launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
-                                               ctx->partials2, ctx->sc, ctx->partials, 0, ctx->st);
+                                               ctx->partials2, ctx->sc, ctx->partials, 0, ctx->x, ctx->st);
                 break;
             default: break;
         }
@@ -1497,7 +1500,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     if (kernel == 0 || kernel == 3) bytes = 5 * n * 8;
     else if (kernel == 1) bytes = 5 * n * 4;
     else if (kernel == 2 || kernel == 4) bytes = 2 * n * 8;
-    else if (kernel == 5) bytes = z32 ? 6.5 * n * 8 : 7 * n * 8;
+    else if (kernel == 5) bytes = z32 ? 8.5 * n * 8 : 9 * n * 8;
     else {
         hipEventDestroy(a);
         hipEventDestroy(b);
@@ -1512,6 +1515,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
         Scalars live = saved;
         live.done = 0;
         live.gpend = 0;
+        live.alpha = 0.0;  // x += 0 p: the traffic of the solve's matvec, x left as it is
         HIPCHK(hipMemcpy(ctx->sc, &live, sizeof(Scalars), hipMemcpyHostToDevice));
     }
     one();  // warm

@@ -161,15 +161,16 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
                double* partials, Scalars* sc, hipStream_t st);
 // FP64 CG with the finishing kernels folded into the next kernel: the matvec
 // sums the pending r'z partials (pa), writes p'Ap partials to pb and returns
-// their count; the update sums both.
+// their count, and takes the previous iteration's x += alpha p_old; the
+// update sums both and takes r -= alpha Ap.
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
-                       int nba, hipStream_t st);
+                       int nba, double* x, hipStream_t st);
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const float* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
-                       int nba, hipStream_t st);
-void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
-                        const double* pa, int nba, const double* pb, int nbb, hipStream_t st);
+                       int nba, double* x, hipStream_t st);
+void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
+                        const double* pb, int nbb, hipStream_t st);
 int mg_up_nblocks(int nx, int ny, int rb);
 template <class T>
 void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,

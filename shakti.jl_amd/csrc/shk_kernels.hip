@@ -956,6 +956,7 @@ __device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
         sc->tired = 0;
         sc->eps_stop = sc->atol + sc->rtol * ref;
         sc->pAp = 0.0;
+        sc->alpha = 0.0;  // the deferred matvec's first x += alpha p (p = 0) adds nothing
         sc->gpend = 0;
         sc->solved = (g == 0.0) || (rn <= sc->eps_stop);
         sc->done = sc->solved;
@@ -1131,23 +1132,35 @@ __device__ __forceinline__ bool deferred_gamma(Scalars* sc, const double* __rest
     return solved;
 }
 
+// DEFER: x (the accumulated correction) takes the previous iteration's
+// x += alpha p here, p being p_old, which this kernel reads anyway; the update
+// then reads and writes only r.  When the stop test says converged the
+// kernel applies that pending update on its tile and stops; a kernel that
+// finds done already set does nothing (the update was applied by the kernel
+// that set it, or, after a zero curvature, is not applied, as in Krylov).
 template <class T, bool DEFER>
 __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const T* __restrict__ aP,
                                                    const T* __restrict__ aE, const T* __restrict__ aN,
                                                    const T* __restrict__ z, const T* __restrict__ p_old,
                                                    T* __restrict__ p_new, T* __restrict__ Ap, double* partials,
-                                                   Scalars* sc, const double* __restrict__ pa, int nba) {
+                                                   Scalars* sc, const double* __restrict__ pa, int nba,
+                                                   T* __restrict__ x) {
     if (sc->done) return;
     int vbx, vby;
     virtual_block(vbx, vby);
     double b = sc->beta;
-    if (deferred_gamma<DEFER>(sc, pa, nba, b)) return;
+    const T alpha = (T)sc->alpha;  // the previous iteration's step (DEFER)
+    if (deferred_gamma<DEFER>(sc, pa, nba, b)) {
+        walk(nx, ny, R, vbx, vby, [&](size_t c) { x[c] = x[c] + alpha * p_old[c]; });
+        return;
+    }
     const T beta = (T)b;
     double acc = 0.0;
     march<T>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t j) -> T { return z[j] + beta * p_old[j]; },
              [&](size_t c, T Au, T u) {
                  p_new[c] = u;
                  Ap[c] = Au;
+                 if constexpr (DEFER) x[c] = x[c] + alpha * p_old[c];
                  acc += (double)u * (double)Au;
              });
     double v[1] = {acc};
@@ -1168,12 +1181,27 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
                                                     const ZT* __restrict__ z, const double* __restrict__ p_old,
                                                     double* __restrict__ p_new, double* __restrict__ Ap,
                                                     double* partials, Scalars* sc, const double* __restrict__ pa,
-                                                    int nba) {
+                                                    int nba, double* __restrict__ x) {
     if (sc->done) return;
     int strip, iy0;
     wide_tile(R, strip, iy0);
     double beta = sc->beta;
-    if (deferred_gamma<DEFER>(sc, pa, nba, beta)) return;
+    const double alpha = sc->alpha;  // the previous iteration's step (DEFER), as k_cg_matvec
+    auto x_step = [&](size_t c) {
+        d2v xv = ld2<NT>(x + c);
+        const d2v po = ld2<false>(p_old + c);
+        xv.x = xv.x + alpha * po.x;
+        xv.y = xv.y + alpha * po.y;
+        st2<NT>(x + c, xv);
+    };
+    if (deferred_gamma<DEFER>(sc, pa, nba, beta)) {
+        // march2's cells of this wave: columns ix, ix + 1 of rows iy0 .. iy1 - 1
+        const int ix = strip * 2 * WAVE + 2 * (int)threadIdx.x;
+        const int iy1 = min(iy0 + R, ny);
+        if (ix < nx)
+            for (int iy = iy0; iy < iy1; ++iy) x_step(cidx(ix, iy, nx));
+        return;
+    }
     double acc = 0.0;
     march2<NT>(
         nx, ny, R, strip, iy0, aP, aE, aN,
@@ -1182,6 +1210,7 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
         [&](size_t c, d2v Au, d2v u) {
             st2<NT>(p_new + c, u);
             st2<NT>(Ap + c, Au);
+            if constexpr (DEFER) x_step(c);
             acc += u.x * Au.x;
             acc += u.y * Au.y;
         });
@@ -1226,7 +1255,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
     const T alpha = (T)al;
     double acc = 0.0;
     walk(nx, ny, R, vbx, vby, [&](size_t c) {
-        x[c] = x[c] + alpha * p[c];
+        if constexpr (!DEFER) x[c] = x[c] + alpha * p[c];  // DEFER: in the next CG matvec
         const T rv = r[c] - alpha * Ap[c];
         r[c] = rv;
         if (jacobi) {
@@ -1245,8 +1274,8 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
 // lane (16 B accesses), grid-stride.  Pointwise work streams faster flat than
 // in the stencil kernels' strips: 36 against 42 us at 4096 x 1024 with the
 // vectors evicted from the caches.  (64, 4) workgroups, as sum_partials needs.
-__global__ __launch_bounds__(256) void k_cg_update_flat(size_t n, const double* __restrict__ p,
-                                                        const double* __restrict__ Ap, double* __restrict__ x,
+// r -= alpha Ap only: x += alpha p is taken by the next CG matvec.
+__global__ __launch_bounds__(256) void k_cg_update_flat(size_t n, const double* __restrict__ Ap,
                                                         double* __restrict__ r, Scalars* sc,
                                                         const double* __restrict__ pa, int nba,
                                                         const double* __restrict__ pb, int nbb) {
@@ -1271,16 +1300,12 @@ __global__ __launch_bounds__(256) void k_cg_update_flat(size_t n, const double* 
     const size_t tid = threadIdx.x + (size_t)threadIdx.y * WAVE;
     for (size_t i = ((size_t)blockIdx.x * (WAVE * WAVES) + tid) * 2; i < n; i += (size_t)gridDim.x * 2 * WAVE * WAVES) {
         if (i + 1 < n) {
-            const d2v pv = *(const d2v*)(p + i), av = *(const d2v*)(Ap + i);
-            d2v xv = *(const d2v*)(x + i), rv = *(const d2v*)(r + i);
-            xv.x = xv.x + alpha * pv.x;
-            xv.y = xv.y + alpha * pv.y;
+            const d2v av = *(const d2v*)(Ap + i);
+            d2v rv = *(const d2v*)(r + i);
             rv.x = rv.x - alpha * av.x;
             rv.y = rv.y - alpha * av.y;
-            *(d2v*)(x + i) = xv;
             *(d2v*)(r + i) = rv;
         } else {
-            x[i] = x[i] + alpha * p[i];
             r[i] = r[i] - alpha * Ap[i];
         }
     }
@@ -3482,16 +3507,16 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
             const Geom W = wide_geom(G);
             if (stream_nt(G))
                 hipLaunchKernelGGL((k_cg_matvec2<true, false>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
-                                   z, p_old, p_new, Ap, partials, sc, nullptr, 0);
+                                   z, p_old, p_new, Ap, partials, sc, nullptr, 0, nullptr);
             else
                 hipLaunchKernelGGL((k_cg_matvec2<false, false>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE,
-                                   aN, z, p_old, p_new, Ap, partials, sc, nullptr, 0);
+                                   aN, z, p_old, p_new, Ap, partials, sc, nullptr, 0, nullptr);
             finish(W, partials, FIN_PAP, 0, sc, st);
             return;
         }
     }
     hipLaunchKernelGGL((k_cg_matvec<T, false>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old, p_new,
-                       Ap, partials, sc, nullptr, 0);
+                       Ap, partials, sc, nullptr, 0, nullptr);
     finish(G, partials, FIN_PAP, 0, sc, st);
 }
 template <class T>
@@ -3503,42 +3528,42 @@ void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, 
 }
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
-                       int nba, hipStream_t st) {
+                       int nba, double* x, hipStream_t st) {
     if (cg_wide_ok(G)) {
         const Geom W = wide_geom(G);
         if (cg_stream_nt(G))
             hipLaunchKernelGGL((k_cg_matvec2<true, true>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
-                               p_old, p_new, Ap, pb, sc, pa, nba);
+                               p_old, p_new, Ap, pb, sc, pa, nba, x);
         else
             hipLaunchKernelGGL((k_cg_matvec2<false, true>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
-                               p_old, p_new, Ap, pb, sc, pa, nba);
+                               p_old, p_new, Ap, pb, sc, pa, nba, x);
         return W.nblocks;
     }
     hipLaunchKernelGGL((k_cg_matvec<double, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, z, p_old,
-                       p_new, Ap, pb, sc, pa, nba);
+                       p_new, Ap, pb, sc, pa, nba, x);
     return G.nblocks;
 }
 // The same with the V-cycle's FP32 z (cg_wide_ok grids only).
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const float* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
-                       int nba, hipStream_t st) {
+                       int nba, double* x, hipStream_t st) {
     const Geom W = wide_geom(G);
     if (cg_stream_nt(G))
         hipLaunchKernelGGL((k_cg_matvec2<true, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
-                           p_old, p_new, Ap, pb, sc, pa, nba);
+                           p_old, p_new, Ap, pb, sc, pa, nba, x);
     else
         hipLaunchKernelGGL((k_cg_matvec2<false, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
-                           z, p_old, p_new, Ap, pb, sc, pa, nba);
+                           z, p_old, p_new, Ap, pb, sc, pa, nba, x);
     return W.nblocks;
 }
-void cg_update_deferred(const Geom& G, const double* p, const double* Ap, double* x, double* r, Scalars* sc,
-                        const double* pa, int nba, const double* pb, int nbb, hipStream_t st) {
+void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
+                        const double* pb, int nbb, hipStream_t st) {
     // 2048 workgroups (one per 8 waves of the chip): each re-sums the partials
     // first, so more workgroups cost more than their better streaming gains
     // (in the solve at 4096 x 1024: 36.9 us at 2048, 38.9 at 4096, 44.2 at 8192)
     const size_t n = (size_t)G.nx * G.ny;
     const unsigned nblk = (unsigned)std::min<size_t>(2048, (n + 511) / 512);
-    hipLaunchKernelGGL(k_cg_update_flat, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, p, Ap, x, r, sc, pa, nba, pb, nbb);
+    hipLaunchKernelGGL(k_cg_update_flat, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, Ap, r, sc, pa, nba, pb, nbb);
 }
 template <class T>
 int cg_update_cheb(const Geom& G, int degree, const T* aP, const T* aE, const T* aN, const T* dg, const T* p,
