@@ -3558,11 +3558,11 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
 }
 void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
                         const double* pb, int nbb, hipStream_t st) {
-    // 2048 workgroups (one per 8 waves of the chip): each re-sums the partials
+    // 1024 workgroups (one per 4 waves of the chip): each re-sums the partials
     // first, so more workgroups cost more than their better streaming gains
-    // (in the solve at 4096 x 1024: 36.9 us at 2048, 38.9 at 4096, 44.2 at 8192)
+    // (in the solve at 4096 x 1024: 20.3 us at 1024, 20.7 at 2048, 23.1 at 4096)
     const size_t n = (size_t)G.nx * G.ny;
-    const unsigned nblk = (unsigned)std::min<size_t>(2048, (n + 511) / 512);
+    const unsigned nblk = (unsigned)std::min<size_t>(1024, (n + 511) / 512);
     hipLaunchKernelGGL(k_cg_update_flat, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, Ap, r, sc, pa, nba, pb, nbb);
 }
 template <class T>
