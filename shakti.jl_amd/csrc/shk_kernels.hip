@@ -1221,7 +1221,8 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
 // x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
 // DEFER: every workgroup sums p'Ap (pb, nbb) and r'z (pa, nba) as k_finish
 // would, applies the zero-curvature test and alpha = r'z / p'Ap (FIN_PAP);
-// workgroup 0 stores pAp, gamma, alpha (or zero_curv and done).
+// workgroup 0 stores pAp, gamma, alpha (or zero_curv and done); x += alpha p
+// is left to the next CG matvec (k_cg_matvec with DEFER).
 template <class T, bool DEFER>
 __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const T* __restrict__ dg,
                                                    const T* __restrict__ p, const T* __restrict__ Ap,
