@@ -480,8 +480,11 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 // zf: where given (FP64 CG with the deferred finish, BF16 finest level), the
 // result goes there in FP32, the precision the V-cycle computes it in; the
 // CG matvec widens it as it loads it, to the values an FP64 z would hold.
+// upd: where given (the same solve, when mg_down_fuses_update), the finest
+// down-leg also takes the CG update, reading r and writing r - alpha Ap to
+// upd->b_out, which the rest of the V-cycle reads as its right-hand side.
 template <class T>
-void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
+void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, const launch::CGUpd* upd = nullptr) {
     auto& mg = ctx->mg;
     const float scale = ctx->system_parabolic ? ctx->mg_scale_par : ctx->mg_scale;
     const size_t L = mg.size() - 1;
@@ -493,8 +496,17 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
         const auto& f = mg[l];
         // reads b, c, aE, aN, D; writes x and the coarse rhs
         const double cells = (double)f.nx * f.ny, coarse = (double)mg[l + 1].nx * mg[l + 1].ny;
+        // (with upd also Ap in and r out, FP64)
         Prof pr(ctx, l == 0 ? P_MGD0 : P_MGD,
-                l == 0 ? (rb0 + cb0 + 4.0) * cells + 4.0 * coarse : 4.0 * (6.0 * cells + coarse));
+                l == 0 ? (rb0 + cb0 + 4.0 + (upd ? 16.0 : 0.0)) * cells + 4.0 * coarse
+                       : 4.0 * (6.0 * cells + coarse));
+        if constexpr (sizeof(T) == 8) {
+            if (l == 0 && upd) {
+                launch::mg_down_bf16_upd(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
+                                         ctx->mg_D16, r, f.xa, mg[1].b, *upd, ctx->st);
+                continue;
+            }
+        }
         if (l == 0 && ctx->mg_bf16)
             launch::mg_down_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
                                     ctx->mg_D16, r, f.xa, mg[1].b, ctx->sc, ctx->st);
@@ -523,6 +535,9 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
         Prof pr(ctx, P_MGC, bytes);
         launch::mg_tail(Tl, ctx->sc, ctx->st);
     }
+    const T* r0 = r;  // the finest level's right-hand side on the way up
+    if constexpr (sizeof(T) == 8)
+        if (upd) r0 = upd->b_out;
     for (size_t l = first; l-- > 0;) {
         const auto& f = mg[l];
         // reads b, c, aE, aN, D, x and the coarse correction; writes the result
@@ -531,11 +546,11 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr) {
                 l == 0 ? (rb0 + cb0 + 4.0 + zb0) * cells + 4.0 * coarse : 4.0 * (7.0 * cells + coarse));
         if (l == 0 && ctx->mg_bf16 && zf)
             launch::mg_up_bf16<T, float>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
-                                         ctx->mg_n16, ctx->mg_D16, r, f.xa, mg[1].xb, zf, mode, ctx->partials,
+                                         ctx->mg_n16, ctx->mg_D16, r0, f.xa, mg[1].xb, zf, mode, ctx->partials,
                                          ctx->sc, ctx->st);
         else if (l == 0 && ctx->mg_bf16)
             launch::mg_up_bf16<T, T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
-                                     ctx->mg_n16, ctx->mg_D16, r, f.xa, mg[1].xb, z, mode, ctx->partials, ctx->sc,
+                                     ctx->mg_n16, ctx->mg_D16, r0, f.xa, mg[1].xb, z, mode, ctx->partials, ctx->sc,
                                      ctx->st);
         else if (l == 0)
             launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, f.c, f.e, f.n, f.D, r, f.xa,
@@ -657,6 +672,10 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     // FP64 CG with the V-cycle (no learned preconditioner): the two finishing
     // kernels of an iteration are folded into the matvec and the update
     const bool defer = sizeof(T) == 8 && multigrid && !nn && ctx->use_defer;
+    // ... with the CG update fused into the finest down-leg where that leg can
+    // take it (r then alternates between w.r and w.r2; batches are even)
+    const bool fuse_upd = defer && zf32 && launch::mg_down_fuses_update(ctx->mg[0].nx, ctx->mg[0].ny) &&
+                          getenv("SHK_NO_FUSED_UPDATE") == nullptr;
     const int nba = multigrid ? launch::mg_up_nblocks(ctx->mg[0].nx, ctx->mg[0].ny, ctx->mg_rb) : 0;
     auto enqueue_batch = [&]() {
         T* p_old = w.pA;
@@ -678,12 +697,21 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                                                                 ctx->partials2, ctx->sc, ctx->partials, nba, w.x,
                                                                 ctx->st);
                     }
-                    {
-                        Prof pr(ctx, P_UPD, 3 * nb);  // reads r, Ap; writes r
-                        launch::cg_update_deferred(G, w.Ap, w.r, ctx->sc, ctx->partials, nba, ctx->partials2, nbb,
-                                                   ctx->st);
+                    if (fuse_upd) {
+                        // the update in the finest down-leg: r_cur in, r_cur - alpha Ap to r_nxt
+                        const launch::CGUpd U{w.Ap, r_nxt, ctx->sc, ctx->partials, nba, ctx->partials2, nbb};
+                        mg_apply<T>(ctx, r_cur, w.z, 2, zf, &U);
+                        T* tr = r_cur;
+                        r_cur = r_nxt;
+                        r_nxt = tr;
+                    } else {
+                        {
+                            Prof pr(ctx, P_UPD, 3 * nb);  // reads r, Ap; writes r
+                            launch::cg_update_deferred(G, w.Ap, w.r, ctx->sc, ctx->partials, nba, ctx->partials2,
+                                                       nbb, ctx->st);
+                        }
+                        mg_apply<T>(ctx, w.r, w.z, 2, zf);
                     }
-                    mg_apply<T>(ctx, w.r, w.z, 2, zf);
                     T* t = p_old;
                     p_old = p_new;
                     p_new = t;
@@ -738,7 +766,8 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         // the V-cycle's coarse scale (a kernel argument) differs between
         // elliptic and parabolic systems: part of the key
         const int key[4] = {(int)sizeof(T),
-                            precond + (learned_batch ? 16 : 0) + (defer ? 32 : 0) + (ctx->system_parabolic ? 64 : 0),
+                            precond + (learned_batch ? 16 : 0) + (defer ? 32 : 0) + (ctx->system_parabolic ? 64 : 0) +
+                                (fuse_upd ? 128 : 0),
                             degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
             if (!memcmp(e.key, key, sizeof(key)) && (!learned_batch || e.net == ctx->net)) *out = e.exec;

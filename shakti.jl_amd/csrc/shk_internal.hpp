@@ -202,6 +202,23 @@ void to_bf16x4(size_t n, const float* a0, const float* a1, const float* a2, cons
 template <class TB>
 void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
                   const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
+// The CG update r_out = r - alpha Ap fused into the finest down-leg (FP64 CG
+// with the deferred finish): alpha from the pending p'Ap (pb, nbb) and r'z
+// (pa, nba) partials, stored to sc by the kernel as the update would.
+struct CGUpd {
+    const double* Ap = nullptr;
+    double* b_out = nullptr;
+    Scalars* sc = nullptr;
+    const double* pa = nullptr;
+    int nba = 0;
+    const double* pb = nullptr;
+    int nbb = 0;
+};
+// Whether the finest level of an nx x ny grid takes the leg that can fuse the
+// update (the BF16 marching leg at more than two rows per wave).
+bool mg_down_fuses_update(int nx, int ny);
+void mg_down_bf16_upd(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
+                      const uint16_t* D, const double* b, float* x, float* bc, const CGUpd& U, hipStream_t st);
 template <class TB, class TO>
 void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
                 const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TO* out,

@@ -2012,15 +2012,52 @@ constexpr int RB_OWN = 2 * (WAVE - 2);
 // indices clamped into the grid, the values of rows outside it zeroed when
 // unpacked) are issued before step t's arithmetic, so their latency overlaps
 // it.
-template <class TB, class CT>
+// UPD (FP64 CG with the deferred finish, TB = double): the CG update is
+// fused in.  Every workgroup first takes k_cg_update_flat's prologue (the
+// p'Ap and r'z partials, the zero-curvature test, alpha; workgroup 0 stores
+// them), then the leg uses b - alpha Ap wherever it reads b and stores it to
+// U.b_out for its own cells (a separate buffer: neighbouring tiles read the
+// old b of the rows and columns they share).
+template <class TB, class CT, bool UPD = false>
 __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, int R, float omega,
                                                      const CT* __restrict__ c, const CT* __restrict__ e,
                                                      const CT* __restrict__ n, const CT* __restrict__ D,
                                                      const TB* __restrict__ b, float* __restrict__ x,
-                                                     float* __restrict__ bc, const Scalars* __restrict__ sc) {
+                                                     float* __restrict__ bc, const Scalars* __restrict__ sc,
+                                                     launch::CGUpd U) {
     typedef typename Pair<TB>::type TB2;
     typedef typename RawPair<CT>::type CR;
+    static_assert(!UPD || sizeof(TB) == 8, "the fused update is FP64");
     if (sc->done) return;
+    double alpha = 0.0;
+    if constexpr (UPD) {
+        const double tb = thread_sum(U.pb, U.nbb), ta = thread_sum(U.pa, U.nba);
+        const double pAp = block_total(tb);
+        const double g = block_total(ta);
+        const double pn2 = sc->pNorm2;
+        const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
+        alpha = g / pAp;
+        if (lead_thread()) {
+            U.sc->pAp = pAp;
+            U.sc->gamma = g;
+            if (zc) {
+                U.sc->zero_curv = 1;
+                U.sc->done = 1;
+            } else {
+                U.sc->alpha = alpha;
+            }
+        }
+        if (zc) return;
+    }
+    // b at cell pair k, with the fused update applied (r - alpha Ap, as the update computes it)
+    auto upd = [&](TB2 v, size_t k) -> TB2 {
+        if constexpr (UPD) {
+            const d2v a = *(const d2v*)(U.Ap + k);
+            v.x = v.x - alpha * a.x;
+            v.y = v.y - alpha * a.y;
+        }
+        return v;
+    };
     int strip, iy0;
     wide_tile(R, strip, iy0);  // iy0 even
     const int lane = threadIdx.x;
@@ -2037,7 +2074,7 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         dj = z2;
         if (j < 0 || j >= ny) return r;
         const size_t k = cidx(ixc, j, nx);
-        const TB2 bv = *(const TB2*)(b + k);
+        const TB2 bv = upd(*(const TB2*)(b + k), k);
         bj = f2v{(float)bv.x, (float)bv.y};  // b is used in FP32 only
         dj = ldc2(D, k);
         if ((j & 1) == 0) r.x = mg_div(omega * bj.x, dj.x);
@@ -2068,11 +2105,13 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
     // step t's loads: b, D of row t+2; aE, aN of row t+1; c of row t
     struct Ld {
         TB2 b;
+        d2v ap;  // UPD: Ap of the same row
         CR d, e, n, c;
     };
     auto issue = [&](int t, Ld& q) {
         const size_t k2 = rowk(t + 2), k1 = rowk(t + 1);
         q.b = *(const TB2*)(b + k2);
+        if constexpr (UPD) q.ap = *(const d2v*)(U.Ap + k2);
         q.d = ldraw2(D, k2);
         q.e = ldraw2(e, k1);
         q.n = ldraw2(n, k1);
@@ -2083,7 +2122,13 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         // red values of row t+2 (t+2 >= 0), as red_row
         f2v b2 = z2, d2 = z2, r2 = z2;
         if (t + 2 < ny) {
-            b2 = f2v{(float)q.b.x, (float)q.b.y};
+            TB2 bb = q.b;
+            if constexpr (UPD) {
+                bb.x = bb.x - alpha * q.ap.x;
+                bb.y = bb.y - alpha * q.ap.y;
+                if (own && t + 2 >= iy0 && t + 2 < iy1) *(TB2*)(U.b_out + cidx(ixc, t + 2, nx)) = bb;
+            }
+            b2 = f2v{(float)bb.x, (float)bb.y};
             d2 = unpack2(q.d);
             if (((t + 2) & 1) == 0) r2.x = mg_div(omega * b2.x, d2.x);
             else r2.y = mg_div(omega * b2.y, d2.y);
@@ -3666,7 +3711,14 @@ void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const
                            x, bc, sc);
     else
         hipLaunchKernelGGL((k_mg_down_rb2<TB, uint16_t>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n,
-                           D, b, x, bc, sc);
+                           D, b, x, bc, sc, CGUpd{});
+}
+bool mg_down_fuses_update(int nx, int ny) { return mg_rb_geom(nx, ny).R > 2; }
+void mg_down_bf16_upd(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
+                      const uint16_t* D, const double* b, float* x, float* bc, const CGUpd& U, hipStream_t st) {
+    const Geom W = mg_rb_geom(nx, ny);  // R > 2 (mg_down_fuses_update)
+    hipLaunchKernelGGL((k_mg_down_rb2<double, uint16_t, true>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c,
+                       e, n, D, b, x, bc, U.sc, U);
 }
 template <class TB, class TO>
 void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
@@ -3734,7 +3786,7 @@ void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, cons
                                    bc, sc);
             else
                 hipLaunchKernelGGL((k_mg_down_rb2<TB, float>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n, D, b,
-                                   x, bc, sc);
+                                   x, bc, sc, CGUpd{});
         }
         return;
     }
