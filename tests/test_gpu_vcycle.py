@@ -151,3 +151,27 @@ def test_non_temporal_cg_matvec_gives_the_same_iterates(hip, monkeypatch):
     print(f"CG iterations: non-temporal {out[0][0]}, cached {out[1][0]}", flush=True)
     assert out[0][0] == out[1][0] > 0
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_fused_cg_update_gives_the_same_iterates(hip, monkeypatch):
+    """FP64 CG with the V-cycle fuses r -= alpha Ap into the finest down-leg,
+    which then writes the updated r to a second buffer (rows and columns shared
+    with neighbouring tiles are read before any tile writes).  On a grid whose
+    last tile is ragged (2052 x 1023: four rows per wave, three in the last
+    tile, odd ny) the iterates are those of the separate update kernel
+    (SHK_NO_FUSED_UPDATE=1), bit for bit."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(2052, 1023, bed_amp=50.0, seed=7)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    out = []
+    for separate in (False, True):
+        if separate:
+            monkeypatch.setenv("SHK_NO_FUSED_UPDATE", "1")
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=1e-10)
+        hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+        out.append((ls.last_stats.niter, hs.get("h")))
+        monkeypatch.delenv("SHK_NO_FUSED_UPDATE", raising=False)
+    print(f"CG iterations: fused update {out[0][0]}, separate {out[1][0]}", flush=True)
+    assert out[0][0] == out[1][0] > 0
+    assert np.array_equal(out[0][1], out[1][1])
