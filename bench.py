@@ -7,27 +7,35 @@ SHMIP-like surface; OCEAN column at x = 0), harmonic K_face, implicit gap
 scheme, regularised-Coulomb sliding (C = 0.5), seasonal melt (1e-8 m/s
 baseline plus a 1e-7 m/s one-day sinusoid) plus 20 seeded moulins of
 1-10 m^3/s, dt = 3600 s; each timestep runs the Picard loop to the 1e-6
-relative tolerance with Chebyshev(4)-preconditioned CG (every solve to a
-relative preconditioned residual of 1e-11), then the gap step.
+relative tolerance, then the gap step.
 
-With ``--workload parabolic`` the run is the parabolic head scheme instead:
-8192 x 2048 cells, bed amplitude 100 m plus a seeded 2 km-wide trough,
-e_v = 1e-3, dt = 600 s, one backward-Euler solve per timestep.
-
-``--forms`` picks the solver forms run from the same initial state (plain
-FP64 Chebyshev(4)-PCG; FP32 inner solves with FP64 refinement; FP64 CG with
-the FP32 multigrid V-cycle; the same warm-started by the U-Net); ``value`` is
-the fastest form's time and every other form is reported beside it with its
-per-field error against the FP64 form.
+``--forms`` picks the solver forms run from the same initial state:
+  fp64        plain FP64 Chebyshev(4)-PCG (the reference's solver, lambda_max
+              from the Gershgorin bound), each solve to a relative
+              preconditioned residual of 1e-13: the FP64 trajectory every
+              other form is compared with;
+  fp32_mg     FP32 V-cycle-preconditioned CG inner solves inside FP64
+              iterative refinement;
+  fp64_mg     FP64 CG preconditioned by the FP32/BF16 geometric-multigrid
+              V-cycle;
+  learned_mg  fp64_mg warm-started by the BF16 U-Net.
+``value`` is the fastest form's median seconds per timestep; every form is
+reported beside it with its per-field error against the fp64 trajectory,
+given only when every fp64 solve met its stopping test.
 
 A step is one whole timestep.  W untimed warm-up steps (the first solves the
-initial-condition transient), then K timed steps bracketed by device
-synchronisation.  The JSON line reports seconds per timestep, the roofline of
-the dominant kernel (algorithmic bytes per launch / mean launch time from HIP
-events on the library's stream), the operator apply's bandwidth, and the CPU
-baseline: the FP64 restatement of the same algorithm timed on the host's
-cores on a bounded sample and scaled by the GPU run's per-step CG and Picard
-counts.
+initial-condition transient), then K timed steps, each bracketed by device
+synchronisation.  The JSON line also carries the roofline of the dominant
+kernel (algorithmic bytes per launch / mean launch time from HIP events on
+the library's stream), the operator apply's bandwidth at this size and cold
+at 8192 x 2048 (beyond the Infinity Cache), a short parabolic 8192 x 2048 run,
+and the CPU baseline: the oracle (FP64 C restatement of the same algorithm)
+timed on the host's cores on a bounded sample, scaled by the GPU run's
+per-step CG and Picard counts, with one whole C1 step timed beside it.
+
+With ``--workload parabolic`` the timed run is the parabolic head scheme
+instead: 8192 x 2048 cells, bed amplitude 100 m plus a seeded 2 km-wide
+trough, e_v = 1e-3, dt = 600 s, one backward-Euler solve per timestep.
 """
 from __future__ import annotations
 
@@ -35,6 +43,7 @@ import argparse
 import json
 import os
 import pathlib
+import statistics
 import sys
 import time
 
@@ -55,73 +64,126 @@ def make_inputs(nx, ny, bed_amp, seed, trough=0.0, trough_depth=0.0):
     return g, raw, moulin
 
 
-class _ParabolicCounts:
-    """PicardSolver-like view of a parabolic run: one linear solve per step."""
+def form_description(form, a):
+    return {
+        "fp64": f"plain FP64 Chebyshev(4)-PCG (Gershgorin lambda_max), rtol {a.ref_rtol} in the Chebyshev norm",
+        "fp32_mg": (f"FP32 V-cycle-PCG inner solves to {a.mg_inner_rtol} inside FP64 iterative refinement to "
+                    f"{a.ir_rtol} in the Jacobi-weighted FP64 residual norm"),
+        "fp64_mg": (f"FP64 CG preconditioned by the FP32/BF16 geometric-multigrid V-cycle, rtol {a.mg_rtol} in "
+                    "the V-cycle norm"),
+        "learned_mg": f"fp64_mg (rtol {a.mg_rtol}) warm-started by the BF16 U-Net",
+        "fp32_refine": (f"FP32 Chebyshev(4)-PCG inner solves to {a.inner_rtol} inside FP64 iterative refinement "
+                        f"to {a.ir_rtol}"),
+    }[form]
 
-    def __init__(self, ls):
-        self.ls = ls
-        self.converged = True
-        self.last_iter = 1
 
-    @property
-    def cg_iters(self):
-        return self.ls.ws.stats.niter
+def solver_kwargs(form, a, g):
+    if form == "fp64":
+        return dict(chebyshev_degree=4, rtol=a.ref_rtol, form="fp64")
+    if form == "fp64_mg":
+        return dict(multigrid=True, rtol=a.mg_rtol, form="fp64")
+    if form == "learned_mg":
+        from shakti_hip import learned as LN
+        H8, W8 = LN.canvas_shape(g.nx, g.ny)
+        net = LN.UNet.load(a.weights, 1, H8, W8)
+        return dict(multigrid=True, rtol=a.mg_rtol, form="learned", network=net)
+    if form == "fp32_refine":
+        return dict(chebyshev_degree=4, rtol=a.ir_rtol, form="fp32_refine", inner_rtol=a.inner_rtol)
+    return dict(multigrid=True, rtol=a.ir_rtol, form="fp32_refine", inner_rtol=a.mg_inner_rtol)
 
 
-def gpu_run(args, g, raw, moulin, form):
+class Health:
+    """Per-solve statistics of a run (the library's solve log): did every
+    linear solve meet its stopping test, and how far is each returned head
+    from solving its own system in FP64."""
+
+    def __init__(self):
+        self.solves = 0
+        self.unsolved = 0
+        self.zero_curvature = 0
+        self.tired = 0
+        self.breakdown = 0
+        self.bounds_warning = 0
+        self.few_iters = 0
+        self.max_true_residual = 0.0
+        self.niter = []
+        self.lam = [float("inf"), 0.0]
+
+    def add(self, entries):
+        for e in entries:
+            self.solves += 1
+            self.unsolved += int(not e["solved"])
+            self.zero_curvature += int(e["zero_curvature"])
+            self.tired += int(e["tired"])
+            self.breakdown += int(e["breakdown"] != 0)
+            self.bounds_warning += int(e["bounds_warning"])
+            self.few_iters += int(e["niter"] < 2 and e["rnorm0"] > 0)
+            self.max_true_residual = max(self.max_true_residual, e["true_residual"])
+            self.niter.append(e["niter"])
+            self.lam = [min(self.lam[0], e["lambda_min"]), max(self.lam[1], e["lambda_max"])]
+
+    def ok(self):
+        return self.solves > 0 and not (self.zero_curvature or self.tired or self.breakdown)
+
+    def summary(self):
+        return {"solves": self.solves, "unsolved": self.unsolved, "zero_curvature": self.zero_curvature,
+                "tired": self.tired, "breakdown": self.breakdown, "bounds_warning": self.bounds_warning,
+                "under_2_iterations": self.few_iters, "max_true_residual": self.max_true_residual,
+                "cg_iters_per_solve_min": min(self.niter) if self.niter else None,
+                "cg_iters_per_solve_max": max(self.niter) if self.niter else None,
+                "lambda_range": self.lam if self.lam[1] > 0 else None}
+
+
+def gpu_run(args, g, raw, moulin, form, parabolic, steps, warmup, profile=True):
     import shakti_hip as H
     from shakti_hip import synthetic as S
-    parabolic = args.workload == "parabolic"
     p = H.ModelParameters(e_v=args.e_v) if parabolic else H.ModelParameters()
     sl = H.RegularizedCoulombSlidingLaw(0.5)
     s = H.State(g, p)
     H.set_initial_conditions(s, g, p, sl, **raw)
-    # fp64: the reference's Chebyshev(4)-PCG in FP64; fp32_refine: FP32 Chebyshev(4)-PCG inside FP64
-    # refinement; fp32_mg: FP32 multigrid-PCG inside FP64 refinement; fp64_mg: FP64 CG preconditioned
-    # by the FP32 multigrid V-cycle; learned_mg: fp64_mg from the U-Net's warm start
-    if form == "fp64":
-        kw = dict(chebyshev_degree=4, rtol=args.rtol, form="fp64")
-    elif form == "fp64_mg":
-        kw = dict(multigrid=True, rtol=args.mg_rtol, form="fp64")
-    elif form == "learned_mg":
-        # fp64_mg warm-started by the trained U-Net (its own forward kernels, BF16 on MFMA)
-        from shakti_hip import learned as LN
-        H8, W8 = LN.canvas_shape(g.nx, g.ny)
-        net = LN.UNet.load(args.weights, 1, H8, W8)
-        kw = dict(multigrid=True, rtol=args.mg_rtol, form="learned", network=net)
-    elif form == "fp32_refine":
-        kw = dict(chebyshev_degree=4, rtol=args.ir_rtol, form="fp32_refine", inner_rtol=args.inner_rtol)
-    else:
-        kw = dict(multigrid=True, rtol=args.ir_rtol, form="fp32_refine", inner_rtol=args.mg_inner_rtol)
-    ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, atol=0.0, check_every=args.check_every, **kw)
+    ls = H.CGIterativeSolver(g, H.HIPMatrixFreeLinearSystem, atol=0.0, check_every=args.check_every,
+                             **solver_kwargs(form, args, g))
     mi = H.SeasonalMoulinMeltInput(moulin, **S.SEASONAL_C2)
+    dt = args.par_dt if parabolic else args.dt
     if parabolic:
         # ParabolicHeadScheme: one backward-Euler solve per step (src/parabolic_solver.jl:16-38)
-        ps = _ParabolicCounts(ls)
-        sim = H.Simulation(g, s, args.warmup + args.steps, args.dt, p, "implicit", [], mi, sl, ls=ls,
-                           k_face_choice="harmonic")
+        sim = H.Simulation(g, s, warmup + steps, dt, p, "implicit", [], mi, sl, ls=ls, k_face_choice="harmonic")
+        ps = None
     else:
         ps = H.PicardSolver(args.max_picard, 1e-6, ls, g)
-        sim = H.Simulation(g, s, args.warmup + args.steps, args.dt, p, "implicit", [], mi, sl, ps=ps,
-                           k_face_choice="harmonic")
-    for w in range(args.warmup):
-        t0 = time.perf_counter()
-        H.step(sim)
-        sim.total_time += sim.dt
-        log(f"[bench {form}] warm-up step {w + 1}/{args.warmup}: {time.perf_counter() - t0:.3f} s, "
-            f"Picard {ps.last_iter} (converged {ps.converged}), CG {ps.cg_iters}")
-    times, picard, cg, conv = [], [], [], []
-    for k in range(args.steps):
+        sim = H.Simulation(g, s, warmup + steps, dt, p, "implicit", [], mi, sl, ps=ps, k_face_choice="harmonic")
+    health = Health()
+    s.solve_log()  # empty it
+
+    def one(tag):
         t0 = time.perf_counter()
         H.step(sim)  # returns after the library's stream has synchronised
-        dt = time.perf_counter() - t0
+        el = time.perf_counter() - t0
         sim.total_time += sim.dt
-        times.append(dt)
-        picard.append(ps.last_iter)
-        cg.append(ps.cg_iters)
-        conv.append(ps.converged)
-        log(f"[bench {form}] step {k + 1}/{args.steps}: {dt:.3f} s, Picard {ps.last_iter} "
-            f"(converged {ps.converged}), CG {ps.cg_iters}")
+        entries = s.solve_log()
+        health.add(entries)
+        conv, last = (True, 1) if parabolic else (ps.converged, ps.last_iter)
+        cg = sum(e["niter"] for e in entries)
+        tr = max((e["true_residual"] for e in entries), default=float("nan"))
+        flags = sum(int(e["zero_curvature"]) + int(e["tired"]) for e in entries)
+        log(f"[bench {form}] {tag}: {el:.3f} s, Picard {last} (converged {conv}), CG {cg} "
+            f"({min(e['niter'] for e in entries)}..{max(e['niter'] for e in entries)} per solve), "
+            f"max ||b-Ah||/||b|| {tr:.2e}{', SHORT SOLVES ' + str(flags) if flags else ''}")
+        return el, last, cg, conv
+
+    for w in range(warmup):
+        one(f"warm-up step {w + 1}/{warmup}")
+    times, picard, cg, conv = [], [], [], []
+    for k in range(steps):
+        el, last, c, cv = one(f"step {k + 1}/{steps}")
+        times.append(el)
+        picard.append(last)
+        cg.append(c)
+        conv.append(cv)
+    fields = {f: s.get(f) for f in ("h", "b", "N", "q_x", "q_y", "mdot", "K")}
+    out = dict(times=times, picard=picard, cg=cg, conv=conv, fields=fields, health=health, state=s, sim=sim)
+    if not profile:
+        return out
     # Per-kernel durations: one more step with HIP events bracketing every
     # launch on the library stream (CG batches are then launched one by one
     # instead of as a captured graph, so this step is not part of the timing).
@@ -129,167 +191,190 @@ def gpu_run(args, g, raw, moulin, form):
     s.profile_enable(None)
     t0 = time.perf_counter()
     H.step(sim)
-    prof_step = time.perf_counter() - t0
+    out["prof_step"] = time.perf_counter() - t0
     sim.total_time += sim.dt
-    prof = s.profile_read()
+    out["prof"] = s.profile_read()
     s.profile_enable([])
-    log(f"[bench {form}] profiled step: {prof_step:.3f} s, Picard {ps.last_iter}, CG {ps.cg_iters}")
-    # operator apply alone, back-to-back launches on the library stream
+    entries = s.solve_log()
+    out["prof_cg"] = sum(e["niter"] for e in entries)
+    log(f"[bench {form}] profiled step: {out['prof_step']:.3f} s, CG {out['prof_cg']}")
+    out["operator"] = time_operator(s)
+    return out
+
+
+def time_operator(s, cg_matvec=True):
+    """The FP64 operator apply alone (back-to-back and cold), the plain copy
+    beside it, and the CG matvec as the V-cycle-preconditioned solve runs it."""
     import ctypes as C
     from shakti_hip import _lib as L
-    mean_ms, nbytes = C.c_double(), C.c_double()
-    L.check(L.load().shk_time_kernel(s.ctx, 0, 50, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
-    op = {"kernel": "operator apply (FP64, symmetric face layout)", "mean_ms": mean_ms.value,
-          "bytes_per_launch": nbytes.value, "achieved_GBs": nbytes.value / (mean_ms.value * 1e-3) / 1e9}
+    lib = L.load()
+    ms, by = C.c_double(), C.c_double()
+
+    def t(kernel, reps):
+        L.check(lib.shk_time_kernel(s.ctx, kernel, reps, C.byref(ms), C.byref(by)), s.ctx, "time_kernel")
+        return ms.value, by.value
+
+    m0, b0 = t(0, 50)
+    op = {"kernel": "operator apply (FP64, symmetric face layout: aP, aE, aN, x in, y out; 40 B/cell)",
+          "mean_ms": m0, "bytes_per_launch": b0, "achieved_GBs": b0 / (m0 * 1e-3) / 1e9}
     op["frac"] = op["achieved_GBs"] / HBM_PEAK_GBS
-    L.check(L.load().shk_time_kernel(s.ctx, 2, 50, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
-    op["copy_reference_GBs"] = nbytes.value / (mean_ms.value * 1e-3) / 1e9
+    m2, b2 = t(2, 50)
+    op["copy_reference_GBs"] = b2 / (m2 * 1e-3) / 1e9
     # cold: each launch alone after a 1 GiB read has flushed L2 and the Infinity Cache
-    L.check(L.load().shk_time_kernel(s.ctx, 3, 10, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
-    op["cold_mean_ms"] = mean_ms.value
-    op["cold_GBs"] = nbytes.value / (mean_ms.value * 1e-3) / 1e9
+    m3, b3 = t(3, 10)
+    op["cold_mean_ms"] = m3
+    op["cold_GBs"] = b3 / (m3 * 1e-3) / 1e9
     op["cold_frac"] = op["cold_GBs"] / HBM_PEAK_GBS
-    L.check(L.load().shk_time_kernel(s.ctx, 4, 10, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
-    op["cold_copy_reference_GBs"] = nbytes.value / (mean_ms.value * 1e-3) / 1e9
-    # the FP64 CG matvec kernel as the solve runs it: each launch timed alone
-    # after a 1 GiB read, since in the solve the V-cycle between two matvecs
-    # streams their inputs out of L2 and the Infinity Cache
-    L.check(L.load().shk_time_kernel(s.ctx, 5, 20, C.byref(mean_ms), C.byref(nbytes)), s.ctx, "time_kernel")
-    op["cg_matvec_mean_ms"] = mean_ms.value
-    op["cg_matvec_bytes_per_launch"] = nbytes.value
-    fields = {f: s.get(f) for f in ("h", "b", "N", "q_x", "q_y", "mdot", "K")}
-    return dict(times=times, picard=picard, cg=cg, conv=conv, prof=prof, prof_step=prof_step,
-                prof_cg=ps.cg_iters, operator=op, fields=fields, state=s)
+    m4, b4 = t(4, 10)
+    op["cold_copy_reference_GBs"] = b4 / (m4 * 1e-3) / 1e9
+    if cg_matvec:
+        # the FP64 CG matvec as the solve runs it: each launch timed alone
+        # after a 1 GiB read, since in the solve the V-cycle between two
+        # matvecs streams their inputs out of L2 and the Infinity Cache
+        m5, b5 = t(5, 20)
+        op["cg_matvec_mean_ms"] = m5
+        op["cg_matvec_bytes_per_launch"] = b5
+    return op
 
 
-def cpu_baseline(args, g, raw, moulin, picard_per_step, cg_per_step):
+def field_errors(ref, fields):
+    import numpy as np
+    errs = {}
+    for f, a in fields.items():
+        scale = float(np.max(np.abs(ref[f])))
+        errs[f] = float(np.max(np.abs(a - ref[f]))) / scale if scale > 0 else float(np.max(np.abs(a)))
+    return errs
+
+
+def compare_forms(runs, K):
+    summary = {}
+    for form, rr in runs.items():
+        summary[form] = {"s_per_timestep": round(statistics.median(rr["times"]), 4),
+                         "s_per_timestep_mean": round(sum(rr["times"]) / K, 4),
+                         "picard_iters_per_step": sum(rr["picard"]) / K,
+                         "cg_iters_per_step": sum(rr["cg"]) / K,
+                         "picard_converged_all": all(rr["conv"]),
+                         "solves": rr["health"].summary()}
+    if "fp64" in runs:
+        ref_ok = runs["fp64"]["health"].ok()
+        if not ref_ok:
+            log("[bench] WARNING: a solve of the fp64 reference trajectory stopped short; per-field errors "
+                "against it are reported as unreliable")
+        ref = runs["fp64"]["fields"]
+        for form, rr in runs.items():
+            if form == "fp64":
+                continue
+            key = "max_rel_err_vs_fp64" if ref_ok else "max_rel_err_vs_fp64_unreliable"
+            summary[form][key] = field_errors(ref, rr["fields"])
+            summary[form]["speedup_vs_fp64"] = round(summary["fp64"]["s_per_timestep"] /
+                                                     summary[form]["s_per_timestep"], 3)
+    return summary
+
+
+def streamed_and_parabolic(args):
+    """The 8192 x 2048 parabolic workload: the operator apply cold (beyond the
+    Infinity Cache), and a few steps of the fastest form and the plain FP64
+    form with per-field errors."""
+    import shakti_hip as H
+    g, raw, moulin = make_inputs(8192, 2048, 100.0, args.seed, trough=2000.0, trough_depth=args.trough_depth)
+    runs = {}
+    for form in ("fp64", "fp32_mg", "fp64_mg"):
+        runs[form] = gpu_run(args, g, raw, moulin, form, True, args.c5_steps, 2, profile=False)
+        if form == "fp64":
+            # the operator of the last assembly, at 8192 x 2048 (672 MB per apply)
+            op = time_operator(runs[form]["state"], cg_matvec=False)
+            op["grid"] = "8192x2048"
+        runs[form]["state"] = None
+        runs[form]["sim"] = None
+    summ = compare_forms(runs, args.c5_steps)
+    best = min((f for f in summ if f != "fp64"), key=lambda f: summ[f]["s_per_timestep"])
+    return op, {"workload": (f"8192x2048 parabolic scheme, e_v {args.e_v}, dt {args.par_dt} s, bed amplitude "
+                             f"100 m + 2 km trough {args.trough_depth} m deep, harmonic K_face, seasonal + 20 "
+                             f"moulins; {args.c5_steps} timed steps after 2"),
+                "fastest": best, "forms": summ}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, g, raw, moulin, picard_per_step, cg_per_step, parabolic):
     """The oracle (FP64 C restatement, OpenMP) on the same inputs: one solve
     capped at 1 CG iteration, one at 1 + S, and one derived refresh, timed on
-    the host; scaled by the GPU run's per-step Picard and CG counts."""
+    the host; scaled by the GPU run's per-step Picard and CG counts.  Beside
+    it, one whole step of the 64 x 32 configuration (Cholesky), fully timed."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
+    from shakti_hip import synthetic as S
     O.build()
     og = O.Grid(g.nx, g.ny, g.lx, g.ly)
-    par = args.workload == "parabolic"
-    p = O.ModelParameters(e_v=args.e_v) if par else O.ModelParameters()
+    p = O.ModelParameters(e_v=args.e_v) if parabolic else O.ModelParameters()
     m = O.Model(kfs=1, sliding_C=0.5)
     s = O.State(og)
     O.set_initial_conditions(s, og, p, m, **raw)
     s.ieb[...] = s.ieb + moulin
-    S = args.cpu_iters
-    sv1 = O.Solver(og, "cg", precond="chebyshev", degree=4, atol=0.0, rtol=1e-30, itmax=1)
+    n_s = args.cpu_iters
     h0 = s.h.copy()
 
     def solve(sv):
-        if par:
-            sv.solve_parabolic(s, p, 1, args.dt)
+        if parabolic:
+            sv.solve_parabolic(s, p, 1, args.par_dt)
         else:
             sv.solve_elliptic(s, p, 1)
 
+    sv1 = O.Solver(og, "cg", precond="chebyshev", degree=4, atol=0.0, rtol=1e-30, itmax=1, gershgorin=True)
     t0 = time.perf_counter()
     solve(sv1)
     t1 = time.perf_counter() - t0
     s.h[...] = h0
-    svS = O.Solver(og, "cg", precond="chebyshev", degree=4, atol=0.0, rtol=1e-30, itmax=S + 1)
+    svS = O.Solver(og, "cg", precond="chebyshev", degree=4, atol=0.0, rtol=1e-30, itmax=n_s + 1, gershgorin=True)
     t0 = time.perf_counter()
     solve(svS)
     tS = time.perf_counter() - t0
     t0 = time.perf_counter()
     O.update_derived(s, og, p, m)
     td = time.perf_counter() - t0
-    per_cg = (tS - t1) / S
+    per_cg = (tS - t1) / n_s
     per_picard = (t1 - per_cg) + td
     est = cg_per_step * per_cg + picard_per_step * per_picard
-    sample = (f"oracle on {g.nx}x{g.ny}: 1 solve capped at 1 CG iteration ({t1:.2f} s), 1 capped at {S + 1} "
+    # one whole step of the 64 x 32 configuration: Cholesky, Picard 500 / 1e-6
+    gg, raw1 = S.glacier(64, 32, bed_amp=10.0, seed=1)
+    g1 = O.Grid(64, 32, gg.lx, gg.ly)
+    p1 = O.ModelParameters()
+    m1 = O.Model(sliding_C=0.5)
+    s1 = O.State(g1)
+    O.set_initial_conditions(s1, g1, p1, m1, **raw1)
+    sv = O.Solver(g1, "cholesky")
+    O.step(sv, s1, g1, p1, m1, 3600.0)  # the initial transient
+    t0 = time.perf_counter()
+    conv, last, _ = O.step(sv, s1, g1, p1, m1, 3600.0)
+    c1 = time.perf_counter() - t0
+    unit = "solve" if parabolic else "Picard iteration"
+    sample = (f"oracle on {g.nx}x{g.ny}: 1 solve capped at 1 CG iteration ({t1:.2f} s), 1 capped at {n_s + 1} "
               f"({tS:.2f} s), 1 derived refresh ({td:.2f} s); {per_cg * 1e3:.1f} ms per Chebyshev(4)-PCG "
-              f"iteration, {per_picard:.2f} s per {'solve' if par else 'Picard iteration'} outside CG; scaled "
-              f"by the GPU run's {picard_per_step:.1f} {'solves' if par else 'Picard iterations'} and "
-              f"{cg_per_step:.0f} CG iterations per step")
-    return {"value": est, "unit": "s/timestep", "cores": O.num_threads(), "kind": "port", "sample": sample}
+              f"iteration, {per_picard:.2f} s per {unit} outside CG; scaled by the GPU fp64 run's "
+              f"{picard_per_step:.1f} {unit}s and {cg_per_step:.0f} CG iterations per step")
+    return {"value": est, "unit": "s/timestep", "cores": O.num_threads(), "kind": "port", "sample": sample,
+            "cpu_model": cpu_model(),
+            "full_step_c1": {"seconds": c1, "picard_iters": last, "converged": conv,
+                             "what": "one whole step (second of the run) of the 64x32 configuration, oracle "
+                                     "Cholesky, fully timed on the same cores"}}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["elliptic", "parabolic"], default="elliptic",
-                    help="elliptic: Picard to 1e-6 per step (4096x1024 defaults); parabolic: one backward-Euler "
-                         "solve per step, e_v 1e-3, dt 600 s, 8192x2048, bed amplitude 100 m and a 2 km trough")
-    ap.add_argument("--nx", type=int, default=None)
-    ap.add_argument("--ny", type=int, default=None)
-    ap.add_argument("--bed-amp", type=float, default=None)
-    ap.add_argument("--dt", type=float, default=None)
-    ap.add_argument("--e-v", type=float, default=1e-3)
-    ap.add_argument("--trough-depth", type=float, default=200.0,
-                    help="parabolic workload: depth (m) of the seeded 2 km-wide trough")
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--rtol", type=float, default=1e-11)
-    ap.add_argument("--check-every", type=int, default=32)
-    ap.add_argument("--forms", default="fp64,fp32_mg,fp64_mg,learned_mg",
-                    help="comma-separated solver forms to run from the same initial state")
-    ap.add_argument("--ir-rtol", type=float, default=1e-8,
-                    help="fp32_refine: FP64 residual reduction per linear solve")
-    ap.add_argument("--inner-rtol", type=float, default=1e-3,
-                    help="fp32_refine: residual reduction of each FP32 inner solve")
-    ap.add_argument("--mg-inner-rtol", type=float, default=1e-4,
-                    help="fp32_mg: residual reduction of each FP32 multigrid-PCG inner solve")
-    ap.add_argument("--mg-rtol", type=float, default=1e-10,
-                    help="fp64_mg: reduction of the multigrid-preconditioned residual norm per linear solve")
-    ap.add_argument("--weights", default=str(ROOT / "shakti.jl_amd" / "shakti_hip" / "weights" /
-                                             "unet_16_32_64_128.npz"),
-                    help="learned_mg: trained U-Net parameters")
-    ap.add_argument("--cpu-iters", type=int, default=30)
-    ap.add_argument("--max-picard", type=int, default=500,
-                    help="Picard iteration cap per step (500 as in the reference configs; lower only for profiling)")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    par = args.workload == "parabolic"
-    args.nx = args.nx or (8192 if par else 4096)
-    args.ny = args.ny or (2048 if par else 1024)
-    args.bed_amp = args.bed_amp if args.bed_amp is not None else (100.0 if par else 50.0)
-    args.dt = args.dt or (600.0 if par else 3600.0)
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the head solve is not sharded across GPUs "
-              "(see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def kernel_table(prof, prof_step):
+    return {k: {"launches": v["count"], "mean_ms": v["ms"] / v["count"],
+                "GBs": (v["bytes"] / v["count"]) / (v["ms"] / v["count"] * 1e-3) / 1e9 if v["bytes"] else None,
+                "share_of_profiled_step": v["ms"] * 1e-3 / prof_step} for k, v in prof.items()}
 
-    log(f"[bench] building inputs {args.nx}x{args.ny}, bed amplitude {args.bed_amp} m, seed {args.seed}")
-    g, raw, moulin = make_inputs(args.nx, args.ny, args.bed_amp, args.seed, trough=2000.0 if par else 0.0,
-                                 trough_depth=args.trough_depth if par else 0.0)
-    forms = args.forms.split(",")
-    runs = {}
-    for form in forms:
-        runs[form] = gpu_run(args, g, raw, moulin, form)
-        runs[form]["state"] = None
-    K = args.steps
-    form_summary = {}
-    for form, rr in runs.items():
-        form_summary[form] = {"s_per_timestep": round(sum(rr["times"]) / K, 4),
-                              "picard_iters_per_step": sum(rr["picard"]) / K,
-                              "cg_iters_per_step": sum(rr["cg"]) / K,
-                              "picard_converged_all": all(rr["conv"])}
-    if "fp64" in runs:
-        import numpy as np
-        ref = runs["fp64"]["fields"]
-        for form, rr in runs.items():
-            if form == "fp64":
-                continue
-            errs = {}
-            for f, a in rr["fields"].items():
-                scale = float(np.max(np.abs(ref[f])))
-                errs[f] = float(np.max(np.abs(a - ref[f]))) / scale if scale > 0 else float(np.max(np.abs(a)))
-            form_summary[form]["max_rel_err_vs_fp64"] = errs
-            form_summary[form]["speedup_vs_fp64"] = round(form_summary["fp64"]["s_per_timestep"] /
-                                                          form_summary[form]["s_per_timestep"], 3)
-    best = min(runs, key=lambda f: form_summary[f]["s_per_timestep"])
-    r = runs[best]
-    tot = sum(r["times"])
-    s_per_step = tot / K
-    picard_per_step = sum(r["picard"]) / K
-    cg_per_step = sum(r["cg"]) / K
+
+def roofline(args, best, r):
     prof = r["prof"]
-    # dominant kernel: largest total time among the profiled CG kernels
     dom_name = max((k for k in prof if prof[k]["bytes"] > 0), key=lambda k: prof[k]["ms"])
     d = prof[dom_name]
     event_mean_ms = d["ms"] / d["count"]  # each launch bracketed alone by events (includes its dispatch)
@@ -317,22 +402,100 @@ def main():
                 rocprof_ms = t[key].get("mean_us_rocprof", 0.0) * 1e-3 or None
         except Exception:
             traffic = None
-    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms, "timing": how,
-                "event_mean_ms_profiled_step": event_mean_ms, "rocprof_mean_ms": rocprof_ms,
-                "launches": d["count"]}
+    return {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_ms": mean_ms, "timing": how,
+            "event_mean_ms_profiled_step": event_mean_ms, "rocprof_mean_ms": rocprof_ms, "launches": d["count"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["elliptic", "parabolic"], default="elliptic",
+                    help="elliptic: Picard to 1e-6 per step (4096x1024 defaults); parabolic: one backward-Euler "
+                         "solve per step, e_v 1e-3, dt 600 s, 8192x2048, bed amplitude 100 m and a 2 km trough")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--bed-amp", type=float, default=None)
+    ap.add_argument("--dt", type=float, default=3600.0)
+    ap.add_argument("--par-dt", type=float, default=600.0)
+    ap.add_argument("--e-v", type=float, default=1e-3)
+    ap.add_argument("--trough-depth", type=float, default=200.0,
+                    help="parabolic workload: depth (m) of the seeded 2 km-wide trough")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ref-rtol", type=float, default=1e-13,
+                    help="fp64: relative preconditioned residual of every solve of the FP64 reference trajectory")
+    ap.add_argument("--check-every", type=int, default=32)
+    ap.add_argument("--forms", default="fp64,fp32_mg,fp64_mg,learned_mg",
+                    help="comma-separated solver forms to run from the same initial state")
+    ap.add_argument("--ir-rtol", type=float, default=1e-8,
+                    help="fp32_mg / fp32_refine: FP64 residual reduction per linear solve")
+    ap.add_argument("--inner-rtol", type=float, default=1e-3,
+                    help="fp32_refine: residual reduction of each FP32 inner solve")
+    ap.add_argument("--mg-inner-rtol", type=float, default=1e-4,
+                    help="fp32_mg: residual reduction of each FP32 multigrid-PCG inner solve")
+    ap.add_argument("--mg-rtol", type=float, default=1e-10,
+                    help="fp64_mg: reduction of the multigrid-preconditioned residual norm per linear solve")
+    ap.add_argument("--weights", default=str(ROOT / "shakti.jl_amd" / "shakti_hip" / "weights" /
+                                             "unet_16_32_64_128.npz"),
+                    help="learned_mg: trained U-Net parameters")
+    ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--c5-steps", type=int, default=5,
+                    help="timed steps of the 8192x2048 parabolic summary (0: skip it and the streamed operator)")
+    ap.add_argument("--max-picard", type=int, default=500,
+                    help="Picard iteration cap per step (500 as in the reference configs; lower only for profiling)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    par = args.workload == "parabolic"
+    args.nx = args.nx or (8192 if par else 4096)
+    args.ny = args.ny or (2048 if par else 1024)
+    args.bed_amp = args.bed_amp if args.bed_amp is not None else (100.0 if par else 50.0)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the head solve is not sharded across GPUs "
+              "(see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"[bench] building inputs {args.nx}x{args.ny}, bed amplitude {args.bed_amp} m, seed {args.seed}")
+    g, raw, moulin = make_inputs(args.nx, args.ny, args.bed_amp, args.seed, trough=2000.0 if par else 0.0,
+                                 trough_depth=args.trough_depth if par else 0.0)
+    forms = args.forms.split(",")
+    runs = {}
+    for form in forms:
+        runs[form] = gpu_run(args, g, raw, moulin, form, par, args.steps, args.warmup)
+        runs[form]["state"] = None
+        runs[form]["sim"] = None
+    K = args.steps
+    form_summary = compare_forms(runs, K)
+    for form in forms:
+        form_summary[form]["solver"] = form_description(form, args)
+    best = min(runs, key=lambda f: form_summary[f]["s_per_timestep"])
+    r = runs[best]
+    s_per_step = statistics.median(r["times"])
+    picard_per_step = sum(r["picard"]) / K
+    cg_per_step = sum(r["cg"]) / K
+    roof = roofline(args, best, r)
+    streamed, c5 = None, None
+    if not par and args.c5_steps > 0:
+        log("[bench] 8192x2048: streamed operator and a short parabolic run ...")
+        streamed, c5 = streamed_and_parabolic(args)
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle restatement) on a bounded sample ...")
         try:
             base = runs.get("fp64", r)  # the CPU restatement runs the plain FP64 algorithm
-            cpu = cpu_baseline(args, g, raw, moulin, sum(base["picard"]) / K, sum(base["cg"]) / K)
+            cpu = cpu_baseline(args, g, raw, moulin, sum(base["picard"]) / K, sum(base["cg"]) / K, par)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] CPU baseline failed: {e!r}")
-    kernels = {k: {"launches": v["count"], "mean_ms": v["ms"] / v["count"],
-                   "GBs": (v["bytes"] / v["count"]) / (v["ms"] / v["count"] * 1e-3) / 1e9 if v["bytes"] else None,
-                   "share_of_profiled_step": v["ms"] * 1e-3 / r["prof_step"]} for k, v in prof.items()}
+    desc = form_description(best, args)
+    if par:
+        workload = (f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m + 2 km trough "
+                    f"{args.trough_depth} m deep, parabolic scheme e_v {args.e_v}, one backward-Euler solve per "
+                    f"step, harmonic K_face, implicit gap, seasonal + 20 moulins, dt {args.par_dt} s; {desc}")
+    else:
+        workload = (f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m, harmonic K_face, "
+                    f"Picard tol 1e-6, implicit gap, seasonal + 20 moulins, dt {args.dt} s; {desc}")
     out = {
         "metric": "s per timestep to 1e-6 Picard tol & HBM GB/s: mixed/learned vs plain FP64 PCG",
         "value": round(s_per_step, 4),
@@ -346,25 +509,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m, harmonic "
-                                f"K_face, Chebyshev(4)-PCG rtol {args.rtol}, Picard tol 1e-6, implicit gap, "
-                                "seasonal + 20 moulins, dt 3600 s") if not par else
-                               (f"{args.nx}x{args.ny} synthetic glacier, bed amplitude {args.bed_amp} m + 2 km trough "
-                                f"{args.trough_depth} m deep, parabolic scheme e_v {args.e_v}, one backward-Euler "
-                                f"solve per step, harmonic K_face, implicit gap, seasonal + 20 moulins, "
-                                f"dt {args.dt} s"),
-                   "form": best, "nx": args.nx, "ny": args.ny, "cells": args.nx * args.ny},
+        "config": {"workload": workload, "form": best, "nx": args.nx, "ny": args.ny, "cells": args.nx * args.ny,
+                   "statistic": "median over the timed steps"},
         "forms": form_summary,
         "timesteps_per_s": round(1.0 / s_per_step, 4),
         "picard_iters_per_step": picard_per_step,
         "picard_iters_per_s": round(picard_per_step / s_per_step, 3),
         "cg_iters_per_step": cg_per_step,
         "picard_converged_all": all(all(rr["conv"]) for rr in runs.values()),
-        "roofline": roofline,
+        "roofline": roof,
         "profiled_step": {"seconds": r["prof_step"], "cg_iters": r["prof_cg"],
                           "note": "one extra step after the timed ones, every launch bracketed by HIP events"},
         "operator": r["operator"],
-        "kernels": kernels,
+        "operator_streamed": streamed,
+        "parabolic_8192x2048": c5,
+        "kernels": kernel_table(r["prof"], r["prof_step"]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

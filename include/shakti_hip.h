@@ -31,12 +31,14 @@
 extern "C" {
 #endif
 
-#define SHK_ABI_VERSION 1
+#define SHK_ABI_VERSION 2
 
 #define SHK_OK 0
 #define SHK_ERR_ARG (-1)      /* bad argument, shape or field id */
 #define SHK_ERR_HIP (-2)      /* a HIP runtime call failed */
-#define SHK_ERR_NUMERIC (-3)  /* NaN/Inf or breakdown detected */
+#define SHK_ERR_NUMERIC (-3)  /* breakdown: r'z < 0 or not finite (preconditioner or operator not positive
+                                 definite; Krylov's sqrt(r'z) throws there), p'Ap or the returned head's
+                                 residual not finite */
 #define SHK_ERR_STATE (-4)    /* call out of order (e.g. solve before init) */
 #define SHK_ERR_NODEVICE (-5) /* no usable gfx950 device */
 
@@ -74,6 +76,24 @@ typedef struct {
 #define SHK_FORM_FP32_REFINE 1  /* FP32 inner PCG inside FP64 iterative refinement */
 #define SHK_FORM_LEARNED 2      /* FP64 PCG warm-started by the attached U-Net (shakti_learned.h) */
 
+/* Chebyshev bounds.  REFERENCE: 0.9 x the smallest and 1.1 x the largest Ritz
+ * value of the nsteps-step estimate with Euclidean dots on D^-1 A, adopted
+ * when finite with 0 < min < max, else the previous bounds
+ * (src/preconditioner.jl:89-134, 197-233).  Those Ritz values are not
+ * bounded by the spectrum of the nonsymmetric D^-1 A: on strongly varying
+ * transmissivity the estimate can be off by many orders of magnitude, and
+ * the shrunken polynomial then trips Krylov's zero-curvature test.
+ * GERSHGORIN (default): the same estimate run in the D-inner product (the
+ * coefficients of Jacobi-preconditioned CG, a symmetric Lanczos process, so
+ * its Ritz values lie in the spectrum) gives lambda_min = 0.9 x the smallest;
+ * lambda_max = the Gershgorin bound of D^-1 A, 1 + max_i sum_j |a_ij| / a_ii
+ * (at most 2 for these diagonally dominant systems).  The reference's
+ * degree-k polynomial is positive exactly below lambda_min + lambda_max, so
+ * with this lambda_max the preconditioner is positive definite.  A deviation
+ * from the reference, which it follows otherwise. */
+#define SHK_CHEB_BOUNDS_GERSHGORIN 0
+#define SHK_CHEB_BOUNDS_REFERENCE 1
+
 /* Linear-solver configuration -- reference CGIterativeSolver(g,
  * MatrixFreeLinearSystem; chebyshev_degree, chebyshev_nsteps_estimate)
  * (src/linear_solver.jl:832-843) plus Krylov's stopping controls. */
@@ -86,6 +106,8 @@ typedef struct {
     int32_t check_every;      /* CG iterations queued between host convergence checks (>= 1) */
     double atol, rtol;        /* stop when sqrt(r'M^-1 r) <= atol + rtol*initial; Krylov default sqrt(eps) each */
     double inner_rtol;        /* FP32_REFINE: relative reduction asked of each inner solve */
+    int32_t cheb_bounds;      /* SHK_CHEB_BOUNDS_* (ABI 2) */
+    int32_t reserved;         /* 0 */
 } shk_solver_cfg;
 
 /* Statistics of the last linear solve (reference ws.stats, ls.precond bounds). */
@@ -99,6 +121,18 @@ typedef struct {
                                from h); bit 1 when the learned preconditioner stalled (plain PCG after) */
     double rnorm0, rnorm;   /* preconditioned residual norm at start / end */
     double lambda_min, lambda_max; /* Chebyshev bounds in use */
+    /* ABI 2 */
+    int32_t tired;          /* 1 when itmax was reached before the stopping test (FP32_REFINE: 64 refinements;
+                               a refinement loop that stalls at the FP64 floor, three refinements without
+                               a 10% gain, ends with solved = 0 and tired = 0) */
+    int32_t breakdown;      /* 0; 1 when r'z < 0 or not finite, 2 when p'Ap is not finite (the solve then
+                               returned SHK_ERR_NUMERIC and left h unchanged) */
+    double breakdown_value; /* the offending r'z or p'Ap */
+    double lz_lambda_min, lz_lambda_max; /* Chebyshev: this solve's estimate, 0.9 x / 1.1 x the extreme Ritz
+                                            values, whether adopted or not (NaN otherwise) */
+    double gershgorin;      /* Chebyshev with SHK_CHEB_BOUNDS_GERSHGORIN: the bound of D^-1 A (NaN otherwise) */
+    double true_residual;   /* ||b - A h||_2 / ||b||_2 of the head the solve returned, in FP64 (before any
+                               under-relaxation) */
 } shk_solve_stats;
 
 /* State field ids, in the order of src/state.jl:13-57. */
@@ -159,9 +193,20 @@ int shk_update_derived(shk_ctx* ctx);
 int shk_picard_iteration(shk_ctx* ctx, double relax_alpha, shk_solve_stats* stats);
 
 /* Picard_loop! (src/elliptic_solver.jl:138-176): converged / last_iter as the
- * reference's ps.converged / ps.last_iter; cg_iters accumulates CG iterations. */
+ * reference's ps.converged / ps.last_iter; cg_iters accumulates CG iterations.
+ * An iteration whose linear solve stopped short (itmax or zero curvature:
+ * stats tired / zero_curvature) does not count as converged, whatever the
+ * head change (a deviation: the reference tests the head change alone).  A
+ * breakdown returns SHK_ERR_NUMERIC. */
 int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every, double relax_alpha,
                     int32_t* converged, int32_t* last_iter, int64_t* cg_iters);
+
+/* Statistics of every linear solve since the last call (oldest first), for
+ * the solves inside shk_picard_loop / shk_step as well.  Copies at most
+ * max_entries (the most recent ones when there are more), sets *n_entries and
+ * empties the log, which keeps the last 65536 solves.  No reference
+ * counterpart beyond ls.ws.stats of the last solve. */
+int shk_solve_log(shk_ctx* ctx, shk_solve_stats* out, int32_t max_entries, int32_t* n_entries);
 
 /* parabolic_solver! (src/parabolic_solver.jl:16-38). */
 int shk_parabolic_step(shk_ctx* ctx, double dt, int64_t* cg_iters);

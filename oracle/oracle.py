@@ -61,13 +61,14 @@ class _State(C.Structure):
 class _CgCfg(C.Structure):
     _fields_ = [("precond", C.c_int32), ("cheb_degree", C.c_int32), ("cheb_nsteps", C.c_int32),
                 ("itmax", C.c_int32), ("atol", C.c_double), ("rtol", C.c_double), ("lambda_min", C.c_double),
-                ("lambda_max", C.c_double)]
+                ("lambda_max", C.c_double), ("gershgorin", C.c_int32), ("pad", C.c_int32)]
 
 
 class CgStats(C.Structure):
     _fields_ = [("niter", C.c_int32), ("solved", C.c_int32), ("zero_curvature", C.c_int32),
                 ("bounds_warning", C.c_int32), ("rnorm0", C.c_double), ("rnorm", C.c_double),
-                ("lambda_min", C.c_double), ("lambda_max", C.c_double)]
+                ("lambda_min", C.c_double), ("lambda_max", C.c_double), ("breakdown", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class _Op(C.Structure):
@@ -106,6 +107,9 @@ def lib():
         L.orc_update_chebyshev_bounds.argtypes = [C.POINTER(_Op), dp, C.c_int32, dp, dp]
         L.orc_cheb_apply.argtypes = [C.POINTER(_Op), C.c_int32, C.c_double, C.c_double, dp, dp, dp, dp, dp]
         L.orc_cg_solve.argtypes = [C.POINTER(_Op), C.POINTER(_CgCfg), dp, dp, C.POINTER(CgStats)]
+        L.orc_gershgorin.argtypes = [C.POINTER(_Op)]
+        L.orc_estimate_eigenvalue_bounds_d.argtypes = [C.POINTER(_Op), dp, C.c_int32, dp, dp]
+        L.orc_gershgorin.restype = C.c_double
         L.orc_cholesky_solve.argtypes = [C.c_int32, C.c_int32] + [dp] * 7
         L.orc_solver_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(_CgCfg)]
         L.orc_solver_create.restype = C.c_void_p
@@ -306,6 +310,15 @@ def estimate_eigenvalue_bounds(coeffs, rhs, nsteps=15, d=None):
     return lo.value, hi.value
 
 
+def estimate_eigenvalue_bounds_d(coeffs, rhs, nsteps=15, d=None):
+    """The estimate in the D-inner product (Jacobi-PCG's own recurrence): 0.9 /
+    1.1 x Ritz values that lie in the spectrum of D^-1 A."""
+    lo, hi = C.c_double(), C.c_double()
+    op = _op(coeffs, d)
+    lib().orc_estimate_eigenvalue_bounds_d(C.byref(op), _p(F(rhs)), nsteps, C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
 def update_chebyshev_bounds(coeffs, rhs, bounds=(1.0, 1.0), nsteps=15, d=None):
     """src/preconditioner.jl:197-233: returns (lambda_min, lambda_max, warned)."""
     lo, hi = C.c_double(bounds[0]), C.c_double(bounds[1])
@@ -324,11 +337,19 @@ def cheb_apply(coeffs, x, degree, lmin, lmax, d=None):
     return y
 
 
+def gershgorin(coeffs, d=None) -> float:
+    """1 + max_i sum_j |a_ij| / |d_i|, the Gershgorin bound of D^-1 A."""
+    op = _op(coeffs, d)
+    return lib().orc_gershgorin(C.byref(op))
+
+
 def cg_solve(coeffs, rhs, x0, precond="jacobi", degree=4, nsteps=15, atol=1.4901161193847656e-8,
-             rtol=1.4901161193847656e-8, itmax=0, bounds=(1.0, 1.0)):
-    """Krylov-semantics PCG as called at src/linear_solver.jl:915-945. Returns (x, stats, bounds)."""
+             rtol=1.4901161193847656e-8, itmax=0, bounds=(1.0, 1.0), gershgorin=False):
+    """Krylov-semantics PCG as called at src/linear_solver.jl:915-945. Returns (x, stats, bounds).
+    gershgorin=True takes lambda_max from the Gershgorin bound (the HIP library's default bounds)."""
     nx, ny = coeffs[0].shape
-    cfg = _CgCfg(1 if precond == "chebyshev" else 0, degree, nsteps, itmax, atol, rtol, bounds[0], bounds[1])
+    cfg = _CgCfg(1 if precond == "chebyshev" else 0, degree, nsteps, itmax, atol, rtol, bounds[0], bounds[1],
+                 1 if gershgorin else 0, 0)
     x = F(x0).reshape((nx, ny), order="F").copy(order="F")
     st = CgStats()
     op = _op(coeffs)
@@ -353,12 +374,13 @@ class Solver:
 
     def __init__(self, g: Grid, kind: str = "cholesky", precond: str = "jacobi", degree: int = 4,
                  nsteps: int = 15, atol: float = 1.4901161193847656e-8, rtol: float = 1.4901161193847656e-8,
-                 itmax: int = 0):
+                 itmax: int = 0, gershgorin: bool = False):
         if precond == "chebyshev" and degree < 1:
             raise ValueError("ChebyshevPreconditioner: degree must be >= 1")
         if precond == "chebyshev" and nsteps < 2:
             raise ValueError("ChebyshevPreconditioner: nsteps_estimate must be >= 2")
-        cfg = _CgCfg(1 if precond == "chebyshev" else 0, degree, nsteps, itmax, atol, rtol, 1.0, 1.0)
+        cfg = _CgCfg(1 if precond == "chebyshev" else 0, degree, nsteps, itmax, atol, rtol, 1.0, 1.0,
+                     1 if gershgorin else 0, 0)
         self.kind = kind
         self.g = g
         self._h = lib().orc_solver_create(g.nx, g.ny, 0 if kind == "cholesky" else 1, C.byref(cfg))
@@ -397,14 +419,18 @@ def picard_loop(sv: Solver, s: State, g: Grid, p: ModelParameters, m: Model, ite
     cgi = C.c_int64(0)
     conv = lib().orc_picard_loop(sv._h, C.byref(g.c()), C.byref(p.c()), C.byref(m.c()), C.byref(s.c()), iters, tol,
                                  check_every, -1.0 if alpha is None else float(alpha), C.byref(last), C.byref(cgi))
+    if conv < 0:
+        raise np.linalg.LinAlgError("oracle linear solve failed (CG breakdown or failed factorisation)")
     return bool(conv), last.value, cgi.value
 
 
 def parabolic_step(sv: Solver, s: State, g: Grid, p: ModelParameters, m: Model, dt: float) -> int:
     """src/parabolic_solver.jl:16-38. Returns CG iterations used."""
     cgi = C.c_int64(0)
-    lib().orc_parabolic_step(sv._h, C.byref(g.c()), C.byref(p.c()), C.byref(m.c()), C.byref(s.c()), dt,
-                             C.byref(cgi))
+    rc = lib().orc_parabolic_step(sv._h, C.byref(g.c()), C.byref(p.c()), C.byref(m.c()), C.byref(s.c()), dt,
+                                  C.byref(cgi))
+    if rc != 0:
+        raise np.linalg.LinAlgError("oracle linear solve failed (CG breakdown or failed factorisation)")
     return cgi.value
 
 
@@ -423,6 +449,8 @@ def step(sv: Solver, s: State, g: Grid, p: ModelParameters, m: Model, dt: float,
     conv = lib().orc_step(sv._h, C.byref(g.c()), C.byref(p.c()), C.byref(m.c()), C.byref(s.c()), dt, ieb_mode,
                           ieb_uniform, mp, iters, tol, check_every, -1.0 if alpha is None else float(alpha),
                           C.byref(last), C.byref(cgi))
+    if conv < 0:
+        raise np.linalg.LinAlgError("oracle linear solve failed (CG breakdown or failed factorisation)")
     return bool(conv), last.value, cgi.value
 
 

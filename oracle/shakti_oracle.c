@@ -730,6 +730,51 @@ void orc_estimate_eigenvalue_bounds(const orc_op* op, const double* rhs, int32_t
     free(r); free(p); free(Ap);
 }
 
+/* The same estimate in the D-inner product <x, y>_D = x'Dy, where D^-1 A is
+ * self-adjoint: CG on D^-1 A x = D^-1 rhs from x0 = 0 (r = rhs ./ d, gamma =
+ * r'Dr, p'Ap -> p'(A p)), i.e. the coefficients of Jacobi-preconditioned CG,
+ * a symmetric Lanczos process whose Ritz values lie in the spectrum of
+ * D^-1 A.  Scaled by 0.9 / 1.1 as the reference scales its own.  The HIP
+ * library's Gershgorin bound mode takes lambda_min from it. */
+void orc_estimate_eigenvalue_bounds_d(const orc_op* op, const double* rhs, int32_t nsteps, double* lmin,
+                                      double* lmax) {
+    size_t n = (size_t)op->nx * op->ny;
+    double* r = malloc(n * sizeof(double));
+    double* p = malloc(n * sizeof(double));
+    double* Ap = malloc(n * sizeof(double));
+    double* Au = malloc(n * sizeof(double));
+    double alphas[256], betas[256];
+    for (size_t i = 0; i < n; ++i) r[i] = rhs[i] / op->d[i];
+    memcpy(p, r, n * sizeof(double));
+    double gamma = 0.0;
+    for (size_t i = 0; i < n; ++i) gamma += r[i] * r[i] * op->d[i];
+    for (int k = 0; k < nsteps; ++k) {
+        orc_matvec(op->nx, op->ny, op->aP, op->aE, op->aW, op->aN, op->aS, p, Au);
+        double pAp = dot(n, p, Au);
+        for (size_t i = 0; i < n; ++i) Ap[i] = Au[i] / op->d[i];
+        alphas[k] = gamma / pAp;
+        if (k < nsteps - 1) {
+            double gn = 0.0;
+            for (size_t i = 0; i < n; ++i) {
+                r[i] -= alphas[k] * Ap[i];
+                gn += r[i] * r[i] * op->d[i];
+            }
+            betas[k] = gn / gamma;
+            for (size_t i = 0; i < n; ++i) p[i] = r[i] + betas[k] * p[i];
+            gamma = gn;
+        }
+    }
+    double dd[256], ee[256];
+    dd[0] = 1.0 / alphas[0];
+    for (int k = 1; k < nsteps; ++k) dd[k] = 1.0 / alphas[k] + betas[k - 1] / alphas[k - 1];
+    for (int k = 0; k < nsteps - 1; ++k) ee[k] = sqrt(betas[k]) / alphas[k];
+    double mn, mx;
+    if (orc_tridiag_extreme_eigs(nsteps, dd, ee, &mn, &mx) != 0) { mn = NAN; mx = NAN; }
+    *lmin = mn * 0.9;
+    *lmax = mx * 1.1;
+    free(r); free(p); free(Ap); free(Au);
+}
+
 /* src/preconditioner.jl:197-233 update_chebyshev_bounds!: adopt the estimate
  * only if finite, positive and non-empty; otherwise keep the previous bounds
  * and report a warning (return 1). */
@@ -762,6 +807,44 @@ void orc_cheb_apply(const orc_op* op, int32_t degree, double lmin, double lmax, 
     }
 }
 
+/* Gershgorin bound of D^-1 A over the couplings the operator applies:
+ * 1 + max_i sum_j |a_ij| / |d_i| (every eigenvalue of D^-1 A lies in
+ * [1 - rho, 1 + rho], rho = that max - 1; Gershgorin's circle theorem).  NaN
+ * when a row has a zero diagonal and no couplings. */
+double orc_gershgorin(const orc_op* op) {
+    const int32_t nx = op->nx, ny = op->ny;
+    double m = -INFINITY;
+    int nan = 0;
+    for (int32_t iy = 0; iy < ny; ++iy)
+        for (int32_t ix = 0; ix < nx; ++ix) {
+            const size_t c = (size_t)ix + (size_t)iy * nx;
+            double s = 0.0;
+            if (ix < nx - 1) s += fabs(op->aE[c]);
+            if (ix > 0) s += fabs(op->aW[c]);
+            if (iy < ny - 1) s += fabs(op->aN[c]);
+            if (iy > 0) s += fabs(op->aS[c]);
+            const double v = 1.0 + s / fabs(op->d[c]);
+            if (isnan(v)) nan = 1;
+            else if (v > m) m = v;
+        }
+    return nan ? NAN : m;
+}
+
+/* The bounds of one solve in the Gershgorin mode: lambda_min from the
+ * D-inner-product estimate, lambda_max = G, adopted when finite with
+ * 0 < lambda_min < G; otherwise the previous lambda_min with this G (the
+ * placeholder (1, 1) stays as it is).  Returns 1 when the estimate was
+ * rejected. */
+int orc_gershgorin_bounds(const orc_op* op, const double* rhs, int32_t nsteps, double* lmin, double* lmax) {
+    double a, b;
+    orc_estimate_eigenvalue_bounds_d(op, rhs, nsteps, &a, &b);
+    const double G = orc_gershgorin(op);
+    const int gok = isfinite(G) && G > 0;
+    if (gok && isfinite(a) && a > 0 && G > a) { *lmin = a; *lmax = G; return 0; }
+    if (gok && *lmin != *lmax && G > *lmin) *lmax = G;
+    return 1;
+}
+
 /* ------------------------------------------------------------ CG solver */
 
 typedef struct {
@@ -771,11 +854,16 @@ typedef struct {
     int32_t itmax;         /* 0 -> 2n, Krylov's default */
     double atol, rtol;     /* Krylov defaults are sqrt(eps) */
     double lambda_min, lambda_max; /* persistent Chebyshev bounds; start at (1, 1) */
+    int32_t gershgorin;    /* 0: the reference's bounds; 1: lambda_max from the Gershgorin bound (the HIP
+                              library's default, a stated deviation from the reference) */
+    int32_t pad;
 } orc_cg_cfg;
 
 typedef struct {
     int32_t niter, solved, zero_curvature, bounds_warning;
     double rnorm0, rnorm, lambda_min, lambda_max;
+    int32_t breakdown; /* 1: r'z < 0 or not finite, where Krylov's sqrt(r'z) throws (x0 left unchanged) */
+    int32_t pad;
 } orc_cg_stats;
 
 /* Krylov.jl 0.10.8 cg! (not vendored; restated from its published algorithm)
@@ -783,7 +871,10 @@ typedef struct {
  * preconditioner applied with ldiv, residual measured in the preconditioner
  * norm sqrt(r'z), stop when rNorm <= atol + rtol*rNorm0 or rNorm + 1 <= 1,
  * zero curvature when |p'Ap| <= eps*||p||^2_M, itmax = 2n by default.
- * x0 is overwritten with x0 + dx (the reference's s.h .= ws.x). */
+ * x0 is overwritten with x0 + dx (the reference's s.h .= ws.x).  A negative
+ * or non-finite r'z stops the solve with st->breakdown = 1, x0 untouched and
+ * a return value of -2: Julia's sqrt throws a DomainError there, so the
+ * reference's cg! never returns and s.h keeps its value. */
 int orc_cg_solve(const orc_op* op, orc_cg_cfg* cfg, const double* b, double* x0, orc_cg_stats* st) {
     size_t n = (size_t)op->nx * op->ny;
     double *x = calloc(n, sizeof(double)), *r = malloc(n * sizeof(double)), *z = malloc(n * sizeof(double));
@@ -792,7 +883,9 @@ int orc_cg_solve(const orc_op* op, orc_cg_cfg* cfg, const double* b, double* x0,
     memset(st, 0, sizeof(*st));
     if (cfg->precond == 1)
         st->bounds_warning =
-            orc_update_chebyshev_bounds(op, b, cfg->cheb_nsteps, &cfg->lambda_min, &cfg->lambda_max);
+            cfg->gershgorin
+                ? orc_gershgorin_bounds(op, b, cfg->cheb_nsteps, &cfg->lambda_min, &cfg->lambda_max)
+                : orc_update_chebyshev_bounds(op, b, cfg->cheb_nsteps, &cfg->lambda_min, &cfg->lambda_max);
     st->lambda_min = cfg->lambda_min;
     st->lambda_max = cfg->lambda_max;
 #define PRECOND(zz, rr)                                                                                      \
@@ -809,6 +902,11 @@ int orc_cg_solve(const orc_op* op, orc_cg_cfg* cfg, const double* b, double* x0,
     double gamma = dot(n, r, z);
     double rNorm = sqrt(gamma);
     st->rnorm0 = rNorm;
+    if (!(gamma >= 0.0) || !isfinite(gamma)) {
+        st->breakdown = 1;
+        st->rnorm = rNorm;
+        goto fail;
+    }
     if (gamma == 0.0) {
         st->solved = 1; st->rnorm = rNorm; st->niter = 0;
         goto done;
@@ -829,6 +927,12 @@ int orc_cg_solve(const orc_op* op, orc_cg_cfg* cfg, const double* b, double* x0,
             PRECOND(z, r);
             double gn = dot(n, r, z);
             rNorm = sqrt(gn);
+            if (!(gn >= 0.0) || !isfinite(gn)) {
+                st->niter = (int32_t)(iter + 1);
+                st->breakdown = 1;
+                st->rnorm = rNorm;
+                goto fail;
+            }
             solved = (rNorm <= eps_stop) || (rNorm + 1.0 <= 1.0);
             if (!solved) {
                 double beta = gn / gamma;
@@ -846,9 +950,10 @@ int orc_cg_solve(const orc_op* op, orc_cg_cfg* cfg, const double* b, double* x0,
     }
 done:
     for (size_t i = 0; i < n; ++i) x0[i] = x[i] + x0[i];
+fail:
 #undef PRECOND
     free(x); free(r); free(z); free(p); free(Ap); free(cr); free(cp); free(cAp);
-    return 0;
+    return st->breakdown ? -2 : 0;
 }
 
 /* --------------------------------------------- banded Cholesky (oracle) */
@@ -962,8 +1067,9 @@ int orc_solve_linear(orc_solver* sv, const orc_grid* g, const orc_params* p, int
 }
 
 /* src/elliptic_solver.jl:138-208 Picard_loop!/Picard_iteration!/relax_h!.
- * relax_alpha <= 0 means NoHeadRelaxation.  Returns converged; *last_iter as
- * the reference's ps.last_iter.  *cg_iters accumulates CG iterations. */
+ * relax_alpha <= 0 means NoHeadRelaxation.  Returns converged (-1 when a
+ * linear solve failed); *last_iter as the reference's ps.last_iter.
+ * *cg_iters accumulates CG iterations. */
 int32_t orc_picard_loop(orc_solver* sv, const orc_grid* g, const orc_params* p, const orc_model* m, orc_state* s,
                         int32_t iters, double tol, int32_t check_every, double relax_alpha, int32_t* last_iter,
                         int64_t* cg_iters) {
@@ -974,7 +1080,11 @@ int32_t orc_picard_loop(orc_solver* sv, const orc_grid* g, const orc_params* p, 
     for (int32_t it = 1; it <= iters; ++it) {
         memcpy(h_prev, s->h, n * sizeof(double));
         orc_cg_stats st;
-        orc_solve_linear(sv, g, p, m->kfs, s, 0, 0.0, &st);
+        if (orc_solve_linear(sv, g, p, m->kfs, s, 0, 0.0, &st) != 0) {
+            /* the reference's solve throws (CG breakdown, failed factorisation) */
+            free(h_prev);
+            return -1;
+        }
         if (cg_iters) *cg_iters += st.niter;
         if (relax_alpha > 0.0)
             for (size_t i = 0; i < n; ++i) s->h[i] = relax_alpha * s->h[i] + (1.0 - relax_alpha) * h_prev[i];
@@ -1031,8 +1141,9 @@ int32_t orc_step(orc_solver* sv, const orc_grid* g, const orc_params* p, const o
     int32_t conv = 1;
     if (p->e_v == 0.0) {
         conv = orc_picard_loop(sv, g, p, m, s, iters, tol, check_every, relax_alpha, last_iter, cg_iters);
+        if (conv < 0) return -1;
     } else {
-        orc_parabolic_step(sv, g, p, m, s, dt, cg_iters);
+        if (orc_parabolic_step(sv, g, p, m, s, dt, cg_iters) != 0) return -1;
         *last_iter = 0;
     }
     orc_step_b(g, p, m, s, dt);

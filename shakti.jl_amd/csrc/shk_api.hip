@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -156,6 +157,9 @@ struct shk_ctx {
     int nn_every = 0;
     double nn_stall = 1.0;
 
+    // statistics of the solves since the last shk_solve_log (at most kLogCap)
+    std::vector<shk_solve_stats> solve_log;
+
     std::string err;
 };
 
@@ -169,6 +173,8 @@ const char* kProfNames[] = {"assemble",   "lanczos",      "cheb_bounds", "cg_ini
 // mg_down / mg_up: the coarser levels; *_fine: the finest level
 enum ProfId { P_ASM, P_LZ, P_CB, P_INIT, P_CHEB, P_MV, P_UPD, P_FIN, P_DER, P_GAP, P_STAG, P_IEB, P_IC, P_MATVEC, P_IR,
               P_FUSED, P_MGD, P_MGU, P_MGC, P_MGS, P_NNF, P_NNW, P_MGD0, P_MGU0, P_NNP, P_NUM };
+
+constexpr size_t kLogCap = 65536;
 
 int fail(shk_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -274,6 +280,8 @@ int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
     if (cfg->form != SHK_FORM_FP64 && cfg->form != SHK_FORM_FP32_REFINE && cfg->form != SHK_FORM_LEARNED)
         return fail(ctx, SHK_ERR_ARG, "unknown solver form");
     if (cfg->check_every < 1) return fail(ctx, SHK_ERR_ARG, "check_every must be >= 1");
+    if (cfg->cheb_bounds != SHK_CHEB_BOUNDS_GERSHGORIN && cfg->cheb_bounds != SHK_CHEB_BOUNDS_REFERENCE)
+        return fail(ctx, SHK_ERR_ARG, "unknown Chebyshev bound mode");
     if (!(cfg->atol >= 0) || !(cfg->rtol >= 0)) return fail(ctx, SHK_ERR_ARG, "atol/rtol must be >= 0");
     return SHK_OK;
 }
@@ -287,6 +295,8 @@ int reset_scalars(shk_ctx* ctx, double atol, double rtol, int itmax) {
     h->atol = atol; h->rtol = rtol; h->itmax = itmax;
     h->counter = 0;
     h->gpend = 0;
+    h->done_defer = 0;
+    h->breakdown = 0;
     HIPCHK(hipMemcpyAsync(ctx->sc, h, sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
     return SHK_OK;
 }
@@ -294,6 +304,60 @@ int reset_scalars(shk_ctx* ctx, double atol, double rtol, int itmax) {
 int read_scalars(shk_ctx* ctx) {
     HIPCHK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+// The statistics of a finished Krylov solve from the device scalars (read
+// last by the solve loop).  cheb: the Chebyshev preconditioner was in use.
+shk_solve_stats stats_of(const shk_ctx* ctx, bool cheb) {
+    const Scalars* h = ctx->sc_host;
+    shk_solve_stats st;
+    memset(&st, 0, sizeof(st));
+    st.niter = h->iter;
+    st.solved = h->solved;
+    st.zero_curvature = h->zero_curv;
+    st.tired = h->tired;
+    st.bounds_warning = cheb ? h->bounds_warning : 0;
+    st.rnorm0 = h->rNorm0;
+    st.rnorm = h->rNorm;
+    st.lambda_min = h->lmin;
+    st.lambda_max = h->lmax;
+    st.breakdown = h->breakdown;
+    st.breakdown_value = h->breakdown ? h->breakdown_value : 0.0;
+    st.lz_lambda_min = cheb ? h->lz_lmin : NAN;
+    st.lz_lambda_max = cheb ? h->lz_lmax : NAN;
+    st.gershgorin = (cheb && ctx->cfg.cheb_bounds == SHK_CHEB_BOUNDS_GERSHGORIN) ? h->gersh : NAN;
+    st.true_residual = NAN;
+    return st;
+}
+
+void log_solve(shk_ctx* ctx, const shk_solve_stats& st) {
+    if (ctx->solve_log.size() >= kLogCap) ctx->solve_log.erase(ctx->solve_log.begin());
+    ctx->solve_log.push_back(st);
+}
+
+// Record a finished solve (stats out, log) and turn a breakdown into the
+// loud failure the reference's Krylov loop gives (sqrt of a negative r'z).
+int conclude(shk_ctx* ctx, const shk_solve_stats& st, shk_solve_stats* out) {
+    if (out) *out = st;
+    log_solve(ctx, st);
+    char msg[256];
+    if (st.breakdown == 1) {
+        snprintf(msg, sizeof msg,
+                 "CG breakdown at iteration %d: r'z = %.17g (the preconditioner or the operator is not positive "
+                 "definite); h left unchanged", st.niter, st.breakdown_value);
+        return fail(ctx, SHK_ERR_NUMERIC, msg);
+    }
+    if (st.breakdown == 2) {
+        snprintf(msg, sizeof msg, "CG breakdown at iteration %d: p'Ap = %.17g is not finite; h left unchanged",
+                 st.niter, st.breakdown_value);
+        return fail(ctx, SHK_ERR_NUMERIC, msg);
+    }
+    if (st.breakdown == 3) {
+        snprintf(msg, sizeof msg, "the solve's head has a non-finite residual (||b - A h|| / ||b|| = %g)",
+                 st.true_residual);
+        return fail(ctx, SHK_ERR_NUMERIC, msg);
+    }
     return SHK_OK;
 }
 
@@ -334,33 +398,40 @@ void cheb_apply(shk_ctx* c, const Work<T>& w, int degree, int mode) {
     }
 }
 
-// Lanczos bound estimate + coefficients (Chebyshev only).
+// Lanczos bound estimate + coefficients (Chebyshev only); with the
+// Gershgorin mode the bound of D^-1 A first.
 template <class T>
 void cheb_refresh(shk_ctx* c, const Work<T>& w, int nsteps, int degree) {
     const Geom& G = c->G;
     const double nb = (double)c->n * sizeof(T);
+    const int mode = c->cfg.cheb_bounds;
+    if (mode == SHK_CHEB_BOUNDS_GERSHGORIN) {
+        Prof pr(c, P_CB, 3 * nb);
+        launch::gershgorin<T>(G, w.dg, w.aE, w.aN, c->partials, c->sc, c->st);
+    }
+    const int sym = mode == SHK_CHEB_BOUNDS_GERSHGORIN;
     {
-        Prof pr(c, P_LZ, 3 * nb);
-        launch::lanczos_init<T>(G, w.b, w.r, w.pA, c->partials, c->sc, c->st);
+        Prof pr(c, P_LZ, (sym ? 4 : 3) * nb);
+        launch::lanczos_init<T>(G, w.b, w.dg, sym, w.r, w.pA, c->partials, c->sc, c->st);
     }
     T* p_old = w.pA;
     T* p_new = w.pB;
     for (int k = 0; k < nsteps; ++k) {
         {
             Prof pr(c, P_LZ, 7 * nb);
-            launch::lanczos_step_a<T>(G, w.aP, w.aE, w.aN, w.dg, w.r, p_old, p_new, w.Ap, k, c->partials, c->sc,
-                                      c->st);
+            launch::lanczos_step_a<T>(G, w.aP, w.aE, w.aN, w.dg, sym, w.r, p_old, p_new, w.Ap, k, c->partials,
+                                      c->sc, c->st);
         }
         if (k < nsteps - 1) {
-            Prof pr(c, P_LZ, 3 * nb);
-            launch::lanczos_step_b<T>(G, w.Ap, w.r, k, c->partials, c->sc, c->st);
+            Prof pr(c, P_LZ, (sym ? 4 : 3) * nb);
+            launch::lanczos_step_b<T>(G, w.Ap, w.dg, sym, w.r, k, c->partials, c->sc, c->st);
         }
         T* t = p_old;
         p_old = p_new;
         p_new = t;
     }
     Prof pr(c, P_CB, 0);
-    launch::cheb_bounds(nsteps, degree, 1, c->sc, c->st);
+    launch::cheb_bounds(nsteps, degree, 1, mode, c->sc, c->st);
 }
 
 // Multigrid hierarchy for the FP32 flux-form system in w (level 0 shares its
@@ -901,9 +972,10 @@ int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_sol
                           ctx->st));
     int refinements = 0;
     long inner = 0;
-    double rnorm0 = 0.0, rnorm = 0.0;
-    int solved = 0, zc = 0;
-    for (int k = 0; k < 64; ++k) {
+    double rnorm0 = 0.0, rnorm = 0.0, best = INFINITY;
+    int solved = 0, zc = 0, tired = 1, stall = 0;
+    constexpr int kMaxRefinements = 64;
+    for (int k = 0; k < kMaxRefinements; ++k) {
         {
             Prof pr(ctx, P_IR, 8.5 * nb);
             launch::ir_residual(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, x, r_in, k == 0, ctx->partials, ctx->sc,
@@ -913,36 +985,71 @@ int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_sol
         if (rc) return rc;
         if (k == 0) rnorm0 = ctx->sc_host->ir_rnorm0;
         rnorm = ctx->sc_host->ir_rnorm;
-        if (ctx->sc_host->ir_done) { solved = 1; break; }
+        if (ctx->sc_host->ir_done) { solved = 1; tired = 0; break; }
+        // the precision floor: three refinements in a row that do not lower
+        // the FP64 residual by 10% end the loop (neither solved nor tired; a
+        // target below what FP64 can represent, as after Picard convergence)
+        if (rnorm < 0.9 * best) {
+            best = rnorm;
+            stall = 0;
+        } else if (++stall >= 3) {
+            tired = 0;
+            break;
+        }
         rc = pcg<float>(ctx, wi, nullptr, cfg.precond, cfg.cheb_degree, cfg.check_every, 0.0, cfg.inner_rtol,
                         itmax);
         if (rc) return rc;
         inner += ctx->sc_host->iter;
         zc |= ctx->sc_host->zero_curv;
+        if (ctx->sc_host->breakdown) {
+            // an inner solve broke down: h is left as it was
+            shk_solve_stats sst = stats_of(ctx, cheb);
+            sst.niter = (int32_t)inner;
+            sst.refinements = refinements;
+            sst.rnorm0 = rnorm0;
+            sst.rnorm = rnorm;
+            if (ctx->prof) prof_collect(ctx);
+            return conclude(ctx, sst, stats);
+        }
         {
             Prof pr(ctx, P_IR, 20.0 * n);
             launch::ir_correct(n, x, ctx->x32, ctx->st);
         }
         ++refinements;
-        if (ctx->sc_host->iter == 0) break;  // the inner solve can make no progress
+        if (ctx->sc_host->iter == 0) {
+            // the inner solve met Krylov's stopping test at its start (its
+            // absolute test rNorm + 1 <= 1 on a residual at the FP64 floor):
+            // no refinement can lower it further
+            solved = ctx->sc_host->solved;
+            tired = 0;
+            break;
+        }
+    }
+    shk_solve_stats sst = stats_of(ctx, cheb);
+    sst.niter = (int32_t)inner;
+    sst.solved = solved;
+    sst.zero_curvature = zc;
+    sst.tired = tired;
+    sst.refinements = refinements;
+    sst.rnorm0 = rnorm0;
+    sst.rnorm = rnorm;
+    {
+        Prof pr(ctx, P_FIN, 6 * nb);
+        launch::true_residual(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, x, nullptr, ctx->partials, ctx->sc, ctx->st);
+    }
+    rc = read_scalars(ctx);
+    if (rc) return rc;
+    sst.true_residual = ctx->sc_host->tres_b > 0.0 ? ctx->sc_host->tres / ctx->sc_host->tres_b : ctx->sc_host->tres;
+    if (!std::isfinite(sst.true_residual)) {
+        sst.breakdown = 3;
+        if (ctx->prof) prof_collect(ctx);
+        return conclude(ctx, sst, stats);
     }
     if (relax_alpha > 0.0) launch::relax(n, x, h_prev, relax_alpha, ctx->st);
     ctx->F.f[SHK_H] = x;
     ctx->h_alt = h_prev;
-    if (stats) {
-        memset(stats, 0, sizeof(*stats));
-        stats->niter = (int32_t)inner;
-        stats->solved = solved;
-        stats->zero_curvature = zc;
-        stats->bounds_warning = cheb ? ctx->sc_host->bounds_warning : 0;
-        stats->refinements = refinements;
-        stats->rnorm0 = rnorm0;
-        stats->rnorm = rnorm;
-        stats->lambda_min = ctx->sc_host->lmin;
-        stats->lambda_max = ctx->sc_host->lmax;
-    }
     if (ctx->prof) prof_collect(ctx);
-    return SHK_OK;
+    return conclude(ctx, sst, stats);
 }
 
 int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats* stats) {
@@ -1040,6 +1147,28 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol, itmax,
                          nn_every, &stalled);
     if (rc) return rc;
+    shk_solve_stats sst = stats_of(ctx, cheb);
+    sst.fallback = learned ? (ctx->sc_host->nn_fallback | (stalled ? 2 : 0)) : 0;
+    if (sst.breakdown) {
+        if (ctx->prof) prof_collect(ctx);
+        return conclude(ctx, sst, stats);
+    }
+    {
+        // the FP64 residual of the head about to be returned, x + x0
+        Prof pr(ctx, P_FIN, 6 * nb);
+        launch::true_residual(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, x0, ctx->partials, ctx->sc, ctx->st);
+    }
+    rc = read_scalars(ctx);
+    if (rc) return rc;
+    {
+        const Scalars* h = ctx->sc_host;
+        sst.true_residual = h->tres_b > 0.0 ? h->tres / h->tres_b : h->tres;
+        if (!std::isfinite(sst.true_residual)) {
+            sst.breakdown = 3;
+            if (ctx->prof) prof_collect(ctx);
+            return conclude(ctx, sst, stats);
+        }
+    }
     if (ctx->rec_kind == 1 && ctx->rec_x) {
         // targets of the pairs this solve recorded: sqrt(aP) (x_final - x_k) / s_k
         const size_t img = (size_t)ctx->W8 * ctx->H8;
@@ -1067,21 +1196,8 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
         ctx->rec_count += 1;
         HIPCHK(hipStreamSynchronize(ctx->st));
     }
-    const Scalars* h = ctx->sc_host;
-    if (stats) {
-        memset(stats, 0, sizeof(*stats));
-        stats->fallback = learned ? (h->nn_fallback | (stalled ? 2 : 0)) : 0;
-        stats->niter = h->iter;
-        stats->solved = h->solved;
-        stats->zero_curvature = h->zero_curv;
-        stats->bounds_warning = cheb ? h->bounds_warning : 0;
-        stats->rnorm0 = h->rNorm0;
-        stats->rnorm = h->rNorm;
-        stats->lambda_min = h->lmin;
-        stats->lambda_max = h->lmax;
-    }
     if (ctx->prof) prof_collect(ctx);
-    return SHK_OK;
+    return conclude(ctx, sst, stats);
 }
 
 int derived(shk_ctx* ctx, const double* h_prev, int check, double tol) {
@@ -1343,7 +1459,9 @@ int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every
         if (check) {
             rc = read_scalars(ctx);
             if (rc) return rc;
-            if (ctx->sc_host->picard_converged) {
+            // a solve that stopped short (itmax, zero curvature) leaves a head
+            // change that says nothing about Picard convergence
+            if (ctx->sc_host->picard_converged && !st.zero_curvature && !st.tired) {
                 conv = 1;
                 last = it;
                 break;
@@ -1355,6 +1473,17 @@ int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (converged) *converged = conv;
     if (last_iter) *last_iter = last;
+    return SHK_OK;
+}
+
+int shk_solve_log(shk_ctx* ctx, shk_solve_stats* out, int32_t max_entries, int32_t* n_entries) {
+    if (!ctx || !n_entries || max_entries < 0 || (max_entries > 0 && !out))
+        return fail(ctx, SHK_ERR_ARG, "shk_solve_log: bad argument");
+    const size_t n = ctx->solve_log.size();
+    const size_t k = std::min(n, (size_t)max_entries);
+    for (size_t i = 0; i < k; ++i) out[i] = ctx->solve_log[n - k + i];
+    *n_entries = (int32_t)k;
+    ctx->solve_log.clear();
     return SHK_OK;
 }
 
@@ -1544,6 +1673,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
         Scalars live = saved;
         live.done = 0;
         live.gpend = 0;
+        live.done_defer = 0;
         live.alpha = 0.0;  // x += 0 p: the traffic of the solve's matvec, x left as it is
         HIPCHK(hipMemcpy(ctx->sc, &live, sizeof(Scalars), hipMemcpyHostToDevice));
     }

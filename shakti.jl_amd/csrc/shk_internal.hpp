@@ -59,7 +59,20 @@ struct Scalars {
     double nn_scale_pc, nn_gamma_pc;
     // r'z partials of the last preconditioner not yet finished (deferred
     // finishing: the next CG matvec sums them)
-    int32_t gpend, gpad;
+    int32_t gpend;
+    // deferred finishing: the CG matvec's stop decision (converged, tired or
+    // breakdown).  The matvec never writes `done`, which its own workgroups
+    // test on entry; the kernel after it publishes done from this flag.
+    int32_t done_defer;
+    // breakdown of the Krylov recurrence: 1 when r'z < 0 or not finite (the
+    // preconditioner or the operator is not positive definite; Krylov's
+    // sqrt(r'z) throws there), 2 when p'Ap is not finite
+    int32_t breakdown, bpad;
+    double breakdown_value;  // the offending r'z or p'Ap
+    // Gershgorin bound of D^-1 A: 1 + max_i sum_j |a_ij| / a_ii
+    double gersh;
+    // FP64 residual of the solution a solve returns: ||b - A h||_2, ||b||_2
+    double tres, tres_b;
     // unused (reductions finish in a separate kernel); reset with the rest
     uint32_t counter;
     uint32_t pad;
@@ -135,15 +148,28 @@ void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, 
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st);
 
-// Lanczos-from-CG bound estimate and the Chebyshev coefficients.
+// Lanczos-from-CG bound estimate and the Chebyshev coefficients; sym != 0
+// runs it in the D-inner product (D = dg), where D^-1 A is self-adjoint.
 template <class T>
-void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Scalars* sc, hipStream_t st);
+void lanczos_init(const Geom& G, const T* rhs, const T* dg, int sym, T* r, T* p0, double* partials, Scalars* sc,
+                  hipStream_t st);
 template <class T>
-void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* r, const T* p_old,
-                    T* p_new, T* Ap, int k, double* partials, Scalars* sc, hipStream_t st);
+void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, int sym, const T* r,
+                    const T* p_old, T* p_new, T* Ap, int k, double* partials, Scalars* sc, hipStream_t st);
 template <class T>
-void lanczos_step_b(const Geom& G, const T* Ap, T* r, int k, double* partials, Scalars* sc, hipStream_t st);
-void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStream_t st);
+void lanczos_step_b(const Geom& G, const T* Ap, const T* dg, int sym, T* r, int k, double* partials, Scalars* sc,
+                    hipStream_t st);
+// mode 0 (SHK_CHEB_BOUNDS_GERSHGORIN): lambda_min from the estimate in the
+// D-inner product, lambda_max the Gershgorin bound of D^-1 A (gershgorin() first);
+// mode 1 (SHK_CHEB_BOUNDS_REFERENCE): both from the estimate, as the reference.
+void cheb_bounds(int nsteps, int degree, int use_estimate, int mode, Scalars* sc, hipStream_t st);
+template <class T>
+void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* partials, Scalars* sc,
+                hipStream_t st);
+// ||b - A h||_2 and ||b||_2 of the FP64 system for h = x + x0 (x0 may be
+// null), into sc->tres, sc->tres_b
+void true_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
+                   const double* x, const double* x0, double* partials, Scalars* sc, hipStream_t st);
 
 // Preconditioned CG pieces (mode 0: set-up, mode 1: iteration).
 template <class T>

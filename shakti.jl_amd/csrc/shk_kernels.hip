@@ -776,26 +776,40 @@ __global__ __launch_bounds__(256) void k_matvec(int nx, int ny, int R, const T* 
 
 // estimate_eigenvalue_bounds (src/preconditioner.jl:89-134): r = rhs, p = r,
 // gamma = r'r (x0 = 0).  p_zero is zeroed so step 0 can form p = r + 0*p.
-template <class T>
+//
+// SYM: the same recurrence in the D-inner product <x, y>_D = x'Dy, in which
+// D^-1 A is self-adjoint: CG on D^-1 A x = D^-1 rhs from x0 = 0, so r =
+// rhs ./ d and gamma = r'Dr; each step's p'Ap is <p, D^-1 A p>_D = p'(A p).
+// These are the coefficients of Jacobi-preconditioned CG on A, a symmetric
+// Lanczos process, so every Ritz value lies inside the spectrum of D^-1 A
+// (the Euclidean dots on the nonsymmetric D^-1 A give no such bound).
+template <class T, bool SYM>
 __global__ __launch_bounds__(256) void k_lz_init(int nx, int ny, int R, const T* __restrict__ rhs,
-                                                 T* __restrict__ r, T* __restrict__ p_zero, double* partials,
-                                                 Scalars* sc) {
+                                                 const T* __restrict__ dg, T* __restrict__ r, T* __restrict__ p_zero,
+                                                 double* partials, Scalars* sc) {
     int vbx, vby;
     virtual_block(vbx, vby);
     double acc = 0.0;
     walk(nx, ny, R, vbx, vby, [&](size_t c) {
-        const T v = rhs[c];
-        r[c] = v;
+        if constexpr (SYM) {
+            const T d = dg[c];
+            const T v = rhs[c] / d;
+            r[c] = v;
+            acc += (double)v * (double)v * (double)d;
+        } else {
+            const T v = rhs[c];
+            r[c] = v;
+            acc += (double)v * (double)v;
+        }
         p_zero[c] = 0;
-        acc += (double)v * (double)v;
     });
     double v[1] = {acc};
     block_partial<1, false>(v, partials);
 }
 
 // step k: p = r + beta_{k-1} p_old (beta = 0 at k = 0), Ap = (A p) ./ d,
-// alpha_k = gamma / p'Ap.
-template <class T>
+// alpha_k = gamma / p'Ap (SYM: gamma / p'(A p)).
+template <class T, bool SYM>
 __global__ __launch_bounds__(256) void k_lz_step_a(int nx, int ny, int R, const T* __restrict__ aP,
                                                    const T* __restrict__ aE, const T* __restrict__ aN,
                                                    const T* __restrict__ dg, const T* __restrict__ r,
@@ -810,16 +824,17 @@ __global__ __launch_bounds__(256) void k_lz_step_a(int nx, int ny, int R, const 
                  const T a = Au / dg[c];
                  p_new[c] = u;
                  Ap[c] = a;
-                 acc += (double)u * (double)a;
+                 acc += (double)u * (double)(SYM ? Au : a);
              });
     double v[1] = {acc};
     block_partial<1, false>(v, partials);
 }
 
-// step k (k < nsteps-1): r -= alpha_k Ap; beta_k = r'r / gamma.
-template <class T>
+// step k (k < nsteps-1): r -= alpha_k Ap; beta_k = r'r / gamma (SYM: r'Dr / gamma).
+template <class T, bool SYM>
 __global__ __launch_bounds__(256) void k_lz_step_b(int nx, int ny, int R, const T* __restrict__ Ap,
-                                                   T* __restrict__ r, int k, double* partials, Scalars* sc) {
+                                                   const T* __restrict__ dg, T* __restrict__ r, int k,
+                                                   double* partials, Scalars* sc) {
     int vbx, vby;
     virtual_block(vbx, vby);
     const T alpha = (T)sc->lz_alpha[k];
@@ -827,7 +842,7 @@ __global__ __launch_bounds__(256) void k_lz_step_b(int nx, int ny, int R, const 
     walk(nx, ny, R, vbx, vby, [&](size_t c) {
         const T v = r[c] - alpha * Ap[c];
         r[c] = v;
-        acc += (double)v * (double)v;
+        acc += SYM ? (double)v * (double)v * (double)dg[c] : (double)v * (double)v;
     });
     double v[1] = {acc};
     block_partial<1, false>(v, partials);
@@ -853,7 +868,25 @@ __device__ int sturm_count(int m, const double* a, const double* e2, double x) {
 // 0.9 / 1.1, and adopts the bounds only if finite, positive and non-empty
 // (update_chebyshev_bounds!, :197-233).  Then precomputes this solve's
 // Chebyshev alpha/beta sequence (ldiv!, :239-264).
-__global__ __launch_bounds__(64) void k_cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc) {
+//
+// mode 1 keeps the reference's bounds exactly: the recurrence with Euclidean
+// dots on the nonsymmetric D^-1 A.  Its Ritz values are not bounded by the
+// spectrum: on a 4096 x 1024 glacier with six decades of transmissivity it
+// returns lambda_max up to 1e17 (the true value is below 2) and lambda_min
+// near 1e-16.  The polynomial then shrinks by that factor, and Krylov's
+// zero-curvature test p'Ap <= eps ||p||^2_M, which scales with the
+// preconditioner, stops CG early.  Too small a lambda_max is no better: the
+// reference's degree-k polynomial turns negative for lambda > lambda_min +
+// lambda_max, an indefinite preconditioner.
+// mode 0 runs the estimate in the D-inner product (k_lz_* with SYM), whose
+// Ritz values lie inside the spectrum, takes lambda_min = 0.9 x the smallest,
+// and lambda_max = G, the Gershgorin bound of D^-1 A: every eigenvalue lies
+// in [1 - rho, 1 + rho], rho = max_i sum_j |a_ij| / a_ii, so the polynomial is
+// positive on the whole spectrum.  The estimate is accepted when finite with
+// 0 < lambda_min < G; otherwise the previous lambda_min is kept (the first
+// solve keeps the placeholder (1, 1), plain Jacobi sweeps).
+__global__ __launch_bounds__(64) void k_cheb_bounds(int nsteps, int degree, int use_estimate, int mode,
+                                                    Scalars* sc) {
     __shared__ double a[MAX_LANCZOS], e2[MAX_LANCZOS];
     __shared__ int bad;
     __shared__ double lo_s, hi_s;
@@ -911,12 +944,26 @@ __global__ __launch_bounds__(64) void k_cheb_bounds(int nsteps, int degree, int 
             const double lmin = res[0] * 0.9, lmax = res[1] * 1.1;
             sc->lz_lmin = lmin;
             sc->lz_lmax = lmax;
-            if (isfinite(lmin) && isfinite(lmax) && lmin > 0 && lmax > lmin) {
-                sc->lmin = lmin;
-                sc->lmax = lmax;
-                sc->bounds_warning = 0;
+            if (mode == 1) {
+                if (isfinite(lmin) && isfinite(lmax) && lmin > 0 && lmax > lmin) {
+                    sc->lmin = lmin;
+                    sc->lmax = lmax;
+                    sc->bounds_warning = 0;
+                } else {
+                    sc->bounds_warning = 1;
+                }
             } else {
-                sc->bounds_warning = 1;
+                const double G = sc->gersh;
+                const bool gok = isfinite(G) && G > 0;
+                if (gok && isfinite(lmin) && lmin > 0 && G > lmin) {
+                    sc->lmin = lmin;
+                    sc->lmax = G;
+                    sc->bounds_warning = 0;
+                } else {
+                    sc->bounds_warning = 1;
+                    // previous lambda_min with this operator's bound (not the placeholder)
+                    if (gok && sc->lmin != sc->lmax && G > sc->lmin) sc->lmax = G;
+                }
             }
         }
     }
@@ -942,6 +989,10 @@ __global__ __launch_bounds__(64) void k_cheb_bounds(int nsteps, int degree, int 
 
 // Finalisers of the preconditioned inner product gamma = r'z.
 // mode 0: initial (Krylov cg! set-up), mode 1: per-iteration update.
+// A negative or non-finite r'z is a breakdown: the preconditioner (or the
+// operator) is not positive definite, and Krylov's rNorm = sqrt(r'z) throws
+// (a DomainError in Julia).  The solve stops there and reports it.
+__device__ __forceinline__ bool gamma_breaks(double g) { return !(g >= 0.0) || !isfinite(g); }
 __device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
     if (mode == 0) {
         const double rn = sqrt(g);
@@ -958,9 +1009,25 @@ __device__ __forceinline__ void fin_gamma(Scalars* sc, double g, int mode) {
         sc->pAp = 0.0;
         sc->alpha = 0.0;  // the deferred matvec's first x += alpha p (p = 0) adds nothing
         sc->gpend = 0;
+        sc->done_defer = 0;
+        sc->breakdown = 0;
         sc->solved = (g == 0.0) || (rn <= sc->eps_stop);
         sc->done = sc->solved;
+        if (gamma_breaks(g)) {
+            sc->breakdown = 1;
+            sc->breakdown_value = g;
+            sc->solved = 0;
+            sc->done = 1;
+        }
     } else {
+        if (gamma_breaks(g)) {
+            sc->breakdown = 1;
+            sc->breakdown_value = g;
+            sc->rNorm = sqrt(g);
+            sc->solved = 0;
+            sc->done = 1;
+            return;
+        }
         const double rn = sqrt(g);
         sc->rNorm = rn;
         const int solved = (rn <= sc->eps_stop) || (rn + 1.0 <= 1.0);
@@ -1107,13 +1174,28 @@ __global__ __launch_bounds__(256) void k_cheb_deg1(int nx, int ny, int R, const 
 // the previous iteration left its r'z partials (pa, nba) without a finishing
 // kernel (gpend set).  Every workgroup sums them as k_finish would and takes
 // the same stop test and beta (Krylov's cg!, as in fin_gamma); workgroup 0
-// stores rNorm, solved, iter, tired, done, beta and pNorm2.  gamma, which the
-// other workgroups read, is stored by the update kernel that follows.
+// stores rNorm, solved, iter, tired, beta and pNorm2, and the stop decision in
+// done_defer, never in done: the workgroups of this launch test done on
+// entry, so a write of done here would let a late workgroup skip its tile.
+// The kernel that follows (the update, or the finest down-leg with the update
+// fused in) publishes done.  gamma, which the other workgroups read, is also
+// stored by that kernel.  Returns 0 (go on), 1 (converged: take the pending
+// x += alpha p_old and stop) or 2 (breakdown: stop, x untouched).
 template <bool DEFER>
-__device__ __forceinline__ bool deferred_gamma(Scalars* sc, const double* __restrict__ pa, int nba, double& beta) {
-    if (!DEFER || !sc->gpend) return false;
+__device__ __forceinline__ int deferred_gamma(Scalars* sc, const double* __restrict__ pa, int nba, double& beta) {
+    if (!DEFER || !sc->gpend) return 0;
     const double g = sum_partials(pa, nba);
     const double rn = sqrt(g);
+    if (gamma_breaks(g)) {
+        if (lead_thread()) {
+            sc->rNorm = rn;
+            sc->solved = 0;
+            sc->breakdown = 1;
+            sc->breakdown_value = g;
+            sc->done_defer = 1;
+        }
+        return 2;
+    }
     const int solved = (rn <= sc->eps_stop) || (rn + 1.0 <= 1.0);
     const double b = g / sc->gamma;
     if (lead_thread()) {
@@ -1126,10 +1208,54 @@ __device__ __forceinline__ bool deferred_gamma(Scalars* sc, const double* __rest
         const int it = sc->iter + 1;
         sc->iter = it;
         sc->tired = it >= sc->itmax;
-        sc->done = solved || it >= sc->itmax;
+        sc->done_defer = solved || it >= sc->itmax;
     }
     beta = b;
-    return solved;
+    return solved ? 1 : 0;
+}
+
+// The first kernel after a deferred CG matvec publishes the matvec's stop
+// decision as done (workgroup 0) and every one of its workgroups returns.
+// done_defer was written by the previous launch, so all workgroups of this
+// one read the same value.
+__device__ __forceinline__ bool publish_deferred_stop(Scalars* sc) {
+    if (!sc->done_defer) return false;
+    if (lead_thread()) sc->done = 1;
+    return true;
+}
+
+// Prologue of the kernel that takes the CG update after a deferred matvec
+// (k_cg_update<DEFER>, k_cg_update_flat, the finest down-leg with the update
+// fused in).  Every workgroup publishes the matvec's stop decision, or sums
+// the p'Ap (pb, nbb) and r'z (pa, nba) partials as k_finish would and takes
+// Krylov's zero-curvature test and alpha = r'z / p'Ap; workgroup 0 stores
+// them.  A non-finite p'Ap is a breakdown.  Returns true when the kernel
+// must return (stopped, zero curvature or breakdown).
+__device__ __forceinline__ bool deferred_update_prologue(Scalars* sc, const double* __restrict__ pa, int nba,
+                                                         const double* __restrict__ pb, int nbb, double& alpha) {
+    if (publish_deferred_stop(sc)) return true;
+    const double tb = thread_sum(pb, nbb), ta = thread_sum(pa, nba);
+    const double pAp = block_total(tb);
+    const double g = block_total(ta);
+    const double pn2 = sc->pNorm2;
+    const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
+    const bool bad = !isfinite(pAp);
+    alpha = g / pAp;
+    if (lead_thread()) {
+        sc->pAp = pAp;
+        sc->gamma = g;
+        if (bad) {
+            sc->breakdown = 2;
+            sc->breakdown_value = pAp;
+            sc->done = 1;
+        } else if (zc) {
+            sc->zero_curv = 1;
+            sc->done = 1;
+        } else {
+            sc->alpha = alpha;
+        }
+    }
+    return zc || bad;
 }
 
 // DEFER: x (the accumulated correction) takes the previous iteration's
@@ -1150,8 +1276,8 @@ __global__ __launch_bounds__(256) void k_cg_matvec(int nx, int ny, int R, const 
     virtual_block(vbx, vby);
     double b = sc->beta;
     const T alpha = (T)sc->alpha;  // the previous iteration's step (DEFER)
-    if (deferred_gamma<DEFER>(sc, pa, nba, b)) {
-        walk(nx, ny, R, vbx, vby, [&](size_t c) { x[c] = x[c] + alpha * p_old[c]; });
+    if (const int stop = deferred_gamma<DEFER>(sc, pa, nba, b)) {
+        if (stop == 1) walk(nx, ny, R, vbx, vby, [&](size_t c) { x[c] = x[c] + alpha * p_old[c]; });
         return;
     }
     const T beta = (T)b;
@@ -1194,11 +1320,11 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
         xv.y = xv.y + alpha * po.y;
         st2<NT>(x + c, xv);
     };
-    if (deferred_gamma<DEFER>(sc, pa, nba, beta)) {
-        // march2's cells of this wave: columns ix, ix + 1 of rows iy0 .. iy1 - 1
+    if (const int stop = deferred_gamma<DEFER>(sc, pa, nba, beta)) {
+        // converged: march2's cells of this wave, columns ix, ix + 1 of rows iy0 .. iy1 - 1
         const int ix = strip * 2 * WAVE + 2 * (int)threadIdx.x;
         const int iy1 = min(iy0 + R, ny);
-        if (ix < nx)
+        if (stop == 1 && ix < nx)
             for (int iy = iy0; iy < iy1; ++iy) x_step(cidx(ix, iy, nx));
         return;
     }
@@ -1235,23 +1361,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int nx, int ny, int R, const 
     virtual_block(vbx, vby);
     double al = sc->alpha;
     if constexpr (DEFER) {
-        const double tb = thread_sum(pb, nbb), ta = thread_sum(pa, nba);
-        const double pAp = block_total(tb);
-        const double g = block_total(ta);
-        const double pn2 = sc->pNorm2;
-        const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
-        al = g / pAp;
-        if (lead_thread()) {
-            sc->pAp = pAp;
-            sc->gamma = g;
-            if (zc) {
-                sc->zero_curv = 1;
-                sc->done = 1;
-            } else {
-                sc->alpha = al;
-            }
-        }
-        if (zc) return;
+        if (deferred_update_prologue(sc, pa, nba, pb, nbb, al)) return;
     }
     const T alpha = (T)al;
     double acc = 0.0;
@@ -1281,23 +1391,8 @@ __global__ __launch_bounds__(256) void k_cg_update_flat(size_t n, const double* 
                                                         const double* __restrict__ pa, int nba,
                                                         const double* __restrict__ pb, int nbb) {
     if (sc->done) return;
-    const double tb = thread_sum(pb, nbb), ta = thread_sum(pa, nba);
-    const double pAp = block_total(tb);
-    const double g = block_total(ta);
-    const double pn2 = sc->pNorm2;
-    const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
-    const double alpha = g / pAp;
-    if (lead_thread()) {
-        sc->pAp = pAp;
-        sc->gamma = g;
-        if (zc) {
-            sc->zero_curv = 1;
-            sc->done = 1;
-        } else {
-            sc->alpha = alpha;
-        }
-    }
-    if (zc) return;
+    double alpha;
+    if (deferred_update_prologue(sc, pa, nba, pb, nbb, alpha)) return;
     const size_t tid = threadIdx.x + (size_t)threadIdx.y * WAVE;
     for (size_t i = ((size_t)blockIdx.x * (WAVE * WAVES) + tid) * 2; i < n; i += (size_t)gridDim.x * 2 * WAVE * WAVES) {
         if (i + 1 < n) {
@@ -1544,6 +1639,47 @@ __global__ __launch_bounds__(256) void k_ir_residual(int nx, int ny, int R, cons
                   });
     double v[1] = {acc};
     block_partial<1, false>(v, partials);
+}
+
+// Gershgorin bound of D^-1 A (D = dg): per cell 1 + sum_j |a_ij| / d_i over
+// the couplings the operator applies (none past the grid's edges), max over
+// the grid (NaN propagates: a zero diagonal with zero couplings gives 0/0).
+template <class T>
+__global__ __launch_bounds__(256) void k_gersh(int nx, int ny, const T* __restrict__ dg, const T* __restrict__ aE,
+                                               const T* __restrict__ aN, double* partials) {
+    const int ix = blockIdx.x * WAVE + threadIdx.x, iy = blockIdx.y * WAVES + threadIdx.y;
+    double m = 0.0;
+    if (ix < nx && iy < ny) {
+        const size_t c = cidx(ix, iy, nx);
+        double s = 0.0;
+        if (ix < nx - 1) s += fabs((double)aE[c]);
+        if (ix > 0) s += fabs((double)aE[c - 1]);
+        if (iy < ny - 1) s += fabs((double)aN[c]);
+        if (iy > 0) s += fabs((double)aN[c - nx]);
+        m = 1.0 + s / fabs((double)dg[c]);
+    }
+    double v[1] = {m};
+    block_partial<1, true>(v, partials);
+}
+
+// The FP64 residual of a returned head h = x + x0 (x0 may be null), formed
+// as k_cg_finish forms it: partials of ||b - A h||^2 and ||b||^2.
+__global__ __launch_bounds__(256) void k_true_resid(int nx, int ny, int R, const double* __restrict__ aP,
+                                                    const double* __restrict__ aE, const double* __restrict__ aN,
+                                                    const double* __restrict__ b, const double* __restrict__ x,
+                                                    const double* __restrict__ x0, double* partials) {
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    double ar = 0.0, ab = 0.0;
+    march<double>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> double { return x0 ? x[k] + x0[k] : x[k]; },
+                  [&](size_t c, double Au, double) {
+                      const double bc = b[c];
+                      const double r = bc - Au;
+                      ar += r * r;
+                      ab += bc * bc;
+                  });
+    double v[2] = {ar, ab};
+    block_partial<2, false>(v, partials);
 }
 
 __global__ __launch_bounds__(256) void k_ir_correct(size_t n, double* __restrict__ x, const float* __restrict__ d) {
@@ -2031,23 +2167,7 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
     if (sc->done) return;
     double alpha = 0.0;
     if constexpr (UPD) {
-        const double tb = thread_sum(U.pb, U.nbb), ta = thread_sum(U.pa, U.nba);
-        const double pAp = block_total(tb);
-        const double g = block_total(ta);
-        const double pn2 = sc->pNorm2;
-        const bool zc = pAp <= EPS64 * pn2 && fabs(pAp) <= EPS64 * pn2;
-        alpha = g / pAp;
-        if (lead_thread()) {
-            U.sc->pAp = pAp;
-            U.sc->gamma = g;
-            if (zc) {
-                U.sc->zero_curv = 1;
-                U.sc->done = 1;
-            } else {
-                U.sc->alpha = alpha;
-            }
-        }
-        if (zc) return;
+        if (deferred_update_prologue(U.sc, U.pa, U.nba, U.pb, U.nbb, alpha)) return;
     }
     // b at cell pair k, with the fused update applied (r - alpha Ap, as the update computes it)
     auto upd = [&](TB2 v, size_t k) -> TB2 {
@@ -3220,6 +3340,8 @@ enum FinMode {
     FIN_GAMMA_NN = 12,   // gamma = r'z after the learned preconditioner (no convergence test)
     FIN_NN_LS_PC = 13,   // learned preconditioner: gamma_pc = e'r / e'Ae (0 when e'Ae <= 0)
     FIN_NN_SCALE_PC = 14,  // learned preconditioner: image scale of r
+    FIN_GERSH = 15,      // max of the Gershgorin row bounds of D^-1 A
+    FIN_TRUE_RES = 16,   // ||b - A h||, ||b|| of a returned head
 };
 
 // One workgroup: sums (or maximises) the nb partials of the preceding
@@ -3232,8 +3354,8 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
          mode == FIN_NN_LS_PC || mode == FIN_NN_SCALE_PC) &&
         sc->done)
         return;
-    const bool is_max = mode == FIN_PICARD;
-    const int nv = (is_max || mode == FIN_NN_LS || mode == FIN_NN_LS_PC) ? 2 : 1;
+    const bool is_max = mode == FIN_PICARD || mode == FIN_GERSH;
+    const int nv = (mode == FIN_PICARD || mode == FIN_NN_LS || mode == FIN_NN_LS_PC || mode == FIN_TRUE_RES) ? 2 : 1;
     const int tid = threadIdx.x + threadIdx.y * WAVE;
     double acc[2] = {0.0, 0.0};
     // each thread takes j = tid, tid + 256, ... in that order; the loads of 16
@@ -3270,7 +3392,11 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
         case FIN_GAMMA_ITER: fin_gamma(sc, v0, 1); break;
         case FIN_PAP:
             sc->pAp = v0;
-            if (v0 <= EPS64 * sc->pNorm2 && fabs(v0) <= EPS64 * sc->pNorm2) {
+            if (!isfinite(v0)) {
+                sc->breakdown = 2;
+                sc->breakdown_value = v0;
+                sc->done = 1;
+            } else if (v0 <= EPS64 * sc->pNorm2 && fabs(v0) <= EPS64 * sc->pNorm2) {
                 sc->zero_curv = 1;
                 sc->done = 1;
             } else {
@@ -3327,6 +3453,11 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
             sc->nn_fallback = ok ? 0 : 1;
             break;
         }
+        case FIN_GERSH: sc->gersh = v0; break;
+        case FIN_TRUE_RES:
+            sc->tres = sqrt(v0);
+            sc->tres_b = sqrt(v1);
+            break;
         case FIN_PICARD:
             sc->dmax = v0;
             sc->hmax = v1;
@@ -3505,24 +3636,53 @@ void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T*
     hipLaunchKernelGGL(k_matvec<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, x, y);
 }
 template <class T>
-void lanczos_init(const Geom& G, const T* rhs, T* r, T* p0, double* partials, Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_lz_init<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, rhs, r, p0, partials, sc);
+void lanczos_init(const Geom& G, const T* rhs, const T* dg, int sym, T* r, T* p0, double* partials, Scalars* sc,
+                  hipStream_t st) {
+    if (sym)
+        hipLaunchKernelGGL((k_lz_init<T, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, rhs, dg, r, p0, partials,
+                           sc);
+    else
+        hipLaunchKernelGGL((k_lz_init<T, false>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, rhs, dg, r, p0, partials,
+                           sc);
     finish(G, partials, FIN_LZ_GAMMA0, 0, sc, st);
 }
 template <class T>
-void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* r, const T* p_old,
-                    T* p_new, T* Ap, int k, double* partials, Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_lz_step_a<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, r, p_old, p_new, Ap,
-                       k, partials, sc);
+void lanczos_step_a(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, int sym, const T* r,
+                    const T* p_old, T* p_new, T* Ap, int k, double* partials, Scalars* sc, hipStream_t st) {
+    if (sym)
+        hipLaunchKernelGGL((k_lz_step_a<T, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, r, p_old,
+                           p_new, Ap, k, partials, sc);
+    else
+        hipLaunchKernelGGL((k_lz_step_a<T, false>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, dg, r,
+                           p_old, p_new, Ap, k, partials, sc);
     finish(G, partials, FIN_LZ_ALPHA, k, sc, st);
 }
 template <class T>
-void lanczos_step_b(const Geom& G, const T* Ap, T* r, int k, double* partials, Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_lz_step_b<T>, G.grid, G.block, 0, st, G.nx, G.ny, G.R, Ap, r, k, partials, sc);
+void lanczos_step_b(const Geom& G, const T* Ap, const T* dg, int sym, T* r, int k, double* partials, Scalars* sc,
+                    hipStream_t st) {
+    if (sym)
+        hipLaunchKernelGGL((k_lz_step_b<T, true>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, Ap, dg, r, k, partials,
+                           sc);
+    else
+        hipLaunchKernelGGL((k_lz_step_b<T, false>), G.grid, G.block, 0, st, G.nx, G.ny, G.R, Ap, dg, r, k, partials,
+                           sc);
     finish(G, partials, FIN_LZ_BETA, k, sc, st);
 }
-void cheb_bounds(int nsteps, int degree, int use_estimate, Scalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_cheb_bounds, dim3(1), dim3(64), 0, st, nsteps, degree, use_estimate, sc);
+void cheb_bounds(int nsteps, int degree, int use_estimate, int mode, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_cheb_bounds, dim3(1), dim3(64), 0, st, nsteps, degree, use_estimate, mode, sc);
+}
+template <class T>
+void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* partials, Scalars* sc,
+                hipStream_t st) {
+    const dim3 grid = cells_grid(G.nx, G.ny);
+    hipLaunchKernelGGL(k_gersh<T>, grid, dim3(WAVE, WAVES), 0, st, G.nx, G.ny, dg, aE, aN, partials);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
+                       (int)FIN_GERSH, 0, sc);
+}
+void true_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
+                   const double* x, const double* x0, double* partials, Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials);
+    finish(G, partials, FIN_TRUE_RES, 0, sc, st);
 }
 template <class T>
 void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* b, const T* x0, T* r, T* z,
@@ -3948,10 +4108,12 @@ void cg_finish(int n, const double* x, const double* h_prev, double* h_new, doub
     template void assemble<T>(const Geom&, const Phys&, Fields, int, double, T*, T*, T*, T*, hipStream_t,         \
                               double*);                                                                           \
     template void matvec<T>(const Geom&, const T*, const T*, const T*, const T*, T*, hipStream_t);                \
-    template void lanczos_init<T>(const Geom&, const T*, T*, T*, double*, Scalars*, hipStream_t);                 \
-    template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, T*,  \
-                                    T*, int, double*, Scalars*, hipStream_t);                                     \
-    template void lanczos_step_b<T>(const Geom&, const T*, T*, int, double*, Scalars*, hipStream_t);              \
+    template void lanczos_init<T>(const Geom&, const T*, const T*, int, T*, T*, double*, Scalars*, hipStream_t);  \
+    template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, int, const T*, const T*, \
+                                    T*, T*, int, double*, Scalars*, hipStream_t);                                 \
+    template void lanczos_step_b<T>(const Geom&, const T*, const T*, int, T*, int, double*, Scalars*,             \
+                                    hipStream_t);                                                                 \
+    template void gershgorin<T>(const Geom&, const T*, const T*, const T*, double*, Scalars*, hipStream_t);       \
     template void cg_init<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, T*, T*, T*,  \
                              T*, int, double*, Scalars*, hipStream_t);                                            \
     template void cheb_pass<T>(const Geom&, const T*, const T*, const T*, const T*, const T*, const T*, const T*, \

@@ -21,10 +21,12 @@ FIELD_NAMES = ["h", "pw", "po", "b", "beta", "abs_ub", "mdot", "shear", "potenti
 FIELD_ID = {n: i for i, n in enumerate(FIELD_NAMES)}
 N_CENTER, N_XFACE = 21, 8
 
+ABI_VERSION = 2  # SHK_ABI_VERSION of the header these structures follow
+
 # Every exported entry point of include/shakti_hip.h; a load fails loudly if one is missing.
 EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "shk_set_solver_cfg", "shk_set_params",
            "shk_set_model", "shk_upload", "shk_download", "shk_set_initial_conditions", "shk_solve_elliptic",
-           "shk_solve_parabolic", "shk_update_derived", "shk_picard_iteration", "shk_picard_loop",
+           "shk_solve_parabolic", "shk_update_derived", "shk_picard_iteration", "shk_picard_loop", "shk_solve_log",
            "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
            "shk_apply_operator", "shk_apply_precond", "shk_download_system", "shk_time_kernel", "shk_profile_enable", "shk_profile_read",
            "shk_profile_reset"]
@@ -55,14 +57,19 @@ class ShkModel(C.Structure):
 class ShkSolverCfg(C.Structure):
     _fields_ = [("precond", C.c_int32), ("cheb_degree", C.c_int32), ("cheb_nsteps", C.c_int32), ("form", C.c_int32),
                 ("itmax", C.c_int32), ("check_every", C.c_int32), ("atol", C.c_double), ("rtol", C.c_double),
-                ("inner_rtol", C.c_double)]
+                ("inner_rtol", C.c_double), ("cheb_bounds", C.c_int32), ("reserved", C.c_int32)]
+
+
+CHEB_BOUNDS = {"gershgorin": 0, "reference": 1}  # SHK_CHEB_BOUNDS_*
 
 
 class ShkSolveStats(C.Structure):
     _fields_ = [("niter", C.c_int32), ("solved", C.c_int32), ("zero_curvature", C.c_int32),
                 ("bounds_warning", C.c_int32), ("refinements", C.c_int32), ("fallback", C.c_int32),
                 ("rnorm0", C.c_double), ("rnorm", C.c_double), ("lambda_min", C.c_double),
-                ("lambda_max", C.c_double)]
+                ("lambda_max", C.c_double), ("tired", C.c_int32), ("breakdown", C.c_int32),
+                ("breakdown_value", C.c_double), ("lz_lambda_min", C.c_double), ("lz_lambda_max", C.c_double),
+                ("gershgorin", C.c_double), ("true_residual", C.c_double)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -107,6 +114,9 @@ def load() -> C.CDLL:
             raise ShaktiHIPError(f"{LIB_PATH} does not export {name}")
     vp, dp, i32, i64, sz, dbl = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int64, C.c_size_t, C.c_double
     L.shk_abi_version.restype = i32
+    if L.shk_abi_version() != ABI_VERSION:
+        raise ShaktiHIPError(f"{LIB_PATH} has ABI version {L.shk_abi_version()}, this binding expects {ABI_VERSION}; "
+                             "rebuild it")
     L.shk_last_error.argtypes = [vp]
     L.shk_last_error.restype = C.c_char_p
     L.shk_create.argtypes = [C.POINTER(ShkGrid), C.POINTER(ShkParams), C.POINTER(ShkModel),
@@ -123,6 +133,7 @@ def load() -> C.CDLL:
     L.shk_update_derived.argtypes = [vp]
     L.shk_picard_iteration.argtypes = [vp, dbl, C.POINTER(ShkSolveStats)]
     L.shk_picard_loop.argtypes = [vp, i32, dbl, i32, dbl, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+    L.shk_solve_log.argtypes = [vp, C.POINTER(ShkSolveStats), i32, C.POINTER(i32)]
     L.shk_parabolic_step.argtypes = [vp, dbl, C.POINTER(i64)]
     L.shk_step_b.argtypes = [vp, dbl]
     L.shk_set_moulin_field.argtypes = [vp, dp, sz]

@@ -225,6 +225,23 @@ class State:
     def profile_reset(self) -> None:
         L.check(L.load().shk_profile_reset(self._ctx), self._ctx, "shk_profile_reset")
 
+    def download_system(self) -> dict:
+        """The last assembly in the reference's five-array layout
+        (shk_download_system): aP, aE, aW, aN, aS and rhs, each (nx, ny)."""
+        shape = (self.grid.nx, self.grid.ny)
+        arrs = [np.zeros(shape, order="F") for _ in range(6)]
+        L.check(L.load().shk_download_system(self._ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]),
+                self._ctx, "shk_download_system")
+        return dict(zip(["aP", "aE", "aW", "aN", "aS", "rhs"], arrs))
+
+    def solve_log(self, max_entries: int = 65536) -> list[dict]:
+        """Statistics of every linear solve since the last call, oldest first
+        (shk_solve_log): those inside the Picard loop and whole steps too."""
+        buf = (L.ShkSolveStats * max_entries)()
+        k = C.c_int32()
+        L.check(L.load().shk_solve_log(self._ctx, buf, max_entries, C.byref(k)), self._ctx, "shk_solve_log")
+        return [buf[i].as_dict() for i in range(k.value)]
+
     # internal: push the per-call choices before an entry point
     def _bind(self, p: ModelParameters | None = None, kfs: AbstractKFaceScheme | None = None,
               sl: AbstractSlidingLaw | None = None, gap_scheme: int | None = None):
@@ -281,6 +298,8 @@ class SparseAssembledLinearSystem(AbstractLinearSystem):
 
 @dataclass
 class _Stats:
+    """Krylov's ws.stats of the last solve, plus what the library measures
+    beside it (breakdown, bound estimate, FP64 residual of the result)."""
     niter: int = 0
     solved: bool = False
     zero_curvature: bool = False
@@ -288,6 +307,8 @@ class _Stats:
     rnorm: float = 0.0
     refinements: int = 0
     fallback: int = 0  # bit 0: warm start rejected; bit 1: learned preconditioner stalled
+    tired: bool = False
+    true_residual: float = float("nan")  # ||b - A h|| / ||b|| of the returned head (FP64)
 
 
 class _Workspace:
@@ -342,7 +363,7 @@ class CGIterativeSolver(AbstractIterativeSolver):
                  chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
                  rtol: float = SQRT_EPS, itmax: int = 0, form: str = "fp64", check_every: int = 8,
                  inner_rtol: float = 1e-3, multigrid: bool = False, network=None,
-                 learned_precond_every: int = 0, stall_ratio: float = 1.0):
+                 learned_precond_every: int = 0, stall_ratio: float = 1.0, chebyshev_bounds: str = "gershgorin"):
         if system is SparseAssembledLinearSystem:
             raise L.ShaktiHIPError("CGIterativeSolver(g, SparseAssembledLinearSystem) is CPU-only; use "
                                    "HIPMatrixFreeLinearSystem on the GPU.")
@@ -357,6 +378,8 @@ class CGIterativeSolver(AbstractIterativeSolver):
                 f"ChebyshevPreconditioner: nsteps_estimate must be >= 2 (got {chebyshev_nsteps_estimate})")
         if form not in FORMS:
             raise L.ShaktiHIPError(f"unknown solver form {form!r}")
+        if chebyshev_bounds not in L.CHEB_BOUNDS:
+            raise L.ShaktiHIPError(f"chebyshev_bounds must be 'gershgorin' or 'reference' (got {chebyshev_bounds!r})")
         if multigrid and chebyshev_degree is not None:
             raise L.ShaktiHIPError("multigrid=True replaces chebyshev_degree: give one of them")
         if (form == "learned") != (network is not None):
@@ -377,6 +400,7 @@ class CGIterativeSolver(AbstractIterativeSolver):
         self.form = form
         self.check_every = int(check_every)
         self.inner_rtol = float(inner_rtol)
+        self.chebyshev_bounds = chebyshev_bounds
         self.ws = _Workspace()
         self.precond = None if chebyshev_degree is None else ChebyshevPreconditioner(chebyshev_degree,
                                                                                      chebyshev_nsteps_estimate)
@@ -388,7 +412,7 @@ class CGIterativeSolver(AbstractIterativeSolver):
     def _record(self, st: L.ShkSolveStats):
         self.last_stats = st
         self.ws.stats = _Stats(st.niter, bool(st.solved), bool(st.zero_curvature), st.rnorm0, st.rnorm,
-                               st.refinements, int(st.fallback))
+                               st.refinements, int(st.fallback), bool(st.tired), st.true_residual)
         if self.precond is not None:
             self.precond.lambda_min, self.precond.lambda_max = st.lambda_min, st.lambda_max
             if st.bounds_warning:
@@ -399,11 +423,11 @@ class CGIterativeSolver(AbstractIterativeSolver):
 
 def _solver_cfg(ls: CGIterativeSolver | None) -> L.ShkSolverCfg:
     if ls is None:
-        return L.ShkSolverCfg(0, 4, 15, 0, 0, 8, SQRT_EPS, SQRT_EPS, 1e-3)
+        return L.ShkSolverCfg(0, 4, 15, 0, 0, 8, SQRT_EPS, SQRT_EPS, 1e-3, 0, 0)
     cheb = ls.chebyshev_degree is not None
     precond = 2 if ls.multigrid else (1 if cheb else 0)
     return L.ShkSolverCfg(precond, ls.chebyshev_degree or 1, ls.nsteps_estimate, FORMS[ls.form], ls.itmax,
-                          ls.check_every, ls.atol, ls.rtol, ls.inner_rtol)
+                          ls.check_every, ls.atol, ls.rtol, ls.inner_rtol, L.CHEB_BOUNDS[ls.chebyshev_bounds], 0)
 
 
 def _bind_solver(s: State, ls: CGIterativeSolver):

@@ -35,3 +35,16 @@ def field_errors(hs, os_, names):
     for n in names:
         out[n] = rel_err(hs.get(n), getattr(os_, n))
     return out
+
+
+def true_residuals(O, hs, h_prev):
+    """The FP64 residual of the head a solve returned, recomputed on the host
+    with the oracle's five-array operator on the library's own assembly:
+    (||b - A h|| / ||b||, ||b - A h|| / ||b - A h_prev||)."""
+    sysd = hs.download_system()
+    co = [sysd[k] for k in ("aP", "aE", "aW", "aN", "aS")]
+    b = sysd["rhs"]
+    r = b - O.matvec(co, hs.get("h"))
+    r0 = b - O.matvec(co, h_prev)
+    nr = float(np.linalg.norm(r))
+    return nr / float(np.linalg.norm(b)), nr / float(np.linalg.norm(r0))

@@ -49,13 +49,32 @@ def test_unet_parameter_layout(hip):
     assert lib.shk_unet_layer_info(15, None, None, None) == -1
 
 
-def test_abi_version_and_struct_sizes(hip):
+def test_abi_version_and_struct_layouts(hip, tmp_path):
+    """The ctypes structures match the header's: every field's offset and each
+    struct's size as gcc lays them out from include/shakti_hip.h."""
+    import subprocess
     from shakti_hip import _lib as L
-    assert L.load().shk_abi_version() == 1
-    assert C.sizeof(L.ShkGrid) == 8 + 6 * 8
-    assert C.sizeof(L.ShkParams) == 16 * 8
-    assert C.sizeof(L.ShkSolverCfg) == 6 * 4 + 3 * 8
-    assert C.sizeof(L.ShkSolveStats) == 6 * 4 + 4 * 8
+    assert L.load().shk_abi_version() == L.ABI_VERSION
+    structs = {"shk_grid": L.ShkGrid, "shk_params": L.ShkParams, "shk_model": L.ShkModel,
+               "shk_solver_cfg": L.ShkSolverCfg, "shk_solve_stats": L.ShkSolveStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "shakti_hip.h"', "int main(void) {",
+             '  printf("ABI %d\\n", SHK_ABI_VERSION);']
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert out[0] == f"ABI {L.ABI_VERSION}"
+    got = {(l.split()[0], l.split()[1]): int(l.split()[2]) for l in out[1:] if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_null_and_bad_arguments_are_errors(hip):

@@ -109,23 +109,38 @@ def test_single_solve_vs_cholesky_6x6(oracle, hip, precond):
     assert rel_err(hs.get("h"), os_.h) < 1e-12
 
 
-def test_chebyshev_bounds_and_iterations_match_oracle_cg(oracle, hip):
+@pytest.mark.parametrize("bounds_mode", ["reference", "gershgorin"])
+def test_chebyshev_bounds_and_iterations_match_oracle_cg(oracle, hip, bounds_mode):
     """Same Lanczos bounds (to rounding) and the same CG iteration count as the
-    oracle's restated Krylov CG with the Chebyshev preconditioner."""
+    oracle's restated Krylov CG with the Chebyshev preconditioner, with the
+    reference's bounds and with the Gershgorin lambda_max."""
     from shakti_hip import synthetic as S
     gg, raw = S.glacier(128, 48, bed_amp=10.0, seed=2)
     g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
     g, p, m, os_ = oracle_state(oracle, g_args, raw, kfs=1)
     hg, hp, hs = hip_state(hip, g_args, raw)
     co = oracle.assemble_mfls(os_, g, p, 1)
-    x_ref, st_ref, bounds = oracle.cg_solve(co, co[5], os_.h, precond="chebyshev", atol=0.0, rtol=1e-10)
+    gersh = bounds_mode == "gershgorin"
+    x_ref, st_ref, bounds = oracle.cg_solve(co, co[5], os_.h, precond="chebyshev", atol=0.0, rtol=1e-10,
+                                            gershgorin=gersh)
+    est = (oracle.estimate_eigenvalue_bounds_d if gersh else oracle.estimate_eigenvalue_bounds)(co, co[5])
     ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-10,
-                               check_every=1)
+                               check_every=1, chebyshev_bounds=bounds_mode)
     hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+    st = ls.last_stats
     assert abs(ls.precond.lambda_min - bounds[0]) <= 1e-6 * abs(bounds[0])
     assert abs(ls.precond.lambda_max - bounds[1]) <= 1e-6 * abs(bounds[1])
+    assert abs(st.lz_lambda_min - est[0]) <= 1e-6 * abs(est[0])
+    assert abs(st.lz_lambda_max - est[1]) <= 1e-6 * abs(est[1])
+    if gersh:
+        # the Gershgorin bound is a max of per-row sums: the same bits as the oracle's
+        assert st.gershgorin == oracle.gershgorin(co) == bounds[1]
     assert abs(ls.ws.stats.niter - st_ref.niter) <= 1, (ls.ws.stats.niter, st_ref.niter)
     assert rel_err(hs.get("h"), x_ref) < 1e-9
+    # the FP64 residual of the returned head, recomputed by the oracle's operator
+    r = co[5] - oracle.matvec(co, hs.get("h"))
+    tr = np.linalg.norm(r) / np.linalg.norm(co[5])
+    assert abs(ls.ws.stats.true_residual - tr) <= 1e-6 * tr + 1e-300
 
 
 def test_picard_loop_6x6_matches_cholesky_oracle(oracle, hip):

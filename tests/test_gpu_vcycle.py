@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from helpers import hip_state
+from helpers import hip_state, true_residuals
 from vcycle_restate import VCycle
 
 pytestmark = pytest.mark.gpu
@@ -106,28 +106,39 @@ def test_parabolic_solve_after_an_elliptic_one_matches_a_fresh_context(hip):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-def test_fp32_z_gives_the_same_iterates(hip, monkeypatch):
+@pytest.mark.parametrize("shape", [(2048, 1024), (8192, 2048)])
+def test_fp32_z_gives_the_same_iterates(oracle, hip, monkeypatch, shape):
     """FP64 CG with the V-cycle keeps z in FP32 where the deferred finish runs, the
-    finest level is BF16 and the matvec is the two-column kernel (2048 x 1024
-    here): the up-leg stores the FP32 values it computes and the matvec widens
-    them on load, so the CG iterates are those of the FP64-z path
-    (SHK_NO_DEFER=1, which finishes each reduction in its own kernel in the same
-    order), bit for bit: the same iteration count and the same head."""
+    finest level is BF16 and the matvec is the two-column kernel: the up-leg
+    stores the FP32 values it computes and the matvec widens them on load, so
+    the CG iterates are those of the FP64-z path (SHK_NO_DEFER=1, which
+    finishes each reduction in its own kernel in the same order), bit for bit:
+    the same iteration count and the same head.  At 8192 x 2048 the deferred
+    matvec runs many more workgroups than the chip holds at once, so a
+    workgroup that started after the stop decision of its own launch would
+    show here (that decision reaches the next launch only).  The head is also
+    held to its FP64 residual, recomputed on the host."""
     from shakti_hip import synthetic as S
-    gg, raw = S.glacier(2048, 1024, bed_amp=50.0, seed=5)
+    gg, raw = S.glacier(*shape, bed_amp=50.0, seed=5)
     g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
     out = []
     for no_defer in (False, True):
         if no_defer:
             monkeypatch.setenv("SHK_NO_DEFER", "1")
         hg, hp, hs = hip_state(hip, g_args, raw)
+        h0 = hs.get("h")
         ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=1e-10)
         hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
-        out.append((ls.last_stats.niter, hs.get("h")))
+        out.append((ls.last_stats.niter, hs.get("h"), ls.last_stats.true_residual, h0, hs))
         monkeypatch.delenv("SHK_NO_DEFER", raising=False)
     print(f"CG iterations: FP32 z {out[0][0]}, FP64 z {out[1][0]}", flush=True)
     assert out[0][0] == out[1][0] > 0
     assert np.array_equal(out[0][1], out[1][1])
+    rel_b, red = true_residuals(oracle, out[0][4], out[0][3])
+    print(f"||b - A h|| / ||b|| = {rel_b:.3e} (library {out[0][2]:.3e}), reduction from the start {red:.3e}",
+          flush=True)
+    assert abs(out[0][2] - rel_b) <= 1e-3 * rel_b
+    assert red <= 1e-8
 
 
 def test_non_temporal_cg_matvec_gives_the_same_iterates(hip, monkeypatch):
