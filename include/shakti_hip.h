@@ -232,6 +232,12 @@ int shk_step(shk_ctx* ctx, double dt, int32_t ieb_mode, double ieb_uniform, int3
  * nx*ny doubles. */
 int shk_assemble(shk_ctx* ctx, int32_t parabolic, double dt);
 int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host);
+/* ||b - A x||_2 and ||b||_2 of the last assembly in FP64 (norms[0], norms[1])
+ * for a host vector x of nx*ny doubles: the true residual of a head, formed
+ * with the operator's own apply and two sums of squares finished in a fixed
+ * order.  Krylov's cg! and the reference keep only the recursive residual;
+ * this is the check the solves report as stats.true_residual. */
+int shk_residual_norms(shk_ctx* ctx, const double* x_host, double* norms);
 /* One application z = M r of the multigrid preconditioner (the configured
  * one must be SHK_PRECOND_MULTIGRID) built from the last assembly, as the
  * solve applies it: one V-cycle of the FP32 flux-form hierarchy with an FP64

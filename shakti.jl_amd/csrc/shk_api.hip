@@ -1035,7 +1035,8 @@ int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_sol
     sst.rnorm = rnorm;
     {
         Prof pr(ctx, P_FIN, 6 * nb);
-        launch::true_residual(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, x, nullptr, ctx->partials, ctx->sc, ctx->st);
+        launch::residual_norms(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, x, nullptr, ctx->partials, &ctx->sc->tres,
+                               ctx->st);
     }
     rc = read_scalars(ctx);
     if (rc) return rc;
@@ -1156,7 +1157,8 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     {
         // the FP64 residual of the head about to be returned, x + x0
         Prof pr(ctx, P_FIN, 6 * nb);
-        launch::true_residual(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, x0, ctx->partials, ctx->sc, ctx->st);
+        launch::residual_norms(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, x0, ctx->partials, &ctx->sc->tres,
+                               ctx->st);
     }
     rc = read_scalars(ctx);
     if (rc) return rc;
@@ -1574,6 +1576,18 @@ int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host) {
         launch::matvec<double>(ctx->G, ctx->aP, ctx->aE, ctx->aN, ctx->d64, ctx->Ap, ctx->st);
     }
     HIPCHK(hipMemcpyAsync(y_host, ctx->Ap, ctx->n * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_residual_norms(shk_ctx* ctx, const double* x_host, double* norms) {
+    if (!ctx || !x_host || !norms) return fail(ctx, SHK_ERR_ARG, "null argument");
+    if (!ctx->have_system) return fail(ctx, SHK_ERR_STATE, "no assembled system");
+    HIPCHK(hipMemcpyAsync(ctx->d64, x_host, ctx->n * 8, hipMemcpyHostToDevice, ctx->st));
+    launch::residual_norms(ctx->G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->d64, nullptr, ctx->partials,
+                           &ctx->sc->tres, ctx->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(norms, &ctx->sc->tres, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return SHK_OK;
 }

@@ -167,9 +167,9 @@ template <class T>
 void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* partials, Scalars* sc,
                 hipStream_t st);
 // ||b - A h||_2 and ||b||_2 of the FP64 system for h = x + x0 (x0 may be
-// null), into sc->tres, sc->tres_b
-void true_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
-                   const double* x, const double* x0, double* partials, Scalars* sc, hipStream_t st);
+// null) into out[0], out[1] (device memory); partials: 2 x G.nblocks doubles
+void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
+                    const double* x, const double* x0, double* partials, double* out, hipStream_t st);
 
 // Preconditioned CG pieces (mode 0: set-up, mode 1: iteration).
 template <class T>

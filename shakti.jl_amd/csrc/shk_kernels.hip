@@ -3341,23 +3341,16 @@ enum FinMode {
     FIN_NN_LS_PC = 13,   // learned preconditioner: gamma_pc = e'r / e'Ae (0 when e'Ae <= 0)
     FIN_NN_SCALE_PC = 14,  // learned preconditioner: image scale of r
     FIN_GERSH = 15,      // max of the Gershgorin row bounds of D^-1 A
-    FIN_TRUE_RES = 16,   // ||b - A h||, ||b|| of a returned head
 };
 
-// One workgroup: sums (or maximises) the nb partials of the preceding
-// reduction kernel in a fixed order and updates the solver scalars.
-// Modes that follow an iteration kernel do nothing once the solve is done
-// (that kernel returned early and wrote no partials).
-__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partials, int nb, int mode, int k,
-                                                Scalars* sc) {
-    if ((mode == FIN_GAMMA_ITER || mode == FIN_PAP || mode == FIN_FCG_BETA || mode == FIN_GAMMA_NN ||
-         mode == FIN_NN_LS_PC || mode == FIN_NN_SCALE_PC) &&
-        sc->done)
-        return;
-    const bool is_max = mode == FIN_PICARD || mode == FIN_GERSH;
-    const int nv = (mode == FIN_PICARD || mode == FIN_NN_LS || mode == FIN_NN_LS_PC || mode == FIN_TRUE_RES) ? 2 : 1;
+// The nv (1 or 2) totals of nb per-workgroup partials (value i at
+// partials[i * nb + block]), summed or maximised in a fixed order by one
+// (64, 4) workgroup; valid in thread 0.
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partials, int nb, int nv, bool is_max,
+                                                double (&acc)[2]) {
     const int tid = threadIdx.x + threadIdx.y * WAVE;
-    double acc[2] = {0.0, 0.0};
+    acc[0] = 0.0;
+    acc[1] = 0.0;
     // each thread takes j = tid, tid + 256, ... in that order; the loads of 16
     // consecutive terms are issued together (the order of the sum is unchanged)
     constexpr int UF = 16, ST = WAVE * WAVES;
@@ -3385,7 +3378,35 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
         block_reduce<1, false>(one);
         acc[0] = one[0];
     }
-    if (tid != 0) return;
+}
+
+// Two norms from the partials of sums of squares (k_true_resid):
+// out[0] = sqrt(total 0), out[1] = sqrt(total 1).
+__global__ __launch_bounds__(256) void k_finish_norms(const double* __restrict__ partials, int nb,
+                                                      double* __restrict__ out) {
+    double acc[2];
+    reduce_partials(partials, nb, 2, false, acc);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        out[0] = sqrt(acc[0]);
+        out[1] = sqrt(acc[1]);
+    }
+}
+
+// One workgroup: sums (or maximises) the nb partials of the preceding
+// reduction kernel in a fixed order and updates the solver scalars.
+// Modes that follow an iteration kernel do nothing once the solve is done
+// (that kernel returned early and wrote no partials).
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partials, int nb, int mode, int k,
+                                                Scalars* sc) {
+    if ((mode == FIN_GAMMA_ITER || mode == FIN_PAP || mode == FIN_FCG_BETA || mode == FIN_GAMMA_NN ||
+         mode == FIN_NN_LS_PC || mode == FIN_NN_SCALE_PC) &&
+        sc->done)
+        return;
+    const bool is_max = mode == FIN_PICARD || mode == FIN_GERSH;
+    const int nv = (mode == FIN_PICARD || mode == FIN_NN_LS || mode == FIN_NN_LS_PC) ? 2 : 1;
+    double acc[2];
+    reduce_partials(partials, nb, nv, is_max, acc);
+    if (threadIdx.x != 0 || threadIdx.y != 0) return;
     const double v0 = acc[0], v1 = acc[1];
     switch (mode) {
         case FIN_GAMMA_INIT: fin_gamma(sc, v0, 0); break;
@@ -3454,10 +3475,6 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
             break;
         }
         case FIN_GERSH: sc->gersh = v0; break;
-        case FIN_TRUE_RES:
-            sc->tres = sqrt(v0);
-            sc->tres_b = sqrt(v1);
-            break;
         case FIN_PICARD:
             sc->dmax = v0;
             sc->hmax = v1;
@@ -3679,10 +3696,10 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                        (int)FIN_GERSH, 0, sc);
 }
-void true_residual(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
-                   const double* x, const double* x0, double* partials, Scalars* sc, hipStream_t st) {
+void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
+                    const double* x, const double* x0, double* partials, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials);
-    finish(G, partials, FIN_TRUE_RES, 0, sc, st);
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(WAVE, WAVES), 0, st, partials, G.nblocks, out);
 }
 template <class T>
 void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* b, const T* x0, T* r, T* z,

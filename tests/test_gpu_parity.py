@@ -95,6 +95,32 @@ def test_operator_apply_matches_oracle(oracle, hip):
     assert rel_err(y, y_ref) < 1e-15
 
 
+def test_residual_norms_match_oracle(oracle, hip):
+    """shk_residual_norms: ||b - A x|| and ||b|| of the last assembly, against
+    the oracle's five-array operator on the library's own assembly, for a
+    random x and for the Cholesky solution (the residual at the FP64 floor)."""
+    from shakti_hip import synthetic as S
+    import ctypes as C
+    from shakti_hip import _lib as L
+    gg, raw = S.glacier(300, 70, bed_amp=10.0, seed=5)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    hs._bind(p=hp, kfs=hip.Harmonic())
+    L.check(L.load().shk_assemble(hs.ctx, 0, 0.0), hs.ctx)
+    sysd = hs.download_system()
+    co = [sysd[k] for k in ("aP", "aE", "aW", "aN", "aS")]
+    b = sysd["rhs"]
+    x_chol = oracle.cholesky_solve(co + [b], b)
+    for x in (np.asfortranarray(np.random.default_rng(2).standard_normal(b.shape)), x_chol):
+        out = np.zeros(2)
+        L.check(L.load().shk_residual_norms(hs.ctx, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                            out.ctypes.data_as(C.POINTER(C.c_double))), hs.ctx)
+        r = b - oracle.matvec(co, x)
+        assert out[1] == pytest.approx(np.linalg.norm(b), rel=1e-13)
+        assert out[0] == pytest.approx(np.linalg.norm(r), rel=1e-6, abs=1e-14 * np.linalg.norm(b))
+
+
 @pytest.mark.parametrize("precond", ["jacobi", "chebyshev"])
 def test_single_solve_vs_cholesky_6x6(oracle, hip, precond):
     g_args, raw = _fixture(oracle)
