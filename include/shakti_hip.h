@@ -146,7 +146,9 @@ enum shk_field {
 
 typedef struct shk_ctx shk_ctx;
 
-/* Library identification: ABI version and the device the context runs on. */
+/* Library identification: ABI version and the device the context runs on.
+ * shk_last_error(NULL) returns the message of the last failed context-free
+ * call of the calling thread. */
 int32_t shk_abi_version(void);
 const char* shk_last_error(const shk_ctx* ctx);
 
@@ -238,6 +240,22 @@ int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host);
  * order.  Krylov's cg! and the reference keep only the recursive residual;
  * this is the check the solves report as stats.true_residual. */
 int shk_residual_norms(shk_ctx* ctx, const double* x_host, double* norms);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The result of shk_residual_norms, bit for bit, for an
+ * FP64 system given in device memory in the symmetric face layout (aP, aE,
+ * aN, b, x: nx*ny doubles each, ix fastest; aE couples (ix, iy) to
+ * (ix+1, iy), aN to (ix, iy+1)): ||b - A x||_2 into out_dev[0] and ||b||_2
+ * into out_dev[1] (device memory).  Enqueued on `stream` (a hipStream_t;
+ * NULL is the null stream) and asynchronous: no allocation, no copy, no
+ * synchronisation.  The per-workgroup partials go to one static device
+ * array of the library, 512 KiB per GPU, held until the process ends; grids
+ * that need more than 32768 workgroups (above about 2.5e8 cells) are
+ * rejected with SHK_ERR_ARG.  Every call on a GPU shares that array, so
+ * calls that overlap on one GPU race.  Takes no context: on failure
+ * shk_last_error(NULL) gives the message (per host thread). */
+int shk_residual_norms_noalloc(int32_t nx, int32_t ny, const double* aP, const double* aE, const double* aN,
+                               const double* b, const double* x, double* out_dev, void* stream);
 /* One application z = M r of the multigrid preconditioner (the configured
  * one must be SHK_PRECOND_MULTIGRID) built from the last assembly, as the
  * solve applies it: one V-cycle of the FP32 flux-form hierarchy with an FP64

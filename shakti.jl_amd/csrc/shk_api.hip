@@ -14,6 +14,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1223,13 +1224,53 @@ int dalloc(shk_ctx* ctx, T** p, size_t count) {
 
 }  // namespace
 
+// ================================================ allocation-free calls
+//
+// Device memory that shk_residual_norms_noalloc uses beyond its inputs and
+// outputs: the per-workgroup partials of its two sums of squares, 2 x
+// kNoallocMaxBlocks doubles (512 KiB), one copy per GPU for the life of the
+// process.  Host code never passes `scratch` itself to a kernel (on the host
+// the name stands for no device address): it takes the device address once
+// per device with hipGetSymbolAddress and uses that.
+constexpr int kNoallocMaxBlocks = 32768;
+static __device__ __align__(256) double scratch[2 * kNoallocMaxBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double* scratch_dev[kMaxDevices] = {};
+std::mutex scratch_mu;
+thread_local std::string noctx_error = "null context";
+
+// The device address of `scratch` on the current device, taken on the first
+// call on that device (with it current) and kept.
+int scratch_on_current_device(double** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        noctx_error = "no current HIP device (or more than 64 devices)";
+        return SHK_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            noctx_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return SHK_ERR_HIP;
+        }
+        scratch_dev[dev] = (double*)p;
+    }
+    *out = scratch_dev[dev];
+    return SHK_OK;
+}
+}  // namespace
+
 // ============================================================= C ABI
 
 extern "C" {
 
 int32_t shk_abi_version(void) { return SHK_ABI_VERSION; }
 
-const char* shk_last_error(const shk_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* shk_last_error(const shk_ctx* ctx) { return ctx ? ctx->err.c_str() : noctx_error.c_str(); }
 
 int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* model, const shk_solver_cfg* cfg,
                shk_ctx** out) {
@@ -1589,6 +1630,36 @@ int shk_residual_norms(shk_ctx* ctx, const double* x_host, double* norms) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(norms, &ctx->sc->tres, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
+    return SHK_OK;
+}
+
+int shk_residual_norms_noalloc(int32_t nx, int32_t ny, const double* aP, const double* aE, const double* aN,
+                               const double* b, const double* x, double* out_dev, void* stream) {
+    if (nx < 2 || ny < 2) {
+        noctx_error = "shk_residual_norms_noalloc: nx and ny must be >= 2";
+        return SHK_ERR_ARG;
+    }
+    if (!aP || !aE || !aN || !b || !x || !out_dev) {
+        noctx_error = "shk_residual_norms_noalloc: null device pointer";
+        return SHK_ERR_ARG;
+    }
+    // the geometry shk_residual_norms runs for this grid (a context's)
+    const Geom G = make_geom(nx, ny);
+    if (G.nblocks > kNoallocMaxBlocks) {
+        noctx_error = "shk_residual_norms_noalloc: a " + std::to_string(nx) + " x " + std::to_string(ny) +
+                      " grid needs " + std::to_string(G.nblocks) + " workgroups, more than the " +
+                      std::to_string(kNoallocMaxBlocks) + " partials of the static scratch";
+        return SHK_ERR_ARG;
+    }
+    double* part = nullptr;
+    const int rc = scratch_on_current_device(&part);
+    if (rc) return rc;
+    launch::residual_norms(G, aP, aE, aN, b, x, nullptr, part, out_dev, (hipStream_t)stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        noctx_error = std::string("shk_residual_norms_noalloc: ") + hipGetErrorString(e);
+        return SHK_ERR_HIP;
+    }
     return SHK_OK;
 }
 

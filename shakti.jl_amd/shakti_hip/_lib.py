@@ -28,7 +28,7 @@ EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "sh
            "shk_set_model", "shk_upload", "shk_download", "shk_set_initial_conditions", "shk_solve_elliptic",
            "shk_solve_parabolic", "shk_update_derived", "shk_picard_iteration", "shk_picard_loop", "shk_solve_log",
            "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
-           "shk_apply_operator", "shk_residual_norms", "shk_apply_precond", "shk_download_system", "shk_time_kernel", "shk_profile_enable", "shk_profile_read",
+           "shk_apply_operator", "shk_residual_norms", "shk_residual_norms_noalloc", "shk_apply_precond", "shk_download_system", "shk_time_kernel", "shk_profile_enable", "shk_profile_read",
            "shk_profile_reset"]
 # ... and of include/shakti_learned.h
 EXPORTS_LEARNED = ["shk_unet_create", "shk_unet_destroy", "shk_unet_last_error", "shk_unet_num_params",
@@ -142,6 +142,7 @@ def load() -> C.CDLL:
     L.shk_assemble.argtypes = [vp, i32, dbl]
     L.shk_apply_operator.argtypes = [vp, dp, dp]
     L.shk_residual_norms.argtypes = [vp, dp, dp]
+    L.shk_residual_norms_noalloc.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp]
     L.shk_apply_precond.argtypes = [vp, dp, dp]
     L.shk_download_system.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.shk_time_kernel.argtypes = [vp, i32, i32, dp, dp]
