@@ -65,38 +65,122 @@ def test_transmissivity_and_cell_reynolds(oracle):
     np.testing.assert_allclose(s.Re, Re, rtol=1e-15)
 
 
-def test_melt_rate_energy_balance(oracle):
-    """L*mdot = G + shear + rho_w g potential + ct cw rho_w sensible, each
-    term the face average of its product (src/melt_rate.jl:106-122)."""
-    O = oracle
-    g, p, m, s = _state(O)
-    sh = np.abs((s.ub_x[1:] * s.taub_x[1:] + s.ub_x[:-1] * s.taub_x[:-1]) / 2 +
-                (s.ub_y[:, 1:] * s.taub_y[:, 1:] + s.ub_y[:, :-1] * s.taub_y[:, :-1]) / 2)
-    pot = np.abs((s.q_x[1:] * s.dhdx[1:] + s.q_x[:-1] * s.dhdx[:-1]) / 2 +
-                 (s.q_y[:, 1:] * s.dhdy[:, 1:] + s.q_y[:, :-1] * s.dhdy[:, :-1]) / 2)
-    sens = ((s.q_x[1:] * s.dpwdx[1:] + s.q_x[:-1] * s.dpwdx[:-1]) / 2 +
-            (s.q_y[:, 1:] * s.dpwdy[:, 1:] + s.q_y[:, :-1] * s.dpwdy[:, :-1]) / 2)
-    np.testing.assert_allclose(s.shear, sh, rtol=1e-14, atol=1e-300)
-    np.testing.assert_allclose(s.potential, pot, rtol=1e-14, atol=1e-300)
-    np.testing.assert_allclose(s.sensible, sens, rtol=1e-13, atol=1e-25)
-    np.testing.assert_allclose(p.L * s.mdot, s.G + sh + p.rho_w * p.g * pot + p.ct * p.cw * p.rho_w * sens,
-                               rtol=1e-12)
-    assert np.all(s.mdot > 0)
+def _uniform_state(O, ub=1e-6, N0=2e5, flat_head=False, G_scale=1.0, nx=40, ny=16, seed=3):
+    """A glacier whose GROUNDED cells share one effective pressure N0 (h set to
+    zb + (po - N0) / (rho_w g)) or, with flat_head, one head (no gradient, so
+    no water flow), with a uniform basal speed ub in x on the faces the mask
+    keeps; A (so lambda = 1.5 A) is uniform."""
+    import shakti_hip.synthetic as S
+    gg, raw = S.glacier(nx, ny, bed_amp=15.0, seed=seed)
+    raw = dict(raw)
+    raw["ub_x"] = np.full_like(raw["ub_x"], ub)
+    raw["G"] = raw["G"] * G_scale
+    g = O.Grid(nx, ny, gg.lx, gg.ly)
+    p = O.ModelParameters()
+    m = O.Model(kfs=0, sliding_C=0.5)
+    s = O.State(g)
+    O.set_initial_conditions(s, g, p, m, **raw)
+    if flat_head:
+        s.h[...] = np.max(s.zb) + 100.0
+    else:
+        s.h[...] = s.zb + (s.po - N0) / (p.rho_w * p.g)
+    O.update_derived(s, g, p, m)
+    return g, p, m, s
 
 
-def test_sliding_stress_coulomb_limit_and_zero_speed(oracle):
-    """taub_x = N_f C (v/(v + |N_f|^n lambda_f))^(1/n) ub/v on interior faces,
-    0 where the face speed is 0 (src/sliding_law.jl:213-249)."""
+def _interior_grounded_x_faces(O, s, ub):
+    """x-faces 1..nx-1 between two GROUNDED cells that both slide at the full
+    speed ub (cells next to a face the mask zeroed have a lower |ub|, and the
+    face speed v is the average of the two cells' |ub|)."""
+    gr = (s.mask == O.GROUNDED) & (np.abs(s.abs_ub - ub) <= 1e-15 * ub)
+    return gr[1:, :] & gr[:-1, :]
+
+
+def test_regularised_coulomb_limits(oracle):
+    """The regularised-Coulomb law taub = C N (v / (v + |N|^n lambda))^(1/n) ub / v
+    (src/sliding_law.jl:213-256): with one N on every grounded cell its face
+    value is that N whatever the averaging, so the law itself is pinned by its
+    limits on the faces whose face speed v equals ub.  Fast sliding
+    (v >> |N|^n lambda): the Coulomb limit C N.  Slow sliding: the power law
+    C (v / lambda)^(1/n), independent of N.  In between |taub| <= C |N| there
+    (elsewhere the bound is C |N| ub_f / v, with v the average of the two
+    cells' speeds)."""
     O = oracle
-    g, p, m, s = _state(O)
-    Nf = (s.N[1:] + s.N[:-1]) / 2
-    lam = getattr(s, "lambda")
-    lf = (lam[1:] + lam[:-1]) / 2
-    v = (s.abs_ub[1:] + s.abs_ub[:-1]) / 2
-    expect = np.where(v > 0, Nf * m.sliding_C * (v / (v + np.abs(Nf) ** 3 * lf)) ** (1 / 3) * (s.ub_x[1:-1] / np.where(v > 0, v, 1)), 0)
-    np.testing.assert_allclose(s.taub_x[1:-1], expect, rtol=1e-13, atol=1e-300)
-    # ub was masked on the boundary faces, so the stress there is zero
-    assert np.all(s.taub_x[0] == 0) and np.all(s.taub_x[-1] == 0)
+    N0 = 2e5
+    lam = 1.5 * 2.5e-25
+    C = 0.5
+    _, _, _, fast = _uniform_state(O, ub=1e3, N0=N0)
+    f = _interior_grounded_x_faces(O, fast, 1e3)
+    assert f.sum() > 100
+    t = fast.taub_x[1:-1][f]
+    # |N|^3 lambda / v = 3e-9: the relative deviation from C N is a third of that
+    np.testing.assert_allclose(t, C * N0, rtol=2e-9)
+    # v / (|N|^3 lambda) = 3.3e-8: the power law to a relative 1.1e-8
+    _, _, _, slow = _uniform_state(O, ub=1e-16, N0=N0)
+    t = slow.taub_x[1:-1][f]
+    np.testing.assert_allclose(t, C * (1e-16 / lam) ** (1 / 3), rtol=5e-8)
+    # the slow limit does not depend on N
+    _, _, _, slow2 = _uniform_state(O, ub=1e-16, N0=2 * N0)
+    np.testing.assert_allclose(slow2.taub_x[1:-1][f], t, rtol=5e-8)
+    for ub in (1e-9, 1e-6, 1e-3):
+        _, _, _, s = _uniform_state(O, ub=ub, N0=N0)
+        assert np.all(np.abs(s.taub_x[1:-1][f]) <= C * N0 * (1 + 1e-15))
+        assert np.all(np.abs(s.taub_x[1:-1][f]) > 0)
+        # where a cell's speed is lowered the stress can exceed C N, by ub_f / v at most
+        v = (s.abs_ub[1:] + s.abs_ub[:-1]) / 2
+        pos = v > 0
+        bound = C * N0 * np.abs(s.ub_x[1:-1][pos]) / v[pos]
+        assert np.all(np.abs(s.taub_x[1:-1][pos]) <= bound * (1 + 1e-14))
+    # no speed, no stress
+    _, _, _, still = _uniform_state(O, ub=0.0, N0=N0)
+    assert np.all(still.taub_x == 0) and np.all(still.taub_y == 0)
+
+
+def test_transmissivity_laminar_limit_and_melt_of_geothermal_heat_alone(oracle):
+    """With no head gradient there is no flow (q = 0, Re = 0), so K is the
+    laminar cubic law b^3 g / (12 nu) (src/water_flux.jl:221-234); with no
+    sliding either, every heat term but the geothermal flux vanishes and
+    mdot = G / L (src/melt_rate.jl:106-122), linear in G."""
+    O = oracle
+    _, p, _, s = _uniform_state(O, ub=0.0, flat_head=True)
+    assert np.all(s.q_x == 0) and np.all(s.q_y == 0) and np.all(s.Re == 0)
+    np.testing.assert_allclose(s.K, s.b ** 3 * p.g / (12 * p.nu), rtol=1e-15, atol=0)
+    assert np.all(s.shear == 0) and np.all(s.potential == 0) and np.all(s.sensible == 0)
+    np.testing.assert_allclose(s.mdot, s.G / p.L, rtol=1e-15, atol=0)
+    _, _, _, s3 = _uniform_state(O, ub=0.0, flat_head=True, G_scale=3.0)
+    np.testing.assert_allclose(s3.mdot, 3 * s.mdot, rtol=1e-15)
+    # sliding adds frictional heat |ub . taub| only: mdot rises above G/L where ub != 0
+    _, _, _, sl = _uniform_state(O, ub=1e-6, flat_head=True)
+    assert np.all(sl.potential == 0) and np.all(sl.sensible == 0)
+    np.testing.assert_allclose(p.L * sl.mdot - sl.G, sl.shear, rtol=1e-12, atol=1e-15 * np.max(sl.G))
+    assert np.all(sl.mdot >= s.mdot) and np.any(sl.mdot > s.mdot)
+
+
+def test_explicit_gap_step_is_forward_euler(oracle):
+    """The explicit scheme (src/gap_height.jl:127-135) is a forward-Euler step
+    b + dt f(b): linear in dt (b(2 dt) - b = 2 (b(dt) - b) to rounding where no
+    clamp acts), and consistent with the implicit backward-Euler step, which
+    it matches to first order: |b_E - b_I| = O(dt^2)."""
+    O = oracle
+    g, p, m, s0 = _state(O)
+
+    def stepped(scheme, dt):
+        s = s0.copy()
+        mm = O.Model(kfs=m.kfs, gap_scheme=scheme, sliding_C=m.sliding_C)
+        O.step_b(s, g, p, mm, dt)
+        return s.b
+
+    gr = s0.mask == O.GROUNDED
+    b0 = s0.b
+    e1, e2 = stepped(1, 60.0), stepped(1, 120.0)
+    keep = gr & (e1 > 0) & (e2 > 0)
+    assert keep.sum() > 100
+    np.testing.assert_allclose((e2 - b0)[keep], 2 * (e1 - b0)[keep], rtol=1e-9, atol=1e-18)
+    assert np.all(e1[~gr] == b0[~gr])
+    d = []
+    for dt in (400.0, 200.0, 100.0):
+        d.append(np.max(np.abs(stepped(1, dt) - stepped(0, dt))[keep]))
+    assert 3.5 < d[0] / d[1] < 4.5 and 3.5 < d[1] / d[2] < 4.5, d
 
 
 def test_gap_height_backward_euler(oracle):
