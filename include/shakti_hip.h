@@ -234,6 +234,15 @@ int shk_step(shk_ctx* ctx, double dt, int32_t ieb_mode, double ieb_uniform, int3
  * nx*ny doubles. */
 int shk_assemble(shk_ctx* ctx, int32_t parabolic, double dt);
 int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host);
+/* update_chebyshev_bounds!(P, rhs) (src/preconditioner.jl:197-233) with the
+ * estimate of src/preconditioner.jl:89-134 run on the operator of the last
+ * assembly from a host rhs of nx*ny doubles, under the configuration's
+ * bound mode and step count (the configured preconditioner must be
+ * Chebyshev).  A valid estimate replaces the persistent bounds; an invalid
+ * one (e.g. from an all-zero rhs) keeps them, the first time the placeholder
+ * (1, 1), and sets *warned.  Any output pointer may be NULL. */
+int shk_update_chebyshev_bounds(shk_ctx* ctx, const double* rhs_host, double* lambda_min, double* lambda_max,
+                                int32_t* warned);
 /* ||b - A x||_2 and ||b||_2 of the last assembly in FP64 (norms[0], norms[1])
  * for a host vector x of nx*ny doubles: the true residual of a head, formed
  * with the operator's own apply and two sums of squares finished in a fixed

@@ -1621,6 +1621,26 @@ int shk_apply_operator(shk_ctx* ctx, const double* x_host, double* y_host) {
     return SHK_OK;
 }
 
+int shk_update_chebyshev_bounds(shk_ctx* ctx, const double* rhs_host, double* lambda_min, double* lambda_max,
+                                int32_t* warned) {
+    if (!ctx || !rhs_host) return fail(ctx, SHK_ERR_ARG, "null argument");
+    if (!ctx->have_system) return fail(ctx, SHK_ERR_STATE, "no assembled system");
+    if (ctx->cfg.precond != SHK_PRECOND_CHEBYSHEV)
+        return fail(ctx, SHK_ERR_STATE, "shk_update_chebyshev_bounds: the configured preconditioner is not Chebyshev");
+    HIPCHK(hipMemcpyAsync(ctx->d64, rhs_host, ctx->n * 8, hipMemcpyHostToDevice, ctx->st));
+    // the solver's own vectors as the estimate's scratch, the given rhs as its start
+    Work<double> w{ctx->aP, ctx->aE, ctx->aN, ctx->aP, ctx->d64, ctx->x, ctx->r, ctx->z, ctx->pA, ctx->pB,
+                   ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->r2};
+    cheb_refresh<double>(ctx, w, ctx->cfg.cheb_nsteps, ctx->cfg.cheb_degree);
+    HIPCHK(hipGetLastError());
+    int rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (lambda_min) *lambda_min = ctx->sc_host->lmin;
+    if (lambda_max) *lambda_max = ctx->sc_host->lmax;
+    if (warned) *warned = ctx->sc_host->bounds_warning;
+    return SHK_OK;
+}
+
 int shk_residual_norms(shk_ctx* ctx, const double* x_host, double* norms) {
     if (!ctx || !x_host || !norms) return fail(ctx, SHK_ERR_ARG, "null argument");
     if (!ctx->have_system) return fail(ctx, SHK_ERR_STATE, "no assembled system");

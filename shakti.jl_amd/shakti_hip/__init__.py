@@ -7,4 +7,4 @@ from .api import (  # noqa: F401
     PicardSolver, Picard_loop, PrescribedSlidingLaw, RegularizedCoulombSlidingLaw, SeasonalMeltInput,
     SeasonalMoulinMeltInput, Simulation, SparseAssembledLinearSystem, State, elliptic_solver, parabolic_solver,
     picard_status, run, set_initial_conditions, solve_elliptic_linear_system, solve_parabolic_linear_system, step,
-    update_derived)
+    update_chebyshev_bounds, update_derived)

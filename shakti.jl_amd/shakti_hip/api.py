@@ -462,6 +462,25 @@ def solve_parabolic_linear_system(ls: CGIterativeSolver, s: State, g: Grid, p: M
     ls._record(st)
 
 
+def update_chebyshev_bounds(ls: CGIterativeSolver, s: State, rhs) -> ChebyshevPreconditioner:
+    """update_chebyshev_bounds!(P, rhs) (src/preconditioner.jl:197-233) on the
+    operator of the State's last assembly: a valid estimate from ``rhs``
+    replaces ``ls.precond``'s bounds, an invalid one keeps them (the first time
+    the placeholder (1, 1)) with a warning, as the reference does."""
+    if ls.precond is None:
+        raise L.ShaktiHIPError("update_chebyshev_bounds needs a CGIterativeSolver with chebyshev_degree")
+    _bind_solver(s, ls)
+    r = np.asfortranarray(np.broadcast_to(np.asarray(rhs, dtype=np.float64), (s.grid.nx, s.grid.ny)))
+    lo, hi, w = C.c_double(), C.c_double(), C.c_int32()
+    L.check(L.load().shk_update_chebyshev_bounds(s.ctx, r.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lo),
+                                                 C.byref(hi), C.byref(w)), s.ctx, "shk_update_chebyshev_bounds")
+    ls.precond.lambda_min, ls.precond.lambda_max = lo.value, hi.value
+    if w.value:
+        import warnings
+        warnings.warn("ChebyshevPreconditioner: eigenvalue bound estimate was invalid; reusing previous bounds")
+    return ls.precond
+
+
 def update_derived(s: State, p: ModelParameters, sl: AbstractSlidingLaw) -> None:
     """The refresh after a solve (src/elliptic_solver.jl:193-206)."""
     s._bind(p=p, sl=sl)

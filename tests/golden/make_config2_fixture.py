@@ -19,9 +19,10 @@ src/run.jl:82-88 as the oracle's orc_picard_loop / orc_step do.
 
 Stored: the Picard (converged, last_iter) of every step; per step max|h|,
 sum h, max b and sum b; h and b after steps 1 and 200; a checksum of the
-inputs.  About 35 minutes on 8 cores:
+inputs.  About 30 minutes on 8 cores; with --checkpoint PATH the State and
+the records are saved there every 10 steps and a rerun resumes from them:
 
-    python tests/golden/make_config2_fixture.py
+    python tests/golden/make_config2_fixture.py [--checkpoint PATH]
 """
 import pathlib
 import sys
@@ -65,7 +66,7 @@ def inputs():
     return gg, raw, moulin
 
 
-def trajectory(log=print):
+def trajectory(log=print, checkpoint=None):
     import oracle as O
     from shakti_hip import synthetic as S
     gg, raw, moulin = inputs()
@@ -77,8 +78,18 @@ def trajectory(log=print):
     eps = np.finfo(np.float64).eps
     out = {"zb_sum": np.float64(np.sum(raw["zb"])), "moulin_sum": np.float64(np.sum(moulin))}
     picard, stats = [], []
-    t = 0.0
-    for step in range(1, STEPS + 1):
+    first = 1
+    if checkpoint and pathlib.Path(checkpoint).exists():
+        with np.load(checkpoint, allow_pickle=False) as c:
+            for f in O.ALL_FIELDS:
+                getattr(s, f)[...] = c["state_" + f]
+            picard = [tuple(r) for r in c["picard"]]
+            stats = [tuple(r) for r in c["stats"]]
+            out.update({k: c[k] for k in c.files if k[:2] in ("h_", "b_")})
+        first = len(picard) + 1
+        log(f"resumed after step {first - 1}", flush=True)
+    t = (first - 1) * DT
+    for step in range(first, STEPS + 1):
         t0 = time.time()
         s.ieb[...] = O.seasonal_ieb(t, **S.SEASONAL_C2) + moulin  # update_ieb!, then the moulins on top
         conv, last = False, 500
@@ -98,13 +109,20 @@ def trajectory(log=print):
             out[f"h_{step}"] = s.h.copy()
             out[f"b_{step}"] = s.b.copy()
         log(f"step {step}: Picard {last} (converged {conv}), {time.time() - t0:.1f} s", flush=True)
+        if checkpoint and step % 10 == 0:
+            np.savez(checkpoint, picard=np.array(picard, dtype=np.int64), stats=np.array(stats),
+                     **{"state_" + f: getattr(s, f) for f in O.ALL_FIELDS},
+                     **{k: v for k, v in out.items() if k[:2] in ("h_", "b_")})
     out["picard"] = np.array(picard, dtype=np.int64)
     out["stats"] = np.array(stats, dtype=np.float64)
     return out
 
 
 if __name__ == "__main__":
+    import argparse
     import oracle as O
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--checkpoint", default=None)
     O.build()
-    np.savez_compressed(OUT, **trajectory())
+    np.savez_compressed(OUT, **trajectory(checkpoint=ap.parse_args().checkpoint))
     print(f"wrote {OUT}")

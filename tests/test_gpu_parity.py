@@ -380,3 +380,81 @@ def test_streaming_kernels_on_a_grid_beyond_the_infinity_cache(hip):
         res[name] = hs.get("h")
         del hs
     assert rel_err(res["cheb"], res["mg"]) < 1e-9
+
+
+@pytest.mark.parametrize("bounds_mode", ["reference", "gershgorin"])
+def test_invalid_estimate_keeps_previous_bounds(oracle, hip, bounds_mode):
+    """test/linear_solver_test.jl:197-222 through the C ABI: an all-zero rhs
+    makes the estimate 0/0, so update_chebyshev_bounds keeps the placeholder
+    (1, 1) with a warning; a real rhs then gives the oracle's bounds; a second
+    zero rhs keeps those (the previous valid bounds), again with a warning."""
+    import warnings
+    from shakti_hip import _lib as L
+    g_args, raw = _fixture(oracle)
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, C=0.25)
+    hg, hp, hs = hip_state(hip, g_args, raw, C=0.25)
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, chebyshev_bounds=bounds_mode)
+    hs._bind(p=hp, kfs=hip.Arithmetic())
+    L.check(L.load().shk_assemble(hs.ctx, 0, 0.0), hs.ctx)
+    zeros = np.zeros((g.nx, g.ny))
+    with pytest.warns(UserWarning, match="invalid"):
+        pc = hip.update_chebyshev_bounds(ls, hs, zeros)
+    assert (pc.lambda_min, pc.lambda_max) == (1.0, 1.0)
+    co = oracle.assemble_mfls(os_, g, p, 0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        pc = hip.update_chebyshev_bounds(ls, hs, co[5])
+    if bounds_mode == "reference":
+        want = oracle.update_chebyshev_bounds(co, co[5])
+    else:
+        lo = oracle.estimate_eigenvalue_bounds_d(co, co[5])[0]
+        want = (lo, oracle.gershgorin(co), False)
+    assert not want[2]
+    assert pc.lambda_min == pytest.approx(want[0], rel=1e-10)
+    assert pc.lambda_max == pytest.approx(want[1], rel=1e-10)
+    kept = (pc.lambda_min, pc.lambda_max)
+    with pytest.warns(UserWarning, match="invalid"):
+        pc = hip.update_chebyshev_bounds(ls, hs, zeros)
+    assert (pc.lambda_min, pc.lambda_max) == kept
+    assert np.isfinite(kept).all() and 0 < kept[0] < kept[1]
+
+
+@pytest.mark.parametrize("case", ["explicit_gap", "relaxation", "seasonal", "seasonal_moulins"])
+def test_trajectory_options_match_oracle(oracle, hip, case):
+    """Six hourly steps at 64 x 32 against the oracle's Cholesky trajectory
+    with each option the configurations use beyond config 1: the explicit gap
+    scheme (src/gap_height.jl:127-135), head relaxation alpha = 0.7
+    (src/elliptic_solver.jl:33-46), SeasonalMeltInput (ieb mode 1,
+    src/melt_input.jl:86-94) and the seasonal input with moulins on top (ieb
+    mode 2).  Same Picard counts, h within 1e-9 and b within 1e-8 each step."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(64, 32, bed_amp=10.0, seed=4)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    gap = 1 if case == "explicit_gap" else 0
+    alpha = 0.7 if case == "relaxation" else None
+    g, p, m, os_ = oracle_state(oracle, g_args, raw, gap=gap)
+    hg, hp, hs = hip_state(hip, g_args, raw)
+    moulin = S.moulin_field(gg, raw["mask"], n=5, seed=4) if case == "seasonal_moulins" else None
+    if case == "seasonal":
+        mi = hip.SeasonalMeltInput(**S.SEASONAL_C2)
+    elif case == "seasonal_moulins":
+        mi = hip.SeasonalMoulinMeltInput(moulin, **S.SEASONAL_C2)
+    else:
+        mi = hip.ConstantMeltInput()
+    sv = oracle.Solver(g, "cholesky")
+    ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, chebyshev_degree=4, atol=0.0, rtol=1e-13)
+    ps = hip.PicardSolver(500, 1e-6, ls, hg, alpha=alpha)
+    sim = hip.Simulation(hg, hs, 6, 3600.0, hp, "explicit" if gap else "implicit", [], mi,
+                         hip.RegularizedCoulombSlidingLaw(0.5), ps=ps)
+    mode = {"seasonal": 1, "seasonal_moulins": 2}.get(case, 0)
+    for t in range(6):
+        u = oracle.seasonal_ieb(sim.total_time, **S.SEASONAL_C2) if mode else 0.0
+        conv, last, _ = oracle.step(sv, os_, g, p, m, 3600.0, ieb_mode=mode, ieb_uniform=u, moulin=moulin,
+                                    alpha=alpha)
+        hip.step(sim)
+        sim.total_time += sim.dt
+        assert (ps.converged, ps.last_iter) == (conv, last), (t, ps.last_iter, last)
+        assert rel_err(hs.get("h"), os_.h) <= 1e-9, t
+        assert rel_err(hs.get("b"), os_.b) <= 1e-8, t
+        if mode:
+            assert rel_err(hs.get("ieb"), os_.ieb) <= 1e-15, t
