@@ -56,14 +56,24 @@ typedef struct {
     double rho_w, rho_sw, rho_i, g, nu, n, omega, L, br, lr, ct, cw, p_atm, b_min, b_max, e_v;
 } shk_params;
 
+/* Sliding laws (src/sliding_law.jl).  The face stress of each: */
+#define SHK_SLIDING_REGULARIZED_COULOMB 0 /* N C (v / (v + |N|^n lambda))^(1/n) ub / v, scalar C (:33, :213-249) */
+#define SHK_SLIDING_PRESCRIBED 1          /* taub as uploaded before set-up, never recomputed (:45, :103-107) */
+#define SHK_SLIDING_LINEAR 2              /* C^2 N ub, C a field (:61-90, :552-602) */
+#define SHK_SLIDING_REGULARIZED_COULOMB_FIELD 3 /* the regularised-Coulomb stress with a field C (:270-396) */
+#define SHK_SLIDING_REGULARIZED_COULOMB_V0 4    /* N C (v / (v + v0))^(1/n) ub / v, C a field (:422-543) */
+
 /* Scheme choices of a Simulation -- reference src/simulation.jl:105-146,
- * src/sliding_law.jl:33,45.  Sensible heat is on iff ct*cw != 0 and opening by
- * sliding iff br != 0, decided from shk_params as the reference does. */
+ * src/sliding_law.jl.  Sensible heat is on iff ct*cw != 0 and opening by
+ * sliding iff br != 0, decided from shk_params as the reference does.  The
+ * field laws (LINEAR, _FIELD, _V0) take their cell-centred coefficient from
+ * shk_set_sliding_coefficient. */
 typedef struct {
     int32_t kfs;         /* 0 Arithmetic, 1 Harmonic (src/k_face_scheme.jl:15,23) */
     int32_t gap_scheme;  /* 0 implicit, 1 explicit (src/run.jl:130-145) */
-    int32_t sliding_law; /* 0 RegularizedCoulombSlidingLaw, 1 PrescribedSlidingLaw */
+    int32_t sliding_law; /* SHK_SLIDING_* */
     double sliding_C;    /* RegularizedCoulombSlidingLaw.C */
+    double sliding_v0;   /* RegularizedCoulombV0SlidingLaw.v0 (m/s) */
 } shk_model;
 
 #define SHK_PRECOND_JACOBI 0    /* CGIterativeSolver(g, MatrixFreeLinearSystem) */
@@ -168,6 +178,16 @@ int shk_set_solver_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg);
  * reference passes p and kfs to every solve, src/linear_solver.jl:915). */
 int shk_set_params(shk_ctx* ctx, const shk_params* params);
 int shk_set_model(shk_ctx* ctx, const shk_model* model);
+
+/* The cell-centred coefficient C (nx*ny values) of the field sliding laws,
+ * staggered onto the faces on the device as their constructors do
+ * (LinearSlidingLaw(g, C) src/sliding_law.jl:73-90,
+ * RegularizedCoulombFieldSlidingLaw(g, C) :284-301,
+ * RegularizedCoulombV0SlidingLaw(g, C, v0) :436-452): an outer face takes its
+ * cell's value, an inner face the mean of its two cells.  The linear law
+ * squares the face value where it uses it (the same product as the
+ * reference's stored Cx^2). */
+int shk_set_sliding_coefficient(shk_ctx* ctx, const double* C_host, size_t count);
 
 /* Copy one State field to / from the device; count must equal its size. */
 int shk_upload(shk_ctx* ctx, int32_t field, const double* host, size_t count);

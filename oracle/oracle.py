@@ -51,7 +51,8 @@ class _Params(C.Structure):
 
 class _Model(C.Structure):
     _fields_ = [("kfs", C.c_int32), ("gap_scheme", C.c_int32), ("sliding_law", C.c_int32),
-                ("sliding_C", C.c_double)]
+                ("sliding_C", C.c_double), ("v0", C.c_double), ("Cx", C.POINTER(C.c_double)),
+                ("Cy", C.POINTER(C.c_double))]
 
 
 class _State(C.Structure):
@@ -130,6 +131,7 @@ def lib():
                                C.c_int32, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.orc_step.restype = C.c_int32
         L.orc_num_threads.restype = C.c_int
+        L.orc_stagger_c.argtypes = [C.POINTER(_Grid), dp, dp, dp]
         _lib = L
     return _lib
 
@@ -187,16 +189,46 @@ class ModelParameters:
         return _Params(*[getattr(self, f.name) for f in fields(self)])
 
 
+SLIDING_LAWS = {"regularized_coulomb": 0, "prescribed": 1, "linear": 2, "regularized_coulomb_field": 3,
+                "regularized_coulomb_v0": 4}
+
+
+def stagger_c(g: Grid, Cc) -> tuple[np.ndarray, np.ndarray]:
+    """C at the cell centres (an (nx, ny) array or a scalar) staggered onto the x- and y-faces as
+    LinearSlidingLaw(g, C), RegularizedCoulombFieldSlidingLaw(g, C) and
+    RegularizedCoulombV0SlidingLaw(g, C, v0) stagger it (src/sliding_law.jl:73-90, 284-301, 436-452);
+    the linear law squares the face values where it uses them."""
+    cc = F(np.broadcast_to(np.asarray(Cc, dtype=np.float64), (g.nx, g.ny)))
+    Cx = np.zeros((g.nx + 1, g.ny), order="F")
+    Cy = np.zeros((g.nx, g.ny + 1), order="F")
+    lib().orc_stagger_c(C.byref(g.c()), _p(cc), _p(Cx), _p(Cy))
+    return Cx, Cy
+
+
 @dataclass
 class Model:
-    """The scheme choices of a Simulation (src/simulation.jl:105-146)."""
+    """The scheme choices of a Simulation (src/simulation.jl:105-146).  ``sliding_law`` is a code of
+    SLIDING_LAWS; the field laws (linear, regularized_coulomb_field, regularized_coulomb_v0) take their
+    face coefficients Cx, Cy from ``stagger_c`` (``Model.field_law`` builds one)."""
     kfs: int = 0          # 0 Arithmetic, 1 Harmonic
     gap_scheme: int = 0   # 0 implicit, 1 explicit
-    sliding_law: int = 0  # 0 RegularizedCoulomb, 1 Prescribed
+    sliding_law: int = 0  # SLIDING_LAWS
     sliding_C: float = 0.5
+    v0: float = 0.0
+    Cx: np.ndarray | None = None
+    Cy: np.ndarray | None = None
+
+    @classmethod
+    def field_law(cls, g: Grid, law: str, Cc, v0: float = 0.0, **kw) -> "Model":
+        Cx, Cy = stagger_c(g, Cc)
+        return cls(sliding_law=SLIDING_LAWS[law], v0=v0, Cx=Cx, Cy=Cy, **kw)
 
     def c(self) -> _Model:
-        return _Model(self.kfs, self.gap_scheme, self.sliding_law, self.sliding_C)
+        if self.sliding_law >= 2 and (self.Cx is None or self.Cy is None):
+            raise ValueError("the field sliding laws need the face coefficients Cx, Cy (Model.field_law)")
+        nul = C.POINTER(C.c_double)()
+        return _Model(self.kfs, self.gap_scheme, self.sliding_law, self.sliding_C, self.v0,
+                      _p(self.Cx) if self.Cx is not None else nul, _p(self.Cy) if self.Cy is not None else nul)
 
 
 class State:

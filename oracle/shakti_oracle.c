@@ -41,8 +41,12 @@ typedef struct {
 typedef struct {
     int32_t kfs;          /* 0 Arithmetic, 1 Harmonic (src/k_face_scheme.jl:15,23) */
     int32_t gap_scheme;   /* 0 implicit, 1 explicit (src/simulation.jl:122-128) */
-    int32_t sliding_law;  /* 0 RegularizedCoulomb, 1 Prescribed (src/sliding_law.jl:33,45) */
+    int32_t sliding_law;  /* 0 RegularizedCoulomb, 1 Prescribed, 2 Linear, 3 RegularizedCoulombField,
+                             4 RegularizedCoulombV0 (src/sliding_law.jl:33, 45, 61, 270, 422) */
     double sliding_C;     /* RegularizedCoulombSlidingLaw.C */
+    double v0;            /* RegularizedCoulombV0SlidingLaw.v0 */
+    const double* Cx;     /* laws 2-4: C staggered onto x-faces ((nx+1) x ny; law 2 squares it at use) */
+    const double* Cy;     /*           and y-faces (nx x (ny+1)) by orc_stagger_c */
 } orc_model;
 
 /* Every State field (src/state.jl:13-57), in the reference's order. */
@@ -261,16 +265,43 @@ static void compute_Re(const orc_grid* g, orc_state* s) {
                                  s->Re_y[YF_(ix, iy + 1)]) / 4.0;
 }
 
-/* src/sliding_law.jl:213-256 compute_taub_xy_kernel! (RegularizedCoulomb) */
-static double taub_face(double Nf, double abs_v, double lf, double ub, double C, double n, double inv_n) {
+/* The constructors LinearSlidingLaw(g, C) (src/sliding_law.jl:73-90),
+ * RegularizedCoulombFieldSlidingLaw(g, C) (:284-301) and
+ * RegularizedCoulombV0SlidingLaw(g, C, v0) (:436-452): C at the cell centres
+ * staggered onto the faces, outer faces the cell's value, inner faces the
+ * mean of the two cells. */
+void orc_stagger_c(const orc_grid* g, const double* C, double* Cx, double* Cy) {
+    const int nx = g->nx, ny = g->ny;
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix <= nx; ++ix)
+            Cx[XF_(ix, iy)] = ix == 0 ? C[C_(0, iy)]
+                              : ix == nx ? C[C_(nx - 1, iy)]
+                                         : (C[C_(ix - 1, iy)] + C[C_(ix, iy)]) / 2.0;
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix)
+            Cy[YF_(ix, iy)] = iy == 0 ? C[C_(ix, 0)]
+                              : iy == ny ? C[C_(ix, ny - 1)]
+                                         : (C[C_(ix, iy - 1)] + C[C_(ix, iy)]) / 2.0;
+}
+
+/* The face stress of each law:
+ * src/sliding_law.jl:213-256 compute_taub_xy_kernel! (RegularizedCoulomb, scalar C),
+ * :360-396 compute_taub_xy_field_kernel! (the same with the face C),
+ * :509-543 compute_taub_xy_v0_kernel! (v0 in place of |N|^n lambda),
+ * :590-602 compute_taub_xy_linear_kernel! (Cx2 * Nf * ub, Cx2 the squared face C). */
+static double taub_face(int law, double Nf, double abs_v, double lf, double ub, double C, double Cf, double v0,
+                        double n, double inv_n) {
+    if (law == 2) return (Cf * Cf) * Nf * ub;
+    if (law == 4) return abs_v > 0.0 ? Nf * Cf * jl_pow(abs_v / (abs_v + v0), inv_n) * (ub / abs_v) : 0.0;
+    const double c = law == 3 ? Cf : C;
     if (abs_v > 0.0)
-        return Nf * C * jl_pow(abs_v / (abs_v + jl_pow(fabs(Nf), n) * lf), inv_n) * (ub / abs_v);
+        return Nf * c * jl_pow(abs_v / (abs_v + jl_pow(fabs(Nf), n) * lf), inv_n) * (ub / abs_v);
     return 0.0;
 }
 static void compute_taub_xy(const orc_grid* g, const orc_params* p, const orc_model* m, orc_state* s) {
     if (m->sliding_law == 1) return; /* Prescribed: no-op (src/sliding_law.jl:115-131) */
-    const int nx = g->nx, ny = g->ny;
-    const double C = m->sliding_C, n = p->n, inv_n = 1.0 / p->n;
+    const int nx = g->nx, ny = g->ny, law = m->sliding_law;
+    const double C = m->sliding_C, n = p->n, inv_n = 1.0 / p->n, v0 = m->v0;
 #pragma omp parallel for schedule(static)
     for (int iy = 0; iy < ny; ++iy)
         for (int ix = 0; ix <= nx; ++ix) {
@@ -284,7 +315,8 @@ static void compute_taub_xy(const orc_grid* g, const orc_params* p, const orc_mo
                 lf = (s->lambda[C_(ix, iy)] + s->lambda[C_(ix - 1, iy)]) / 2.0;
                 av = (s->abs_ub[C_(ix, iy)] + s->abs_ub[C_(ix - 1, iy)]) / 2.0;
             }
-            s->taub_x[XF_(ix, iy)] = taub_face(Nf, av, lf, s->ub_x[XF_(ix, iy)], C, n, inv_n);
+            s->taub_x[XF_(ix, iy)] = taub_face(law, Nf, av, lf, s->ub_x[XF_(ix, iy)], C, law >= 2 ? m->Cx[XF_(ix, iy)] : 0.0,
+                                               v0, n, inv_n);
         }
 #pragma omp parallel for schedule(static)
     for (int iy = 0; iy <= ny; ++iy)
@@ -299,7 +331,8 @@ static void compute_taub_xy(const orc_grid* g, const orc_params* p, const orc_mo
                 lf = (s->lambda[C_(ix, iy)] + s->lambda[C_(ix, iy - 1)]) / 2.0;
                 av = (s->abs_ub[C_(ix, iy)] + s->abs_ub[C_(ix, iy - 1)]) / 2.0;
             }
-            s->taub_y[YF_(ix, iy)] = taub_face(Nf, av, lf, s->ub_y[YF_(ix, iy)], C, n, inv_n);
+            s->taub_y[YF_(ix, iy)] = taub_face(law, Nf, av, lf, s->ub_y[YF_(ix, iy)], C, law >= 2 ? m->Cy[YF_(ix, iy)] : 0.0,
+                                               v0, n, inv_n);
         }
 }
 
@@ -699,11 +732,12 @@ int orc_tridiag_extreme_eigs(int32_t m, const double* a, const double* e, double
  * steps on D^-1 A with Euclidean dots from x0 = 0, Lanczos tridiagonal from the
  * CG coefficients, Ritz extremes scaled by 0.9 / 1.1. */
 void orc_estimate_eigenvalue_bounds(const orc_op* op, const double* rhs, int32_t nsteps, double* lmin, double* lmax) {
+    if (nsteps < 1 || nsteps > 256) { *lmin = NAN; *lmax = NAN; return; } /* the Lanczos arrays hold 256 */
     size_t n = (size_t)op->nx * op->ny;
     double* r = malloc(n * sizeof(double));
     double* p = malloc(n * sizeof(double));
     double* Ap = malloc(n * sizeof(double));
-    double alphas[256], betas[256];
+    double alphas[256] = {0}, betas[256] = {0};
     memcpy(r, rhs, n * sizeof(double));
     memcpy(p, r, n * sizeof(double));
     double gamma = dot(n, r, r);
@@ -738,12 +772,13 @@ void orc_estimate_eigenvalue_bounds(const orc_op* op, const double* rhs, int32_t
  * library's Gershgorin bound mode takes lambda_min from it. */
 void orc_estimate_eigenvalue_bounds_d(const orc_op* op, const double* rhs, int32_t nsteps, double* lmin,
                                       double* lmax) {
+    if (nsteps < 1 || nsteps > 256) { *lmin = NAN; *lmax = NAN; return; } /* the Lanczos arrays hold 256 */
     size_t n = (size_t)op->nx * op->ny;
     double* r = malloc(n * sizeof(double));
     double* p = malloc(n * sizeof(double));
     double* Ap = malloc(n * sizeof(double));
     double* Au = malloc(n * sizeof(double));
-    double alphas[256], betas[256];
+    double alphas[256] = {0}, betas[256] = {0};
     for (size_t i = 0; i < n; ++i) r[i] = rhs[i] / op->d[i];
     memcpy(p, r, n * sizeof(double));
     double gamma = 0.0;

@@ -64,6 +64,8 @@ struct shk_ctx {
     Fields F{};
     double* h_alt = nullptr;  // ping-pong partner of the h field
     double* moulin = nullptr;
+    // face coefficient of the field sliding laws (shk_set_sliding_coefficient)
+    double *slide_cx = nullptr, *slide_cy = nullptr;
 
     // linear system (FP64) and workspace
     double *aP = nullptr, *aE = nullptr, *aN = nullptr, *rhs = nullptr;
@@ -261,6 +263,9 @@ void fill_phys(shk_ctx* c) {
     P.gap_scheme = c->m.gap_scheme;
     P.sliding_law = c->m.sliding_law;
     P.C = c->m.sliding_C;
+    P.v0 = c->m.sliding_v0;
+    P.Cx = c->slide_cx;
+    P.Cy = c->slide_cy;
 }
 
 size_t field_count(const shk_ctx* c, int f) {
@@ -1203,7 +1208,17 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     return conclude(ctx, sst, stats);
 }
 
+// The field sliding laws read their staggered coefficient.
+int check_sliding(shk_ctx* ctx) {
+    const int law = ctx->m.sliding_law;
+    if ((law == SHK_SLIDING_LINEAR || law == SHK_SLIDING_REGULARIZED_COULOMB_FIELD ||
+         law == SHK_SLIDING_REGULARIZED_COULOMB_V0) && !ctx->slide_cx)
+        return fail(ctx, SHK_ERR_STATE, "this sliding law needs its coefficient: call shk_set_sliding_coefficient");
+    return SHK_OK;
+}
+
 int derived(shk_ctx* ctx, const double* h_prev, int check, double tol) {
+    if (int rc = check_sliding(ctx)) return rc;
     if (check) {
         ctx->sc_host->picard_tol = tol;
         // only picard_tol changes; write that field alone
@@ -1350,7 +1365,7 @@ int shk_destroy(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     if (ctx->st) hipStreamSynchronize(ctx->st);
     for (int f = 0; f < SHK_NUM_FIELDS; ++f) hipFree(ctx->F.f[f]);
-    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->r2, ctx->r2_32, ctx->cP, ctx->dg32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
+    void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->slide_cx, ctx->slide_cy, ctx->r2, ctx->r2_32, ctx->cP, ctx->dg32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
                     ctx->pA, ctx->pB, ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->d64, ctx->aP32, ctx->aE32,
                     ctx->aN32, ctx->b32, ctx->x32, ctx->r32, ctx->z32, ctx->pA32, ctx->pB32, ctx->Ap32,
                     ctx->s32, ctx->cA32, ctx->cB32, ctx->partials, ctx->partials2, ctx->sc};
@@ -1404,9 +1419,26 @@ int shk_set_params(shk_ctx* ctx, const shk_params* params) {
 int shk_set_model(shk_ctx* ctx, const shk_model* model) {
     if (!ctx || !model) return SHK_ERR_ARG;
     if (model->kfs < 0 || model->kfs > 1 || model->gap_scheme < 0 || model->gap_scheme > 1 ||
-        model->sliding_law < 0 || model->sliding_law > 1)
+        model->sliding_law < SHK_SLIDING_REGULARIZED_COULOMB || model->sliding_law > SHK_SLIDING_REGULARIZED_COULOMB_V0)
         return fail(ctx, SHK_ERR_ARG, "unknown k-face scheme, gap scheme or sliding law");
+    if (model->sliding_law == SHK_SLIDING_REGULARIZED_COULOMB_V0 && !(model->sliding_v0 >= 0.0))
+        return fail(ctx, SHK_ERR_ARG, "RegularizedCoulombV0SlidingLaw: v0 must be >= 0");
     ctx->m = *model;
+    fill_phys(ctx);
+    return SHK_OK;
+}
+
+int shk_set_sliding_coefficient(shk_ctx* ctx, const double* C_host, size_t count) {
+    if (!ctx || !C_host || count != ctx->n)
+        return fail(ctx, SHK_ERR_ARG, "the sliding coefficient must hold nx*ny cell values");
+    if (!ctx->slide_cx) {
+        HIPCHK(hipMalloc((void**)&ctx->slide_cx, ctx->nxf * 8 + 256));
+        HIPCHK(hipMalloc((void**)&ctx->slide_cy, ctx->nyf * 8 + 256));
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d64, C_host, count * 8, hipMemcpyHostToDevice, ctx->st));
+    launch::stagger(ctx->g.nx, ctx->g.ny, ctx->d64, ctx->slide_cx, ctx->slide_cy, ctx->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
     fill_phys(ctx);
     return SHK_OK;
 }
@@ -1437,6 +1469,7 @@ int shk_download(shk_ctx* ctx, int32_t field, double* host, size_t count) {
 
 int shk_set_initial_conditions(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
+    if (int rc = check_sliding(ctx)) return rc;
     {
         Prof pr(ctx, P_IC, 0);
         launch::ic_cells(ctx->P, ctx->F, ctx->st);

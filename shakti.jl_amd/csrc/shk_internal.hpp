@@ -126,13 +126,17 @@ struct Phys {
     double rho_w, rho_sw, rho_i, g, nu, n, omega, L, br, lr, ct, cw, p_atm, b_min, b_max, e_v;
     Expo n_exp, n1_exp, inv_n_exp;
     int with_sensible, with_open_by_sliding, kfs, gap_scheme, sliding_law;
-    double C;
+    double C, v0;
+    // staggered coefficient of the field sliding laws ((nx+1) x ny, nx x (ny+1))
+    const double *Cx, *Cy;
 };
 
 // Kernel launchers (shk_kernels.hip).  All are asynchronous on `st`.
 namespace launch {
 void ic_cells(const Phys& P, Fields F, hipStream_t st);
 void stagger_b(const Phys& P, Fields F, hipStream_t st);
+// cell-centred c onto the faces: outer faces the cell value, inner the mean
+void stagger(int nx, int ny, const double* c, double* fx, double* fy, hipStream_t st);
 // mode 0: refresh after a solve (pw from h); mode 1: initial conditions (pw as stored).
 // check != 0 also reduces max|h - h_prev| and max|h| and sets the Picard flag.
 void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,

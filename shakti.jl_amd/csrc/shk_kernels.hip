@@ -316,11 +316,46 @@ __device__ __forceinline__ void qre(const Phys& P, double bf, double grad, doubl
     q = -jl_sign(grad) * P.nu * R;
 }
 
-// RegularizedCoulomb face stress (src/sliding_law.jl:213-249).
-__device__ __forceinline__ double taub_face(const Phys& P, double Nf, double abs_v, double lf, double ub) {
-    if (abs_v > 0.0)
-        return Nf * P.C * cpow(abs_v / (abs_v + cpow(fabs(Nf), P.n_exp) * lf), P.inv_n_exp) * (ub / abs_v);
-    return 0.0;
+// Face stress of the sliding laws (src/sliding_law.jl): Nf, abs_v and lf are
+// the face values of N, |ub| and lambda (outer faces their cell's, inner the
+// mean of the two cells), Cf the face coefficient of the field laws.
+//   regularised Coulomb (:213-249) and its field form (:360-396):
+//     N C (v / (v + |N|^n lambda))^(1/n) ub / v, 0 where v = 0;
+//   V0 (:509-543): N C (v / (v + v0))^(1/n) ub / v, 0 where v = 0;
+//   linear (:590-602): C^2 N ub, C^2 the square of the staggered C.
+__device__ __forceinline__ double taub_face(const Phys& P, double Nf, double abs_v, double lf, double ub, double Cf) {
+    switch (P.sliding_law) {
+        case SHK_SLIDING_LINEAR: return (Cf * Cf) * Nf * ub;
+        case SHK_SLIDING_REGULARIZED_COULOMB_V0:
+            return abs_v > 0.0 ? Nf * Cf * cpow(abs_v / (abs_v + P.v0), P.inv_n_exp) * (ub / abs_v) : 0.0;
+        default: {
+            const double C = P.sliding_law == SHK_SLIDING_REGULARIZED_COULOMB_FIELD ? Cf : P.C;
+            if (abs_v > 0.0)
+                return Nf * C * cpow(abs_v / (abs_v + cpow(fabs(Nf), P.n_exp) * lf), P.inv_n_exp) * (ub / abs_v);
+            return 0.0;
+        }
+    }
+}
+
+// cells onto faces as the field sliding laws' constructors stagger C
+__global__ __launch_bounds__(256) void k_stagger(int nx, int ny, const double* __restrict__ c, double* __restrict__ fx,
+                                                 double* __restrict__ fy) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix <= nx && iy < ny) {
+        double v;
+        if (ix == 0) v = c[cidx(0, iy, nx)];
+        else if (ix == nx) v = c[cidx(nx - 1, iy, nx)];
+        else v = (c[cidx(ix - 1, iy, nx)] + c[cidx(ix, iy, nx)]) / 2.0;
+        fx[xidx(ix, iy, nx)] = v;
+    }
+    if (ix < nx && iy <= ny) {
+        double v;
+        if (iy == 0) v = c[cidx(ix, 0, nx)];
+        else if (iy == ny) v = c[cidx(ix, ny - 1, nx)];
+        else v = (c[cidx(ix, iy - 1, nx)] + c[cidx(ix, iy, nx)]) / 2.0;
+        fy[cidx(ix, iy, nx)] = v;
+    }
 }
 
 // The refresh after a linear solve (the reference runs 9 launches:
@@ -366,7 +401,7 @@ __global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
         }
         s.q_x[k] = q;
         s.Re_x[k] = Re;
-        if (P.sliding_law == 0) {
+        if (P.sliding_law != SHK_SLIDING_PRESCRIBED) {
             double Nf, av, lf;
             if (fx == 0) { Nf = NN(cidx(0, iy, nx)); av = s.abs_ub[cidx(0, iy, nx)]; lf = s.lambda[cidx(0, iy, nx)]; }
             else if (fx == nx) { const size_t cc = cidx(nx - 1, iy, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
@@ -376,7 +411,7 @@ __global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
                 lf = (s.lambda[ce] + s.lambda[cw]) / 2.0;
                 av = (s.abs_ub[ce] + s.abs_ub[cw]) / 2.0;
             }
-            s.taub_x[k] = taub_face(P, Nf, av, lf, s.ub_x[k]);
+            s.taub_x[k] = taub_face(P, Nf, av, lf, s.ub_x[k], P.Cx ? P.Cx[k] : 0.0);
         }
     }
     // y face fy = iy of column ix
@@ -402,7 +437,7 @@ __global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
         }
         s.q_y[k] = q;
         s.Re_y[k] = Re;
-        if (P.sliding_law == 0) {
+        if (P.sliding_law != SHK_SLIDING_PRESCRIBED) {
             double Nf, av, lf;
             if (fy == 0) { const size_t cc = cidx(ix, 0, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
             else if (fy == ny) { const size_t cc = cidx(ix, ny - 1, nx); Nf = NN(cc); av = s.abs_ub[cc]; lf = s.lambda[cc]; }
@@ -412,7 +447,7 @@ __global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
                 lf = (s.lambda[cn] + s.lambda[cs]) / 2.0;
                 av = (s.abs_ub[cn] + s.abs_ub[cs]) / 2.0;
             }
-            s.taub_y[k] = taub_face(P, Nf, av, lf, s.ub_y[k]);
+            s.taub_y[k] = taub_face(P, Nf, av, lf, s.ub_y[k], P.Cy ? P.Cy[k] : 0.0);
         }
     }
 }
@@ -3585,6 +3620,9 @@ void ic_cells(const Phys& P, Fields F, hipStream_t st) {
 }
 void stagger_b(const Phys& P, Fields F, hipStream_t st) {
     hipLaunchKernelGGL(k_stagger_b, cells_grid(P.nx + 1, P.ny + 1), dim3(64, 4), 0, st, P, F);
+}
+void stagger(int nx, int ny, const double* c, double* fx, double* fy, hipStream_t st) {
+    hipLaunchKernelGGL(k_stagger, cells_grid(nx + 1, ny + 1), dim3(64, 4), 0, st, nx, ny, c, fx, fy);
 }
 void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,
              Scalars* sc, hipStream_t st) {

@@ -48,7 +48,7 @@ struct ShkParams
     b_max::Float64; e_v::Float64
 end
 struct ShkModel
-    kfs::Int32; gap_scheme::Int32; sliding_law::Int32; sliding_C::Float64
+    kfs::Int32; gap_scheme::Int32; sliding_law::Int32; sliding_C::Float64; sliding_v0::Float64
 end
 const SHK_ABI_VERSION = 2
 
@@ -194,7 +194,7 @@ function HIPPCGSolver(g::Shakti.Grid; chebyshev_degree::Union{Nothing, Int} = no
                        check_every, atol, rtol, inner_rtol, chebyshev_bounds === :reference ? 1 : 0, 0)
     sg = Ref(ShkGrid(g.nx, g.ny, g.lx, g.ly, g.dx, g.dy, g.dx2, g.dy2))
     sp = Ref(params_c(Shakti.ModelParameters()))
-    sm = Ref(ShkModel(0, 0, 0, 0.5))
+    sm = Ref(ShkModel(0, 0, 0, 0.5, 0.0))
     sc = Ref(cfg)
     out = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:shk_create, libshk), Cint,
@@ -233,7 +233,7 @@ function sync_inputs!(ls::HIPPCGSolver, s, p, kfs)
     check(ccall((:shk_set_params, libshk), Cint, (Ptr{Cvoid}, Ref{ShkParams}), ls.ctx, Ref(params_c(p))),
           ls.ctx, "shk_set_params")
     check(ccall((:shk_set_model, libshk), Cint, (Ptr{Cvoid}, Ref{ShkModel}), ls.ctx,
-                Ref(ShkModel(kfs_code(kfs), 0, 0, 0.5))), ls.ctx, "shk_set_model")
+                Ref(ShkModel(kfs_code(kfs), 0, 0, 0.5, 0.0))), ls.ctx, "shk_set_model")
     check(ccall((:shk_set_solver_cfg, libshk), Cint, (Ptr{Cvoid}, Ref{ShkSolverCfg}), ls.ctx, Ref(ls.cfg)),
           ls.ctx, "shk_set_solver_cfg")
     for (f, a) in ((SHK_MASK, s.mask), (SHK_ZB, s.zb), (SHK_H, s.h), (SHK_K, s.K), (SHK_A_VISC, s.A_visc),

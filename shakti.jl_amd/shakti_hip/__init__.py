@@ -3,8 +3,9 @@ linear-solver plugin seam, driven through the C ABI of libshakti_hip.so."""
 from ._lib import LIB_PATH, ShaktiHIPError, load  # noqa: F401
 from .api import (  # noqa: F401
     GROUNDED, LAND, OCEAN, OTHER_BASIN, Arithmetic, CGIterativeSolver, ConstantMeltInput, EllipticHeadScheme, Grid,
-    Harmonic, HIPMatrixFreeLinearSystem, MatrixFreeLinearSystem, ModelParameters, ParabolicHeadScheme,
-    PicardSolver, Picard_loop, PrescribedSlidingLaw, RegularizedCoulombSlidingLaw, SeasonalMeltInput,
+    Harmonic, HIPMatrixFreeLinearSystem, LinearSlidingLaw, MatrixFreeLinearSystem, ModelParameters,
+    ParabolicHeadScheme, PicardSolver, Picard_loop, PrescribedSlidingLaw, RegularizedCoulombFieldSlidingLaw,
+    RegularizedCoulombSlidingLaw, RegularizedCoulombV0SlidingLaw, SeasonalMeltInput,
     SeasonalMoulinMeltInput, Simulation, SparseAssembledLinearSystem, State, elliptic_solver, parabolic_solver,
     picard_status, run, set_initial_conditions, solve_elliptic_linear_system, solve_parabolic_linear_system, step,
     update_chebyshev_bounds, update_derived)

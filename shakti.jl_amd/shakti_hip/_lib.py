@@ -28,8 +28,9 @@ EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "sh
            "shk_set_model", "shk_upload", "shk_download", "shk_set_initial_conditions", "shk_solve_elliptic",
            "shk_solve_parabolic", "shk_update_derived", "shk_picard_iteration", "shk_picard_loop", "shk_solve_log",
            "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
-           "shk_apply_operator", "shk_update_chebyshev_bounds", "shk_residual_norms", "shk_residual_norms_noalloc", "shk_apply_precond", "shk_download_system", "shk_time_kernel", "shk_profile_enable", "shk_profile_read",
-           "shk_profile_reset"]
+           "shk_apply_operator", "shk_update_chebyshev_bounds", "shk_residual_norms", "shk_residual_norms_noalloc",
+           "shk_apply_precond", "shk_download_system", "shk_set_sliding_coefficient", "shk_time_kernel",
+           "shk_profile_enable", "shk_profile_read", "shk_profile_reset"]
 # ... and of include/shakti_learned.h
 EXPORTS_LEARNED = ["shk_unet_create", "shk_unet_destroy", "shk_unet_last_error", "shk_unet_num_params",
                    "shk_unet_layer_info", "shk_unet_init_params", "shk_unet_set_params", "shk_unet_get_params",
@@ -51,7 +52,7 @@ class ShkParams(C.Structure):
 
 class ShkModel(C.Structure):
     _fields_ = [("kfs", C.c_int32), ("gap_scheme", C.c_int32), ("sliding_law", C.c_int32),
-                ("sliding_C", C.c_double)]
+                ("sliding_C", C.c_double), ("sliding_v0", C.c_double)]
 
 
 class ShkSolverCfg(C.Structure):
@@ -144,6 +145,7 @@ def load() -> C.CDLL:
     L.shk_update_chebyshev_bounds.argtypes = [vp, dp, dp, dp, C.POINTER(i32)]
     L.shk_residual_norms.argtypes = [vp, dp, dp]
     L.shk_residual_norms_noalloc.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp]
+    L.shk_set_sliding_coefficient.argtypes = [vp, dp, sz]
     L.shk_apply_precond.argtypes = [vp, dp, dp]
     L.shk_download_system.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.shk_time_kernel.argtypes = [vp, i32, i32, dp, dp]

@@ -94,6 +94,45 @@ class PrescribedSlidingLaw(AbstractSlidingLaw):
     """src/sliding_law.jl:45: taub fixed at set-up."""
 
 
+class _FieldSlidingLaw(AbstractSlidingLaw):
+    """A law whose coefficient C is a field given at the cell centres (an (nx, ny) array) or a
+    uniform scalar.  The constructors of the reference stagger C onto the faces on the host; here
+    the cell values are kept and staggered on the device (shk_set_sliding_coefficient) when a State
+    first uses the law, with the same face values (outer faces the cell's value, inner faces the
+    mean of the two cells)."""
+    code = -1
+
+    def __init__(self, g: Grid, C):
+        a = np.asarray(C, dtype=np.float64)
+        if a.ndim == 0:
+            a = np.full((g.nx, g.ny), float(a))
+        if a.shape != (g.nx, g.ny):
+            raise ValueError(f"C must be a scalar or an array of shape {(g.nx, g.ny)}, not {a.shape}")
+        self.grid = g
+        self.C_cells = np.asfortranarray(a)
+
+
+class LinearSlidingLaw(_FieldSlidingLaw):
+    """LinearSlidingLaw(g, C) (src/sliding_law.jl:61-90): taub = C^2 N ub on every face."""
+    code = 2
+
+
+class RegularizedCoulombFieldSlidingLaw(_FieldSlidingLaw):
+    """RegularizedCoulombFieldSlidingLaw(g, C) (src/sliding_law.jl:270-301): the regularised-Coulomb
+    stress with a field C."""
+    code = 3
+
+
+class RegularizedCoulombV0SlidingLaw(_FieldSlidingLaw):
+    """RegularizedCoulombV0SlidingLaw(g, C, v0) (src/sliding_law.jl:422-452): taub =
+    N C (|ub| / (|ub| + v0))^(1/n) ub / |ub|, v0 in the units of ub (m/s)."""
+    code = 4
+
+    def __init__(self, g: Grid, C, v0: float):
+        super().__init__(g, C)
+        self.v0 = float(v0)
+
+
 class AbstractMeltInput:
     def value(self, t: float):
         return None
@@ -154,7 +193,7 @@ class State:
         lib = L.load()
         self._ctx = C.c_void_p()
         cfg = _solver_cfg(None)
-        m = L.ShkModel(0, 0, 0, 0.5)
+        m = L.ShkModel(0, 0, 0, 0.5, 0.0)
         L.check(lib.shk_create(C.byref(g.c()), C.byref((p or ModelParameters()).c()), C.byref(m), C.byref(cfg),
                                C.byref(self._ctx)), None, "shk_create")
         weakref.finalize(self, _finalize_ctx, self._ctx)
@@ -249,7 +288,7 @@ class State:
         if p is not None:
             L.check(lib.shk_set_params(self._ctx, C.byref(p.c())), self._ctx, "shk_set_params")
         if kfs is not None or sl is not None or gap_scheme is not None:
-            m = getattr(self, "_model", L.ShkModel(0, 0, 0, 0.5))
+            m = getattr(self, "_model", L.ShkModel(0, 0, 0, 0.5, 0.0))
             if kfs is not None:
                 m.kfs = kfs.code
             if sl is not None:
@@ -257,6 +296,16 @@ class State:
                     m.sliding_law, m.sliding_C = 0, float(sl.C)
                 elif isinstance(sl, PrescribedSlidingLaw):
                     m.sliding_law = 1
+                elif isinstance(sl, _FieldSlidingLaw):
+                    if (sl.grid.nx, sl.grid.ny) != (self.grid.nx, self.grid.ny):
+                        raise ValueError("the sliding law was built on another grid")
+                    m.sliding_law, m.sliding_v0 = sl.code, getattr(sl, "v0", 0.0)
+                    # the coefficient goes up once per law object
+                    if getattr(self, "_sliding_bound", None) is not sl:
+                        c = sl.C_cells
+                        L.check(lib.shk_set_sliding_coefficient(self._ctx, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                                                c.size), self._ctx, "shk_set_sliding_coefficient")
+                        self._sliding_bound = sl
                 else:
                     raise L.ShaktiHIPError(f"unsupported sliding law {type(sl).__name__}")
             if gap_scheme is not None:

@@ -4,20 +4,23 @@ import numpy as np
 FIELDS_EXACT = ["mask", "valid_x", "valid_y", "ub_x", "ub_y"]
 
 
-def oracle_state(O, g_args, raw, p_kw=None, kfs=0, C=0.5, gap=0):
+def oracle_state(O, g_args, raw, p_kw=None, kfs=0, C=0.5, gap=0, model=None):
+    """``model(g)``, when given, builds the oracle Model (the field sliding laws need the grid)."""
     g = O.Grid(*g_args)
     p = O.ModelParameters(**(p_kw or {}))
-    m = O.Model(kfs=kfs, gap_scheme=gap, sliding_C=C)
+    m = model(g) if model is not None else O.Model(kfs=kfs, gap_scheme=gap, sliding_C=C)
     s = O.State(g)
     O.set_initial_conditions(s, g, p, m, **raw)
     return g, p, m, s
 
 
-def hip_state(H, g_args, raw, p_kw=None, C=0.5):
+def hip_state(H, g_args, raw, p_kw=None, C=0.5, law=None):
+    """``law(g)``, when given, builds the sliding law (the field laws need the grid)."""
     g = H.Grid(*g_args)
     p = H.ModelParameters(**(p_kw or {}))
     s = H.State(g, p)
-    H.set_initial_conditions(s, g, p, H.RegularizedCoulombSlidingLaw(C), **raw)
+    sl = law(g) if law is not None else H.RegularizedCoulombSlidingLaw(C)
+    H.set_initial_conditions(s, g, p, sl, **raw)
     return g, p, s
 
 

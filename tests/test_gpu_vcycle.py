@@ -186,3 +186,28 @@ def test_fused_cg_update_gives_the_same_iterates(hip, monkeypatch):
     print(f"CG iterations: fused update {out[0][0]}, separate {out[1][0]}", flush=True)
     assert out[0][0] == out[1][0] > 0
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precond", ["multigrid", "chebyshev"])
+def test_captured_graphs_give_the_iterates_of_kernel_by_kernel_launches(hip, monkeypatch, precond):
+    """CG batches are captured once per batch size into hipGraphs and replayed;
+    with SHK_NO_GRAPHS=1 the same kernels are launched one by one.  Two Picard
+    iterations (several batch sizes, graphs re-keyed between solves) give the
+    same CG counts and the same head, bit for bit."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(1030, 514, bed_amp=50.0, seed=8)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    out = []
+    for no_graphs in (False, True):
+        if no_graphs:
+            monkeypatch.setenv("SHK_NO_GRAPHS", "1")
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        kw = dict(multigrid=True) if precond == "multigrid" else dict(chebyshev_degree=4)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, rtol=1e-11, check_every=16, **kw)
+        ps = hip.PicardSolver(2, 1e-6, ls, hg)
+        hip.elliptic_solver(ps, hs, hg, hp, hip.Harmonic(), hip.RegularizedCoulombSlidingLaw(0.5))
+        out.append(([e["niter"] for e in hs.solve_log()], hs.get("h")))
+        monkeypatch.delenv("SHK_NO_GRAPHS", raising=False)
+    print(f"{precond}: CG iterations with graphs {out[0][0]}, without {out[1][0]}", flush=True)
+    assert out[0][0] == out[1][0] and len(out[0][0]) == 2
+    assert np.array_equal(out[0][1], out[1][1])
