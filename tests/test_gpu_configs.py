@@ -60,8 +60,6 @@ def test_config2_plain_fp64_matches_the_oracle_trajectory(oracle, hip):
     per-step max |h|, sum h, max b and sum b within the same bounds, and the
     face masks bit for bit."""
     from shakti_hip import synthetic as S
-    if not (GOLDEN / "config2_1024x256_200steps.npz").exists():
-        pytest.skip("the config-2 fixture is not generated yet (tests/golden/make_config2_fixture.py)")
     f = np.load(GOLDEN / "config2_1024x256_200steps.npz", allow_pickle=False)
     gg, raw = S.glacier(1024, 256, bed_amp=10.0, seed=2)
     moulin = S.moulin_field(gg, raw["mask"], n=20, seed=2)
