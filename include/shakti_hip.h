@@ -193,6 +193,46 @@ int shk_set_sliding_coefficient(shk_ctx* ctx, const double* C_host, size_t count
 int shk_upload(shk_ctx* ctx, int32_t field, const double* host, size_t count);
 int shk_download(shk_ctx* ctx, int32_t field, double* host, size_t count);
 
+/* Checkpoint / restart: save_checkpoint(path, sim, t) and
+ * load_checkpoint!(sim, path) (src/checkpoint.jl:26-64), which run! calls
+ * every checkpoint_every steps and on restart (src/run.jl:21-47).
+ *
+ * shk_checkpoint_save writes every State field plus t and total_time to
+ * path + ".tmp", flushes it to disk and renames it over path, so path always
+ * holds a complete checkpoint.  The fields are first copied on the device
+ * (the State may change as soon as the call returns); with async != 0 the
+ * copy to the host and the file write then proceed on a second stream and a
+ * host thread while later steps run, and shk_checkpoint_wait returns the
+ * write's outcome (it is also awaited by the next save, a load and
+ * shk_destroy).  The file is the library's own format: a 64-byte header
+ * (magic "SHKCKPT1", version 1, field count, nx, ny, t, total_time, lx, ly,
+ * data offset), one 64-byte record per field (the reference State's field
+ * name, id, rows, cols, offset, bytes, checksum), then each field in FP64,
+ * column-major, 512-byte aligned.  The checksum of n values with bit patterns
+ * w_i is sum_i splitmix64(w_i + i * 0x9E3779B97F4A7C15) mod 2^64.
+ *
+ * shk_checkpoint_load checks the header, the grid, every field's shape and
+ * checksum before it replaces any field (all or nothing), then returns t and
+ * total_time.  A failed check leaves the State as it was and returns
+ * SHK_ERR_ARG (unreadable, wrong grid or shape) or SHK_ERR_NUMERIC (checksum). */
+int shk_checkpoint_save(shk_ctx* ctx, const char* path, int64_t t, double total_time, int32_t async);
+int shk_checkpoint_wait(shk_ctx* ctx);
+int shk_checkpoint_load(shk_ctx* ctx, const char* path, int64_t* t, double* total_time);
+
+/* Asynchronous field output for observers (src/observer.jl:263-348 reads the
+ * tracked fields at each tracked time).  shk_snapshot copies the listed
+ * fields (all of them for field_ids NULL or nfields 0) on the device and
+ * queues their copy to pinned host memory on a second stream, so the caller
+ * can go on stepping; shk_snapshot_fetch waits for that copy and hands out one
+ * field of the last snapshot. */
+int shk_snapshot(shk_ctx* ctx, const int32_t* field_ids, int32_t nfields);
+int shk_snapshot_fetch(shk_ctx* ctx, int32_t field, double* host, size_t count);
+
+/* min, max and mean of one State field, reduced on the device: the row the
+ * CSV observer writes per tracked field (extrema and mean,
+ * src/observer.jl:324-339); NaN anywhere gives NaN for all three. */
+int shk_field_stats(shk_ctx* ctx, int32_t field, double* out3);
+
 /* set_initial_conditions! (src/initial_conditions.jl:14-126) on the device.
  * The raw inputs mask, A_visc, zb, zs, b, G, ub_x, ub_y, ieb (and taub_x,
  * taub_y for the prescribed law) must have been uploaded first. */

@@ -20,6 +20,7 @@
 
 #include "shakti_learned.h"
 #include "shk_internal.hpp"
+#include "shk_io.hpp"
 
 using namespace shk;
 
@@ -66,6 +67,8 @@ struct shk_ctx {
     double* moulin = nullptr;
     // face coefficient of the field sliding laws (shk_set_sliding_coefficient)
     double *slide_cx = nullptr, *slide_cy = nullptr;
+    // snapshots, checkpoints and field statistics (shk_io.hip), created on first use
+    shkio::Snapshot* io = nullptr;
 
     // linear system (FP64) and workspace
     double *aP = nullptr, *aE = nullptr, *aN = nullptr, *rhs = nullptr;
@@ -1364,6 +1367,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
 int shk_destroy(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     if (ctx->st) hipStreamSynchronize(ctx->st);
+    shkio::destroy(ctx->io);  // joins a running checkpoint writer
     for (int f = 0; f < SHK_NUM_FIELDS; ++f) hipFree(ctx->F.f[f]);
     void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->slide_cx, ctx->slide_cy, ctx->r2, ctx->r2_32, ctx->cP, ctx->dg32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
                     ctx->pA, ctx->pB, ctx->Ap, ctx->s, ctx->cA, ctx->cB, ctx->d64, ctx->aP32, ctx->aE32,
@@ -1465,6 +1469,106 @@ int shk_download(shk_ctx* ctx, int32_t field, double* host, size_t count) {
     HIPCHK(hipMemcpyAsync(host, ctx->F.f[field], count * 8, hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return SHK_OK;
+}
+
+// ---------------------------------------------------------------- snapshots and checkpoints
+
+namespace {
+
+// the reference State's field names, in shk_field order (src/state.jl:13-57)
+const char* const kFieldNames[SHK_NUM_FIELDS] = {
+    "h",    "pw",   "po",    "b",      "beta",   "abs_ub", "mdot",    "shear",   "potential", "sensible",
+    "Re",   "K",    "G",     "zb",     "zs",     "H",      "ieb",     "lambda",  "A_visc",    "N",
+    "mask", "dhdx", "q_x",   "Re_x",   "b_x",    "ub_x",   "taub_x",  "dpwdx",   "valid_x",   "dhdy",
+    "q_y",  "Re_y", "b_y",   "ub_y",   "taub_y", "dpwdy",  "valid_y"};
+
+void field_descs(const shk_ctx* c, shkio::FieldDesc* d) {
+    for (int f = 0; f < SHK_NUM_FIELDS; ++f) {
+        d[f].name = kFieldNames[f];
+        d[f].rows = c->g.nx + (f > SHK_MASK && f <= SHK_VALID_X ? 1 : 0);
+        d[f].cols = c->g.ny + (f > SHK_VALID_X ? 1 : 0);
+        d[f].dev = c->F.f[f];
+    }
+}
+
+int io_ready(shk_ctx* ctx) {
+    if (ctx->io) return SHK_OK;
+    std::string e;
+    ctx->io = shkio::create(e);
+    return ctx->io ? SHK_OK : fail(ctx, SHK_ERR_HIP, e);
+}
+
+int snapshot_all(shk_ctx* ctx) {
+    shkio::FieldDesc d[SHK_NUM_FIELDS];
+    field_descs(ctx, d);
+    int ids[SHK_NUM_FIELDS];
+    for (int f = 0; f < SHK_NUM_FIELDS; ++f) ids[f] = f;
+    std::string e;
+    const int rc = shkio::begin(ctx->io, ctx->st, SHK_NUM_FIELDS, ids, d, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
+}
+
+}  // namespace
+
+int shk_snapshot(shk_ctx* ctx, const int32_t* field_ids, int32_t nfields) {
+    if (!ctx || nfields < 0) return fail(ctx, SHK_ERR_ARG, "bad arguments");
+    if (int rc = io_ready(ctx)) return rc;
+    if (!field_ids || nfields == 0) return snapshot_all(ctx);
+    shkio::FieldDesc d[SHK_NUM_FIELDS];
+    field_descs(ctx, d);
+    std::vector<int> ids(field_ids, field_ids + nfields);
+    std::string e;
+    const int rc = shkio::begin(ctx->io, ctx->st, nfields, ids.data(), d, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
+}
+
+int shk_snapshot_fetch(shk_ctx* ctx, int32_t field, double* host, size_t count) {
+    if (!ctx || !host) return fail(ctx, SHK_ERR_ARG, "null argument");
+    if (!ctx->io) return fail(ctx, SHK_ERR_STATE, "no snapshot has been taken");
+    std::string e;
+    const int rc = shkio::fetch(ctx->io, field, host, count, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
+}
+
+int shk_checkpoint_save(shk_ctx* ctx, const char* path, int64_t t, double total_time, int32_t async) {
+    if (!ctx || !path || !*path) return fail(ctx, SHK_ERR_ARG, "a checkpoint needs a path");
+    if (int rc = io_ready(ctx)) return rc;
+    if (int rc = snapshot_all(ctx)) return rc;
+    std::string e;
+    const int rc = shkio::write_checkpoint(ctx->io, path, ctx->g.nx, ctx->g.ny, ctx->g.lx, ctx->g.ly, t, total_time,
+                                           async != 0, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
+}
+
+int shk_checkpoint_wait(shk_ctx* ctx) {
+    if (!ctx) return SHK_ERR_ARG;
+    if (!ctx->io) return SHK_OK;
+    std::string e;
+    const int rc = shkio::wait(ctx->io, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
+}
+
+int shk_checkpoint_load(shk_ctx* ctx, const char* path, int64_t* t, double* total_time) {
+    if (!ctx || !path || !t || !total_time) return fail(ctx, SHK_ERR_ARG, "null argument");
+    if (int rc = io_ready(ctx)) return rc;
+    shkio::FieldDesc d[SHK_NUM_FIELDS];
+    field_descs(ctx, d);
+    std::string e;
+    const int rc = shkio::read_checkpoint(ctx->io, ctx->st, path, ctx->g.nx, ctx->g.ny, SHK_NUM_FIELDS, d, t,
+                                          total_time, e);
+    if (rc) return fail(ctx, rc, e);
+    ctx->have_system = false;  // the assembled system belongs to the State that was replaced
+    return SHK_OK;
+}
+
+int shk_field_stats(shk_ctx* ctx, int32_t field, double* out) {
+    if (!ctx || !out) return fail(ctx, SHK_ERR_ARG, "null argument");
+    const size_t n = field_count(ctx, field);
+    if (!n) return fail(ctx, SHK_ERR_ARG, "unknown field id " + std::to_string(field));
+    if (int rc = io_ready(ctx)) return rc;
+    std::string e;
+    const int rc = shkio::field_stats(ctx->io, ctx->st, ctx->F.f[field], n, out, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
 }
 
 int shk_set_initial_conditions(shk_ctx* ctx) {

@@ -29,8 +29,9 @@ EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "sh
            "shk_solve_parabolic", "shk_update_derived", "shk_picard_iteration", "shk_picard_loop", "shk_solve_log",
            "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
            "shk_apply_operator", "shk_update_chebyshev_bounds", "shk_residual_norms", "shk_residual_norms_noalloc",
-           "shk_apply_precond", "shk_download_system", "shk_set_sliding_coefficient", "shk_time_kernel",
-           "shk_profile_enable", "shk_profile_read", "shk_profile_reset"]
+           "shk_apply_precond", "shk_download_system", "shk_set_sliding_coefficient", "shk_checkpoint_save",
+           "shk_checkpoint_wait", "shk_checkpoint_load", "shk_snapshot", "shk_snapshot_fetch", "shk_field_stats",
+           "shk_time_kernel", "shk_profile_enable", "shk_profile_read", "shk_profile_reset"]
 # ... and of include/shakti_learned.h
 EXPORTS_LEARNED = ["shk_unet_create", "shk_unet_destroy", "shk_unet_last_error", "shk_unet_num_params",
                    "shk_unet_layer_info", "shk_unet_init_params", "shk_unet_set_params", "shk_unet_get_params",
@@ -146,6 +147,12 @@ def load() -> C.CDLL:
     L.shk_residual_norms.argtypes = [vp, dp, dp]
     L.shk_residual_norms_noalloc.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp]
     L.shk_set_sliding_coefficient.argtypes = [vp, dp, sz]
+    L.shk_checkpoint_save.argtypes = [vp, C.c_char_p, i64, dbl, i32]
+    L.shk_checkpoint_wait.argtypes = [vp]
+    L.shk_checkpoint_load.argtypes = [vp, C.c_char_p, C.POINTER(i64), dp]
+    L.shk_snapshot.argtypes = [vp, C.POINTER(i32), i32]
+    L.shk_snapshot_fetch.argtypes = [vp, i32, dp, sz]
+    L.shk_field_stats.argtypes = [vp, i32, dp]
     L.shk_apply_precond.argtypes = [vp, dp, dp]
     L.shk_download_system.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.shk_time_kernel.argtypes = [vp, i32, i32, dp, dp]

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import time
 import weakref
 from dataclasses import dataclass, fields
@@ -18,6 +19,7 @@ from dataclasses import dataclass, fields
 import numpy as np
 
 from . import _lib as L
+from . import observer as O
 
 GROUNDED, OCEAN, LAND, OTHER_BASIN = 0.0, 1.0, 2.0, 3.0  # src/mask.jl:21-41
 SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)            # Krylov's default atol/rtol
@@ -230,6 +232,29 @@ class State:
 
     def to_dict(self) -> dict:
         return {n: self.get(n) for n in L.FIELD_NAMES}
+
+    def snapshot(self, names=None) -> None:
+        """Copy the named fields (every field for None) on the device and queue
+        their copy to pinned host memory (shk_snapshot); the State may change
+        once this returns.  ``fetch`` hands them out."""
+        if names is None:
+            L.check(L.load().shk_snapshot(self._ctx, None, 0), self._ctx, "shk_snapshot")
+            return
+        ids = (C.c_int32 * len(names))(*[L.FIELD_ID[n] for n in names])
+        L.check(L.load().shk_snapshot(self._ctx, ids, len(names)), self._ctx, "shk_snapshot")
+
+    def fetch(self, name: str) -> np.ndarray:
+        """One field of the last snapshot (waits for its copy)."""
+        out = np.empty(self.shape_of(name), order="F")
+        L.check(L.load().shk_snapshot_fetch(self._ctx, L.FIELD_ID[name], out.ctypes.data_as(C.POINTER(C.c_double)),
+                                            out.size), self._ctx, f"shk_snapshot_fetch {name}")
+        return out
+
+    def stats(self, name: str) -> tuple[float, float, float]:
+        """(min, max, mean) of a field, reduced on the device (shk_field_stats)."""
+        out = (C.c_double * 3)()
+        L.check(L.load().shk_field_stats(self._ctx, L.FIELD_ID[name], out), self._ctx, f"shk_field_stats {name}")
+        return float(out[0]), float(out[1]), float(out[2])
 
     # per-kernel HIP-event timing inside the library (see shk_profile_*)
     PROFILE_IDS = ["assemble", "lanczos", "cheb_bounds", "cg_init", "cheb_pass", "cg_matvec", "cg_update",
@@ -594,14 +619,19 @@ class ParabolicHeadScheme:
 
 
 class Simulation:
-    """src/simulation.jl:105-184 (observers are not part of this library:
-    tracked_obs must be empty)."""
+    """src/simulation.jl:105-184, with the observer choices of :162-181
+    (which_observer "IO" or "Live"; which_file_writer "CSV" or "NetCDF" --
+    "HDF5" and "JLD2" need an HDF5 library this installation lacks)."""
 
     def __init__(self, grid: Grid, state: State, tsteps: int, dt: float, p: ModelParameters, gap_scheme_choice: str,
                  tracked_obs, mi: AbstractMeltInput, sl: AbstractSlidingLaw, ps: PicardSolver | None = None,
-                 ls: AbstractLinearSolver | None = None, k_face_choice: str = "arithmetic", verbose: bool = False):
-        if tracked_obs:
-            raise L.ShaktiHIPError("observers are outside this library; pass tracked_obs=[]")
+                 ls: AbstractLinearSolver | None = None, which_observer: str | None = None,
+                 which_file_writer: str | None = None, tracked_times=None, path=None,
+                 k_face_choice: str = "arithmetic", verbose: bool = False):
+        tracked_obs = list(tracked_obs)
+        for name in tracked_obs:
+            if name not in L.FIELD_ID:
+                raise L.ShaktiHIPError(f'Unknown tracked observable: "{name}" is not a field of State')
         if p.e_v == 0:
             if ps is None:
                 raise L.ShaktiHIPError("ps (a PicardSolver) must be provided when p.e_v == 0 (elliptic head scheme)")
@@ -630,6 +660,35 @@ class Simulation:
                     state.ctx, "shk_set_moulin_field")
         self.step_times: list[float] = []
         self.picard_log: list[tuple] = []
+        self.observer = _make_observer(tracked_obs, which_observer, which_file_writer, tracked_times, path)
+
+
+def _make_observer(tracked_obs, which_observer, which_file_writer, tracked_times, path):
+    """The observer set-up of Simulation (src/simulation.jl:148-181), same errors."""
+    if not tracked_obs:
+        return O.NoObserver()
+    if which_observer is None:
+        raise L.ShaktiHIPError('which_observer should be specified as "IO" or "Live" when tracked_obs is not empty.')
+    if which_observer == "IO":
+        writers = {"NetCDF": O.NetCDFFileWriter, "HDF5": O.HDF5FileWriter, "JLD2": O.JLD2FileWriter,
+                   "CSV": O.CSVFileWriter}
+        if which_file_writer is None:
+            raise L.ShaktiHIPError('which_file_writer should be specified as "NetCDF", "HDF5", "JLD2", or "CSV" '
+                                   'when which_observer is "IO".')
+        if which_file_writer not in writers:
+            raise L.ShaktiHIPError(f'Unknown which_file_writer: "{which_file_writer}" '
+                                   '(expected "NetCDF", "HDF5", "JLD2", or "CSV")')
+        fr = writers[which_file_writer]()
+        if tracked_times is None:
+            raise L.ShaktiHIPError('tracked_times should be specified when which_observer is "IO".')
+        if path is None:
+            raise L.ShaktiHIPError('path should be specified when which_observer is "IO".')
+        return O.IOObserver(tracked_obs, tracked_times, fr, path)
+    if which_observer == "Live":
+        if tracked_times is None:
+            raise L.ShaktiHIPError('tracked_times should be specified when which_observer is "Live".')
+        return O.LiveObserver(tracked_obs, tracked_times)
+    raise L.ShaktiHIPError(f'Unknown which_observer: "{which_observer}" (expected "IO" or "Live")')
 
 
 def picard_status(hs):
@@ -665,15 +724,56 @@ def step(sim: Simulation) -> None:
         sim.hs.ls.ws.stats.niter = cgi.value
 
 
-def run(sim: Simulation) -> None:
-    """src/run.jl:21-63 without observers/checkpoints: steps 1..tsteps,
-    total_time += dt after each, the per-step report line of :51."""
-    sim.total_time = 0.0
-    for t in range(1, sim.tsteps + 1):
+def save_checkpoint(path, sim: Simulation, t: int, asynchronous: bool = False) -> None:
+    """save_checkpoint (src/checkpoint.jl:26-38): every State field, t and
+    total_time, written to path + ".tmp" and renamed over path.  With
+    asynchronous, the call returns once the fields are copied on the device;
+    the copy to the host and the file write go on behind later steps, and
+    wait_checkpoint (or the next save) reports their outcome."""
+    L.check(L.load().shk_checkpoint_save(sim.state.ctx, os.fsencode(str(path)), int(t), float(sim.total_time),
+                                         1 if asynchronous else 0), sim.state.ctx, "shk_checkpoint_save")
+
+
+def wait_checkpoint(sim: Simulation) -> None:
+    """Waits for an asynchronous save_checkpoint and raises if it failed."""
+    L.check(L.load().shk_checkpoint_wait(sim.state.ctx), sim.state.ctx, "shk_checkpoint_wait")
+
+
+def load_checkpoint(sim: Simulation, path) -> int:
+    """load_checkpoint! (src/checkpoint.jl:47-64): restores every State field
+    and sim.total_time in place (all or nothing, each field's checksum checked
+    first) and returns the saved timestep."""
+    t, tt = C.c_int64(), C.c_double()
+    L.check(L.load().shk_checkpoint_load(sim.state.ctx, os.fsencode(str(path)), C.byref(t), C.byref(tt)),
+            sim.state.ctx, "shk_checkpoint_load")
+    sim.total_time = tt.value
+    return int(t.value)
+
+
+def run(sim: Simulation, checkpoint_every: int | None = None, checkpoint_path=None, restart_path=None) -> None:
+    """run! (src/run.jl:21-63): steps from 1 (or from the checkpoint at
+    restart_path) to sim.tsteps, total_time += dt after each, the observer,
+    a checkpoint every checkpoint_every steps and the per-step report line of
+    :51.  Checkpoints are written asynchronously (the next steps run while
+    the previous one goes to disk); run returns once the last one is on disk."""
+    if (checkpoint_every is None) != (checkpoint_path is None):
+        raise L.ShaktiHIPError("checkpoint_every and checkpoint_path must be given together")
+    if restart_path is None:
+        sim.total_time = 0.0
+        O.prepare(sim.observer, sim.state)
+        O.observe(sim.observer, sim.state, 0, sim.total_time)
+        start_t = 0
+    else:
+        start_t = load_checkpoint(sim, restart_path)
+        O.resume(sim.observer, sim.state, start_t)
+    for t in range(start_t + 1, sim.tsteps + 1):
         t0 = time.perf_counter()
         step(sim)
         dt_wall = time.perf_counter() - t0
         sim.total_time += sim.dt
+        O.observe(sim.observer, sim.state, t, sim.total_time)
+        if checkpoint_every is not None and t % checkpoint_every == 0:
+            save_checkpoint(checkpoint_path, sim, t, asynchronous=True)
         sim.step_times.append(dt_wall)
         conv, last = picard_status(sim.hs)
         sim.picard_log.append((conv, last))
@@ -682,6 +782,9 @@ def run(sim: Simulation) -> None:
             li = "missing" if last is None else last
             print(f"{t} / {sim.tsteps} completed in {round(dt_wall, 4)}s. Picard converged: {c} in {li} iterations",
                   flush=True)
+    if checkpoint_every is not None:
+        wait_checkpoint(sim)
+    O.finalize(sim.observer, sim.state)
 
 
 def abi_version() -> int:
