@@ -238,6 +238,37 @@ int shk_field_stats(shk_ctx* ctx, int32_t field, double* out3);
  * taub_y for the prescribed law) must have been uploaded first. */
 int shk_set_initial_conditions(shk_ctx* ctx);
 
+/* The synthetic glacier of the benchmark configurations, generated on the
+ * device: writes the raw inputs of shk_set_initial_conditions (mask, A_visc,
+ * zb, zs, b, G, ub_x, ub_y, ieb, taub_x, taub_y) and, with n_moulins > 0, the
+ * moulin field of ieb mode 2.  No reference counterpart: the reference's
+ * examples build these inputs on the host (src/initial_conditions.jl:14-126
+ * takes them as arguments); this removes the host construction and upload of
+ * large grids.  Definition, reproducible bit for bit by a CPU restatement:
+ *   stream  o_i = SplitMix64 output i from state seed;
+ *   noise   w_c = sum_{k<12} (o_{12c+k} >> 16) 2^-48 - 6, c = ix + iy nx;
+ *   bed     periodic Gaussian smoothing along x then y, weights
+ *           exp(-(k/s)^2/2) normalised over k = -R..R, s = corr_len /
+ *           (sqrt(2) dx) (dy along y), R = ceil(4 s) (none for corr_len 0),
+ *           so the covariance is exp(-r^2 / (2 corr_len^2)); then shifted to
+ *           zero mean and scaled to standard deviation bed_amp; minus a
+ *           Gaussian trough depth exp(-((y - y0) / (width/2))^2 / 2),
+ *           y0 = (1/4 + u/2) ly, when trough_width > 0;
+ *   zs      6 (sqrt(x + 5000) - sqrt(5000)) + 1, x = ix dx;
+ *   mask    GROUNDED except the ix = 0 column (OCEAN); A_visc, b = b0, G,
+ *           ieb uniform; ub_x = ub, ub_y = 0, taub = 0;
+ *   moulins n_moulins distinct interior cells, Q ~ U[q_min, q_max] m^3/s,
+ *           moulin = Q / (dx dy) there and 0 elsewhere.
+ * Requires nx, ny >= 2, and nx, ny >= 3 for moulins. */
+typedef struct {
+    uint64_t seed;
+    double bed_amp, corr_len, trough_width, trough_depth;
+    double b0, A_visc, ub, G, ieb;
+    int32_t n_moulins, reserved;
+    double q_min, q_max;
+} shk_glacier_spec;
+int shk_generate_glacier(shk_ctx* ctx, const shk_glacier_spec* spec);
+
 /* solve_elliptic_linear_system!(ls, s, g, p, kfs) for the matrix-free CG
  * solver (src/linear_solver.jl:915-925): assemble, refresh Jacobi and the
  * Chebyshev bounds, PCG warm-started from h; overwrites device h. */

@@ -55,6 +55,13 @@ class _Model(C.Structure):
                 ("Cy", C.POINTER(C.c_double))]
 
 
+class _GlacierSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64)] + [(n, C.c_double) for n in ["bed_amp", "corr_len", "trough_width",
+                                                                    "trough_depth", "b0", "A_visc", "ub", "G",
+                                                                    "ieb"]] + \
+        [("n_moulins", C.c_int32), ("reserved", C.c_int32), ("q_min", C.c_double), ("q_max", C.c_double)]
+
+
 class _State(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ALL_FIELDS]
 
@@ -132,6 +139,10 @@ def lib():
         L.orc_step.restype = C.c_int32
         L.orc_num_threads.restype = C.c_int
         L.orc_stagger_c.argtypes = [C.POINTER(_Grid), dp, dp, dp]
+        L.orc_generate_glacier.argtypes = [C.POINTER(_Grid), C.POINTER(_GlacierSpec), C.POINTER(_State), dp]
+        L.orc_generate_glacier.restype = C.c_int
+        L.orc_splitmix64.argtypes = [C.c_uint64]
+        L.orc_splitmix64.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -527,3 +538,27 @@ def perturb_like_reference_tests(s: State, g: Grid, p: ModelParameters, m: Model
 if __name__ == "__main__":
     build(force=True)
     print("oracle built:", _LIB_PATH, "threads:", num_threads(), os.environ.get("OMP_NUM_THREADS"))
+
+
+RAW_INPUTS = ["mask", "A_visc", "zb", "zs", "b", "G", "ub_x", "ub_y", "ieb", "taub_x", "taub_y"]
+
+
+def generate_glacier(g: Grid, seed: int = 0, bed_amp: float = 10.0, corr_len: float = 2000.0,
+                     trough_width: float = 0.0, trough_depth: float = 0.0, b0: float = 0.005, A: float = 2.5e-25,
+                     ub: float = 1e-6, G: float = 0.05, ieb: float = 1e-8, n_moulins: int = 0, qmin: float = 1.0,
+                     qmax: float = 10.0):
+    """The device glacier generator restated (orc_generate_glacier): returns
+    (raw inputs dict for set_initial_conditions, moulin field or None)."""
+    s = State(g)
+    moulin = np.zeros((g.nx, g.ny), order="F") if n_moulins > 0 else None
+    spec = _GlacierSpec(int(seed), bed_amp, corr_len, trough_width, trough_depth, b0, A, ub, G, ieb,
+                        int(n_moulins), 0, qmin, qmax)
+    rc = lib().orc_generate_glacier(C.byref(g.c()), C.byref(spec), C.byref(s.c()),
+                                    _p(moulin) if moulin is not None else C.cast(0, C.POINTER(C.c_double)))
+    if rc != 0:
+        raise ValueError("bad generator arguments")
+    return {n: getattr(s, n) for n in RAW_INPUTS}, moulin
+
+
+def splitmix64(z: int) -> int:
+    return int(lib().orc_splitmix64(z))

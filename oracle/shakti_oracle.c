@@ -1192,3 +1192,194 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------
+ * The synthetic glacier of shk_generate_glacier, restated on the CPU (no
+ * reference counterpart: the reference takes these inputs as arguments,
+ * src/initial_conditions.jl:14-126).  Same stream, same loops, same order of
+ * every sum, so the fields agree bit for bit:
+ *   o_i = SplitMix64 output i from state seed (Vigna's generator);
+ *   noise w_c = sum_{k<12} (o_{12c+k} >> 16) 2^-48 - 6;
+ *   periodic Gaussian smoothing along x then y (weights exp(-(k/s)^2/2)
+ *   normalised over k = -R..R, s = corr_len / (sqrt(2) d), R = ceil(4 s));
+ *   zero mean and standard deviation bed_amp (sums over 16384 interleaved
+ *   chunks, then 128 groups of 128 chunk sums, then the groups);
+ *   Gaussian trough; zs = 6 (sqrt(x + 5000) - sqrt(5000)) + 1; uniform fields;
+ *   moulins on distinct interior cells.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t seed;
+    double bed_amp, corr_len, trough_width, trough_depth;
+    double b0, A_visc, ub, G, ieb;
+    int32_t n_moulins, reserved;
+    double q_min, q_max;
+} orc_glacier_spec;
+
+#define GEN_PHI 0x9E3779B97F4A7C15ull
+#define GEN_NCHUNK 16384
+
+static uint64_t gen_splitmix64(uint64_t z) {
+    z += GEN_PHI;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t orc_splitmix64(uint64_t z) { return gen_splitmix64(z); }
+
+static double gen_unit53(uint64_t o) { return (double)(o >> 11) * 0x1p-53; }
+
+/* exp(-(k/s)^2/2) normalised over k = -R..R; returns R (weights in *w, malloc'd) */
+static int gen_weights(double s, double** w) {
+    if (!(s > 0.0)) {
+        *w = malloc(sizeof(double));
+        (*w)[0] = 1.0;
+        return 0;
+    }
+    const int R = (int)ceil(4.0 * s);
+    double* v = malloc((size_t)(2 * R + 1) * sizeof(double));
+    double sum = 0.0;
+    for (int k = -R; k <= R; ++k) {
+        const double t = (double)k / s;
+        v[k + R] = exp(-0.5 * t * t);
+        sum = sum + v[k + R];
+    }
+    for (int k = 0; k <= 2 * R; ++k) v[k] = v[k] / sum;
+    *w = v;
+    return R;
+}
+
+static double gen_sum(const double* f, size_t n, double mean, int square) {
+    double* part = malloc(GEN_NCHUNK * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < GEN_NCHUNK; ++c) {
+        double acc = 0.0;
+        for (size_t i = (size_t)c; i < n; i += GEN_NCHUNK) {
+            double v = f[i];
+            if (square) {
+                v = v - mean;
+                v = v * v;
+            }
+            acc = acc + v;
+        }
+        part[c] = acc;
+    }
+    double grp[128];
+    for (int t = 0; t < 128; ++t) {
+        double acc = 0.0;
+        for (int c = 0; c < 128; ++c) acc = acc + part[128 * t + c];
+        grp[t] = acc;
+    }
+    double s = 0.0;
+    for (int t = 0; t < 128; ++t) s = s + grp[t];
+    free(part);
+    return s;
+}
+
+/* Writes mask, A_visc, zb, zs, b, G, ub_x, ub_y, ieb, taub_x, taub_y of s and,
+ * when moulin is not NULL, the moulin field.  Returns 0, or -1 on bad input. */
+int orc_generate_glacier(const orc_grid* g, const orc_glacier_spec* sp, orc_state* s, double* moulin) {
+    const int nx = g->nx, ny = g->ny;
+    const size_t n = (size_t)nx * ny;
+    if (nx < 2 || ny < 2) return -1;
+    if (sp->n_moulins > 0 && (nx < 3 || ny < 3 || (int64_t)sp->n_moulins > (int64_t)(nx - 2) * (ny - 2))) return -1;
+    double* w = malloc(n * sizeof(double));
+    double* tmp = malloc(n * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < n; ++c) {
+        const uint64_t base = sp->seed + (uint64_t)(12 * c) * GEN_PHI;
+        double acc = 0.0;
+        for (int k = 0; k < 12; ++k) acc = acc + (double)(gen_splitmix64(base + (uint64_t)k * GEN_PHI) >> 16) * 0x1p-48;
+        w[c] = acc - 6.0;
+    }
+    double *wx, *wy;
+    const int Rx = gen_weights(sp->corr_len / (sqrt(2.0) * g->dx), &wx);
+    const int Ry = gen_weights(sp->corr_len / (sqrt(2.0) * g->dy), &wy);
+#pragma omp parallel for schedule(static)
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            int j = (ix - Rx) % nx;
+            if (j < 0) j += nx;
+            double acc = 0.0;
+            for (int k = 0; k <= 2 * Rx; ++k) {
+                acc = acc + wx[k] * w[(size_t)iy * nx + j];
+                if (++j == nx) j = 0;
+            }
+            tmp[(size_t)iy * nx + ix] = acc;
+        }
+#pragma omp parallel for schedule(static)
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            int j = (iy - Ry) % ny;
+            if (j < 0) j += ny;
+            double acc = 0.0;
+            for (int k = 0; k <= 2 * Ry; ++k) {
+                acc = acc + wy[k] * tmp[(size_t)j * nx + ix];
+                if (++j == ny) j = 0;
+            }
+            w[(size_t)iy * nx + ix] = acc;
+        }
+    const double mean = gen_sum(w, n, 0.0, 0) / (double)n;
+    const double sd = sqrt(gen_sum(w, n, mean, 1) / (double)n);
+    const double scale = sd > 0.0 ? sp->bed_amp / sd : 1.0;
+    const int has_trough = sp->trough_width > 0.0 && sp->trough_depth != 0.0;
+    uint64_t next = 12 * (uint64_t)n;
+    double y0 = 0.0;
+    if (has_trough) y0 = (0.25 + 0.5 * gen_unit53(gen_splitmix64(sp->seed + next * GEN_PHI))) * g->ly;
+    ++next;
+    const double sig = sp->trough_width / 2.0;
+    for (int iy = 0; iy < ny; ++iy) {
+        double prof = 0.0;
+        if (has_trough) {
+            const double t = ((double)iy * g->dy - y0) / sig;
+            prof = exp(-0.5 * t * t);
+        }
+        for (int ix = 0; ix < nx; ++ix) {
+            const size_t c = C_(ix, iy);
+            const double d = w[c] - mean;
+            double z = sd > 0.0 ? d * scale : d;
+            if (has_trough) z = z - sp->trough_depth * prof;
+            s->zb[c] = z;
+            s->zs[c] = 6.0 * (sqrt((double)ix * g->dx + 5000.0) - sqrt(5000.0)) + 1.0;
+            s->mask[c] = ix == 0 ? OCEAN : GROUNDED;
+            s->A_visc[c] = sp->A_visc;
+            s->b[c] = sp->b0;
+            s->G[c] = sp->G;
+            s->ieb[c] = sp->ieb;
+        }
+    }
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix <= nx; ++ix) {
+            s->ub_x[XF_(ix, iy)] = sp->ub;
+            s->taub_x[XF_(ix, iy)] = 0.0;
+        }
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            s->ub_y[YF_(ix, iy)] = 0.0;
+            s->taub_y[YF_(ix, iy)] = 0.0;
+        }
+    if (moulin && sp->n_moulins > 0) {
+        memset(moulin, 0, n * sizeof(double));
+        const int64_t mx = nx - 2, M = (int64_t)(nx - 2) * (ny - 2);
+        int64_t* cells = malloc((size_t)sp->n_moulins * sizeof(int64_t));
+        for (int k = 0; k < sp->n_moulins; ++k) {
+            int64_t c;
+            for (;;) {
+                const int64_t j = (int64_t)(gen_unit53(gen_splitmix64(sp->seed + next++ * GEN_PHI)) * (double)M);
+                c = (1 + j % mx) + (1 + j / mx) * (int64_t)nx;
+                int taken = 0;
+                for (int q = 0; q < k; ++q) taken |= cells[q] == c;
+                if (!taken) break;
+            }
+            const double u = gen_unit53(gen_splitmix64(sp->seed + next++ * GEN_PHI));
+            cells[k] = c;
+            moulin[c] = (sp->q_min + (sp->q_max - sp->q_min) * u) / (g->dx * g->dy);
+        }
+        free(cells);
+    }
+    free(w);
+    free(tmp);
+    free(wx);
+    free(wy);
+    return 0;
+}

@@ -1571,6 +1571,27 @@ int shk_field_stats(shk_ctx* ctx, int32_t field, double* out) {
     return rc ? fail(ctx, rc, e) : SHK_OK;
 }
 
+int shk_generate_glacier(shk_ctx* ctx, const shk_glacier_spec* spec) {
+    if (!ctx || !spec) return fail(ctx, SHK_ERR_ARG, "null argument");
+    const shk_glacier_spec& s = *spec;
+    const int64_t interior = (int64_t)(ctx->g.nx - 2) * (ctx->g.ny - 2);
+    if (ctx->g.nx < 2 || ctx->g.ny < 2) return fail(ctx, SHK_ERR_ARG, "the generator needs nx, ny >= 2");
+    if (!(s.bed_amp >= 0.0) || !(s.corr_len >= 0.0) || !(s.trough_width >= 0.0) || !std::isfinite(s.trough_depth) ||
+        !std::isfinite(s.bed_amp) || !std::isfinite(s.corr_len) || !std::isfinite(s.trough_width))
+        return fail(ctx, SHK_ERR_ARG, "bed_amp, corr_len and trough_width must be finite and >= 0");
+    if (s.corr_len / (sqrt(2.0) * std::min(ctx->g.dx, ctx->g.dy)) > 1e6)
+        return fail(ctx, SHK_ERR_ARG, "corr_len spans more than a million cells");
+    if (s.n_moulins < 0 || (s.n_moulins > 0 && (ctx->g.nx < 3 || ctx->g.ny < 3 || s.n_moulins > interior)))
+        return fail(ctx, SHK_ERR_ARG, "n_moulins must be between 0 and the number of interior cells");
+    if (s.n_moulins > 0 && !(s.q_max >= s.q_min))
+        return fail(ctx, SHK_ERR_ARG, "q_max must be >= q_min");
+    if (s.n_moulins > 0 && !ctx->moulin) HIPCHK(hipMalloc((void**)&ctx->moulin, ctx->n * 8));
+    std::string e;
+    const int rc = shkgen::generate(ctx->g, s, ctx->F, ctx->d64, ctx->r, s.n_moulins > 0 ? ctx->moulin : nullptr,
+                                    ctx->st, e);
+    return rc ? fail(ctx, rc, e) : SHK_OK;
+}
+
 int shk_set_initial_conditions(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     if (int rc = check_sliding(ctx)) return rc;

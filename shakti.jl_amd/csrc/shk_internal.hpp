@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "shakti_hip.h"
 
 namespace shk {
@@ -294,3 +296,11 @@ void relax(size_t n, double* h, const double* h_prev, double alpha, hipStream_t 
 }  // namespace launch
 
 }  // namespace shk
+
+// The synthetic glacier generator (shk_gen.hip): writes the raw inputs of
+// set_initial_conditions into F, and the moulin field when moulin is given,
+// using w and tmp (nx*ny each) as scratch.  Synchronous on st.
+namespace shkgen {
+int generate(const shk_grid& g, const shk_glacier_spec& sp, shk::Fields F, double* w, double* tmp, double* moulin,
+             hipStream_t st, std::string& err);
+}  // namespace shkgen

@@ -30,7 +30,7 @@ EXPORTS = ["shk_abi_version", "shk_last_error", "shk_create", "shk_destroy", "sh
            "shk_parabolic_step", "shk_step_b", "shk_set_moulin_field", "shk_update_ieb", "shk_step", "shk_assemble",
            "shk_apply_operator", "shk_update_chebyshev_bounds", "shk_residual_norms", "shk_residual_norms_noalloc",
            "shk_apply_precond", "shk_download_system", "shk_set_sliding_coefficient", "shk_checkpoint_save",
-           "shk_checkpoint_wait", "shk_checkpoint_load", "shk_snapshot", "shk_snapshot_fetch", "shk_field_stats",
+           "shk_checkpoint_wait", "shk_checkpoint_load", "shk_snapshot", "shk_snapshot_fetch", "shk_field_stats", "shk_generate_glacier",
            "shk_time_kernel", "shk_profile_enable", "shk_profile_read", "shk_profile_reset"]
 # ... and of include/shakti_learned.h
 EXPORTS_LEARNED = ["shk_unet_create", "shk_unet_destroy", "shk_unet_last_error", "shk_unet_num_params",
@@ -54,6 +54,13 @@ class ShkParams(C.Structure):
 class ShkModel(C.Structure):
     _fields_ = [("kfs", C.c_int32), ("gap_scheme", C.c_int32), ("sliding_law", C.c_int32),
                 ("sliding_C", C.c_double), ("sliding_v0", C.c_double)]
+
+
+class ShkGlacierSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64)] + [(n, C.c_double) for n in ["bed_amp", "corr_len", "trough_width",
+                                                                    "trough_depth", "b0", "A_visc", "ub", "G",
+                                                                    "ieb"]] + \
+        [("n_moulins", C.c_int32), ("reserved", C.c_int32), ("q_min", C.c_double), ("q_max", C.c_double)]
 
 
 class ShkSolverCfg(C.Structure):
@@ -153,6 +160,7 @@ def load() -> C.CDLL:
     L.shk_snapshot.argtypes = [vp, C.POINTER(i32), i32]
     L.shk_snapshot_fetch.argtypes = [vp, i32, dp, sz]
     L.shk_field_stats.argtypes = [vp, i32, dp]
+    L.shk_generate_glacier.argtypes = [vp, C.POINTER(ShkGlacierSpec)]
     L.shk_apply_precond.argtypes = [vp, dp, dp]
     L.shk_download_system.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.shk_time_kernel.argtypes = [vp, i32, i32, dp, dp]

@@ -170,11 +170,12 @@ class SeasonalMeltInput(AbstractMeltInput):
 class SeasonalMoulinMeltInput(SeasonalMeltInput):
     """A SeasonalMeltInput plus a static field of point moulins added on top
     (ieb = uniform(t) + moulin).  No counterpart in the reference, whose
-    update_ieb! overwrites ieb uniformly (src/melt_input.jl:92)."""
+    update_ieb! overwrites ieb uniformly (src/melt_input.jl:92).  moulin None
+    uses the field already on the device (synthetic.generate_on_device)."""
 
-    def __init__(self, moulin: np.ndarray, **kw):
+    def __init__(self, moulin: np.ndarray | None, **kw):
         super().__init__(**kw)
-        self.moulin = np.asfortranarray(np.asarray(moulin, dtype=np.float64))
+        self.moulin = None if moulin is None else np.asfortranarray(np.asarray(moulin, dtype=np.float64))
 
 
 # ------------------------------------------------------------------ State
@@ -346,6 +347,15 @@ def set_initial_conditions(s: State, g: Grid, p: ModelParameters, sl: AbstractSl
     for name, val in [("mask", mask), ("A_visc", A_visc), ("zb", zb), ("zs", zs), ("b", b), ("G", G),
                       ("ub_x", ub_x), ("ub_y", ub_y), ("ieb", ieb), ("taub_x", taub_x), ("taub_y", taub_y)]:
         s.set(name, val)
+    L.check(L.load().shk_set_initial_conditions(s.ctx), s.ctx, "shk_set_initial_conditions")
+    return s
+
+
+def set_initial_conditions_on_device(s: State, g: Grid, p: ModelParameters, sl: AbstractSlidingLaw) -> State:
+    """set_initial_conditions! from the raw inputs already in the device
+    State (written by synthetic.generate_on_device or uploaded field by
+    field): nothing crosses PCIe."""
+    s._bind(p=p, sl=sl)
     L.check(L.load().shk_set_initial_conditions(s.ctx), s.ctx, "shk_set_initial_conditions")
     return s
 
@@ -654,7 +664,7 @@ class Simulation:
         self.gap_scheme = 0 if gap_scheme_choice == "implicit" else 1
         self.mi, self.sl, self.verbose = mi, sl, verbose
         self.total_time = 0.0
-        if isinstance(mi, SeasonalMoulinMeltInput):
+        if isinstance(mi, SeasonalMoulinMeltInput) and mi.moulin is not None:
             a = mi.moulin
             L.check(L.load().shk_set_moulin_field(state.ctx, a.ctypes.data_as(C.POINTER(C.c_double)), a.size),
                     state.ctx, "shk_set_moulin_field")

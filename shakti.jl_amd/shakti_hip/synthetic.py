@@ -17,6 +17,12 @@ bit-identical inputs from the same seed:
   ``G = 0.05`` W/m^2, ``ieb = 1e-8`` m/s, ``taub = 0``;
 * moulins: ``n`` distinct GROUNDED interior cells chosen with the seed, each
   with ``Q ~ U[qmin, qmax]`` m^3/s, giving ``ieb += Q / (dx dy)``.
+
+``generate_on_device`` builds a glacier of the same description on the GPU
+(shk_generate_glacier) from a counter-based stream (SplitMix64) instead of
+numpy's PCG64 and a truncated direct convolution instead of the FFT, so its
+fields differ from ``glacier``'s for the same seed; a CPU restatement
+reproduces them bit for bit.
 """
 from __future__ import annotations
 
@@ -96,6 +102,20 @@ def glacier(nx: int, ny: int, bed_amp: float = 10.0, corr_len: float = 2000.0, s
         taub_y=np.zeros((nx, ny + 1), order="F"),
     )
     return g, raw
+
+
+def generate_on_device(s, seed: int = 0, bed_amp: float = 10.0, corr_len: float = 2000.0,
+                       trough_width: float = 0.0, trough_depth: float = 0.0, b0: float = 0.005, A: float = 2.5e-25,
+                       ub: float = 1e-6, G: float = 0.05, ieb: float = 1e-8, n_moulins: int = 0, qmin: float = 1.0,
+                       qmax: float = 10.0) -> None:
+    """Writes the raw inputs of set_initial_conditions (and, with n_moulins,
+    the moulin field) into the device State ``s`` (shk_generate_glacier);
+    follow with api.set_initial_conditions_on_device."""
+    import ctypes as C
+    from . import _lib as L
+    spec = L.ShkGlacierSpec(int(seed), bed_amp, corr_len, trough_width, trough_depth, b0, A, ub, G, ieb,
+                            int(n_moulins), 0, qmin, qmax)
+    L.check(L.load().shk_generate_glacier(s.ctx, C.byref(spec)), s.ctx, "shk_generate_glacier")
 
 
 def seasonal_c2(t: float) -> float:
