@@ -104,6 +104,15 @@ typedef struct {
 #define SHK_CHEB_BOUNDS_GERSHGORIN 0
 #define SHK_CHEB_BOUNDS_REFERENCE 1
 
+/* shk_solver_cfg.flags.  PICARD_ATOL: inside a Picard loop, every solve after
+ * the first stops at atol + rtol * rNorm0 with atol raised to rtol times the
+ * first solve's initial residual norm, so a later solve, which starts from a
+ * head already close to the fixed point, is held to the first solve's
+ * absolute target instead of reducing its own small residual by rtol again.
+ * No reference counterpart (Krylov's atol is a fixed number); off (0) by
+ * default. */
+#define SHK_CFG_PICARD_ATOL 1
+
 /* Linear-solver configuration -- reference CGIterativeSolver(g,
  * MatrixFreeLinearSystem; chebyshev_degree, chebyshev_nsteps_estimate)
  * (src/linear_solver.jl:832-843) plus Krylov's stopping controls. */
@@ -117,7 +126,7 @@ typedef struct {
     double atol, rtol;        /* stop when sqrt(r'M^-1 r) <= atol + rtol*initial; Krylov default sqrt(eps) each */
     double inner_rtol;        /* FP32_REFINE: relative reduction asked of each inner solve */
     int32_t cheb_bounds;      /* SHK_CHEB_BOUNDS_* (ABI 2) */
-    int32_t reserved;         /* 0 */
+    int32_t flags;            /* SHK_CFG_* bits (ABI 2) */
 } shk_solver_cfg;
 
 /* Statistics of the last linear solve (reference ws.stats, ls.precond bounds). */

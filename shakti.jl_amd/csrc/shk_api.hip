@@ -69,6 +69,8 @@ struct shk_ctx {
     double *slide_cx = nullptr, *slide_cy = nullptr;
     // snapshots, checkpoints and field statistics (shk_io.hip), created on first use
     shkio::Snapshot* io = nullptr;
+    // a lower bound on atol for the solves of a Picard loop (SHK_CFG_PICARD_ATOL)
+    double atol_floor = 0.0;
 
     // linear system (FP64) and workspace
     double *aP = nullptr, *aE = nullptr, *aN = nullptr, *rhs = nullptr;
@@ -292,6 +294,7 @@ int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
     if (cfg->cheb_bounds != SHK_CHEB_BOUNDS_GERSHGORIN && cfg->cheb_bounds != SHK_CHEB_BOUNDS_REFERENCE)
         return fail(ctx, SHK_ERR_ARG, "unknown Chebyshev bound mode");
     if (!(cfg->atol >= 0) || !(cfg->rtol >= 0)) return fail(ctx, SHK_ERR_ARG, "atol/rtol must be >= 0");
+    if (cfg->flags & ~SHK_CFG_PICARD_ATOL) return fail(ctx, SHK_ERR_ARG, "unknown solver flag bits");
     return SHK_OK;
 }
 
@@ -975,7 +978,7 @@ int solve_ir(shk_ctx* ctx, int parabolic, double dt, double relax_alpha, shk_sol
     HIPCHK(hipMemcpyAsync(x, h_prev, n * 8, hipMemcpyDeviceToDevice, ctx->st));
     int rc = reset_scalars(ctx, 0.0, 0.0, itmax);
     if (rc) return rc;
-    ctx->sc_host->ir_atol = cfg.atol;
+    ctx->sc_host->ir_atol = std::max(cfg.atol, ctx->atol_floor);
     ctx->sc_host->ir_rtol = cfg.rtol;
     HIPCHK(hipMemcpyAsync(&ctx->sc->ir_atol, &ctx->sc_host->ir_atol, 2 * sizeof(double), hipMemcpyHostToDevice,
                           ctx->st));
@@ -1117,7 +1120,7 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     const double* x0 = ctx->F.f[SHK_H];
     const size_t slot = (size_t)ctx->rec_count * ctx->W8 * ctx->H8;
     if (learned || record) {
-        int rc0 = reset_scalars(ctx, cfg.atol, cfg.rtol, itmax);
+        int rc0 = reset_scalars(ctx, std::max(cfg.atol, ctx->atol_floor), cfg.rtol, itmax);
         if (rc0) return rc0;
         void* xin = record ? (void*)((char*)ctx->rec_x + slot * 16) : ctx->nn_x;
         {
@@ -1154,7 +1157,8 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     // the learned preconditioner needs a fixed preconditioner to compose with
     const int nn_every = (learned && (multigrid || cheb)) ? ctx->nn_every : 0;
     int stalled = 0;
-    int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every, cfg.atol, cfg.rtol, itmax,
+    int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every,
+                         std::max(cfg.atol, ctx->atol_floor), cfg.rtol, itmax,
                          nn_every, &stalled);
     if (rc) return rc;
     shk_solve_stats sst = stats_of(ctx, cheb);
@@ -1649,10 +1653,17 @@ int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every
                     int32_t* converged, int32_t* last_iter, int64_t* cg_iters) {
     if (!ctx || iters < 1 || check_every < 1) return fail(ctx, SHK_ERR_ARG, "iters and check_every must be >= 1");
     int32_t conv = 0, last = 0;
+    struct FloorReset {
+        shk_ctx* c;
+        ~FloorReset() { c->atol_floor = 0.0; }
+    } floor_reset{ctx};
+    ctx->atol_floor = 0.0;
     for (int32_t it = 1; it <= iters; ++it) {
         shk_solve_stats st;
         int rc = solve(ctx, 0, 0.0, relax_alpha, &st);
         if (rc) return rc;
+        if (it == 1 && (ctx->cfg.flags & SHK_CFG_PICARD_ATOL) && std::isfinite(st.rnorm0))
+            ctx->atol_floor = ctx->cfg.rtol * st.rnorm0;
         if (cg_iters) *cg_iters += st.niter;
         const int check = (it % check_every == 0) || (it == iters);
         rc = derived(ctx, ctx->h_alt /* previous h */, check, tol);

@@ -55,7 +55,7 @@ const SHK_ABI_VERSION = 2
 struct ShkSolverCfg
     precond::Int32; cheb_degree::Int32; cheb_nsteps::Int32; form::Int32; itmax::Int32; check_every::Int32
     atol::Float64; rtol::Float64; inner_rtol::Float64
-    cheb_bounds::Int32; reserved::Int32
+    cheb_bounds::Int32; flags::Int32
 end
 mutable struct ShkSolveStats
     niter::Int32; solved::Int32; zero_curvature::Int32; bounds_warning::Int32; refinements::Int32; fallback::Int32

@@ -66,7 +66,7 @@ class ShkGlacierSpec(C.Structure):
 class ShkSolverCfg(C.Structure):
     _fields_ = [("precond", C.c_int32), ("cheb_degree", C.c_int32), ("cheb_nsteps", C.c_int32), ("form", C.c_int32),
                 ("itmax", C.c_int32), ("check_every", C.c_int32), ("atol", C.c_double), ("rtol", C.c_double),
-                ("inner_rtol", C.c_double), ("cheb_bounds", C.c_int32), ("reserved", C.c_int32)]
+                ("inner_rtol", C.c_double), ("cheb_bounds", C.c_int32), ("flags", C.c_int32)]
 
 
 CHEB_BOUNDS = {"gershgorin": 0, "reference": 1}  # SHK_CHEB_BOUNDS_*

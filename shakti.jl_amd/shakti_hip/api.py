@@ -447,7 +447,8 @@ class CGIterativeSolver(AbstractIterativeSolver):
                  chebyshev_nsteps_estimate: int = 15, amg: bool = False, atol: float = SQRT_EPS,
                  rtol: float = SQRT_EPS, itmax: int = 0, form: str = "fp64", check_every: int = 8,
                  inner_rtol: float = 1e-3, multigrid: bool = False, network=None,
-                 learned_precond_every: int = 0, stall_ratio: float = 1.0, chebyshev_bounds: str = "gershgorin"):
+                 learned_precond_every: int = 0, stall_ratio: float = 1.0, chebyshev_bounds: str = "gershgorin",
+                 picard_atol: bool = False):
         if system is SparseAssembledLinearSystem:
             raise L.ShaktiHIPError("CGIterativeSolver(g, SparseAssembledLinearSystem) is CPU-only; use "
                                    "HIPMatrixFreeLinearSystem on the GPU.")
@@ -485,6 +486,8 @@ class CGIterativeSolver(AbstractIterativeSolver):
         self.check_every = int(check_every)
         self.inner_rtol = float(inner_rtol)
         self.chebyshev_bounds = chebyshev_bounds
+        # inside a Picard loop, later solves stop at rtol x the first solve's initial residual (SHK_CFG_PICARD_ATOL)
+        self.picard_atol = bool(picard_atol)
         self.ws = _Workspace()
         self.precond = None if chebyshev_degree is None else ChebyshevPreconditioner(chebyshev_degree,
                                                                                      chebyshev_nsteps_estimate)
@@ -511,7 +514,8 @@ def _solver_cfg(ls: CGIterativeSolver | None) -> L.ShkSolverCfg:
     cheb = ls.chebyshev_degree is not None
     precond = 2 if ls.multigrid else (1 if cheb else 0)
     return L.ShkSolverCfg(precond, ls.chebyshev_degree or 1, ls.nsteps_estimate, FORMS[ls.form], ls.itmax,
-                          ls.check_every, ls.atol, ls.rtol, ls.inner_rtol, L.CHEB_BOUNDS[ls.chebyshev_bounds], 0)
+                          ls.check_every, ls.atol, ls.rtol, ls.inner_rtol, L.CHEB_BOUNDS[ls.chebyshev_bounds],
+                          1 if ls.picard_atol else 0)
 
 
 def _bind_solver(s: State, ls: CGIterativeSolver):

@@ -525,3 +525,45 @@ def test_field_sliding_law_without_coefficient_is_refused(hip):
     assert rc == -4, rc  # SHK_ERR_STATE
     assert b"coefficient" in lib.shk_last_error(hs.ctx)
     assert np.array_equal(hs.get("taub_x"), before)
+
+
+@pytest.mark.parametrize("form", ["fp64_mg", "fp32_mg"])
+def test_picard_atol_trajectory_matches_oracle(oracle, hip, form):
+    """SHK_CFG_PICARD_ATOL (later Picard solves held to rtol x the first
+    solve's initial residual) at 96 x 40 with seasonal forcing and moulins:
+    the Picard counts of the oracle's Cholesky trajectory every step, h within
+    1e-9 and b within 1e-8, fewer CG iterations than without the flag, and an
+    unknown flag bit refused."""
+    import ctypes as C
+    from shakti_hip import _lib as L
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(96, 40, bed_amp=10.0, seed=7)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    moulin = S.moulin_field(gg, raw["mask"], n=5, seed=7)
+    kw = (dict(multigrid=True, rtol=1e-10, form="fp64") if form == "fp64_mg" else
+          dict(multigrid=True, rtol=1e-8, form="fp32_refine", inner_rtol=1e-4))
+    iters = {}
+    for flag in (False, True):
+        g, p, m, os_ = oracle_state(oracle, g_args, raw)
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        sv = oracle.Solver(g, "cholesky")
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, picard_atol=flag, **kw)
+        ps = hip.PicardSolver(500, 1e-6, ls, hg)
+        mi = hip.SeasonalMoulinMeltInput(moulin, **S.SEASONAL_C2)
+        sim = hip.Simulation(hg, hs, 6, 3600.0, hp, "implicit", [], mi, hip.RegularizedCoulombSlidingLaw(0.5), ps=ps)
+        total = 0
+        for t in range(6):
+            u = oracle.seasonal_ieb(sim.total_time, **S.SEASONAL_C2)
+            conv, last, _ = oracle.step(sv, os_, g, p, m, 3600.0, ieb_mode=2, ieb_uniform=u, moulin=moulin)
+            hip.step(sim)
+            sim.total_time += sim.dt
+            total += ps.cg_iters
+            assert (ps.converged, ps.last_iter) == (conv, last), (flag, t)
+            assert rel_err(hs.get("h"), os_.h) <= 1e-9, (flag, t)
+            assert rel_err(hs.get("b"), os_.b) <= 1e-8, (flag, t)
+        iters[flag] = total
+    print(f"{form}: CG iterations over 6 steps {iters[False]} without, {iters[True]} with the flag", flush=True)
+    assert iters[True] < iters[False]
+    cfg = ls.cfg()
+    cfg.flags = 2
+    assert L.load().shk_set_solver_cfg(hs.ctx, C.byref(cfg)) == -1  # SHK_ERR_ARG
