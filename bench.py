@@ -414,8 +414,8 @@ def roofline(args, best, r):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["elliptic", "parabolic"], default="elliptic",
                     help="elliptic: Picard to 1e-6 per step (4096x1024 defaults); parabolic: one backward-Euler "
                          "solve per step, e_v 1e-3, dt 600 s, 8192x2048, bed amplitude 100 m and a 2 km trough")
@@ -444,7 +444,7 @@ def main():
     ap.add_argument("--weights", default=str(ROOT / "shakti.jl_amd" / "shakti_hip" / "weights" /
                                              "unet_16_32_64_128.npz"),
                     help="learned_mg: trained U-Net parameters")
-    ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--cpu-iters", type=int, default=200)
     ap.add_argument("--c5-steps", type=int, default=5,
                     help="timed steps of the 8192x2048 parabolic summary (0: skip it and the streamed operator)")
     ap.add_argument("--max-picard", type=int, default=500,
