@@ -297,6 +297,53 @@ def streamed_and_parabolic(args):
                 "fastest": best, "forms": summ}
 
 
+def start_and_checkpoint(args, host_inputs_s):
+    """The device start (shk_generate_glacier + initial conditions) against
+    the host generator, and one asynchronous checkpoint of the whole State:
+    how long the caller is held (the device snapshot), how long the copy to
+    the host and the file write take, and the load back."""
+    import tempfile
+    import types
+    import shakti_hip as H
+    from shakti_hip import synthetic as S
+    g = H.Grid(args.nx, args.ny, 100e3, 20e3)
+    p = H.ModelParameters()
+    s = H.State(g, p)
+    S.generate_on_device(s, seed=args.seed, bed_amp=args.bed_amp, n_moulins=20)  # warm
+    t0 = time.perf_counter()
+    S.generate_on_device(s, seed=args.seed, bed_amp=args.bed_amp, n_moulins=20)
+    t1 = time.perf_counter()
+    H.set_initial_conditions_on_device(s, g, p, H.RegularizedCoulombSlidingLaw(0.5))
+    t2 = time.perf_counter()
+    sim = types.SimpleNamespace(state=s, total_time=3600.0)
+    out = {"grid": f"{args.nx}x{args.ny}", "device_generator_ms": round((t1 - t0) * 1e3, 2),
+           "device_initial_conditions_ms": round((t2 - t1) * 1e3, 2),
+           "host_generator_and_moulins_s": round(host_inputs_s, 3)}
+    with tempfile.TemporaryDirectory(prefix="shk_ckpt_") as d:
+        path = os.path.join(d, "state.shk")
+        H.save_checkpoint(path, sim, 1, asynchronous=True)  # warm: staging areas allocated
+        H.wait_checkpoint(sim)
+        t0 = time.perf_counter()
+        H.save_checkpoint(path, sim, 1, asynchronous=True)
+        t1 = time.perf_counter()
+        s.stats("h")  # a kernel queued behind the snapshot: returns once the device copy is done
+        ts = time.perf_counter()
+        H.wait_checkpoint(sim)
+        t2 = time.perf_counter()
+        nbytes = os.path.getsize(path)
+        t3 = time.perf_counter()
+        H.load_checkpoint(sim, path)
+        t4 = time.perf_counter()
+    out.update({"checkpoint_bytes": nbytes, "checkpoint_call_ms": round((t1 - t0) * 1e3, 2),
+                "checkpoint_device_copy_ms": round((ts - t0) * 1e3, 2),
+                "checkpoint_write_s": round(t2 - t0, 3), "checkpoint_GBs": round(nbytes / (t2 - t0) / 1e9, 2),
+                "load_s": round(t4 - t3, 3),
+                "note": "call: host time to queue the snapshot; device copy: until a kernel queued behind it on "
+                        "the library stream has run (the time later steps wait); write: until the file is on "
+                        "disk (copy to pinned host memory on a second stream, then a host thread)"})
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -469,8 +516,10 @@ def main():
         sys.exit(2)
 
     log(f"[bench] building inputs {args.nx}x{args.ny}, bed amplitude {args.bed_amp} m, seed {args.seed}")
+    t_in = time.perf_counter()
     g, raw, moulin = make_inputs(args.nx, args.ny, args.bed_amp, args.seed, trough=2000.0 if par else 0.0,
                                  trough_depth=args.trough_depth if par else 0.0)
+    host_inputs_s = time.perf_counter() - t_in
     forms = args.forms.split(",")
     runs = {}
     for form in forms:
@@ -491,6 +540,12 @@ def main():
     if not par and args.c5_steps > 0:
         log("[bench] 8192x2048: streamed operator and a short parabolic run ...")
         streamed, c5 = streamed_and_parabolic(args)
+    start_ckpt = None
+    try:
+        log("[bench] device start and an asynchronous checkpoint ...")
+        start_ckpt = start_and_checkpoint(args, host_inputs_s)
+    except Exception as e:  # reported, never required
+        log(f"[bench] start/checkpoint measurement failed: {e!r}")
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle restatement) on a bounded sample ...")
@@ -535,6 +590,7 @@ def main():
         "operator_streamed": streamed,
         "parabolic_8192x2048": c5,
         "kernels": kernel_table(r["prof"], r["prof_step"]),
+        "start_and_checkpoint": start_ckpt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -34,6 +34,7 @@
 #include <string.h>
 
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "shk_internal.hpp"
@@ -161,14 +162,14 @@ __global__ __launch_bounds__(256) void k_gen_finish(int nx, int ny, const double
     }
 }
 
-__global__ __launch_bounds__(256) void k_gen_moulins(size_t n, double* __restrict__ moulin, int m,
-                                                     const int64_t* __restrict__ cell, const double* __restrict__ val) {
-    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
-        double v = 0.0;
-        for (int k = 0; k < m; ++k)
-            if ((size_t)cell[k] == c) v = val[k];
-        moulin[c] = v;
-    }
+__global__ __launch_bounds__(256) void k_gen_zero(size_t n, double* __restrict__ a) {
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) a[c] = 0.0;
+}
+// one thread per moulin; the cells are distinct, so no two threads write one cell
+__global__ __launch_bounds__(256) void k_gen_moulins(int m, const int64_t* __restrict__ cell,
+                                                     const double* __restrict__ val, double* __restrict__ moulin) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < m) moulin[cell[k]] = val[k];
 }
 
 double unit53(uint64_t o) { return (double)(o >> 11) * 0x1p-53; }
@@ -232,14 +233,14 @@ int generate(const shk_grid& g, const shk_glacier_spec& sp, Fields F, double* w,
     std::vector<double> qs;
     if (sp.n_moulins > 0) {
         const int64_t mx = nx - 2, M = (int64_t)(nx - 2) * (ny - 2);
+        std::unordered_set<int64_t> taken;
+        taken.reserve((size_t)sp.n_moulins * 2);
         for (int k = 0; k < sp.n_moulins; ++k) {
             int64_t c;
             for (;;) {
                 const int64_t j = (int64_t)(unit53(splitmix64(sp.seed + next++ * PHI)) * (double)M);
                 c = (1 + j % mx) + (1 + j / mx) * (int64_t)nx;
-                bool taken = false;
-                for (int64_t q : cells) taken = taken || q == c;
-                if (!taken) break;
+                if (taken.insert(c).second) break;
             }
             const double u = unit53(splitmix64(sp.seed + next++ * PHI));
             cells.push_back(c);
@@ -291,9 +292,11 @@ int generate(const shk_grid& g, const shk_glacier_spec& sp, Fields F, double* w,
         hipLaunchKernelGGL(k_gen_finish, dim3((nx + 64) / 64, (ny + 4) / 4), blk, 0, st, nx, ny, w, mean,
                            sd > 0.0 ? sp.bed_amp / sd : 1.0, sd > 0.0 ? 1 : 0, has_trough ? dtr : nullptr,
                            sp.trough_depth, dzs, u, F);
-        if (moulin && !cells.empty())
-            hipLaunchKernelGGL(k_gen_moulins, dim3(blocks(n)), dim3(256), 0, st, n, moulin, (int)cells.size(), dcell,
-                               dq);
+        if (moulin && !cells.empty()) {
+            hipLaunchKernelGGL(k_gen_zero, dim3(blocks(n)), dim3(256), 0, st, n, moulin);
+            hipLaunchKernelGGL(k_gen_moulins, dim3((unsigned)((cells.size() + 255) / 256)), dim3(256), 0, st,
+                               (int)cells.size(), dcell, dq, moulin);
+        }
         GENCHK(hipGetLastError());
         GENCHK(hipStreamSynchronize(st));
         rc = SHK_OK;

@@ -101,7 +101,8 @@ class _FieldSlidingLaw(AbstractSlidingLaw):
     uniform scalar.  The constructors of the reference stagger C onto the faces on the host; here
     the cell values are kept and staggered on the device (shk_set_sliding_coefficient) when a State
     first uses the law, with the same face values (outer faces the cell's value, inner faces the
-    mean of the two cells)."""
+    mean of the two cells).  Like the reference's immutable laws, a law object's C is fixed once a
+    State has used it: build a new law to change C."""
     code = -1
 
     def __init__(self, g: Grid, C):

@@ -22,6 +22,7 @@ CASES = [
     ((1031, 517), dict(seed=4, bed_amp=50.0, corr_len=700.0, n_moulins=20)),
     ((2048, 1024), dict(seed=5, bed_amp=100.0, corr_len=2000.0, trough_width=2000.0, trough_depth=100.0)),
     ((96, 40), dict(seed=6, bed_amp=5.0, corr_len=0.0, b0=0.004, A=3e-25, ub=2e-6, G=0.06, ieb=2e-8)),
+    ((40, 12), dict(seed=9, n_moulins=380)),  # every interior cell: the draw-again rule to the end
 ]
 
 
