@@ -103,6 +103,7 @@ struct shk_ctx {
         float *c, *e, *n, *D, *b, *xa, *xb;
         bool owned;
         bool packed = false;  // c, e, n, D live in mg_pack (tail levels)
+        bool lin = false;     // reaches the next level by the linear transfers
     };
     std::vector<MGLevel> mg;
     float* mg_pack = nullptr;  // the tail's [c | e | n | D] (stride mg_stride) and the coarsest inverse
@@ -123,6 +124,15 @@ struct shk_ctx {
     // omega < 2 and scale < 2.
     int mg_rb = 1;
     float mg_omega = 1.2f, mg_scale = 1.8f, mg_scale_par = 1.0f;
+    // Transfers from level mg_lin_from down (red-black smoothing only): the
+    // cell-centred linear ones, R = P^T, with the lumped Galerkin coarse
+    // operator, in place of the 2 x 2 aggregation; their coarse correction
+    // needs no over-correction (scale 0.8 elliptic, 1.0 parabolic).  On the
+    // assembled glacier systems (1024 x 256, FP32/BF16 V-cycle, CG to 1e-10)
+    // 22 CG iterations against aggregation's 34 with the finest transfer
+    // still aggregating, 21 with every transfer linear.
+    int mg_lin_from = 1;
+    float mg_scale_lin = 0.8f, mg_scale_lin_par = 1.0f;
     // The finest level's c, aE, aN, D in BF16 for its red-black legs (where
     // those are the two-column kernels): half the coefficient bytes of the
     // two full-size V-cycle kernels, at no cost in CG iterations measured
@@ -130,6 +140,7 @@ struct shk_ctx {
     bool mg_bf16 = false;
     uint16_t *mg_c16 = nullptr, *mg_e16 = nullptr, *mg_n16 = nullptr, *mg_D16 = nullptr;
     size_t mg_tail_first = 1;                 // first level handled by the single-workgroup tail kernel
+    size_t mg_tail_h = 0;                     // the tail's linear-restriction staging (floats)
 
     // captured CG iteration batches, keyed by (element size, jacobi, degree, batch)
     struct GraphEntry {
@@ -477,26 +488,36 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
         // inverse in LDS: stay within what one workgroup may hold (160 KB on
         // gfx950), less the kernel's 4 KB of static LDS (degenerate shapes
         // such as n x 1 halve only once per level)
+        if (getenv("SHK_MG_JACOBI")) ctx->mg_rb = 0;
+        if (const char* e = getenv("SHK_MG_LIN_FROM")) ctx->mg_lin_from = atoi(e);  // negative: none
+        for (size_t l = 0; l + 1 < ctx->mg.size(); ++l)
+            ctx->mg[l].lin = ctx->mg_rb && ctx->mg_lin_from >= 0 && (int)l >= ctx->mg_lin_from;
         auto tail_cells = [&](size_t f) {
             size_t c = 0;
             for (size_t l = f; l < ctx->mg.size(); ++l) c += (size_t)ctx->mg[l].nx * ctx->mg[l].ny;
             return c;
+        };
+        // the linear restriction's staging rows in the tail (coarse width x fine height)
+        auto tail_h = [&](size_t f) {
+            size_t h = 0;
+            for (size_t l = f; l + 1 < ctx->mg.size(); ++l)
+                if (ctx->mg[l].lin) h = std::max(h, (size_t)ctx->mg[l + 1].nx * ctx->mg[l].ny);
+            return h;
         };
         int dev = 0, lds_max = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         const size_t ncl = (size_t)ctx->mg.back().nx * ctx->mg.back().ny;
         const size_t budget = lds_max > 4096 ? (size_t)lds_max - 4096 : 0;
-        while (ctx->mg_tail_first + 1 < ctx->mg.size() &&
-               launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl) > budget)
+        auto tail_lds = [&](size_t f) { return launch::mg_tail_lds(tail_cells(f), ncl, tail_h(f)); };
+        while (ctx->mg_tail_first + 1 < ctx->mg.size() && tail_lds(ctx->mg_tail_first) > budget)
             ++ctx->mg_tail_first;
         // above 64 KB the kernel must opt in; without it, stay within 64 KB
         const size_t dflt = 65536 - 4096;
-        if (launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl) > dflt &&
-            launch::mg_tail_reserve(launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl)))
-            while (ctx->mg_tail_first + 1 < ctx->mg.size() &&
-                   launch::mg_tail_lds(tail_cells(ctx->mg_tail_first), ncl) > dflt)
+        if (tail_lds(ctx->mg_tail_first) > dflt && launch::mg_tail_reserve(tail_lds(ctx->mg_tail_first)))
+            while (ctx->mg_tail_first + 1 < ctx->mg.size() && tail_lds(ctx->mg_tail_first) > dflt)
                 ++ctx->mg_tail_first;
+        ctx->mg_tail_h = tail_h(ctx->mg_tail_first);
         if (ctx->mg.size() - ctx->mg_tail_first > (size_t)MG_MAX_TAIL)
             return fail(ctx, SHK_ERR_STATE, "multigrid hierarchy deeper than the tail kernel holds");
         const size_t nc = (size_t)ctx->mg.back().nx * ctx->mg.back().ny;
@@ -519,13 +540,14 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
         }
         ctx->mg_inv = ctx->mg_pack + 4 * tp;
         if (getenv("SHK_MG_JACOBI")) {
-            ctx->mg_rb = 0;
             ctx->mg_omega = 0.95f;
             ctx->mg_scale = ctx->mg_scale_par = 1.5f;
         }
         if (const char* e = getenv("SHK_MG_OMEGA")) ctx->mg_omega = (float)atof(e);
         if (const char* e = getenv("SHK_MG_SCALE")) ctx->mg_scale = (float)atof(e);
         if (const char* e = getenv("SHK_MG_SCALE_PARABOLIC")) ctx->mg_scale_par = (float)atof(e);
+        if (const char* e = getenv("SHK_MG_SCALE_LIN")) ctx->mg_scale_lin = (float)atof(e);
+        if (const char* e = getenv("SHK_MG_SCALE_LIN_PARABOLIC")) ctx->mg_scale_lin_par = (float)atof(e);
     }
     auto& L0 = ctx->mg[0];
     L0.c = (float*)w.aP;
@@ -544,7 +566,10 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
     for (size_t l = 1; l < ctx->mg.size(); ++l) {
         auto& f = ctx->mg[l - 1];
         auto& c = ctx->mg[l];
-        launch::mg_coarsen(f.nx, f.ny, c.nx, c.ny, f.c, f.e, f.n, c.c, c.e, c.n, ctx->st);
+        if (f.lin)
+            launch::mg_coarsen_lin(f.nx, f.ny, c.nx, c.ny, f.c, f.e, f.n, c.c, c.e, c.n, ctx->st);
+        else
+            launch::mg_coarsen(f.nx, f.ny, c.nx, c.ny, f.c, f.e, f.n, c.c, c.e, c.n, ctx->st);
         launch::mg_diag(c.nx, c.ny, c.c, c.e, c.n, c.D, ctx->st);
     }
     auto& Lc = ctx->mg.back();
@@ -569,7 +594,8 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
 template <class T>
 void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, const launch::CGUpd* upd = nullptr) {
     auto& mg = ctx->mg;
-    const float scale = ctx->system_parabolic ? ctx->mg_scale_par : ctx->mg_scale;
+    const float scale_agg = ctx->system_parabolic ? ctx->mg_scale_par : ctx->mg_scale;
+    const float scale_lin = ctx->system_parabolic ? ctx->mg_scale_lin_par : ctx->mg_scale_lin;
     const size_t L = mg.size() - 1;
     const size_t first = ctx->mg_tail_first;  // >= 1
     // bytes per cell of the finest level: its rhs (T), its c, aE, aN, D (BF16 or
@@ -583,32 +609,61 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
         Prof pr(ctx, l == 0 ? P_MGD0 : P_MGD,
                 l == 0 ? (rb0 + cb0 + 4.0 + (upd ? 16.0 : 0.0)) * cells + 4.0 * coarse
                        : 4.0 * (6.0 * cells + coarse));
+        const auto& cl = mg[l + 1];
+        bool fused = false;
         if constexpr (sizeof(T) == 8) {
             if (l == 0 && upd) {
                 launch::mg_down_bf16_upd(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
                                          ctx->mg_D16, r, f.xa, mg[1].b, *upd, ctx->st);
-                continue;
+                fused = true;
             }
         }
-        if (l == 0 && ctx->mg_bf16)
+        if (fused)
+            ;
+        else if (l == 0 && ctx->mg_bf16)
             launch::mg_down_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
                                     ctx->mg_D16, r, f.xa, mg[1].b, ctx->sc, ctx->st);
         else if (l == 0)
             launch::mg_down<T>(f.nx, f.ny, mg[1].nx, mg[1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, r, f.xa, mg[1].b,
                                ctx->sc, ctx->mg_rb, ctx->st);
+        else if (f.lin && launch::mg_down_lin<float, float>(f.nx, f.ny, cl.nx, ctx->mg_omega, f.c, f.e, f.n, f.D,
+                                                                f.b, f.xa, cl.b, ctx->sc, ctx->st))
+            continue;
         else
             launch::mg_down<float>(f.nx, f.ny, mg[l + 1].nx, mg[l + 1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, f.b,
                                    f.xa, mg[l + 1].b, ctx->sc, ctx->mg_rb, ctx->st);
+        if (f.lin) {
+            // the linear restriction from the smoothed x the leg stored (its
+            // aggregated coarse rhs is overwritten)
+            if (l > 0)
+                launch::mg_restrict_lin<float, float>(f.nx, f.ny, cl.nx, cl.ny, f.c, f.e, f.n, f.b, f.xa, cl.b,
+                                                      ctx->sc, ctx->st);
+            else {
+                const T* b0 = r;
+                if constexpr (sizeof(T) == 8)
+                    if (upd) b0 = upd->b_out;
+                if (ctx->mg_bf16)
+                    launch::mg_restrict_lin<T, uint16_t>(f.nx, f.ny, cl.nx, cl.ny, ctx->mg_c16, ctx->mg_e16,
+                                                         ctx->mg_n16, b0, f.xa, cl.b, ctx->sc, ctx->st);
+                else
+                    launch::mg_restrict_lin<T, float>(f.nx, f.ny, cl.nx, cl.ny, f.c, f.e, f.n, b0, f.xa, cl.b,
+                                                      ctx->sc, ctx->st);
+            }
+        }
     }
     {
         MGTail Tl{};
         Tl.nlev = (int)(L - first + 1);
         Tl.omega = ctx->mg_omega;
-        Tl.scale = scale;
+        Tl.scale = scale_agg;
         Tl.pack = ctx->mg_pack;
         Tl.stride = ctx->mg_stride;
         Tl.inv = ctx->mg_inv;
         Tl.rb = ctx->mg_rb;
+        Tl.scale_lin = scale_lin;
+        Tl.hcells = (int)ctx->mg_tail_h;
+        for (size_t l = first; l < L; ++l)
+            if (mg[l].lin) Tl.lin_mask |= 1 << (l - first);
         double bytes = 0.0;
         for (size_t l = first; l <= L; ++l) {
             const auto& v = mg[l];
@@ -627,6 +682,17 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
         const double cells = (double)f.nx * f.ny, coarse = (double)mg[l + 1].nx * mg[l + 1].ny;
         Prof pr(ctx, l == 0 ? P_MGU0 : P_MGU,
                 l == 0 ? (rb0 + cb0 + 4.0 + zb0) * cells + 4.0 * coarse : 4.0 * (7.0 * cells + coarse));
+        if (l > 0 && f.lin &&
+            launch::mg_up_lin<float, float, float>(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, scale_lin, f.c, f.e,
+                                                   f.n, f.D, f.b, f.xa, mg[l + 1].xb, f.xb, -1, ctx->partials,
+                                                   ctx->sc, ctx->st))
+            continue;
+        // the linear prolongation into x first; the leg then adds nothing
+        float scale = scale_agg;
+        if (f.lin) {
+            launch::mg_prolong_lin(f.nx, f.ny, mg[l + 1].nx, scale_lin, f.xa, mg[l + 1].xb, ctx->sc, ctx->st);
+            scale = 0.f;
+        }
         if (l == 0 && ctx->mg_bf16 && zf)
             launch::mg_up_bf16<T, float>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
                                          ctx->mg_n16, ctx->mg_D16, r0, f.xa, mg[1].xb, zf, mode, ctx->partials,

@@ -103,6 +103,14 @@ struct MGTail {
     const float* pack;
     int stride;
     int rb;  // 1: red-black Gauss-Seidel smoothing, 0: damped Jacobi
+    // bit l: tail level l reaches level l+1 by the linear transfers (red-black
+    // smoothing only), with coarse-correction scale scale_lin; their
+    // restriction stages the x-restricted residual rows in LDS after the
+    // packed block (hcells floats, the largest such level's coarse width x
+    // fine height)
+    int lin_mask;
+    float scale_lin;
+    int hcells;
     MGLevelDev L[MG_MAX_TAIL];
 };
 
@@ -220,6 +228,25 @@ void cg_finish(int n, const double* x, const double* h_prev, double* h_new, doub
 void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float* ef, const float* nf, float* cc,
                 float* ec, float* nc, hipStream_t st);
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st);
+// the linear transfers: the lumped Galerkin coarse operator, the restriction
+// of a level's residual from its stored x (FP32 or BF16 coefficients), and
+// u = x + scale P xc in place
+void mg_coarsen_lin(int nxf, int nyf, int nxc, int nyc, const float* cf, const float* ef, const float* nf, float* cc,
+                    float* ec, float* nc, hipStream_t st);
+template <class TB, class CT>
+void mg_restrict_lin(int nx, int ny, int nxc, int nyc, const CT* c, const CT* e, const CT* n, const TB* b,
+                     const float* x, float* bc, const Scalars* sc, hipStream_t st);
+void mg_prolong_lin(int nx, int ny, int nxc, float scale, float* x, const float* xc, const Scalars* sc,
+                    hipStream_t st);
+// the red-black legs with the linear transfers fused in, where the level's
+// legs are the two-row kernels; false (nothing launched) elsewhere
+template <class TB, class CT>
+bool mg_down_lin(int nx, int ny, int nxc, float omega, const CT* c, const CT* e, const CT* n, const CT* D,
+                 const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
+template <class TB, class CT, class TO>
+bool mg_up_lin(int nx, int ny, int nxc, float omega, float scale, const CT* c, const CT* e, const CT* n,
+               const CT* D, const TB* b, const float* x, const float* xc, TO* out, int dot_mode, double* partials,
+               Scalars* sc, hipStream_t st);
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
              const float* D, const TB* b, float* x, float* bc, const Scalars* sc, int rb, hipStream_t st);
@@ -258,7 +285,7 @@ void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_
 // dynamic LDS bytes of the tail kernel (rhs, x, c, e, n, D per cell and the
 // coarsest inverse), and the opt-in for more than the default 64 KB (0 = ok)
 size_t mg_tail_stride(size_t tail_cells);
-size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells);
+size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells, size_t hcells = 0);
 int mg_tail_reserve(size_t bytes);
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, float* inv, hipStream_t st);
 

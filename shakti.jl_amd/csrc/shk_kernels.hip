@@ -1755,6 +1755,142 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(int nxf, int nyf, int nxc, i
     nc[(size_t)I + (size_t)J * nxc] = (J < nyc - 1) ? n : 0.f;
 }
 
+// Cell-centred linear transfers.  P gives fine cell x (in 1D) 3/4 of its own
+// coarse cell I0 = x / 2 and 1/4 of the coarse neighbour on its side
+// (I0 - 1 for even x, I0 + 1 for odd x); a neighbour outside the coarse grid
+// folds onto I0 (weight 1).  In 2D the weights multiply (9/16, 3/16, 3/16,
+// 1/16).  Restriction is P^T: coarse cell I gathers fine cells 2I-1 .. 2I+2
+// with weights 1/4, 3/4, 3/4, 1/4, where fine cell 0 (for I = 0) and an odd
+// last fine cell f - 1 (for I = (f+1)/2 - 1) carry weight 1 and cells outside
+// the grid none.
+__device__ __forceinline__ float lin_rw(int x, int q, int f) {  // fine x = 2I - 1 + q
+    if (x < 0 || x >= f) return 0.f;
+    if (q == 1 || q == 2) return (x == 0 || (x == f - 1 && (x & 1))) ? 1.f : 0.75f;
+    return 0.25f;
+}
+// Prolongation weights of fine x: its own coarse cell (w0) and the neighbour
+// on its side (w1 = 1 - w0, at index I1, folded into the grid).
+struct LinP {
+    int I0, I1;
+    float w0, w1;
+};
+__device__ __forceinline__ LinP lin_p(int x, int f) {
+    const int c = (f + 1) >> 1, I0 = x >> 1, I1 = (x & 1) ? I0 + 1 : I0 - 1;
+    const bool out = I1 < 0 || I1 > c - 1;
+    return LinP{I0, out ? I0 : I1, out ? 1.f : 0.75f, out ? 0.f : 0.25f};
+}
+
+// The coarse operator of the linear transfers: the Galerkin product P^T A P of
+// the flux-form operator, lumped back to the 5-point flux form.  c = P^T c;
+// a fine face of coupling a between cells i and j adds a (P_i - P_j)(P_i - P_j)^T,
+// whose links are laid onto the lattice paths between their end cells (half
+// on each of a diagonal link's two paths).  For a fine x-face the links along
+// y cancel against the diagonal halves and what remains lies on the coarse
+// x-faces of the two coarse rows it touches, weighted by the restriction
+// weights in y:
+//   ec(I, J) = sum_y wy(y, J) [a(2I+1, y) / 4 + (a(2I, y) + a(2I+2, y)) / 16]
+// (the face inside coarse cell I+1 or I reaches across the face I | I+1 with
+// a quarter of the quarter-weighted difference of its neighbours' values);
+// y-faces likewise.  Sums in FP64, one rounding to FP32.
+__global__ __launch_bounds__(256) void k_mg_coarsen_lin(int nxf, int nyf, int nxc, int nyc,
+                                                        const float* __restrict__ cf, const float* __restrict__ ef,
+                                                        const float* __restrict__ nf, float* __restrict__ cc,
+                                                        float* __restrict__ ec, float* __restrict__ nc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= nxc || J >= nyc) return;
+    double wx[4], wy[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        wx[q] = lin_rw(2 * I - 1 + q, q, nxf);
+        wy[q] = lin_rw(2 * J - 1 + q, q, nyf);
+    }
+    auto E = [&](int x, int y) -> double { return x < nxf - 1 ? (double)ef[(size_t)x + (size_t)y * nxf] : 0.0; };
+    auto N = [&](int x, int y) -> double { return y < nyf - 1 ? (double)nf[(size_t)x + (size_t)y * nxf] : 0.0; };
+    double c = 0.0, e = 0.0, n = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int y = 2 * J - 1 + q;
+        if (wy[q] == 0.0) continue;  // outside the grid
+        double s = 0.0;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int x = 2 * I - 1 + p;
+            if (wx[p] != 0.0) s += wx[p] * (double)cf[(size_t)x + (size_t)y * nxf];
+        }
+        c += wy[q] * s;
+        if (I < nxc - 1) e += wy[q] * (0.25 * E(2 * I + 1, y) + 0.0625 * (E(2 * I, y) + E(2 * I + 2, y)));
+    }
+    if (J < nyc - 1)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int x = 2 * I - 1 + p;
+            if (wx[p] != 0.0) n += wx[p] * (0.25 * N(x, 2 * J + 1) + 0.0625 * (N(x, 2 * J) + N(x, 2 * J + 2)));
+        }
+    const size_t k = (size_t)I + (size_t)J * nxc;
+    cc[k] = (float)c;
+    ec[k] = (float)e;
+    nc[k] = (float)n;
+}
+
+// A coefficient as FP32 from FP32 or BF16 storage.
+__device__ __forceinline__ float ldc1(const float* p, size_t k) { return p[k]; }
+__device__ __forceinline__ float ldc1(const uint16_t* p, size_t k) { return __uint_as_float((uint32_t)p[k] << 16); }
+
+// Restriction by P^T of the residual b - A x of a level whose x is stored:
+// one thread per coarse cell, the 4 x 4 fine residuals it gathers recomputed
+// from x (each fine residual is taken by up to four coarse cells).  Rows are
+// combined with the x weights first, then the rows with the y weights.
+template <class TB, class CT>
+__global__ __launch_bounds__(256) void k_mg_restrict_lin(int nx, int ny, int nxc, int nyc, const CT* __restrict__ c,
+                                                         const CT* __restrict__ e, const CT* __restrict__ n,
+                                                         const TB* __restrict__ b, const float* __restrict__ x,
+                                                         float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= nxc || J >= nyc) return;
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int iy = 2 * J - 1 + q;
+        const float wy = lin_rw(iy, q, ny);
+        if (wy == 0.f) continue;
+        float h = 0.f;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int ix = 2 * I - 1 + p;
+            const float wx = lin_rw(ix, p, nx);
+            if (wx == 0.f) continue;
+            const size_t k = (size_t)ix + (size_t)iy * nx;
+            const float u = x[k];
+            float a = ldc1(c, k) * u;
+            if (ix < nx - 1) a += ldc1(e, k) * (u - x[k + 1]);
+            if (ix > 0) a += ldc1(e, k - 1) * (u - x[k - 1]);
+            if (iy < ny - 1) a += ldc1(n, k) * (u - x[k + nx]);
+            if (iy > 0) a += ldc1(n, k - nx) * (u - x[k - nx]);
+            h += wx * ((float)b[k] - a);
+        }
+        acc += wy * h;
+    }
+    bc[(size_t)I + (size_t)J * nxc] = acc;
+}
+
+// u = x + scale P xc in place, one thread per fine cell.
+__global__ __launch_bounds__(256) void k_mg_prolong_lin(int nx, int ny, int nxc, float scale, float* __restrict__ x,
+                                                        const float* __restrict__ xc, const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    const LinP px = lin_p(ix, nx), py = lin_p(iy, ny);
+    auto X = [&](int I, int J) -> float { return xc[(size_t)I + (size_t)J * nxc]; };
+    const float v0 = px.w0 * X(px.I0, py.I0) + px.w1 * X(px.I1, py.I0);
+    const float v1 = px.w0 * X(px.I0, py.I1) + px.w1 * X(px.I1, py.I1);
+    const size_t k = (size_t)ix + (size_t)iy * nx;
+    x[k] = x[k] + scale * (py.w0 * v0 + py.w1 * v1);
+}
+
 // Jacobi diagonal of a flux-form level: c + aE + aW + aN + aS.
 __global__ __launch_bounds__(256) void k_mg_diag(int nx, int ny, const float* __restrict__ c,
                                                  const float* __restrict__ e, const float* __restrict__ n,
@@ -2466,53 +2602,70 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
 // indices clamped into the grid, rows outside it zeroed after the load) is
 // issued at once, and the done flag is read with them; the arithmetic per
 // row is that of k_mg_down_rb2 / k_mg_up_rb2, in the same order.
-template <class TB, class CT>
+// LIN: the level reaches the next one by the linear transfers.  The
+// restriction of coarse row J = iy0 / 2 gathers the residuals of fine rows
+// iy0-1 .. iy0+2 and, in each, of the columns one beyond the lane's pair on
+// either side; so the wave forms one more row of x above and below (E = 1)
+// and keeps two halo lanes on each side of its owned columns (RB_OWN_LIN):
+// the inner halo lane's x is exact (its black cell's outer red neighbour is
+// in the outer halo lane), and so is its residual.
+constexpr int RB_OWN_LIN = 2 * (WAVE - 4);
+template <bool LIN>
+struct RBLanes {
+    static constexpr int HALO = LIN ? 2 : 1, OWN = 2 * (WAVE - 2 * HALO);
+};
+template <class TB, class CT, bool LIN = false>
 __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, float omega,
                                                       const CT* __restrict__ c, const CT* __restrict__ e,
                                                       const CT* __restrict__ n, const CT* __restrict__ D,
                                                       const TB* __restrict__ b, float* __restrict__ x,
                                                       float* __restrict__ bc, const Scalars* __restrict__ sc) {
     typedef typename Pair<TB>::type TB2;
+    constexpr int E = LIN ? 1 : 0, HALO = RBLanes<LIN>::HALO, OWN = RBLanes<LIN>::OWN;
+    constexpr int NB = 6 + 2 * E, NN = 5 + 2 * E, NE = 4 + 2 * E, NC = 2 + 2 * E, NX = 4 + 2 * E;
     const int done = sc->done;  // arrives with the loads below
     int strip, iy0;
     wide_tile(2, strip, iy0);  // iy0 even
     const int lane = threadIdx.x;
-    const int ix = strip * RB_OWN - 2 + 2 * lane;
-    if (iy0 >= ny || strip * RB_OWN >= nx) return;  // uniform over the wave; no barriers follow
-    const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
+    const int ix = strip * OWN - 2 * HALO + 2 * lane;
+    if (iy0 >= ny || strip * OWN >= nx) return;  // uniform over the wave; no barriers follow
+    const bool own = lane >= HALO && lane <= WAVE - 1 - HALO && ix < nx;
+    const bool inx = ix >= 0 && ix < nx;
     const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
     const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
     const f2v z2 = {0.f, 0.f};
     auto row = [&](int j) -> size_t { return cidx(ixc, min(max(j, 0), ny - 1), nx); };
-    // b and D of rows iy0-2 .. iy0+3, n of rows iy0-2 .. iy0+2, e of rows iy0-1 .. iy0+2, c of rows iy0, iy0+1
-    f2v Bv[6], Dv[6], Nv[5], Ev[4], Cv[2];
+    // b and D of rows iy0-2-E .. iy0+3+E, n of rows iy0-2-E .. iy0+2+E, e of
+    // rows iy0-1-E .. iy0+2+E, c of rows iy0-E .. iy0+1+E
+    const int jb = iy0 - 2 - E;
+    f2v Bv[NB], Dv[NB], Nv[NN], Ev[NE], Cv[NC];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const size_t k = row(iy0 - 2 + q);
+    for (int q = 0; q < NB; ++q) {
+        const size_t k = row(jb + q);
         const TB2 v = *(const TB2*)(b + k);
         Bv[q] = f2v{(float)v.x, (float)v.y};
         Dv[q] = ldc2(D, k);
     }
 #pragma unroll
-    for (int q = 0; q < 5; ++q) Nv[q] = ldc2(n, row(iy0 - 2 + q));
+    for (int q = 0; q < NN; ++q) Nv[q] = ldc2(n, row(jb + q));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Ev[q] = ldc2(e, row(iy0 - 1 + q));
+    for (int q = 0; q < NE; ++q) Ev[q] = ldc2(e, row(jb + 1 + q));
 #pragma unroll
-    for (int q = 0; q < 2; ++q) Cv[q] = ldc2(c, row(iy0 + q));
+    for (int q = 0; q < NC; ++q) Cv[q] = ldc2(c, row(jb + 2 + q));
     // every load is issued before the flag decides
 #pragma unroll
-    for (int q = 0; q < 6; ++q) asm volatile("" : : "v"(Bv[q].x), "v"(Bv[q].y), "v"(Dv[q].x), "v"(Dv[q].y));
+    for (int q = 0; q < NB; ++q) asm volatile("" : : "v"(Bv[q].x), "v"(Bv[q].y), "v"(Dv[q].x), "v"(Dv[q].y));
 #pragma unroll
-    for (int q = 0; q < 5; ++q) asm volatile("" : : "v"(Nv[q].x), "v"(Nv[q].y));
+    for (int q = 0; q < NN; ++q) asm volatile("" : : "v"(Nv[q].x), "v"(Nv[q].y));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) asm volatile("" : : "v"(Ev[q].x), "v"(Ev[q].y));
+    for (int q = 0; q < NE; ++q) asm volatile("" : : "v"(Ev[q].x), "v"(Ev[q].y));
 #pragma unroll
-    for (int q = 0; q < 2; ++q) asm volatile("" : : "v"(Cv[q].x), "v"(Cv[q].y));
+    for (int q = 0; q < NC; ++q) asm volatile("" : : "v"(Cv[q].x), "v"(Cv[q].y));
     if (done) return;
-    f2v Rv[6];
+    f2v Rv[NB];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int j = iy0 - 2 + q;
+    for (int q = 0; q < NB; ++q) {
+        const int j = jb + q;
         Rv[q] = z2;
         if (j >= 0 && j < ny) {
             if ((j & 1) == 0) Rv[q].x = mg_div(omega * Bv[q].x, Dv[q].x);
@@ -2523,11 +2676,11 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
             Dv[q] = z2;
         }
     }
-    // full x of rows iy0-1 .. iy0+2
-    f2v Xv[4];
+    // full x of rows iy0-1-E .. iy0+2+E
+    f2v Xv[NX];
 #pragma unroll
-    for (int q = 1; q <= 4; ++q) {
-        const int j = iy0 - 2 + q;
+    for (int q = 1; q <= NX; ++q) {
+        const int j = jb + q;
         f2v v = z2;
         if (j >= 0 && j < ny) {
             const f2v rm = Rv[q - 1], rc = Rv[q], rp = Rv[q + 1], ej = Ev[q - 1], nj = Nv[q], nm = Nv[q - 1];
@@ -2545,11 +2698,13 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
         }
         Xv[q - 1] = v;
     }
-    float hrow = 0.f;
+    // the residual of rows iy0-E .. iy0+1+E (row t: X row index t - jb - 1)
+    float hrow = 0.f, acc = 0.f;
+    const float wx0 = ix == 0 ? 1.f : 0.75f, wx1 = ix + 1 == nx - 1 ? 1.f : 0.75f;
 #pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2) {
-        const int t = iy0 + s2;
-        if (t < ny) {
+    for (int s2 = 0; s2 < 2 + 2 * E; ++s2) {
+        const int t = iy0 - E + s2;
+        if (t >= 0 && t < ny) {  // uniform over the wave
             const f2v X0 = Xv[s2 + 1], Xm = Xv[s2], X1 = Xv[s2 + 2];
             const f2v e0 = Ev[s2 + 1], n0 = Nv[s2 + 2], nm = Nv[s2 + 1], cc = Cv[s2], b0 = Bv[s2 + 2];
             const float ue = __shfl_down(X0.x, 1, WAVE);
@@ -2565,18 +2720,40 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
             a1 += e0.x * (X0.y - X0.x);
             if (t < ny - 1) a1 += n0.y * (X0.y - X1.y);
             if (t > 0) a1 += nm.y * (X0.y - Xm.y);
-            const float q0 = own ? b0.x - a0 : 0.f, q1 = own ? b0.y - a1 : 0.f;
-            if (own) *(f2v*)(x + cidx(ix, t, nx)) = X0;
-            const float hs = q0 + q1;
-            if (s2 == 0) hrow = hs;
-            else if (own) bc[(size_t)(ix >> 1) + (size_t)(t >> 1) * nxc] = hrow + hs;
+            if constexpr (LIN) {
+                // P^T: this row's residuals restricted along x (columns
+                // ix-1 .. ix+2), weighted into coarse row iy0 / 2
+                const float q0 = inx ? b0.x - a0 : 0.f, q1 = inx ? b0.y - a1 : 0.f;
+                const float rl = __shfl_up(q1, 1, WAVE), rr = __shfl_down(q0, 1, WAVE);
+                float h = 0.25f * rl;
+                h += wx0 * q0;
+                h += wx1 * q1;
+                h += 0.25f * rr;
+                acc += lin_rw(t, s2, ny) * h;
+                if (own && (t == iy0 || t == iy0 + 1)) *(f2v*)(x + cidx(ix, t, nx)) = X0;
+            } else {
+                const float q0 = own ? b0.x - a0 : 0.f, q1 = own ? b0.y - a1 : 0.f;
+                if (own) *(f2v*)(x + cidx(ix, t, nx)) = X0;
+                const float hs = q0 + q1;
+                if (s2 == 0) hrow = hs;
+                else if (own) bc[(size_t)(ix >> 1) + (size_t)(t >> 1) * nxc] = hrow + hs;
+            }
         }
     }
-    // odd ny: the last (even) row has no partner row
-    if (iy0 + 1 >= ny && own) bc[(size_t)(ix >> 1) + (size_t)(iy0 >> 1) * nxc] = hrow + 0.f;
+    if constexpr (LIN) {
+        if (own) bc[(size_t)(ix >> 1) + (size_t)(iy0 >> 1) * nxc] = acc;
+    } else {
+        // odd ny: the last (even) row has no partner row
+        if (iy0 + 1 >= ny && own) bc[(size_t)(ix >> 1) + (size_t)(iy0 >> 1) * nxc] = hrow + 0.f;
+    }
 }
 
-template <class TB, class CT, class TO>
+// LIN: u = x + scale P xc with the linear prolongation: each lane reads its
+// coarse column in coarse rows iy0/2-2 .. iy0/2+2, interpolates along y, and
+// takes its neighbours' values along x by shuffle (two halo lanes a side, as
+// the down-leg: the outer halo lane's outer column is wrong and nothing owned
+// depends on it).
+template <class TB, class CT, class TO, bool LIN = false>
 __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, float omega, float scale,
                                                     const CT* __restrict__ c, const CT* __restrict__ e,
                                                     const CT* __restrict__ n, const CT* __restrict__ D,
@@ -2584,26 +2761,39 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
                                                     const float* __restrict__ xc, TO* __restrict__ out, int with_dot,
                                                     double* partials, Scalars* __restrict__ sc) {
     typedef typename Pair<TB>::type TB2;
+    constexpr int HALO = RBLanes<LIN>::HALO, OWN = RBLanes<LIN>::OWN;
     const int done = sc->done;  // arrives with the loads below
     int strip, iy0;
     wide_tile(2, strip, iy0);
     const int lane = threadIdx.x;
-    const int ix = strip * RB_OWN - 2 + 2 * lane;
+    const int ix = strip * OWN - 2 * HALO + 2 * lane;
     double acc = 0.0;
-    const bool active = iy0 < ny && strip * RB_OWN < nx;  // uniform over the wave
+    const bool active = iy0 < ny && strip * OWN < nx;  // uniform over the wave
     const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
     const int iyc = active ? iy0 : 0;  // rows of an inactive wave: any in-grid rows (never used)
     auto row = [&](int j) -> size_t { return cidx(ixc, min(max(j, 0), ny - 1), nx); };
     // u (x and xc) of rows iy0-2 .. iy0+3, b c D e of rows iy0-1 .. iy0+2, n of rows iy0-2 .. iy0+2
+    // (LIN: xc of coarse rows iy0/2-2 .. iy0/2+2 in Lc)
     f2v Xv[6], Cx[6];
+    float Lc[5];
     TB2 Bt[4];
     f2v Cv[4], Dv[4], Ev[4], Nv[5];
+    const int nyc = (ny + 1) >> 1;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int j = min(max(iyc - 2 + q, 0), ny - 1);
         Xv[q] = *(const f2v*)(x + cidx(ixc, j, nx));
-        const float cv = xc[(size_t)(ixc >> 1) + (size_t)(j >> 1) * nxc];
-        Cx[q] = f2v{cv, cv};
+        if constexpr (!LIN) {
+            const float cv = xc[(size_t)(ixc >> 1) + (size_t)(j >> 1) * nxc];
+            Cx[q] = f2v{cv, cv};
+        }
+    }
+    if constexpr (LIN) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int J = min(max((iyc >> 1) - 2 + q, 0), nyc - 1);
+            Lc[q] = xc[(size_t)(ixc >> 1) + (size_t)J * nxc];
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -2616,7 +2806,14 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
 #pragma unroll
     for (int q = 0; q < 5; ++q) Nv[q] = ldc2(n, row(iyc - 2 + q));
 #pragma unroll
-    for (int q = 0; q < 6; ++q) asm volatile("" : : "v"(Xv[q].x), "v"(Xv[q].y), "v"(Cx[q].x));
+    for (int q = 0; q < 6; ++q) {
+        asm volatile("" : : "v"(Xv[q].x), "v"(Xv[q].y));
+        if constexpr (!LIN) asm volatile("" : : "v"(Cx[q].x));
+    }
+    if constexpr (LIN) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) asm volatile("" : : "v"(Lc[q]));
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         asm volatile("" : : "v"(Bt[q].x), "v"(Bt[q].y), "v"(Cv[q].x), "v"(Cv[q].y), "v"(Dv[q].x), "v"(Dv[q].y),
@@ -2625,14 +2822,31 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
     for (int q = 0; q < 5; ++q) asm volatile("" : : "v"(Nv[q].x), "v"(Nv[q].y));
     if (done) return;  // uniform over the workgroup, before the partial's barriers
     if (active) {
-        const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
+        const bool own = lane >= HALO && lane <= WAVE - 1 - HALO && ix < nx;
         const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
         const f2v z2 = {0.f, 0.f};
         f2v U[6];
+        if constexpr (LIN) {
+            // fine row j = iy0-2+q: own coarse row J0 = iy0/2-1+q/2 (Lc index
+            // 1 + q/2), neighbour J0 - 1 (even j) or J0 + 1 (odd j), folded
+            const float wl = ix == 0 ? 0.f : 0.25f, wr = ix + 1 == nx - 1 ? 0.f : 0.25f;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int j = iy0 - 2 + q;
-            U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * Cx[q].x, Xv[q].y + scale * Cx[q].y} : z2;
+            for (int q = 0; q < 6; ++q) {
+                const int j = iy0 - 2 + q;
+                const int i0 = 1 + (q >> 1), i1 = (q & 1) ? i0 + 1 : i0 - 1;
+                const int J1 = (iy0 >> 1) - 2 + i1;
+                const bool fold = J1 < 0 || J1 > nyc - 1;
+                const float v = fold ? Lc[i0] : 0.75f * Lc[i0] + 0.25f * Lc[i1];
+                const float vl = __shfl_up(v, 1, WAVE), vr = __shfl_down(v, 1, WAVE);
+                const float p0 = (1.f - wl) * v + wl * vl, p1 = (1.f - wr) * v + wr * vr;
+                U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * p0, Xv[q].y + scale * p1} : z2;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int j = iy0 - 2 + q;
+                U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * Cx[q].x, Xv[q].y + scale * Cx[q].y} : z2;
+            }
         }
         // rows iy0-1 .. iy0+2 with their black cell relaxed
         f2v V[4];
@@ -2893,6 +3107,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
     float* Ns = lds + 4 * tp;  // north couplings
     float* Ds = lds + 5 * tp;  // Jacobi diagonal
     float* Is = lds + 6 * tp;  // coarsest inverse, ncell x ncell
+    float* Hs = lds + 2 * tp + ((4 * tp + ncell * ncell + 3) & ~3);  // linear restriction: x-restricted rows
     {
         constexpr int UB = 4, UP = 8;
         const float* b0 = T.L[0].b;
@@ -2975,6 +3190,37 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
             }
             __syncthreads();
             auto xk = [&](size_t k) -> float { return x[k]; };
+            if ((T.lin_mask >> l) & 1) {
+                // P^T (b - A x): each fine row restricted along x into Hs
+                // (cnx x fny), then the rows combined with the y weights
+                for (int q = tid; q < cnx * fny; q += nt) {
+                    const int iy = dcx.quot(q), I = q - iy * cnx;
+                    float h = 0.f;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        const int ix = 2 * I - 1 + p;
+                        const float wx = lin_rw(ix, p, fnx);
+                        if (wx == 0.f) continue;
+                        const size_t k = (size_t)ix + (size_t)iy * fnx;
+                        h += wx * (b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xk));
+                    }
+                    Hs[q] = h;
+                }
+                __syncthreads();
+                for (int q = tid; q < cnx * cny; q += nt) {
+                    const int J = dcx.quot(q), I = q - J * cnx;
+                    float acc = 0.f;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        const int iy = 2 * J - 1 + p;
+                        const float wy = lin_rw(iy, p, fny);
+                        if (wy != 0.f) acc += wy * Hs[I + iy * cnx];
+                    }
+                    bc[q] = acc;
+                }
+                __syncthreads();
+                continue;
+            }
             for (int q = tid; q < cnx * cny; q += nt) {
                 const int J = dcx.quot(q), I = q - J * cnx;
                 float acc = 0.f;
@@ -3036,7 +3282,8 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         float* x = Xs + off[l];
         const float* zc = Xs + off[l + 1];
         const FastDiv dfx(fnx);
-        if (T.rb && (fnx & 1) == 0) {
+        const bool lin = (T.lin_mask >> l) & 1;
+        if (T.rb && (fnx & 1) == 0 && !lin) {
             // even width: u = x + scale P xc formed where it is read, black cells
             // relaxing from u, then red cells (own u, relaxed black neighbours);
             // each pass takes only its own colour's cells
@@ -3063,9 +3310,19 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
             __syncthreads();
             continue;
         }
-        for (int k = tid; k < nf; k += nt) {
-            const int iy = dfx.quot(k), ix = k - iy * fnx;
-            x[k] = x[k] + scale * zc[(ix >> 1) + (iy >> 1) * cnx];
+        if (lin) {
+            for (int k = tid; k < nf; k += nt) {
+                const int iy = dfx.quot(k), ix = k - iy * fnx;
+                const LinP px = lin_p(ix, fnx), py = lin_p(iy, fny);
+                const float v0 = px.w0 * zc[px.I0 + py.I0 * cnx] + px.w1 * zc[px.I1 + py.I0 * cnx];
+                const float v1 = px.w0 * zc[px.I0 + py.I1 * cnx] + px.w1 * zc[px.I1 + py.I1 * cnx];
+                x[k] = x[k] + T.scale_lin * (py.w0 * v0 + py.w1 * v1);
+            }
+        } else {
+            for (int k = tid; k < nf; k += nt) {
+                const int iy = dfx.quot(k), ix = k - iy * fnx;
+                x[k] = x[k] + scale * zc[(ix >> 1) + (iy >> 1) * cnx];
+            }
         }
         __syncthreads();
         auto us = [&](size_t k) -> float { return x[k]; };
@@ -3860,6 +4117,37 @@ void mg_coarsen(int nxf, int nyf, int nxc, int nyc, const float* cf, const float
 void mg_diag(int nx, int ny, const float* c, const float* e, const float* n, float* D, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_diag, cells_grid(nx, ny), dim3(64, 4), 0, st, nx, ny, c, e, n, D);
 }
+void mg_coarsen_lin(int nxf, int nyf, int nxc, int nyc, const float* cf, const float* ef, const float* nf, float* cc,
+                    float* ec, float* nc, hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_coarsen_lin, cells_grid(nxc, nyc), dim3(64, 4), 0, st, nxf, nyf, nxc, nyc, cf, ef, nf,
+                       cc, ec, nc);
+}
+template <class TB, class CT>
+void mg_restrict_lin(int nx, int ny, int nxc, int nyc, const CT* c, const CT* e, const CT* n, const TB* b,
+                     const float* x, float* bc, const Scalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL((k_mg_restrict_lin<TB, CT>), cells_grid(nxc, nyc), dim3(64, 4), 0, st, nx, ny, nxc, nyc, c, e,
+                       n, b, x, bc, sc);
+}
+template void mg_restrict_lin<float, float>(int, int, int, int, const float*, const float*, const float*,
+                                            const float*, const float*, float*, const Scalars*, hipStream_t);
+template void mg_restrict_lin<double, float>(int, int, int, int, const float*, const float*, const float*,
+                                             const double*, const float*, float*, const Scalars*, hipStream_t);
+template void mg_restrict_lin<float, uint16_t>(int, int, int, int, const uint16_t*, const uint16_t*,
+                                               const uint16_t*, const float*, const float*, float*, const Scalars*,
+                                               hipStream_t);
+template void mg_restrict_lin<double, uint16_t>(int, int, int, int, const uint16_t*, const uint16_t*,
+                                                const uint16_t*, const double*, const float*, float*,
+                                                const Scalars*, hipStream_t);
+void mg_prolong_lin(int nx, int ny, int nxc, float scale, float* x, const float* xc, const Scalars* sc,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_mg_prolong_lin, cells_grid(nx, ny), dim3(64, 4), 0, st, nx, ny, nxc, scale, x, xc, sc);
+}
+// SHK_MG_LIN_UNFUSED=1: the linear transfers always through their separate
+// kernels (A/B measurements and the fused kernels' parity check).
+static bool getenv_flag_lin_unfused() {
+    static const bool v = getenv("SHK_MG_LIN_UNFUSED") != nullptr;
+    return v;
+}
 // Largest level (cells) that runs the single-phase kernels; SHK_MG_FLAT
 // overrides MG_FLAT_CELLS for A/B measurements.
 static long mg_flat_cells() {
@@ -3897,9 +4185,10 @@ static Geom mg_wide_geom(int nx, int ny) {
 // against 36.5 / 36.2 for 8192 waves and 44.8 / 38.4 for 2048 waves of 16
 // rows: the two rows of lookahead a wave reads before its first residual
 // cost less over more rows.
-static Geom mg_rb_geom(int nx, int ny) {
+static Geom mg_rb_geom(int nx, int ny, bool lin = false) {
     Geom W = make_geom(nx, ny);
-    const long strips = (nx + RB_OWN - 1) / RB_OWN;
+    const int own = lin ? RB_OWN_LIN : RB_OWN;
+    const long strips = (nx + own - 1) / own;
     long r = ((long)ny * strips) / 4096;
     W.R = (int)(r < 2 ? 2 : (r > 8 ? 8 : r + (r & 1)));
     W.grid = dim3((unsigned)((strips + WAVES - 1) / WAVES), (unsigned)((ny + W.R - 1) / W.R));
@@ -4052,6 +4341,40 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
 }
+// The red-black legs of a level that reaches the next one by the linear
+// transfers, where they are fused into the two-row kernels (even width above
+// MG_FLAT_CELLS cells, two rows per wave); false where they are not, and the
+// caller takes the plain legs with the separate transfer kernels.
+template <class TB, class CT>
+bool mg_down_lin(int nx, int ny, int nxc, float omega, const CT* c, const CT* e, const CT* n, const CT* D,
+                 const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+    if (mg_rb_flat(nx, ny) || getenv_flag_lin_unfused()) return false;
+    const Geom W = mg_rb_geom(nx, ny, true);
+    if (W.R != 2) return false;
+    hipLaunchKernelGGL((k_mg_down_rb2s<TB, CT, true>), W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D, b, x,
+                       bc, sc);
+    return true;
+}
+template <class TB, class CT, class TO>
+bool mg_up_lin(int nx, int ny, int nxc, float omega, float scale, const CT* c, const CT* e, const CT* n,
+               const CT* D, const TB* b, const float* x, const float* xc, TO* out, int dot_mode, double* partials,
+               Scalars* sc, hipStream_t st) {
+    if (mg_rb_flat(nx, ny) || getenv_flag_lin_unfused()) return false;
+    const Geom M = mg_rb_geom(nx, ny, true);
+    if (M.R != 2) return false;
+    const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
+    hipLaunchKernelGGL((k_mg_up_rb2s<TB, CT, TO, true>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n,
+                       D, b, x, xc, out, with_dot, partials, sc);
+    if (dot_mode == 0 || dot_mode == 1)
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)M.nblocks,
+                           dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
+    return true;
+}
+template bool mg_down_lin<float, float>(int, int, int, float, const float*, const float*, const float*, const float*,
+                                        const float*, float*, float*, const Scalars*, hipStream_t);
+template bool mg_up_lin<float, float, float>(int, int, int, float, float, const float*, const float*, const float*,
+                                             const float*, const float*, const float*, const float*, float*, int,
+                                             double*, Scalars*, hipStream_t);
 template void mg_down<float>(int, int, int, int, float, const float*, const float*, const float*, const float*,
                              const float*, float*, float*, const Scalars*, int, hipStream_t);
 template void mg_down<double>(int, int, int, int, float, const float*, const float*, const float*, const float*,
@@ -4063,10 +4386,13 @@ template void mg_up<double>(int, int, int, float, float, const float*, const flo
                             const double*, const float*, const float*, double*, int, double*, Scalars*, int,
                             hipStream_t);
 size_t mg_tail_stride(size_t tail_cells) { return (tail_cells + 3) & ~(size_t)3; }
-size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells) {
-    // rhs, x and the packed block (c, e, n, D, inverse; rounded up to 16 B)
-    return (2 * mg_tail_stride(tail_cells) + ((4 * mg_tail_stride(tail_cells) + coarsest_cells * coarsest_cells + 3) & ~(size_t)3)) *
-           sizeof(float);
+// the packed block (c, e, n, D, inverse) in floats, rounded up to 16 B
+static size_t mg_tail_pack(size_t tail_cells, size_t coarsest_cells) {
+    return (4 * mg_tail_stride(tail_cells) + coarsest_cells * coarsest_cells + 3) & ~(size_t)3;
+}
+size_t mg_tail_lds(size_t tail_cells, size_t coarsest_cells, size_t hcells) {
+    // rhs, x, the packed block and the restriction's staging rows
+    return (2 * mg_tail_stride(tail_cells) + mg_tail_pack(tail_cells, coarsest_cells) + hcells) * sizeof(float);
 }
 int mg_tail_reserve(size_t bytes) {
     return hipFuncSetAttribute((const void*)k_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
@@ -4078,7 +4404,7 @@ void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st) {
     size_t cells = 0;
     for (int l = 0; l < T.nlev; ++l) cells += (size_t)T.L[l].nx * T.L[l].ny;
     const size_t nc = (size_t)T.L[T.nlev - 1].nx * T.L[T.nlev - 1].ny;
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), mg_tail_lds(cells, nc), st, T, sc);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), mg_tail_lds(cells, nc, (size_t)T.hcells), st, T, sc);
 }
 void mg_dense_inverse(int nx, int ny, const float* c, const float* e, const float* n, float* inv, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_dense_inverse_lds, dim3(1), dim3(WAVE, 16), 0, st, nx, ny, c, e, n, inv);

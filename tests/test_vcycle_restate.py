@@ -7,14 +7,20 @@ pinned by what the mathematics fixes on a real assembled glacier system (CPU):
 * red-black smoothing needs fewer CG iterations than the same cycle with damped
   Jacobi smoothing (omega 0.95, scale 1.5), the reason it is the default;
 * the finest level's coefficients in BF16 cost at most one iteration, the
-  reason the library stores them so.
+  reason the library stores them so;
+* the linear transfers: P reproduces constants everywhere and linear fields
+  away from the edges, the restriction is its transpose, the lumped coarse
+  operator equals the explicit Galerkin product P^T A P with every link laid
+  onto its lattice paths, and the cycle with them stays symmetric positive
+  definite and needs fewer CG iterations than with aggregation, the reason
+  the library uses them below the finest level.
 
 The GPU test (test_gpu_vcycle.py) then checks the library's V-cycle against it."""
 import numpy as np
 import pytest
 
 from helpers import oracle_state
-from vcycle_restate import VCycle, F32, flux_apply
+from vcycle_restate import VCycle, F32, flux_apply, coarsen_lin, prolong_lin, restrict_lin
 
 NX, NY = 256, 64
 
@@ -72,9 +78,10 @@ class _JacobiCycle(VCycle):
         return u + self.omega * (b - flux_apply(c, e, n, u)) / D
 
 
-def test_vcycle_is_symmetric_positive_definite(system):
+@pytest.mark.parametrize("bil_from", [None, 1, 0])
+def test_vcycle_is_symmetric_positive_definite(system, bil_from):
     co = system
-    M = VCycle(co[0], co[1], co[3])
+    M = VCycle(co[0], co[1], co[3], bil_from=bil_from)
     rng = np.random.default_rng(0)
     rs = [np.asfortranarray(rng.standard_normal(co[0].shape)) for _ in range(3)]
     zs = [M(r) for r in rs]
@@ -96,3 +103,133 @@ def test_red_black_cycle_beats_jacobi_pcg_and_the_jacobi_cycle(oracle, system):
     # the finest level's coefficients rounded to BF16 cost at most one iteration
     rb16 = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True))
     assert rb16 <= rb + 1, (rb16, rb)
+
+
+def _p1d(f):
+    """The 1D linear prolongation as a dense f x (f+1)//2 matrix, from its
+    definition: fine x takes 3/4 of coarse x // 2 and 1/4 of the coarse
+    neighbour on its side, a neighbour outside the grid folding onto x // 2."""
+    c = (f + 1) // 2
+    P = np.zeros((f, c))
+    for x in range(f):
+        i0 = x // 2
+        i1 = i0 - 1 if x % 2 == 0 else i0 + 1
+        if 0 <= i1 < c:
+            P[x, i0] += 0.75
+            P[x, i1] += 0.25
+        else:
+            P[x, i0] += 1.0
+    return P
+
+
+SHAPES = [(8, 6), (9, 7), (7, 12), (2, 3), (33, 17), (1, 5)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_linear_transfers(shape):
+    fx, fy = shape
+    cx, cy = (fx + 1) // 2, (fy + 1) // 2
+    rng = np.random.default_rng(1)
+    # constants everywhere, linear fields between the edge cells
+    assert np.allclose(prolong_lin(np.ones((cx, cy)), fx, fy), 1.0, rtol=0, atol=1e-15)
+    X = (2 * np.arange(cx) + 1.0)[:, None] + 0 * np.arange(cy)[None, :]  # coarse centres, fine units
+    u = prolong_lin(X, fx, fy)
+    want = np.arange(fx) + 0.5
+    inner = slice(1, fx - 1 if fx % 2 == 0 else fx)
+    np.testing.assert_allclose(u[inner, 0], want[inner], rtol=0, atol=1e-13)
+    # the matrix of the stated rule, and P^T as the restriction
+    xc = rng.standard_normal((cx, cy))
+    r = rng.standard_normal((fx, fy))
+    Px, Py = _p1d(fx), _p1d(fy)
+    np.testing.assert_allclose(prolong_lin(xc, fx, fy), Px @ xc @ Py.T, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(restrict_lin(r), Px.T @ r @ Py, rtol=0, atol=1e-14)
+
+
+def _flux_matrix(c, e, n):
+    fx, fy = c.shape
+    idx = lambda x, y: x + y * fx
+    A = np.diag(c.ravel(order="F").astype(np.float64))
+    for y in range(fy):
+        for x in range(fx):
+            for (x2, y2, a) in ((x + 1, y, e[x, y]), (x, y + 1, n[x, y])):
+                if x2 < fx and y2 < fy:
+                    i, j = idx(x, y), idx(x2, y2)
+                    A[i, i] += a
+                    A[j, j] += a
+                    A[i, j] -= a
+                    A[j, i] -= a
+    return A
+
+
+def _lump(Ac, cx, cy):
+    """Every off-diagonal link w = -Ac[i, j] laid onto the coarse faces of the
+    x-first and the y-first lattice path between its cells (w / 2 each when
+    both paths differ, w on the one path otherwise)."""
+    e = np.zeros((cx, cy))
+    n = np.zeros((cx, cy))
+
+    def walk(x, y, tx, ty, w, xfirst):
+        order = ("x", "y") if xfirst else ("y", "x")
+        for ax in order:
+            while (x != tx) if ax == "x" else (y != ty):
+                if ax == "x":
+                    s = 1 if tx > x else -1
+                    e[min(x, x + s), y] += w
+                    x += s
+                else:
+                    s = 1 if ty > y else -1
+                    n[x, min(y, y + s)] += w
+                    y += s
+
+    m = cx * cy
+    for i in range(m):
+        for j in range(i + 1, m):
+            w = -Ac[i, j]
+            if w == 0.0:
+                continue
+            (ix, iy), (jx, jy) = (i % cx, i // cx), (j % cx, j // cx)
+            if ix != jx and iy != jy:
+                walk(ix, iy, jx, jy, w / 2, True)
+                walk(ix, iy, jx, jy, w / 2, False)
+            else:
+                walk(ix, iy, jx, jy, w, True)
+    return e, n
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_lumped_galerkin_coarsening(shape):
+    """coarsen_lin against P^T A P formed with dense matrices and lumped link by
+    link: c = P^T c (the face part has zero row sums), and the faces' links
+    lumped onto the coarse faces."""
+    fx, fy = shape
+    cx, cy = (fx + 1) // 2, (fy + 1) // 2
+    rng = np.random.default_rng(2)
+    c = rng.random((fx, fy))
+    e = rng.random((fx, fy))
+    n = rng.random((fx, fy))
+    P = np.kron(_p1d(fy), _p1d(fx))  # cell index x + y fx
+    Af = _flux_matrix(np.zeros_like(c), e, n)
+    Ac = P.T @ Af @ P
+    assert np.allclose(Ac.sum(axis=1), 0.0, atol=1e-12)
+    e0, n0 = _lump(Ac, cx, cy)
+    c0 = (P.T @ c.ravel(order="F")).reshape((cx, cy), order="F")
+    cc, ec, nc = coarsen_lin(c.astype(F32), e.astype(F32), n.astype(F32))
+    # the restatement rounds once to FP32 from FP64 sums of the FP32 inputs
+    c32, e32, n32 = (a.astype(F32).astype(np.float64) for a in (c, e, n))
+    Ac32 = P.T @ _flux_matrix(np.zeros_like(c), e32, n32) @ P
+    e1, n1 = _lump(Ac32, cx, cy)
+    np.testing.assert_allclose(cc, (P.T @ c32.ravel(order="F")).reshape((cx, cy), order="F"), rtol=1.2e-7, atol=0)
+    np.testing.assert_allclose(ec, e1, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(nc, n1, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(ec, e0, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(cc, c0, rtol=1e-6)
+    assert np.all(ec[-1, :] == 0) and np.all(nc[:, -1] == 0)
+
+
+def test_linear_transfers_cut_cg_iterations(oracle, system):
+    co = system
+    agg = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True))
+    lin1 = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=1))
+    lin0 = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=0))
+    print(f"PCG iterations: aggregation {agg}, linear below the finest level {lin1}, linear everywhere {lin0}")
+    assert lin1 < 0.85 * agg and lin0 < 0.85 * agg, (agg, lin1, lin0)

@@ -15,13 +15,80 @@ from a reference file:
   coarse correction u = x + scale P xc, black then red;
 * bf16_level0: the finest level's smoothing and residual use its c, e, n, D
   rounded to BF16 (round to nearest even), as the library does where the
-  finest level takes the two-column legs; coarsening uses the FP32 values.
+  finest level takes the two-column legs; coarsening uses the FP32 values;
+* bil_from: the transfers from level bil_from down are cell-centred linear
+  ones instead: P takes a fine cell 3/4 of its own coarse cell's value plus
+  1/4 of the neighbour on its side, in x and in y (9/16, 3/16, 3/16, 1/16),
+  a neighbour beyond the grid folding onto the cell's own (weight 1);
+  restriction is P^T and the coarse-correction scale is scale_bil; the
+  coarser level's operator is the Galerkin product P^T A P lumped back to the
+  5-point flux form: c = P^T c, and each fine face of coupling a between
+  cells i, j contributes a * (P_i - P_j) (P_i - P_j)^T, whose links are laid
+  onto the lattice paths between their cells (half on each of the two paths
+  of a diagonal link).  For an x-face that leaves the couplings
+    ec(I, J) = sum_y wy(y, J) [a(2I+1, y) / 4 + (a(2I, y) + a(2I+2, y)) / 16]
+  (faces beyond the grid taken as zero; wy the restriction weights in y) and
+  nothing on the y-faces; y-faces likewise.
 
-Arrays are indexed [ix, iy] with shape (nx, ny); everything is float32.
+Arrays are indexed [ix, iy] with shape (nx, ny); everything is float32 (the
+linear transfers and the coarsening in float64, rounded once).
 """
 import numpy as np
 
 F32 = np.float32
+
+
+def lin_weights(f):
+    """The cell-centred linear transfer on a 1D level of f cells, per fine index
+    x: (I0, w0, I1, w1), the two coarse cells and their weights; a neighbour
+    outside [0, (f + 1) // 2) folds onto I0 (w0 = 1, w1 = 0)."""
+    c = (f + 1) // 2
+    x = np.arange(f)
+    I0 = x // 2
+    I1 = np.where(x % 2 == 0, I0 - 1, I0 + 1)
+    out = (I1 < 0) | (I1 > c - 1)
+    return I0, np.where(out, 1.0, 0.75), np.clip(I1, 0, c - 1), np.where(out, 0.0, 0.25)
+
+
+def _restrict1(a, axis):
+    f = a.shape[axis]
+    I0, w0, I1, w1 = lin_weights(f)
+    a = np.moveaxis(np.asarray(a, np.float64), axis, 0)
+    out = np.zeros(((f + 1) // 2,) + a.shape[1:])
+    np.add.at(out, I0, w0[:, None] * a)
+    np.add.at(out, I1, w1[:, None] * a)
+    return np.moveaxis(out, 0, axis)
+
+
+def _prolong1(a, f, axis):
+    I0, w0, I1, w1 = lin_weights(f)
+    a = np.moveaxis(np.asarray(a, np.float64), axis, 0)
+    return np.moveaxis(w0[:, None] * a[I0] + w1[:, None] * a[I1], 0, axis)
+
+
+def restrict_lin(r):
+    """P^T r (float64)."""
+    return _restrict1(_restrict1(r, 0), 1)
+
+
+def prolong_lin(xc, fx, fy):
+    """P xc onto an fx x fy level (float64)."""
+    return _prolong1(_prolong1(xc, fx, 0), fy, 1)
+
+
+def coarsen_lin(c, e, n):
+    """The lumped Galerkin operator of the linear transfers (float64 in, FP32 out)."""
+    fx, fy = c.shape
+    cx, cy = (fx + 1) // 2, (fy + 1) // 2
+    ep = np.zeros((2 * cx + 2, fy))
+    ep[:fx - 1] = np.asarray(e, np.float64)[:fx - 1]
+    hx = 0.25 * ep[1:2 * cx:2] + 0.0625 * (ep[0:2 * cx:2] + ep[2:2 * cx + 2:2])
+    hx[-1] = 0.0
+    npd = np.zeros((fx, 2 * cy + 2))
+    npd[:, :fy - 1] = np.asarray(n, np.float64)[:, :fy - 1]
+    hy = 0.25 * npd[:, 1:2 * cy:2] + 0.0625 * (npd[:, 0:2 * cy:2] + npd[:, 2:2 * cy + 2:2])
+    hy[:, -1] = 0.0
+    return restrict_lin(c).astype(F32), _restrict1(hx, 1).astype(F32), _restrict1(hy, 0).astype(F32)
 
 
 def flux_apply(c, e, n, u):
@@ -118,15 +185,18 @@ def dense_inverse(c, e, n):
 
 
 class VCycle:
-    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False):
-        self.omega, self.scale = F32(omega), F32(scale)
+    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False, bil_from=None, scale_bil=0.8):
+        self.omega, self.scale, self.scale_bil = F32(omega), F32(scale), F32(scale_bil)
         c, e, n, D = fine_level(aP, aE, aN)
         self.levels = [(c, e, n, D)]
+        self.lin = []  # per level: the transfer to the next level is linear
         while True:
             c, e, n, _ = self.levels[-1]
             if c.size <= 64 and len(self.levels) > 1:
                 break
-            cc, ec, nc = coarsen(c, e, n)
+            lin = bil_from is not None and len(self.levels) - 1 >= bil_from
+            cc, ec, nc = (coarsen_lin if lin else coarsen)(c, e, n)
+            self.lin.append(lin)
             self.levels.append((cc, ec, nc, diag(cc, ec, nc)))
         if bf16_level0:
             self.levels[0] = tuple(bf16(a) for a in self.levels[0])
@@ -150,11 +220,15 @@ class VCycle:
         r = b - flux_apply(c, e, n, x)
         fx, fy = b.shape
         cx, cy = self.levels[l + 1][0].shape
-        rp = np.zeros((2 * cx, 2 * cy), F32)
-        rp[:fx, :fy] = r
-        rc = (rp[0::2, 0::2] + rp[1::2, 0::2]) + (rp[0::2, 1::2] + rp[1::2, 1::2])
-        xc = self._cycle(rc.astype(F32), l + 1)
-        u = x + self.scale * np.repeat(np.repeat(xc, 2, 0), 2, 1)[:fx, :fy]
+        if self.lin[l]:
+            xc = self._cycle(restrict_lin(r).astype(F32), l + 1)
+            u = x + (self.scale_bil * prolong_lin(xc, fx, fy)).astype(F32)
+        else:
+            rp = np.zeros((2 * cx, 2 * cy), F32)
+            rp[:fx, :fy] = r
+            rc = (rp[0::2, 0::2] + rp[1::2, 0::2]) + (rp[0::2, 1::2] + rp[1::2, 1::2])
+            xc = self._cycle(rc.astype(F32), l + 1)
+            u = x + self.scale * np.repeat(np.repeat(xc, 2, 0), 2, 1)[:fx, :fy]
         for mask in (blk, red):
             t = u + om * (b - flux_apply(c, e, n, u)) / D
             u[mask] = t[mask]
