@@ -132,6 +132,9 @@ struct shk_ctx {
     // 22 CG iterations against aggregation's 34 with the finest transfer
     // still aggregating, 21 with every transfer linear.
     int mg_lin_from = 1;
+    // smoothing sweeps each way on the levels below the finest that take the
+    // two-row legs with the linear transfers (1 or 2; SHK_MG_NU)
+    int mg_nu = 2;
     float mg_scale_lin = 0.8f, mg_scale_lin_par = 1.0f;
     // The finest level's c, aE, aN, D in BF16 for its red-black legs (where
     // those are the two-column kernels): half the coefficient bytes of the
@@ -155,6 +158,7 @@ struct shk_ctx {
     bool use_defer = true;  // FP64 CG + V-cycle: finishing kernels folded into the next kernel
     bool use_adaptive = true;  // batch sizes from the observed convergence rate
     int last_niter = 0;        // CG iterations of the previous solve
+    int batch_margin = 2;      // the first batch stops this many iterations short of last_niter
 
     // learned warm start and training-pair recording: the network input is
     // the grid padded with zero cells to W8 x H8 (multiples of 64 and 32)
@@ -490,6 +494,7 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
         // such as n x 1 halve only once per level)
         if (getenv("SHK_MG_JACOBI")) ctx->mg_rb = 0;
         if (const char* e = getenv("SHK_MG_LIN_FROM")) ctx->mg_lin_from = atoi(e);  // negative: none
+        if (const char* e = getenv("SHK_MG_NU")) ctx->mg_nu = atoi(e) == 1 ? 1 : 2;
         for (size_t l = 0; l + 1 < ctx->mg.size(); ++l)
             ctx->mg[l].lin = ctx->mg_rb && ctx->mg_lin_from >= 0 && (int)l >= ctx->mg_lin_from;
         auto tail_cells = [&](size_t f) {
@@ -627,7 +632,7 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
             launch::mg_down<T>(f.nx, f.ny, mg[1].nx, mg[1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, r, f.xa, mg[1].b,
                                ctx->sc, ctx->mg_rb, ctx->st);
         else if (f.lin && launch::mg_down_lin<float, float>(f.nx, f.ny, cl.nx, ctx->mg_omega, f.c, f.e, f.n, f.D,
-                                                                f.b, f.xa, cl.b, ctx->sc, ctx->st))
+                                                                f.b, f.xa, cl.b, ctx->sc, ctx->mg_nu, ctx->st))
             continue;
         else
             launch::mg_down<float>(f.nx, f.ny, mg[l + 1].nx, mg[l + 1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, f.b,
@@ -685,7 +690,7 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
         if (l > 0 && f.lin &&
             launch::mg_up_lin<float, float, float>(f.nx, f.ny, mg[l + 1].nx, ctx->mg_omega, scale_lin, f.c, f.e,
                                                    f.n, f.D, f.b, f.xa, mg[l + 1].xb, f.xb, -1, ctx->partials,
-                                                   ctx->sc, ctx->st))
+                                                   ctx->sc, ctx->mg_nu, ctx->st))
             continue;
         // the linear prolongation into x first; the leg then adds nothing
         float scale = scale_agg;
@@ -805,22 +810,28 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     }
     const int every = nn_every + (nn_every & 1);
     const int bmax = nn ? every : check_every + (check_every & 1);
-    // plain solves size each batch to the iterations the convergence rate so
-    // far predicts (a power of two, 2..bmax), so few iterations are queued
-    // past convergence; the first batch follows the previous solve's count
-    const bool adaptive = !nn && ctx->use_adaptive;
-    auto pow2_batch = [&](double need) {
-        int b = 2;
-        while (b < bmax && b < need) b *= 2;
-        return b < bmax ? b : bmax;
-    };
-    int batch = (adaptive && ctx->last_niter > 0) ? pow2_batch(ctx->last_niter + 1) : bmax;
     const bool fused = precond == SHK_PRECOND_CHEBYSHEV && degree >= 2 && degree <= 8 && ctx->use_fused && !nn;
     bool learned_batch = false;
     int rc_nn = SHK_OK;
     // FP64 CG with the V-cycle (no learned preconditioner): the two finishing
     // kernels of an iteration are folded into the matvec and the update
     const bool defer = sizeof(T) == 8 && multigrid && !nn && ctx->use_defer;
+    // plain solves size each batch to the iterations the convergence rate so
+    // far predicts (an even count, 2..bmax), so few iterations are queued
+    // past convergence.  The first batch stops batch_margin iterations short
+    // of the previous solve's count: one more batch costs a host round trip,
+    // about what one iteration queued past convergence costs (its launches
+    // read the done flag and exit).  Deferred finishing publishes the stop
+    // decision one launch later, so its batches hold one iteration more.
+    const bool adaptive = !nn && ctx->use_adaptive;
+    auto even_batch = [&](double need) {
+        int b = need <= 2.0 ? 2 : (need >= bmax ? bmax : (int)ceil(need));
+        b += b & 1;
+        return b < bmax ? b : bmax;
+    };
+    int batch = (adaptive && ctx->last_niter > 0)
+                    ? even_batch((double)ctx->last_niter + (defer ? 1 : 0) - ctx->batch_margin)
+                    : bmax;
     // ... with the CG update fused into the finest down-leg where that leg can
     // take it (r then alternates between w.r and w.r2; batches are even)
     const bool fuse_upd = defer && zf32 && launch::mg_down_fuses_update(ctx->mg[0].nx, ctx->mg[0].ny) &&
@@ -968,7 +979,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                 const double per_it = log(ratio) / batch;  // log of the mean reduction per iteration
                 need = ceil(log(h->eps_stop / h->rNorm) / per_it) + (defer ? 1 : 0);
             }
-            const int nb2 = pow2_batch(need);
+            const int nb2 = even_batch(need);
             if (nb2 != batch) {
                 batch = nb2;
                 rc = batch_graph(&exec);
@@ -1388,6 +1399,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     ctx->use_fused = getenv("SHK_NO_FUSED") == nullptr;
     ctx->use_defer = getenv("SHK_NO_DEFER") == nullptr;
     ctx->use_adaptive = getenv("SHK_FIXED_BATCH") == nullptr;
+    if (const char* e = getenv("SHK_BATCH_MARGIN")) ctx->batch_margin = atoi(e);
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;

@@ -92,6 +92,8 @@ constexpr int MG_MAX_TAIL = 12;
 // levels up to this size use the single-phase kernels, larger ones the
 // marching kernels (measured faster down to 16k-cell levels at 4096 x 1024)
 constexpr long MG_FLAT_CELLS = 4096;
+// levels above this many cells take four rows per wave in the two-sweep red-black legs
+constexpr long MG_RBN4_CELLS = 1L << 19;
 constexpr long MG_TAIL_CELLS = 4096;  // at most 4 cells per thread; levels of ~5.5k cells take ~150 KB of LDS
 struct MGTail {
     int nlev;
@@ -239,14 +241,18 @@ void mg_restrict_lin(int nx, int ny, int nxc, int nyc, const CT* c, const CT* e,
 void mg_prolong_lin(int nx, int ny, int nxc, float scale, float* x, const float* xc, const Scalars* sc,
                     hipStream_t st);
 // the red-black legs with the linear transfers fused in, where the level's
-// legs are the two-row kernels; false (nothing launched) elsewhere
+// legs are the two-row kernels (nu sweeps each way where mg_lin_nu2 allows
+// two); false (nothing launched) elsewhere
 template <class TB, class CT>
 bool mg_down_lin(int nx, int ny, int nxc, float omega, const CT* c, const CT* e, const CT* n, const CT* D,
-                 const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
+                 const TB* b, float* x, float* bc, const Scalars* sc, int nu, hipStream_t st);
 template <class TB, class CT, class TO>
 bool mg_up_lin(int nx, int ny, int nxc, float omega, float scale, const CT* c, const CT* e, const CT* n,
                const CT* D, const TB* b, const float* x, const float* xc, TO* out, int dot_mode, double* partials,
-               Scalars* sc, hipStream_t st);
+               Scalars* sc, int nu, hipStream_t st);
+// whether an nx x ny level with the linear transfers takes two sweeps each way
+// (nu = 2 in mg_down_lin / mg_up_lin)
+bool mg_lin_nu2(int nx, int ny);
 template <class TB>
 void mg_down(int nx, int ny, int nxc, int nyc, float omega, const float* c, const float* e, const float* n,
              const float* D, const TB* b, float* x, float* bc, const Scalars* sc, int rb, hipStream_t st);

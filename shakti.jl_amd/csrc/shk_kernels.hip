@@ -31,6 +31,8 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "shk_internal.hpp"
 
 namespace shk {
@@ -2913,6 +2915,283 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
     }
 }
 
+// Red-black legs with NU sweeps each way on the levels that reach the next
+// one by the linear transfers, two rows per wave (k_mg_down_rb2s /
+// k_mg_up_rb2s take one).  These levels are latency-bound: a wave's loads are
+// issued at once and every sweep is formed in registers, so the extra sweep
+// costs arithmetic and wider loads, not another launch.  Each sweep beyond
+// the first needs two more rows above and below and one more halo lane on
+// either side: a value that a half-sweep forms is wrong one column further in
+// from the wave's edge than the values it reads, and the down-leg's
+// restriction reads one column beyond a lane's pair, the up-leg's
+// prolongation one.  Every cell is formed with the arithmetic of the one-sweep
+// legs in the same order; sweeps after the first relax with rb_relax.
+template <int NU>
+struct RBN {
+    static constexpr int G = NU - 1;              // sweeps beyond the first
+    static constexpr int HALO = 1 + NU;           // lanes on either side
+    static constexpr int OWN = 2 * (WAVE - 2 * HALO);
+};
+// Relax the red (red_pass) or black cells of register row q (grid row j) of
+// U from its neighbours in rows q - 1 and q + 1 and the lanes beside it;
+// the pair is (ix, ix + 1) with ix even, so row j's red cell is .x for even j.
+template <int NR>
+__device__ __forceinline__ void rbn_relax_row(f2v (&U)[NR], int q, int j, bool red_pass, int ny, float omega,
+                                              bool hW0, bool hE1, f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) {
+    const f2v um = U[q - 1], uc = U[q], up = U[q + 1];
+    const bool hN = j < ny - 1, hS = j > 0;
+    if (((j & 1) == 0) == red_pass) {
+        const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+        U[q].x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f, hW0 ? uW : 0.f, hN,
+                          hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f, hS ? um.x : 0.f);
+    } else {
+        const float uE = __shfl_down(uc.x, 1, WAVE);
+        U[q].y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x, uc.x, hN,
+                          hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
+    }
+}
+// Down-leg for the R rows iy0 .. iy0+R-1 of a wave: from x = 0, NU times red
+// then black, then the residual of rows iy0 - 1 .. iy0 + R restricted into
+// coarse rows iy0 / 2 .. iy0 / 2 + R / 2 - 1.  Register row q is grid row
+// jb + q, jb = iy0 - 3 - 2G: b and D of rows 0 .. NB-1, n of rows 0 .. NB-2,
+// e of rows 1 .. NB-2, c of rows 2 .. NB-3.
+template <class TB, int NU, int R>
+__global__ __launch_bounds__(256) void k_mg_down_rbn(int nx, int ny, int nxc, float omega,
+                                                     const float* __restrict__ c, const float* __restrict__ e,
+                                                     const float* __restrict__ n, const float* __restrict__ D,
+                                                     const TB* __restrict__ b, float* __restrict__ x,
+                                                     float* __restrict__ bc, const Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    constexpr int G = RBN<NU>::G, HALO = RBN<NU>::HALO, OWN = RBN<NU>::OWN;
+    constexpr int NB = R + 6 + 4 * G;
+    static_assert(R == 2 || R == 4, "two or four rows per wave");
+    const int done = sc->done;  // arrives with the loads below
+    int strip, iy0;
+    wide_tile(R, strip, iy0);  // iy0 even
+    const int lane = threadIdx.x;
+    const int ix = strip * OWN - 2 * HALO + 2 * lane;
+    if (iy0 >= ny || strip * OWN >= nx) return;  // uniform over the wave; no barriers follow
+    const bool own = lane >= HALO && lane <= WAVE - 1 - HALO && ix < nx;
+    const bool inx = ix >= 0 && ix < nx;
+    const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
+    const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+    const f2v z2 = {0.f, 0.f};
+    const int jb = iy0 - 3 - 2 * G;
+    auto row = [&](int q) -> size_t { return cidx(ixc, min(max(jb + q, 0), ny - 1), nx); };
+    f2v Bv[NB], Dv[NB], Nv[NB], Ev[NB], Cv[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const TB2 v = *(const TB2*)(b + row(q));
+        Bv[q] = f2v{(float)v.x, (float)v.y};
+        Dv[q] = ldc2(D, row(q));
+        Nv[q] = q <= NB - 2 ? ldc2(n, row(q)) : z2;
+        Ev[q] = (q >= 1 && q <= NB - 2) ? ldc2(e, row(q)) : z2;
+        Cv[q] = (q >= 2 && q <= NB - 3) ? ldc2(c, row(q)) : z2;
+    }
+    // every load is issued before the flag decides
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+        asm volatile("" : : "v"(Bv[q].x), "v"(Bv[q].y), "v"(Dv[q].x), "v"(Dv[q].y), "v"(Nv[q].x), "v"(Nv[q].y),
+                     "v"(Ev[q].x), "v"(Ev[q].y), "v"(Cv[q].x), "v"(Cv[q].y));
+    if (done) return;
+    // the first sweep's red cells, from zero
+    f2v X[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int j = jb + q;
+        X[q] = z2;
+        if (j >= 0 && j < ny) {
+            if ((j & 1) == 0) X[q].x = mg_div(omega * Bv[q].x, Dv[q].x);
+            else X[q].y = mg_div(omega * Bv[q].y, Dv[q].y);
+        } else {
+            Bv[q] = z2;
+            Dv[q] = z2;
+        }
+    }
+    // its black cells (rows 1 .. NB-2) from their red neighbours
+#pragma unroll
+    for (int q = 1; q <= NB - 2; ++q) {
+        const int j = jb + q;
+        if (j >= 0 && j < ny) {  // uniform over the wave
+            const f2v rm = X[q - 1], rc = X[q], rp = X[q + 1], ej = Ev[q], nj = Nv[q], nm = Nv[q - 1];
+            const bool hN = j < ny - 1, hS = j > 0;
+            if ((j & 1) == 0) {
+                const float xE = __shfl_down(rc.x, 1, WAVE);
+                X[q].y = rb_black0(Bv[q].y, Dv[q].y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
+                                   hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
+            } else {
+                const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                X[q].x = rb_black0(Bv[q].x, Dv[q].x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
+                                   hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
+            }
+        }
+    }
+    // further sweeps: red cells of rows 2g .. NB-1-2g, then black cells of rows 2g+1 .. NB-2-2g
+#pragma unroll
+    for (int g = 1; g <= G; ++g) {
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+            for (int q = 2 * g + pass; q <= NB - 1 - 2 * g - pass; ++q) {
+                const int j = jb + q;
+                if (j >= 0 && j < ny)
+                    rbn_relax_row(X, q, j, pass == 0, ny, omega, hW0, hE1, Bv[q], Cv[q], Dv[q], Ev[q], Nv[q],
+                                  Nv[q - 1]);
+            }
+        }
+    }
+    // the residual of rows iy0-1 .. iy0+R (register rows 2+2G .. 3+2G+R),
+    // restricted: coarse row iy0/2 + k takes fine rows iy0+2k-1 .. iy0+2k+2
+    float acc[R / 2];
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) acc[k] = 0.f;
+    const float wx0 = ix == 0 ? 1.f : 0.75f, wx1 = ix + 1 == nx - 1 ? 1.f : 0.75f;
+#pragma unroll
+    for (int s2 = 0; s2 < R + 2; ++s2) {
+        const int q = 2 + 2 * G + s2, t = iy0 - 1 + s2;
+        if (t >= 0 && t < ny) {  // uniform over the wave
+            const f2v X0 = X[q], Xm = X[q - 1], X1 = X[q + 1];
+            const f2v e0 = Ev[q], n0 = Nv[q], nm = Nv[q - 1], cc = Cv[q], b0 = Bv[q];
+            const float ue = __shfl_down(X0.x, 1, WAVE);
+            const float uw = __shfl_up(X0.y, 1, WAVE);
+            const float ew = __shfl_up(e0.y, 1, WAVE);
+            float a0 = cc.x * X0.x;
+            a0 += e0.x * (X0.x - X0.y);
+            if (hW0) a0 += ew * (X0.x - uw);
+            if (t < ny - 1) a0 += n0.x * (X0.x - X1.x);
+            if (t > 0) a0 += nm.x * (X0.x - Xm.x);
+            float a1 = cc.y * X0.y;
+            if (hE1) a1 += e0.y * (X0.y - ue);
+            a1 += e0.x * (X0.y - X0.x);
+            if (t < ny - 1) a1 += n0.y * (X0.y - X1.y);
+            if (t > 0) a1 += nm.y * (X0.y - Xm.y);
+            // P^T: this row's residuals restricted along x (columns ix-1 ..
+            // ix+2), weighted into coarse row iy0 / 2
+            const float q0 = inx ? b0.x - a0 : 0.f, q1 = inx ? b0.y - a1 : 0.f;
+            const float rl = __shfl_up(q1, 1, WAVE), rr = __shfl_down(q0, 1, WAVE);
+            float h = 0.25f * rl;
+            h += wx0 * q0;
+            h += wx1 * q1;
+            h += 0.25f * rr;
+#pragma unroll
+            for (int k = 0; k < R / 2; ++k)
+                if (s2 >= 2 * k && s2 <= 2 * k + 3) acc[k] += lin_rw(t, s2 - 2 * k, ny) * h;
+            if (own && t >= iy0 && t < iy0 + R) *(f2v*)(x + cidx(ix, t, nx)) = X0;
+        }
+    }
+    const int nyc = (ny + 1) >> 1;
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k)
+        if (own && (iy0 >> 1) + k < nyc) bc[(size_t)(ix >> 1) + (size_t)((iy0 >> 1) + k) * nxc] = acc[k];
+}
+
+// Up-leg for the R rows iy0 .. iy0+R-1 of a wave: u = x + scale P xc on rows
+// iy0-2-2G .. iy0+R+1+2G (register row q is grid row iy0-2-2G+q), then NU
+// times black then red, the last red pass on the wave's rows; b, c, D, e of
+// rows 1 .. NR-2, n of rows 0 .. NR-2, xc of coarse rows
+// iy0/2-2-G .. iy0/2+R/2+1+G.
+template <class TB, class TO, int NU, int R>
+__global__ __launch_bounds__(256) void k_mg_up_rbn(int nx, int ny, int nxc, float omega, float scale,
+                                                   const float* __restrict__ c, const float* __restrict__ e,
+                                                   const float* __restrict__ n, const float* __restrict__ D,
+                                                   const TB* __restrict__ b, const float* __restrict__ x,
+                                                   const float* __restrict__ xc, TO* __restrict__ out, int with_dot,
+                                                   double* partials, Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    constexpr int G = RBN<NU>::G, HALO = RBN<NU>::HALO, OWN = RBN<NU>::OWN;
+    constexpr int NR = R + 4 + 4 * G, NL = R / 2 + 4 + 2 * G;
+    static_assert(R == 2 || R == 4, "two or four rows per wave");
+    const int done = sc->done;  // arrives with the loads below
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
+    const int lane = threadIdx.x;
+    const int ix = strip * OWN - 2 * HALO + 2 * lane;
+    double acc = 0.0;
+    const bool active = iy0 < ny && strip * OWN < nx;  // uniform over the wave
+    const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
+    const int iyc = active ? iy0 : 0;  // rows of an inactive wave: any in-grid rows (never used)
+    const int jb = iyc - 2 - 2 * G;
+    auto row = [&](int q) -> size_t { return cidx(ixc, min(max(jb + q, 0), ny - 1), nx); };
+    const int nyc = (ny + 1) >> 1;
+    f2v Xv[NR], Cv[NR], Dv[NR], Ev[NR], Nv[NR];
+    TB2 Bt[NR];
+    float Lc[NL];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        Xv[q] = *(const f2v*)(x + row(q));
+        const bool mid = q >= 1 && q <= NR - 2;
+        Bt[q] = mid ? *(const TB2*)(b + row(q)) : TB2{};
+        Cv[q] = mid ? ldc2(c, row(q)) : f2v{0.f, 0.f};
+        Dv[q] = mid ? ldc2(D, row(q)) : f2v{0.f, 0.f};
+        Ev[q] = mid ? ldc2(e, row(q)) : f2v{0.f, 0.f};
+        Nv[q] = q <= NR - 2 ? ldc2(n, row(q)) : f2v{0.f, 0.f};
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int J = min(max((iyc >> 1) - 2 - G + q, 0), nyc - 1);
+        Lc[q] = xc[(size_t)(ixc >> 1) + (size_t)J * nxc];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        asm volatile("" : : "v"(Xv[q].x), "v"(Xv[q].y), "v"(Cv[q].x), "v"(Cv[q].y), "v"(Dv[q].x), "v"(Dv[q].y),
+                     "v"(Ev[q].x), "v"(Ev[q].y), "v"(Nv[q].x), "v"(Nv[q].y));
+#pragma unroll
+    for (int q = 0; q < NL; ++q) asm volatile("" : : "v"(Lc[q]));
+    if (done) return;  // uniform over the workgroup, before the partial's barriers
+    if (active) {
+        const bool own = lane >= HALO && lane <= WAVE - 1 - HALO && ix < nx;
+        const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+        const f2v z2 = {0.f, 0.f};
+        // u = x + scale P xc: fine row j's own coarse row is Lc[1 + q/2], its
+        // neighbour Lc[q/2] (even j) or Lc[2 + q/2] (odd j), folded at the edges
+        const float wl = ix == 0 ? 0.f : 0.25f, wr = ix + 1 == nx - 1 ? 0.f : 0.25f;
+        f2v U[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const int j = jb + q;
+            const int i0 = 1 + (q >> 1), i1 = (q & 1) ? i0 + 1 : i0 - 1;
+            const int J1 = (iy0 >> 1) - 2 - G + i1;
+            const bool fold = J1 < 0 || J1 > nyc - 1;
+            const float v = fold ? Lc[i0] : 0.75f * Lc[i0] + 0.25f * Lc[i1];
+            const float vl = __shfl_up(v, 1, WAVE), vr = __shfl_down(v, 1, WAVE);
+            const float p0 = (1.f - wl) * v + wl * vl, p1 = (1.f - wr) * v + wr * vr;
+            U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * p0, Xv[q].y + scale * p1} : z2;
+        }
+        // NU times: black cells of rows 1+2g .. NR-2-2g, red cells of rows 2+2g .. NR-3-2g
+#pragma unroll
+        for (int g = 0; g <= G; ++g) {
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+                for (int q = 1 + 2 * g + pass; q <= NR - 2 - 2 * g - pass; ++q) {
+                    const int j = jb + q;
+                    if (j >= 0 && j < ny)
+                        rbn_relax_row(U, q, j, pass == 1, ny, omega, hW0, hE1,
+                                      f2v{(float)Bt[q].x, (float)Bt[q].y}, Cv[q], Dv[q], Ev[q], Nv[q], Nv[q - 1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < R; ++s2) {
+            const int q = 2 + 2 * G + s2, t = iy0 + s2;
+            if (own && t < ny) {
+                const f2v z = U[q];
+                typename Pair<TO>::type zv;
+                zv.x = (TO)z.x;
+                zv.y = (TO)z.y;
+                *(typename Pair<TO>::type*)(out + cidx(ix, t, nx)) = zv;
+                acc += (double)Bt[q].x * (double)z.x;
+                acc += (double)Bt[q].y * (double)z.y;
+            }
+        }
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
+    }
+}
+
 // Red-black legs for the small levels and for levels of odd width: one
 // thread per coarse cell (down) or fine cell (up), every neighbour's value
 // formed from global memory with the helpers above.
@@ -4185,9 +4464,9 @@ static Geom mg_wide_geom(int nx, int ny) {
 // against 36.5 / 36.2 for 8192 waves and 44.8 / 38.4 for 2048 waves of 16
 // rows: the two rows of lookahead a wave reads before its first residual
 // cost less over more rows.
-static Geom mg_rb_geom(int nx, int ny, bool lin = false) {
+static Geom mg_rb_geom(int nx, int ny, bool lin = false, int own_cols = 0) {
     Geom W = make_geom(nx, ny);
-    const int own = lin ? RB_OWN_LIN : RB_OWN;
+    const int own = own_cols ? own_cols : (lin ? RB_OWN_LIN : RB_OWN);
     const long strips = (nx + own - 1) / own;
     long r = ((long)ny * strips) / 4096;
     W.R = (int)(r < 2 ? 2 : (r > 8 ? 8 : r + (r & 1)));
@@ -4345,9 +4624,43 @@ void mg_up(int nx, int ny, int nxc, float omega, float scale, const float* c, co
 // transfers, where they are fused into the two-row kernels (even width above
 // MG_FLAT_CELLS cells, two rows per wave); false where they are not, and the
 // caller takes the plain legs with the separate transfer kernels.
+// nu = 2: two sweeps each way (k_mg_down_rbn / k_mg_up_rbn) where the level
+// takes two rows per wave with their narrower owned strip; one otherwise.
+bool mg_lin_nu2(int nx, int ny) {
+    return !mg_rb_flat(nx, ny) && !getenv_flag_lin_unfused() && mg_rb_geom(nx, ny, true, RBN<2>::OWN).R == 2;
+}
+// Rows per wave of the two-sweep legs: four on levels of more than
+// MG_RBN4_CELLS cells, where the legs are throughput-bound and four rows share
+// the halo rows two would re-read; two below, where they are latency-bound.
+static int mg_rbn_rows(int nx, int ny) {
+    static const long cells = [] {
+        const char* env = getenv("SHK_MG_RBN4");
+        return env ? atol(env) : MG_RBN4_CELLS;
+    }();
+    return (long)nx * ny > cells ? 4 : 2;
+}
+static Geom mg_rbn_geom(int nx, int ny) {
+    Geom W = mg_rb_geom(nx, ny, true, RBN<2>::OWN);
+    W.R = mg_rbn_rows(nx, ny);
+    W.grid.y = (unsigned)((ny + W.R - 1) / W.R);
+    W.nblocks = W.grid.x * W.grid.y;
+    return W;
+}
 template <class TB, class CT>
 bool mg_down_lin(int nx, int ny, int nxc, float omega, const CT* c, const CT* e, const CT* n, const CT* D,
-                 const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+                 const TB* b, float* x, float* bc, const Scalars* sc, int nu, hipStream_t st) {
+    if constexpr (std::is_same<CT, float>::value) {
+        if (nu == 2 && mg_lin_nu2(nx, ny)) {
+            const Geom W = mg_rbn_geom(nx, ny);
+            if (W.R == 4)
+                hipLaunchKernelGGL((k_mg_down_rbn<TB, 2, 4>), W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D,
+                                   b, x, bc, sc);
+            else
+                hipLaunchKernelGGL((k_mg_down_rbn<TB, 2, 2>), W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D,
+                                   b, x, bc, sc);
+            return true;
+        }
+    }
     if (mg_rb_flat(nx, ny) || getenv_flag_lin_unfused()) return false;
     const Geom W = mg_rb_geom(nx, ny, true);
     if (W.R != 2) return false;
@@ -4358,23 +4671,36 @@ bool mg_down_lin(int nx, int ny, int nxc, float omega, const CT* c, const CT* e,
 template <class TB, class CT, class TO>
 bool mg_up_lin(int nx, int ny, int nxc, float omega, float scale, const CT* c, const CT* e, const CT* n,
                const CT* D, const TB* b, const float* x, const float* xc, TO* out, int dot_mode, double* partials,
-               Scalars* sc, hipStream_t st) {
-    if (mg_rb_flat(nx, ny) || getenv_flag_lin_unfused()) return false;
-    const Geom M = mg_rb_geom(nx, ny, true);
-    if (M.R != 2) return false;
+               Scalars* sc, int nu, hipStream_t st) {
+    const bool nu2 = std::is_same<CT, float>::value && nu == 2 && mg_lin_nu2(nx, ny);
+    if (!nu2 && (mg_rb_flat(nx, ny) || getenv_flag_lin_unfused())) return false;
+    const Geom M = nu2 ? mg_rbn_geom(nx, ny) : mg_rb_geom(nx, ny, true);
+    if (!nu2 && M.R != 2) return false;
     const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
-    hipLaunchKernelGGL((k_mg_up_rb2s<TB, CT, TO, true>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n,
-                       D, b, x, xc, out, with_dot, partials, sc);
+    if constexpr (std::is_same<CT, float>::value) {
+        if (nu2 && M.R == 4)
+            hipLaunchKernelGGL((k_mg_up_rbn<TB, TO, 2, 4>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e,
+                               n, D, b, x, xc, out, with_dot, partials, sc);
+        else if (nu2)
+            hipLaunchKernelGGL((k_mg_up_rbn<TB, TO, 2, 2>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e,
+                               n, D, b, x, xc, out, with_dot, partials, sc);
+        else
+            hipLaunchKernelGGL((k_mg_up_rb2s<TB, CT, TO, true>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale,
+                               c, e, n, D, b, x, xc, out, with_dot, partials, sc);
+    } else {
+        hipLaunchKernelGGL((k_mg_up_rb2s<TB, CT, TO, true>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e,
+                           n, D, b, x, xc, out, with_dot, partials, sc);
+    }
     if (dot_mode == 0 || dot_mode == 1)
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)M.nblocks,
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
     return true;
 }
 template bool mg_down_lin<float, float>(int, int, int, float, const float*, const float*, const float*, const float*,
-                                        const float*, float*, float*, const Scalars*, hipStream_t);
+                                        const float*, float*, float*, const Scalars*, int, hipStream_t);
 template bool mg_up_lin<float, float, float>(int, int, int, float, float, const float*, const float*, const float*,
                                              const float*, const float*, const float*, const float*, float*, int,
-                                             double*, Scalars*, hipStream_t);
+                                             double*, Scalars*, int, hipStream_t);
 template void mg_down<float>(int, int, int, int, float, const float*, const float*, const float*, const float*,
                              const float*, float*, float*, const Scalars*, int, hipStream_t);
 template void mg_down<double>(int, int, int, int, float, const float*, const float*, const float*, const float*,

@@ -13,14 +13,16 @@ pinned by what the mathematics fixes on a real assembled glacier system (CPU):
   operator equals the explicit Galerkin product P^T A P with every link laid
   onto its lattice paths, and the cycle with them stays symmetric positive
   definite and needs fewer CG iterations than with aggregation, the reason
-  the library uses them below the finest level.
+  the library uses them below the finest level;
+* two sweeps each way on the levels below the finest keep the cycle symmetric
+  positive definite and cut CG iterations further.
 
 The GPU test (test_gpu_vcycle.py) then checks the library's V-cycle against it."""
 import numpy as np
 import pytest
 
 from helpers import oracle_state
-from vcycle_restate import VCycle, F32, flux_apply, coarsen_lin, prolong_lin, restrict_lin
+from vcycle_restate import VCycle, F32, flux_apply, coarsen_lin, prolong_lin, restrict_lin, library_nu2
 
 NX, NY = 256, 64
 
@@ -78,10 +80,10 @@ class _JacobiCycle(VCycle):
         return u + self.omega * (b - flux_apply(c, e, n, u)) / D
 
 
-@pytest.mark.parametrize("bil_from", [None, 1, 0])
-def test_vcycle_is_symmetric_positive_definite(system, bil_from):
+@pytest.mark.parametrize("bil_from,nu2", [(None, False), (1, False), (0, False), (1, True)])
+def test_vcycle_is_symmetric_positive_definite(system, bil_from, nu2):
     co = system
-    M = VCycle(co[0], co[1], co[3], bil_from=bil_from)
+    M = VCycle(co[0], co[1], co[3], bil_from=bil_from, nu2=(lambda nx, ny: True) if nu2 else None)
     rng = np.random.default_rng(0)
     rs = [np.asfortranarray(rng.standard_normal(co[0].shape)) for _ in range(3)]
     zs = [M(r) for r in rs]
@@ -233,3 +235,25 @@ def test_linear_transfers_cut_cg_iterations(oracle, system):
     lin0 = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=0))
     print(f"PCG iterations: aggregation {agg}, linear below the finest level {lin1}, linear everywhere {lin0}")
     assert lin1 < 0.85 * agg and lin0 < 0.85 * agg, (agg, lin1, lin0)
+
+
+def test_two_sweeps_below_the_finest_level_cut_cg_iterations(oracle, system):
+    """Two red-black sweeps each way on the levels below the finest (the
+    library's choice where those levels take the two-row legs): fewer PCG
+    iterations than one sweep, on the same hierarchy."""
+    co = system
+    one = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=1))
+    two = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=1, nu2=lambda nx, ny: True))
+    print(f"PCG iterations: one sweep each way {one}, two below the finest level {two}")
+    assert two < one, (one, two)
+
+
+def test_library_rule_for_two_sweeps():
+    """The levels that take two sweeps: more than 4096 cells, even width, and two
+    rows per wave at 116 owned columns (ny * ceil(nx / 116) // 4096 <= 2)."""
+    assert [library_nu2(nx, ny) for nx, ny in [(2048, 512), (1024, 256), (512, 128), (256, 64), (128, 32)]] == \
+        [True, True, True, True, False]
+    assert not library_nu2(4096, 1024)      # 36 strips x 1024 rows: eight rows per wave
+    assert not library_nu2(513, 256)        # odd width: the flat legs
+    assert library_nu2(1026, 512) and library_nu2(2050, 513) and library_nu2(130, 33)
+    assert not library_nu2(64, 64)          # 4096 cells: the tail's

@@ -30,6 +30,14 @@ from a reference file:
   (faces beyond the grid taken as zero; wy the restriction weights in y) and
   nothing on the y-faces; y-faces likewise.
 
+* nu2: levels (below the finest, reached by the linear transfers) that take
+  two sweeps each way: down-leg red, black, red, black (the first red cells
+  from zero, every later half-sweep x += omega (b - A x) / D on its colour),
+  up-leg black, red, black, red.  `library_nu2` is the library's rule for
+  which levels do: those that take the two-row legs (even width, more than
+  4096 cells) and keep two rows per wave with the wider halo, 116 owned
+  columns per wave: ny * ceil(nx / 116) // 4096 <= 2.
+
 Arrays are indexed [ix, iy] with shape (nx, ny); everything is float32 (the
 linear transfers and the coarsening in float64, rounded once).
 """
@@ -184,8 +192,18 @@ def dense_inverse(c, e, n):
     return np.linalg.inv(A).astype(F32)
 
 
+def library_nu2(nx, ny):
+    """Whether the library's level of nx x ny cells (with the linear transfers)
+    smooths twice each way."""
+    if nx * ny <= 4096 or nx % 2:
+        return False
+    strips = -(-nx // 116)
+    return (ny * strips) // 4096 <= 2
+
+
 class VCycle:
-    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False, bil_from=None, scale_bil=0.8):
+    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False, bil_from=None, scale_bil=0.8,
+                 nu2=None):
         self.omega, self.scale, self.scale_bil = F32(omega), F32(scale), F32(scale_bil)
         c, e, n, D = fine_level(aP, aE, aN)
         self.levels = [(c, e, n, D)]
@@ -204,6 +222,10 @@ class VCycle:
         self.inv = dense_inverse(c, e, n)
         self.red = [((np.arange(L[0].shape[0])[:, None] + np.arange(L[0].shape[1])[None, :]) % 2) == 0
                     for L in self.levels]
+        # sweeps each way per level: two where nu2 (a rule on the level's
+        # shape) says so on a level below the finest with the linear transfers
+        self.nu = [2 if (nu2 is not None and 0 < l < len(self.lin) and self.lin[l] and nu2(*L[0].shape)) else 1
+                   for l, L in enumerate(self.levels)]
 
     def _cycle(self, b, l):
         c, e, n, D = self.levels[l]
@@ -213,10 +235,18 @@ class VCycle:
             return (self.inv @ flat).reshape((nx, ny), order="F").astype(F32)
         om, red = self.omega, self.red[l]
         blk = ~red
+
+        def relax(u, mask):
+            t = u + om * (b - flux_apply(c, e, n, u)) / D
+            u[mask] = t[mask]
+
         x = np.zeros_like(b)
         x[red] = om * b[red] / D[red]
         s = b + neighbour_sum(e, n, x)
         x[blk] = om * s[blk] / D[blk]
+        for _ in range(self.nu[l] - 1):
+            relax(x, red)
+            relax(x, blk)
         r = b - flux_apply(c, e, n, x)
         fx, fy = b.shape
         cx, cy = self.levels[l + 1][0].shape
@@ -229,9 +259,9 @@ class VCycle:
             rc = (rp[0::2, 0::2] + rp[1::2, 0::2]) + (rp[0::2, 1::2] + rp[1::2, 1::2])
             xc = self._cycle(rc.astype(F32), l + 1)
             u = x + self.scale * np.repeat(np.repeat(xc, 2, 0), 2, 1)[:fx, :fy]
-        for mask in (blk, red):
-            t = u + om * (b - flux_apply(c, e, n, u)) / D
-            u[mask] = t[mask]
+        for _ in range(self.nu[l]):
+            relax(u, blk)
+            relax(u, red)
         return u
 
     def __call__(self, r):
