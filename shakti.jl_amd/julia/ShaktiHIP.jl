@@ -24,6 +24,10 @@ A breakdown of the Krylov recurrence (r'z < 0: the preconditioner or the
 operator is not positive definite) raises an `ErrorException`, as the
 reference's `sqrt(r'z)` does inside `cg!`, and leaves `s.h` unchanged.
 
+`HIPSimulation(sim)` keeps the whole State on the GPU instead: its `step!`
+and `run!` advance it with whole-step library calls and refresh the host State
+only for observers, so nothing crosses PCIe inside a step.
+
 The learned form: `HIPUNet(params)` uploads a trained U-Net (the Float32
 parameter vector, e.g. `NPZ.npzread(path)["params"]` of
 `shakti_hip/weights/unet_16_32_64_128.npz`), and
@@ -266,6 +270,175 @@ function solve_elliptic_linear_system!(ls::HIPPCGSolver, s::Shakti.State, g::Sha
     check(rc, ls.ctx, "shk_solve_elliptic")
     download!(s.h, ls, SHK_H)
     return nothing
+end
+
+# ---- the whole timestep on a device-resident State ------------------------------
+
+# every State field in the order of enum shk_field (src/state.jl:13-57)
+const STATE_FIELDS = (:h, :pw, :po, :b, :beta, :abs_ub, :mdot, :shear, :potential, :sensible, :Re, :K, :G,
+                      :zb, :zs, :H, :ieb, :lambda, :A_visc, :N, :mask, :dhdx, :q_x, :Re_x, :b_x, :ub_x,
+                      :taub_x, :dpwdx, :valid_x, :dhdy, :q_y, :Re_y, :b_y, :ub_y, :taub_y, :dpwdy, :valid_y)
+
+gap_code(::Shakti.ImplicitGapScheme) = Int32(0)
+gap_code(::Shakti.ExplicitGapScheme) = Int32(1)
+sliding_model(sl::Shakti.RegularizedCoulombSlidingLaw) = (Int32(0), Float64(sl.C), 0.0, false)
+sliding_model(::Shakti.PrescribedSlidingLaw) = (Int32(1), 0.5, 0.0, false)
+sliding_model(::Shakti.LinearSlidingLaw) = (Int32(2), 0.5, 0.0, true)
+sliding_model(::Shakti.RegularizedCoulombFieldSlidingLaw) = (Int32(3), 0.5, 0.0, true)
+sliding_model(sl::Shakti.RegularizedCoulombV0SlidingLaw) = (Int32(4), 0.5, Float64(sl.v0), true)
+relax_alpha(::Shakti.NoHeadRelaxation) = 0.0
+relax_alpha(hr::Shakti.UnderHeadRelaxation) = Float64(hr.alpha)
+
+"""
+    HIPSimulation(sim::Shakti.Simulation; solver = HIPPCGSolver(sim.grid; multigrid = true, rtol = 1e-10,
+                  atol = 0.0, check_every = 32), sliding_C = nothing, moulin = nothing)
+
+`sim` with its State kept on the GPU: `step!` / `run!` below advance it with
+whole-step library calls (`shk_step`: update_ieb!, step_h!, step_b!,
+src/run.jl:82-164), so no field crosses PCIe inside a step.  The host State
+`sim.state` is uploaded once here and refreshed from the device only where
+something reads it: an observer that records (src/observer.jl), the verbose
+diagnostics line, and `sync_host!`.  The head scheme (`EllipticHeadScheme`
+with its `PicardSolver`'s iters, tol, check_every and relaxation, or
+`ParabolicHeadScheme`, which takes the parabolic solve of `shk_step` through
+`p.e_v`), the gap scheme, the K_face scheme and the sliding law are read from
+`sim`; `ps.converged` / `ps.last_iter` are set as Picard_loop! sets them.
+
+The field sliding laws store only their face values (`Cx`, `Cx2`); give
+their cell-centred coefficient as `sliding_C` (the library staggers it as
+their constructors do, src/sliding_law.jl:73-90, 284-301, 436-452).  The
+melt input is refreshed on the host by the reference's own `update_ieb!`,
+which writes one uniform value (src/melt_input.jl:86-94), and the library
+writes that value on the device; `moulin` (an nx x ny field of point-moulin
+input) is added on top of it every step (no reference counterpart).
+"""
+mutable struct HIPSimulation
+    sim::Shakti.Simulation
+    ls::HIPPCGSolver
+    moulin::Bool
+end
+
+function HIPSimulation(sim::Shakti.Simulation;
+                       solver::HIPPCGSolver = HIPPCGSolver(sim.grid; multigrid = true, rtol = 1e-10, atol = 0.0,
+                                                           check_every = 32),
+                       sliding_C::Union{Nothing, AbstractMatrix} = nothing,
+                       moulin::Union{Nothing, AbstractMatrix} = nothing)
+    ls, s = solver, sim.state
+    code, C, v0, field = sliding_model(sim.sl)
+    field && sliding_C === nothing &&
+        error("HIPSimulation: $(typeof(sim.sl)) needs its cell-centred coefficient as sliding_C")
+    check(ccall((:shk_set_params, libshk), Cint, (Ptr{Cvoid}, Ref{ShkParams}), ls.ctx, Ref(params_c(sim.p))),
+          ls.ctx, "shk_set_params")
+    check(ccall((:shk_set_model, libshk), Cint, (Ptr{Cvoid}, Ref{ShkModel}), ls.ctx,
+                Ref(ShkModel(kfs_code(sim.kfs), gap_code(sim.gs), code, C, v0))), ls.ctx, "shk_set_model")
+    check(ccall((:shk_set_solver_cfg, libshk), Cint, (Ptr{Cvoid}, Ref{ShkSolverCfg}), ls.ctx, Ref(ls.cfg)),
+          ls.ctx, "shk_set_solver_cfg")
+    for (i, f) in enumerate(STATE_FIELDS)
+        upload(ls, Int32(i - 1), getfield(s, f))
+    end
+    if field
+        Ch = Array{Float64}(sliding_C)
+        GC.@preserve Ch check(ccall((:shk_set_sliding_coefficient, libshk), Cint,
+                                    (Ptr{Cvoid}, Ptr{Float64}, Csize_t), ls.ctx, Ch, length(Ch)),
+                              ls.ctx, "shk_set_sliding_coefficient")
+    end
+    if moulin !== nothing
+        M = Array{Float64}(moulin)
+        GC.@preserve M check(ccall((:shk_set_moulin_field, libshk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Csize_t),
+                                   ls.ctx, M, length(M)), ls.ctx, "shk_set_moulin_field")
+    end
+    return HIPSimulation(sim, ls, moulin !== nothing)
+end
+
+"Download every State field of the device into `h.sim.state`."
+function sync_host!(h::HIPSimulation)
+    for (i, f) in enumerate(STATE_FIELDS)
+        download!(getfield(h.sim.state, f), h.ls, Int32(i - 1))
+    end
+    return h.sim.state
+end
+
+picard_args(hs::Shakti.EllipticHeadScheme) = (Int32(hs.ps.iters), Float64(hs.ps.tol), Int32(hs.ps.check_every),
+                                              relax_alpha(hs.ps.hr))
+picard_args(::Shakti.ParabolicHeadScheme) = (Int32(1), 0.0, Int32(1), 0.0)
+set_picard!(hs::Shakti.EllipticHeadScheme, conv, last) = (hs.ps.converged = conv; hs.ps.last_iter = last)
+set_picard!(::Shakti.ParabolicHeadScheme, conv, last) = nothing
+
+"step! (src/run.jl:82-88) on the device: update_ieb!, step_h!, step_b! in one library call."
+function Shakti.step!(h::HIPSimulation)
+    sim = h.sim
+    if sim.mi isa Shakti.ConstantMeltInput
+        mode, uniform = h.moulin ? Int32(2) : Int32(0), 0.0
+    else
+        # the reference's update_ieb! writes one value everywhere; the device takes that value
+        Shakti.update_ieb!(sim.mi, sim.state, sim.total_time[])
+        mode, uniform = h.moulin ? Int32(2) : Int32(1), Float64(Array(sim.state.ieb)[1])
+    end
+    iters, tol, check_every, alpha = picard_args(sim.hs)
+    conv, last, cg = Ref{Int32}(0), Ref{Int32}(0), Ref{Int64}(0)
+    check(ccall((:shk_step, libshk), Cint,
+                (Ptr{Cvoid}, Float64, Int32, Float64, Int32, Float64, Int32, Float64, Ref{Int32}, Ref{Int32},
+                 Ref{Int64}), h.ls.ctx, Float64(sim.dt), mode, uniform, iters, tol, check_every, alpha, conv, last,
+                cg), h.ls.ctx, "shk_step")
+    set_picard!(sim.hs, conv[] != 0, Int(last[]))
+    return h
+end
+
+observes(::Shakti.NoObserver) = false
+observes(::Shakti.AbstractObserver) = true
+
+"""
+    run!(h::HIPSimulation; checkpoint_every, checkpoint_path, restart_path)
+
+run! (src/run.jl:21-60) on the device State: the same loop, observer calls
+and verbose line, with checkpoints written and read by the library
+(shk_checkpoint_save / shk_checkpoint_load: the whole device State, written
+asynchronously while the next steps run, atomically renamed, checksummed).
+"""
+function Shakti.run!(h::HIPSimulation; checkpoint_every::Union{Nothing, Int} = nothing,
+                     checkpoint_path::Union{Nothing, String} = nothing,
+                     restart_path::Union{Nothing, String} = nothing)
+    sim, ctx = h.sim, h.ls.ctx
+    (checkpoint_every === nothing) != (checkpoint_path === nothing) &&
+        error("checkpoint_every and checkpoint_path must be given together")
+    obs = observes(sim.observer)
+    if restart_path === nothing
+        sim.total_time[] = zero(sim.dt)
+        Shakti.prepare!(sim.observer, sim.state)
+        Shakti.observe!(sim.observer, sim.state, 0, sim.total_time[])
+        start_t = 0
+    else
+        t_ref, tt_ref = Ref{Int64}(0), Ref{Float64}(0.0)
+        check(ccall((:shk_checkpoint_load, libshk), Cint, (Ptr{Cvoid}, Cstring, Ref{Int64}, Ref{Float64}),
+                    ctx, restart_path, t_ref, tt_ref), ctx, "shk_checkpoint_load")
+        start_t = Int(t_ref[])
+        sim.total_time[] = tt_ref[]
+        sync_host!(h)
+        Shakti.resume!(sim.observer, sim.state, start_t)
+    end
+    for t in (start_t + 1):sim.tsteps
+        step_time = @elapsed Shakti.step!(h)
+        sim.total_time[] += sim.dt
+        obs && sync_host!(h)
+        Shakti.observe!(sim.observer, sim.state, t, sim.total_time[])
+        if checkpoint_every !== nothing && t % checkpoint_every == 0
+            check(ccall((:shk_checkpoint_save, libshk), Cint, (Ptr{Cvoid}, Cstring, Int64, Float64, Int32),
+                        ctx, checkpoint_path, t, sim.total_time[], 1), ctx, "shk_checkpoint_save")
+        end
+        if sim.verbose
+            converged, last_iter = Shakti.picard_status(sim.hs)
+            println("$t / $(sim.tsteps) completed in $(round(step_time; digits = 4))s. Picard converged: $converged in $last_iter iterations")
+            if converged === false
+                s = obs ? sim.state : sync_host!(h)
+                println("  diagnostics: N=$(extrema(Array(s.N))) Re=$(extrema(Array(s.Re))) b=$(extrema(Array(s.b))) h=$(extrema(Array(s.h)))")
+            end
+            flush(stdout)
+        end
+    end
+    checkpoint_every !== nothing &&
+        check(ccall((:shk_checkpoint_wait, libshk), Cint, (Ptr{Cvoid},), ctx), ctx, "shk_checkpoint_wait")
+    Shakti.finalize!(sim.observer, sim.state)
+    return h
 end
 
 "solve_parabolic_linear_system! for the GPU solver (src/linear_solver.jl:935-945)."

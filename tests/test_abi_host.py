@@ -132,3 +132,125 @@ def test_synthetic_inputs_deterministic(hip):
 
 
 H_OCEAN = 1.0
+
+
+# ---- the Julia binding (shakti.jl_amd/julia/ShaktiHIP.jl) against the C ABI ----
+# Julia is not in the image, so the binding is checked as text: every ccall
+# names a symbol the library exports and the headers declare, with as many
+# argument types as the declaration has parameters, and every mirrored struct
+# has the C struct's fields, in order and of the same widths.
+
+JL = ROOT / "shakti.jl_amd" / "julia" / "ShaktiHIP.jl"
+
+
+def _header_text():
+    text = "".join((ROOT / "include" / h).read_text() for h in ("shakti_hip.h", "shakti_learned.h"))
+    return re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+
+
+def _top_level_items(s):
+    """Comma-separated items of s at bracket depth 0 (empty items dropped)."""
+    items, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "({[":
+            depth += 1
+        elif ch in ")}]":
+            depth -= 1
+        if ch == "," and depth == 0:
+            items.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    items.append(cur.strip())
+    return [i for i in items if i]
+
+
+def _c_arity():
+    out = {}
+    for m in re.finditer(r"\b(shk_\w+)\s*\(([^()]*)\)\s*;", _header_text()):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else len(_top_level_items(args))
+    return out
+
+
+def _jl_ccalls():
+    src = JL.read_text()
+    calls = []
+    for m in re.finditer(r"ccall\(\(:(\w+),\s*libshk\),\s*[\w{}]+,\s*\(", src):
+        i = j = m.end() - 1
+        depth = 0
+        while True:
+            depth += {"(": 1, ")": -1}.get(src[j], 0)
+            if depth == 0:
+                break
+            j += 1
+        calls.append((m.group(1), len(_top_level_items(src[i + 1:j]))))
+    return calls
+
+
+def test_julia_ccalls_match_the_exported_declarations(hip):
+    from shakti_hip import _lib as L
+    lib = C.CDLL(str(L.LIB_PATH))
+    arity = _c_arity()
+    calls = _jl_ccalls()
+    names = {n for n, _ in calls}
+    # the plug-in's solve path, the device-resident simulation and the learned form
+    for need in ("shk_create", "shk_solve_elliptic", "shk_solve_parabolic", "shk_step", "shk_checkpoint_save",
+                 "shk_checkpoint_load", "shk_unet_create", "shk_attach_unet", "shk_set_moulin_field"):
+        assert need in names, need
+    bad = [(n, k, arity.get(n)) for n, k in calls if arity.get(n) != k or not hasattr(lib, n)]
+    assert not bad, bad
+
+
+_JL_WIDTH = {"Int32": ("int32_t", 4), "Int64": ("int64_t", 8), "Float64": ("double", 8), "Float32": ("float", 4),
+             "UInt64": ("uint64_t", 8)}
+
+
+def _c_struct(name):
+    m = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + r"\s*;", _header_text())
+    assert m, name
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        ctype, rest = decl.split(None, 1)
+        fields += [(f.strip(), ctype) for f in rest.split(",")]
+    return fields
+
+
+def _jl_struct(name):
+    m = re.search(r"(?:mutable )?struct " + name + r"\n(.*?)\nend", JL.read_text(), flags=re.S)
+    assert m, name
+    fields = []
+    for line in m.group(1).splitlines():
+        if "=" in line:  # an inner constructor
+            continue
+        for item in line.split(";"):
+            item = item.strip()
+            if "::" in item:
+                f, t = item.split("::")
+                fields.append((f.strip(), _JL_WIDTH[t.strip()][0]))
+    return fields
+
+
+@pytest.mark.parametrize("jl,c", [("ShkGrid", "shk_grid"), ("ShkParams", "shk_params"), ("ShkModel", "shk_model"),
+                                  ("ShkSolverCfg", "shk_solver_cfg"), ("ShkSolveStats", "shk_solve_stats"),
+                                  ("ShkUnetCfg", "shk_unet_cfg")])
+def test_julia_structs_mirror_the_c_structs(jl, c):
+    assert _jl_struct(jl) == _c_struct(c)
+
+
+def test_julia_state_fields_follow_the_field_ids():
+    """STATE_FIELDS (the device-resident simulation's upload/download order)
+    is enum shk_field's order, src/state.jl:13-57, and names each field as
+    the reference's State does."""
+    text = _header_text()
+    enum = re.search(r"enum shk_field \{(.*?)\};", text, flags=re.S).group(1)
+    ids = [e.strip().split("=")[0].strip() for e in enum.split(",") if e.strip()]
+    assert ids[-1] == "SHK_NUM_FIELDS"
+    m = re.search(r"const STATE_FIELDS = \((.*?)\)\n", JL.read_text(), flags=re.S)
+    jl = [f.strip().lstrip(":") for f in m.group(1).replace("\n", " ").split(",") if f.strip()]
+    assert len(jl) == len(ids) - 1 == 37
+    # the ice thickness is State.H, SHK_HICE in the header
+    assert ["SHK_HICE" if f == "H" else "SHK_" + f.upper() for f in jl] == ids[:-1]
