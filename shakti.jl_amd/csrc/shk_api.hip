@@ -464,7 +464,22 @@ void cheb_refresh(shk_ctx* c, const Work<T>& w, int nsteps, int degree) {
 // Multigrid hierarchy for the FP32 flux-form system in w (level 0 shares its
 // arrays); coarsens 2x2 until at most 64 cells remain (at least once), then
 // inverts the coarsest operator densely.
-int mg_setup(shk_ctx* ctx, const Work<float>& w) {
+// Whether the finest level takes BF16 coefficients, and their arrays: decided
+// before the assembly, which can write them (launch::AsmLow).
+int mg_prepare_fine(shk_ctx* ctx) {
+    const int nx0 = ctx->g.nx, ny0 = ctx->g.ny;
+    ctx->mg_bf16 = ctx->mg_rb && launch::mg_bf16_ok(nx0, ny0) && !getenv("SHK_MG_FP32_FINE");
+    if (ctx->mg_bf16 && !ctx->mg_c16) {
+        const size_t n0 = (size_t)nx0 * ny0;
+        uint16_t** ps[] = {&ctx->mg_c16, &ctx->mg_e16, &ctx->mg_n16, &ctx->mg_D16};
+        for (auto pp : ps) HIPCHK(hipMalloc((void**)pp, n0 * sizeof(uint16_t) + 256));
+    }
+    return SHK_OK;
+}
+
+// fine_ready: the finest level's BF16 coefficients were written already (by
+// the assembly, from the FP32 values w holds).
+int mg_setup(shk_ctx* ctx, const Work<float>& w, bool fine_ready = false) {
     const int nx0 = ctx->g.nx, ny0 = ctx->g.ny;
     if (ctx->mg.empty()) {
         shk_ctx::MGLevel L0{nx0, ny0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, false};
@@ -492,7 +507,6 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
         // inverse in LDS: stay within what one workgroup may hold (160 KB on
         // gfx950), less the kernel's 4 KB of static LDS (degenerate shapes
         // such as n x 1 halve only once per level)
-        if (getenv("SHK_MG_JACOBI")) ctx->mg_rb = 0;
         if (const char* e = getenv("SHK_MG_LIN_FROM")) ctx->mg_lin_from = atoi(e);  // negative: none
         if (const char* e = getenv("SHK_MG_NU")) ctx->mg_nu = atoi(e) == 1 ? 1 : 2;
         for (size_t l = 0; l + 1 < ctx->mg.size(); ++l)
@@ -559,14 +573,12 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w) {
     L0.e = (float*)w.aE;
     L0.n = (float*)w.aN;
     L0.D = (float*)w.dg;
-    ctx->mg_bf16 = ctx->mg_rb && launch::mg_bf16_ok(L0.nx, L0.ny) && !getenv("SHK_MG_FP32_FINE");
-    if (ctx->mg_bf16) {
-        const size_t n0 = (size_t)L0.nx * L0.ny;
-        if (!ctx->mg_c16) {
-            uint16_t** ps[] = {&ctx->mg_c16, &ctx->mg_e16, &ctx->mg_n16, &ctx->mg_D16};
-            for (auto pp : ps) HIPCHK(hipMalloc((void**)pp, n0 * sizeof(uint16_t) + 256));
-        }
-        launch::to_bf16x4(n0, L0.c, L0.e, L0.n, L0.D, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16, ctx->mg_D16, ctx->st);
+    if (!fine_ready) {
+        int rc = mg_prepare_fine(ctx);
+        if (rc) return rc;
+        if (ctx->mg_bf16)
+            launch::to_bf16x4((size_t)L0.nx * L0.ny, L0.c, L0.e, L0.n, L0.D, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
+                              ctx->mg_D16, ctx->st);
     }
     for (size_t l = 1; l < ctx->mg.size(); ++l) {
         auto& f = ctx->mg[l - 1];
@@ -1167,10 +1179,27 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
                     "and the multigrid or Chebyshev preconditioner");
     const int rec_first = ctx->rec_count;
     const bool need_c = multigrid || learned || record;
+    // FP64 CG with the V-cycle: the assembly also writes the FP32 flux form the
+    // hierarchy is built from and the finest level's BF16 coefficients
+    launch::AsmLow lo;
+    if (multigrid) {
+        int rc0 = mg_prepare_fine(ctx);
+        if (rc0) return rc0;
+        lo.c = ctx->aP32;
+        lo.e = ctx->aE32;
+        lo.n = ctx->aN32;
+        lo.d = ctx->dg32;
+        if (ctx->mg_bf16) {
+            lo.c16 = ctx->mg_c16;
+            lo.e16 = ctx->mg_e16;
+            lo.n16 = ctx->mg_n16;
+            lo.d16 = ctx->mg_D16;
+        }
+    }
     {
-        Prof pr(ctx, P_ASM, (need_c ? 18 : 17) * nb);
+        Prof pr(ctx, P_ASM, (need_c ? 18 : 17) * nb + (multigrid ? (ctx->mg_bf16 ? 3.0 : 2.0) * nb : 0.0));
         launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st,
-                                 need_c ? ctx->cP : nullptr);
+                                 need_c ? ctx->cP : nullptr, lo);
     }
     ctx->have_system = true;
     ctx->system_parabolic = parabolic;
@@ -1181,17 +1210,10 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     if (cheb) cheb_refresh<double>(ctx, w, cfg.cheb_nsteps, cfg.cheb_degree);
     if (multigrid) {
         // FP64 CG preconditioned by the FP32 V-cycle: the hierarchy is built
-        // from the FP32 flux form of this system
-        {
-            Prof pr(ctx, P_IR, 4 * 12.0 * ctx->n);
-            launch::to_f32(ctx->n, ctx->cP, ctx->aP32, ctx->st);
-            launch::to_f32(ctx->n, ctx->aE, ctx->aE32, ctx->st);
-            launch::to_f32(ctx->n, ctx->aN, ctx->aN32, ctx->st);
-            launch::to_f32(ctx->n, ctx->aP, ctx->dg32, ctx->st);
-        }
+        // from the FP32 flux form of this system (written by the assembly)
         Work<float> wi{ctx->aP32, ctx->aE32, ctx->aN32, ctx->dg32, nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-        int rc0 = mg_setup(ctx, wi);
+        int rc0 = mg_setup(ctx, wi, true);
         if (rc0) return rc0;
     }
     const double* x0 = ctx->F.f[SHK_H];
@@ -1399,6 +1421,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     ctx->use_fused = getenv("SHK_NO_FUSED") == nullptr;
     ctx->use_defer = getenv("SHK_NO_DEFER") == nullptr;
     ctx->use_adaptive = getenv("SHK_FIXED_BATCH") == nullptr;
+    if (getenv("SHK_MG_JACOBI")) ctx->mg_rb = 0;
     if (const char* e = getenv("SHK_BATCH_MARGIN")) ctx->batch_margin = atoi(e);
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;

@@ -158,9 +158,16 @@ void fill_ieb(int n, double* ieb, double uniform, const double* moulin, hipStrea
 void zero(double* p, size_t n, hipStream_t st);
 void copy(const double* a, double* b, size_t n, hipStream_t st);
 
+// Low-precision copies the assembly can write beside the FP64 system: the
+// FP32 flux form (c = cP, e = aE, n = aN, d = aP, each rounded to nearest) and
+// those FP32 values rounded to BF16 (nearest even); a null pointer skips a set.
+struct AsmLow {
+    float *c = nullptr, *e = nullptr, *n = nullptr, *d = nullptr;
+    uint16_t *c16 = nullptr, *e16 = nullptr, *n16 = nullptr, *d16 = nullptr;
+};
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
-              hipStream_t st, double* cP = nullptr);
+              hipStream_t st, double* cP = nullptr, const AsmLow& lo = AsmLow{});
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st);
 

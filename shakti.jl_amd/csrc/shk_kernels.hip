@@ -87,6 +87,11 @@ __device__ __forceinline__ double jl_max0(double v) { return isnan(v) ? v : (v >
 __device__ __forceinline__ double jl_clamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
 
 __device__ __forceinline__ size_t cidx(int ix, int iy, int nx) { return (size_t)ix + (size_t)iy * (size_t)nx; }
+// FP32 -> BF16 bits, round to nearest even
+__device__ __forceinline__ uint16_t bf16_rne(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
 __device__ __forceinline__ size_t xidx(int ix, int iy, int nx) { return (size_t)ix + (size_t)iy * (size_t)(nx + 1); }
 
 // Virtual workgroup id: XCD k (hardware id % 8) receives one contiguous range.
@@ -566,7 +571,7 @@ __device__ __forceinline__ double bK(const Phys& P, double K1, double K2, double
 template <bool PARABOLIC, class T>
 __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T* __restrict__ aP,
                                                   T* __restrict__ aE, T* __restrict__ aN, T* __restrict__ rhs,
-                                                  double* __restrict__ cP) {
+                                                  double* __restrict__ cP, launch::AsmLow lo) {
     const int nx = P.nx, ny = P.ny;
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
@@ -574,17 +579,35 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T
     FV s(F);
     {
         const size_t c = cidx(ix, iy, nx);
+        // the low-precision copies of one row of the flux form (c, e, n, d)
+        auto low = [&](double cc, double ee, double nn, double dd) {
+            if (lo.c) {
+                const float c32 = (float)cc, e32 = (float)ee, n32 = (float)nn, d32 = (float)dd;
+                lo.c[c] = c32;
+                lo.e[c] = e32;
+                lo.n[c] = n32;
+                lo.d[c] = d32;
+                if (lo.c16) {
+                    lo.c16[c] = bf16_rne(c32);
+                    lo.e16[c] = bf16_rne(e32);
+                    lo.n16[c] = bf16_rne(n32);
+                    lo.d16[c] = bf16_rne(d32);
+                }
+            }
+        };
         const double m = s.mask[c];
         if (m == OCEAN || m == LAND) {
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
             rhs[c] = dirichlet_head(P, m, s.zb[c]);
             if (cP) cP[c] = 1.0;
+            low(1.0, 0.0, 0.0, 1.0);
             return;
         }
         if (m == OTHER_BASIN) {
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
             rhs[c] = s.h[c];
             if (cP) cP[c] = 1.0;
+            low(1.0, 0.0, 0.0, 1.0);
             return;
         }
         const double Kc = s.K[c];
@@ -615,6 +638,7 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T
             if (iy < ny - 1 && (mN == OCEAN || mN == LAND)) c0 += fN;
             if (iy > 0 && (mS == OCEAN || mS == LAND)) c0 += fS;
             cP[c] = c0;
+            low(c0, (double)(T)cE, (double)(T)cN, (double)(T)diag);
         }
         double r;
         if (PARABOLIC)
@@ -2338,38 +2362,68 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
     typedef typename RawPair<CT>::type CR;
     static_assert(!UPD || sizeof(TB) == 8, "the fused update is FP64");
     if (sc->done) return;
-    double alpha = 0.0;
-    if constexpr (UPD) {
-        if (deferred_update_prologue(U.sc, U.pa, U.nba, U.pb, U.nbb, alpha)) return;
-    }
-    // b at cell pair k, with the fused update applied (r - alpha Ap, as the update computes it)
-    auto upd = [&](TB2 v, size_t k) -> TB2 {
-        if constexpr (UPD) {
-            const d2v a = *(const d2v*)(U.Ap + k);
-            v.x = v.x - alpha * a.x;
-            v.y = v.y - alpha * a.y;
-        }
-        return v;
-    };
     int strip, iy0;
     wide_tile(R, strip, iy0);  // iy0 even
     const int lane = threadIdx.x;
     const int ix = strip * RB_OWN - 2 + 2 * lane;
     const int iy1 = min(iy0 + R, ny);
-    if (iy0 >= ny || strip * RB_OWN >= nx) return;  // uniform over the wave; no barriers follow
     const bool own = lane >= 1 && lane <= WAVE - 2 && ix < nx;
     const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);  // clamped pair for the loads of lanes outside
     const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;         // faces west of ix, east of ix + 1
     const f2v z2 = {0.f, 0.f};
-    auto red_row = [&](int j, f2v& bj, f2v& dj) -> f2v {
+    auto rowk = [&](int j) -> size_t { return cidx(ixc, j < 0 ? 0 : (j < ny ? j : ny - 1), nx); };
+    // rows iy0-2 and iy0-1 (b, Ap, D) and n of row iy0-2: loaded (clamped into
+    // the grid, so in bounds for every wave) before the update's prologue,
+    // whose reductions they overlap; none of them needs alpha
+    TB2 bv0[2];
+    d2v av0[2];
+    CR dv0[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const size_t k = rowk(iy0 - 2 + q);
+        bv0[q] = *(const TB2*)(b + k);
+        if constexpr (UPD) av0[q] = *(const d2v*)(U.Ap + k);
+        dv0[q] = ldraw2(D, k);
+    }
+    const CR nv0 = ldraw2(n, rowk(iy0 - 2));
+    // step t's loads: b, D of row t+2; aE, aN of row t+1; c of row t
+    struct Ld {
+        TB2 b;
+        d2v ap;  // UPD: Ap of the same row
+        CR d, e, n, c;
+    };
+    auto issue = [&](int t, Ld& q) {
+        const size_t k2 = rowk(t + 2), k1 = rowk(t + 1);
+        q.b = *(const TB2*)(b + k2);
+        if constexpr (UPD) q.ap = *(const d2v*)(U.Ap + k2);
+        q.d = ldraw2(D, k2);
+        q.e = ldraw2(e, k1);
+        q.n = ldraw2(n, k1);
+        q.c = ldraw2(c, rowk(t));
+    };
+    // the first step's set goes out with them
+    Ld qa, qb;
+    issue(iy0 - 2, qa);
+    double alpha = 0.0;
+    if constexpr (UPD) {
+        if (deferred_update_prologue(U.sc, U.pa, U.nba, U.pb, U.nbb, alpha)) return;
+    }
+    if (iy0 >= ny || strip * RB_OWN >= nx) return;  // uniform over the wave; no barriers follow
+    // the red values of row iy0-2+q from its loaded b (with the fused update
+    // applied, r - alpha Ap as the update computes it) and D
+    auto red_row = [&](int q, f2v& bj, f2v& dj) -> f2v {
+        const int j = iy0 - 2 + q;
         f2v r = z2;
         bj = z2;
         dj = z2;
         if (j < 0 || j >= ny) return r;
-        const size_t k = cidx(ixc, j, nx);
-        const TB2 bv = upd(*(const TB2*)(b + k), k);
+        TB2 bv = bv0[q];
+        if constexpr (UPD) {
+            bv.x = bv.x - alpha * av0[q].x;
+            bv.y = bv.y - alpha * av0[q].y;
+        }
         bj = f2v{(float)bv.x, (float)bv.y};  // b is used in FP32 only
-        dj = ldc2(D, k);
+        dj = unpack2(dv0[q]);
         if ((j & 1) == 0) r.x = mg_div(omega * bj.x, dj.x);
         else r.y = mg_div(omega * bj.y, dj.y);
         return r;
@@ -2390,26 +2444,10 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         return v;
     };
     f2v b0, b1, d0, d1;
-    f2v r0 = red_row(iy0 - 2, b0, d0);
-    f2v r1 = red_row(iy0 - 1, b1, d1);
+    f2v r0 = red_row(0, b0, d0);
+    f2v r1 = red_row(1, b1, d1);
     f2v e0 = z2, n0 = z2, nm = z2, X0 = z2, Xm = z2;
-    if (iy0 >= 2) n0 = ldc2(n, cidx(ixc, iy0 - 2, nx));
-    auto rowk = [&](int j) -> size_t { return cidx(ixc, j < 0 ? 0 : (j < ny ? j : ny - 1), nx); };
-    // step t's loads: b, D of row t+2; aE, aN of row t+1; c of row t
-    struct Ld {
-        TB2 b;
-        d2v ap;  // UPD: Ap of the same row
-        CR d, e, n, c;
-    };
-    auto issue = [&](int t, Ld& q) {
-        const size_t k2 = rowk(t + 2), k1 = rowk(t + 1);
-        q.b = *(const TB2*)(b + k2);
-        if constexpr (UPD) q.ap = *(const d2v*)(U.Ap + k2);
-        q.d = ldraw2(D, k2);
-        q.e = ldraw2(e, k1);
-        q.n = ldraw2(n, k1);
-        q.c = ldraw2(c, rowk(t));
-    };
+    if (iy0 >= 2) n0 = unpack2(nv0);
     float hrow = 0.f;  // the even row's pair sum
     auto step = [&](int t, const Ld& q) {
         // red values of row t+2 (t+2 >= 0), as red_row
@@ -2473,8 +2511,6 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
     // (the last step's set loads rows that no step uses: clamped, in bounds,
     // issued unconditionally so that no branch around a load merges the wait
     // counts of the two paths)
-    Ld qa, qb;
-    issue(iy0 - 2, qa);
     for (int t = iy0 - 2; t < iy1; t += 2) {
         issue(t + 1, qb);
         step(t, qa);
@@ -4205,12 +4241,14 @@ void zero(double* p, size_t n, hipStream_t st) {
 
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
-              hipStream_t st, double* cP) {
+              hipStream_t st, double* cP, const AsmLow& lo) {
     const dim3 grid = cells_grid(G.nx, G.ny), blk(WAVE, WAVES);
+    AsmLow l = lo;
+    if (!cP) l = AsmLow{};  // the copies come from the flux form's c
     if (parabolic)
-        hipLaunchKernelGGL((k_assemble<true, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP);
+        hipLaunchKernelGGL((k_assemble<true, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP, l);
     else
-        hipLaunchKernelGGL((k_assemble<false, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP);
+        hipLaunchKernelGGL((k_assemble<false, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP, l);
 }
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st) {
@@ -4541,10 +4579,7 @@ __global__ __launch_bounds__(256) void k_to_bf16x4(size_t n, const float* __rest
                                                    uint16_t* __restrict__ o2, uint16_t* __restrict__ o3) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    auto rn = [](float v) -> uint16_t {
-        const uint32_t u = __float_as_uint(v);
-        return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-    };
+    auto rn = [](float v) -> uint16_t { return bf16_rne(v); };
     o0[i] = rn(a0[i]);
     o1[i] = rn(a1[i]);
     o2[i] = rn(a2[i]);
@@ -4813,7 +4848,7 @@ void cg_finish(int n, const double* x, const double* h_prev, double* h_new, doub
 
 #define SHK_INST(T)                                                                                               \
     template void assemble<T>(const Geom&, const Phys&, Fields, int, double, T*, T*, T*, T*, hipStream_t,         \
-                              double*);                                                                           \
+                              double*, const AsmLow&);                                                            \
     template void matvec<T>(const Geom&, const T*, const T*, const T*, const T*, T*, hipStream_t);                \
     template void lanczos_init<T>(const Geom&, const T*, const T*, int, T*, T*, double*, Scalars*, hipStream_t);  \
     template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, int, const T*, const T*, \

@@ -4,8 +4,9 @@
   implicit gap): the plain FP64 form against the oracle's direct-solve
   trajectory committed in tests/golden (tests/golden/make_config2_fixture.py).
 * 4096 x 1024 (bed amplitude 50 m): the precision form (FP32 V-cycle-PCG
-  inside FP64 refinement) and FP64 CG with the V-cycle against the plain FP64
-  form solved to 1e-13, where a banded direct solve would need ~34 GB.
+  inside FP64 refinement), FP64 CG with the V-cycle and the learned form (the
+  trained U-Net's warm start ahead of it) against the plain FP64 form solved
+  to 1e-13, where a banded direct solve would need ~34 GB.
 * 8192 x 2048 parabolic (e_v 1e-3, bed amplitude 100 m plus a trough): the
   same forms against the plain FP64 form.
 
@@ -112,14 +113,24 @@ def c3(hip):
     return gg, raw, moulin, out, picard, log
 
 
-@pytest.mark.parametrize("form", ["fp32_mg", "fp64_mg"])
+@pytest.mark.parametrize("form", ["fp32_mg", "fp64_mg", "learned_mg"])
 def test_config3_forms_against_fp64(hip, c3, form):
     """4096 x 1024, two steps (the first one the initial transient, 16 Picard
     iterations): the same Picard counts as the plain FP64 form and every field
-    within the stated bounds of it: h 1e-9, b 1e-8, the derived fields 1e-7."""
+    within the stated bounds of it: h 1e-9, b 1e-8, the derived fields 1e-7.
+    learned_mg is the evaluation of the learned form: the committed trained
+    U-Net's warm start ahead of FP64 CG with the V-cycle, at the size it is
+    evaluated at (it was trained at 512 x 128)."""
     gg, raw, moulin, ref, picard_ref, log_ref = c3
-    kw = (dict(multigrid=True, form="fp32_refine", rtol=1e-10, inner_rtol=1e-4) if form == "fp32_mg"
-          else dict(multigrid=True, form="fp64", rtol=1e-10))
+    if form == "fp32_mg":
+        kw = dict(multigrid=True, form="fp32_refine", rtol=1e-10, inner_rtol=1e-4)
+    elif form == "fp64_mg":
+        kw = dict(multigrid=True, form="fp64", rtol=1e-10)
+    else:
+        from shakti_hip import learned as LN
+        H8, W8 = LN.canvas_shape(gg.nx, gg.ny)
+        net = LN.UNet.load(LN.WEIGHTS_DIR / "unet_16_32_64_128.npz", 1, H8, W8)
+        kw = dict(multigrid=True, form="learned", network=net, rtol=1e-10)
     s, log, picard = _run(hip, gg, raw, moulin, 2, **kw)
     assert picard == picard_ref
     errs = {f: rel_err(s.get(f), ref[f]) for f in FIELDS}
