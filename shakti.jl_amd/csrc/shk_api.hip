@@ -133,8 +133,9 @@ struct shk_ctx {
     // still aggregating, 21 with every transfer linear.
     int mg_lin_from = 1;
     // smoothing sweeps each way on the levels below the finest that take the
-    // two-row legs with the linear transfers (1 or 2; SHK_MG_NU)
-    int mg_nu = 2;
+    // two-row legs with the linear transfers (1 or 2; SHK_MG_NU), and on the
+    // finest level where it takes the BF16 legs (SHK_MG_NU0)
+    int mg_nu = 2, mg_nu0 = 2;
     float mg_scale_lin = 0.8f, mg_scale_lin_par = 1.0f;
     // The finest level's c, aE, aN, D in BF16 for its red-black legs (where
     // those are the two-column kernels): half the coefficient bytes of the
@@ -509,6 +510,7 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w, bool fine_ready = false) {
         // such as n x 1 halve only once per level)
         if (const char* e = getenv("SHK_MG_LIN_FROM")) ctx->mg_lin_from = atoi(e);  // negative: none
         if (const char* e = getenv("SHK_MG_NU")) ctx->mg_nu = atoi(e) == 1 ? 1 : 2;
+        if (const char* e = getenv("SHK_MG_NU0")) ctx->mg_nu0 = atoi(e) == 1 ? 1 : 2;
         for (size_t l = 0; l + 1 < ctx->mg.size(); ++l)
             ctx->mg[l].lin = ctx->mg_rb && ctx->mg_lin_from >= 0 && (int)l >= ctx->mg_lin_from;
         auto tail_cells = [&](size_t f) {
@@ -631,7 +633,7 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
         if constexpr (sizeof(T) == 8) {
             if (l == 0 && upd) {
                 launch::mg_down_bf16_upd(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
-                                         ctx->mg_D16, r, f.xa, mg[1].b, *upd, ctx->st);
+                                         ctx->mg_D16, r, f.xa, mg[1].b, *upd, ctx->mg_nu0, ctx->st);
                 fused = true;
             }
         }
@@ -639,7 +641,7 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
             ;
         else if (l == 0 && ctx->mg_bf16)
             launch::mg_down_bf16<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
-                                    ctx->mg_D16, r, f.xa, mg[1].b, ctx->sc, ctx->st);
+                                    ctx->mg_D16, r, f.xa, mg[1].b, ctx->sc, ctx->mg_nu0, ctx->st);
         else if (l == 0)
             launch::mg_down<T>(f.nx, f.ny, mg[1].nx, mg[1].ny, ctx->mg_omega, f.c, f.e, f.n, f.D, r, f.xa, mg[1].b,
                                ctx->sc, ctx->mg_rb, ctx->st);
@@ -713,11 +715,11 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
         if (l == 0 && ctx->mg_bf16 && zf)
             launch::mg_up_bf16<T, float>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
                                          ctx->mg_n16, ctx->mg_D16, r0, f.xa, mg[1].xb, zf, mode, ctx->partials,
-                                         ctx->sc, ctx->st);
+                                         ctx->sc, ctx->mg_nu0, ctx->st);
         else if (l == 0 && ctx->mg_bf16)
             launch::mg_up_bf16<T, T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, ctx->mg_c16, ctx->mg_e16,
                                      ctx->mg_n16, ctx->mg_D16, r0, f.xa, mg[1].xb, z, mode, ctx->partials, ctx->sc,
-                                     ctx->st);
+                                     ctx->mg_nu0, ctx->st);
         else if (l == 0)
             launch::mg_up<T>(f.nx, f.ny, mg[1].nx, ctx->mg_omega, scale, f.c, f.e, f.n, f.D, r, f.xa,
                              mg[1].xb, z, mode, ctx->partials, ctx->sc, ctx->mg_rb, ctx->st);
@@ -848,7 +850,9 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     // take it (r then alternates between w.r and w.r2; batches are even)
     const bool fuse_upd = defer && zf32 && launch::mg_down_fuses_update(ctx->mg[0].nx, ctx->mg[0].ny) &&
                           getenv("SHK_NO_FUSED_UPDATE") == nullptr;
-    const int nba = multigrid ? launch::mg_up_nblocks(ctx->mg[0].nx, ctx->mg[0].ny, ctx->mg_rb) : 0;
+    const int nba = multigrid ? launch::mg_up_nblocks(ctx->mg[0].nx, ctx->mg[0].ny, ctx->mg_rb,
+                                                      ctx->mg_bf16 ? ctx->mg_nu0 : 1)
+                              : 0;
     auto enqueue_batch = [&]() {
         T* p_old = w.pA;
         T* p_new = w.pB;

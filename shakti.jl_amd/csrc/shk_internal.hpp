@@ -220,7 +220,7 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
                        int nba, double* x, hipStream_t st);
 void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
                         const double* pb, int nbb, hipStream_t st);
-int mg_up_nblocks(int nx, int ny, int rb);
+int mg_up_nblocks(int nx, int ny, int rb, int nu = 1);
 template <class T>
 void cg_update(const Geom& G, const T* dg, const T* p, const T* Ap, T* x, T* r, T* z, int jacobi, double* partials,
                Scalars* sc, hipStream_t st);
@@ -271,9 +271,11 @@ void mg_tail(const MGTail& T, const Scalars* sc, hipStream_t st);
 bool mg_bf16_ok(int nx, int ny);
 void to_bf16x4(size_t n, const float* a0, const float* a1, const float* a2, const float* a3, uint16_t* o0,
                uint16_t* o1, uint16_t* o2, uint16_t* o3, hipStream_t st);
+// The finest level's BF16 legs; nu = 2: two red-black sweeps each way
+// (k_mg_down_sw2 / k_mg_up_sw2), one otherwise.
 template <class TB>
 void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
-                  const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st);
+                  const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, int nu, hipStream_t st);
 // The CG update r_out = r - alpha Ap fused into the finest down-leg (FP64 CG
 // with the deferred finish): alpha from the pending p'Ap (pb, nbb) and r'z
 // (pa, nba) partials, stored to sc by the kernel as the update would.
@@ -290,11 +292,12 @@ struct CGUpd {
 // update (the BF16 marching leg at more than two rows per wave).
 bool mg_down_fuses_update(int nx, int ny);
 void mg_down_bf16_upd(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
-                      const uint16_t* D, const double* b, float* x, float* bc, const CGUpd& U, hipStream_t st);
+                      const uint16_t* D, const double* b, float* x, float* bc, const CGUpd& U, int nu,
+                      hipStream_t st);
 template <class TB, class TO>
 void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
                 const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TO* out,
-                int dot_mode, double* partials, Scalars* sc, hipStream_t st);
+                int dot_mode, double* partials, Scalars* sc, int nu, hipStream_t st);
 // dynamic LDS bytes of the tail kernel (rhs, x, c, e, n, D per cell and the
 // coarsest inverse), and the opt-in for more than the default 64 KB (0 = ok)
 size_t mg_tail_stride(size_t tail_cells);

@@ -2635,6 +2635,310 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
     }
 }
 
+// The finest level's legs with two red-black sweeps each way (two columns per
+// lane, marching; BF16 or FP32 coefficients; the fused CG update as in
+// k_mg_down_rb2).  A marching step t takes five stages on five rows, each
+// from the rows the stage before it left:
+//   down: red from zero on row t+4, black from red on t+3, red relaxed on
+//         t+2, black relaxed on t+1, the residual of row t;
+//   up:   u = x + scale P xc on row t+4, black relaxed on t+3, red on t+2,
+//         black on t+1, red on t (z).
+// Each stage changes one colour of its row and reads the other colour of the
+// rows beside it, so one register window of the current x serves all five.
+// A value formed by a half-sweep is wrong one column further in from the
+// wave's edge than what it reads: four half-sweeps and the residual need two
+// halo lanes a side (120 owned columns of the wave's 128).  Every cell is
+// formed with the arithmetic of the one-sweep legs in the same order; later
+// sweeps relax with rb_relax.
+constexpr int SW2_HALO = 2, SW2_OWN = 2 * (WAVE - 2 * SW2_HALO);
+
+// red (or black) cells of a row pair relaxed in place from the rows beside it
+__device__ __forceinline__ void sw2_relax(f2v& uc, f2v um, f2v up, int j, bool red_pass, int ny, float omega,
+                                          bool hW0, bool hE1, f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) {
+    const bool hN = j < ny - 1, hS = j > 0;
+    if (((j & 1) == 0) == red_pass) {
+        const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+        uc.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f, hW0 ? uW : 0.f, hN,
+                        hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f, hS ? um.x : 0.f);
+    } else {
+        const float uE = __shfl_down(uc.x, 1, WAVE);
+        uc.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x, uc.x, hN,
+                        hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
+    }
+}
+
+template <class TB, class CT, bool UPD = false>
+__global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, int R, float omega,
+                                                     const CT* __restrict__ c, const CT* __restrict__ e,
+                                                     const CT* __restrict__ n, const CT* __restrict__ D,
+                                                     const TB* __restrict__ b, float* __restrict__ x,
+                                                     float* __restrict__ bc, const Scalars* __restrict__ sc,
+                                                     launch::CGUpd U) {
+    typedef typename Pair<TB>::type TB2;
+    typedef typename RawPair<CT>::type CR;
+    static_assert(!UPD || sizeof(TB) == 8, "the fused update is FP64");
+    if (sc->done) return;
+    int strip, iy0;
+    wide_tile(R, strip, iy0);  // iy0 even
+    const int lane = threadIdx.x;
+    const int ix = strip * SW2_OWN - 2 * SW2_HALO + 2 * lane;
+    const int iy1 = min(iy0 + R, ny);
+    const bool own = lane >= SW2_HALO && lane <= WAVE - 1 - SW2_HALO && ix < nx;
+    const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);  // clamped pair for the loads of lanes outside
+    const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+    const f2v z2 = {0.f, 0.f};
+    auto rowk = [&](int j) -> size_t { return cidx(ixc, j < 0 ? 0 : (j < ny ? j : ny - 1), nx); };
+    auto in = [&](int j) { return j >= 0 && j < ny; };
+    // step t's loads: b (with Ap) and D of row t+4, e and n of row t+3, c of row t+2
+    struct Ld {
+        TB2 b;
+        d2v ap;  // UPD: Ap of the same row
+        CR d, e, n, c;
+    };
+    auto issue = [&](int t, Ld& q) {
+        const size_t k4 = rowk(t + 4), k3 = rowk(t + 3);
+        q.b = *(const TB2*)(b + k4);
+        if constexpr (UPD) q.ap = *(const d2v*)(U.Ap + k4);
+        q.d = ldraw2(D, k4);
+        q.e = ldraw2(e, k3);
+        q.n = ldraw2(n, k3);
+        q.c = ldraw2(c, rowk(t + 2));
+    };
+    const int t0 = iy0 - 8;  // the first row whose red cells row iy0's residual depends on is iy0 - 4
+    Ld qa, qb;
+    issue(t0, qa);  // in flight across the update's prologue
+    double alpha = 0.0;
+    if constexpr (UPD) {
+        if (deferred_update_prologue(U.sc, U.pa, U.nba, U.pb, U.nbb, alpha)) return;
+    }
+    if (iy0 >= ny || strip * SW2_OWN >= nx) return;  // uniform over the wave; no barriers follow
+    // windows, by offset from the step's row t: X rows t-1 .. t+4 (the current
+    // x), Bw rows t .. t+4, Dw t+1 .. t+4, Ew t .. t+3, Nw t-1 .. t+3, Cw t .. t+2
+    f2v X[6], Bw[5], Dw[4], Ew[4], Nw[5], Cw[3];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) X[k] = z2;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Bw[k] = Nw[k] = z2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Dw[k] = Ew[k] = z2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Cw[k] = z2;
+    float hrow = 0.f;  // the even row's pair sum
+    auto step = [&](int t, const Ld& q) {
+        const int j4 = t + 4, j3 = t + 3, j2 = t + 2, j1 = t + 1;
+        // row t+4: b (the fused update applied) and D, and its red cells from zero
+        f2v b4 = z2, d4 = z2, x4 = z2;
+        if (in(j4)) {
+            TB2 bb = q.b;
+            if constexpr (UPD) {
+                bb.x = bb.x - alpha * q.ap.x;
+                bb.y = bb.y - alpha * q.ap.y;
+                if (own && j4 >= iy0 && j4 < iy1) *(TB2*)(U.b_out + cidx(ixc, j4, nx)) = bb;
+            }
+            b4 = f2v{(float)bb.x, (float)bb.y};  // b is used in FP32 only
+            d4 = unpack2(q.d);
+            if ((j4 & 1) == 0) x4.x = mg_div(omega * b4.x, d4.x);
+            else x4.y = mg_div(omega * b4.y, d4.y);
+        }
+        X[5] = x4;
+        Bw[4] = b4;
+        Dw[3] = d4;
+        Ew[3] = in(j3) ? unpack2(q.e) : z2;
+        Nw[4] = in(j3) ? unpack2(q.n) : z2;
+        Cw[2] = in(j2) ? unpack2(q.c) : z2;
+        // row t+3: black cells from the red ones
+        if (in(j3)) {
+            const f2v rm = X[3], rc = X[4], rp = X[5], bj = Bw[3], dj = Dw[2], ej = Ew[3], nj = Nw[4], nm = Nw[3];
+            const bool hN = j3 < ny - 1, hS = j3 > 0;
+            if ((j3 & 1) == 0) {
+                const float xE = __shfl_down(rc.x, 1, WAVE);
+                X[4].y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f, hN ? nj.y : 0.f,
+                                   hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
+            } else {
+                const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                X[4].x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y, hN ? nj.x : 0.f,
+                                   hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
+            }
+        }
+        // row t+2: red cells relaxed; row t+1: black cells relaxed
+        if (in(j2)) sw2_relax(X[3], X[2], X[4], j2, true, ny, omega, hW0, hE1, Bw[2], Cw[2], Dw[1], Ew[2], Nw[3], Nw[2]);
+        if (in(j1)) sw2_relax(X[2], X[1], X[3], j1, false, ny, omega, hW0, hE1, Bw[1], Cw[1], Dw[0], Ew[1], Nw[2], Nw[1]);
+        // row t: the residual of the final x, restricted by 2 x 2 sums
+        if (t >= iy0) {
+            const size_t k = cidx(ixc, t, nx);
+            const f2v X0 = X[1], Xm = X[0], X1 = X[2], cc = Cw[0], e0 = Ew[0], n0 = Nw[1], nm = Nw[0], b0 = Bw[0];
+            const float ue = __shfl_down(X0.x, 1, WAVE);  // column ix + 2
+            const float uw = __shfl_up(X0.y, 1, WAVE);    // column ix - 1
+            const float ew = __shfl_up(e0.y, 1, WAVE);    // coupling (ix - 1, ix)
+            float a0 = cc.x * X0.x;
+            a0 += e0.x * (X0.x - X0.y);
+            if (hW0) a0 += ew * (X0.x - uw);
+            if (t < ny - 1) a0 += n0.x * (X0.x - X1.x);
+            if (t > 0) a0 += nm.x * (X0.x - Xm.x);
+            float a1 = cc.y * X0.y;
+            if (hE1) a1 += e0.y * (X0.y - ue);
+            a1 += e0.x * (X0.y - X0.x);
+            if (t < ny - 1) a1 += n0.y * (X0.y - X1.y);
+            if (t > 0) a1 += nm.y * (X0.y - Xm.y);
+            const float q0 = own ? b0.x - a0 : 0.f, q1 = own ? b0.y - a1 : 0.f;
+            if (own) *(f2v*)(x + k) = X0;
+            const float hs = q0 + q1;
+            if ((t & 1) == 0) {
+                hrow = hs;
+            } else if (own) {
+                bc[(size_t)(ix >> 1) + (size_t)(t >> 1) * nxc] = hrow + hs;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = X[k + 1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Bw[k] = Bw[k + 1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Dw[k] = Dw[k + 1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Ew[k] = Ew[k + 1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Nw[k] = Nw[k + 1];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) Cw[k] = Cw[k + 1];
+    };
+    // two load sets used alternately, so that a set's loads stay in flight
+    // across the loop's back edge (the last step's set loads rows that no
+    // step uses: clamped, in bounds)
+    for (int t = t0; t < iy1; t += 2) {
+        issue(t + 1, qb);
+        step(t, qa);
+        if (t + 1 >= iy1) break;
+        issue(t + 2, qa);
+        step(t + 1, qb);
+    }
+    // odd ny: the last (even) row has no partner row
+    if ((iy1 & 1) == 1 && own) bc[(size_t)(ix >> 1) + (size_t)((iy1 - 1) >> 1) * nxc] = hrow + 0.f;
+}
+
+template <class TB, class CT, class TO>
+__global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int R, float omega, float scale,
+                                                   const CT* __restrict__ c, const CT* __restrict__ e,
+                                                   const CT* __restrict__ n, const CT* __restrict__ D,
+                                                   const TB* __restrict__ b, const float* __restrict__ x,
+                                                   const float* __restrict__ xc, TO* __restrict__ out, int with_dot,
+                                                   double* partials, Scalars* __restrict__ sc) {
+    typedef typename Pair<TB>::type TB2;
+    if (sc->done) return;
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
+    const int lane = threadIdx.x;
+    const int ix = strip * SW2_OWN - 2 * SW2_HALO + 2 * lane;
+    const int iy1 = min(iy0 + R, ny);
+    double acc = 0.0;
+    if (iy0 < ny && strip * SW2_OWN < nx) {  // uniform over the wave (the partial below needs every wave)
+        const bool own = lane >= SW2_HALO && lane <= WAVE - 1 - SW2_HALO && ix < nx;
+        const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
+        const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
+        const f2v z2 = {0.f, 0.f};
+        auto in = [&](int j) { return j >= 0 && j < ny; };
+        // windows by offset from the step's row t: U rows t-1 .. t+4, the
+        // coefficients and b rows t .. t+3, n rows t-1 .. t+3
+        f2v Uw[6], Bw[4], Cw[4], Dw[4], Ew[4], Nw[5];
+        TB2 Braw[4];  // b in its own precision, for the dot product
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Uw[k] = z2;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            Bw[k] = Cw[k] = Dw[k] = Ew[k] = z2;
+            Braw[k].x = (TB)0;
+            Braw[k].y = (TB)0;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) Nw[k] = z2;
+        // step t's loads: x and xc of row t+4; b, c, D, e, n of row t+3 (rows
+        // clamped into the grid; rows outside it are zeroed where they are used)
+        typedef typename RawPair<CT>::type CR;
+        struct Ld {
+            f2v x;
+            float xc;
+            TB2 b;
+            CR c, d, e, n;
+        };
+        auto rowk = [&](int j) -> int { return j < 0 ? 0 : (j < ny ? j : ny - 1); };
+        auto issue = [&](int t, Ld& q) {
+            const int r4 = rowk(t + 4);
+            const size_t k3 = cidx(ixc, rowk(t + 3), nx);
+            q.x = *(const f2v*)(x + cidx(ixc, r4, nx));
+            q.xc = xc[(size_t)(ixc >> 1) + (size_t)(r4 >> 1) * nxc];
+            q.b = *(const TB2*)(b + k3);
+            q.c = ldraw2(c, k3);
+            q.d = ldraw2(D, k3);
+            q.e = ldraw2(e, k3);
+            q.n = ldraw2(n, k3);
+        };
+        auto step = [&](int t, const Ld& q) {
+            const int j4 = t + 4, j3 = t + 3, j2 = t + 2, j1 = t + 1;
+            Uw[5] = in(j4) ? f2v{q.x.x + scale * q.xc, q.x.y + scale * q.xc} : z2;
+            TB2 b3;
+            b3.x = (TB)0;
+            b3.y = (TB)0;
+            f2v c3 = z2, d3 = z2, e3 = z2, n3 = z2;
+            if (in(j3)) {
+                b3 = q.b;
+                c3 = unpack2(q.c);
+                d3 = unpack2(q.d);
+                e3 = unpack2(q.e);
+                n3 = unpack2(q.n);
+            }
+            Braw[3] = b3;
+            Bw[3] = f2v{(float)b3.x, (float)b3.y};
+            Cw[3] = c3;
+            Dw[3] = d3;
+            Ew[3] = e3;
+            Nw[4] = n3;
+            // black on t+3 (from u), red on t+2, black on t+1, red on t
+            if (in(j3)) sw2_relax(Uw[4], Uw[3], Uw[5], j3, false, ny, omega, hW0, hE1, Bw[3], Cw[3], Dw[3], Ew[3], Nw[4], Nw[3]);
+            if (in(j2)) sw2_relax(Uw[3], Uw[2], Uw[4], j2, true, ny, omega, hW0, hE1, Bw[2], Cw[2], Dw[2], Ew[2], Nw[3], Nw[2]);
+            if (in(j1)) sw2_relax(Uw[2], Uw[1], Uw[3], j1, false, ny, omega, hW0, hE1, Bw[1], Cw[1], Dw[1], Ew[1], Nw[2], Nw[1]);
+            if (t >= iy0) {
+                sw2_relax(Uw[1], Uw[0], Uw[2], t, true, ny, omega, hW0, hE1, Bw[0], Cw[0], Dw[0], Ew[0], Nw[1], Nw[0]);
+                if (own) {
+                    const f2v z = Uw[1];
+                    typename Pair<TO>::type zv;
+                    zv.x = (TO)z.x;
+                    zv.y = (TO)z.y;
+                    *(typename Pair<TO>::type*)(out + cidx(ix, t, nx)) = zv;
+                    acc += (double)Braw[0].x * (double)z.x;
+                    acc += (double)Braw[0].y * (double)z.y;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) Uw[k] = Uw[k + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Bw[k] = Bw[k + 1];
+                Braw[k] = Braw[k + 1];
+                Cw[k] = Cw[k + 1];
+                Dw[k] = Dw[k + 1];
+                Ew[k] = Ew[k + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Nw[k] = Nw[k + 1];
+        };
+        // two load sets used alternately, a step ahead (as in the down-leg)
+        Ld qa, qb;
+        issue(iy0 - 8, qa);
+        for (int t = iy0 - 8; t < iy1; t += 2) {
+            issue(t + 1, qb);
+            step(t, qa);
+            if (t + 1 >= iy1) break;
+            issue(t + 2, qa);
+            step(t + 1, qb);
+        }
+    }
+    if (with_dot) {
+        double v[1] = {acc};
+        block_partial<1, false>(v, partials);
+        if (with_dot == 2 && lead_thread()) sc->gpend = 1;  // the next CG matvec finishes r'z
+    }
+}
+
 // The same two legs for the levels where a wave takes two rows (R = 2):
 // these levels are latency-bound, so every load of the wave's rows (row
 // indices clamped into the grid, rows outside it zeroed after the load) is
@@ -4515,8 +4819,27 @@ static Geom mg_rb_geom(int nx, int ny, bool lin = false, int own_cols = 0) {
 // Red-black smoothing: the two-column marching legs on levels of even width
 // above MG_FLAT_CELLS cells, the flat legs elsewhere (small or odd-width levels).
 static bool mg_rb_flat(int nx, int ny) { return (long)nx * ny <= mg_flat_cells() || nx % 2 != 0; }
+// The two-sweep finest legs: 120 owned columns per wave and R rows (even, 2
+// to 16) for about 2048 waves, half mg_rb_geom's count: a wave's first eight
+// marching steps only fill its five-stage pipeline, so longer waves pay off
+// (4096 x 1024, fp64_mg s/step at R = 8 / 12 / 16 / 24 / 32: 0.0382 / 0.0367
+// / 0.0360 / 0.0364 / 0.0383).  SHK_MG_SW2_R overrides R.
+static Geom sw2_geom(int nx, int ny) {
+    Geom W = mg_rb_geom(nx, ny, false, SW2_OWN);
+    static const int r_env = [] {
+        const char* e = getenv("SHK_MG_SW2_R");
+        return e ? atoi(e) : 0;
+    }();
+    const long strips = (nx + SW2_OWN - 1) / SW2_OWN;
+    const long r = ((long)ny * strips) / 2048;
+    W.R = r_env >= 2 ? r_env + (r_env & 1) : (int)(r < 2 ? 2 : (r > 16 ? 16 : r + (r & 1)));
+    W.grid.y = (unsigned)((ny + W.R - 1) / W.R);
+    W.nblocks = W.grid.x * W.grid.y;
+    return W;
+}
 // Workgroups (and so r'z partials) of the up-leg on an nx x ny level.
-int mg_up_nblocks(int nx, int ny, int rb) {
+int mg_up_nblocks(int nx, int ny, int rb, int nu) {
+    if (rb && nu == 2 && !mg_rb_flat(nx, ny)) return sw2_geom(nx, ny).nblocks;
     if (rb) return mg_rb_flat(nx, ny) ? (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y) : mg_rb_geom(nx, ny).nblocks;
     if ((long)nx * ny <= mg_flat_cells()) return (int)(cells_grid(nx, ny).x * cells_grid(nx, ny).y);
     return mg_wide(nx) ? mg_wide_geom(nx, ny).nblocks : mg_march_geom(nx, ny).nblocks;
@@ -4525,7 +4848,13 @@ int mg_up_nblocks(int nx, int ny, int rb) {
 // width that take the two-column legs; see mg_bf16_ok).
 template <class TB>
 void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
-                  const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, hipStream_t st) {
+                  const uint16_t* D, const TB* b, float* x, float* bc, const Scalars* sc, int nu, hipStream_t st) {
+    if (nu == 2) {
+        const Geom W = sw2_geom(nx, ny);
+        hipLaunchKernelGGL((k_mg_down_sw2<TB, uint16_t>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c, e, n,
+                           D, b, x, bc, sc, CGUpd{});
+        return;
+    }
     const Geom W = mg_rb_geom(nx, ny);
     if (W.R == 2)
         hipLaunchKernelGGL((k_mg_down_rb2s<TB, uint16_t>), W.grid, W.block, 0, st, nx, ny, nxc, omega, c, e, n, D, b,
@@ -4536,7 +4865,14 @@ void mg_down_bf16(int nx, int ny, int nxc, float omega, const uint16_t* c, const
 }
 bool mg_down_fuses_update(int nx, int ny) { return mg_rb_geom(nx, ny).R > 2; }
 void mg_down_bf16_upd(int nx, int ny, int nxc, float omega, const uint16_t* c, const uint16_t* e, const uint16_t* n,
-                      const uint16_t* D, const double* b, float* x, float* bc, const CGUpd& U, hipStream_t st) {
+                      const uint16_t* D, const double* b, float* x, float* bc, const CGUpd& U, int nu,
+                      hipStream_t st) {
+    if (nu == 2) {
+        const Geom W = sw2_geom(nx, ny);
+        hipLaunchKernelGGL((k_mg_down_sw2<double, uint16_t, true>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega,
+                           c, e, n, D, b, x, bc, U.sc, U);
+        return;
+    }
     const Geom W = mg_rb_geom(nx, ny);  // R > 2 (mg_down_fuses_update)
     hipLaunchKernelGGL((k_mg_down_rb2<double, uint16_t, true>), W.grid, W.block, 0, st, nx, ny, nxc, W.R, omega, c,
                        e, n, D, b, x, bc, U.sc, U);
@@ -4544,10 +4880,13 @@ void mg_down_bf16_upd(int nx, int ny, int nxc, float omega, const uint16_t* c, c
 template <class TB, class TO>
 void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_t* c, const uint16_t* e,
                 const uint16_t* n, const uint16_t* D, const TB* b, const float* x, const float* xc, TO* out,
-                int dot_mode, double* partials, Scalars* sc, hipStream_t st) {
-    const Geom M = mg_rb_geom(nx, ny);
+                int dot_mode, double* partials, Scalars* sc, int nu, hipStream_t st) {
+    const Geom M = nu == 2 ? sw2_geom(nx, ny) : mg_rb_geom(nx, ny);
     const int with_dot = dot_mode == 2 ? 2 : (dot_mode >= 0 ? 1 : 0);
-    if (M.R == 2)
+    if (nu == 2)
+        hipLaunchKernelGGL((k_mg_up_sw2<TB, uint16_t, TO>), M.grid, M.block, 0, st, nx, ny, nxc, M.R, omega, scale, c,
+                           e, n, D, b, x, xc, out, with_dot, partials, sc);
+    else if (M.R == 2)
         hipLaunchKernelGGL((k_mg_up_rb2s<TB, uint16_t, TO>), M.grid, M.block, 0, st, nx, ny, nxc, omega, scale, c, e, n,
                            D, b, x, xc, out, with_dot, partials, sc);
     else
@@ -4558,18 +4897,18 @@ void mg_up_bf16(int nx, int ny, int nxc, float omega, float scale, const uint16_
                            dot_mode == 0 ? (int)FIN_GAMMA_INIT : (int)FIN_GAMMA_ITER, 0, sc);
 }
 template void mg_down_bf16<float>(int, int, int, float, const uint16_t*, const uint16_t*, const uint16_t*,
-                                  const uint16_t*, const float*, float*, float*, const Scalars*, hipStream_t);
+                                  const uint16_t*, const float*, float*, float*, const Scalars*, int, hipStream_t);
 template void mg_down_bf16<double>(int, int, int, float, const uint16_t*, const uint16_t*, const uint16_t*,
-                                   const uint16_t*, const double*, float*, float*, const Scalars*, hipStream_t);
+                                   const uint16_t*, const double*, float*, float*, const Scalars*, int, hipStream_t);
 template void mg_up_bf16<float, float>(int, int, int, float, float, const uint16_t*, const uint16_t*,
                                        const uint16_t*, const uint16_t*, const float*, const float*, const float*,
-                                       float*, int, double*, Scalars*, hipStream_t);
+                                       float*, int, double*, Scalars*, int, hipStream_t);
 template void mg_up_bf16<double, double>(int, int, int, float, float, const uint16_t*, const uint16_t*,
                                          const uint16_t*, const uint16_t*, const double*, const float*, const float*,
-                                         double*, int, double*, Scalars*, hipStream_t);
+                                         double*, int, double*, Scalars*, int, hipStream_t);
 template void mg_up_bf16<double, float>(int, int, int, float, float, const uint16_t*, const uint16_t*,
                                         const uint16_t*, const uint16_t*, const double*, const float*, const float*,
-                                        float*, int, double*, Scalars*, hipStream_t);
+                                        float*, int, double*, Scalars*, int, hipStream_t);
 // Levels whose red-black legs are the two-column kernels (and so can read BF16 coefficients).
 bool mg_bf16_ok(int nx, int ny) { return !mg_rb_flat(nx, ny); }
 // FP32 -> BF16, round to nearest even, four arrays of n values

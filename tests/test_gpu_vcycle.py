@@ -39,8 +39,8 @@ def test_vcycle_matches_restatement(hip, shape):
     # the library's default smoothing weight and coarse scales (linear transfers
     # below the finest level); BF16 finest-level coefficients where the finest
     # level takes the two-column legs
-    M = VCycle(aP, aE, aN, bf16_level0=(nx % 2 == 0 and nx * ny > 4096), bil_from=1, scale_bil=0.8,
-               nu2=library_nu2)
+    two_col = nx % 2 == 0 and nx * ny > 4096  # the finest level takes the BF16 two-column legs
+    M = VCycle(aP, aE, aN, bf16_level0=two_col, bil_from=1, scale_bil=0.8, nu2=library_nu2, nu2_fine=two_col)
     rng = np.random.default_rng(7)
     rs = [np.asfortranarray(rng.standard_normal((nx, ny))) for _ in range(2)]
     zs = []
@@ -75,7 +75,7 @@ def test_vcycle_parabolic_system(hip):
     arrs = [np.zeros((nx, ny), order="F") for _ in range(6)]
     L.check(lib.shk_download_system(hs.ctx, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]), hs.ctx)
     M = VCycle(arrs[0], arrs[1], arrs[3], omega=1.2, scale=1.0, bf16_level0=True, bil_from=1, scale_bil=1.0,
-               nu2=library_nu2)
+               nu2=library_nu2, nu2_fine=True)
     r = np.asfortranarray(np.random.default_rng(8).standard_normal((nx, ny)))
     z = np.zeros((nx, ny), order="F")
     L.check(lib.shk_apply_precond(hs.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),

@@ -80,10 +80,11 @@ class _JacobiCycle(VCycle):
         return u + self.omega * (b - flux_apply(c, e, n, u)) / D
 
 
-@pytest.mark.parametrize("bil_from,nu2", [(None, False), (1, False), (0, False), (1, True)])
-def test_vcycle_is_symmetric_positive_definite(system, bil_from, nu2):
+@pytest.mark.parametrize("bil_from,nu2,fine", [(None, False, False), (1, False, False), (0, False, False),
+                                               (1, True, False), (1, True, True)])
+def test_vcycle_is_symmetric_positive_definite(system, bil_from, nu2, fine):
     co = system
-    M = VCycle(co[0], co[1], co[3], bil_from=bil_from, nu2=(lambda nx, ny: True) if nu2 else None)
+    M = VCycle(co[0], co[1], co[3], bil_from=bil_from, nu2=(lambda nx, ny: True) if nu2 else None, nu2_fine=fine)
     rng = np.random.default_rng(0)
     rs = [np.asfortranarray(rng.standard_normal(co[0].shape)) for _ in range(3)]
     zs = [M(r) for r in rs]
@@ -237,15 +238,18 @@ def test_linear_transfers_cut_cg_iterations(oracle, system):
     assert lin1 < 0.85 * agg and lin0 < 0.85 * agg, (agg, lin1, lin0)
 
 
-def test_two_sweeps_below_the_finest_level_cut_cg_iterations(oracle, system):
+def test_two_sweeps_cut_cg_iterations(oracle, system):
     """Two red-black sweeps each way on the levels below the finest (the
-    library's choice where those levels take the two-row legs): fewer PCG
-    iterations than one sweep, on the same hierarchy."""
+    library's choice where those levels take the two-row legs), and on the
+    finest level too (where it takes the BF16 two-column legs): fewer PCG
+    iterations than one sweep, on the same hierarchy, and fewer again."""
     co = system
     one = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=1))
     two = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=1, nu2=lambda nx, ny: True))
-    print(f"PCG iterations: one sweep each way {one}, two below the finest level {two}")
-    assert two < one, (one, two)
+    every = _pcg(oracle, co, VCycle(co[0], co[1], co[3], bf16_level0=True, bil_from=1, nu2=lambda nx, ny: True,
+                                    nu2_fine=True))
+    print(f"PCG iterations: one sweep each way {one}, two below the finest level {two}, two on every level {every}")
+    assert every < two < one, (one, two, every)
 
 
 def test_library_rule_for_two_sweeps():

@@ -30,6 +30,8 @@ from a reference file:
   (faces beyond the grid taken as zero; wy the restriction weights in y) and
   nothing on the y-faces; y-faces likewise.
 
+* nu2_fine: the finest level takes two sweeps each way too (the library's
+  choice where the finest level takes the BF16 two-column legs);
 * nu2: levels (below the finest, reached by the linear transfers) that take
   two sweeps each way: down-leg red, black, red, black (the first red cells
   from zero, every later half-sweep x += omega (b - A x) / D on its colour),
@@ -203,7 +205,7 @@ def library_nu2(nx, ny):
 
 class VCycle:
     def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False, bil_from=None, scale_bil=0.8,
-                 nu2=None):
+                 nu2=None, nu2_fine=False):
         self.omega, self.scale, self.scale_bil = F32(omega), F32(scale), F32(scale_bil)
         c, e, n, D = fine_level(aP, aE, aN)
         self.levels = [(c, e, n, D)]
@@ -226,6 +228,8 @@ class VCycle:
         # shape) says so on a level below the finest with the linear transfers
         self.nu = [2 if (nu2 is not None and 0 < l < len(self.lin) and self.lin[l] and nu2(*L[0].shape)) else 1
                    for l, L in enumerate(self.levels)]
+        if nu2_fine and len(self.levels) > 1:
+            self.nu[0] = 2
 
     def _cycle(self, b, l):
         c, e, n, D = self.levels[l]
