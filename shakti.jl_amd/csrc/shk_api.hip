@@ -115,15 +115,19 @@ struct shk_ctx {
     // 0.95, scale 1.5; SHK_MG_JACOBI=1) at the same memory traffic per
     // V-cycle.  The piecewise-constant coarse correction is scaled up to make
     // up for the energy it misses, which depends on how diagonally dominant
-    // the system is: 1.8 for the elliptic systems (the reaction term orders
-    // of magnitude below the conductances), 1.0 (no over-correction) for the
-    // backward-Euler parabolic ones, where the storage term dominates (on
-    // the 8192 x 2048 workload, CG iterations per solve for scale 0.8 / 0.9
-    // / 1.0 / 1.1 / 1.2 / 1.8: 50 / 42 / 37 / 41 / 49 / 150).  The symmetric
-    // V-cycle (red-black down, black-red up) stays positive definite for
-    // omega < 2 and scale < 2.
+    // the system is: 1.9 for the elliptic systems (the reaction term orders
+    // of magnitude below the conductances; with two sweeps on every level,
+    // 4096 x 1024 CG iterations per step at 1.6 / 1.8 / 1.9 / 1.95 / 2.0:
+    // 124 / 116 / 112 / 111 / 110, the ragged test systems within one
+    // iteration), 1.0 (no over-correction) for the backward-Euler parabolic
+    // ones, where the storage term dominates (on the 8192 x 2048 workload,
+    // CG iterations per solve for scale 0.8 / 0.9 / 1.0 / 1.1 / 1.2 / 1.8:
+    // 50 / 42 / 37 / 41 / 49 / 150).  The symmetric V-cycle (red-black down,
+    // black-red up) stays positive definite for omega < 2 and scale < 2
+    // (with an exact coarse solve the correction I - s pi, pi a projection,
+    // is a contraction exactly for 0 < s < 2).
     int mg_rb = 1;
-    float mg_omega = 1.2f, mg_scale = 1.8f, mg_scale_par = 1.0f;
+    float mg_omega = 1.2f, mg_scale = 1.9f, mg_scale_par = 1.0f;
     // Transfers from level mg_lin_from down (red-black smoothing only): the
     // cell-centred linear ones, R = P^T, with the lumped Galerkin coarse
     // operator, in place of the 2 x 2 aggregation; their coarse correction

@@ -204,7 +204,7 @@ def library_nu2(nx, ny):
 
 
 class VCycle:
-    def __init__(self, aP, aE, aN, omega=1.2, scale=1.8, bf16_level0=False, bil_from=None, scale_bil=0.8,
+    def __init__(self, aP, aE, aN, omega=1.2, scale=1.9, bf16_level0=False, bil_from=None, scale_bil=0.8,
                  nu2=None, nu2_fine=False):
         self.omega, self.scale, self.scale_bil = F32(omega), F32(scale), F32(scale_bil)
         c, e, n, D = fine_level(aP, aE, aN)
