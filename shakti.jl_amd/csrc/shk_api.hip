@@ -1274,11 +1274,15 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
         if (ctx->prof) prof_collect(ctx);
         return conclude(ctx, sst, stats);
     }
+    // the FP64 residual of the head about to be returned, x + x0; in the usual
+    // case (warm start from h) the same kernel writes that head, under-relaxed,
+    // into the alternate buffer, which becomes h only when the residual is finite
+    double* h_prev = ctx->F.f[SHK_H];
+    const bool fused_finish = x0 == h_prev && !(ctx->rec_kind == 1 && ctx->rec_x);
     {
-        // the FP64 residual of the head about to be returned, x + x0
-        Prof pr(ctx, P_FIN, 6 * nb);
+        Prof pr(ctx, P_FIN, (fused_finish ? 7 : 6) * nb);
         launch::residual_norms(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, x0, ctx->partials, &ctx->sc->tres,
-                               ctx->st);
+                               ctx->st, fused_finish ? ctx->h_alt : nullptr, relax);
     }
     rc = read_scalars(ctx);
     if (rc) return rc;
@@ -1299,9 +1303,9 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
                                  ctx->rec_scale + k, ctx->rec_t + (size_t)k * img, ctx->st);
         HIPCHK(hipStreamSynchronize(ctx->st));
     }
-    // h_new = x + x0 into the alternate buffer, then swap so F.f[SHK_H] is current
-    double* h_prev = ctx->F.f[SHK_H];
-    {
+    // h_new = x + x0 into the alternate buffer (done above when fused_finish),
+    // then swap so F.f[SHK_H] is current
+    if (!fused_finish) {
         Prof pr(ctx, P_FIN, 3 * nb);
         if (x0 == h_prev) {
             launch::cg_finish((int)ctx->n, ctx->x, h_prev, ctx->h_alt, relax, ctx->st);

@@ -191,8 +191,11 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
                 hipStream_t st);
 // ||b - A h||_2 and ||b||_2 of the FP64 system for h = x + x0 (x0 may be
 // null) into out[0], out[1] (device memory); partials: 2 x G.nblocks doubles
+// ||b - A(x + x0)||^2 and ||b||^2 into out[0..1] (x0 may be null); with
+// h_out, also h_out = x + x0 (under-relaxed towards x0 when relax > 0)
 void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
-                    const double* x, const double* x0, double* partials, double* out, hipStream_t st);
+                    const double* x, const double* x0, double* partials, double* out, hipStream_t st,
+                    double* h_out = nullptr, double relax = 0.0);
 
 // Preconditioned CG pieces (mode 0: set-up, mode 1: iteration).
 template <class T>

@@ -1725,19 +1725,24 @@ __global__ __launch_bounds__(256) void k_gersh(int nx, int ny, const T* __restri
 
 // The FP64 residual of a returned head h = x + x0 (x0 may be null), formed
 // as k_cg_finish forms it: partials of ||b - A h||^2 and ||b||^2.
+// h_out: where given, also the returned head h = x + x0 of every cell (the
+// operand at the cell), under-relaxed towards x0 when relax > 0 (as
+// k_cg_finish writes it)
 __global__ __launch_bounds__(256) void k_true_resid(int nx, int ny, int R, const double* __restrict__ aP,
                                                     const double* __restrict__ aE, const double* __restrict__ aN,
                                                     const double* __restrict__ b, const double* __restrict__ x,
-                                                    const double* __restrict__ x0, double* partials) {
+                                                    const double* __restrict__ x0, double* partials,
+                                                    double* __restrict__ h_out, double relax) {
     int vbx, vby;
     virtual_block(vbx, vby);
     double ar = 0.0, ab = 0.0;
     march<double>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> double { return x0 ? x[k] + x0[k] : x[k]; },
-                  [&](size_t c, double Au, double) {
+                  [&](size_t c, double Au, double u) {
                       const double bc = b[c];
                       const double r = bc - Au;
                       ar += r * r;
                       ab += bc * bc;
+                      if (h_out) h_out[c] = relax > 0.0 ? relax * u + (1.0 - relax) * x0[c] : u;
                   });
     double v[2] = {ar, ab};
     block_partial<2, false>(v, partials);
@@ -4613,8 +4618,10 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
                        (int)FIN_GERSH, 0, sc);
 }
 void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
-                    const double* x, const double* x0, double* partials, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials);
+                    const double* x, const double* x0, double* partials, double* out, hipStream_t st, double* h_out,
+                    double relax) {
+    hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials, h_out,
+                       relax);
     hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(WAVE, WAVES), 0, st, partials, G.nblocks, out);
 }
 template <class T>
