@@ -2656,6 +2656,11 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
 // formed with the arithmetic of the one-sweep legs in the same order; later
 // sweeps relax with rb_relax.
 constexpr int SW2_HALO = 2, SW2_OWN = 2 * (WAVE - 2 * SW2_HALO);
+// a marching step's phase in a loop unrolled six times, as a type
+template <int P>
+struct Phase {
+    static constexpr int value = P;
+};
 
 // red (or black) cells of a row pair relaxed in place from the rows beside it
 __device__ __forceinline__ void sw2_relax(f2v& uc, f2v um, f2v up, int j, bool red_pass, int ny, float omega,
@@ -2717,19 +2722,19 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
         if (deferred_update_prologue(U.sc, U.pa, U.nba, U.pb, U.nbb, alpha)) return;
     }
     if (iy0 >= ny || strip * SW2_OWN >= nx) return;  // uniform over the wave; no barriers follow
-    // windows, by offset from the step's row t: X rows t-1 .. t+4 (the current
-    // x), Bw rows t .. t+4, Dw t+1 .. t+4, Ew t .. t+3, Nw t-1 .. t+3, Cw t .. t+2
-    f2v X[6], Bw[5], Dw[4], Ew[4], Nw[5], Cw[3];
+    // six-row ring windows (x and every coefficient): at marching step t, row
+    // t + o (o = -1 .. 4) sits in slot (P + 1 + o) % 6, P = (t - t0) % 6, so a
+    // step names its rows by fixed slots and nothing moves between steps; the
+    // loop below is unrolled six times.  A row enters at o = 4 (b, D, x), 3
+    // (e, n) or 2 (c) and is last read at o = -1 (x, n) or o = 0.
+    f2v X[6], Bw[6], Dw[6], Ew[6], Nw[6], Cw[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) X[k] = z2;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Bw[k] = Nw[k] = z2;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) Dw[k] = Ew[k] = z2;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) Cw[k] = z2;
+    for (int k = 0; k < 6; ++k) X[k] = Bw[k] = Dw[k] = Ew[k] = Nw[k] = Cw[k] = z2;
     float hrow = 0.f;  // the even row's pair sum
-    auto step = [&](int t, const Ld& q) {
+    auto step = [&](auto ph, int t, const Ld& q) {
+        constexpr int P = decltype(ph)::value;
+        constexpr int om = P % 6, o0 = (P + 1) % 6, o1 = (P + 2) % 6, o2 = (P + 3) % 6, o3 = (P + 4) % 6,
+                      o4 = (P + 5) % 6;
         const int j4 = t + 4, j3 = t + 3, j2 = t + 2, j1 = t + 1;
         // row t+4: b (the fused update applied) and D, and its red cells from zero
         f2v b4 = z2, d4 = z2, x4 = z2;
@@ -2745,33 +2750,39 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
             if ((j4 & 1) == 0) x4.x = mg_div(omega * b4.x, d4.x);
             else x4.y = mg_div(omega * b4.y, d4.y);
         }
-        X[5] = x4;
-        Bw[4] = b4;
-        Dw[3] = d4;
-        Ew[3] = in(j3) ? unpack2(q.e) : z2;
-        Nw[4] = in(j3) ? unpack2(q.n) : z2;
-        Cw[2] = in(j2) ? unpack2(q.c) : z2;
+        X[o4] = x4;
+        Bw[o4] = b4;
+        Dw[o4] = d4;
+        Ew[o3] = in(j3) ? unpack2(q.e) : z2;
+        Nw[o3] = in(j3) ? unpack2(q.n) : z2;
+        Cw[o2] = in(j2) ? unpack2(q.c) : z2;
         // row t+3: black cells from the red ones
         if (in(j3)) {
-            const f2v rm = X[3], rc = X[4], rp = X[5], bj = Bw[3], dj = Dw[2], ej = Ew[3], nj = Nw[4], nm = Nw[3];
+            const f2v rm = X[o2], rc = X[o3], rp = X[o4], bj = Bw[o3], dj = Dw[o3], ej = Ew[o3], nj = Nw[o3],
+                      nm = Nw[o2];
             const bool hN = j3 < ny - 1, hS = j3 > 0;
             if ((j3 & 1) == 0) {
                 const float xE = __shfl_down(rc.x, 1, WAVE);
-                X[4].y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f, hN ? nj.y : 0.f,
-                                   hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
+                X[o3].y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
+                                    hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
             } else {
                 const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
-                X[4].x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y, hN ? nj.x : 0.f,
-                                   hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
+                X[o3].x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
+                                    hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
             }
         }
         // row t+2: red cells relaxed; row t+1: black cells relaxed
-        if (in(j2)) sw2_relax(X[3], X[2], X[4], j2, true, ny, omega, hW0, hE1, Bw[2], Cw[2], Dw[1], Ew[2], Nw[3], Nw[2]);
-        if (in(j1)) sw2_relax(X[2], X[1], X[3], j1, false, ny, omega, hW0, hE1, Bw[1], Cw[1], Dw[0], Ew[1], Nw[2], Nw[1]);
+        if (in(j2))
+            sw2_relax(X[o2], X[o1], X[o3], j2, true, ny, omega, hW0, hE1, Bw[o2], Cw[o2], Dw[o2], Ew[o2], Nw[o2],
+                      Nw[o1]);
+        if (in(j1))
+            sw2_relax(X[o1], X[o0], X[o2], j1, false, ny, omega, hW0, hE1, Bw[o1], Cw[o1], Dw[o1], Ew[o1], Nw[o1],
+                      Nw[o0]);
         // row t: the residual of the final x, restricted by 2 x 2 sums
         if (t >= iy0) {
             const size_t k = cidx(ixc, t, nx);
-            const f2v X0 = X[1], Xm = X[0], X1 = X[2], cc = Cw[0], e0 = Ew[0], n0 = Nw[1], nm = Nw[0], b0 = Bw[0];
+            const f2v X0 = X[o0], Xm = X[om], X1 = X[o1], cc = Cw[o0], e0 = Ew[o0], n0 = Nw[o0], nm = Nw[om],
+                      b0 = Bw[o0];
             const float ue = __shfl_down(X0.x, 1, WAVE);  // column ix + 2
             const float uw = __shfl_up(X0.y, 1, WAVE);    // column ix - 1
             const float ew = __shfl_up(e0.y, 1, WAVE);    // coupling (ix - 1, ix)
@@ -2794,28 +2805,28 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
                 bc[(size_t)(ix >> 1) + (size_t)(t >> 1) * nxc] = hrow + hs;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 5; ++k) X[k] = X[k + 1];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Bw[k] = Bw[k + 1];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) Dw[k] = Dw[k + 1];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) Ew[k] = Ew[k + 1];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Nw[k] = Nw[k + 1];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) Cw[k] = Cw[k + 1];
     };
     // two load sets used alternately, so that a set's loads stay in flight
-    // across the loop's back edge (the last step's set loads rows that no
-    // step uses: clamped, in bounds)
-    for (int t = t0; t < iy1; t += 2) {
+    // across the step before it (the last step's set loads rows that no step
+    // uses: clamped, in bounds)
+    for (int t = t0; t < iy1; t += 6) {
         issue(t + 1, qb);
-        step(t, qa);
+        step(Phase<0>{}, t, qa);
         if (t + 1 >= iy1) break;
         issue(t + 2, qa);
-        step(t + 1, qb);
+        step(Phase<1>{}, t + 1, qb);
+        if (t + 2 >= iy1) break;
+        issue(t + 3, qb);
+        step(Phase<2>{}, t + 2, qa);
+        if (t + 3 >= iy1) break;
+        issue(t + 4, qa);
+        step(Phase<3>{}, t + 3, qb);
+        if (t + 4 >= iy1) break;
+        issue(t + 5, qb);
+        step(Phase<4>{}, t + 4, qa);
+        if (t + 5 >= iy1) break;
+        issue(t + 6, qa);
+        step(Phase<5>{}, t + 5, qb);
     }
     // odd ny: the last (even) row has no partner row
     if ((iy1 & 1) == 1 && own) bc[(size_t)(ix >> 1) + (size_t)((iy1 - 1) >> 1) * nxc] = hrow + 0.f;
@@ -2842,20 +2853,17 @@ __global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int 
         const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
         const f2v z2 = {0.f, 0.f};
         auto in = [&](int j) { return j >= 0 && j < ny; };
-        // windows by offset from the step's row t: U rows t-1 .. t+4, the
-        // coefficients and b rows t .. t+3, n rows t-1 .. t+3
-        f2v Uw[6], Bw[4], Cw[4], Dw[4], Ew[4], Nw[5];
-        TB2 Braw[4];  // b in its own precision, for the dot product
+        // six-row ring windows as in k_mg_down_sw2: row t + o of step t in slot
+        // (P + 1 + o) % 6, P = (t - t0) % 6; u enters at o = 4, b and the
+        // coefficients at o = 3
+        f2v Uw[6], Bw[6], Cw[6], Dw[6], Ew[6], Nw[6];
+        TB2 Braw[6];  // b in its own precision, for the dot product
 #pragma unroll
-        for (int k = 0; k < 6; ++k) Uw[k] = z2;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            Bw[k] = Cw[k] = Dw[k] = Ew[k] = z2;
+        for (int k = 0; k < 6; ++k) {
+            Uw[k] = Bw[k] = Cw[k] = Dw[k] = Ew[k] = Nw[k] = z2;
             Braw[k].x = (TB)0;
             Braw[k].y = (TB)0;
         }
-#pragma unroll
-        for (int k = 0; k < 5; ++k) Nw[k] = z2;
         // step t's loads: x and xc of row t+4; b, c, D, e, n of row t+3 (rows
         // clamped into the grid; rows outside it are zeroed where they are used)
         typedef typename RawPair<CT>::type CR;
@@ -2877,9 +2885,12 @@ __global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int 
             q.e = ldraw2(e, k3);
             q.n = ldraw2(n, k3);
         };
-        auto step = [&](int t, const Ld& q) {
+        auto step = [&](auto ph, int t, const Ld& q) {
+            constexpr int P = decltype(ph)::value;
+            constexpr int om = P % 6, o0 = (P + 1) % 6, o1 = (P + 2) % 6, o2 = (P + 3) % 6, o3 = (P + 4) % 6,
+                          o4 = (P + 5) % 6;
             const int j4 = t + 4, j3 = t + 3, j2 = t + 2, j1 = t + 1;
-            Uw[5] = in(j4) ? f2v{q.x.x + scale * q.xc, q.x.y + scale * q.xc} : z2;
+            Uw[o4] = in(j4) ? f2v{q.x.x + scale * q.xc, q.x.y + scale * q.xc} : z2;
             TB2 b3;
             b3.x = (TB)0;
             b3.y = (TB)0;
@@ -2891,50 +2902,58 @@ __global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int 
                 e3 = unpack2(q.e);
                 n3 = unpack2(q.n);
             }
-            Braw[3] = b3;
-            Bw[3] = f2v{(float)b3.x, (float)b3.y};
-            Cw[3] = c3;
-            Dw[3] = d3;
-            Ew[3] = e3;
-            Nw[4] = n3;
+            Braw[o3] = b3;
+            Bw[o3] = f2v{(float)b3.x, (float)b3.y};
+            Cw[o3] = c3;
+            Dw[o3] = d3;
+            Ew[o3] = e3;
+            Nw[o3] = n3;
             // black on t+3 (from u), red on t+2, black on t+1, red on t
-            if (in(j3)) sw2_relax(Uw[4], Uw[3], Uw[5], j3, false, ny, omega, hW0, hE1, Bw[3], Cw[3], Dw[3], Ew[3], Nw[4], Nw[3]);
-            if (in(j2)) sw2_relax(Uw[3], Uw[2], Uw[4], j2, true, ny, omega, hW0, hE1, Bw[2], Cw[2], Dw[2], Ew[2], Nw[3], Nw[2]);
-            if (in(j1)) sw2_relax(Uw[2], Uw[1], Uw[3], j1, false, ny, omega, hW0, hE1, Bw[1], Cw[1], Dw[1], Ew[1], Nw[2], Nw[1]);
+            if (in(j3))
+                sw2_relax(Uw[o3], Uw[o2], Uw[o4], j3, false, ny, omega, hW0, hE1, Bw[o3], Cw[o3], Dw[o3], Ew[o3],
+                          Nw[o3], Nw[o2]);
+            if (in(j2))
+                sw2_relax(Uw[o2], Uw[o1], Uw[o3], j2, true, ny, omega, hW0, hE1, Bw[o2], Cw[o2], Dw[o2], Ew[o2],
+                          Nw[o2], Nw[o1]);
+            if (in(j1))
+                sw2_relax(Uw[o1], Uw[o0], Uw[o2], j1, false, ny, omega, hW0, hE1, Bw[o1], Cw[o1], Dw[o1], Ew[o1],
+                          Nw[o1], Nw[o0]);
             if (t >= iy0) {
-                sw2_relax(Uw[1], Uw[0], Uw[2], t, true, ny, omega, hW0, hE1, Bw[0], Cw[0], Dw[0], Ew[0], Nw[1], Nw[0]);
+                sw2_relax(Uw[o0], Uw[om], Uw[o1], t, true, ny, omega, hW0, hE1, Bw[o0], Cw[o0], Dw[o0], Ew[o0],
+                          Nw[o0], Nw[om]);
                 if (own) {
-                    const f2v z = Uw[1];
+                    const f2v z = Uw[o0];
                     typename Pair<TO>::type zv;
                     zv.x = (TO)z.x;
                     zv.y = (TO)z.y;
                     *(typename Pair<TO>::type*)(out + cidx(ix, t, nx)) = zv;
-                    acc += (double)Braw[0].x * (double)z.x;
-                    acc += (double)Braw[0].y * (double)z.y;
+                    acc += (double)Braw[o0].x * (double)z.x;
+                    acc += (double)Braw[o0].y * (double)z.y;
                 }
             }
-#pragma unroll
-            for (int k = 0; k < 5; ++k) Uw[k] = Uw[k + 1];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                Bw[k] = Bw[k + 1];
-                Braw[k] = Braw[k + 1];
-                Cw[k] = Cw[k + 1];
-                Dw[k] = Dw[k + 1];
-                Ew[k] = Ew[k + 1];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) Nw[k] = Nw[k + 1];
         };
         // two load sets used alternately, a step ahead (as in the down-leg)
+        const int t0 = iy0 - 8;
         Ld qa, qb;
-        issue(iy0 - 8, qa);
-        for (int t = iy0 - 8; t < iy1; t += 2) {
+        issue(t0, qa);
+        for (int t = t0; t < iy1; t += 6) {
             issue(t + 1, qb);
-            step(t, qa);
+            step(Phase<0>{}, t, qa);
             if (t + 1 >= iy1) break;
             issue(t + 2, qa);
-            step(t + 1, qb);
+            step(Phase<1>{}, t + 1, qb);
+            if (t + 2 >= iy1) break;
+            issue(t + 3, qb);
+            step(Phase<2>{}, t + 2, qa);
+            if (t + 3 >= iy1) break;
+            issue(t + 4, qa);
+            step(Phase<3>{}, t + 3, qb);
+            if (t + 4 >= iy1) break;
+            issue(t + 5, qb);
+            step(Phase<4>{}, t + 4, qa);
+            if (t + 5 >= iy1) break;
+            issue(t + 6, qa);
+            step(Phase<5>{}, t + 5, qb);
         }
     }
     if (with_dot) {
