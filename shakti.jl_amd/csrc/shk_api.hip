@@ -164,6 +164,10 @@ struct shk_ctx {
     bool use_adaptive = true;  // batch sizes from the observed convergence rate
     int last_niter = 0;        // CG iterations of the previous solve
     int batch_margin = 2;      // the first batch stops this many iterations short of last_niter
+    // derived-field refresh: one marching pass (or the two-pass kernels); a
+    // Picard iteration that another one follows refreshes only N, K and mdot
+    bool derived_two_pass = false;
+    bool lean_refresh = true;
 
     // learned warm start and training-pair recording: the network input is
     // the grid padded with zero cells to W8 x H8 (multiples of 64 and 32)
@@ -1335,17 +1339,35 @@ int check_sliding(shk_ctx* ctx) {
     return SHK_OK;
 }
 
-int derived(shk_ctx* ctx, const double* h_prev, int check, double tol) {
+// The derived-field refresh (mode 0 after a solve, mode 1 of the initial
+// conditions).  full == false writes only N, K and mdot: what the next
+// assembly reads when another Picard iteration follows.
+void derived_launch(shk_ctx* ctx, int mode, bool full) {
+    if (ctx->derived_two_pass)
+        launch::derived(ctx->G, ctx->P, ctx->F, nullptr, mode, 0, ctx->partials, ctx->sc, ctx->st);
+    else
+        launch::derived_march(ctx->P, ctx->F, mode, full || !ctx->lean_refresh, ctx->st);
+}
+
+int derived(shk_ctx* ctx, bool full) {
     if (int rc = check_sliding(ctx)) return rc;
-    if (check) {
-        ctx->sc_host->picard_tol = tol;
-        // only picard_tol changes; write that field alone
-        HIPCHK(hipMemcpyAsync(&ctx->sc->picard_tol, &ctx->sc_host->picard_tol, sizeof(double), hipMemcpyHostToDevice,
-                              ctx->st));
-    }
     Prof pr(ctx, P_DER, 0);
-    launch::derived(ctx->G, ctx->P, ctx->F, h_prev, 0, check, ctx->partials, ctx->sc, ctx->st);
+    derived_launch(ctx, 0, full);
     return SHK_OK;
+}
+
+// The Picard test of the current head against h_prev (src/elliptic_solver.jl:
+// 153-169), the flag read back into sc_host->picard_converged.
+int picard_test(shk_ctx* ctx, const double* h_prev, double tol) {
+    ctx->sc_host->picard_tol = tol;
+    // only picard_tol changes; write that field alone
+    HIPCHK(hipMemcpyAsync(&ctx->sc->picard_tol, &ctx->sc_host->picard_tol, sizeof(double), hipMemcpyHostToDevice,
+                          ctx->st));
+    {
+        Prof pr(ctx, P_DER, 0);
+        launch::picard_check(ctx->P, ctx->F.f[SHK_H], h_prev, ctx->partials, ctx->sc, ctx->st);
+    }
+    return read_scalars(ctx);
 }
 
 template <class T>
@@ -1435,6 +1457,8 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     ctx->use_adaptive = getenv("SHK_FIXED_BATCH") == nullptr;
     if (getenv("SHK_MG_JACOBI")) ctx->mg_rb = 0;
     if (const char* e = getenv("SHK_BATCH_MARGIN")) ctx->batch_margin = atoi(e);
+    ctx->derived_two_pass = getenv("SHK_DERIVED_TWO_PASS") != nullptr;
+    ctx->lean_refresh = getenv("SHK_NO_LEAN_REFRESH") == nullptr;
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;
@@ -1716,7 +1740,7 @@ int shk_set_initial_conditions(shk_ctx* ctx) {
         Prof pr(ctx, P_IC, 0);
         launch::ic_cells(ctx->P, ctx->F, ctx->st);
         launch::stagger_b(ctx->P, ctx->F, ctx->st);
-        launch::derived(ctx->G, ctx->P, ctx->F, nullptr, 1, 0, ctx->partials, ctx->sc, ctx->st);
+        derived_launch(ctx, 1, true);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1744,7 +1768,7 @@ int shk_solve_parabolic(shk_ctx* ctx, double dt, shk_solve_stats* stats) {
 
 int shk_update_derived(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
-    int rc = derived(ctx, nullptr, 0, 0.0);
+    int rc = derived(ctx, true);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1755,7 +1779,7 @@ int shk_picard_iteration(shk_ctx* ctx, double relax_alpha, shk_solve_stats* stat
     if (!ctx) return SHK_ERR_ARG;
     int rc = solve(ctx, 0, 0.0, relax_alpha, stats);
     if (rc) return rc;
-    rc = derived(ctx, nullptr, 0, 0.0);
+    rc = derived(ctx, true);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1778,19 +1802,24 @@ int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every
         if (it == 1 && (ctx->cfg.flags & SHK_CFG_PICARD_ATOL) && std::isfinite(st.rnorm0))
             ctx->atol_floor = ctx->cfg.rtol * st.rnorm0;
         if (cg_iters) *cg_iters += st.niter;
+        // The test reads only the heads, so it runs before the refresh: the
+        // refresh after the loop's last iteration writes every field, the
+        // others only what the next assembly reads.
         const int check = (it % check_every == 0) || (it == iters);
-        rc = derived(ctx, ctx->h_alt /* previous h */, check, tol);
-        if (rc) return rc;
+        bool done = false;
         if (check) {
-            rc = read_scalars(ctx);
+            rc = picard_test(ctx, ctx->h_alt /* previous h */, tol);
             if (rc) return rc;
             // a solve that stopped short (itmax, zero curvature) leaves a head
             // change that says nothing about Picard convergence
-            if (ctx->sc_host->picard_converged && !st.zero_curvature && !st.tired) {
-                conv = 1;
-                last = it;
-                break;
-            }
+            done = ctx->sc_host->picard_converged && !st.zero_curvature && !st.tired;
+        }
+        rc = derived(ctx, done || it == iters);
+        if (rc) return rc;
+        if (done) {
+            conv = 1;
+            last = it;
+            break;
         }
     }
     if (!conv) last = iters;
@@ -1818,7 +1847,7 @@ int shk_parabolic_step(shk_ctx* ctx, double dt, int64_t* cg_iters) {
     int rc = solve(ctx, 1, dt, -1.0, &st);
     if (rc) return rc;
     if (cg_iters) *cg_iters += st.niter;
-    rc = derived(ctx, nullptr, 0, 0.0);
+    rc = derived(ctx, true);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));

@@ -153,6 +153,13 @@ void stagger(int nx, int ny, const double* c, double* fx, double* fy, hipStream_
 // check != 0 also reduces max|h - h_prev| and max|h| and sets the Picard flag.
 void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int mode, int check, double* partials,
              Scalars* sc, hipStream_t st);
+// The same refresh in one marching pass (mode as above, no check): full writes
+// every field; otherwise only N, K and mdot, for a Picard iteration that
+// another one follows.
+void derived_march(const Phys& P, Fields F, int mode, bool full, hipStream_t st);
+// The Picard test alone: max|h - h_prev| and max|h|, then the Picard flag.
+void picard_check(const Phys& P, const double* h, const double* h_prev, double* partials, Scalars* sc,
+                  hipStream_t st);
 void gap(const Phys& P, Fields F, double dt, hipStream_t st);
 void fill_ieb(int n, double* ieb, double uniform, const double* moulin, hipStream_t st);
 void zero(double* p, size_t n, hipStream_t st);

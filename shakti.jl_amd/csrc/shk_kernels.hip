@@ -514,6 +514,203 @@ __global__ __launch_bounds__(256) void k_derived_cells(Phys P, Fields F, const d
     }
 }
 
+// ------------------------------------------------ one-pass refresh
+//
+// The refresh of k_derived_faces + k_derived_cells in one marching pass that
+// keeps the face values in registers.  Each wave owns DER_OWN = 63 columns
+// of a band of R rows.  Per row, lane l evaluates the x face ix = base + l
+// (its cell's west face; lane 63's is the east face of lane 62's cell) and
+// the y face above its cell; a cell's east face comes from the next lane and
+// its south face from the previous row.  Each face is evaluated once, except
+// the x faces between strips and the south face of a band's first row, which
+// two waves evaluate.  The values are the same expressions of the same inputs
+// as in the two-pass kernels, so they agree bit for bit.
+// FULL writes every field the two-pass refresh writes.  Otherwise only N, K
+// and mdot are written: what the next assembly reads (with the fields a
+// Picard loop leaves unchanged) when another Picard iteration follows.
+
+constexpr int DER_OWN = WAVE - 1;
+
+struct DCell {  // what a face reads of an adjacent cell
+    double h, pw, N, ub, lam;
+};
+struct DFace {  // one face: gradients, flux, Reynolds number, stress, |ub| face value
+    double dh, dp, q, Re, tb, ub;
+};
+
+template <int MODE>
+__device__ __forceinline__ DCell dcell(const Phys& P, const FV& s, double rg, size_t c) {
+    DCell v;
+    v.h = s.h[c];
+    v.pw = MODE == 1 ? s.pw[c] : rg * (v.h - s.zb[c]);
+    v.N = jl_boolmul(s.mask[c] == GROUNDED, s.po[c] - v.pw);
+    const bool slide = P.sliding_law != SHK_SLIDING_PRESCRIBED;
+    v.ub = slide ? s.abs_ub[c] : 0.0;
+    v.lam = slide ? s.lambda[c] : 0.0;
+    return v;
+}
+
+// Face k between cells lo (west or south) and hi (east or north), d the cell
+// spacing across it.  edge -1: the domain's first face (only hi exists);
+// +1: its last face (only lo exists); 0: an inner face.  Boundary faces keep
+// their stored gradients, as in k_derived_faces.
+__device__ __forceinline__ DFace dface(const Phys& P, int edge, const DCell& lo, const DCell& hi, double d,
+                                       const double* __restrict__ dh_a, const double* __restrict__ dp_a,
+                                       const double* __restrict__ b_a, const double* __restrict__ ub_a,
+                                       const double* __restrict__ v_a, const double* __restrict__ tb_a,
+                                       const double* __restrict__ C_a, size_t k) {
+    DFace f;
+    if (edge == 0) {
+        const double v = v_a[k];
+        f.dh = ((hi.h - lo.h) / d) * v;
+        f.dp = ((hi.pw - lo.pw) / d) * v;
+    } else {
+        f.dh = dh_a[k];
+        f.dp = dp_a[k];
+    }
+    qre(P, b_a[k], f.dh, f.q, f.Re);
+    f.ub = ub_a[k];
+    if (P.sliding_law != SHK_SLIDING_PRESCRIBED) {
+        double Nf, av, lf;
+        if (edge < 0) {
+            Nf = hi.N; av = hi.ub; lf = hi.lam;
+        } else if (edge > 0) {
+            Nf = lo.N; av = lo.ub; lf = lo.lam;
+        } else {
+            Nf = (hi.N + lo.N) / 2.0;
+            lf = (hi.lam + lo.lam) / 2.0;
+            av = (hi.ub + lo.ub) / 2.0;
+        }
+        f.tb = taub_face(P, Nf, av, lf, f.ub, C_a ? C_a[k] : 0.0);
+    } else {
+        f.tb = tb_a[k];
+    }
+    return f;
+}
+
+__device__ __forceinline__ void store_face(const Phys& P, size_t k, const DFace& f, bool inner, double* dh_a,
+                                           double* dp_a, double* q_a, double* Re_a, double* tb_a) {
+    if (inner) {
+        dh_a[k] = f.dh;
+        dp_a[k] = f.dp;
+    }
+    q_a[k] = f.q;
+    Re_a[k] = f.Re;
+    if (P.sliding_law != SHK_SLIDING_PRESCRIBED) tb_a[k] = f.tb;
+}
+
+__device__ __forceinline__ DFace shfl_down_face(const DFace& f) {
+    DFace g;
+    g.dh = __shfl_down(f.dh, 1, WAVE);
+    g.dp = __shfl_down(f.dp, 1, WAVE);
+    g.q = __shfl_down(f.q, 1, WAVE);
+    g.Re = __shfl_down(f.Re, 1, WAVE);
+    g.tb = __shfl_down(f.tb, 1, WAVE);
+    g.ub = __shfl_down(f.ub, 1, WAVE);
+    return g;
+}
+__device__ __forceinline__ DCell shfl_up_cell(const DCell& c) {
+    DCell g;
+    g.h = __shfl_up(c.h, 1, WAVE);
+    g.pw = __shfl_up(c.pw, 1, WAVE);
+    g.N = __shfl_up(c.N, 1, WAVE);
+    g.ub = __shfl_up(c.ub, 1, WAVE);
+    g.lam = __shfl_up(c.lam, 1, WAVE);
+    return g;
+}
+
+template <int MODE, bool FULL>
+__global__ __launch_bounds__(256) void k_derived_march(Phys P, Fields F, int R) {
+    const int nx = P.nx, ny = P.ny;
+    int vbx, vby;
+    virtual_block(vbx, vby);
+    const int lane = threadIdx.x;
+    const int ix = vbx * DER_OWN + lane;
+    const int iy0 = (vby * WAVES + threadIdx.y) * R;
+    if (iy0 >= ny) return;  // uniform over the wave
+    const int iy1 = min(iy0 + R, ny);
+    FV s(F);
+    const double rg = P.rho_w * P.g;
+    const bool own = lane < DER_OWN && ix < nx;                   // this strip's cell
+    const bool xstore = ix <= nx && (lane < DER_OWN || ix == nx);  // this strip writes x face ix
+    const int ixc = min(ix, nx - 1);                               // clamped column of cell loads
+    const int ixf = min(ix, nx);                                   // clamped x face
+    const int xedge = ix == 0 ? -1 : (ix >= nx ? 1 : 0);
+    DCell cur = dcell<MODE>(P, s, rg, cidx(ixc, iy0, nx));
+    DFace fs;  // the south face of the current row
+    {
+        const size_t k = cidx(ixc, iy0, nx);
+        if (iy0 == 0) {
+            fs = dface(P, -1, cur, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
+            if (FULL && own) store_face(P, k, fs, false, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
+        } else {
+            const DCell below = dcell<MODE>(P, s, rg, cidx(ixc, iy0 - 1, nx));
+            fs = dface(P, 0, below, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
+        }
+    }
+    for (int iy = iy0; iy < iy1; ++iy) {
+        const DCell nxt = iy + 1 < ny ? dcell<MODE>(P, s, rg, cidx(ixc, iy + 1, nx)) : cur;
+        DCell west = shfl_up_cell(cur);
+        if (lane == 0 && ix > 0) west = dcell<MODE>(P, s, rg, cidx(ix - 1, iy, nx));
+        const size_t kx = xidx(ixf, iy, nx), ky = cidx(ixc, iy + 1, nx);
+        const DFace fw = dface(P, xedge, west, cur, P.dx, s.dhdx, s.dpwdx, s.b_x, s.ub_x, s.valid_x, s.taub_x, P.Cx, kx);
+        const int yedge = iy + 1 == ny ? 1 : 0;
+        const DFace fn = dface(P, yedge, cur, nxt, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, ky);
+        const DFace fe = shfl_down_face(fw);
+        if (FULL) {
+            if (xstore) store_face(P, kx, fw, xedge == 0, s.dhdx, s.dpwdx, s.q_x, s.Re_x, s.taub_x);
+            if (own) store_face(P, ky, fn, yedge == 0, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
+        }
+        if (own) {
+            // the cell pass of k_derived_cells, same expressions
+            const size_t c = cidx(ix, iy, nx);
+            const double Re_c = (fw.Re + fe.Re + fs.Re + fn.Re) / 4.0;
+            const double sh = fabs((fe.ub * fe.tb + fw.ub * fw.tb) / 2.0 + (fn.ub * fn.tb + fs.ub * fs.tb) / 2.0);
+            const double pot = fabs((fe.q * fe.dh + fw.q * fw.dh) / 2.0 + (fn.q * fn.dh + fs.q * fs.dh) / 2.0);
+            const double Linv = 1.0 / P.L;
+            if (P.with_sensible) {
+                const double sens = (fe.q * fe.dp + fw.q * fw.dp) / 2.0 + (fn.q * fn.dp + fs.q * fs.dp) / 2.0;
+                if (FULL) s.sensible[c] = sens;
+                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot + P.ct * P.cw * P.rho_w * sens);
+            } else {
+                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot);
+            }
+            const double bc = s.b[c];
+            s.K[c] = (bc * bc * bc * P.g) / (12.0 * P.nu * (1.0 + P.omega * Re_c));
+            s.N[c] = cur.N;
+            if (FULL) {
+                if (MODE == 0) s.pw[c] = cur.pw;
+                s.Re[c] = Re_c;
+                s.shear[c] = sh;
+                s.potential[c] = pot;
+            }
+        }
+        fs = fn;
+        cur = nxt;
+    }
+}
+
+// The Picard test's maxima (src/elliptic_solver.jl:153-169): max|h - h_prev|
+// (NaN if either head holds a NaN) and max|h| per workgroup, finished by
+// k_finish (FIN_PICARD), as in k_derived_cells with CHECK.
+__global__ __launch_bounds__(256) void k_picard_check(int nx, int ny, const double* __restrict__ h,
+                                                      const double* __restrict__ h_prev, double* partials) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    double dmax = 0.0, hmax = 0.0;
+    bool any_nan = false;
+    if (ix < nx && iy < ny) {
+        const size_t c = cidx(ix, iy, nx);
+        const double hc = h[c];
+        const double d = fabs(hc - h_prev[c]), a = fabs(hc);
+        any_nan |= isnan(d) || isnan(a);
+        dmax = d > dmax ? d : dmax;
+        hmax = a > hmax ? a : hmax;
+    }
+    double v[2] = {any_nan ? NAN : dmax, hmax};
+    block_partial<2, true>(v, partials);
+}
+
 // compute_b! (src/gap_height.jl:117-135) then beta (src/gap_height.jl:47-52)
 __global__ __launch_bounds__(256) void k_gap(Phys P, Fields F, double dt) {
     const int n = P.nx * P.ny;
@@ -4538,6 +4735,24 @@ void derived(const Geom& G, const Phys& P, Fields F, const double* h_prev, int m
     } else {
         hipLaunchKernelGGL((k_derived_cells<1, false>), cgrid, blk, 0, st, P, F, h_prev, partials);
     }
+}
+void derived_march(const Phys& P, Fields F, int mode, bool full, hipStream_t st) {
+    static const int r_env = [] {
+        const char* e = getenv("SHK_DERIVED_R");
+        return e ? atoi(e) : 0;
+    }();
+    const int R = r_env > 0 ? r_env : 8;
+    const dim3 grid((P.nx + DER_OWN - 1) / DER_OWN, ((P.ny + R - 1) / R + WAVES - 1) / WAVES), blk(WAVE, WAVES);
+    if (mode == 1) hipLaunchKernelGGL((k_derived_march<1, true>), grid, blk, 0, st, P, F, R);
+    else if (full) hipLaunchKernelGGL((k_derived_march<0, true>), grid, blk, 0, st, P, F, R);
+    else hipLaunchKernelGGL((k_derived_march<0, false>), grid, blk, 0, st, P, F, R);
+}
+void picard_check(const Phys& P, const double* h, const double* h_prev, double* partials, Scalars* sc,
+                  hipStream_t st) {
+    const dim3 cgrid = cells_grid(P.nx, P.ny);
+    hipLaunchKernelGGL(k_picard_check, cgrid, dim3(WAVE, WAVES), 0, st, P.nx, P.ny, h, h_prev, partials);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(cgrid.x * cgrid.y),
+                       (int)FIN_PICARD, 0, sc);
 }
 void gap(const Phys& P, Fields F, double dt, hipStream_t st) {
     const int n = P.nx * P.ny;

@@ -567,3 +567,59 @@ def test_picard_atol_trajectory_matches_oracle(oracle, hip, form):
     cfg = ls.cfg()
     cfg.flags = 2
     assert L.load().shk_set_solver_cfg(hs.ctx, C.byref(cfg)) == -1  # SHK_ERR_ARG
+
+
+_REFRESH_LAWS = {
+    "coulomb": lambda H, g, c: H.RegularizedCoulombSlidingLaw(0.5),
+    "prescribed": lambda H, g, c: H.PrescribedSlidingLaw(),
+    "linear_field": lambda H, g, c: H.LinearSlidingLaw(g, c * 1e2),
+    "coulomb_v0_field": lambda H, g, c: H.RegularizedCoulombV0SlidingLaw(g, c, 3e-7),
+}
+
+
+@pytest.mark.parametrize("shape", [(1283, 1001), (126, 65)])
+@pytest.mark.parametrize("case", list(_REFRESH_LAWS) + ["no_sensible"])
+def test_one_pass_refresh_matches_two_pass_kernels(hip, monkeypatch, shape, case):
+    """The one-pass derived-field refresh (63 owned columns per wave, bands of
+    rows) and the lean refresh of the Picard iterations that another one
+    follows (N, K and mdot only) leave the State of the two-pass kernels bit
+    for bit: after the initial conditions and after each of three hourly
+    steps of the multigrid form (same Picard counts).  1283 x 1001 has a
+    partial last strip and a one-row last band; 126 = 2 x 63 puts the east
+    boundary face on the last strip's halo lane.  Each sliding law, and the
+    melt rate without the sensible-heat term (ct = 0)."""
+    from shakti_hip import synthetic as S
+    from shakti_hip._lib import FIELD_NAMES
+    nx, ny = shape
+    gg, raw = S.glacier(nx, ny, bed_amp=10.0, seed=21)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    Cc = np.asfortranarray(np.random.default_rng(8).uniform(0.2, 0.8, (nx, ny)))
+    law = _REFRESH_LAWS.get(case, _REFRESH_LAWS["coulomb"])
+    p_kw = {"ct": 0.0} if case == "no_sensible" else None
+    runs = {}
+    for two_pass in (True, False):
+        if two_pass:
+            monkeypatch.setenv("SHK_DERIVED_TWO_PASS", "1")
+        else:
+            monkeypatch.delenv("SHK_DERIVED_TWO_PASS", raising=False)
+        print(f"{shape} {case}: {'two-pass' if two_pass else 'one-pass'} refresh", flush=True)
+        hg, hp, hs = hip_state(hip, g_args, raw, p_kw=p_kw, law=lambda g_: law(hip, g_, Cc))
+        states = [{n: hs.get(n) for n in FIELD_NAMES}]
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, atol=0.0, rtol=1e-10, multigrid=True)
+        ps = hip.PicardSolver(500, 1e-6, ls, hg)
+        sim = hip.Simulation(hg, hs, 3, 3600.0, hp, "implicit", [], hip.ConstantMeltInput(), law(hip, hg, Cc),
+                             ps=ps)
+        counts = []
+        for t in range(3):
+            hip.step(sim)
+            sim.total_time += sim.dt
+            counts.append((ps.converged, ps.last_iter))
+            states.append({n: hs.get(n) for n in FIELD_NAMES})
+        runs[two_pass] = (states, counts)
+        del sim, ps, ls, hs
+    (sa, ca), (sb, cb) = runs[True], runs[False]
+    assert ca == cb, (ca, cb)
+    assert max(last for _, last in ca) > 1  # lean refreshes took part
+    for t, (a, b) in enumerate(zip(sa, sb)):
+        bad = [n for n in FIELD_NAMES if not np.array_equal(a[n], b[n], equal_nan=True)]
+        assert not bad, (t, bad)
