@@ -104,6 +104,42 @@ __device__ __forceinline__ void virtual_block(int& vbx, int& vby) {
     vby = (int)(v / gridDim.x);
 }
 
+// Wave-wide lane shifts through DPP (GFX9 wave_shr:1 / wave_shl:1): the value
+// of lane i-1 (from the west) or lane i+1 (from the east); lanes without a
+// source receive 0.
+__device__ __forceinline__ int dpp_west_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_east_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ double from_west(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_west_i((int)(b & 0xffffffffll)), hi = dpp_west_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_east(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_east_i((int)(b & 0xffffffffll)), hi = dpp_east_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float from_west(float v) { return __int_as_float(dpp_west_i(__float_as_int(v))); }
+__device__ __forceinline__ float from_east(float v) { return __int_as_float(dpp_east_i(__float_as_int(v))); }
+// One-lane shifts with the edge semantics of __shfl_up/__shfl_down(v, 1):
+// the edge lane without a source keeps its own value (DPP with bound_ctrl
+// off writes nothing there and old = v).  A DPP move is one VALU instruction,
+// where the shuffle is an LDS permute with its round trip.
+__device__ __forceinline__ int lane_up1_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int lane_down1_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ float lane_up1(float v) { return __int_as_float(lane_up1_i(__float_as_int(v))); }
+__device__ __forceinline__ float lane_down1(float v) { return __int_as_float(lane_down1_i(__float_as_int(v))); }
+__device__ __forceinline__ double lane_up1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_up1_i((int)(b & 0xffffffffll)), hi = lane_up1_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_down1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_down1_i((int)(b & 0xffffffffll)), hi = lane_down1_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
@@ -537,6 +573,17 @@ struct DCell {  // what a face reads of an adjacent cell
 struct DFace {  // one face: gradients, flux, Reynolds number, stress, |ub| face value
     double dh, dp, q, Re, tb, ub;
 };
+struct DProd {  // what the cell pass reads of a face: Re, ub*taub, q*dh/dx, q*dpw/dx
+    double Re, ut, qh, qp;
+};
+__device__ __forceinline__ DProd dprod(const DFace& f) {
+    DProd p;
+    p.Re = f.Re;
+    p.ut = f.ub * f.tb;
+    p.qh = f.q * f.dh;
+    p.qp = f.q * f.dp;
+    return p;
+}
 
 template <int MODE>
 __device__ __forceinline__ DCell dcell(const Phys& P, const FV& s, double rg, size_t c) {
@@ -599,23 +646,21 @@ __device__ __forceinline__ void store_face(const Phys& P, size_t k, const DFace&
     if (P.sliding_law != SHK_SLIDING_PRESCRIBED) tb_a[k] = f.tb;
 }
 
-__device__ __forceinline__ DFace shfl_down_face(const DFace& f) {
-    DFace g;
-    g.dh = __shfl_down(f.dh, 1, WAVE);
-    g.dp = __shfl_down(f.dp, 1, WAVE);
-    g.q = __shfl_down(f.q, 1, WAVE);
-    g.Re = __shfl_down(f.Re, 1, WAVE);
-    g.tb = __shfl_down(f.tb, 1, WAVE);
-    g.ub = __shfl_down(f.ub, 1, WAVE);
+__device__ __forceinline__ DProd prod_from_east(const DProd& f, bool sens) {
+    DProd g;
+    g.Re = from_east(f.Re);
+    g.ut = from_east(f.ut);
+    g.qh = from_east(f.qh);
+    g.qp = sens ? from_east(f.qp) : 0.0;
     return g;
 }
-__device__ __forceinline__ DCell shfl_up_cell(const DCell& c) {
+__device__ __forceinline__ DCell cell_from_west(const DCell& c) {
     DCell g;
-    g.h = __shfl_up(c.h, 1, WAVE);
-    g.pw = __shfl_up(c.pw, 1, WAVE);
-    g.N = __shfl_up(c.N, 1, WAVE);
-    g.ub = __shfl_up(c.ub, 1, WAVE);
-    g.lam = __shfl_up(c.lam, 1, WAVE);
+    g.h = from_west(c.h);
+    g.pw = from_west(c.pw);
+    g.N = from_west(c.N);
+    g.ub = from_west(c.ub);
+    g.lam = from_west(c.lam);
     return g;
 }
 
@@ -636,10 +681,12 @@ __global__ __launch_bounds__(256) void k_derived_march(Phys P, Fields F, int R) 
     const int ixc = min(ix, nx - 1);                               // clamped column of cell loads
     const int ixf = min(ix, nx);                                   // clamped x face
     const int xedge = ix == 0 ? -1 : (ix >= nx ? 1 : 0);
+    const bool sens = P.with_sensible;
     DCell cur = dcell<MODE>(P, s, rg, cidx(ixc, iy0, nx));
-    DFace fs;  // the south face of the current row
+    DProd ps;  // the south face of the current row
     {
         const size_t k = cidx(ixc, iy0, nx);
+        DFace fs;
         if (iy0 == 0) {
             fs = dface(P, -1, cur, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
             if (FULL && own) store_face(P, k, fs, false, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
@@ -647,31 +694,37 @@ __global__ __launch_bounds__(256) void k_derived_march(Phys P, Fields F, int R) 
             const DCell below = dcell<MODE>(P, s, rg, cidx(ixc, iy0 - 1, nx));
             fs = dface(P, 0, below, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
         }
+        ps = dprod(fs);
     }
     for (int iy = iy0; iy < iy1; ++iy) {
         const DCell nxt = iy + 1 < ny ? dcell<MODE>(P, s, rg, cidx(ixc, iy + 1, nx)) : cur;
-        DCell west = shfl_up_cell(cur);
+        DCell west = cell_from_west(cur);
         if (lane == 0 && ix > 0) west = dcell<MODE>(P, s, rg, cidx(ix - 1, iy, nx));
         const size_t kx = xidx(ixf, iy, nx), ky = cidx(ixc, iy + 1, nx);
-        const DFace fw = dface(P, xedge, west, cur, P.dx, s.dhdx, s.dpwdx, s.b_x, s.ub_x, s.valid_x, s.taub_x, P.Cx, kx);
+        DProd pw_, pe, pn;
+        {
+            const DFace fw = dface(P, xedge, west, cur, P.dx, s.dhdx, s.dpwdx, s.b_x, s.ub_x, s.valid_x, s.taub_x, P.Cx, kx);
+            if (FULL && xstore) store_face(P, kx, fw, xedge == 0, s.dhdx, s.dpwdx, s.q_x, s.Re_x, s.taub_x);
+            pw_ = dprod(fw);
+        }
+        pe = prod_from_east(pw_, sens);
         const int yedge = iy + 1 == ny ? 1 : 0;
-        const DFace fn = dface(P, yedge, cur, nxt, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, ky);
-        const DFace fe = shfl_down_face(fw);
-        if (FULL) {
-            if (xstore) store_face(P, kx, fw, xedge == 0, s.dhdx, s.dpwdx, s.q_x, s.Re_x, s.taub_x);
-            if (own) store_face(P, ky, fn, yedge == 0, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
+        {
+            const DFace fn = dface(P, yedge, cur, nxt, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, ky);
+            if (FULL && own) store_face(P, ky, fn, yedge == 0, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
+            pn = dprod(fn);
         }
         if (own) {
             // the cell pass of k_derived_cells, same expressions
             const size_t c = cidx(ix, iy, nx);
-            const double Re_c = (fw.Re + fe.Re + fs.Re + fn.Re) / 4.0;
-            const double sh = fabs((fe.ub * fe.tb + fw.ub * fw.tb) / 2.0 + (fn.ub * fn.tb + fs.ub * fs.tb) / 2.0);
-            const double pot = fabs((fe.q * fe.dh + fw.q * fw.dh) / 2.0 + (fn.q * fn.dh + fs.q * fs.dh) / 2.0);
+            const double Re_c = (pw_.Re + pe.Re + ps.Re + pn.Re) / 4.0;
+            const double sh = fabs((pe.ut + pw_.ut) / 2.0 + (pn.ut + ps.ut) / 2.0);
+            const double pot = fabs((pe.qh + pw_.qh) / 2.0 + (pn.qh + ps.qh) / 2.0);
             const double Linv = 1.0 / P.L;
-            if (P.with_sensible) {
-                const double sens = (fe.q * fe.dp + fw.q * fw.dp) / 2.0 + (fn.q * fn.dp + fs.q * fs.dp) / 2.0;
-                if (FULL) s.sensible[c] = sens;
-                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot + P.ct * P.cw * P.rho_w * sens);
+            if (sens) {
+                const double sn = (pe.qp + pw_.qp) / 2.0 + (pn.qp + ps.qp) / 2.0;
+                if (FULL) s.sensible[c] = sn;
+                s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot + P.ct * P.cw * P.rho_w * sn);
             } else {
                 s.mdot[c] = Linv * (s.G[c] + sh + P.rho_w * P.g * pot);
             }
@@ -685,7 +738,7 @@ __global__ __launch_bounds__(256) void k_derived_march(Phys P, Fields F, int R) 
                 s.potential[c] = pot;
             }
         }
-        fs = fn;
+        ps = pn;
         cur = nxt;
     }
 }
@@ -887,9 +940,9 @@ __device__ __forceinline__ void march(int nx, int ny, int R, int vbx, int vby, c
         const size_t c = cidx(ixc, iy, nx);
         const T up = (iy + 1 < ny) ? opd(c + nx) : T(0);
         const T aPc = aP[c], aEc = aE[c], aNc = aN[c];
-        T ue = __shfl_down(uc, 1, WAVE);
-        T uw = __shfl_up(uc, 1, WAVE);
-        aEw = __shfl_up(aEc, 1, WAVE);
+        T ue = lane_down1(uc);
+        T uw = lane_up1(uc);
+        aEw = lane_up1(aEc);
         if (lane == WAVE - 1 && ix < nx - 1) ue = opd(c + 1);
         if (lane == 0 && ix > 0) {
             uw = opd(c - 1);
@@ -966,9 +1019,9 @@ __device__ __forceinline__ void march2(int nx, int ny, int R, int strip, int iy0
         d2v up = {0.0, 0.0};
         if (iy + 1 < ny) up = opd2(c + nx);
         const d2v aPc = ld2<NT>(aP + c), aEc = ld2<NT>(aE + c), aNc = ld2<NT>(aN + c);
-        double ue = __shfl_down(uc.x, 1, WAVE);   // column ix + 2
-        double uw = __shfl_up(uc.y, 1, WAVE);     // column ix - 1
-        double aEw = __shfl_up(aEc.y, 1, WAVE);   // coupling (ix - 1, ix)
+        double ue = lane_down1(uc.x);   // column ix + 2
+        double uw = lane_up1(uc.y);     // column ix - 1
+        double aEw = lane_up1(aEc.y);   // coupling (ix - 1, ix)
         if (lane == WAVE - 1 && ix + 2 < nx) ue = opd1(c + 2);
         if (lane == 0 && ix > 0) {
             uw = opd1(c - 1);
@@ -1718,23 +1771,6 @@ __global__ __launch_bounds__(256) void k_zero(size_t n, double* p) {
 // unfused passes (k_cg_update + k_cheb_pass), so the results are identical.
 // Reads r_old/Ap/aP/aE/aN with the halo; writes x, r_new, z on owned cells
 // only (r is ping-ponged), so no workgroup reads what another writes.
-// Wave-wide lane shifts through DPP (GFX9 wave_shr:1 / wave_shl:1): the value
-// of lane i-1 (from the west) or lane i+1 (from the east); lanes without a
-// source receive 0.
-__device__ __forceinline__ int dpp_west_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
-__device__ __forceinline__ int dpp_east_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
-__device__ __forceinline__ double from_west(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_west_i((int)(b & 0xffffffffll)), hi = dpp_west_i((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ double from_east(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_east_i((int)(b & 0xffffffffll)), hi = dpp_east_i((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ float from_west(float v) { return __int_as_float(dpp_west_i(__float_as_int(v))); }
-__device__ __forceinline__ float from_east(float v) { return __int_as_float(dpp_east_i(__float_as_int(v))); }
 
 // (Non-temporal hints were measured here as well: at 8192 x 2048 they slowed
 // this kernel from 320 to 452 us, the overlapping strips re-reading halo
@@ -2248,9 +2284,9 @@ __global__ __launch_bounds__(256) void k_mg_down_march(int nx, int ny, int nxc, 
         const float up = (iy + 1 < ny) ? xs(k + nx) : 0.f;
         const float cc = c[k], ec = e[k], nc = n[k];
         const float bk = (float)b[k];
-        float ue = __shfl_down(uc, 1, WAVE);
-        float uw = __shfl_up(uc, 1, WAVE);
-        float ew = __shfl_up(ec, 1, WAVE);
+        float ue = lane_down1(uc);
+        float uw = lane_up1(uc);
+        float ew = lane_up1(ec);
         if (lane == WAVE - 1 && ix < nx - 1) ue = xs(k + 1);
         if (lane == 0 && ix > 0) {
             uw = xs(k - 1);
@@ -2263,7 +2299,7 @@ __global__ __launch_bounds__(256) void k_mg_down_march(int nx, int ny, int nxc, 
         if (iy > 0) a += nm * (uc - um);
         const float res = inx ? bk - a : 0.f;
         if (inx) x[k] = uc;
-        const float hs = res + __shfl_down(res, 1, WAVE);  // meaningful on even lanes
+        const float hs = res + lane_down1(res);  // meaningful on even lanes
         if ((iy & 1) == 0) {
             hrow = hs;
         } else if (even && inx) {
@@ -2312,9 +2348,9 @@ __global__ __launch_bounds__(256) void k_mg_up_march(int nx, int ny, int nxc, in
             const float up = (iy + 1 < ny) ? us(ixc, iy + 1) : 0.f;
             const float cc = c[k], ec = e[k], nc = n[k], dk = D[k];
             const TB bk = b[k];
-            float ue = __shfl_down(uc, 1, WAVE);
-            float uw = __shfl_up(uc, 1, WAVE);
-            float ew = __shfl_up(ec, 1, WAVE);
+            float ue = lane_down1(uc);
+            float uw = lane_up1(uc);
+            float ew = lane_up1(ec);
             if (lane == WAVE - 1 && ix < nx - 1) ue = us(ix + 1, iy);
             if (lane == 0 && ix > 0) {
                 uw = us(ix - 1, iy);
@@ -2394,9 +2430,9 @@ __global__ __launch_bounds__(256) void k_mg_down_march2(int nx, int ny, int nxc,
         if (iy + 1 < ny) up = xs2(k + nx);
         const f2v cc = *(const f2v*)(c + k), ec = *(const f2v*)(e + k), nc = *(const f2v*)(n + k);
         const TB2 bk = *(const TB2*)(b + k);
-        float ue = __shfl_down(uc.x, 1, WAVE);  // column ix + 2
-        float uw = __shfl_up(uc.y, 1, WAVE);    // column ix - 1
-        float ew = __shfl_up(ec.y, 1, WAVE);    // coupling (ix - 1, ix)
+        float ue = lane_down1(uc.x);  // column ix + 2
+        float uw = lane_up1(uc.y);    // column ix - 1
+        float ew = lane_up1(ec.y);    // coupling (ix - 1, ix)
         if (lane == WAVE - 1 && ix + 2 < nx) ue = xs1(k + 2);
         if (lane == 0 && ix > 0) {
             uw = xs1(k - 1);
@@ -2467,9 +2503,9 @@ __global__ __launch_bounds__(256) void k_mg_up_march2(int nx, int ny, int nxc, i
             const f2v cc = *(const f2v*)(c + k), ec = *(const f2v*)(e + k), nc = *(const f2v*)(n + k),
                       dk = *(const f2v*)(D + k);
             const TB2 bk = *(const TB2*)(b + k);
-            float ue = __shfl_down(uc.x, 1, WAVE);
-            float uw = __shfl_up(uc.y, 1, WAVE);
-            float ew = __shfl_up(ec.y, 1, WAVE);
+            float ue = lane_down1(uc.x);
+            float uw = lane_up1(uc.y);
+            float ew = lane_up1(ec.y);
             if (lane == WAVE - 1 && ix + 2 < nx) ue = us1(ix + 2, iy);
             if (lane == 0 && ix > 0) {
                 uw = us1(ix - 1, iy);
@@ -2635,11 +2671,11 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         const bool hN = j < ny - 1, hS = j > 0;
         f2v v = rc;
         if ((j & 1) == 0) {  // black: column ix + 1
-            const float xE = __shfl_down(rc.x, 1, WAVE);
+            const float xE = lane_down1(rc.x);
             v.y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f, hN ? nj.y : 0.f,
                             hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
         } else {  // black: column ix
-            const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+            const float xW = lane_up1(rc.y), aW = lane_up1(ej.y);
             v.x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y, hN ? nj.x : 0.f,
                             hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
         }
@@ -2675,9 +2711,9 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2(int nx, int ny, int nxc, in
         if (t >= iy0) {
             const size_t k = cidx(ixc, t, nx);
             const f2v cc = unpack2(q.c);
-            const float ue = __shfl_down(X0.x, 1, WAVE);  // column ix + 2
-            const float uw = __shfl_up(X0.y, 1, WAVE);    // column ix - 1
-            const float ew = __shfl_up(e0.y, 1, WAVE);    // coupling (ix - 1, ix)
+            const float ue = lane_down1(X0.x);  // column ix + 2
+            const float uw = lane_up1(X0.y);    // column ix - 1
+            const float ew = lane_up1(e0.y);    // coupling (ix - 1, ix)
             float a0 = cc.x * X0.x;
             a0 += e0.x * (X0.x - X0.y);  // ix < nx - 1 for every owned pair
             if (hW0) a0 += ew * (X0.x - uw);
@@ -2761,11 +2797,11 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
             const bool hN = j < ny - 1, hS = j > 0;
             f2v v = uc;
             if ((j & 1) == 0) {  // black: column ix + 1
-                const float uE = __shfl_down(uc.x, 1, WAVE);
+                const float uE = lane_down1(uc.x);
                 v.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x,
                                uc.x, hN, hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
             } else {  // black: column ix
-                const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                const float uW = lane_up1(uc.y), aW = lane_up1(ej.y);
                 v.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f,
                                hW0 ? uW : 0.f, hN, hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f,
                                hS ? um.x : 0.f);
@@ -2776,12 +2812,12 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2(int nx, int ny, int nxc, int 
             const bool hN = j < ny - 1, hS = j > 0;
             f2v z = vc;
             if ((j & 1) == 0) {  // red: column ix
-                const float vW = __shfl_up(vc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                const float vW = lane_up1(vc.y), aW = lane_up1(ej.y);
                 z.x = rb_relax(bj.x, cj.x, dj.x, omega, vc.x, true, ej.x, vc.y, hW0, hW0 ? aW : 0.f,
                                hW0 ? vW : 0.f, hN, hN ? nj.x : 0.f, hN ? vp.x : 0.f, hS, hS ? nm.x : 0.f,
                                hS ? vm.x : 0.f);
             } else {  // red: column ix + 1
-                const float vE = __shfl_down(vc.x, 1, WAVE);
+                const float vE = lane_down1(vc.x);
                 z.y = rb_relax(bj.y, cj.y, dj.y, omega, vc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? vE : 0.f, true, ej.x,
                                vc.x, hN, hN ? nj.y : 0.f, hN ? vp.y : 0.f, hS, hS ? nm.y : 0.f, hS ? vm.y : 0.f);
             }
@@ -2864,11 +2900,11 @@ __device__ __forceinline__ void sw2_relax(f2v& uc, f2v um, f2v up, int j, bool r
                                           bool hW0, bool hE1, f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) {
     const bool hN = j < ny - 1, hS = j > 0;
     if (((j & 1) == 0) == red_pass) {
-        const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+        const float uW = lane_up1(uc.y), aW = lane_up1(ej.y);
         uc.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f, hW0 ? uW : 0.f, hN,
                         hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f, hS ? um.x : 0.f);
     } else {
-        const float uE = __shfl_down(uc.x, 1, WAVE);
+        const float uE = lane_down1(uc.x);
         uc.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x, uc.x, hN,
                         hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
     }
@@ -2959,11 +2995,11 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
                       nm = Nw[o2];
             const bool hN = j3 < ny - 1, hS = j3 > 0;
             if ((j3 & 1) == 0) {
-                const float xE = __shfl_down(rc.x, 1, WAVE);
+                const float xE = lane_down1(rc.x);
                 X[o3].y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
                                     hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
             } else {
-                const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                const float xW = lane_up1(rc.y), aW = lane_up1(ej.y);
                 X[o3].x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
                                     hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
             }
@@ -2980,9 +3016,9 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
             const size_t k = cidx(ixc, t, nx);
             const f2v X0 = X[o0], Xm = X[om], X1 = X[o1], cc = Cw[o0], e0 = Ew[o0], n0 = Nw[o0], nm = Nw[om],
                       b0 = Bw[o0];
-            const float ue = __shfl_down(X0.x, 1, WAVE);  // column ix + 2
-            const float uw = __shfl_up(X0.y, 1, WAVE);    // column ix - 1
-            const float ew = __shfl_up(e0.y, 1, WAVE);    // coupling (ix - 1, ix)
+            const float ue = lane_down1(X0.x);  // column ix + 2
+            const float uw = lane_up1(X0.y);    // column ix - 1
+            const float ew = lane_up1(e0.y);    // coupling (ix - 1, ix)
             float a0 = cc.x * X0.x;
             a0 += e0.x * (X0.x - X0.y);
             if (hW0) a0 += ew * (X0.x - uw);
@@ -3250,11 +3286,11 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
             const bool hN = j < ny - 1, hS = j > 0;
             v = rc;
             if ((j & 1) == 0) {
-                const float xE = __shfl_down(rc.x, 1, WAVE);
+                const float xE = lane_down1(rc.x);
                 v.y = rb_black0(Bv[q].y, Dv[q].y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
                                 hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
             } else {
-                const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                const float xW = lane_up1(rc.y), aW = lane_up1(ej.y);
                 v.x = rb_black0(Bv[q].x, Dv[q].x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
                                 hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
             }
@@ -3270,9 +3306,9 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
         if (t >= 0 && t < ny) {  // uniform over the wave
             const f2v X0 = Xv[s2 + 1], Xm = Xv[s2], X1 = Xv[s2 + 2];
             const f2v e0 = Ev[s2 + 1], n0 = Nv[s2 + 2], nm = Nv[s2 + 1], cc = Cv[s2], b0 = Bv[s2 + 2];
-            const float ue = __shfl_down(X0.x, 1, WAVE);
-            const float uw = __shfl_up(X0.y, 1, WAVE);
-            const float ew = __shfl_up(e0.y, 1, WAVE);
+            const float ue = lane_down1(X0.x);
+            const float uw = lane_up1(X0.y);
+            const float ew = lane_up1(e0.y);
             float a0 = cc.x * X0.x;
             a0 += e0.x * (X0.x - X0.y);
             if (hW0) a0 += ew * (X0.x - uw);
@@ -3287,7 +3323,7 @@ __global__ __launch_bounds__(256) void k_mg_down_rb2s(int nx, int ny, int nxc, f
                 // P^T: this row's residuals restricted along x (columns
                 // ix-1 .. ix+2), weighted into coarse row iy0 / 2
                 const float q0 = inx ? b0.x - a0 : 0.f, q1 = inx ? b0.y - a1 : 0.f;
-                const float rl = __shfl_up(q1, 1, WAVE), rr = __shfl_down(q0, 1, WAVE);
+                const float rl = lane_up1(q1), rr = lane_down1(q0);
                 float h = 0.25f * rl;
                 h += wx0 * q0;
                 h += wx1 * q1;
@@ -3400,7 +3436,7 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
                 const int J1 = (iy0 >> 1) - 2 + i1;
                 const bool fold = J1 < 0 || J1 > nyc - 1;
                 const float v = fold ? Lc[i0] : 0.75f * Lc[i0] + 0.25f * Lc[i1];
-                const float vl = __shfl_up(v, 1, WAVE), vr = __shfl_down(v, 1, WAVE);
+                const float vl = lane_up1(v), vr = lane_down1(v);
                 const float p0 = (1.f - wl) * v + wl * vl, p1 = (1.f - wr) * v + wr * vr;
                 U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * p0, Xv[q].y + scale * p1} : z2;
             }
@@ -3424,12 +3460,12 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
                 const bool hN = j < ny - 1, hS = j > 0;
                 v = uc;
                 if ((j & 1) == 0) {
-                    const float uE = __shfl_down(uc.x, 1, WAVE);
+                    const float uE = lane_down1(uc.x);
                     v.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true,
                                    ej.x, uc.x, hN, hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f,
                                    hS ? um.y : 0.f);
                 } else {
-                    const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                    const float uW = lane_up1(uc.y), aW = lane_up1(ej.y);
                     v.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f,
                                    hW0 ? uW : 0.f, hN, hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f,
                                    hS ? um.x : 0.f);
@@ -3448,12 +3484,12 @@ __global__ __launch_bounds__(256) void k_mg_up_rb2s(int nx, int ny, int nxc, flo
                 const bool hN = t < ny - 1, hS = t > 0;
                 f2v z = vc;
                 if ((t & 1) == 0) {
-                    const float vW = __shfl_up(vc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                    const float vW = lane_up1(vc.y), aW = lane_up1(ej.y);
                     z.x = rb_relax(bj.x, cj.x, dj.x, omega, vc.x, true, ej.x, vc.y, hW0, hW0 ? aW : 0.f,
                                    hW0 ? vW : 0.f, hN, hN ? nj.x : 0.f, hN ? vp.x : 0.f, hS, hS ? nm.x : 0.f,
                                    hS ? vm.x : 0.f);
                 } else {
-                    const float vE = __shfl_down(vc.x, 1, WAVE);
+                    const float vE = lane_down1(vc.x);
                     z.y = rb_relax(bj.y, cj.y, dj.y, omega, vc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? vE : 0.f, true,
                                    ej.x, vc.x, hN, hN ? nj.y : 0.f, hN ? vp.y : 0.f, hS, hS ? nm.y : 0.f,
                                    hS ? vm.y : 0.f);
@@ -3502,11 +3538,11 @@ __device__ __forceinline__ void rbn_relax_row(f2v (&U)[NR], int q, int j, bool r
     const f2v um = U[q - 1], uc = U[q], up = U[q + 1];
     const bool hN = j < ny - 1, hS = j > 0;
     if (((j & 1) == 0) == red_pass) {
-        const float uW = __shfl_up(uc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+        const float uW = lane_up1(uc.y), aW = lane_up1(ej.y);
         U[q].x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f, hW0 ? uW : 0.f, hN,
                           hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f, hS ? um.x : 0.f);
     } else {
-        const float uE = __shfl_down(uc.x, 1, WAVE);
+        const float uE = lane_down1(uc.x);
         U[q].y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x, uc.x, hN,
                           hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
     }
@@ -3577,11 +3613,11 @@ __global__ __launch_bounds__(256) void k_mg_down_rbn(int nx, int ny, int nxc, fl
             const f2v rm = X[q - 1], rc = X[q], rp = X[q + 1], ej = Ev[q], nj = Nv[q], nm = Nv[q - 1];
             const bool hN = j < ny - 1, hS = j > 0;
             if ((j & 1) == 0) {
-                const float xE = __shfl_down(rc.x, 1, WAVE);
+                const float xE = lane_down1(rc.x);
                 X[q].y = rb_black0(Bv[q].y, Dv[q].y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
                                    hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
             } else {
-                const float xW = __shfl_up(rc.y, 1, WAVE), aW = __shfl_up(ej.y, 1, WAVE);
+                const float xW = lane_up1(rc.y), aW = lane_up1(ej.y);
                 X[q].x = rb_black0(Bv[q].x, Dv[q].x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
                                    hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
             }
@@ -3613,9 +3649,9 @@ __global__ __launch_bounds__(256) void k_mg_down_rbn(int nx, int ny, int nxc, fl
         if (t >= 0 && t < ny) {  // uniform over the wave
             const f2v X0 = X[q], Xm = X[q - 1], X1 = X[q + 1];
             const f2v e0 = Ev[q], n0 = Nv[q], nm = Nv[q - 1], cc = Cv[q], b0 = Bv[q];
-            const float ue = __shfl_down(X0.x, 1, WAVE);
-            const float uw = __shfl_up(X0.y, 1, WAVE);
-            const float ew = __shfl_up(e0.y, 1, WAVE);
+            const float ue = lane_down1(X0.x);
+            const float uw = lane_up1(X0.y);
+            const float ew = lane_up1(e0.y);
             float a0 = cc.x * X0.x;
             a0 += e0.x * (X0.x - X0.y);
             if (hW0) a0 += ew * (X0.x - uw);
@@ -3629,7 +3665,7 @@ __global__ __launch_bounds__(256) void k_mg_down_rbn(int nx, int ny, int nxc, fl
             // P^T: this row's residuals restricted along x (columns ix-1 ..
             // ix+2), weighted into coarse row iy0 / 2
             const float q0 = inx ? b0.x - a0 : 0.f, q1 = inx ? b0.y - a1 : 0.f;
-            const float rl = __shfl_up(q1, 1, WAVE), rr = __shfl_down(q0, 1, WAVE);
+            const float rl = lane_up1(q1), rr = lane_down1(q0);
             float h = 0.25f * rl;
             h += wx0 * q0;
             h += wx1 * q1;
@@ -3714,7 +3750,7 @@ __global__ __launch_bounds__(256) void k_mg_up_rbn(int nx, int ny, int nxc, floa
             const int J1 = (iy0 >> 1) - 2 - G + i1;
             const bool fold = J1 < 0 || J1 > nyc - 1;
             const float v = fold ? Lc[i0] : 0.75f * Lc[i0] + 0.25f * Lc[i1];
-            const float vl = __shfl_up(v, 1, WAVE), vr = __shfl_down(v, 1, WAVE);
+            const float vl = lane_up1(v), vr = lane_down1(v);
             const float p0 = (1.f - wl) * v + wl * vl, p1 = (1.f - wr) * v + wr * vr;
             U[q] = (j >= 0 && j < ny) ? f2v{Xv[q].x + scale * p0, Xv[q].y + scale * p1} : z2;
         }
