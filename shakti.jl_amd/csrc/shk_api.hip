@@ -163,7 +163,11 @@ struct shk_ctx {
     bool use_defer = true;  // FP64 CG + V-cycle: finishing kernels folded into the next kernel
     bool use_adaptive = true;  // batch sizes from the observed convergence rate
     int last_niter = 0;        // CG iterations of the previous solve
-    int batch_margin = 2;      // the first batch stops this many iterations short of last_niter
+    int batch_margin = 0;      // the first batch stops this many iterations short of the prediction
+    // CG iterations of the FP64 solve of each Picard iteration of the last loop
+    // (the first batch of the same iteration of the next loop is sized from it)
+    std::vector<int> niter_by_picard;
+    int picard_it = 0;  // 1-based Picard iteration of the solve under way, 0 outside a loop
     // derived-field refresh: one marching pass (or the two-pass kernels); a
     // Picard iteration that another one follows refreshes only N, K and mdot
     bool derived_two_pass = false;
@@ -839,21 +843,28 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     // kernels of an iteration are folded into the matvec and the update
     const bool defer = sizeof(T) == 8 && multigrid && !nn && ctx->use_defer;
     // plain solves size each batch to the iterations the convergence rate so
-    // far predicts (an even count, 2..bmax), so few iterations are queued
-    // past convergence.  The first batch stops batch_margin iterations short
-    // of the previous solve's count: one more batch costs a host round trip,
-    // about what one iteration queued past convergence costs (its launches
-    // read the done flag and exit).  Deferred finishing publishes the stop
-    // decision one launch later, so its batches hold one iteration more.
+    // far predicts (1..bmax), so few iterations are queued past convergence.
+    // The first batch takes the count of the same Picard iteration of the
+    // previous loop (else of the previous solve), less batch_margin: one more
+    // batch costs a host round trip, about what one iteration queued past
+    // convergence costs (its launches read the done flag and exit, a few
+    // microseconds each).  A deferred batch ends with the stop decision the
+    // next matvec would take (cg_defer_stop), so it needs no iteration beyond
+    // the solve's own.  An odd batch leaves the p (and r) ping-pong buffers in
+    // swapped roles: the next batch starts from the other phase.
     const bool adaptive = !nn && ctx->use_adaptive;
-    auto even_batch = [&](double need) {
-        int b = need <= 2.0 ? 2 : (need >= bmax ? bmax : (int)ceil(need));
-        b += b & 1;
+    auto size_batch = [&](double need) {
+        const int b = need <= 1.0 ? 1 : (need >= bmax ? bmax : (int)ceil(need));
         return b < bmax ? b : bmax;
     };
-    int batch = (adaptive && ctx->last_niter > 0)
-                    ? even_batch((double)ctx->last_niter + (defer ? 1 : 0) - ctx->batch_margin)
-                    : bmax;
+    int predicted = ctx->last_niter;
+    if constexpr (sizeof(T) == 8) {
+        const int k = ctx->picard_it;
+        if (k > 0 && k <= (int)ctx->niter_by_picard.size() && ctx->niter_by_picard[k - 1] > 0)
+            predicted = ctx->niter_by_picard[k - 1];
+    }
+    int batch = (adaptive && predicted > 0) ? size_batch((double)predicted - ctx->batch_margin) : bmax;
+    int phase = 0;  // the ping-pong roles at the start of the next batch
     // ... with the CG update fused into the finest down-leg where that leg can
     // take it (r then alternates between w.r and w.r2; batches are even)
     const bool fuse_upd = defer && zf32 && launch::mg_down_fuses_update(ctx->mg[0].nx, ctx->mg[0].ny) &&
@@ -862,10 +873,10 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                                                       ctx->mg_bf16 ? ctx->mg_nu0 : 1)
                               : 0;
     auto enqueue_batch = [&]() {
-        T* p_old = w.pA;
-        T* p_new = w.pB;
-        T* r_cur = w.r;
-        T* r_nxt = w.r2;
+        T* p_old = phase ? w.pB : w.pA;
+        T* p_new = phase ? w.pA : w.pB;
+        T* r_cur = phase ? w.r2 : w.r;
+        T* r_nxt = phase ? w.r : w.r2;
         for (int k = 0; k < batch; ++k) {
             if constexpr (sizeof(T) == 8) {
                 if (defer) {
@@ -940,6 +951,12 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             p_old = p_new;
             p_new = t;
         }
+        if constexpr (sizeof(T) == 8) {
+            if (defer) {
+                Prof pr(ctx, P_FIN, 0);
+                launch::cg_defer_stop(ctx->n, ctx->sc, ctx->partials, nba, p_old, w.x, ctx->st);
+            }
+        }
     };
     // one graph per batch kind; the learned batch's graph holds the network's
     // kernels too (they run on ctx->st), so it is keyed on the network and
@@ -951,7 +968,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         // elliptic and parabolic systems: part of the key
         const int key[4] = {(int)sizeof(T),
                             precond + (learned_batch ? 16 : 0) + (defer ? 32 : 0) + (ctx->system_parabolic ? 64 : 0) +
-                                (fuse_upd ? 128 : 0),
+                                (fuse_upd ? 128 : 0) + (phase ? 256 : 0),
                             degree * (fused ? 100 : 1), batch};
         for (auto& e : ctx->graphs)
             if (!memcmp(e.key, key, sizeof(key)) && (!learned_batch || e.net == ctx->net)) *out = e.exec;
@@ -989,10 +1006,11 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         else enqueue_batch();
         if (rc_nn) return rc_nn;
         queued += batch;
+        phase ^= batch & 1;
         if (ctx->prof) prof_collect(ctx);
         rc = read_scalars(ctx);
         if (rc) return rc;
-        if (ctx->sc_host->done) break;
+        if (ctx->sc_host->done || ctx->sc_host->done_defer) break;
         if (queued > (long)itmax + batch) break;  // cannot happen: tired sets done
         if (adaptive) {
             const Scalars* h = ctx->sc_host;
@@ -1001,14 +1019,11 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             double need = bmax;
             if (ratio > 0.0 && ratio < 1.0 && h->rNorm > 0.0 && h->eps_stop > 0.0) {
                 const double per_it = log(ratio) / batch;  // log of the mean reduction per iteration
-                need = ceil(log(h->eps_stop / h->rNorm) / per_it) + (defer ? 1 : 0);
+                need = ceil(log(h->eps_stop / h->rNorm) / per_it);
             }
-            const int nb2 = even_batch(need);
-            if (nb2 != batch) {
-                batch = nb2;
-                rc = batch_graph(&exec);
-                if (rc) return rc;
-            }
+            batch = size_batch(need);
+            rc = batch_graph(&exec);  // the size or the phase may have changed
+            if (rc) return rc;
         }
         if (nn && !plain_done) {
             // the plain batch's reduction is the reference; learned batches follow
@@ -1031,6 +1046,13 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         rn_start = ctx->sc_host->rNorm;
     }
     ctx->last_niter = ctx->sc_host->iter;
+    if constexpr (sizeof(T) == 8) {
+        const int k = ctx->picard_it;
+        if (k > 0) {
+            if ((int)ctx->niter_by_picard.size() < k) ctx->niter_by_picard.resize(k, 0);
+            ctx->niter_by_picard[k - 1] = ctx->sc_host->iter;
+        }
+    }
     return SHK_OK;
 }
 
@@ -1792,12 +1814,17 @@ int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every
     int32_t conv = 0, last = 0;
     struct FloorReset {
         shk_ctx* c;
-        ~FloorReset() { c->atol_floor = 0.0; }
+        ~FloorReset() {
+            c->atol_floor = 0.0;
+            c->picard_it = 0;
+        }
     } floor_reset{ctx};
     ctx->atol_floor = 0.0;
     for (int32_t it = 1; it <= iters; ++it) {
         shk_solve_stats st;
+        ctx->picard_it = it;
         int rc = solve(ctx, 0, 0.0, relax_alpha, &st);
+        ctx->picard_it = 0;
         if (rc) return rc;
         if (it == 1 && (ctx->cfg.flags & SHK_CFG_PICARD_ATOL) && std::isfinite(st.rnorm0))
             ctx->atol_floor = ctx->cfg.rtol * st.rnorm0;

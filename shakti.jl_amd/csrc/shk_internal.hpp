@@ -222,6 +222,10 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
 // sums the pending r'z partials (pa), writes p'Ap partials to pb and returns
 // their count, and takes the previous iteration's x += alpha p_old; the
 // update sums both and takes r -= alpha Ap.
+// The stop decision at the end of a batch of deferred iterations (done_defer
+// and the pending x += alpha p_old when the solve stops there).
+void cg_defer_stop(size_t n, Scalars* sc, const double* pa, int nba, const double* p_old, double* x,
+                   hipStream_t st);
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
                        int nba, double* x, hipStream_t st);

@@ -1655,6 +1655,40 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
     block_partial<1, false>(v, partials);
 }
 
+// The end of a batch of deferred CG iterations: the stop decision that the
+// next iteration's CG matvec would take (deferred_gamma on the r'z partials
+// of the batch's last V-cycle).  When the solve stops there, this kernel makes
+// that matvec's writes to the scalars and its pending step x += alpha p_old
+// (the same expression, on the flat index), and raises done_defer, on which
+// the host stops; a batch then needs no iteration past its last V-cycle.
+// When the solve goes on it writes only rNorm (the value the next matvec
+// writes), and the next batch's matvec takes the same decision.  It never
+// writes done, which every workgroup reads on entry.  (64, 4) workgroups,
+// grid-stride over pairs of cells.
+__global__ __launch_bounds__(256) void k_cg_defer_stop(size_t n, Scalars* sc, const double* __restrict__ pa,
+                                                       int nba, const double* __restrict__ p_old,
+                                                       double* __restrict__ x) {
+    if (sc->done || !sc->gpend) return;
+    const double g = sum_partials(pa, nba);
+    const double rn = sqrt(g);
+    // the same tests as deferred_gamma; iter is written below only when every
+    // workgroup stops, and then a later reader of it stops as well
+    const bool stop = gamma_breaks(g) || rn <= sc->eps_stop || rn + 1.0 <= 1.0 || sc->iter + 1 >= sc->itmax;
+    if (!stop) {
+        if (lead_thread()) sc->rNorm = rn;
+        return;
+    }
+    const double alpha = sc->alpha;
+    double beta;
+    if (deferred_gamma<true>(sc, pa, nba, beta) == 2) return;  // breakdown: no step, as in the matvec
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    for (size_t i = 2 * ((size_t)blockIdx.x * WAVE * WAVES + tid); i < n; i += 2 * stride) {
+        x[i] = x[i] + alpha * p_old[i];
+        if (i + 1 < n) x[i + 1] = x[i + 1] + alpha * p_old[i + 1];
+    }
+}
+
 // x += alpha p; r -= alpha Ap; with Jacobi also z = r ./ d and gamma = r'z.
 // DEFER: every workgroup sums p'Ap (pb, nbb) and r'z (pa, nba) as k_finish
 // would, applies the zero-curvature test and alpha = r'z / p'Ap (FIN_PAP);
@@ -4971,6 +5005,11 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
         hipLaunchKernelGGL((k_cg_matvec2<false, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
                            z, p_old, p_new, Ap, pb, sc, pa, nba, x);
     return W.nblocks;
+}
+void cg_defer_stop(size_t n, Scalars* sc, const double* pa, int nba, const double* p_old, double* x,
+                   hipStream_t st) {
+    const unsigned nblk = (unsigned)std::min<size_t>(1024, (n + 511) / 512);
+    hipLaunchKernelGGL(k_cg_defer_stop, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, sc, pa, nba, p_old, x);
 }
 void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
                         const double* pb, int nbb, hipStream_t st) {
