@@ -4102,19 +4102,37 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
             auto xk = [&](size_t k) -> float { return x[k]; };
             if ((T.lin_mask >> l) & 1) {
                 // P^T (b - A x): each fine row restricted along x into Hs
-                // (cnx x fny), then the rows combined with the y weights
-                for (int q = tid; q < cnx * fny; q += nt) {
-                    const int iy = dcx.quot(q), I = q - iy * cnx;
+                // (cnx x fny), then the rows combined with the y weights.
+                // Thread q = (I, iy) forms the residuals of fine cells 2I and
+                // 2I+1; those of 2I-1 and 2I+2 come from the lanes of I-1 and
+                // I+1 (q-1, q+1), or are formed here at a wave's edge.  Each
+                // term is the same value, added in the same order, as when
+                // every thread formed its four.  Every lane runs every trip,
+                // so the lane shifts read active lanes.
+                auto res = [&](int ix, int iy) -> float {
+                    const size_t k = (size_t)ix + (size_t)iy * fnx;
+                    return b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xk);
+                };
+                const int nq = cnx * fny;
+                for (int q0 = 0; q0 < nq; q0 += nt) {
+                    const int q = q0 + tid;
+                    const bool act = q < nq;
+                    const int qc = act ? q : nq - 1;
+                    const int iy = dcx.quot(qc), I = qc - iy * cnx;
+                    const int lane = tid & (WAVE - 1);
+                    const float ra = res(2 * I, iy);
+                    const float rb = 2 * I + 1 < fnx ? res(2 * I + 1, iy) : 0.f;
+                    float rm = lane_up1(rb), rp = lane_down1(ra);
+                    const float w0 = lin_rw(2 * I - 1, 0, fnx), w1 = lin_rw(2 * I, 1, fnx);
+                    const float w2 = lin_rw(2 * I + 1, 2, fnx), w3 = lin_rw(2 * I + 2, 3, fnx);
+                    if (w0 != 0.f && (lane == 0 || q == q0)) rm = res(2 * I - 1, iy);
+                    if (w3 != 0.f && (lane == WAVE - 1 || qc != q)) rp = res(2 * I + 2, iy);
                     float h = 0.f;
-#pragma unroll
-                    for (int p = 0; p < 4; ++p) {
-                        const int ix = 2 * I - 1 + p;
-                        const float wx = lin_rw(ix, p, fnx);
-                        if (wx == 0.f) continue;
-                        const size_t k = (size_t)ix + (size_t)iy * fnx;
-                        h += wx * (b[k] - flux_apply(fnx, fny, ix, iy, k, c, e, n, xk));
-                    }
-                    Hs[q] = h;
+                    if (w0 != 0.f) h += w0 * rm;
+                    if (w1 != 0.f) h += w1 * ra;
+                    if (w2 != 0.f) h += w2 * rb;
+                    if (w3 != 0.f) h += w3 * rp;
+                    if (act) Hs[q] = h;
                 }
                 __syncthreads();
                 for (int q = tid; q < cnx * cny; q += nt) {
@@ -4169,17 +4187,19 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
     }
     {
         // coarsest: x = inv b with the symmetric inverse read by columns;
-        // 16 threads per row, 4 products each, summed in a fixed order
+        // 16 threads per row (one per wave, the lanes of a wave on 64
+        // consecutive rows, so a wave reads 64 consecutive floats), 4
+        // products each, the 16 parts summed in a fixed order
         const float* b = Bs + off[lc];
-        const int i = tid >> 4, part = tid & 15;
+        const int i = tid & (WAVE - 1), part = tid >> 6;
         float acc = 0.f;
         if (i < ncell)
             for (int j = part * 4; j < part * 4 + 4 && j < ncell; ++j) acc += Is[j * ncell + i] * b[j];
-        red[tid] = acc;
+        red[part * WAVE + i] = acc;
         __syncthreads();
         if (part == 0 && i < ncell) {
             float s = 0.f;
-            for (int q = 0; q < 16; ++q) s += red[tid + q];
+            for (int q = 0; q < 16; ++q) s += red[q * WAVE + i];
             Xs[off[lc] + i] = s;
         }
         __syncthreads();
@@ -4238,15 +4258,27 @@ __global__ __launch_bounds__(1024) void k_mg_tail(MGTail T, const Scalars* __res
         auto us = [&](size_t k) -> float { return x[k]; };
         if (T.rb) {
             // black cells relax from u in place (they read only red neighbours),
-            // then red cells from the relaxed black ones
+            // then red cells from the relaxed black ones; on even widths each
+            // pass takes only its own colour's cells (hx per row)
+            auto relax_at = [&](int k, int ix, int iy) {
+                const bool hE = ix < fnx - 1, hW = ix > 0, hN = iy < fny - 1, hS = iy > 0;
+                x[k] = rb_relax(b[k], c[k], D[k], omega, x[k], hE, hE ? e[k] : 0.f, hE ? x[k + 1] : 0.f, hW,
+                                hW ? e[k - 1] : 0.f, hW ? x[k - 1] : 0.f, hN, hN ? n[k] : 0.f,
+                                hN ? x[k + fnx] : 0.f, hS, hS ? n[k - fnx] : 0.f, hS ? x[k - fnx] : 0.f);
+            };
+            const int hx = fnx >> 1;
+            const FastDiv dh(hx > 0 ? hx : 1);
             for (int colour = 1; colour >= 0; --colour) {
-                for (int k = tid; k < nf; k += nt) {
-                    const int iy = dfx.quot(k), ix = k - iy * fnx;
-                    if (((ix + iy) & 1) != colour) continue;
-                    const bool hE = ix < fnx - 1, hW = ix > 0, hN = iy < fny - 1, hS = iy > 0;
-                    x[k] = rb_relax(b[k], c[k], D[k], omega, x[k], hE, hE ? e[k] : 0.f, hE ? x[k + 1] : 0.f, hW,
-                                    hW ? e[k - 1] : 0.f, hW ? x[k - 1] : 0.f, hN, hN ? n[k] : 0.f,
-                                    hN ? x[k + fnx] : 0.f, hS, hS ? n[k - fnx] : 0.f, hS ? x[k - fnx] : 0.f);
+                if ((fnx & 1) == 0) {
+                    for (int p = tid; p < hx * fny; p += nt) {
+                        const int iy = dh.quot(p), ix = 2 * (p - iy * hx) + ((iy + colour) & 1);
+                        relax_at(ix + iy * fnx, ix, iy);
+                    }
+                } else {
+                    for (int k = tid; k < nf; k += nt) {
+                        const int iy = dfx.quot(k), ix = k - iy * fnx;
+                        if (((ix + iy) & 1) == colour) relax_at(k, ix, iy);
+                    }
                 }
                 __syncthreads();
             }
