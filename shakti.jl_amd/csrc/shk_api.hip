@@ -59,6 +59,12 @@ struct shk_ctx {
     Phys P{};
     Geom G{};
     hipStream_t st = nullptr;
+    // the coarsest multigrid inverse is formed on st_side, beside the work
+    // that precedes the first V-cycle tail (one workgroup would hold the
+    // whole chip on st); st waits on inv_done before the next tail
+    hipStream_t st_side = nullptr;
+    hipEvent_t inv_fork = nullptr, inv_done = nullptr;
+    bool inv_pending = false;
     int device = 0;
     size_t n = 0, nxf = 0, nyf = 0;
 
@@ -168,6 +174,9 @@ struct shk_ctx {
     // (the first batch of the same iteration of the next loop is sized from it)
     std::vector<int> niter_by_picard;
     int picard_it = 0;  // 1-based Picard iteration of the solve under way, 0 outside a loop
+    // the Picard loop asks the solve to take the Picard test in its epilogue
+    // (picard_fuse); the solve says whether it did (picard_fused)
+    bool picard_fuse = false, picard_fused = false;
     // derived-field refresh: one marching pass (or the two-pass kernels); a
     // Picard iteration that another one follows refreshes only N, K and mdot
     bool derived_two_pass = false;
@@ -327,17 +336,17 @@ int check_cfg(shk_ctx* ctx, const shk_solver_cfg* cfg) {
 }
 
 // Upload the host-side scalars the next solve starts from.
+// The device writes the reset fields itself (no host round trip); the host
+// mirror takes the same values and is refreshed whole by the next read.
 int reset_scalars(shk_ctx* ctx, double atol, double rtol, int itmax) {
-    HIPCHK(hipStreamSynchronize(ctx->st));
     Scalars* h = ctx->sc_host;
-    HIPCHK(hipMemcpy(h, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     h->done = 0; h->solved = 0; h->zero_curv = 0; h->tired = 0; h->iter = 0;
     h->atol = atol; h->rtol = rtol; h->itmax = itmax;
     h->counter = 0;
     h->gpend = 0;
     h->done_defer = 0;
     h->breakdown = 0;
-    HIPCHK(hipMemcpyAsync(ctx->sc, h, sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
+    launch::reset_scalars(ctx->sc, atol, rtol, itmax, ctx->st);
     return SHK_OK;
 }
 
@@ -492,6 +501,14 @@ int mg_prepare_fine(shk_ctx* ctx) {
 
 // fine_ready: the finest level's BF16 coefficients were written already (by
 // the assembly, from the FP32 values w holds).
+// st waits for the coarsest inverse formed on the side stream, if one is due.
+int mg_join_inverse(shk_ctx* ctx) {
+    if (!ctx->inv_pending) return SHK_OK;
+    HIPCHK(hipStreamWaitEvent(ctx->st, ctx->inv_done, 0));
+    ctx->inv_pending = false;
+    return SHK_OK;
+}
+
 int mg_setup(shk_ctx* ctx, const Work<float>& w, bool fine_ready = false) {
     const int nx0 = ctx->g.nx, ny0 = ctx->g.ny;
     if (ctx->mg.empty()) {
@@ -594,6 +611,8 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w, bool fine_ready = false) {
             launch::to_bf16x4((size_t)L0.nx * L0.ny, L0.c, L0.e, L0.n, L0.D, ctx->mg_c16, ctx->mg_e16, ctx->mg_n16,
                               ctx->mg_D16, ctx->st);
     }
+    // the coarsening rewrites what an inverse still running on the side stream reads
+    if (int rc = mg_join_inverse(ctx)) return rc;
     for (size_t l = 1; l < ctx->mg.size(); ++l) {
         auto& f = ctx->mg[l - 1];
         auto& c = ctx->mg[l];
@@ -604,10 +623,21 @@ int mg_setup(shk_ctx* ctx, const Work<float>& w, bool fine_ready = false) {
         launch::mg_diag(c.nx, c.ny, c.c, c.e, c.n, c.D, ctx->st);
     }
     auto& Lc = ctx->mg.back();
-    Prof pr(ctx, P_MGS, 0);
-    launch::mg_dense_inverse(Lc.nx, Lc.ny, Lc.c, Lc.e, Lc.n, ctx->mg_inv, ctx->st);
+    if (ctx->prof || !ctx->st_side) {
+        Prof pr(ctx, P_MGS, 0);
+        launch::mg_dense_inverse(Lc.nx, Lc.ny, Lc.c, Lc.e, Lc.n, ctx->mg_inv, ctx->st);
+        return SHK_OK;
+    }
+    // on the side stream, after the coarsest level's coefficients; the next
+    // tail waits for it (mg_join_inverse)
+    HIPCHK(hipEventRecord(ctx->inv_fork, ctx->st));
+    HIPCHK(hipStreamWaitEvent(ctx->st_side, ctx->inv_fork, 0));
+    launch::mg_dense_inverse(Lc.nx, Lc.ny, Lc.c, Lc.e, Lc.n, ctx->mg_inv, ctx->st_side);
+    HIPCHK(hipEventRecord(ctx->inv_done, ctx->st_side));
+    ctx->inv_pending = true;
     return SHK_OK;
 }
+
 
 // z = V(r): one V-cycle (one damped-Jacobi sweep down and up on every level,
 // dense solve on the coarsest), finishing gamma = r'z (mode 0 set-up, 1
@@ -702,6 +732,8 @@ void mg_apply(shk_ctx* ctx, const T* r, T* z, int mode, float* zf = nullptr, con
             bytes += 4.0 * 13.0 * v.nx * v.ny;
         }
         Prof pr(ctx, P_MGC, bytes);
+        // a failed wait is reported by the entry point's hipGetLastError check
+        (void)mg_join_inverse(ctx);
         launch::mg_tail(Tl, ctx->sc, ctx->st);
     }
     const T* r0 = r;  // the finest level's right-hand side on the way up
@@ -976,6 +1008,7 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         shk_ctx::GraphEntry e;
         memcpy(e.key, key, sizeof(key));
         e.net = learned_batch ? ctx->net : nullptr;
+        if (int rc = mg_join_inverse(ctx)) return rc;  // no event from outside the capture inside it
         HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
         enqueue_batch();
         HIPCHK(hipStreamEndCapture(ctx->st, &e.graph));
@@ -1305,13 +1338,17 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     // into the alternate buffer, which becomes h only when the residual is finite
     double* h_prev = ctx->F.f[SHK_H];
     const bool fused_finish = x0 == h_prev && !(ctx->rec_kind == 1 && ctx->rec_x);
+    // ... and, for a Picard loop, the Picard test of that head against h
+    const bool fuse_pic = fused_finish && ctx->picard_fuse;
     {
         Prof pr(ctx, P_FIN, (fused_finish ? 7 : 6) * nb);
         launch::residual_norms(G, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, x0, ctx->partials, &ctx->sc->tres,
-                               ctx->st, fused_finish ? ctx->h_alt : nullptr, relax);
+                               ctx->st, fused_finish ? ctx->h_alt : nullptr, relax,
+                               fuse_pic ? ctx->partials2 : nullptr, ctx->sc);
     }
     rc = read_scalars(ctx);
     if (rc) return rc;
+    ctx->picard_fused = fuse_pic;
     {
         const Scalars* h = ctx->sc_host;
         sst.true_residual = h->tres_b > 0.0 ? h->tres / h->tres_b : h->tres;
@@ -1473,6 +1510,9 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
         return SHK_ERR_NODEVICE;
     }
     HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    if (!getenv("SHK_NO_SIDE_INVERSE")) HIPCHK(hipStreamCreateWithFlags(&ctx->st_side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ctx->inv_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ctx->inv_done, hipEventDisableTiming));
     ctx->use_graphs = getenv("SHK_NO_GRAPHS") == nullptr;
     ctx->use_fused = getenv("SHK_NO_FUSED") == nullptr;
     ctx->use_defer = getenv("SHK_NO_DEFER") == nullptr;
@@ -1530,6 +1570,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
 int shk_destroy(shk_ctx* ctx) {
     if (!ctx) return SHK_ERR_ARG;
     if (ctx->st) hipStreamSynchronize(ctx->st);
+    if (ctx->st_side) hipStreamSynchronize(ctx->st_side);
     shkio::destroy(ctx->io);  // joins a running checkpoint writer
     for (int f = 0; f < SHK_NUM_FIELDS; ++f) hipFree(ctx->F.f[f]);
     void* ptrs[] = {ctx->h_alt, ctx->moulin, ctx->slide_cx, ctx->slide_cy, ctx->r2, ctx->r2_32, ctx->cP, ctx->dg32, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->x, ctx->r, ctx->z,
@@ -1564,6 +1605,9 @@ int shk_destroy(shk_ctx* ctx) {
     for (auto& pe : ctx->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
     if (ctx->st) hipStreamDestroy(ctx->st);
+    if (ctx->st_side) hipStreamDestroy(ctx->st_side);
+    if (ctx->inv_fork) hipEventDestroy(ctx->inv_fork);
+    if (ctx->inv_done) hipEventDestroy(ctx->inv_done);
     delete ctx;
     return SHK_OK;
 }
@@ -1817,26 +1861,37 @@ int shk_picard_loop(shk_ctx* ctx, int32_t iters, double tol, int32_t check_every
         ~FloorReset() {
             c->atol_floor = 0.0;
             c->picard_it = 0;
+            c->picard_fuse = false;
         }
     } floor_reset{ctx};
     ctx->atol_floor = 0.0;
+    // the tolerance of the Picard test, on the device for the whole loop
+    ctx->sc_host->picard_tol = tol;
+    HIPCHK(hipMemcpyAsync(&ctx->sc->picard_tol, &ctx->sc_host->picard_tol, sizeof(double), hipMemcpyHostToDevice,
+                          ctx->st));
     for (int32_t it = 1; it <= iters; ++it) {
         shk_solve_stats st;
+        const int check = (it % check_every == 0) || (it == iters);
         ctx->picard_it = it;
+        ctx->picard_fuse = check;
+        ctx->picard_fused = false;
         int rc = solve(ctx, 0, 0.0, relax_alpha, &st);
         ctx->picard_it = 0;
+        ctx->picard_fuse = false;
         if (rc) return rc;
         if (it == 1 && (ctx->cfg.flags & SHK_CFG_PICARD_ATOL) && std::isfinite(st.rnorm0))
             ctx->atol_floor = ctx->cfg.rtol * st.rnorm0;
         if (cg_iters) *cg_iters += st.niter;
         // The test reads only the heads, so it runs before the refresh: the
         // refresh after the loop's last iteration writes every field, the
-        // others only what the next assembly reads.
-        const int check = (it % check_every == 0) || (it == iters);
+        // others only what the next assembly reads.  The solve takes it in
+        // its epilogue where it can (the flag then came with its last read).
         bool done = false;
         if (check) {
-            rc = picard_test(ctx, ctx->h_alt /* previous h */, tol);
-            if (rc) return rc;
+            if (!ctx->picard_fused) {
+                rc = picard_test(ctx, ctx->h_alt /* previous h */, tol);
+                if (rc) return rc;
+            }
             // a solve that stopped short (itmax, zero curvature) leaves a head
             // change that says nothing about Picard convergence
             done = ctx->sc_host->picard_converged && !st.zero_curvature && !st.tired;

@@ -200,9 +200,13 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
 // null) into out[0], out[1] (device memory); partials: 2 x G.nblocks doubles
 // ||b - A(x + x0)||^2 and ||b||^2 into out[0..1] (x0 may be null); with
 // h_out, also h_out = x + x0 (under-relaxed towards x0 when relax > 0)
+// ppart (with h_out and sc): also the Picard test of h_out against x0, the
+// flag into sc (partials of the same count as the norms').
 void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
                     const double* x, const double* x0, double* partials, double* out, hipStream_t st,
-                    double* h_out = nullptr, double relax = 0.0);
+                    double* h_out = nullptr, double relax = 0.0, double* ppart = nullptr, Scalars* sc = nullptr);
+// The fields a solve starts from (done, iter, tolerances, ...), reset on the device.
+void reset_scalars(Scalars* sc, double atol, double rtol, int itmax, hipStream_t st);
 
 // Preconditioned CG pieces (mode 0: set-up, mode 1: iteration).
 template <class T>

@@ -1995,24 +1995,40 @@ __global__ __launch_bounds__(256) void k_gersh(int nx, int ny, const T* __restri
 // h_out: where given, also the returned head h = x + x0 of every cell (the
 // operand at the cell), under-relaxed towards x0 when relax > 0 (as
 // k_cg_finish writes it)
+// With ppart (and h_out) it also takes the Picard test of the head it writes
+// against x0, the previous head (the maxima of k_picard_check, per workgroup).
 __global__ __launch_bounds__(256) void k_true_resid(int nx, int ny, int R, const double* __restrict__ aP,
                                                     const double* __restrict__ aE, const double* __restrict__ aN,
                                                     const double* __restrict__ b, const double* __restrict__ x,
                                                     const double* __restrict__ x0, double* partials,
-                                                    double* __restrict__ h_out, double relax) {
+                                                    double* __restrict__ h_out, double relax, double* ppart) {
     int vbx, vby;
     virtual_block(vbx, vby);
-    double ar = 0.0, ab = 0.0;
+    double ar = 0.0, ab = 0.0, dmax = 0.0, hmax = 0.0;
+    bool any_nan = false;
     march<double>(nx, ny, R, vbx, vby, aP, aE, aN, [&](size_t k) -> double { return x0 ? x[k] + x0[k] : x[k]; },
                   [&](size_t c, double Au, double u) {
                       const double bc = b[c];
                       const double r = bc - Au;
                       ar += r * r;
                       ab += bc * bc;
-                      if (h_out) h_out[c] = relax > 0.0 ? relax * u + (1.0 - relax) * x0[c] : u;
+                      if (h_out) {
+                          const double hn = relax > 0.0 ? relax * u + (1.0 - relax) * x0[c] : u;
+                          h_out[c] = hn;
+                          if (ppart) {
+                              const double d = fabs(hn - x0[c]), a = fabs(hn);
+                              any_nan |= isnan(d) || isnan(a);
+                              dmax = d > dmax ? d : dmax;
+                              hmax = a > hmax ? a : hmax;
+                          }
+                      }
                   });
     double v[2] = {ar, ab};
     block_partial<2, false>(v, partials);
+    if (ppart) {
+        double w[2] = {any_nan ? NAN : dmax, hmax};
+        block_partial<2, true>(w, ppart);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_ir_correct(size_t n, double* __restrict__ x, const float* __restrict__ d) {
@@ -4616,14 +4632,48 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
 
 // Two norms from the partials of sums of squares (k_true_resid):
 // out[0] = sqrt(total 0), out[1] = sqrt(total 1).
+// The Picard flag from max|h - h_prev| (v0) and max|h| (v1)
+// (src/elliptic_solver.jl:153-169).
+__device__ __forceinline__ void picard_flag(Scalars* sc, double v0, double v1) {
+    sc->dmax = v0;
+    sc->hmax = v1;
+    sc->picard_nan = isnan(v0) || isnan(v1);
+    sc->picard_converged = (!isnan(v0) && !isnan(v1) && v0 / (v1 + EPS64) < sc->picard_tol) ? 1 : 0;
+}
+
+// ppart: also the Picard test's maxima of k_true_resid, into sc.
 __global__ __launch_bounds__(256) void k_finish_norms(const double* __restrict__ partials, int nb,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, const double* __restrict__ ppart,
+                                                      Scalars* sc) {
     double acc[2];
     reduce_partials(partials, nb, 2, false, acc);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         out[0] = sqrt(acc[0]);
         out[1] = sqrt(acc[1]);
     }
+    if (ppart) {
+        double m[2];
+        reduce_partials(ppart, nb, 2, true, m);
+        if (threadIdx.x == 0 && threadIdx.y == 0) picard_flag(sc, m[0], m[1]);
+    }
+}
+
+// The per-solve reset of the device scalars (the fields a solve starts
+// from), written on the device so that a solve's start needs no host round
+// trip.  One thread.
+__global__ void k_reset_scalars(Scalars* sc, double atol, double rtol, int itmax) {
+    sc->done = 0;
+    sc->solved = 0;
+    sc->zero_curv = 0;
+    sc->tired = 0;
+    sc->iter = 0;
+    sc->atol = atol;
+    sc->rtol = rtol;
+    sc->itmax = itmax;
+    sc->counter = 0;
+    sc->gpend = 0;
+    sc->done_defer = 0;
+    sc->breakdown = 0;
 }
 
 // One workgroup: sums (or maximises) the nb partials of the preceding
@@ -4709,12 +4759,7 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
             break;
         }
         case FIN_GERSH: sc->gersh = v0; break;
-        case FIN_PICARD:
-            sc->dmax = v0;
-            sc->hmax = v1;
-            sc->picard_nan = isnan(v0) || isnan(v1);
-            sc->picard_converged = (!isnan(v0) && !isnan(v1) && v0 / (v1 + EPS64) < sc->picard_tol) ? 1 : 0;
-            break;
+        case FIN_PICARD: picard_flag(sc, v0, v1); break;
         default: break;
     }
 }
@@ -4955,10 +5000,14 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
 }
 void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
                     const double* x, const double* x0, double* partials, double* out, hipStream_t st, double* h_out,
-                    double relax) {
+                    double relax, double* ppart, Scalars* sc) {
+    if (!h_out || !sc) ppart = nullptr;
     hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials, h_out,
-                       relax);
-    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(WAVE, WAVES), 0, st, partials, G.nblocks, out);
+                       relax, ppart);
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(WAVE, WAVES), 0, st, partials, G.nblocks, out, ppart, sc);
+}
+void reset_scalars(Scalars* sc, double atol, double rtol, int itmax, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1), 0, st, sc, atol, rtol, itmax);
 }
 template <class T>
 void cg_init(const Geom& G, const T* aP, const T* aE, const T* aN, const T* dg, const T* b, const T* x0, T* r, T* z,
