@@ -499,8 +499,6 @@ int mg_prepare_fine(shk_ctx* ctx) {
     return SHK_OK;
 }
 
-// fine_ready: the finest level's BF16 coefficients were written already (by
-// the assembly, from the FP32 values w holds).
 // st waits for the coarsest inverse formed on the side stream, if one is due.
 int mg_join_inverse(shk_ctx* ctx) {
     if (!ctx->inv_pending) return SHK_OK;
@@ -509,6 +507,8 @@ int mg_join_inverse(shk_ctx* ctx) {
     return SHK_OK;
 }
 
+// fine_ready: the finest level's BF16 coefficients were written already (by
+// the assembly, from the FP32 values w holds).
 int mg_setup(shk_ctx* ctx, const Work<float>& w, bool fine_ready = false) {
     const int nx0 = ctx->g.nx, ny0 = ctx->g.ny;
     if (ctx->mg.empty()) {
