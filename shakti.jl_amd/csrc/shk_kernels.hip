@@ -366,13 +366,14 @@ __device__ __forceinline__ void qre(const Phys& P, double bf, double grad, doubl
 //     N C (v / (v + |N|^n lambda))^(1/n) ub / v, 0 where v = 0;
 //   V0 (:509-543): N C (v / (v + v0))^(1/n) ub / v, 0 where v = 0;
 //   linear (:590-602): C^2 N ub, C^2 the square of the staggered C.
-__device__ __forceinline__ double taub_face(const Phys& P, double Nf, double abs_v, double lf, double ub, double Cf) {
-    switch (P.sliding_law) {
+__device__ __forceinline__ double taub_face(const Phys& P, int law, double Nf, double abs_v, double lf, double ub,
+                                           double Cf) {
+    switch (law) {
         case SHK_SLIDING_LINEAR: return (Cf * Cf) * Nf * ub;
         case SHK_SLIDING_REGULARIZED_COULOMB_V0:
             return abs_v > 0.0 ? Nf * Cf * cpow(abs_v / (abs_v + P.v0), P.inv_n_exp) * (ub / abs_v) : 0.0;
         default: {
-            const double C = P.sliding_law == SHK_SLIDING_REGULARIZED_COULOMB_FIELD ? Cf : P.C;
+            const double C = law == SHK_SLIDING_REGULARIZED_COULOMB_FIELD ? Cf : P.C;
             if (abs_v > 0.0)
                 return Nf * C * cpow(abs_v / (abs_v + cpow(fabs(Nf), P.n_exp) * lf), P.inv_n_exp) * (ub / abs_v);
             return 0.0;
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
                 lf = (s.lambda[ce] + s.lambda[cw]) / 2.0;
                 av = (s.abs_ub[ce] + s.abs_ub[cw]) / 2.0;
             }
-            s.taub_x[k] = taub_face(P, Nf, av, lf, s.ub_x[k], P.Cx ? P.Cx[k] : 0.0);
+            s.taub_x[k] = taub_face(P, P.sliding_law, Nf, av, lf, s.ub_x[k], P.Cx ? P.Cx[k] : 0.0);
         }
     }
     // y face fy = iy of column ix
@@ -490,7 +491,7 @@ __global__ __launch_bounds__(256) void k_derived_faces(Phys P, Fields F) {
                 lf = (s.lambda[cn] + s.lambda[cs]) / 2.0;
                 av = (s.abs_ub[cn] + s.abs_ub[cs]) / 2.0;
             }
-            s.taub_y[k] = taub_face(P, Nf, av, lf, s.ub_y[k], P.Cy ? P.Cy[k] : 0.0);
+            s.taub_y[k] = taub_face(P, P.sliding_law, Nf, av, lf, s.ub_y[k], P.Cy ? P.Cy[k] : 0.0);
         }
     }
 }
@@ -585,13 +586,20 @@ __device__ __forceinline__ DProd dprod(const DFace& f) {
     return p;
 }
 
-template <int MODE>
+// LAW: the sliding law as a compile-time constant (the kernel is instantiated
+// per law, so each holds only its own stress code), or -1 for P.sliding_law.
+template <int LAW>
+__device__ __forceinline__ int law_of(const Phys& P) {
+    return LAW >= 0 ? LAW : P.sliding_law;
+}
+
+template <int MODE, int LAW>
 __device__ __forceinline__ DCell dcell(const Phys& P, const FV& s, double rg, size_t c) {
     DCell v;
     v.h = s.h[c];
     v.pw = MODE == 1 ? s.pw[c] : rg * (v.h - s.zb[c]);
     v.N = jl_boolmul(s.mask[c] == GROUNDED, s.po[c] - v.pw);
-    const bool slide = P.sliding_law != SHK_SLIDING_PRESCRIBED;
+    const bool slide = law_of<LAW>(P) != SHK_SLIDING_PRESCRIBED;
     v.ub = slide ? s.abs_ub[c] : 0.0;
     v.lam = slide ? s.lambda[c] : 0.0;
     return v;
@@ -601,6 +609,7 @@ __device__ __forceinline__ DCell dcell(const Phys& P, const FV& s, double rg, si
 // spacing across it.  edge -1: the domain's first face (only hi exists);
 // +1: its last face (only lo exists); 0: an inner face.  Boundary faces keep
 // their stored gradients, as in k_derived_faces.
+template <int LAW>
 __device__ __forceinline__ DFace dface(const Phys& P, int edge, const DCell& lo, const DCell& hi, double d,
                                        const double* __restrict__ dh_a, const double* __restrict__ dp_a,
                                        const double* __restrict__ b_a, const double* __restrict__ ub_a,
@@ -617,7 +626,8 @@ __device__ __forceinline__ DFace dface(const Phys& P, int edge, const DCell& lo,
     }
     qre(P, b_a[k], f.dh, f.q, f.Re);
     f.ub = ub_a[k];
-    if (P.sliding_law != SHK_SLIDING_PRESCRIBED) {
+    const int law = law_of<LAW>(P);
+    if (law != SHK_SLIDING_PRESCRIBED) {
         double Nf, av, lf;
         if (edge < 0) {
             Nf = hi.N; av = hi.ub; lf = hi.lam;
@@ -628,13 +638,14 @@ __device__ __forceinline__ DFace dface(const Phys& P, int edge, const DCell& lo,
             lf = (hi.lam + lo.lam) / 2.0;
             av = (hi.ub + lo.ub) / 2.0;
         }
-        f.tb = taub_face(P, Nf, av, lf, f.ub, C_a ? C_a[k] : 0.0);
+        f.tb = taub_face(P, law, Nf, av, lf, f.ub, C_a ? C_a[k] : 0.0);
     } else {
         f.tb = tb_a[k];
     }
     return f;
 }
 
+template <int LAW>
 __device__ __forceinline__ void store_face(const Phys& P, size_t k, const DFace& f, bool inner, double* dh_a,
                                            double* dp_a, double* q_a, double* Re_a, double* tb_a) {
     if (inner) {
@@ -643,7 +654,7 @@ __device__ __forceinline__ void store_face(const Phys& P, size_t k, const DFace&
     }
     q_a[k] = f.q;
     Re_a[k] = f.Re;
-    if (P.sliding_law != SHK_SLIDING_PRESCRIBED) tb_a[k] = f.tb;
+    if (law_of<LAW>(P) != SHK_SLIDING_PRESCRIBED) tb_a[k] = f.tb;
 }
 
 __device__ __forceinline__ DProd prod_from_east(const DProd& f, bool sens) {
@@ -664,7 +675,7 @@ __device__ __forceinline__ DCell cell_from_west(const DCell& c) {
     return g;
 }
 
-template <int MODE, bool FULL>
+template <int MODE, bool FULL, int LAW>
 __global__ __launch_bounds__(256) void k_derived_march(Phys P, Fields F, int R) {
     const int nx = P.nx, ny = P.ny;
     int vbx, vby;
@@ -682,36 +693,36 @@ __global__ __launch_bounds__(256) void k_derived_march(Phys P, Fields F, int R) 
     const int ixf = min(ix, nx);                                   // clamped x face
     const int xedge = ix == 0 ? -1 : (ix >= nx ? 1 : 0);
     const bool sens = P.with_sensible;
-    DCell cur = dcell<MODE>(P, s, rg, cidx(ixc, iy0, nx));
+    DCell cur = dcell<MODE, LAW>(P, s, rg, cidx(ixc, iy0, nx));
     DProd ps;  // the south face of the current row
     {
         const size_t k = cidx(ixc, iy0, nx);
         DFace fs;
         if (iy0 == 0) {
-            fs = dface(P, -1, cur, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
-            if (FULL && own) store_face(P, k, fs, false, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
+            fs = dface<LAW>(P, -1, cur, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
+            if (FULL && own) store_face<LAW>(P, k, fs, false, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
         } else {
-            const DCell below = dcell<MODE>(P, s, rg, cidx(ixc, iy0 - 1, nx));
-            fs = dface(P, 0, below, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
+            const DCell below = dcell<MODE, LAW>(P, s, rg, cidx(ixc, iy0 - 1, nx));
+            fs = dface<LAW>(P, 0, below, cur, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, k);
         }
         ps = dprod(fs);
     }
     for (int iy = iy0; iy < iy1; ++iy) {
-        const DCell nxt = iy + 1 < ny ? dcell<MODE>(P, s, rg, cidx(ixc, iy + 1, nx)) : cur;
+        const DCell nxt = iy + 1 < ny ? dcell<MODE, LAW>(P, s, rg, cidx(ixc, iy + 1, nx)) : cur;
         DCell west = cell_from_west(cur);
-        if (lane == 0 && ix > 0) west = dcell<MODE>(P, s, rg, cidx(ix - 1, iy, nx));
+        if (lane == 0 && ix > 0) west = dcell<MODE, LAW>(P, s, rg, cidx(ix - 1, iy, nx));
         const size_t kx = xidx(ixf, iy, nx), ky = cidx(ixc, iy + 1, nx);
         DProd pw_, pe, pn;
         {
-            const DFace fw = dface(P, xedge, west, cur, P.dx, s.dhdx, s.dpwdx, s.b_x, s.ub_x, s.valid_x, s.taub_x, P.Cx, kx);
-            if (FULL && xstore) store_face(P, kx, fw, xedge == 0, s.dhdx, s.dpwdx, s.q_x, s.Re_x, s.taub_x);
+            const DFace fw = dface<LAW>(P, xedge, west, cur, P.dx, s.dhdx, s.dpwdx, s.b_x, s.ub_x, s.valid_x, s.taub_x, P.Cx, kx);
+            if (FULL && xstore) store_face<LAW>(P, kx, fw, xedge == 0, s.dhdx, s.dpwdx, s.q_x, s.Re_x, s.taub_x);
             pw_ = dprod(fw);
         }
         pe = prod_from_east(pw_, sens);
         const int yedge = iy + 1 == ny ? 1 : 0;
         {
-            const DFace fn = dface(P, yedge, cur, nxt, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, ky);
-            if (FULL && own) store_face(P, ky, fn, yedge == 0, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
+            const DFace fn = dface<LAW>(P, yedge, cur, nxt, P.dy, s.dhdy, s.dpwdy, s.b_y, s.ub_y, s.valid_y, s.taub_y, P.Cy, ky);
+            if (FULL && own) store_face<LAW>(P, ky, fn, yedge == 0, s.dhdy, s.dpwdy, s.q_y, s.Re_y, s.taub_y);
             pn = dprod(fn);
         }
         if (own) {
@@ -4890,9 +4901,26 @@ void derived_march(const Phys& P, Fields F, int mode, bool full, hipStream_t st)
     }();
     const int R = r_env > 0 ? r_env : 8;
     const dim3 grid((P.nx + DER_OWN - 1) / DER_OWN, ((P.ny + R - 1) / R + WAVES - 1) / WAVES), blk(WAVE, WAVES);
-    if (mode == 1) hipLaunchKernelGGL((k_derived_march<1, true>), grid, blk, 0, st, P, F, R);
-    else if (full) hipLaunchKernelGGL((k_derived_march<0, true>), grid, blk, 0, st, P, F, R);
-    else hipLaunchKernelGGL((k_derived_march<0, false>), grid, blk, 0, st, P, F, R);
+    // one instantiation per sliding law (the initial-condition refresh, once, keeps the switch)
+    auto go = [&](auto law) {
+        constexpr int L = decltype(law)::value;
+        if (full) hipLaunchKernelGGL((k_derived_march<0, true, L>), grid, blk, 0, st, P, F, R);
+        else hipLaunchKernelGGL((k_derived_march<0, false, L>), grid, blk, 0, st, P, F, R);
+    };
+    if (mode == 1) {
+        hipLaunchKernelGGL((k_derived_march<1, true, -1>), grid, blk, 0, st, P, F, R);
+        return;
+    }
+    switch (P.sliding_law) {
+        case SHK_SLIDING_REGULARIZED_COULOMB: go(std::integral_constant<int, SHK_SLIDING_REGULARIZED_COULOMB>{}); break;
+        case SHK_SLIDING_PRESCRIBED: go(std::integral_constant<int, SHK_SLIDING_PRESCRIBED>{}); break;
+        case SHK_SLIDING_LINEAR: go(std::integral_constant<int, SHK_SLIDING_LINEAR>{}); break;
+        case SHK_SLIDING_REGULARIZED_COULOMB_FIELD:
+            go(std::integral_constant<int, SHK_SLIDING_REGULARIZED_COULOMB_FIELD>{});
+            break;
+        case SHK_SLIDING_REGULARIZED_COULOMB_V0: go(std::integral_constant<int, SHK_SLIDING_REGULARIZED_COULOMB_V0>{}); break;
+        default: go(std::integral_constant<int, -1>{}); break;
+    }
 }
 void picard_check(const Phys& P, const double* h, const double* h_prev, double* partials, Scalars* sc,
                   hipStream_t st) {
