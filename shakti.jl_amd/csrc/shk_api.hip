@@ -285,6 +285,7 @@ Expo canon(double e) {
     x.e = e;
     x.is_int = (e == (double)(long)e) && e > -64 && e < 64;
     x.k = x.is_int ? (int)e : 0;
+    x.third = e == 1.0 / 3.0;
     return x;
 }
 

@@ -25,6 +25,7 @@ struct Expo {
     int is_int;
     int k;
     double e;
+    int third;  // e is the double nearest 1/3 (the sliding laws' 1/n for n = 3)
 };
 
 // Device copy of the State: one pointer per field, in shk_field order.
