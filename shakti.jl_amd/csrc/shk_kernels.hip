@@ -2976,18 +2976,22 @@ struct Phase {
     static constexpr int value = P;
 };
 
-// red (or black) cells of a row pair relaxed in place from the rows beside it
+// red (or black) cells of a row pair relaxed in place from the rows beside it.
+// Every east and west term is taken: a coupling across the grid's edge is
+// exactly zero (the assembly writes aE = 0 on the last column, and the legs
+// zero the couplings of the lanes west of column 0), so it adds zero times a
+// finite value, and no lane-dependent select is needed.
 __device__ __forceinline__ void sw2_relax(f2v& uc, f2v um, f2v up, int j, bool red_pass, int ny, float omega,
-                                          bool hW0, bool hE1, f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) {
+                                          f2v bj, f2v cj, f2v dj, f2v ej, f2v nj, f2v nm) {
     const bool hN = j < ny - 1, hS = j > 0;
     if (((j & 1) == 0) == red_pass) {
         const float uW = lane_up1(uc.y), aW = lane_up1(ej.y);
-        uc.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, hW0, hW0 ? aW : 0.f, hW0 ? uW : 0.f, hN,
-                        hN ? nj.x : 0.f, hN ? up.x : 0.f, hS, hS ? nm.x : 0.f, hS ? um.x : 0.f);
+        uc.x = rb_relax(bj.x, cj.x, dj.x, omega, uc.x, true, ej.x, uc.y, true, aW, uW, hN, hN ? nj.x : 0.f,
+                        hN ? up.x : 0.f, hS, hS ? nm.x : 0.f, hS ? um.x : 0.f);
     } else {
         const float uE = lane_down1(uc.x);
-        uc.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, hE1, hE1 ? ej.y : 0.f, hE1 ? uE : 0.f, true, ej.x, uc.x, hN,
-                        hN ? nj.y : 0.f, hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
+        uc.y = rb_relax(bj.y, cj.y, dj.y, omega, uc.y, true, ej.y, uE, true, ej.x, uc.x, hN, hN ? nj.y : 0.f,
+                        hN ? up.y : 0.f, hS, hS ? nm.y : 0.f, hS ? um.y : 0.f);
     }
 }
 
@@ -3009,7 +3013,6 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
     const int iy1 = min(iy0 + R, ny);
     const bool own = lane >= SW2_HALO && lane <= WAVE - 1 - SW2_HALO && ix < nx;
     const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);  // clamped pair for the loads of lanes outside
-    const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
     const f2v z2 = {0.f, 0.f};
     auto rowk = [&](int j) -> size_t { return cidx(ixc, j < 0 ? 0 : (j < ny ? j : ny - 1), nx); };
     auto in = [&](int j) { return j >= 0 && j < ny; };
@@ -3067,7 +3070,7 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
         X[o4] = x4;
         Bw[o4] = b4;
         Dw[o4] = d4;
-        Ew[o3] = in(j3) ? unpack2(q.e) : z2;
+        Ew[o3] = in(j3) && ix >= 0 ? unpack2(q.e) : z2;  // lanes west of column 0: no coupling
         Nw[o3] = in(j3) ? unpack2(q.n) : z2;
         Cw[o2] = in(j2) ? unpack2(q.c) : z2;
         // row t+3: black cells from the red ones
@@ -3077,20 +3080,20 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
             const bool hN = j3 < ny - 1, hS = j3 > 0;
             if ((j3 & 1) == 0) {
                 const float xE = lane_down1(rc.x);
-                X[o3].y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, hE1 ? ej.y : 0.f, hE1 ? xE : 0.f,
-                                    hN ? nj.y : 0.f, hN ? rp.y : 0.f, hS ? nm.y : 0.f, hS ? rm.y : 0.f);
+                X[o3].y = rb_black0(bj.y, dj.y, omega, ej.x, rc.x, ej.y, xE, hN ? nj.y : 0.f, hN ? rp.y : 0.f,
+                                    hS ? nm.y : 0.f, hS ? rm.y : 0.f);
             } else {
                 const float xW = lane_up1(rc.y), aW = lane_up1(ej.y);
-                X[o3].x = rb_black0(bj.x, dj.x, omega, hW0 ? aW : 0.f, hW0 ? xW : 0.f, ej.x, rc.y,
-                                    hN ? nj.x : 0.f, hN ? rp.x : 0.f, hS ? nm.x : 0.f, hS ? rm.x : 0.f);
+                X[o3].x = rb_black0(bj.x, dj.x, omega, aW, xW, ej.x, rc.y, hN ? nj.x : 0.f, hN ? rp.x : 0.f,
+                                    hS ? nm.x : 0.f, hS ? rm.x : 0.f);
             }
         }
         // row t+2: red cells relaxed; row t+1: black cells relaxed
         if (in(j2))
-            sw2_relax(X[o2], X[o1], X[o3], j2, true, ny, omega, hW0, hE1, Bw[o2], Cw[o2], Dw[o2], Ew[o2], Nw[o2],
+            sw2_relax(X[o2], X[o1], X[o3], j2, true, ny, omega, Bw[o2], Cw[o2], Dw[o2], Ew[o2], Nw[o2],
                       Nw[o1]);
         if (in(j1))
-            sw2_relax(X[o1], X[o0], X[o2], j1, false, ny, omega, hW0, hE1, Bw[o1], Cw[o1], Dw[o1], Ew[o1], Nw[o1],
+            sw2_relax(X[o1], X[o0], X[o2], j1, false, ny, omega, Bw[o1], Cw[o1], Dw[o1], Ew[o1], Nw[o1],
                       Nw[o0]);
         // row t: the residual of the final x, restricted by 2 x 2 sums
         if (t >= iy0) {
@@ -3102,11 +3105,11 @@ __global__ __launch_bounds__(256) void k_mg_down_sw2(int nx, int ny, int nxc, in
             const float ew = lane_up1(e0.y);    // coupling (ix - 1, ix)
             float a0 = cc.x * X0.x;
             a0 += e0.x * (X0.x - X0.y);
-            if (hW0) a0 += ew * (X0.x - uw);
+            a0 += ew * (X0.x - uw);
             if (t < ny - 1) a0 += n0.x * (X0.x - X1.x);
             if (t > 0) a0 += nm.x * (X0.x - Xm.x);
             float a1 = cc.y * X0.y;
-            if (hE1) a1 += e0.y * (X0.y - ue);
+            a1 += e0.y * (X0.y - ue);
             a1 += e0.x * (X0.y - X0.x);
             if (t < ny - 1) a1 += n0.y * (X0.y - X1.y);
             if (t > 0) a1 += nm.y * (X0.y - Xm.y);
@@ -3164,7 +3167,6 @@ __global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int 
     if (iy0 < ny && strip * SW2_OWN < nx) {  // uniform over the wave (the partial below needs every wave)
         const bool own = lane >= SW2_HALO && lane <= WAVE - 1 - SW2_HALO && ix < nx;
         const int ixc = ix < 0 ? 0 : (ix < nx ? ix : nx - 2);
-        const bool hW0 = ix > 0, hE1 = ix + 1 < nx - 1;
         const f2v z2 = {0.f, 0.f};
         auto in = [&](int j) { return j >= 0 && j < ny; };
         // six-row ring windows as in k_mg_down_sw2: row t + o of step t in slot
@@ -3213,7 +3215,7 @@ __global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int 
                 b3 = q.b;
                 c3 = unpack2(q.c);
                 d3 = unpack2(q.d);
-                e3 = unpack2(q.e);
+                e3 = ix >= 0 ? unpack2(q.e) : z2;  // lanes west of column 0: no coupling
                 n3 = unpack2(q.n);
             }
             Braw[o3] = b3;
@@ -3224,16 +3226,16 @@ __global__ __launch_bounds__(256) void k_mg_up_sw2(int nx, int ny, int nxc, int 
             Nw[o3] = n3;
             // black on t+3 (from u), red on t+2, black on t+1, red on t
             if (in(j3))
-                sw2_relax(Uw[o3], Uw[o2], Uw[o4], j3, false, ny, omega, hW0, hE1, Bw[o3], Cw[o3], Dw[o3], Ew[o3],
+                sw2_relax(Uw[o3], Uw[o2], Uw[o4], j3, false, ny, omega, Bw[o3], Cw[o3], Dw[o3], Ew[o3],
                           Nw[o3], Nw[o2]);
             if (in(j2))
-                sw2_relax(Uw[o2], Uw[o1], Uw[o3], j2, true, ny, omega, hW0, hE1, Bw[o2], Cw[o2], Dw[o2], Ew[o2],
+                sw2_relax(Uw[o2], Uw[o1], Uw[o3], j2, true, ny, omega, Bw[o2], Cw[o2], Dw[o2], Ew[o2],
                           Nw[o2], Nw[o1]);
             if (in(j1))
-                sw2_relax(Uw[o1], Uw[o0], Uw[o2], j1, false, ny, omega, hW0, hE1, Bw[o1], Cw[o1], Dw[o1], Ew[o1],
+                sw2_relax(Uw[o1], Uw[o0], Uw[o2], j1, false, ny, omega, Bw[o1], Cw[o1], Dw[o1], Ew[o1],
                           Nw[o1], Nw[o0]);
             if (t >= iy0) {
-                sw2_relax(Uw[o0], Uw[om], Uw[o1], t, true, ny, omega, hW0, hE1, Bw[o0], Cw[o0], Dw[o0], Ew[o0],
+                sw2_relax(Uw[o0], Uw[om], Uw[o1], t, true, ny, omega, Bw[o0], Cw[o0], Dw[o0], Ew[o0],
                           Nw[o0], Nw[om]);
                 if (own) {
                     const f2v z = Uw[o0];
