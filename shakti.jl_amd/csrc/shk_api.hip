@@ -181,6 +181,9 @@ struct shk_ctx {
     // Picard iteration that another one follows refreshes only N, K and mdot
     bool derived_two_pass = false;
     bool lean_refresh = true;
+    // the FP64 multigrid solve from h has its CG start (r = b - A h, x = 0,
+    // p = 0) formed by the assembly
+    bool asm_start = true;
 
     // learned warm start and training-pair recording: the network input is
     // the grid padded with zero cells to W8 x H8 (multiples of 64 and 32)
@@ -838,13 +841,14 @@ static int nn_precond(shk_ctx* ctx, const Work<double>& w, int precond, int degr
 // solve goes on as plain PCG from where it is (*nn_stalled = 1).
 template <class T>
 int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, int check_every, double atol,
-        double rtol, int itmax, int nn_every = 0, int* nn_stalled = nullptr) {
+        double rtol, int itmax, int nn_every = 0, int* nn_stalled = nullptr, bool started = false) {
     const Geom& G = ctx->G;
     const double nb = (double)ctx->n * sizeof(T);
     const int jacobi = precond == SHK_PRECOND_JACOBI;
     int rc = reset_scalars(ctx, atol, rtol, itmax);
     if (rc) return rc;
-    {
+    // started: r, x and p_zero hold the start from x0 already (no Jacobi)
+    if (!started || jacobi) {
         Prof pr(ctx, P_INIT, (x0 ? 8 : 5) * nb);
         launch::cg_init<T>(G, w.aP, w.aE, w.aN, w.dg, w.b, x0, w.r, w.z, w.x, w.pA, jacobi, ctx->partials, ctx->sc,
                            ctx->st);
@@ -1250,6 +1254,11 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     // FP64 CG with the V-cycle: the assembly also writes the FP32 flux form the
     // hierarchy is built from and the finest level's BF16 coefficients
     launch::AsmLow lo;
+    // the CG start from h, formed by the assembly (nothing runs between the
+    // two that reads or writes r, x or p)
+    const bool asm_start = multigrid && !learned && !record && ctx->asm_start;
+    launch::AsmInit ini;
+    if (asm_start) ini = launch::AsmInit{ctx->r, ctx->x, ctx->pA};
     if (multigrid) {
         int rc0 = mg_prepare_fine(ctx);
         if (rc0) return rc0;
@@ -1265,9 +1274,10 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
         }
     }
     {
-        Prof pr(ctx, P_ASM, (need_c ? 18 : 17) * nb + (multigrid ? (ctx->mg_bf16 ? 3.0 : 2.0) * nb : 0.0));
+        Prof pr(ctx, P_ASM, (need_c ? 18 : 17) * nb + (multigrid ? (ctx->mg_bf16 ? 3.0 : 2.0) * nb : 0.0) +
+                                (asm_start ? 3.0 * nb : 0.0));
         launch::assemble<double>(G, ctx->P, ctx->F, parabolic, dt, ctx->aP, ctx->aE, ctx->aN, ctx->rhs, ctx->st,
-                                 need_c ? ctx->cP : nullptr, lo);
+                                 need_c ? ctx->cP : nullptr, lo, ini);
     }
     ctx->have_system = true;
     ctx->system_parabolic = parabolic;
@@ -1326,7 +1336,7 @@ int solve(shk_ctx* ctx, int parabolic, double dt, double relax, shk_solve_stats*
     int stalled = 0;
     int rc = pcg<double>(ctx, w, x0, cfg.precond, cfg.cheb_degree, cfg.check_every,
                          std::max(cfg.atol, ctx->atol_floor), cfg.rtol, itmax,
-                         nn_every, &stalled);
+                         nn_every, &stalled, asm_start && x0 == ctx->F.f[SHK_H]);
     if (rc) return rc;
     shk_solve_stats sst = stats_of(ctx, cheb);
     sst.fallback = learned ? (ctx->sc_host->nn_fallback | (stalled ? 2 : 0)) : 0;
@@ -1522,6 +1532,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     if (const char* e = getenv("SHK_BATCH_MARGIN")) ctx->batch_margin = atoi(e);
     ctx->derived_two_pass = getenv("SHK_DERIVED_TWO_PASS") != nullptr;
     ctx->lean_refresh = getenv("SHK_NO_LEAN_REFRESH") == nullptr;
+    ctx->asm_start = getenv("SHK_NO_ASM_START") == nullptr;
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;

@@ -173,9 +173,14 @@ struct AsmLow {
     float *c = nullptr, *e = nullptr, *n = nullptr, *d = nullptr;
     uint16_t *c16 = nullptr, *e16 = nullptr, *n16 = nullptr, *d16 = nullptr;
 };
+// The start of a CG solve from x0 = h, formed by the FP64 assembly itself
+// (k_cg_init without Jacobi): r = rhs - A h, x = 0, p = 0.
+struct AsmInit {
+    double *r = nullptr, *x = nullptr, *p = nullptr;
+};
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
-              hipStream_t st, double* cP = nullptr, const AsmLow& lo = AsmLow{});
+              hipStream_t st, double* cP = nullptr, const AsmLow& lo = AsmLow{}, const AsmInit& in = AsmInit{});
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st);
 

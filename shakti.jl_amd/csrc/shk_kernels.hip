@@ -848,16 +848,37 @@ __device__ __forceinline__ double bK(const Phys& P, double K1, double K2, double
 // cP (optional) receives the diagonal minus the kept couplings -- the creep
 // (or storage) reaction plus the conductances of faces to Dirichlet cells --
 // summed without cancellation, for the FP32 flux-form operator.
+// With in.r, also the start of a CG solve from x0 = h (k_cg_init): r = rhs -
+// A h, x = 0, p = 0.  The row's own aP, aE, aN are at hand, and the couplings
+// its west and south neighbours store toward it (aE(ix-1), aN(iy-1)) are the
+// same symmetric face expression, so A h is formed with the operator apply's
+// products, in its order and under its conditions (march), bit for bit.
 // One thread per cell (a plain 2D grid of 64 x 4 workgroups).
 template <bool PARABOLIC, class T>
 __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T* __restrict__ aP,
                                                   T* __restrict__ aE, T* __restrict__ aN, T* __restrict__ rhs,
-                                                  double* __restrict__ cP, launch::AsmLow lo) {
+                                                  double* __restrict__ cP, launch::AsmLow lo, launch::AsmInit in) {
     const int nx = P.nx, ny = P.ny;
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
     if (ix >= nx || iy >= ny) return;
     FV s(F);
+    // the CG start of a row: aPc, aEc, aNc its own, aEw, aNs those its west
+    // and south neighbours store toward it
+    auto start = [&](size_t c, T aPc, T aEc, T aNc, T aEw, T aNs, T b) {
+        if constexpr (sizeof(T) == 8) {  // the FP64 diagonal form only
+            if (!in.r) return;
+            const double* u = s.h;
+            T y = aPc * u[c];
+            if (ix < nx - 1) y -= aEc * u[c + 1];
+            if (ix > 0) y -= aEw * u[c - 1];
+            if (iy < ny - 1) y -= aNc * u[c + nx];
+            if (iy > 0) y -= aNs * u[c - nx];
+            in.r[c] = b - y;
+            in.x[c] = 0;
+            in.p[c] = 0;
+        }
+    };
     {
         const size_t c = cidx(ix, iy, nx);
         // the low-precision copies of one row of the flux form (c, e, n, d)
@@ -877,18 +898,14 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T
             }
         };
         const double m = s.mask[c];
-        if (m == OCEAN || m == LAND) {
+        if (m == OCEAN || m == LAND || m == OTHER_BASIN) {
+            // an identity row; its neighbours keep no coupling toward it
+            const T b = m == OTHER_BASIN ? (T)s.h[c] : (T)dirichlet_head(P, m, s.zb[c]);
             aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
-            rhs[c] = dirichlet_head(P, m, s.zb[c]);
+            rhs[c] = b;
             if (cP) cP[c] = 1.0;
             low(1.0, 0.0, 0.0, 1.0);
-            return;
-        }
-        if (m == OTHER_BASIN) {
-            aP[c] = 1.0; aE[c] = 0.0; aN[c] = 0.0;
-            rhs[c] = s.h[c];
-            if (cP) cP[c] = 1.0;
-            low(1.0, 0.0, 0.0, 1.0);
+            start(c, (T)1.0, (T)0.0, (T)0.0, (T)0.0, (T)0.0, b);
             return;
         }
         const double Kc = s.K[c];
@@ -932,6 +949,14 @@ __global__ __launch_bounds__(256) void k_assemble(Phys P, Fields F, double dt, T
         aE[c] = (T)cE;
         aN[c] = (T)cN;
         rhs[c] = (T)r;
+        // the west (south) row keeps its coupling toward this one when both
+        // are rows of the system and this one is GROUNDED; it is then bK of
+        // the same two conductivities, whose form is symmetric in them
+        // (sum, and product over sum: bit for bit the same either way round)
+        const bool g = m == GROUNDED;
+        const bool wRow = !(mW == OCEAN || mW == LAND || mW == OTHER_BASIN);
+        const bool sRow = !(mS == OCEAN || mS == LAND || mS == OTHER_BASIN);
+        start(c, (T)diag, (T)cE, (T)cN, (g && wRow) ? (T)fW : (T)0.0, (g && sRow) ? (T)fS : (T)0.0, (T)r);
     }
 }
 
@@ -4981,14 +5006,15 @@ void zero(double* p, size_t n, hipStream_t st) {
 
 template <class T>
 void assemble(const Geom& G, const Phys& P, Fields F, int parabolic, double dt, T* aP, T* aE, T* aN, T* rhs,
-              hipStream_t st, double* cP, const AsmLow& lo) {
+              hipStream_t st, double* cP, const AsmLow& lo, const AsmInit& in) {
     const dim3 grid = cells_grid(G.nx, G.ny), blk(WAVE, WAVES);
     AsmLow l = lo;
     if (!cP) l = AsmLow{};  // the copies come from the flux form's c
+    const AsmInit i = sizeof(T) == 8 ? in : AsmInit{};
     if (parabolic)
-        hipLaunchKernelGGL((k_assemble<true, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP, l);
+        hipLaunchKernelGGL((k_assemble<true, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP, l, i);
     else
-        hipLaunchKernelGGL((k_assemble<false, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP, l);
+        hipLaunchKernelGGL((k_assemble<false, T>), grid, blk, 0, st, P, F, dt, aP, aE, aN, rhs, cP, l, i);
 }
 template <class T>
 void matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* x, T* y, hipStream_t st) {
@@ -5634,7 +5660,7 @@ void cg_finish(int n, const double* x, const double* h_prev, double* h_new, doub
 
 #define SHK_INST(T)                                                                                               \
     template void assemble<T>(const Geom&, const Phys&, Fields, int, double, T*, T*, T*, T*, hipStream_t,         \
-                              double*, const AsmLow&);                                                            \
+                              double*, const AsmLow&, const AsmInit&);                                            \
     template void matvec<T>(const Geom&, const T*, const T*, const T*, const T*, T*, hipStream_t);                \
     template void lanczos_init<T>(const Geom&, const T*, const T*, int, T*, T*, double*, Scalars*, hipStream_t);  \
     template void lanczos_step_a<T>(const Geom&, const T*, const T*, const T*, const T*, int, const T*, const T*, \
