@@ -387,7 +387,9 @@ int shk_download_system(shk_ctx* ctx, double* aP, double* aE, double* aW, double
  * previous iteration's x += alpha p_old, here with alpha = 0 so x keeps its
  * values) on the solve's work vectors, in the variant the
  * V-cycle-preconditioned solve runs (z read in FP32 where that solve keeps it
- * so: 68 instead of 72 bytes per cell), cold like 3 and 4: in that solve the
+ * so: 68 instead of 72 bytes per cell; where it also takes the x steps in
+ * pairs, the odd iteration's kernel, without x, and the even one's, with two
+ * steps, in turn: 52 and 76, a mean of 64), cold like 3 and 4: in that solve the
  * V-cycle between two matvecs streams their inputs out of the caches (the
  * device scalars are restored afterwards). */
 int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms, double* bytes_per_launch);

@@ -184,6 +184,9 @@ struct shk_ctx {
     // the FP64 multigrid solve from h has its CG start (r = b - A h, x = 0,
     // p = 0) formed by the assembly
     bool asm_start = true;
+    // FP64 CG with the V-cycle's FP32 z: the matvecs take the x steps in
+    // pairs (x read and written every other iteration)
+    bool pair_x = true;
 
     // learned warm start and training-pair recording: the network input is
     // the grid padded with zero cells to W8 x H8 (multiples of 64 and 32)
@@ -909,6 +912,9 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
     const int nba = multigrid ? launch::mg_up_nblocks(ctx->mg[0].nx, ctx->mg[0].ny, ctx->mg_rb,
                                                       ctx->mg_bf16 ? ctx->mg_nu0 : 1)
                               : 0;
+    // x steps in pairs: iteration j (from 1) is odd where phase + k is even
+    const bool pairx = defer && zf32 && ctx->pair_x;
+    auto xs_of = [&](int k) { return pairx ? (((phase + k) & 1) == 0 ? 0 : 2) : 1; };
     auto enqueue_batch = [&]() {
         T* p_old = phase ? w.pB : w.pA;
         T* p_new = phase ? w.pA : w.pB;
@@ -920,11 +926,13 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
                     int nbb;
                     {
                         // reads aP, aE, aN, z (FP32 or FP64), p_old, x; writes p_new, Ap, x
-                        // (x += alpha p_old: the previous iteration's step)
-                        Prof pr(ctx, P_MV, (zf32 ? 8.5 : 9.0) * nb);
+                        // (x += alpha p_old: the previous iteration's step); paired
+                        // steps: no x (odd), or also p_new read (even)
+                        const int xs = xs_of(k);
+                        Prof pr(ctx, P_MV, (zf32 ? 8.5 : 9.0) * nb + (xs == 0 ? -2.0 : xs == 2 ? 1.0 : 0.0) * nb);
                         nbb = zf32 ? launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, (const float*)zf, p_old, p_new,
                                                                 w.Ap, ctx->partials2, ctx->sc, ctx->partials, nba,
-                                                                w.x, ctx->st)
+                                                                w.x, ctx->st, xs)
                                    : launch::cg_matvec_deferred(G, w.aP, w.aE, w.aN, w.z, p_old, p_new, w.Ap,
                                                                 ctx->partials2, ctx->sc, ctx->partials, nba, w.x,
                                                                 ctx->st);
@@ -991,7 +999,8 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
         if constexpr (sizeof(T) == 8) {
             if (defer) {
                 Prof pr(ctx, P_FIN, 0);
-                launch::cg_defer_stop(ctx->n, ctx->sc, ctx->partials, nba, p_old, w.x, ctx->st);
+                launch::cg_defer_stop(ctx->n, ctx->sc, ctx->partials, nba, p_old, w.x, ctx->st, xs_of(batch),
+                                      p_new);
             }
         }
     };
@@ -1082,6 +1091,12 @@ int pcg(shk_ctx* ctx, const Work<T>& w, const T* x0, int precond, int degree, in
             }
         }
         rn_start = ctx->sc_host->rNorm;
+    }
+    if constexpr (sizeof(T) == 8) {
+        if (pairx) {
+            Prof pr(ctx, P_FIN, 0);
+            launch::cg_xflush(ctx->n, ctx->sc, w.x, ctx->st);
+        }
     }
     ctx->last_niter = ctx->sc_host->iter;
     if constexpr (sizeof(T) == 8) {
@@ -1533,6 +1548,7 @@ int shk_create(const shk_grid* grid, const shk_params* params, const shk_model* 
     ctx->derived_two_pass = getenv("SHK_DERIVED_TWO_PASS") != nullptr;
     ctx->lean_refresh = getenv("SHK_NO_LEAN_REFRESH") == nullptr;
     ctx->asm_start = getenv("SHK_NO_ASM_START") == nullptr;
+    ctx->pair_x = getenv("SHK_NO_PAIR_X") == nullptr;
     const size_t nx = grid->nx, ny = grid->ny;
     ctx->n = nx * ny;
     ctx->nxf = (nx + 1) * ny;
@@ -2146,6 +2162,9 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     // the variant the V-cycle-preconditioned solve runs: z in FP32 where its
     // finest level is BF16 and the matvec the two-column kernel
     const bool z32 = ctx->use_defer && ctx->mg_bf16 && cg_wide_ok(G);
+    // ... taking its x steps in pairs: the odd and even iterations' kernels in turn
+    const bool pair = z32 && ctx->pair_x;
+    int calls = 0;
     auto one = [&]() {
         switch (kernel) {
             case 0:
@@ -2157,7 +2176,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
                 if (z32)
                     launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, (const float*)ctx->z32, ctx->pA,
                                                ctx->pB, ctx->Ap, ctx->partials2, ctx->sc, ctx->partials, 0, ctx->x,
-                                               ctx->st);
+                                               ctx->st, pair ? ((calls++ & 1) ? 2 : 0) : 1);
                 else
                     launch::cg_matvec_deferred(G, ctx->aP, ctx->aE, ctx->aN, ctx->z, ctx->pA, ctx->pB, ctx->Ap,
                                                ctx->partials2, ctx->sc, ctx->partials, 0, ctx->x, ctx->st);
@@ -2168,7 +2187,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
     if (kernel == 0 || kernel == 3) bytes = 5 * n * 8;
     else if (kernel == 1) bytes = 5 * n * 4;
     else if (kernel == 2 || kernel == 4) bytes = 2 * n * 8;
-    else if (kernel == 5) bytes = z32 ? 8.5 * n * 8 : 9 * n * 8;
+    else if (kernel == 5) bytes = pair ? 8.0 * n * 8 : z32 ? 8.5 * n * 8 : 9 * n * 8;  // pair: the mean of 6.5 and 9.5
     else {
         hipEventDestroy(a);
         hipEventDestroy(b);
@@ -2185,6 +2204,7 @@ int shk_time_kernel(shk_ctx* ctx, int32_t kernel, int32_t reps, double* mean_ms,
         live.gpend = 0;
         live.done_defer = 0;
         live.alpha = 0.0;  // x += 0 p: the traffic of the solve's matvec, x left as it is
+        live.alpha_pend = 0.0;
         HIPCHK(hipMemcpy(ctx->sc, &live, sizeof(Scalars), hipMemcpyHostToDevice));
     }
     one();  // warm

@@ -79,6 +79,11 @@ struct Scalars {
     // unused (reductions finish in a separate kernel); reset with the rest
     uint32_t counter;
     uint32_t pad;
+    // CG matvecs taking the x steps in pairs: the step an odd iteration left
+    // owed (alpha_pend p, p = xpend_p) while xpend_n is 1
+    double alpha_pend;
+    const double* xpend_p;
+    int32_t xpend_n, xpend_pad;
 };
 
 // One level of the multigrid hierarchy as the device sees it, and the small
@@ -233,15 +238,19 @@ void cg_matvec(const Geom& G, const T* aP, const T* aE, const T* aN, const T* z,
 // their count, and takes the previous iteration's x += alpha p_old; the
 // update sums both and takes r -= alpha Ap.
 // The stop decision at the end of a batch of deferred iterations (done_defer
-// and the pending x += alpha p_old when the solve stops there).
+// and the pending x += alpha p_old when the solve stops there).  xs: the
+// x-step mode of the matvec it stands in for (1 single steps; 0, 2 the odd and
+// even iterations of paired steps, 2 also taking the owed step from p_pend).
 void cg_defer_stop(size_t n, Scalars* sc, const double* pa, int nba, const double* p_old, double* x,
-                   hipStream_t st);
+                   hipStream_t st, int xs = 1, const double* p_pend = nullptr);
+// The owed x step of paired steps that no later kernel took (end of a solve).
+void cg_xflush(size_t n, const Scalars* sc, double* x, hipStream_t st);
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const double* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
                        int nba, double* x, hipStream_t st);
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const float* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
-                       int nba, double* x, hipStream_t st);
+                       int nba, double* x, hipStream_t st, int xs = 1);
 void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
                         const double* pb, int nbb, hipStream_t st);
 int mg_up_nblocks(int nx, int ny, int rb, int nu = 1);

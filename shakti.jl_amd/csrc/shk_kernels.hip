@@ -1668,11 +1668,24 @@ __device__ __forceinline__ d2v ldz2(const float* p) {
     const f2v v = *(const f2v*)p;
     return d2v{(double)v.x, (double)v.y};
 }
-template <bool NT, bool DEFER, class ZT = double>
+//
+// XS (DEFER) selects how the pending x steps are taken.  1: each matvec takes
+// the previous iteration's step, x += alpha p_old.  Pairs (0 and 2, in turn):
+// the matvec of an odd iteration j takes none and leaves step j-1 owed
+// (alpha_pend, xpend_p, xpend_n = 1); the next one, of iteration j+1, takes
+// both, x = (x + alpha_pend p_{j-1}) + alpha p_j, reading p_{j-1} from p_new
+// (the ping-pong buffer it then overwrites) before its store to each cell.
+// The same operations in the same order as two single steps, so the same x,
+// bit for bit, with x read and written once per two iterations.  Every end
+// of the solve takes the owed step where the single steps would have taken
+// it: a stop in the matvec (converged: every pending step; breakdown: the
+// owed one only), k_cg_defer_stop likewise, and k_cg_xflush at the end of the
+// solve for the rest (tired, zero curvature after an odd iteration).
+template <bool NT, bool DEFER, class ZT = double, int XS = 1>
 __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const double* __restrict__ aP,
                                                     const double* __restrict__ aE, const double* __restrict__ aN,
                                                     const ZT* __restrict__ z, const double* __restrict__ p_old,
-                                                    double* __restrict__ p_new, double* __restrict__ Ap,
+                                                    double* p_new, double* __restrict__ Ap,
                                                     double* partials, Scalars* sc, const double* __restrict__ pa,
                                                     int nba, double* __restrict__ x) {
     if (sc->done) return;
@@ -1680,20 +1693,44 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
     wide_tile(R, strip, iy0);
     double beta = sc->beta;
     const double alpha = sc->alpha;  // the previous iteration's step (DEFER), as k_cg_matvec
-    auto x_step = [&](size_t c) {
+    const double alpha_pend = XS == 2 ? sc->alpha_pend : 0.0;  // the step owed from the iteration before
+    // the pending steps of the pair (XS 2: the owed one from p_new, read
+    // before p_new is stored; then the previous iteration's, unless `owed_only`)
+    auto x_step = [&](size_t c, d2v pp, bool owed_only) {
         d2v xv = ld2<NT>(x + c);
-        const d2v po = ld2<false>(p_old + c);
-        xv.x = xv.x + alpha * po.x;
-        xv.y = xv.y + alpha * po.y;
+        if constexpr (XS == 2) {
+            xv.x = xv.x + alpha_pend * pp.x;
+            xv.y = xv.y + alpha_pend * pp.y;
+        }
+        if (!owed_only) {
+            const d2v po = ld2<false>(p_old + c);
+            xv.x = xv.x + alpha * po.x;
+            xv.y = xv.y + alpha * po.y;
+        }
         st2<NT>(x + c, xv);
     };
     if (const int stop = deferred_gamma<DEFER>(sc, pa, nba, beta)) {
         // converged: march2's cells of this wave, columns ix, ix + 1 of rows iy0 .. iy1 - 1
         const int ix = strip * 2 * WAVE + 2 * (int)threadIdx.x;
         const int iy1 = min(iy0 + R, ny);
-        if (stop == 1 && ix < nx)
-            for (int iy = iy0; iy < iy1; ++iy) x_step(cidx(ix, iy, nx));
+        if (ix < nx && (stop == 1 || XS == 2))
+            for (int iy = iy0; iy < iy1; ++iy) {
+                const size_t c = cidx(ix, iy, nx);
+                x_step(c, XS == 2 ? ld2<NT>(p_new + c) : d2v{0.0, 0.0}, stop != 1);
+            }
+        if constexpr (XS == 2) {
+            if (lead_thread()) sc->xpend_n = 0;
+        }
         return;
+    }
+    if constexpr (XS == 0) {
+        if (lead_thread()) {
+            sc->alpha_pend = alpha;
+            sc->xpend_p = p_old;
+            sc->xpend_n = 1;
+        }
+    } else if constexpr (XS == 2) {
+        if (lead_thread()) sc->xpend_n = 0;
     }
     double acc = 0.0;
     march2<NT>(
@@ -1701,9 +1738,10 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
         [&](size_t j) -> d2v { return ldz2(z + j) + beta * ld2<false>(p_old + j); },
         [&](size_t j) -> double { return (double)z[j] + beta * p_old[j]; },
         [&](size_t c, d2v Au, d2v u) {
+            const d2v pp = XS == 2 ? ld2<NT>(p_new + c) : d2v{0.0, 0.0};
             st2<NT>(p_new + c, u);
             st2<NT>(Ap + c, Au);
-            if constexpr (DEFER) x_step(c);
+            if constexpr (DEFER && XS != 0) x_step(c, pp, false);
             acc += u.x * Au.x;
             acc += u.y * Au.y;
         });
@@ -1721,9 +1759,13 @@ __global__ __launch_bounds__(256) void k_cg_matvec2(int nx, int ny, int R, const
 // writes), and the next batch's matvec takes the same decision.  It never
 // writes done, which every workgroup reads on entry.  (64, 4) workgroups,
 // grid-stride over pairs of cells.
+// xs: the x-step mode (k_cg_matvec2's XS) of the matvec it stands in for;
+// with 2 it also takes the owed step, from p_pend (that matvec's p_new),
+// on a stop of either kind.
 __global__ __launch_bounds__(256) void k_cg_defer_stop(size_t n, Scalars* sc, const double* __restrict__ pa,
                                                        int nba, const double* __restrict__ p_old,
-                                                       double* __restrict__ x) {
+                                                       double* __restrict__ x, int xs,
+                                                       const double* __restrict__ p_pend) {
     if (sc->done || !sc->gpend) return;
     const double g = sum_partials(pa, nba);
     const double rn = sqrt(g);
@@ -1735,13 +1777,39 @@ __global__ __launch_bounds__(256) void k_cg_defer_stop(size_t n, Scalars* sc, co
         return;
     }
     const double alpha = sc->alpha;
+    const double alpha_pend = sc->alpha_pend;
     double beta;
-    if (deferred_gamma<true>(sc, pa, nba, beta) == 2) return;  // breakdown: no step, as in the matvec
+    // breakdown: no step, as in the matvec (but the owed one)
+    const bool owed_only = deferred_gamma<true>(sc, pa, nba, beta) == 2;
+    if (xs == 2 && lead_thread()) sc->xpend_n = 0;
+    if (owed_only && xs != 2) return;
+    const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
+    const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
+    auto step = [&](size_t i) {
+        double v = x[i];
+        if (xs == 2) v = v + alpha_pend * p_pend[i];
+        if (!owed_only) v = v + alpha * p_old[i];
+        x[i] = v;
+    };
+    for (size_t i = 2 * ((size_t)blockIdx.x * WAVE * WAVES + tid); i < n; i += 2 * stride) {
+        step(i);
+        if (i + 1 < n) step(i + 1);
+    }
+}
+
+// The end of a solve whose CG matvecs take the x steps in pairs: the step an
+// odd iteration's matvec left owed, where no later kernel took it.  Reads
+// xpend_n and writes nothing of the scalars (k_reset_scalars clears it).
+__global__ __launch_bounds__(256) void k_cg_xflush(size_t n, const Scalars* __restrict__ sc,
+                                                   double* __restrict__ x) {
+    if (!sc->xpend_n) return;
+    const double a = sc->alpha_pend;
+    const double* __restrict__ p = sc->xpend_p;
     const size_t tid = (size_t)threadIdx.x + threadIdx.y * WAVE;
     const size_t stride = (size_t)gridDim.x * WAVE * WAVES;
     for (size_t i = 2 * ((size_t)blockIdx.x * WAVE * WAVES + tid); i < n; i += 2 * stride) {
-        x[i] = x[i] + alpha * p_old[i];
-        if (i + 1 < n) x[i + 1] = x[i + 1] + alpha * p_old[i + 1];
+        x[i] = x[i] + a * p[i];
+        if (i + 1 < n) x[i + 1] = x[i + 1] + a * p[i + 1];
     }
 }
 
@@ -4732,6 +4800,7 @@ __global__ void k_reset_scalars(Scalars* sc, double atol, double rtol, int itmax
     sc->gpend = 0;
     sc->done_defer = 0;
     sc->breakdown = 0;
+    sc->xpend_n = 0;
 }
 
 // One workgroup: sums (or maximises) the nb partials of the preceding
@@ -5150,23 +5219,37 @@ int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const 
                        p_new, Ap, pb, sc, pa, nba, x);
     return G.nblocks;
 }
-// The same with the V-cycle's FP32 z (cg_wide_ok grids only).
+// The same with the V-cycle's FP32 z (cg_wide_ok grids only); xs as
+// k_cg_matvec2's XS.
 int cg_matvec_deferred(const Geom& G, const double* aP, const double* aE, const double* aN, const float* z,
                        const double* p_old, double* p_new, double* Ap, double* pb, Scalars* sc, const double* pa,
-                       int nba, double* x, hipStream_t st) {
+                       int nba, double* x, hipStream_t st, int xs) {
     const Geom W = wide_geom(G);
-    if (cg_stream_nt(G))
-        hipLaunchKernelGGL((k_cg_matvec2<true, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z,
-                           p_old, p_new, Ap, pb, sc, pa, nba, x);
-    else
-        hipLaunchKernelGGL((k_cg_matvec2<false, true, float>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN,
-                           z, p_old, p_new, Ap, pb, sc, pa, nba, x);
+#define SHK_MV2(NT, XS)                                                                                          \
+    hipLaunchKernelGGL((k_cg_matvec2<NT, true, float, XS>), W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, z, \
+                       p_old, p_new, Ap, pb, sc, pa, nba, x)
+    const bool nt = cg_stream_nt(G);
+    if (xs == 0) {
+        if (nt) SHK_MV2(true, 0);
+        else SHK_MV2(false, 0);
+    } else if (xs == 2) {
+        if (nt) SHK_MV2(true, 2);
+        else SHK_MV2(false, 2);
+    } else {
+        if (nt) SHK_MV2(true, 1);
+        else SHK_MV2(false, 1);
+    }
+#undef SHK_MV2
     return W.nblocks;
 }
 void cg_defer_stop(size_t n, Scalars* sc, const double* pa, int nba, const double* p_old, double* x,
-                   hipStream_t st) {
+                   hipStream_t st, int xs, const double* p_pend) {
     const unsigned nblk = (unsigned)std::min<size_t>(1024, (n + 511) / 512);
-    hipLaunchKernelGGL(k_cg_defer_stop, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, sc, pa, nba, p_old, x);
+    hipLaunchKernelGGL(k_cg_defer_stop, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, sc, pa, nba, p_old, x, xs, p_pend);
+}
+void cg_xflush(size_t n, const Scalars* sc, double* x, hipStream_t st) {
+    const unsigned nblk = (unsigned)std::min<size_t>(1024, (n + 511) / 512);
+    hipLaunchKernelGGL(k_cg_xflush, dim3(nblk), dim3(WAVE, WAVES), 0, st, n, sc, x);
 }
 void cg_update_deferred(const Geom& G, const double* Ap, double* r, Scalars* sc, const double* pa, int nba,
                         const double* pb, int nbb, hipStream_t st) {

@@ -214,3 +214,34 @@ def test_captured_graphs_give_the_iterates_of_kernel_by_kernel_launches(hip, mon
     print(f"{precond}: CG iterations with graphs {out[0][0]}, without {out[1][0]}", flush=True)
     assert out[0][0] == out[1][0] and len(out[0][0]) == 2
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("itmax,check_every", [(0, 8), (0, 3), (0, 1), (7, 8), (6, 8)])
+def test_paired_x_steps_give_the_iterates_of_single_steps(hip, monkeypatch, itmax, check_every):
+    """FP64 CG with the V-cycle's FP32 z takes the x steps in pairs: the matvec
+    of an odd iteration leaves its step owed and the next one takes both,
+    x = (x + a1 p1) + a2 p2, the operations of two single steps.  Whatever
+    iteration and batch the solve stops at (converged inside a batch or at its
+    end, batches of odd and even size, or tired after an odd or an even count),
+    the head is that of single steps (SHK_NO_PAIR_X=1), bit for bit."""
+    from shakti_hip import synthetic as S
+    gg, raw = S.glacier(2052, 1023, bed_amp=50.0, seed=8)
+    g_args = (gg.nx, gg.ny, gg.lx, gg.ly)
+    out = []
+    for single in (False, True):
+        if single:
+            monkeypatch.setenv("SHK_NO_PAIR_X", "1")
+        hg, hp, hs = hip_state(hip, g_args, raw)
+        ls = hip.CGIterativeSolver(hg, hip.HIPMatrixFreeLinearSystem, multigrid=True, atol=0.0, rtol=1e-10,
+                                   itmax=itmax, check_every=check_every)
+        hip.solve_elliptic_linear_system(ls, hs, hg, hp, hip.Harmonic())
+        st = ls.last_stats
+        out.append((st.niter, st.true_residual, hs.get("h")))
+        monkeypatch.delenv("SHK_NO_PAIR_X", raising=False)
+    print(f"itmax {itmax}, check_every {check_every}: CG iterations paired {out[0][0]}, single {out[1][0]}",
+          flush=True)
+    assert out[0][0] == out[1][0] > 0
+    if itmax:
+        assert out[0][0] == itmax
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
