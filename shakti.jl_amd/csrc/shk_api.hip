@@ -2086,9 +2086,10 @@ int shk_residual_norms_noalloc(int32_t nx, int32_t ny, const double* aP, const d
     }
     // the geometry shk_residual_norms runs for this grid (a context's)
     const Geom G = make_geom(nx, ny);
-    if (G.nblocks > kNoallocMaxBlocks) {
+    const int nblk = launch::residual_norms_nblocks(G);  // the true-residual kernel's workgroups
+    if (nblk > kNoallocMaxBlocks) {
         noctx_error = "shk_residual_norms_noalloc: a " + std::to_string(nx) + " x " + std::to_string(ny) +
-                      " grid needs " + std::to_string(G.nblocks) + " workgroups, more than the " +
+                      " grid needs " + std::to_string(nblk) + " workgroups, more than the " +
                       std::to_string(kNoallocMaxBlocks) + " partials of the static scratch";
         return SHK_ERR_ARG;
     }

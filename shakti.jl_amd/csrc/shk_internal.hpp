@@ -213,6 +213,8 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
 // h_out, also h_out = x + x0 (under-relaxed towards x0 when relax > 0)
 // ppart (with h_out and sc): also the Picard test of h_out against x0, the
 // flag into sc (partials of the same count as the norms').
+// The workgroups (partials) residual_norms launches for a grid.
+int residual_norms_nblocks(const Geom& G);
 void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
                     const double* x, const double* x0, double* partials, double* out, hipStream_t st,
                     double* h_out = nullptr, double relax = 0.0, double* ppart = nullptr, Scalars* sc = nullptr);

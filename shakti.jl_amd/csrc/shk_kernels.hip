@@ -2155,6 +2155,58 @@ __global__ __launch_bounds__(256) void k_true_resid(int nx, int ny, int R, const
     }
 }
 
+// k_true_resid on two columns per lane (march2, the wide tiling; even nx):
+// the same operand, A h and per-cell expressions, so the same head and
+// Picard maxima; the norms' partials sum the cells in another order.
+__global__ __launch_bounds__(256) void k_true_resid2(int nx, int ny, int R, const double* __restrict__ aP,
+                                                     const double* __restrict__ aE, const double* __restrict__ aN,
+                                                     const double* __restrict__ b, const double* __restrict__ x,
+                                                     const double* __restrict__ x0, double* partials,
+                                                     double* __restrict__ h_out, double relax, double* ppart) {
+    int strip, iy0;
+    wide_tile(R, strip, iy0);
+    double ar = 0.0, ab = 0.0, dmax = 0.0, hmax = 0.0;
+    bool any_nan = false;
+    auto cell = [&](size_t c, double Au, double u) {
+        const double bc = b[c];
+        const double r = bc - Au;
+        ar += r * r;
+        ab += bc * bc;
+        if (h_out) {
+            const double hn = relax > 0.0 ? relax * u + (1.0 - relax) * x0[c] : u;
+            h_out[c] = hn;
+            if (ppart) {
+                const double d = fabs(hn - x0[c]), a = fabs(hn);
+                any_nan |= isnan(d) || isnan(a);
+                dmax = d > dmax ? d : dmax;
+                hmax = a > hmax ? a : hmax;
+            }
+        }
+    };
+    march2<false>(
+        nx, ny, R, strip, iy0, aP, aE, aN,
+        [&](size_t k) -> d2v {
+            d2v v = ld2<false>(x + k);
+            if (x0) {
+                const d2v w = ld2<false>(x0 + k);
+                v.x = v.x + w.x;
+                v.y = v.y + w.y;
+            }
+            return v;
+        },
+        [&](size_t k) -> double { return x0 ? x[k] + x0[k] : x[k]; },
+        [&](size_t c, d2v Au, d2v u) {
+            cell(c, Au.x, u.x);
+            cell(c + 1, Au.y, u.y);
+        });
+    double v[2] = {ar, ab};
+    block_partial<2, false>(v, partials);
+    if (ppart) {
+        double w[2] = {any_nan ? NAN : dmax, hmax};
+        block_partial<2, true>(w, ppart);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_ir_correct(size_t n, double* __restrict__ x, const float* __restrict__ d) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = x[i] + (double)d[i];
@@ -5143,13 +5195,22 @@ void gershgorin(const Geom& G, const T* dg, const T* aE, const T* aN, double* pa
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(WAVE, WAVES), 0, st, partials, (int)(grid.x * grid.y),
                        (int)FIN_GERSH, 0, sc);
 }
+int residual_norms_nblocks(const Geom& G) { return cg_wide_ok(G) ? wide_geom(G).nblocks : G.nblocks; }
 void residual_norms(const Geom& G, const double* aP, const double* aE, const double* aN, const double* b,
                     const double* x, const double* x0, double* partials, double* out, hipStream_t st, double* h_out,
                     double relax, double* ppart, Scalars* sc) {
     if (!h_out || !sc) ppart = nullptr;
-    hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials, h_out,
-                       relax, ppart);
-    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(WAVE, WAVES), 0, st, partials, G.nblocks, out, ppart, sc);
+    int nb = G.nblocks;
+    if (cg_wide_ok(G)) {
+        const Geom W = wide_geom(G);
+        hipLaunchKernelGGL(k_true_resid2, W.grid, W.block, 0, st, W.nx, W.ny, W.R, aP, aE, aN, b, x, x0, partials,
+                           h_out, relax, ppart);
+        nb = W.nblocks;
+    } else {
+        hipLaunchKernelGGL(k_true_resid, G.grid, G.block, 0, st, G.nx, G.ny, G.R, aP, aE, aN, b, x, x0, partials,
+                           h_out, relax, ppart);
+    }
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(WAVE, WAVES), 0, st, partials, nb, out, ppart, sc);
 }
 void reset_scalars(Scalars* sc, double atol, double rtol, int itmax, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1), 0, st, sc, atol, rtol, itmax);
