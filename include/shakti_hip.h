@@ -359,7 +359,8 @@ int shk_residual_norms(shk_ctx* ctx, const double* x_host, double* norms);
  * NULL is the null stream) and asynchronous: no allocation, no copy, no
  * synchronisation.  The per-workgroup partials go to one static device
  * array of the library, 512 KiB per GPU, held until the process ends; grids
- * that need more than 32768 workgroups (above about 2.5e8 cells) are
+ * that need more than 32768 workgroups (above about 1.3e8 cells for even nx,
+ * 2.7e8 for odd nx) are
  * rejected with SHK_ERR_ARG.  Every call on a GPU shares that array, so
  * calls that overlap on one GPU race.  Takes no context: on failure
  * shk_last_error(NULL) gives the message (per host thread). */
